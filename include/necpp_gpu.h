@@ -1,0 +1,90 @@
+/*
+ * necpp_gpu.h -- entry points of the MI355X engine that have no counterpart
+ * in nec2++'s libnecpp.h.  They sit beside the drop-in API and never change
+ * the behaviour of the reference calls.  Return conventions follow
+ * libnecpp.h: 0 = success, 1 = error (text via nec_error_message()).
+ */
+#pragma once
+
+#include <stdint.h>
+
+#include "libnecpp.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Library and device identification: writes a NUL-terminated text of at
+ * most len bytes ("<gcn arch> <device name> <library version>"). */
+long nec_gpu_info(char* out, int len);
+
+/* Number of segments of a completed geometry (-1 before geometry_complete). */
+long nec_gpu_segment_count(nec_context* ctx);
+
+/* Segment table after geometry_complete (metres): out receives 8*n doubles,
+ * per segment x, y, z (centre), length, radius, cos(alpha)cos(beta),
+ * cos(alpha)sin(beta), sin(alpha); conn receives 3*n ints: icon1, icon2,
+ * tag.  Either pointer may be NULL. */
+long nec_gpu_segments(nec_context* ctx, double* out, int* conn, long n);
+
+/* Precision of the factorisation for the following XQ / RP cards:
+ * 0 = FP64 LU (default), 1 = complex FP32 LU on MFMA with FP64 iterative
+ * refinement against the FP64 matrix. */
+long nec_gpu_set_precision(nec_context* ctx, int mode);
+
+/* Segment currents (amperes, complex interleaved re,im) of antenna-input
+ * result freq_index: writes 2*n doubles, n = nec_gpu_segment_count(). */
+long nec_gpu_currents(nec_context* ctx, int freq_index, double* out, long n);
+
+/* Stage times in milliseconds of the most recent solve, measured with HIP
+ * events on the engine's stream: out[0] fill, out[1] factor, out[2] solve
+ * + current coefficients, out[3] pattern. */
+long nec_gpu_stage_times(nec_context* ctx, double* out);
+
+/* Benchmark: repeat the whole solve of the most recent XQ / RP at its last
+ * frequency (fill, LU, solve, current coefficients, Z_in and, after RP, the
+ * pattern) warmup + steps times entirely on the device, inputs resident in
+ * HBM, no host round trip inside a solve.  out[0] = ms per solve over the
+ * timed steps; out[1..4] = mean fill / factor / solve / pattern ms; out[5],
+ * out[6] = Z_in of the first feed from the last repetition. */
+long nec_gpu_benchmark_solve(nec_context* ctx, int warmup, int steps, double* out);
+
+/* Impedance matrix of the completed geometry at freq_mhz (FP64 fill only,
+ * no factorisation): out receives 2*n*n doubles, Z[i][k] (observation
+ * segment i, basis k) at out[2*(i*n+k)], i.e. nec2++'s "cm" layout.  Leaves
+ * the context needing a refill before the next solve. */
+long nec_gpu_fill_matrix(nec_context* ctx, double freq_mhz, double* out, long n);
+
+/* Pattern arrays of pattern result freq_index.  which: 0 gain, 1 vertical
+ * (or major-axis) gain, 2 horizontal (or minor-axis) gain, 3 total gain,
+ * 4 RHCP, 5 LHCP, 6 axial ratio, 7 tilt, 8 sense (as doubles), 9 averaging
+ * scale, 10 E_theta, 11 E_phi (complex: 2 doubles per point).  Points are
+ * ordered phi_index * n_theta + theta_index.  len = capacity in doubles. */
+long nec_gpu_pattern_array(nec_context* ctx, int freq_index, int which, double* out, long len);
+
+/* Average power gain and its solid angle / pi of pattern result freq_index
+ * (A = 1 or 2 on the RP card). */
+long nec_gpu_pattern_average(nec_context* ctx, int freq_index, double* avg_gain, double* solid_angle);
+
+/* ------------------------------------------------------------------ */
+/* Device-pointer entry points (no host round trip).  Matrices are      */
+/* row-major complex128 (interleaved), leading dimension lda entries.   */
+/* ------------------------------------------------------------------ */
+
+/* In-place partial-pivot LU of the n x n matrix at device pointer a.
+ * piv: device int buffer of 2*n entries (transpositions, then the composed
+ * row permutation).  info: device int, set to 1 + first zero pivot or 0.
+ * stream: hipStream_t (NULL = default stream). */
+long necgpu_zgetrf(double* a, long long lda, int n, int* piv, int* info, void* stream);
+
+/* Solve with the factors of necgpu_zgetrf: x (device, n complex) is
+ * overwritten.  work: device buffer of necgpu_zgetrs_workspace(n) bytes,
+ * zero-initialised once; epoch: strictly increasing per call on the same
+ * workspace. */
+long long necgpu_zgetrs_workspace(int n);
+long necgpu_zgetrs(const double* lu, long long lda, int n, const int* piv, double* x, void* work, unsigned epoch,
+                   void* stream);
+
+#ifdef __cplusplus
+}
+#endif

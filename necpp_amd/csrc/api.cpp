@@ -1,0 +1,346 @@
+// api.cpp -- the extern "C" boundary.
+//
+// Same conventions as nec2++'s libNEC.cpp (src/libNEC.cpp:27-282): card
+// functions catch the engine's exceptions, keep the message in one
+// process-global string and return 1; getters return -999.0 when the result
+// does not exist.
+#include <cstring>
+#include <string>
+
+#include "../../include/libnecpp.h"
+#include "../../include/necpp_gpu.h"
+#include "context.h"
+
+using necgpu::Context;
+using necgpu::NecError;
+
+namespace {
+
+std::string g_err_message;
+
+template <class F>
+long guarded(F&& f) {
+  try {
+    f();
+  } catch (const NecError& e) {
+    g_err_message = e.what();
+    return 1;
+  } catch (const std::exception& e) {
+    g_err_message = e.what();
+    return 1;
+  }
+  return 0;
+}
+
+long unsupported(const char* card) {
+  g_err_message = std::string(card) + ": not supported by the GPU path";
+  return 1;
+}
+
+const necgpu::PatternResult* pat(nec_context* c, int k) { return c ? c->pattern(k) : nullptr; }
+
+}  // namespace
+
+#pragma GCC visibility push(default)
+extern "C" {
+
+nec_context* nec_create(void) {
+  try {
+    return new Context();
+  } catch (const std::exception& e) {
+    g_err_message = e.what();
+    return nullptr;
+  }
+}
+
+long nec_delete(nec_context* ctx) {
+  return guarded([&] { delete ctx; });
+}
+
+long nec_benchmark(void) { return 0; }
+
+long nec_wire(nec_context* ctx, int tag_id, int segment_count, double xw1, double yw1, double zw1, double xw2,
+              double yw2, double zw2, double rad, double rdel, double rrad) {
+  return guarded([&] { ctx->wire(tag_id, segment_count, xw1, yw1, zw1, xw2, yw2, zw2, rad, rdel, rrad); });
+}
+
+long nec_sp_card(nec_context*, int, double, double, double, double, double, double) { return unsupported("SP"); }
+long nec_sc_card(nec_context*, int, double, double, double, double, double, double) { return unsupported("SC"); }
+long nec_gm_card(nec_context*, int, int, double, double, double, double, double, double, int) {
+  return unsupported("GM");
+}
+long nec_gx_card(nec_context*, int, int) { return unsupported("GX"); }
+
+long nec_geometry_complete(nec_context* ctx, int gpflag) {
+  return guarded([&] { ctx->geometry_complete(gpflag); });
+}
+
+const char* nec_error_message(void) { return g_err_message.c_str(); }
+
+long nec_medium_parameters(nec_context*, double, double) { return unsupported("MP"); }
+
+long nec_gn_card(nec_context*, int iperf, int, double, double, double, double, double, double) {
+  if (iperf == -1) return 0;  // GN -1: free space, which is the only ground here
+  return unsupported("GN");
+}
+
+long nec_fr_card(nec_context* ctx, int in_ifrq, int in_nfrq, double in_freq_mhz, double in_del_freq) {
+  return guarded([&] { ctx->fr_card(in_ifrq, in_nfrq, in_freq_mhz, in_del_freq); });
+}
+
+long nec_ek_card(nec_context* ctx, int itmp1) {
+  return guarded([&] { ctx->ek_card(itmp1 != -1); });
+}
+
+long nec_ld_card(nec_context*, int, int, int, int, double, double, double) { return unsupported("LD"); }
+
+long nec_ex_card(nec_context* ctx, int extype, int i2, int i3, int i4, double tmp1, double tmp2, double tmp3,
+                 double tmp4, double tmp5, double tmp6) {
+  return guarded([&] { ctx->ex_card(extype, i2, i3, i4, tmp1, tmp2, tmp3, tmp4, tmp5, tmp6); });
+}
+
+long nec_excitation_voltage(nec_context* ctx, int tag, int segment, double v_real, double v_imag) {
+  return guarded([&] { ctx->ex_card(0, tag, segment, 0, v_real, v_imag, 0.0, 0.0, 0.0, 0.0); });
+}
+
+long nec_excitation_current(nec_context*, double, double, double, double, double, double) {
+  return unsupported("EX 4");
+}
+
+long nec_excitation_planewave(nec_context*, int, int, double, double, double, double, double, double) {
+  return unsupported("EX 1");
+}
+
+long nec_tl_card(nec_context*, int, int, int, int, double, double, double, double, double, double) {
+  return unsupported("TL");
+}
+long nec_nt_card(nec_context*, int, int, int, int, double, double, double, double, double, double) {
+  return unsupported("NT");
+}
+
+long nec_xq_card(nec_context* ctx, int itmp1) {
+  return guarded([&] { ctx->xq_card(itmp1); });
+}
+
+long nec_gd_card(nec_context*, double, double, double, double) { return unsupported("GD"); }
+
+long nec_rp_card(nec_context* ctx, int calc_mode, int n_theta, int n_phi, int output_format, int normalization, int D,
+                 int A, double theta0, double phi0, double delta_theta, double delta_phi, double radial_distance,
+                 double gain_norm) {
+  return guarded([&] {
+    ctx->rp_card(calc_mode, n_theta, n_phi, output_format, normalization, D, A, theta0, phi0, delta_theta, delta_phi,
+                 radial_distance, gain_norm);
+  });
+}
+
+long nec_pt_card(nec_context*, int, int, int, int) { return 0; }
+long nec_pq_card(nec_context*, int, int, int, int) { return 0; }
+
+long nec_kh_card(nec_context* ctx, double tmp1) {
+  return guarded([&] { ctx->kh_card(tmp1); });
+}
+
+long nec_ne_card(nec_context*, int, int, int, int, double, double, double, double, double, double) {
+  return unsupported("NE");
+}
+long nec_nh_card(nec_context*, int, int, int, int, double, double, double, double, double, double) {
+  return unsupported("NH");
+}
+long nec_cp_card(nec_context*, int, int, int, int) { return unsupported("CP"); }
+long nec_pl_card(nec_context*, char*, int, int, int, int) { return unsupported("PL"); }
+
+double nec_gain(nec_context* ctx, int freq_index, int theta_index, int phi_index) {
+  const auto* p = pat(ctx, freq_index);
+  if (!p) return -999.0;
+  if (theta_index < 0 || theta_index >= p->n_theta || phi_index < 0 || phi_index >= p->n_phi) return -999.0;
+  return p->a.gain[(size_t)phi_index * p->n_theta + theta_index];
+}
+
+#define NEC_STAT_GETTER(name, idx)                      \
+  double name(nec_context* ctx, int freq_index) {       \
+    const auto* p = pat(ctx, freq_index);               \
+    return p ? p->stats[idx] : -999.0;                  \
+  }
+NEC_STAT_GETTER(nec_gain_max, 0)
+NEC_STAT_GETTER(nec_gain_min, 1)
+NEC_STAT_GETTER(nec_gain_mean, 2)
+NEC_STAT_GETTER(nec_gain_sd, 3)
+NEC_STAT_GETTER(nec_gain_rhcp_max, 4)
+NEC_STAT_GETTER(nec_gain_rhcp_min, 5)
+NEC_STAT_GETTER(nec_gain_rhcp_mean, 6)
+NEC_STAT_GETTER(nec_gain_rhcp_sd, 7)
+NEC_STAT_GETTER(nec_gain_lhcp_max, 8)
+NEC_STAT_GETTER(nec_gain_lhcp_min, 9)
+NEC_STAT_GETTER(nec_gain_lhcp_mean, 10)
+NEC_STAT_GETTER(nec_gain_lhcp_sd, 11)
+#undef NEC_STAT_GETTER
+
+double nec_impedance_real_indexed(nec_context* ctx, int freq_index, int feed_index) {
+  const auto* in = ctx ? ctx->input(freq_index) : nullptr;
+  if (!in || feed_index < 0 || (size_t)feed_index >= in->impedance.size()) return -999.0;
+  return in->impedance[feed_index].re;
+}
+
+double nec_impedance_imag_indexed(nec_context* ctx, int freq_index, int feed_index) {
+  const auto* in = ctx ? ctx->input(freq_index) : nullptr;
+  if (!in || feed_index < 0 || (size_t)feed_index >= in->impedance.size()) return -999.0;
+  return in->impedance[feed_index].im;
+}
+
+double nec_impedance_real(nec_context* ctx, int freq_index) { return nec_impedance_real_indexed(ctx, freq_index, 0); }
+double nec_impedance_imag(nec_context* ctx, int freq_index) { return nec_impedance_imag_indexed(ctx, freq_index, 0); }
+
+/* ---------------------------- GPU extras ---------------------------- */
+
+long nec_gpu_info(char* out, int len) {
+  return guarded([&] {
+    hipDeviceProp_t prop;
+    int dev = 0;
+    std::string s;
+    if (hipGetDevice(&dev) == hipSuccess && hipGetDeviceProperties(&prop, dev) == hipSuccess)
+      s = std::string(prop.gcnArchName) + " " + prop.name;
+    else
+      s = "no-device";
+    s += " necpp_amd 0.1";
+    if (out && len > 0) {
+      std::strncpy(out, s.c_str(), (size_t)len - 1);
+      out[len - 1] = 0;
+    }
+  });
+}
+
+long nec_gpu_segment_count(nec_context* ctx) {
+  if (!ctx || !ctx->geometry().is_complete()) return -1;
+  return static_cast<long>(ctx->geometry().size());
+}
+
+long nec_gpu_segments(nec_context* ctx, double* out, int* conn, long n) {
+  return guarded([&] {
+    const auto& g = ctx->geometry();
+    if (!g.is_complete()) throw NecError("geometry is not complete");
+    if (n != g.size()) throw NecError("n does not match the segment count");
+    for (long i = 0; i < n; ++i) {
+      if (out) {
+        double* o = out + 8 * i;
+        o[0] = g.x[i];
+        o[1] = g.y[i];
+        o[2] = g.z[i];
+        o[3] = g.len[i];
+        o[4] = g.rad[i];
+        o[5] = g.cab[i];
+        o[6] = g.sab[i];
+        o[7] = g.salp[i];
+      }
+      if (conn) {
+        conn[i] = g.icon1[i];
+        conn[n + i] = g.icon2[i];
+        conn[2 * n + i] = g.tags[i];
+      }
+    }
+  });
+}
+
+long nec_gpu_set_precision(nec_context* ctx, int mode) {
+  return guarded([&] {
+    if (mode != 0 && mode != 1) throw NecError("precision mode must be 0 (FP64) or 1 (mixed)");
+    ctx->set_precision(mode);
+  });
+}
+
+long nec_gpu_currents(nec_context* ctx, int freq_index, double* out, long n) {
+  return guarded([&] {
+    const auto* in = ctx->input(freq_index);
+    if (!in) throw NecError("no antenna input result " + std::to_string(freq_index));
+    if (n < (long)in->segment_currents.size()) throw NecError("output too small");
+    for (size_t i = 0; i < in->segment_currents.size(); ++i) {
+      out[2 * i] = in->segment_currents[i].re;
+      out[2 * i + 1] = in->segment_currents[i].im;
+    }
+  });
+}
+
+long nec_gpu_stage_times(nec_context* ctx, double* out) {
+  return guarded([&] {
+    const auto& t = ctx->last_times();
+    out[0] = t.fill_ms;
+    out[1] = t.factor_ms;
+    out[2] = t.solve_ms;
+    out[3] = t.pattern_ms;
+  });
+}
+
+long nec_gpu_benchmark_solve(nec_context* ctx, int warmup, int steps, double* out) {
+  return guarded([&] {
+    if (warmup < 0 || steps < 1) throw NecError("benchmark: need warmup >= 0 and steps >= 1");
+    ctx->benchmark_solve(warmup, steps, out);
+  });
+}
+
+long nec_gpu_fill_matrix(nec_context* ctx, double freq_mhz, double* out, long n) {
+  return guarded([&] {
+    if (!ctx->geometry().is_complete()) throw NecError("geometry is not complete");
+    if (n != ctx->geometry().size()) throw NecError("n does not match the segment count");
+    std::vector<double> Z;
+    ctx->fill_matrix(freq_mhz, &Z);
+    std::memcpy(out, Z.data(), Z.size() * sizeof(double));
+  });
+}
+
+long nec_gpu_pattern_array(nec_context* ctx, int freq_index, int which, double* out, long len) {
+  return guarded([&] {
+    const auto* p = ctx->pattern(freq_index);
+    if (!p) throw NecError("no pattern result " + std::to_string(freq_index));
+    const auto& a = p->a;
+    const std::vector<double>* src = nullptr;
+    switch (which) {
+      case 0: src = &a.gain; break;
+      case 1: src = &a.pg_vert; break;
+      case 2: src = &a.pg_horiz; break;
+      case 3: src = &a.pg_tot; break;
+      case 4: src = &a.pg_rhcp; break;
+      case 5: src = &a.pg_lhcp; break;
+      case 6: src = &a.axial; break;
+      case 7: src = &a.tilt; break;
+      case 8: {
+        if (len < (long)a.sense.size()) throw NecError("output too small");
+        for (size_t i = 0; i < a.sense.size(); ++i) out[i] = a.sense[i];
+        return;
+      }
+      case 9: src = &a.avg_scale; break;
+      case 10: src = &a.e_theta; break;
+      case 11: src = &a.e_phi; break;
+      default: throw NecError("unknown pattern array " + std::to_string(which));
+    }
+    if (len < (long)src->size()) throw NecError("output too small");
+    std::memcpy(out, src->data(), src->size() * sizeof(double));
+  });
+}
+
+long nec_gpu_pattern_average(nec_context* ctx, int freq_index, double* avg_gain, double* solid_angle) {
+  return guarded([&] {
+    const auto* p = ctx->pattern(freq_index);
+    if (!p) throw NecError("no pattern result " + std::to_string(freq_index));
+    *avg_gain = p->average_power_gain;
+    *solid_angle = p->average_power_solid_angle;
+  });
+}
+
+long necgpu_zgetrf(double* a, long long lda, int n, int* piv, int* info, void* stream) {
+  return guarded([&] {
+    if (necgpu::launch_lu_fp64(a, lda, n, piv, info, static_cast<hipStream_t>(stream)))
+      necgpu::hip_check(hipGetLastError(), "necgpu_zgetrf");
+  });
+}
+
+long long necgpu_zgetrs_workspace(int n) { return static_cast<long long>(necgpu::solve_workspace_bytes(n)); }
+
+long necgpu_zgetrs(const double* lu, long long lda, int n, const int* piv, double* x, void* work, unsigned epoch,
+                   void* stream) {
+  return guarded([&] {
+    if (necgpu::launch_lu_solve_fp64(lu, lda, n, piv, x, work, epoch, static_cast<hipStream_t>(stream)))
+      necgpu::hip_check(hipGetLastError(), "necgpu_zgetrs");
+  });
+}
+
+}  // extern "C"
+#pragma GCC visibility pop

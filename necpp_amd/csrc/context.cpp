@@ -1,0 +1,385 @@
+// context.cpp -- card state machine of the nec_context behind the C API.
+#include "context.h"
+
+#include <cmath>
+
+namespace necgpu {
+
+Context::Context() { calc_prepare(); }
+Context::~Context() = default;
+
+DeviceSolver& Context::device() {
+  if (!dev_) dev_ = std::make_unique<DeviceSolver>();
+  return *dev_;
+}
+
+void Context::calc_prepare() {
+  // src/nec_context.cpp:123-164
+  iflow_ = 1;
+  state_ = MEMORY_ALLOC;
+  rkh_ = 1.;
+  use_ek_ = false;
+  ifar_ = -1;
+  freq_mhz_ = em_speed_of_light() / 1.0e6;
+  nfrq_ = 1;
+}
+
+void Context::reset_to_structure_loading() {
+  if (state_ != MEMORY_ALLOC) state_ = STRUCTURE_LOADING;
+}
+
+void Context::reset_to_excitation_setup() {
+  if (state_ != MEMORY_ALLOC && state_ != STRUCTURE_LOADING) state_ = EXCITATION_SETUP;
+}
+
+void Context::wire(int tag, int ns, double x1, double y1, double z1, double x2, double y2, double z2, double rad,
+                   double rdel, double rrad) {
+  geo_.add_wire(tag, ns, x1, y1, z1, x2, y2, z2, rad, rdel, rrad);
+}
+
+void Context::geometry_complete(int gpflag) {
+  geo_.complete(gpflag);
+  calc_prepare();
+}
+
+void Context::fr_card(int ifrq, int nfrq, double freq_mhz, double del_freq) {
+  // src/nec_context.cpp:406-420
+  ifrq_ = ifrq;
+  nfrq_ = (nfrq == 0) ? 1 : nfrq;
+  freq_mhz_ = freq_mhz;
+  delfrq_ = del_freq;
+  state_ = MEMORY_ALLOC;
+  iflow_ = 1;
+}
+
+void Context::ex_card(int type, int i2, int i3, int i4, double f1, double f2, double f3, double f4, double f5,
+                      double f6) {
+  // src/nec_context.cpp:519-612 (voltage sources only)
+  (void)i4;
+  (void)f3;
+  (void)f4;
+  (void)f5;
+  (void)f6;
+  if (type != 0) throw NecError("EX: excitation type " + std::to_string(type) + " not supported by the GPU path");
+  if (!geo_.is_complete()) throw NecError("EX: geometry is not complete");
+  if (iflow_ != 5) {
+    src_seg_.clear();
+    src_v_.clear();
+    iflow_ = 5;
+    reset_to_excitation_setup();
+  }
+  const int64_t seg = geo_.segment_number(i2, i3);
+  if (seg > geo_.size())
+    throw NecError("CHECK DATA, PARAMETER SPECIFYING EXCITATION SOURCE SEGMENT [" + std::to_string(seg) +
+                   "] IS TOO LARGE");
+  zc v = zmk(f1, f2);
+  if (std::hypot(v.re, v.im) < 1.e-20) v = zmk(1.0, 0.0);
+  src_seg_.push_back(seg);
+  src_v_.push_back(v);
+}
+
+void Context::kh_card(double rkh) {
+  rkh_ = rkh;
+  reset_to_structure_loading();
+  iflow_ = 1;
+}
+
+void Context::ek_card(bool on) {
+  use_ek_ = on;
+  reset_to_structure_loading();
+  iflow_ = 1;
+}
+
+void Context::xq_card(int itmp1) {
+  // src/nec_context.cpp:741-779
+  if ((iflow_ == 10 && itmp1 == 0) || (nfrq_ == 1 && itmp1 == 0 && iflow_ > 7)) return;
+  if (itmp1 == 0) {
+    iflow_ = (iflow_ > 7) ? 11 : 7;
+    last_was_pattern_ = false;
+  } else {
+    ifar_ = 0;
+    rfld_ = 0.;
+    ipd_ = 0;
+    iavp_ = 0;
+    rp_norm_ = 0;
+    rp_format_ = 0;
+    nth_ = 91;
+    nph_ = 1;
+    thets_ = 0.0;
+    phis_ = 0.0;
+    dth_ = 1.0;
+    dph_ = 0.0;
+    if (itmp1 == 2) phis_ = 90.0;
+    if (itmp1 == 3) {
+      nph_ = 2;
+      dph_ = 90.0;
+    }
+  }
+  simulate(true);
+}
+
+void Context::rp_card(int calc_mode, int n_theta, int n_phi, int output_format, int normalization, int D, int A,
+                      double theta0, double phi0, double delta_theta, double delta_phi, double radial_distance,
+                      double gain_norm) {
+  // src/nec_context.cpp:828-868
+  if (calc_mode != 0)
+    throw NecError("RP: calculation mode " + std::to_string(calc_mode) + " not supported by the GPU path");
+  ifar_ = calc_mode;
+  nth_ = (n_theta == 0) ? 1 : n_theta;
+  nph_ = (n_phi == 0) ? 1 : n_phi;
+  rp_format_ = (output_format != 0) ? 1 : 0;
+  rp_norm_ = normalization;
+  ipd_ = (D != 0) ? 1 : 0;
+  iavp_ = A;
+  if (nth_ < 2 || nph_ < 2 || ifar_ == 1) iavp_ = 0;
+  if (rp_norm_ < 0 || rp_norm_ > 5) throw NecError("Unknown Gain Normalization Encountered.");
+  thets_ = theta0;
+  phis_ = phi0;
+  dth_ = delta_theta;
+  dph_ = delta_phi;
+  rfld_ = radial_distance;
+  gnor_ = gain_norm;
+  iflow_ = 10;
+  simulate(true);
+}
+
+void Context::simulate(bool far_field_flag) {
+  // src/nec_context.cpp:1031-1159 for voltage excitation.
+  if (!geo_.is_complete()) throw NecError("geometry is not complete");
+  State igox = (far_field_flag && state_ == NEAR_FIELD) ? FAR_FIELD : state_;
+  int mhz = 1;
+  do {
+    if (igox <= MEMORY_ALLOC) {
+      if (mhz != 1) {
+        if (ifrq_ == 1) freq_mhz_ *= delfrq_;
+        else freq_mhz_ += delfrq_;
+      }
+      wavelength_ = em_speed_of_light() / (1.0e6 * freq_mhz_);
+      device().load(geo_, geo_.scaled(freq_mhz_), use_ek_);
+      state_ = STRUCTURE_LOADING;
+    }
+    if (igox <= STRUCTURE_LOADING) {
+      device().fill(rkh_);
+      device().factor();
+      times_.fill_ms = device().times().fill_ms;
+      times_.factor_ms = device().times().factor_ms;
+      state_ = EXCITATION_SETUP;
+    }
+    // excitation_loop, src/nec_context.cpp:1793-1855
+    if (igox < EXCITATION_LOOP) {
+      excitation(mhz);
+      state_ = EXCITATION_LOOP;
+      if (iflow_ == 7) {
+        if (nfrq_ != 1) continue;
+        return;
+      }
+    }
+    if (igox < NEAR_FIELD) state_ = NEAR_FIELD;
+    far_field();
+    if (mhz == nfrq_) ifar_ = -1;
+    if (nfrq_ == 1) return;
+  } while (++mhz <= nfrq_);
+}
+
+void Context::excitation(int mhz) {
+  // setup_excitation + netwk (src/nec_context.cpp:1561-1617, 3193-3210,
+  // 4850-5047): RHS of the voltage sources, solve, current coefficients,
+  // antenna input parameters.
+  (void)mhz;
+  const int n = static_cast<int>(geo_.size());
+  const ScaledSegments seg = geo_.scaled(freq_mhz_);
+  std::vector<zc> e(n, zmk(0.0, 0.0));
+  for (size_t s = 0; s < src_seg_.size(); ++s) {
+    const int64_t k = src_seg_[s] - 1;
+    e[k] = zdivr(zneg(src_v_[s]), seg.s[k] * wavelength_);
+  }
+  std::vector<zc> curx;
+  device().solve(e, &curx, &abc_);
+  times_.solve_ms = device().times().solve_ms;
+  if (src_seg_.empty()) return;  // the reference records no antenna input without sources
+  AntennaInput in;
+  in.freq_hz = freq_mhz_ / 1.e-6;
+  input_power_ = 0.0;
+  for (size_t s = 0; s < src_seg_.size(); ++s) {
+    const int64_t k = src_seg_[s] - 1;
+    const zc volt = src_v_[s];
+    const zc cur = zmulr(curx[k], wavelength_);
+    const zc adm = zdiv(cur, volt);
+    const zc imp = zdiv(volt, cur);
+    const double power = 0.5 * zmul(volt, zconj(cur)).re;
+    input_power_ += power;
+    in.tag.push_back(geo_.tags[k]);
+    in.segment.push_back(static_cast<int>(k + 1));
+    in.voltage.push_back(volt);
+    in.current.push_back(cur);
+    in.impedance.push_back(imp);
+    in.admittance.push_back(adm);
+    in.power.push_back(power);
+  }
+  in.segment_currents.resize(n);
+  for (int i = 0; i < n; ++i) in.segment_currents[i] = zmulr(curx[i], wavelength_);
+  inputs_.push_back(std::move(in));
+}
+
+PatternSpec Context::pattern_spec(std::vector<double>* trig_out) const {
+  // gain constants and angle tables of nec_radiation_pattern::analyze
+  // (src/nec_radiation_pattern.cpp:221-262)
+  const double eta = em_impedance();
+  const double pinr = input_power_, pnlr = 0.0;
+  PatternSpec sp;
+  sp.n_theta = nth_;
+  sp.n_phi = nph_;
+  sp.output_format = rp_format_;
+  sp.normalization = rp_norm_;
+  sp.ipd = ipd_;
+  sp.iavp = iavp_;
+  sp.wavelength = wavelength_;
+  sp.gcop = wavelength_ * wavelength_ * 2.0 * kPi / (eta * pinr);
+  const double prad = pinr - 0.0 - pnlr;
+  sp.gcon = sp.gcop;
+  if (ipd_ != 0) sp.gcon = sp.gcon * pinr / prad;
+  sp.rfld = rfld_;
+  sp.exrm = 0.;
+  sp.exra = 0.;
+  if (rfld_ >= 1.0e-20) {
+    sp.exrm = 1. / rfld_;
+    sp.exra = rfld_ / wavelength_;
+    sp.exra = -360. * (sp.exra - std::floor(sp.exra));
+  }
+  sp.dphi_rad = deg_to_rad(dph_);
+  sp.half_dtheta_rad = 0.5 * deg_to_rad(dth_);
+  // the reference accumulates the angles by repeated addition
+  std::vector<double> trig(3 * (size_t)nth_ + 2 * (size_t)nph_);
+  double thet = thets_ - dth_;
+  for (int k = 0; k < nth_; ++k) {
+    thet += dth_;
+    const double tha = deg_to_rad(thet);
+    trig[k] = std::sin(tha);
+    trig[nth_ + k] = std::cos(tha);
+    trig[2 * nth_ + k] = tha;
+  }
+  double phi = phis_ - dph_;
+  for (int k = 0; k < nph_; ++k) {
+    phi += dph_;
+    const double pha = deg_to_rad(phi);
+    trig[3 * nth_ + k] = std::sin(pha);
+    trig[3 * nth_ + nph_ + k] = std::cos(pha);
+  }
+  *trig_out = std::move(trig);
+  return sp;
+}
+
+void Context::far_field() {
+  // excitation_compute_far_field + nec_radiation_pattern::analyze
+  // (src/nec_context.cpp:1761-1791, src/nec_radiation_pattern.cpp:209-450)
+  if (ifar_ == -1) return;
+  last_was_pattern_ = true;
+  PatternResult pr;
+  pr.freq_hz = freq_mhz_ / 1.e-6;
+  pr.n_theta = nth_;
+  pr.n_phi = nph_;
+  pr.theta0 = thets_;
+  pr.phi0 = phis_;
+  pr.dtheta = dth_;
+  pr.dphi = dph_;
+  pr.normalization = rp_norm_;
+  pr.output_format = rp_format_;
+  pr.iavp = iavp_;
+  std::vector<double> trig;
+  const PatternSpec sp = pattern_spec(&trig);
+  device().pattern(sp, trig, &pr.a);
+  times_.pattern_ms = device().times().pattern_ms;
+  const int64_t nd = (int64_t)nth_ * nph_;
+  if (iavp_ != 0) {
+    double pint = 0.0;
+    for (int64_t d = 0; d < nd; ++d) pint += pr.a.pint[d];
+    const double ts = deg_to_rad(thets_);
+    const double tr = deg_to_rad(dth_) * static_cast<double>(nth_ - 1);
+    const double prr = deg_to_rad(dph_) * static_cast<double>(nph_ - 1);
+    const double sa = std::fabs(prr * (std::cos(ts) - std::cos(ts + tr)));
+    pr.average_power_gain = pint / sa;
+    pr.average_power_solid_angle = sa / kPi;
+  }
+  double mx = pr.a.gain[0];
+  for (int64_t d = 1; d < nd; ++d)
+    if (pr.a.gain[d] > mx) mx = pr.a.gain[d];
+  pr.max_gain = mx;
+  // statistics, nec_radiation_pattern.h:252-333 (rows = theta, cols = phi)
+  const std::vector<double>* arrs[3] = {&pr.a.pg_tot, &pr.a.pg_rhcp, &pr.a.pg_lhcp};
+  for (int a = 0; a < 3; ++a) {
+    const std::vector<double>& p = *arrs[a];
+    double mxv = p[0], mnv = p[0];
+    for (int64_t d = 1; d < nd; ++d) {
+      if (p[d] > mxv) mxv = p[d];
+      if (p[d] < mnv) mnv = p[d];
+    }
+    double sum = 0.0;
+    for (int i = 0; i < nth_; ++i)
+      for (int j = 0; j < nph_; ++j) sum += p[(int64_t)j * nth_ + i] * pr.a.avg_scale[(int64_t)j * nth_ + i];
+    const double norm = static_cast<double>(nd * 2) / kPi;
+    const double mean = sum / norm;
+    double s2 = 0.0;
+    for (int i = 0; i < nth_; ++i)
+      for (int j = 0; j < nph_; ++j) {
+        const double diff = p[(int64_t)j * nth_ + i] - mean;
+        s2 += diff * diff * pr.a.avg_scale[(int64_t)j * nth_ + i];
+      }
+    pr.stats[4 * a + 0] = mxv;
+    pr.stats[4 * a + 1] = mnv;
+    pr.stats[4 * a + 2] = mean;
+    pr.stats[4 * a + 3] = std::sqrt(s2 / norm);
+  }
+  patterns_.push_back(std::move(pr));
+}
+
+const AntennaInput* Context::input(int k) const {
+  if (k < 0 || k >= static_cast<int>(inputs_.size())) return nullptr;
+  return &inputs_[k];
+}
+
+const PatternResult* Context::pattern(int k) const {
+  if (k < 0 || k >= static_cast<int>(patterns_.size())) return nullptr;
+  return &patterns_[k];
+}
+
+void Context::benchmark_solve(int warmup, int steps, double* out) {
+  if (!dev_ || inputs_.empty()) throw NecError("benchmark_solve needs a completed XQ or RP first");
+  const int n = static_cast<int>(geo_.size());
+  const ScaledSegments seg = geo_.scaled(freq_mhz_);
+  std::vector<zc> e(n, zmk(0.0, 0.0));
+  std::vector<int> segs;
+  for (size_t s = 0; s < src_seg_.size(); ++s) {
+    const int64_t k = src_seg_[s] - 1;
+    e[k] = zdivr(zneg(src_v_[s]), seg.s[k] * wavelength_);
+    segs.push_back(static_cast<int>(k));
+  }
+  std::vector<double> trig;
+  PatternSpec sp{};
+  const PatternSpec* psp = nullptr;
+  if (last_was_pattern_ && nth_ > 0 && nph_ > 0) {
+    sp = pattern_spec(&trig);
+    psp = &sp;
+  }
+  double ms = 0;
+  StageTimes st;
+  std::vector<double> zin;
+  device().benchmark_pipeline(rkh_, e, segs, src_v_, wavelength_, psp, trig, warmup, steps, &ms, &st, &zin);
+  out[0] = ms;
+  out[1] = st.fill_ms;
+  out[2] = st.factor_ms;
+  out[3] = st.solve_ms;
+  out[4] = st.pattern_ms;
+  out[5] = zin.empty() ? 0.0 : zin[0];
+  out[6] = zin.empty() ? 0.0 : zin[1];
+}
+
+void Context::fill_matrix(double freq_mhz, std::vector<double>* Z) {
+  if (!geo_.is_complete()) throw NecError("geometry is not complete");
+  DeviceSolver& d = device();
+  d.load(geo_, geo_.scaled(freq_mhz), use_ek_);
+  d.fill(rkh_);
+  d.download_matrix(Z);
+  state_ = MEMORY_ALLOC;  // the device matrix no longer holds factors
+}
+
+}  // namespace necgpu

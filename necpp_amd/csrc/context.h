@@ -1,0 +1,113 @@
+// context.h -- the nec_context behind the C interface.
+//
+// Card state and the simulate() state machine of nec_context
+// (src/nec_context.cpp:84-164, 406-420, 519-612, 741-868, 1031-1159,
+// 1523-1855) for the wire / free-space / voltage-source subset, with the
+// numerical stages delegated to the DeviceSolver.  Results accumulate in
+// order and are fetched by ordinal like nec_results (src/nec_results.h:711-722).
+#pragma once
+
+#include <memory>
+#include <string>
+#include <vector>
+
+#include "engine.h"
+#include "geometry.h"
+
+namespace necgpu {
+
+struct AntennaInput {
+  double freq_hz = 0;
+  std::vector<int> tag, segment;
+  std::vector<zc> voltage, current, impedance, admittance;
+  std::vector<double> power;
+  std::vector<zc> segment_currents;  // all segments, amperes (curx * wavelength)
+};
+
+struct PatternResult {
+  double freq_hz = 0;
+  int n_theta = 0, n_phi = 0;
+  double theta0 = 0, phi0 = 0, dtheta = 0, dphi = 0;
+  int normalization = 0, output_format = 0, iavp = 0;
+  HostPattern a;
+  double max_gain = -999.0;
+  double average_power_gain = 0, average_power_solid_angle = 0;
+  double stats[12] = {0};  // tot/rhcp/lhcp x (max, min, mean, sd)
+};
+
+class Context {
+ public:
+  Context();
+  ~Context();
+
+  // cards
+  void wire(int tag, int ns, double x1, double y1, double z1, double x2, double y2, double z2, double rad,
+            double rdel, double rrad);
+  void geometry_complete(int gpflag);
+  void fr_card(int ifrq, int nfrq, double freq_mhz, double del_freq);
+  void ex_card(int type, int i2, int i3, int i4, double f1, double f2, double f3, double f4, double f5, double f6);
+  void xq_card(int itmp1);
+  void rp_card(int calc_mode, int n_theta, int n_phi, int output_format, int normalization, int D, int A,
+               double theta0, double phi0, double delta_theta, double delta_phi, double radial_distance,
+               double gain_norm);
+  void kh_card(double rkh);
+  void ek_card(bool on);
+
+  // results
+  const AntennaInput* input(int k) const;
+  const PatternResult* pattern(int k) const;
+  int input_count() const { return static_cast<int>(inputs_.size()); }
+  int pattern_count() const { return static_cast<int>(patterns_.size()); }
+
+  // GPU extras
+  void set_precision(int mode) { precision_ = mode; }
+  const StageTimes& last_times() const { return times_; }
+  const Geometry& geometry() const { return geo_; }
+  double wavelength() const { return wavelength_; }
+  double frequency_mhz() const { return freq_mhz_; }
+  bool use_ek() const { return use_ek_; }
+  double rkh() const { return rkh_; }
+  // Fill only (no factorisation) at the current frequency; Z row-major 2*n*n.
+  void fill_matrix(double freq_mhz, std::vector<double>* Z);
+  // Repeat the last frequency's whole solve (fill, LU, solve, Z_in and, if
+  // the last card was RP, the pattern) on the device.  out: ms per solve,
+  // fill / factor / solve / pattern ms, Z_in of feed 0 (re, im).
+  void benchmark_solve(int warmup, int steps, double* out);
+  DeviceSolver& device();
+
+ private:
+  enum State { MEMORY_ALLOC = 1, STRUCTURE_LOADING = 2, EXCITATION_SETUP = 3, EXCITATION_LOOP = 4,
+               NEAR_FIELD = 5, FAR_FIELD = 6 };
+  void calc_prepare();
+  void simulate(bool far_field_flag);
+  void excitation(int mhz);
+  void far_field();
+  PatternSpec pattern_spec(std::vector<double>* trig) const;
+  void reset_to_structure_loading();
+  void reset_to_excitation_setup();
+
+  Geometry geo_;
+  std::unique_ptr<DeviceSolver> dev_;
+  int iflow_ = 1;
+  State state_ = MEMORY_ALLOC;
+  int ifrq_ = 0, nfrq_ = 1;
+  double freq_mhz_ = 0, delfrq_ = 0, wavelength_ = 0;
+  double rkh_ = 1.0;
+  bool use_ek_ = false;
+  int precision_ = 0;
+  // voltage sources (EX 0)
+  std::vector<int64_t> src_seg_;
+  std::vector<zc> src_v_;
+  // RP card
+  int ifar_ = -1;
+  int nth_ = 0, nph_ = 0, rp_format_ = 0, rp_norm_ = 0, ipd_ = 0, iavp_ = 0;
+  double thets_ = 0, phis_ = 0, dth_ = 0, dph_ = 0, rfld_ = 0, gnor_ = 0;
+  double input_power_ = 0;
+  bool last_was_pattern_ = false;
+  std::vector<double> abc_;  // current coefficients of the last solve
+  StageTimes times_;
+  std::vector<AntennaInput> inputs_;
+  std::vector<PatternResult> patterns_;
+};
+
+}  // namespace necgpu

@@ -1,0 +1,456 @@
+// engine.cpp -- device-resident MoM solver for one structure.
+#include "engine.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdlib>
+#include <cstring>
+
+#include "basis.h"
+
+namespace necgpu {
+
+void hip_check(hipError_t e, const char* what) {
+  if (e != hipSuccess) throw NecError(std::string("HIP error in ") + what + ": " + hipGetErrorString(e));
+}
+
+namespace {
+
+// NECGPU_FAIL_ALLOC=<bytes>: every device allocation of at least that size
+// fails, so tests can drive the allocation-failure path from the host.
+size_t forced_alloc_failure_threshold() {
+  const char* v = std::getenv("NECGPU_FAIL_ALLOC");
+  if (!v || !*v) return 0;
+  return static_cast<size_t>(std::strtoull(v, nullptr, 10));
+}
+
+constexpr int kTableCap = 64;
+constexpr size_t kFieldBudget = size_t(1) << 30;  // bytes of E staged per fill pass
+
+// test_ek_approximation, src/c_geometry.cpp:3449-3458
+int ek_test(const ScaledSegments& s, int s1, int s2) {
+  const double ratio = s.b[s2] / s.b[s1];
+  const double xi = std::fabs(s.cab[s1] * s.cab[s2] + s.sab[s1] * s.sab[s2] + s.salp[s1] * s.salp[s2]);
+  return (xi < 0.999999 || std::fabs(ratio - 1.0) > 1.e-6) ? 2 : 0;
+}
+
+// End conditions of the extended kernel for source j, cmww
+// src/nec_context.cpp:2535-2601.
+void ek_ends(const Geometry& geo, const ScaledSegments& s, int j, int* ind1, int* ind2) {
+  const int jx = j + 1;
+  const double cabj = s.cab[j], sabj = s.sab[j];
+  int ipr = geo.icon1[j];
+  const int iprx = std::abs(ipr) - 1;
+  *ind1 = 2;
+  if (ipr > kPchcon) *ind1 = 0;
+  else if (ipr == 0) *ind1 = 1;
+  else if (ipr == jx && cabj * cabj + sabj * sabj <= 1.e-8) *ind1 = 0;
+  else if (ipr > 0 && geo.icon2[iprx] == jx) *ind1 = ek_test(s, j, ipr - 1);
+  else if (ipr < 0 && geo.icon1[iprx] == -jx) *ind1 = ek_test(s, j, -ipr - 1);
+  ipr = geo.icon2[j];
+  if (ipr > kPchcon) *ind2 = 2;
+  else if (ipr == 0) *ind2 = 1;
+  else if (ipr < 0) {
+    const int k = -ipr - 1;
+    *ind2 = (-geo.icon2[k] != jx) ? 2 : ek_test(s, j, k);
+  } else if (ipr != jx) {
+    const int k = ipr - 1;
+    *ind2 = (geo.icon1[k] != jx) ? 2 : ek_test(s, j, k);
+  } else {
+    *ind2 = (cabj * cabj + sabj * sabj > 1.e-8) ? 2 : 0;
+  }
+}
+
+template <class T>
+void upload(DevBuf& buf, const std::vector<T>& v, hipStream_t s) {
+  void* p = buf.ensure(std::max<size_t>(v.size() * sizeof(T), 1));
+  if (!v.empty()) hip_check(hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, s), "upload");
+}
+
+}  // namespace
+
+DevBuf::~DevBuf() { release(); }
+
+void DevBuf::release() {
+  if (ptr_) (void)hipFree(ptr_);
+  ptr_ = nullptr;
+  bytes_ = 0;
+}
+
+void* DevBuf::ensure(size_t bytes) {
+  if (bytes <= bytes_ && ptr_) return ptr_;
+  release();
+  const size_t fail_at = forced_alloc_failure_threshold();
+  if (fail_at && bytes >= fail_at)
+    throw NecError("HIP error in hipMalloc: out of memory (forced by NECGPU_FAIL_ALLOC)");
+  hip_check(hipMalloc(&ptr_, bytes ? bytes : 1), "hipMalloc");
+  bytes_ = bytes;
+  return ptr_;
+}
+
+DeviceSolver::DeviceSolver() {}
+
+DeviceSolver::~DeviceSolver() {
+  if (stream_) {
+    (void)hipStreamSynchronize(stream_);
+    for (auto& e : ev_)
+      if (e) (void)hipEventDestroy(e);
+    (void)hipStreamDestroy(stream_);
+  }
+}
+
+void DeviceSolver::ensure_device() {
+  if (stream_) return;
+  int count = 0;
+  if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
+    throw NecError("no HIP device available: the GPU path needs an MI355X (gfx950)");
+  hip_check(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking), "hipStreamCreate");
+  for (auto& e : ev_) hip_check(hipEventCreate(&e), "hipEventCreate");
+}
+
+float DeviceSolver::elapsed(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  hip_check(hipEventElapsedTime(&ms, a, b), "hipEventElapsedTime");
+  return ms;
+}
+
+DevSegments DeviceSolver::dev_segments() const {
+  const double* p = seg_.as<double>();
+  DevSegments g;
+  g.x = p;
+  g.y = p + n_;
+  g.z = p + 2 * n_;
+  g.s = p + 3 * n_;
+  g.b = p + 4 * n_;
+  g.cab = p + 5 * n_;
+  g.sab = p + 6 * n_;
+  g.salp = p + 7 * n_;
+  return g;
+}
+
+void DeviceSolver::load(const Geometry& geo, const ScaledSegments& seg, bool use_ek) {
+  ensure_device();
+  const int n = static_cast<int>(geo.size());
+  n_ = n;
+  ld_ = n;
+  use_ek_ = use_ek;
+  factored_ = false;
+
+  std::vector<double> segv(8 * (size_t)n);
+  for (int i = 0; i < n; ++i) {
+    segv[i] = seg.x[i];
+    segv[n + i] = seg.y[i];
+    segv[2 * n + i] = seg.z[i];
+    segv[3 * n + i] = seg.s[i];
+    segv[4 * n + i] = seg.b[i];
+    segv[5 * n + i] = seg.cab[i];
+    segv[6 * n + i] = seg.sab[i];
+    segv[7 * n + i] = seg.salp[i];
+  }
+
+  // Basis tables.  trio(j) gives the bases with support on source j; the
+  // fill gathers per column k in ascending j.  tbf(i) gives the support of
+  // basis i; the current coefficients gather per segment in ascending i.
+  std::vector<int> icon1(geo.icon1.begin(), geo.icon1.end()), icon2(geo.icon2.begin(), geo.icon2.end());
+  SegView sv{icon1.data(), icon2.data(), seg.s.data(), seg.b.data()};
+  std::vector<int> ccount(n + 1, 0), scount(n + 1, 0);
+  std::vector<int> tj, tk, sj, si;
+  std::vector<double> tv, sv3;
+  tj.reserve(8 * (size_t)n);
+  int jco[kTableCap];
+  double ax[kTableCap], bx[kTableCap], cx[kTableCap];
+  for (int j = 0; j < n; ++j) {
+    const int cnt = trio(sv, j + 1, jco, ax, bx, cx, kTableCap);
+    if (cnt < 0) throw NecError("TRIO - SEGMENT CONNECTION ERROR FOR SEGMENT " + std::to_string(j + 1));
+    for (int q = 0; q < cnt; ++q) {
+      tj.push_back(j);
+      tk.push_back(jco[q] - 1);
+      tv.push_back(ax[q]);
+      tv.push_back(bx[q]);
+      tv.push_back(cx[q]);
+      ++ccount[jco[q]];
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    const int cnt = tbf(sv, i + 1, 1, jco, ax, bx, cx, kTableCap);
+    if (cnt < 0) throw NecError("TBF - SEGMENT CONNECTION ERROR FOR SEGMENT " + std::to_string(i + 1));
+    for (int q = 0; q < cnt; ++q) {
+      si.push_back(i);
+      sj.push_back(jco[q] - 1);
+      sv3.push_back(ax[q]);
+      sv3.push_back(bx[q]);
+      sv3.push_back(cx[q]);
+      ++scount[jco[q]];
+    }
+  }
+  // counting sort keeps the source order (ascending j, resp. ascending i)
+  std::vector<int> cptr(n + 1, 0), sptr(n + 1, 0);
+  for (int k = 0; k < n; ++k) {
+    cptr[k + 1] = cptr[k] + ccount[k + 1];
+    sptr[k + 1] = sptr[k] + scount[k + 1];
+  }
+  std::vector<int> cj(tj.size()), sb(si.size()), cfill(cptr.begin(), cptr.end() - 1), sfill(sptr.begin(), sptr.end() - 1);
+  std::vector<double> cv(tv.size()), sval(sv3.size());
+  for (size_t e = 0; e < tj.size(); ++e) {
+    const int p = cfill[tk[e]]++;
+    cj[p] = tj[e];
+    cv[3 * p] = tv[3 * e];
+    cv[3 * p + 1] = tv[3 * e + 1];
+    cv[3 * p + 2] = tv[3 * e + 2];
+  }
+  for (size_t e = 0; e < si.size(); ++e) {
+    const int p = sfill[sj[e]]++;
+    sb[p] = si[e];
+    sval[3 * p] = sv3[3 * e];
+    sval[3 * p + 1] = sv3[3 * e + 1];
+    sval[3 * p + 2] = sv3[3 * e + 2];
+  }
+  std::vector<int> ek(2 * (size_t)n, 2);
+  if (use_ek)
+    for (int j = 0; j < n; ++j) ek_ends(geo, seg, j, &ek[2 * j], &ek[2 * j + 1]);
+
+  upload(seg_, segv, stream_);
+  upload(colptr_, cptr, stream_);
+  upload(colj_, cj, stream_);
+  upload(colabc_, cv, stream_);
+  upload(segptr_, sptr, stream_);
+  upload(segbasis_, sb, stream_);
+  upload(segabc_, sval, stream_);
+  upload(ekind_, ek, stream_);
+  hip_check(hipStreamSynchronize(stream_), "load");
+}
+
+void DeviceSolver::enqueue_fill(double rkh) {
+  const int n = n_;
+  Z_.ensure((size_t)ld_ * n * 16);
+  const int rows = static_cast<int>(std::min<size_t>(n, std::max<size_t>(1, kFieldBudget / ((size_t)n * 48))));
+  E_.ensure((size_t)rows * n * 48);
+  const DevSegments g = dev_segments();
+  for (int i0 = 0; i0 < n; i0 += rows) {
+    const int r = std::min(rows, n - i0);
+    if (launch_fill(g, n, i0, r, rkh, use_ek_ ? 1 : 0, ekind_.as<int>(), E_.as<double>(), colptr_.as<int>(),
+                    colj_.as<int>(), colabc_.as<double>(), Z_.as<double>(), ld_, stream_))
+      hip_check(hipGetLastError(), "fill launch");
+  }
+}
+
+void DeviceSolver::enqueue_factor() {
+  const int n = n_;
+  piv_.ensure((size_t)2 * n * sizeof(int));
+  info_.ensure(sizeof(int));
+  hip_check(hipMemsetAsync(info_.as<void>(), 0, sizeof(int), stream_), "memset");
+  if (launch_lu_fp64(Z_.as<double>(), ld_, n, piv_.as<int>(), info_.as<int>(), stream_))
+    hip_check(hipGetLastError(), "lu launch");
+}
+
+void DeviceSolver::enqueue_solve(const void* rhs_dev, hipMemcpyKind kind, const void* rhs_host) {
+  const int n = n_;
+  x_.ensure((size_t)n * 16);
+  abc_.ensure((size_t)6 * n * sizeof(double));
+  curx_.ensure((size_t)n * 16);
+  const size_t wbytes = solve_workspace_bytes(n);
+  if (solve_work_.size() < wbytes) {
+    solve_work_.ensure(wbytes);
+    hip_check(hipMemsetAsync(solve_work_.as<void>(), 0, wbytes, stream_), "memset");
+    epoch_ = 0;
+  }
+  ++epoch_;
+  hip_check(hipMemcpyAsync(x_.as<void>(), rhs_dev ? rhs_dev : rhs_host, (size_t)n * 16, kind, stream_), "rhs");
+  if (launch_lu_solve_fp64(Z_.as<double>(), ld_, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(), epoch_,
+                           stream_))
+    hip_check(hipGetLastError(), "solve launch");
+  if (launch_currents(n, x_.as<double>(), segptr_.as<int>(), segbasis_.as<int>(), segabc_.as<double>(),
+                      abc_.as<double>(), curx_.as<double>(), stream_))
+    hip_check(hipGetLastError(), "currents launch");
+}
+
+PatternOut DeviceSolver::pattern_buffers(int nd) {
+  const size_t per = (size_t)nd;
+  double* base = static_cast<double*>(pat_out_.ensure(per * 15 * sizeof(double)));
+  int* sense = static_cast<int*>(pat_sense_.ensure(per * sizeof(int)));
+  PatternOut po;
+  po.gain = base;
+  po.pg_vert = base + per;
+  po.pg_horiz = base + 2 * per;
+  po.pg_tot = base + 3 * per;
+  po.pg_rhcp = base + 4 * per;
+  po.pg_lhcp = base + 5 * per;
+  po.axial = base + 6 * per;
+  po.tilt = base + 7 * per;
+  po.avg_scale = base + 8 * per;
+  po.pint = base + 9 * per;
+  po.e_theta = base + 10 * per;
+  po.e_phi = base + 12 * per;
+  po.sense = sense;
+  return po;
+}
+
+void DeviceSolver::enqueue_pattern(const PatternSpec& spec, const PatternOut& po) {
+  const int nd = spec.n_theta * spec.n_phi;
+  pat_part_.ensure((size_t)pattern_slices(n_, nd) * nd * 6 * sizeof(double));
+  if (launch_pattern(dev_segments(), n_, abc_.as<double>(), spec, trig_.as<double>(), pat_part_.as<double>(), po,
+                     stream_))
+    hip_check(hipGetLastError(), "pattern launch");
+}
+
+void DeviceSolver::fill(double rkh) {
+  hip_check(hipEventRecord(ev_[0], stream_), "event");
+  enqueue_fill(rkh);
+  hip_check(hipEventRecord(ev_[1], stream_), "event");
+  hip_check(hipEventSynchronize(ev_[1]), "fill");
+  times_.fill_ms = elapsed(ev_[0], ev_[1]);
+  factored_ = false;
+}
+
+void DeviceSolver::factor() {
+  hip_check(hipEventRecord(ev_[0], stream_), "event");
+  enqueue_factor();
+  hip_check(hipEventRecord(ev_[1], stream_), "event");
+  int info = 0;
+  hip_check(hipMemcpyAsync(&info, info_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
+  hip_check(hipStreamSynchronize(stream_), "factor");
+  times_.factor_ms = elapsed(ev_[0], ev_[1]);
+  lu_info_ = info;
+  factored_ = true;
+}
+
+void DeviceSolver::solve(const std::vector<zc>& e, std::vector<zc>* curx, std::vector<double>* abc) {
+  if (!factored_) throw NecError("solve requested before the matrix was factored");
+  const int n = n_;
+  hip_check(hipEventRecord(ev_[0], stream_), "event");
+  enqueue_solve(nullptr, hipMemcpyHostToDevice, e.data());
+  hip_check(hipEventRecord(ev_[1], stream_), "event");
+  if (curx) {
+    curx->resize(n);
+    hip_check(hipMemcpyAsync(curx->data(), curx_.as<void>(), (size_t)n * 16, hipMemcpyDeviceToHost, stream_), "curx");
+  }
+  if (abc) {
+    abc->resize(6 * (size_t)n);
+    hip_check(hipMemcpyAsync(abc->data(), abc_.as<void>(), (size_t)6 * n * 8, hipMemcpyDeviceToHost, stream_), "abc");
+  }
+  int status = 0;
+  const int nblk = (n + 63) / 64;
+  hip_check(hipMemcpyAsync(&status, static_cast<char*>(solve_work_.as<void>()) + 2 * nblk * sizeof(unsigned),
+                           sizeof(int), hipMemcpyDeviceToHost, stream_),
+            "status");
+  hip_check(hipStreamSynchronize(stream_), "solve");
+  if (status) throw NecError("triangular solve: block wait timed out");
+  times_.solve_ms = elapsed(ev_[0], ev_[1]);
+}
+
+void DeviceSolver::pattern(const PatternSpec& spec, const std::vector<double>& trig, HostPattern* out) {
+  const int nd = spec.n_theta * spec.n_phi;
+  upload(trig_, trig, stream_);
+  const PatternOut po = pattern_buffers(nd);
+  hip_check(hipMemsetAsync(pat_out_.as<void>(), 0, (size_t)nd * 15 * sizeof(double), stream_), "memset");
+  hip_check(hipEventRecord(ev_[0], stream_), "event");
+  enqueue_pattern(spec, po);
+  hip_check(hipEventRecord(ev_[1], stream_), "event");
+  auto get = [&](std::vector<double>& v, const double* src, size_t cnt) {
+    v.resize(cnt);
+    hip_check(hipMemcpyAsync(v.data(), src, cnt * sizeof(double), hipMemcpyDeviceToHost, stream_), "pattern copy");
+  };
+  get(out->gain, po.gain, nd);
+  get(out->pg_vert, po.pg_vert, nd);
+  get(out->pg_horiz, po.pg_horiz, nd);
+  get(out->pg_tot, po.pg_tot, nd);
+  get(out->pg_rhcp, po.pg_rhcp, nd);
+  get(out->pg_lhcp, po.pg_lhcp, nd);
+  get(out->axial, po.axial, nd);
+  get(out->tilt, po.tilt, nd);
+  get(out->avg_scale, po.avg_scale, nd);
+  get(out->pint, po.pint, nd);
+  get(out->e_theta, po.e_theta, 2 * (size_t)nd);
+  get(out->e_phi, po.e_phi, 2 * (size_t)nd);
+  out->sense.resize(nd);
+  hip_check(hipMemcpyAsync(out->sense.data(), po.sense, (size_t)nd * sizeof(int), hipMemcpyDeviceToHost, stream_),
+            "pattern copy");
+  hip_check(hipStreamSynchronize(stream_), "pattern");
+  times_.pattern_ms = elapsed(ev_[0], ev_[1]);
+}
+
+void DeviceSolver::benchmark_pipeline(double rkh, const std::vector<zc>& e, const std::vector<int>& src_seg,
+                                      const std::vector<zc>& src_v, double wavelength, const PatternSpec* spec,
+                                      const std::vector<double>& trig, int warmup, int steps, double* ms_per_solve,
+                                      StageTimes* stage_ms, std::vector<double>* zin_out) {
+  const int n = n_;
+  const int nsrc = static_cast<int>(src_seg.size());
+  upload(rhs_, e, stream_);
+  std::vector<double> srcv(2 * (size_t)nsrc);
+  for (int s = 0; s < nsrc; ++s) {
+    srcv[2 * s] = src_v[s].re;
+    srcv[2 * s + 1] = src_v[s].im;
+  }
+  std::vector<int> srcs(src_seg);
+  // voltages first (16-byte aligned), then the segment indices
+  src_.ensure(std::max<size_t>(1, (size_t)nsrc * (sizeof(int) + 16)));
+  if (nsrc) {
+    hip_check(hipMemcpyAsync(src_.as<void>(), srcv.data(), nsrc * 16, hipMemcpyHostToDevice, stream_), "src");
+    hip_check(hipMemcpyAsync(static_cast<char*>(src_.as<void>()) + nsrc * 16, srcs.data(), nsrc * sizeof(int),
+                             hipMemcpyHostToDevice, stream_),
+              "src");
+  }
+  const double* dvolt = src_.as<double>();
+  const int* dseg = reinterpret_cast<const int*>(static_cast<char*>(src_.as<void>()) + nsrc * 16);
+  zin_.ensure(std::max<size_t>(1, (size_t)nsrc * 64));
+  PatternOut po{};
+  if (spec) {
+    upload(trig_, trig, stream_);
+    po = pattern_buffers(spec->n_theta * spec->n_phi);
+  }
+  StageTimes acc;
+  hipEvent_t t0 = ev_[4], t1 = ev_[5];
+  for (int it = 0; it < warmup + steps; ++it) {
+    const bool timed = it >= warmup;
+    if (it == warmup) hip_check(hipEventRecord(t0, stream_), "event");
+    if (timed) hip_check(hipEventRecord(ev_[0], stream_), "event");
+    enqueue_fill(rkh);
+    if (timed) hip_check(hipEventRecord(ev_[1], stream_), "event");
+    enqueue_factor();
+    if (timed) hip_check(hipEventRecord(ev_[2], stream_), "event");
+    enqueue_solve(rhs_.as<void>(), hipMemcpyDeviceToDevice, nullptr);
+    if (launch_zin(nsrc, dseg, dvolt, curx_.as<double>(), wavelength, zin_.as<double>(), stream_))
+      hip_check(hipGetLastError(), "zin launch");
+    if (timed) hip_check(hipEventRecord(ev_[3], stream_), "event");
+    if (spec) enqueue_pattern(*spec, po);
+    if (timed) {
+      // per-stage times of this step (the events are reused, so wait here;
+      // the wait sits outside the stage intervals)
+      hip_check(hipEventSynchronize(ev_[3]), "event");
+      acc.fill_ms += elapsed(ev_[0], ev_[1]);
+      acc.factor_ms += elapsed(ev_[1], ev_[2]);
+      acc.solve_ms += elapsed(ev_[2], ev_[3]);
+    }
+  }
+  hip_check(hipEventRecord(t1, stream_), "event");
+  hip_check(hipEventSynchronize(t1), "event");
+  const double total = elapsed(t0, t1);
+  *ms_per_solve = steps > 0 ? total / steps : 0.0;
+  if (stage_ms) {
+    stage_ms->fill_ms = acc.fill_ms / std::max(1, steps);
+    stage_ms->factor_ms = acc.factor_ms / std::max(1, steps);
+    stage_ms->solve_ms = acc.solve_ms / std::max(1, steps);
+    stage_ms->pattern_ms = *ms_per_solve - stage_ms->fill_ms - stage_ms->factor_ms - stage_ms->solve_ms;
+  }
+  if (zin_out) {
+    zin_out->resize(8 * (size_t)nsrc);
+    hip_check(hipMemcpy(zin_out->data(), zin_.as<void>(), (size_t)nsrc * 64, hipMemcpyDeviceToHost), "zin");
+  }
+  (void)n;
+  factored_ = true;
+}
+
+void DeviceSolver::download_matrix(std::vector<double>* Z) {
+  const size_t cnt = (size_t)2 * n_ * n_;
+  Z->resize(cnt);
+  if (ld_ == n_) {
+    hip_check(hipMemcpyAsync(Z->data(), Z_.as<void>(), cnt * 8, hipMemcpyDeviceToHost, stream_), "matrix copy");
+  } else {
+    hip_check(hipMemcpy2DAsync(Z->data(), (size_t)n_ * 16, Z_.as<void>(), (size_t)ld_ * 16, (size_t)n_ * 16, n_,
+                               hipMemcpyDeviceToHost, stream_),
+              "matrix copy");
+  }
+  hip_check(hipStreamSynchronize(stream_), "matrix copy");
+}
+
+}  // namespace necgpu

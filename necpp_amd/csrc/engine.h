@@ -1,0 +1,119 @@
+// engine.h -- device-resident MoM solver for one structure (one GPU).
+//
+// Owns the HBM buffers of the path and runs the stages that nec_context's
+// simulate() runs for a voltage-excited structure at one frequency:
+//   fill      cmset                 -> Z (row-major, "cm" layout)
+//   factor    factrs / lu_decompose -> L\U in place, pivots
+//   solve     etmns -> solves -> get_current_coefficients -> Z_in
+//   pattern   nec_radiation_pattern::analyze / ffld
+// Host-side tables (segments in wavelengths, basis coefficients) are built
+// from the Geometry and uploaded once per frequency.  Every stage has an
+// enqueue_* form that only queues work on the solver's stream, so a whole
+// solve can run without a host round trip (benchmark_pipeline).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "geometry.h"
+#include "kernels.h"
+#include "nec_math.h"
+
+namespace necgpu {
+
+void hip_check(hipError_t e, const char* what);
+
+// Device allocation that grows on demand and frees on destruction.
+class DevBuf {
+ public:
+  DevBuf() = default;
+  DevBuf(const DevBuf&) = delete;
+  DevBuf& operator=(const DevBuf&) = delete;
+  ~DevBuf();
+  void* ensure(size_t bytes);
+  void release();
+  template <class T>
+  T* as() const {
+    return static_cast<T*>(ptr_);
+  }
+  size_t size() const { return bytes_; }
+
+ private:
+  void* ptr_ = nullptr;
+  size_t bytes_ = 0;
+};
+
+struct StageTimes {
+  double fill_ms = 0, factor_ms = 0, solve_ms = 0, pattern_ms = 0;
+};
+
+// Per-direction pattern arrays on the host (n_theta * n_phi each).
+struct HostPattern {
+  std::vector<double> gain, pg_vert, pg_horiz, pg_tot, pg_rhcp, pg_lhcp, axial, tilt, avg_scale, pint;
+  std::vector<int> sense;
+  std::vector<double> e_theta, e_phi;  // interleaved complex
+};
+
+class DeviceSolver {
+ public:
+  DeviceSolver();
+  ~DeviceSolver();
+  DeviceSolver(const DeviceSolver&) = delete;
+  DeviceSolver& operator=(const DeviceSolver&) = delete;
+
+  // Upload the segment table and basis tables of one frequency.
+  void load(const Geometry& geo, const ScaledSegments& seg, bool use_ek);
+
+  // synchronous stages (timed with HIP events)
+  void fill(double rkh);
+  void factor();
+  // e: right-hand side (n complex).  On return curx = segment-centre currents
+  // (get_current_coefficients, not yet scaled by the wavelength) and abc (6n)
+  // the current coefficients.
+  void solve(const std::vector<zc>& e, std::vector<zc>* curx, std::vector<double>* abc);
+  // Pattern of the last solve.  trig: sin/cos/angle tables, see kernels.h.
+  void pattern(const PatternSpec& spec, const std::vector<double>& trig, HostPattern* out);
+
+  // Whole solve (fill, LU, solve, currents, Z_in, pattern) repeated on the
+  // device with inputs already in HBM; returns the mean time per solve and
+  // the mean per-stage times.  sources: 0-based segments and voltages.
+  void benchmark_pipeline(double rkh, const std::vector<zc>& e, const std::vector<int>& src_seg,
+                          const std::vector<zc>& src_v, double wavelength, const PatternSpec* spec,
+                          const std::vector<double>& trig, int warmup, int steps, double* ms_per_solve,
+                          StageTimes* stage_ms, std::vector<double>* zin_out);
+
+  void download_matrix(std::vector<double>* Z);  // current content of Z (2*n*n)
+  int n() const { return n_; }
+  int lu_info() const { return lu_info_; }
+  const StageTimes& times() const { return times_; }
+  hipStream_t stream() const { return stream_; }
+
+ private:
+  void ensure_device();
+  DevSegments dev_segments() const;
+  void enqueue_fill(double rkh);
+  void enqueue_factor();
+  void enqueue_solve(const void* rhs_dev, hipMemcpyKind kind, const void* rhs_host);
+  PatternOut pattern_buffers(int nd);
+  void enqueue_pattern(const PatternSpec& spec, const PatternOut& po);
+  float elapsed(hipEvent_t a, hipEvent_t b);
+
+  int n_ = 0;
+  long long ld_ = 0;
+  bool use_ek_ = false;
+  bool factored_ = false;
+  int lu_info_ = 0;
+  unsigned epoch_ = 0;
+  hipStream_t stream_ = nullptr;
+  hipEvent_t ev_[6] = {};
+  StageTimes times_;
+
+  DevBuf seg_, ekind_, colptr_, colj_, colabc_, segptr_, segbasis_, segabc_;
+  DevBuf Z_, E_, piv_, info_, x_, abc_, curx_, solve_work_, rhs_, src_, zin_;
+  DevBuf trig_, pat_part_, pat_out_, pat_sense_;
+};
+
+}  // namespace necgpu

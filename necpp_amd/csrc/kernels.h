@@ -1,0 +1,63 @@
+// kernels.h -- launch interface of the HIP kernels (host-callable).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace necgpu {
+
+// Segment tables on the device, in wavelengths (structure of arrays).
+struct DevSegments {
+  const double *x, *y, *z, *s, *b, *cab, *sab, *salp;
+};
+
+// fill.hip: fields of rows [i0, i0+rows) into E (rows*n*6 doubles), then
+// gather into Z (row-major complex, leading dimension ldz complex entries).
+int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind, double* E,
+                const int* colptr, const int* colj, const double* colabc, double* Z, long long ldz,
+                hipStream_t stream);
+
+// lu.hip: in-place partial-pivot LU of the row-major complex n x n matrix A
+// (leading dimension lda complex).  piv[k] = row exchanged with row k at step
+// k.  info (device int) receives 1 + the first exactly-zero pivot, or stays 0.
+int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, hipStream_t stream);
+
+// solve.hip: x <- A^-1 x with the LU factors of launch_lu_fp64.  work: device
+// scratch of solve_workspace_bytes(n) bytes; epoch: a value that increases
+// with every call on the same workspace.
+size_t solve_workspace_bytes(int n);
+int launch_lu_solve_fp64(const double* LU, long long lda, int n, const int* piv, double* x, void* work,
+                         unsigned epoch, hipStream_t stream);
+
+// current.hip: basis amplitudes x (complex) -> per-segment coefficients
+// abc (6*n: air, aii, bir, bii, cir, cii) and segment-centre currents curx.
+int launch_currents(int n, const double* x, const int* segptr, const int* segbasis, const double* segabc,
+                    double* abc, double* curx, hipStream_t stream);
+
+// current.hip: antenna input parameters of nsrc voltage sources (seg 0-based,
+// volt complex): out[8*s..] = Z, Y, I (complex), P.
+int launch_zin(int nsrc, const int* seg, const double* volt, const double* curx, double wavelength, double* out,
+               hipStream_t stream);
+
+// pattern.hip
+struct PatternSpec {
+  int n_theta, n_phi;
+  int output_format, normalization, ipd, iavp;
+  double wavelength, gcon, gcop, exrm, exra, rfld;
+  double dphi_rad, half_dtheta_rad;
+};
+struct PatternOut {
+  double *gain, *pg_vert, *pg_horiz, *pg_tot, *pg_rhcp, *pg_lhcp, *axial, *tilt, *avg_scale;
+  int* sense;
+  double *e_theta, *e_phi, *pint;  // e_*: 2 doubles per direction
+};
+// Directions d = kph * n_theta + kth; trig holds per-theta (sin, cos) then
+// per-phi (sin, cos) of the reference's accumulated angles, and the angles
+// themselves (radians) for theta.
+// Number of segment slices launch_pattern uses (scratch = slices*n_dir*6 doubles).
+int pattern_slices(int n, int n_dir);
+int launch_pattern(const DevSegments& g, int n, const double* abc, const PatternSpec& spec, const double* trig,
+                   double* eth_eph_scratch, const PatternOut& out, hipStream_t stream);
+
+}  // namespace necgpu

@@ -1,0 +1,112 @@
+"""Benchmark and test structures as lists of GW cards.
+
+Each deck is a dict with
+  wires:  list of (tag, ns, x1, y1, z1, x2, y2, z2, rad, rdel, rrad)
+  gpflag: GE flag
+  freq_mhz, source (tag, segment_in_tag), voltage
+and optionally rp (the nec_rp_card arguments).  `apply(deck, sink)` feeds
+the cards to anything with nec_wire / nec_geometry_complete style methods.
+"""
+
+C_REF = 299792458.0  # only used to size the Yagi designs in wavelengths
+
+
+def dipole(ns=51, half=0.25, rad=0.001, freq_mhz=299.792458, rp=(181, 361)):
+    """Centre-fed straight dipole along z (the N=51 CPU-runnable case)."""
+    deck = {
+        "name": f"dipole{ns}",
+        "wires": [(1, ns, 0.0, 0.0, -half, 0.0, 0.0, half, rad, 1.0, 1.0)],
+        "gpflag": 0,
+        "freq_mhz": freq_mhz,
+        "source": (1, ns // 2 + 1),
+        "voltage": 1.0 + 0.0j,
+    }
+    if rp:
+        deck["rp"] = (0, rp[0], rp[1], 1, 0, 0, 0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0)
+    return deck
+
+
+def _node(k, nn, spacing):
+    return float(f"{-(nn - 1) * spacing / 2.0 + k * spacing:.10f}")
+
+
+def grid(nn=41, spacing=0.05, rad=0.001, freq_mhz=299.792458, rp=(91, 361)):
+    """Planar square wire grid in z = 0 with nn x nn nodes, one segment per edge.
+
+    Horizontal edges row by row first, then vertical edges column by column;
+    tag k+1 for the k-th edge.  The source is the first edge whose centre is
+    nearest the origin (nn=41 -> tag 820, N=3,280; nn=90 -> tag 3961,
+    N=16,020).
+    """
+    xs = [_node(k, nn, spacing) for k in range(nn)]
+    wires = []
+    for r in range(nn):
+        for c in range(nn - 1):
+            wires.append((0, 1, xs[c], xs[r], 0.0, xs[c + 1], xs[r], 0.0, rad, 1.0, 1.0))
+    for c in range(nn):
+        for r in range(nn - 1):
+            wires.append((0, 1, xs[c], xs[r], 0.0, xs[c], xs[r + 1], 0.0, rad, 1.0, 1.0))
+    best, btag = None, None
+    out = []
+    for k, w in enumerate(wires):
+        tag = k + 1
+        out.append((tag,) + w[1:])
+        cx, cy = 0.5 * (w[2] + w[5]), 0.5 * (w[3] + w[6])
+        d = cx * cx + cy * cy
+        if best is None or d < best - 1e-15:
+            best, btag = d, tag
+    deck = {
+        "name": f"grid{nn}",
+        "wires": out,
+        "gpflag": 0,
+        "freq_mhz": freq_mhz,
+        "source": (btag, 1),
+        "voltage": 1.0 + 0.0j,
+    }
+    if rp:
+        deck["rp"] = (0, rp[0], rp[1], 1, 0, 0, 0, 0.0, 0.0, 1.0, 1.0, 0.0, 0.0)
+    return deck
+
+
+def yagi(half_lengths, spacings, rad, seg_per_element=21, freq_mhz=145.0, rp=(1, 361)):
+    """Yagi-Uda array: elements parallel to y, boom along x, source on the
+    centre segment of element 2 (lengths in metres)."""
+    wires = []
+    x = 0.0
+    for e, h in enumerate(half_lengths):
+        if e > 0:
+            x += spacings[e - 1]
+        wires.append((e + 1, seg_per_element, x, -h, 0.0, x, h, 0.0, rad, 1.0, 1.0))
+    deck = {
+        "name": f"yagi{len(half_lengths)}x{seg_per_element}",
+        "wires": wires,
+        "gpflag": 0,
+        "freq_mhz": freq_mhz,
+        "source": (2, seg_per_element // 2 + 1),
+        "voltage": 1.0 + 0.0j,
+    }
+    if rp:
+        # E-plane cut: theta = 90 deg (the array plane), phi 0..360
+        deck["rp"] = (0, rp[0], rp[1], 1, 0, 0, 0, 90.0, 0.0, 0.0, 1.0, 0.0, 0.0)
+    return deck
+
+
+def apply_to_api(deck, api, ctx, with_rp=True):
+    """Run the deck through a libnecpp-style API object (module or wrapper)."""
+    for w in deck["wires"]:
+        rc = api.nec_wire(ctx, *w)
+        if rc != 0:
+            raise RuntimeError(api.nec_error_message())
+    for call, args in (("nec_geometry_complete", (deck["gpflag"],)),
+                       ("nec_fr_card", (0, 1, deck["freq_mhz"], 0.0)),
+                       ("nec_ex_card", (0, deck["source"][0], deck["source"][1], 0,
+                                        deck["voltage"].real, deck["voltage"].imag, 0.0, 0.0, 0.0, 0.0))):
+        rc = getattr(api, call)(ctx, *args)
+        if rc != 0:
+            raise RuntimeError(api.nec_error_message())
+    if with_rp and "rp" in deck:
+        rc = api.nec_rp_card(ctx, *deck["rp"])
+    else:
+        rc = api.nec_xq_card(ctx, 0)
+    if rc != 0:
+        raise RuntimeError(api.nec_error_message())

@@ -1,0 +1,164 @@
+"""Python binding of libnecpp_amd.so with the reference's C API names.
+
+``from necpp_amd.necpp import *`` gives the same functions the external
+``necpp`` module exposes to nec2++'s Python tests (nec_create, nec_wire, ...,
+nec_impedance_real), plus the GPU extras of include/necpp_gpu.h.  Every call
+goes through the C ABI; there is no Python or CPU fallback: loading fails
+loudly when the library has not been built.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libnecpp_amd.so")
+
+_lib = None
+
+# (name, restype, argtypes) of every exported entry point
+_D, _I, _L, _P, _LL, _U = ctypes.c_double, ctypes.c_int, ctypes.c_long, ctypes.c_void_p, ctypes.c_longlong, ctypes.c_uint
+_SIGS = [
+    ("nec_create", _P, []),
+    ("nec_delete", _L, [_P]),
+    ("nec_benchmark", _L, []),
+    ("nec_wire", _L, [_P, _I, _I] + [_D] * 9),
+    ("nec_sp_card", _L, [_P, _I] + [_D] * 6),
+    ("nec_sc_card", _L, [_P, _I] + [_D] * 6),
+    ("nec_gm_card", _L, [_P, _I, _I] + [_D] * 6 + [_I]),
+    ("nec_gx_card", _L, [_P, _I, _I]),
+    ("nec_geometry_complete", _L, [_P, _I]),
+    ("nec_error_message", ctypes.c_char_p, []),
+    ("nec_medium_parameters", _L, [_P, _D, _D]),
+    ("nec_gn_card", _L, [_P, _I, _I] + [_D] * 6),
+    ("nec_fr_card", _L, [_P, _I, _I, _D, _D]),
+    ("nec_ek_card", _L, [_P, _I]),
+    ("nec_ld_card", _L, [_P, _I, _I, _I, _I, _D, _D, _D]),
+    ("nec_ex_card", _L, [_P, _I, _I, _I, _I] + [_D] * 6),
+    ("nec_excitation_voltage", _L, [_P, _I, _I, _D, _D]),
+    ("nec_excitation_current", _L, [_P] + [_D] * 6),
+    ("nec_excitation_planewave", _L, [_P, _I, _I] + [_D] * 6),
+    ("nec_tl_card", _L, [_P, _I, _I, _I, _I] + [_D] * 6),
+    ("nec_nt_card", _L, [_P, _I, _I, _I, _I] + [_D] * 6),
+    ("nec_xq_card", _L, [_P, _I]),
+    ("nec_gd_card", _L, [_P, _D, _D, _D, _D]),
+    ("nec_rp_card", _L, [_P, _I, _I, _I, _I, _I, _I, _I] + [_D] * 6),
+    ("nec_pt_card", _L, [_P, _I, _I, _I, _I]),
+    ("nec_pq_card", _L, [_P, _I, _I, _I, _I]),
+    ("nec_kh_card", _L, [_P, _D]),
+    ("nec_ne_card", _L, [_P, _I, _I, _I, _I] + [_D] * 6),
+    ("nec_nh_card", _L, [_P, _I, _I, _I, _I] + [_D] * 6),
+    ("nec_cp_card", _L, [_P, _I, _I, _I, _I]),
+    ("nec_pl_card", _L, [_P, ctypes.c_char_p, _I, _I, _I, _I]),
+    ("nec_gain", _D, [_P, _I, _I, _I]),
+] + [(f"nec_gain{s}_{k}", _D, [_P, _I]) for s in ("", "_rhcp", "_lhcp") for k in ("max", "min", "mean", "sd")] + [
+    ("nec_impedance_real", _D, [_P, _I]),
+    ("nec_impedance_imag", _D, [_P, _I]),
+    ("nec_impedance_real_indexed", _D, [_P, _I, _I]),
+    ("nec_impedance_imag_indexed", _D, [_P, _I, _I]),
+    ("nec_gpu_info", _L, [ctypes.c_char_p, _I]),
+    ("nec_gpu_segment_count", _L, [_P]),
+    ("nec_gpu_segments", _L, [_P, _P, _P, _L]),
+    ("nec_gpu_set_precision", _L, [_P, _I]),
+    ("nec_gpu_currents", _L, [_P, _I, _P, _L]),
+    ("nec_gpu_stage_times", _L, [_P, _P]),
+    ("nec_gpu_benchmark_solve", _L, [_P, _I, _I, _P]),
+    ("nec_gpu_fill_matrix", _L, [_P, _D, _P, _L]),
+    ("nec_gpu_pattern_array", _L, [_P, _I, _I, _P, _L]),
+    ("nec_gpu_pattern_average", _L, [_P, _I, _P, _P]),
+    ("necgpu_zgetrf", _L, [_P, _LL, _I, _P, _P, _P]),
+    ("necgpu_zgetrs_workspace", _LL, [_I]),
+    ("necgpu_zgetrs", _L, [_P, _LL, _I, _P, _P, _P, _U, _P]),
+]
+
+EXPORTS = [s[0] for s in _SIGS]
+
+
+def lib():
+    """Load libnecpp_amd.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is not built: run `python -m necpp_amd.build`")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, res, args in _SIGS:
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _wrap(name):
+    def f(*args):
+        return getattr(lib(), name)(*args)
+    f.__name__ = name
+    return f
+
+
+for _name in EXPORTS:
+    if _name not in ("nec_error_message",):
+        globals()[_name] = _wrap(_name)
+
+
+def nec_error_message():
+    m = lib().nec_error_message()
+    return m.decode() if m else ""
+
+
+# ------------------------------------------------------------------ helpers
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+def gpu_currents(ctx, freq_index=0):
+    n = lib().nec_gpu_segment_count(ctx)
+    out = np.zeros(2 * n)
+    if lib().nec_gpu_currents(ctx, freq_index, _ptr(out), n) != 0:
+        raise RuntimeError(nec_error_message())
+    return out[0::2] + 1j * out[1::2]
+
+
+def gpu_segments(ctx):
+    """(table n x 8, icon1, icon2, tags) of a completed geometry."""
+    n = lib().nec_gpu_segment_count(ctx)
+    out = np.zeros(8 * n)
+    conn = np.zeros(3 * n, np.int32)
+    if lib().nec_gpu_segments(ctx, _ptr(out), _ptr(conn), n) != 0:
+        raise RuntimeError(nec_error_message())
+    return out.reshape(n, 8), conn[:n].copy(), conn[n:2 * n].copy(), conn[2 * n:].copy()
+
+
+def gpu_stage_times(ctx):
+    out = np.zeros(4)
+    if lib().nec_gpu_stage_times(ctx, _ptr(out)) != 0:
+        raise RuntimeError(nec_error_message())
+    return dict(zip(("fill_ms", "factor_ms", "solve_ms", "pattern_ms"), out.tolist()))
+
+
+def gpu_fill_matrix(ctx, freq_mhz):
+    n = lib().nec_gpu_segment_count(ctx)
+    out = np.zeros(2 * n * n)
+    if lib().nec_gpu_fill_matrix(ctx, freq_mhz, _ptr(out), n) != 0:
+        raise RuntimeError(nec_error_message())
+    return (out[0::2] + 1j * out[1::2]).reshape(n, n)
+
+
+PATTERN_ARRAYS = ("gain", "pg_vert", "pg_horiz", "pg_tot", "pg_rhcp", "pg_lhcp", "axial_ratio", "tilt", "sense",
+                  "avg_scale", "e_theta", "e_phi")
+
+
+def gpu_pattern_array(ctx, freq_index, name, npoints):
+    which = PATTERN_ARRAYS.index(name)
+    cplx = name in ("e_theta", "e_phi")
+    out = np.zeros(npoints * (2 if cplx else 1))
+    if lib().nec_gpu_pattern_array(ctx, freq_index, which, _ptr(out), out.size) != 0:
+        raise RuntimeError(nec_error_message())
+    return out[0::2] + 1j * out[1::2] if cplx else out
+
+
+def gpu_info():
+    buf = ctypes.create_string_buffer(256)
+    lib().nec_gpu_info(buf, 256)
+    return buf.value.decode()

@@ -1,0 +1,1498 @@
+/*
+ * necpp_oracle.c -- CPU restatement of nec2++'s free-space thin-wire MoM
+ * path.  TEST INFRASTRUCTURE ONLY: used by tests/, the smoke check and the
+ * bench's cpu_baseline leg as the checker; the product never calls it.
+ *
+ * Every function names the reference routine it restates.  The arithmetic
+ * keeps the reference's evaluation order, and complex operations are
+ * spelled out the way g++ evaluates std::complex<double> on x86-64 without
+ * FMA: a*b = (ar*br - ai*bi, ar*bi + ai*br); a/b (both complex) = libgcc's
+ * Smith division; complex*real and complex/real act per component.
+ * Build with -ffp-contract=off.
+ */
+#include "necpp_oracle.h"
+
+#include <math.h>
+#include <stdarg.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Complex helpers (std::complex<double> semantics under g++)          */
+/* ------------------------------------------------------------------ */
+typedef struct { double re, im; } cx;
+
+static inline cx mk(double r, double i) { cx z; z.re = r; z.im = i; return z; }
+static inline cx cadd(cx a, cx b) { return mk(a.re + b.re, a.im + b.im); }
+static inline cx csub(cx a, cx b) { return mk(a.re - b.re, a.im - b.im); }
+static inline cx cneg(cx a) { return mk(-a.re, -a.im); }
+static inline cx cmul(cx a, cx b) { return mk(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re); }
+static inline cx cmulr(cx a, double s) { return mk(a.re * s, a.im * s); }
+static inline cx cdivr(cx a, double s) { return mk(a.re / s, a.im / s); }
+/* libgcc __divdc3 (Smith's method), the path g++ takes for complex/complex. */
+static inline cx cdiv(cx a, cx b) {
+  double ratio, denom;
+  cx r;
+  if (fabs(b.re) < fabs(b.im)) {
+    ratio = b.re / b.im;
+    denom = (b.re * ratio) + b.im;
+    r.re = ((a.re * ratio) + a.im) / denom;
+    r.im = ((a.im * ratio) - a.re) / denom;
+  } else {
+    ratio = b.im / b.re;
+    denom = (b.im * ratio) + b.re;
+    r.re = ((a.im * ratio) + a.re) / denom;
+    r.im = (a.im - (a.re * ratio)) / denom;
+  }
+  return r;
+}
+
+/* ------------------------------------------------------------------ */
+/* Constants (src/math_util.h:72-107, src/electromag.cpp:22-47,         */
+/* src/common.h:57, src/c_geometry.h:26)                               */
+/* ------------------------------------------------------------------ */
+static const double PI_T = 3.1415926536;
+#define TWO_PI (2.0 * PI_T)
+#define FOUR_PI (4.0 * PI_T)
+#define SMIN 1.e-3
+#define PCHCON 100000
+
+static double eps0(void) { return 8.854e-12; }
+static double mu0(void) { return FOUR_PI * 1.0e-7; }
+double ora_speed_of_light(void) { return 1.0 / sqrt(mu0() * eps0()); }
+double ora_impedance(void) { return sqrt(mu0() / eps0()); }
+double ora_wavelength(double freq_mhz) { return ora_speed_of_light() / (1.0e6 * freq_mhz); }
+static double deg_to_rad(double d) { return d * ((2 * PI_T) / 360.0); }
+static double rad_to_deg(double r) { return r * (360.0 / (2 * PI_T)); }
+static double db10(double x) { return (x < 1.0e-20) ? -999.99 : 10.0 * log10(x); }
+static double from_db10(double x) { return (x < -99.9) ? 0.0 : pow(10, x / 10.0); }
+
+/* ------------------------------------------------------------------ */
+/* Errors                                                              */
+/* ------------------------------------------------------------------ */
+static char g_err[512];
+static void set_err(const char* fmt, ...) {
+  va_list ap;
+  va_start(ap, fmt);
+  vsnprintf(g_err, sizeof g_err, fmt, ap);
+  va_end(ap);
+}
+const char* ora_last_error(void) { return g_err; }
+
+/* ------------------------------------------------------------------ */
+/* Geometry state                                                      */
+/* ------------------------------------------------------------------ */
+struct ora_geom {
+  int64_t n, cap;
+  double *x, *y, *z, *x2, *y2, *z2, *rad;
+  int32_t* tag;
+  int32_t *icon1, *icon2;
+  double *len, *cab, *sab, *salp;   /* after geometry_complete */
+  double *xu, *yu, *zu, *su, *bu;   /* unscaled copies */
+  double *xs, *ys, *zs, *ss, *bs;   /* frequency-scaled */
+  int complete;
+};
+
+static void* xrealloc(void* p, size_t n) {
+  void* q = realloc(p, n ? n : 1);
+  if (!q) { fprintf(stderr, "oracle: out of memory\n"); abort(); }
+  return q;
+}
+
+ora_geom* ora_geom_new(void) { return (ora_geom*)calloc(1, sizeof(ora_geom)); }
+
+void ora_geom_free(ora_geom* g) {
+  if (!g) return;
+  double** d[] = {&g->x, &g->y, &g->z, &g->x2, &g->y2, &g->z2, &g->rad, &g->len, &g->cab, &g->sab, &g->salp,
+                  &g->xu, &g->yu, &g->zu, &g->su, &g->bu, &g->xs, &g->ys, &g->zs, &g->ss, &g->bs};
+  for (size_t i = 0; i < sizeof d / sizeof d[0]; i++) free(*d[i]);
+  free(g->tag); free(g->icon1); free(g->icon2);
+  free(g);
+}
+
+int64_t ora_nseg(const ora_geom* g) { return g->n; }
+
+static void grow(ora_geom* g, int64_t n) {
+  if (n <= g->cap) return;
+  int64_t c = g->cap ? g->cap : 16;
+  while (c < n) c *= 2;
+  size_t b = (size_t)c * sizeof(double);
+  g->x = xrealloc(g->x, b); g->y = xrealloc(g->y, b); g->z = xrealloc(g->z, b);
+  g->x2 = xrealloc(g->x2, b); g->y2 = xrealloc(g->y2, b); g->z2 = xrealloc(g->z2, b);
+  g->rad = xrealloc(g->rad, b);
+  g->tag = xrealloc(g->tag, (size_t)c * sizeof(int32_t));
+  g->cap = c;
+}
+
+/* c_geometry::wire, src/c_geometry.cpp:614-683. */
+int ora_wire(ora_geom* g, int tag, int ns, double xw1, double yw1, double zw1,
+             double xw2, double yw2, double zw2, double rad, double rdel, double rrad) {
+  if (ns < 1) return 0;
+  int64_t istart = g->n;
+  grow(g, g->n + ns);
+  g->n += ns;
+  g->complete = 0;
+  double dx = xw2 - xw1, dy = yw2 - yw1, dz = zw2 - zw1, delz, rd;
+  if (fabs(rdel - 1.0) >= 1.0e-6) {
+    delz = sqrt((dx * dx + dy * dy) + dz * dz);
+    dx /= delz; dy /= delz; dz /= delz;
+    delz = delz * (1. - rdel) / (1. - pow(rdel, ns));
+    rd = rdel;
+  } else {
+    double fns = ns;
+    dx /= fns; dy /= fns; dz /= fns;
+    delz = 1.0;
+    rd = 1.0;
+  }
+  double radz = rad, xs1 = xw1, ys1 = yw1, zs1 = zw1;
+  for (int64_t i = istart; i < g->n; i++) {
+    g->tag[i] = tag;
+    double xs2 = xs1 + dx * delz, ys2 = ys1 + dy * delz, zs2 = zs1 + dz * delz;
+    g->x[i] = xs1; g->y[i] = ys1; g->z[i] = zs1;
+    g->x2[i] = xs2; g->y2[i] = ys2; g->z2[i] = zs2;
+    g->rad[i] = radz;
+    delz = delz * rd;
+    radz = radz * rrad;
+    xs1 = xs2; ys1 = ys2; zs1 = zs2;
+  }
+  g->x2[g->n - 1] = xw2; g->y2[g->n - 1] = yw2; g->z2[g->n - 1] = zw2;
+  return 0;
+}
+
+static double l1(double ax, double ay, double az, double bx, double by, double bz) {
+  return (fabs(ax - bx) + fabs(ay - by)) + fabs(az - bz);
+}
+
+/* build_connections, src/c_geometry.cpp:1705-1915 (wires only). */
+static int build_connections(ora_geom* g, int ignd) {
+  int64_t n = g->n;
+  for (int64_t i = 0; i < n; i++) {
+    int64_t iz = i + 1;
+    double zi1 = g->z[i], zi2 = g->z2[i];
+    double ex = g->x2[i] - g->x[i], ey = g->y2[i] - g->y[i], ez = g->z2[i] - g->z[i];
+    double slen = sqrt((ex * ex + ey * ey) + ez * ez) * SMIN;
+    int on_ground = 0;
+    if (ignd > 0) {
+      if (zi1 <= -slen) { set_err("GEOMETRY DATA ERROR--SEGMENT %lld EXTENDS BELOW GROUND", (long long)iz); return 1; }
+      if (zi1 <= slen) { g->icon1[i] = (int32_t)iz; g->z[i] = 0.; on_ground = 1; }
+    }
+    if (!on_ground) {
+      double vx = g->x[i], vy = g->y[i], vz = g->z[i];
+      int64_t ic = i;
+      int found = 0;
+      for (int64_t j = 1; j < n; j++) {
+        ic++;
+        if (ic >= n) ic = 0;
+        if (l1(vx, vy, vz, g->x[ic], g->y[ic], g->z[ic]) <= slen) { g->icon1[i] = (int32_t)(-(ic + 1)); found = 1; break; }
+        if (l1(vx, vy, vz, g->x2[ic], g->y2[ic], g->z2[ic]) <= slen) { g->icon1[i] = (int32_t)(ic + 1); found = 1; break; }
+      }
+      if (!found) g->icon1[i] = 0;
+    }
+    if ((ignd > 0) || on_ground) {
+      if (zi2 <= -slen) { set_err("GEOMETRY DATA ERROR--SEGMENT %lld EXTENDS BELOW GROUND", (long long)iz); return 1; }
+      if (zi2 <= slen) {
+        if (g->icon1[i] == iz) { set_err("GEOMETRY DATA ERROR--SEGMENT %lld LIES IN GROUND PLANE", (long long)iz); return 1; }
+        g->icon2[i] = (int32_t)iz;
+        g->z2[i] = 0.;
+        continue;
+      }
+    }
+    double vx = g->x2[i], vy = g->y2[i], vz = g->z2[i];
+    int64_t ic = i;
+    int found = 0;
+    for (int64_t j = 1; j < n; j++) {
+      ic++;
+      if (ic >= n) ic = 0;
+      if (l1(vx, vy, vz, g->x[ic], g->y[ic], g->z[ic]) <= slen) { g->icon2[i] = (int32_t)(ic + 1); found = 1; break; }
+      if (l1(vx, vy, vz, g->x2[ic], g->y2[ic], g->z2[ic]) <= slen) { g->icon2[i] = (int32_t)(-(ic + 1)); found = 1; break; }
+    }
+    if (!found) g->icon2[i] = 0;
+  }
+  return 0;
+}
+
+/* resolve_junctions, src/c_geometry.cpp:1918-2074: average the coordinates
+ * of all segment ends meeting at a node, in connection-chain order. */
+static int resolve_junctions(ora_geom* g) {
+  int64_t n = g->n;
+  int64_t jcap = 8;
+  int64_t* jco = (int64_t*)malloc((size_t)jcap * sizeof(int64_t));
+  for (int64_t j = 0; j < n; j++) {
+    int64_t jx = j + 1;
+    int iend = -1, jend = -1;
+    int64_t ix = g->icon1[j];
+    int64_t ic = 1;
+    jco[0] = -jx;
+    double xa = g->x[j], ya = g->y[j], za = g->z[j];
+    for (;;) {
+      if ((ix != 0) && (ix != jx) && (ix <= n)) {
+        int jump = 0;
+        do {
+          if (ix < 0) ix = -ix; else jend = -jend;
+          jump = 0;
+          if (ix == jx) break;
+          if (ix < jx) { jump = 1; break; }
+          ic++;
+          if (ic > jcap) { jcap *= 2; jco = (int64_t*)xrealloc(jco, (size_t)jcap * sizeof(int64_t)); }
+          jco[ic - 1] = ix * jend;
+          int64_t ixx = ix - 1;
+          if (ixx < 0 || ixx >= n) { free(jco); set_err("CONNECT - SEGMENT CONNECTION ERROR (bad index): %lld", (long long)ix); return 1; }
+          if (jend != 1) {
+            xa = xa + g->x[ixx]; ya = ya + g->y[ixx]; za = za + g->z[ixx];
+            ix = g->icon1[ixx];
+          } else {
+            xa = xa + g->x2[ixx]; ya = ya + g->y2[ixx]; za = za + g->z2[ixx];
+            ix = g->icon2[ixx];
+          }
+        } while (ix != 0);
+        if (jump && (iend == 1)) break;
+        if (jump) {
+          iend = 1; jend = 1; ix = g->icon2[j]; ic = 1; jco[0] = jx;
+          xa = g->x2[j]; ya = g->y2[j]; za = g->z2[j];
+          continue;
+        }
+        double sep = (double)ic;
+        xa = xa / sep; ya = ya / sep; za = za / sep;
+        for (int64_t i = 0; i < ic; i++) {
+          int64_t e = jco[i];
+          if (e <= 0) {
+            int64_t ixx = -e - 1;
+            g->x[ixx] = xa; g->y[ixx] = ya; g->z[ixx] = za;
+          } else {
+            int64_t ixx = e - 1;
+            g->x2[ixx] = xa; g->y2[ixx] = ya; g->z2[ixx] = za;
+          }
+        }
+      }
+      if (iend == 1) break;
+      iend = 1; jend = 1; ix = g->icon2[j]; ic = 1; jco[0] = jx;
+      xa = g->x2[j]; ya = g->y2[j]; za = g->z2[j];
+    }
+  }
+  free(jco);
+  return 0;
+}
+
+/* c_geometry::geometry_complete, src/c_geometry.cpp:454-604 (wires). */
+int ora_geometry_complete(ora_geom* g, int gpflag) {
+  int64_t n = g->n;
+  if (n == 0) { set_err("Geometry has no wires or patches."); return 1; }
+  if (n <= 1) { set_err("GEOMETRY HAS ONE OR FEWER SEGMENTS and no patches."); return 1; }
+  size_t b = (size_t)n * sizeof(double), bi = (size_t)n * sizeof(int32_t);
+  g->icon1 = xrealloc(g->icon1, bi); g->icon2 = xrealloc(g->icon2, bi);
+  if (build_connections(g, gpflag)) return 1;
+  if (resolve_junctions(g)) return 1;
+  g->len = xrealloc(g->len, b); g->cab = xrealloc(g->cab, b);
+  g->sab = xrealloc(g->sab, b); g->salp = xrealloc(g->salp, b);
+  for (int64_t i = 0; i < n; i++) {
+    double xw1 = g->x2[i] - g->x[i], yw1 = g->y2[i] - g->y[i], zw1 = g->z2[i] - g->z[i];
+    g->x[i] = (g->x[i] + g->x2[i]) * .5;
+    g->y[i] = (g->y[i] + g->y2[i]) * .5;
+    g->z[i] = (g->z[i] + g->z2[i]) * .5;
+    double xw2 = xw1 * xw1 + yw1 * yw1 + zw1 * zw1;
+    double yw2 = sqrt(xw2);
+    yw2 = (xw2 / yw2 + yw2) * .5;
+    g->len[i] = yw2;
+    g->cab[i] = xw1 / yw2;
+    g->sab[i] = yw1 / yw2;
+    xw2 = zw1 / yw2;
+    if (xw2 > 1.) xw2 = 1.;
+    if (xw2 < -1.) xw2 = -1.;
+    g->salp[i] = xw2;
+    if ((g->len[i] <= 1.e-20) || (g->rad[i] <= 0.)) { set_err("SEGMENT DATA ERROR"); return 1; }
+  }
+  g->xu = xrealloc(g->xu, b); g->yu = xrealloc(g->yu, b); g->zu = xrealloc(g->zu, b);
+  g->su = xrealloc(g->su, b); g->bu = xrealloc(g->bu, b);
+  g->xs = xrealloc(g->xs, b); g->ys = xrealloc(g->ys, b); g->zs = xrealloc(g->zs, b);
+  g->ss = xrealloc(g->ss, b); g->bs = xrealloc(g->bs, b);
+  for (int64_t i = 0; i < n; i++) {
+    g->xu[i] = g->x[i]; g->yu[i] = g->y[i]; g->zu[i] = g->z[i];
+    g->su[i] = g->len[i]; g->bu[i] = g->rad[i];
+    g->xs[i] = g->x[i]; g->ys[i] = g->y[i]; g->zs[i] = g->z[i];
+    g->ss[i] = g->len[i]; g->bs[i] = g->rad[i];
+  }
+  g->complete = 1;
+  return 0;
+}
+
+void ora_segments(const ora_geom* g, double* out, int32_t* icon1, int32_t* icon2, int32_t* tags) {
+  for (int64_t i = 0; i < g->n; i++) {
+    if (out) {
+      double* o = out + 8 * i;
+      o[0] = g->xu[i]; o[1] = g->yu[i]; o[2] = g->zu[i]; o[3] = g->su[i];
+      o[4] = g->bu[i]; o[5] = g->cab[i]; o[6] = g->sab[i]; o[7] = g->salp[i];
+    }
+    if (icon1) icon1[i] = g->icon1[i];
+    if (icon2) icon2[i] = g->icon2[i];
+    if (tags) tags[i] = g->tag[i];
+  }
+}
+
+/* c_geometry::get_segment_number, src/c_geometry.cpp:76-99. */
+int64_t ora_segment_number(const ora_geom* g, int tag, int m) {
+  if (m <= 0) { set_err("CHECK DATA, PARAMETER SPECIFYING SEGMENT POSITION IN A GROUP OF EQUAL TAGS MUST NOT BE ZERO"); return -1; }
+  if (tag == 0) return m;
+  int64_t count = 0;
+  for (int64_t i = 0; i < g->n; i++)
+    if (g->tag[i] == tag && ++count == m) return i + 1;
+  set_err("NO SEGMENT HAS AN ITAG OF %d", tag);
+  return -1;
+}
+
+/* c_geometry::frequency_scale, src/c_geometry.cpp:3389-3418. */
+void ora_frequency_scale(ora_geom* g, double freq_mhz) {
+  double fr = (1.0e6 * freq_mhz) / ora_speed_of_light();
+  for (int64_t i = 0; i < g->n; i++) {
+    g->xs[i] = g->xu[i] * fr; g->ys[i] = g->yu[i] * fr; g->zs[i] = g->zu[i] * fr;
+    g->ss[i] = g->su[i] * fr; g->bs[i] = g->bu[i] * fr;
+  }
+}
+
+/* ------------------------------------------------------------------ */
+/* Basis functions: sbf, trio, tbf                                      */
+/* ------------------------------------------------------------------ */
+static double omc_of(double d, double sdh, double cdh, int use_cd, double cd) {
+  double omc;
+  if (d <= 0.015) {
+    omc = 4. * d * d;
+    omc = ((1.3888889e-3 * omc - 4.1666666667e-2) * omc + .5) * omc;
+  } else {
+    omc = use_cd ? (1. - cd) : (1. - cdh * cdh + sdh * sdh);
+  }
+  return omc;
+}
+
+static double log_weight(double b) { return 1. / (log(1. / (PI_T * b)) - .577215664); }
+
+/* c_geometry::sbf, src/c_geometry.cpp:3102-3289: coefficient of the basis
+ * function centred on segment i, evaluated on segment is (both 1-based). */
+static int sbf(const ora_geom* g, int64_t i, int64_t is, double* aa, double* bb, double* cc) {
+  const int32_t* icon1 = g->icon1;
+  const int32_t* icon2 = g->icon2;
+  const double* sl = g->ss;
+  const double* sr = g->bs;
+  int64_t jcox, jcoxx, ix, local = 0, njun1 = 0, njun2;
+  int jend, iend, june = 0;
+  double d, sig, pp = 0., sdh, cdh, sd, omc, aj, pm = 0, cd, ap, qp, qm, xxi;
+  *aa = 0.; *bb = 0.; *cc = 0.;
+  ix = i - 1;
+  jcox = icon1[ix];
+  if (jcox > PCHCON) jcox = i;
+  jend = -1; iend = -1; sig = -1.;
+  do {
+    if (jcox != 0) {
+      if (jcox < 0) jcox = -jcox;
+      else { sig = -sig; jend = -jend; }
+      jcoxx = jcox - 1;
+      local++;
+      d = PI_T * sl[jcoxx];
+      sdh = sin(d);
+      cdh = cos(d);
+      sd = 2. * sdh * cdh;
+      omc = omc_of(d, sdh, cdh, 0, 0.);
+      aj = log_weight(sr[jcoxx]);
+      pp -= omc / sd * aj;
+      if (jcox == is) {
+        *aa = aj / sd * sig;
+        *bb = aj / (2. * cdh);
+        *cc = -aj / (2. * sdh) * sig;
+        june = iend;
+      }
+      if (jcox != i) {
+        if (jend != 1) jcox = icon1[jcoxx]; else jcox = icon2[jcoxx];
+        if (llabs(jcox) != i) {
+          if (jcox == 0) { set_err("SBF - SEGMENT CONNECTION ERROR FOR SEGMENT %lld", (long long)i); return 1; }
+          continue;
+        }
+      } else if (jcox == is) {
+        *bb = -*bb;
+      }
+      if (iend == 1) break;
+    }
+    pm = -pp;
+    pp = 0.;
+    njun1 = local;
+    jcox = icon2[ix];
+    if (jcox > PCHCON) jcox = i;
+    jend = 1; iend = 1; sig = -1.;
+  } while (jcox != 0);
+  njun2 = local - njun1;
+  d = PI_T * sl[ix];
+  sdh = sin(d);
+  cdh = cos(d);
+  sd = 2. * sdh * cdh;
+  cd = cdh * cdh - sdh * sdh;
+  omc = omc_of(d, sdh, cdh, 1, cd);
+  ap = log_weight(sr[ix]);
+  aj = ap;
+  if (njun1 == 0) {
+    if (njun2 == 0) {
+      *aa = -1.;
+      qp = PI_T * sr[ix];
+      xxi = qp * qp;
+      xxi = qp * (1. - .5 * xxi) / (1. - xxi);
+      *cc = 1. / (cdh - xxi * sdh);
+      return 0;
+    }
+    qp = PI_T * sr[ix];
+    xxi = qp * qp;
+    xxi = qp * (1. - .5 * xxi) / (1. - xxi);
+    qp = -(omc + xxi * sd) / (sd * (ap + xxi * pp) + cd * (xxi * ap - pp));
+    if (june == 1) {
+      *aa = -*aa * qp;
+      *bb = *bb * qp;
+      *cc = -*cc * qp;
+      if (i != is) return 0;
+    }
+    *aa -= 1.;
+    d = cd - xxi * sd;
+    *bb += (sdh + ap * qp * (cdh - xxi * sdh)) / d;
+    *cc += (cdh + ap * qp * (sdh + xxi * cdh)) / d;
+    return 0;
+  }
+  if (njun2 == 0) {
+    qm = PI_T * sr[ix];
+    xxi = qm * qm;
+    xxi = qm * (1. - .5 * xxi) / (1. - xxi);
+    qm = (omc + xxi * sd) / (sd * (aj - xxi * pm) + cd * (pm + xxi * aj));
+    if (june == -1) {
+      *aa = *aa * qm;
+      *bb = *bb * qm;
+      *cc = *cc * qm;
+      if (i != is) return 0;
+    }
+    *aa -= 1.;
+    d = cd - xxi * sd;
+    *bb += (aj * qm * (cdh - xxi * sdh) - sdh) / d;
+    *cc += (cdh - aj * qm * (sdh + xxi * cdh)) / d;
+    return 0;
+  }
+  qp = sd * (pm * pp + aj * ap) + cd * (pm * ap - pp * aj);
+  qm = (ap * omc - pp * sd) / qp;
+  qp = -(aj * omc + pm * sd) / qp;
+  if (june != 0) {
+    if (june < 0) {
+      *aa = *aa * qm; *bb = *bb * qm; *cc = *cc * qm;
+    } else {
+      *aa = -*aa * qp; *bb = *bb * qp; *cc = -*cc * qp;
+    }
+    if (i != is) return 0;
+  }
+  *aa -= 1.;
+  *bb += (aj * qm + ap * qp) * sdh / sd;
+  *cc += (aj * qm - ap * qp) * cdh / sd;
+  return 0;
+}
+
+/* c_geometry::trio, src/c_geometry.cpp:2982-3087. */
+int ora_trio(const ora_geom* g, int j, int32_t* jco, double* ax, double* bx, double* cx, int cap) {
+  int jsno = 0, jend = 0, iend = 0;
+  int64_t jx = j - 1;
+  int64_t jcox = g->icon1[jx], jcoxx;
+  if (jcox <= PCHCON) { jend = -1; iend = -1; }
+  if ((jcox == 0) || (jcox > PCHCON)) {
+    jcox = g->icon2[jx];
+    if (jcox <= PCHCON) { jend = 1; iend = 1; }
+    if (jcox == 0 || (jcox > PCHCON)) {
+      if (cap < 1) return -1;
+      if (sbf(g, j, j, &ax[0], &bx[0], &cx[0])) return -1;
+      jco[0] = j;
+      return 1;
+    }
+  }
+  do {
+    if (jcox < 0) jcox = -jcox; else jend = -jend;
+    jcoxx = jcox - 1;
+    if (jcox != j) {
+      if (jsno >= cap) { set_err("trio capacity"); return -1; }
+      if (sbf(g, jcox, j, &ax[jsno], &bx[jsno], &cx[jsno])) return -1;
+      jco[jsno] = (int32_t)jcox;
+      jsno++;
+      if (jend != 1) jcox = g->icon1[jcoxx]; else jcox = g->icon2[jcoxx];
+      if (jcox == 0) { set_err("TRIO - SEGMENT CONNENTION ERROR FOR SEGMENT %d", j); return -1; }
+      continue;
+    }
+    if (iend == 1) break;
+    jcox = g->icon2[jx];
+    if (jcox > PCHCON) break;
+    jend = 1; iend = 1;
+  } while (jcox != 0);
+  if (jsno >= cap) { set_err("trio capacity"); return -1; }
+  if (sbf(g, j, j, &ax[jsno], &bx[jsno], &cx[jsno])) return -1;
+  jco[jsno] = j;
+  return jsno + 1;
+}
+
+/* c_geometry::tbf, src/c_geometry.cpp:2763-2977: all coefficients of the
+ * basis function centred on segment i (1-based). */
+int ora_tbf(const ora_geom* g, int i, int icap, int32_t* jco, double* ax, double* bx, double* cx, int cap) {
+  const int32_t* icon1 = g->icon1;
+  const int32_t* icon2 = g->icon2;
+  const double* sl = g->ss;
+  const double* sr = g->bs;
+  int64_t jcoxx, njun1 = 0, njun2, jsnop, jsnox;
+  double sdh, cdh, sd, omc, aj, pm = 0, cd, ap, qp, qm, xxi, d;
+  int64_t jsno = 0;
+  double pp = 0.0;
+  int64_t ix = i - 1;
+  int64_t jcox = icon1[ix];
+  if (jcox > PCHCON) jcox = i;
+  int jend = -1, iend = -1;
+  double sig = -1.0;
+  do {
+    if (jcox != 0) {
+      if (jcox < 0) jcox = -jcox;
+      else { sig = -sig; jend = -jend; }
+      jcoxx = jcox - 1;
+      jsno++;
+      jsnox = jsno - 1;
+      if (jsnox >= cap) { set_err("tbf capacity"); return -1; }
+      jco[jsnox] = (int32_t)jcox;
+      d = PI_T * sl[jcoxx];
+      sdh = sin(d);
+      cdh = cos(d);
+      sd = 2.0 * sdh * cdh;
+      omc = omc_of(d, sdh, cdh, 0, 0.);
+      aj = log_weight(sr[jcoxx]);
+      pp = pp - omc / sd * aj;
+      ax[jsnox] = aj / sd * sig;
+      bx[jsnox] = aj / (2. * cdh);
+      cx[jsnox] = -aj / (2. * sdh) * sig;
+      if (jcox != i) {
+        if (jend == 1) jcox = icon2[jcoxx]; else jcox = icon1[jcoxx];
+        if (llabs(jcox) != i) {
+          if (jcox != 0) continue;
+          set_err("TBF - SEGMENT CONNECTION ERROR FOR SEGMENT %d", i);
+          return -1;
+        }
+      } else {
+        bx[jsnox] = -bx[jsnox];
+      }
+      if (iend == 1) break;
+    }
+    pm = -pp;
+    pp = 0.;
+    njun1 = jsno;
+    jcox = icon2[ix];
+    if (jcox > PCHCON) jcox = i;
+    jend = 1; iend = 1; sig = -1.;
+  } while (jcox != 0);
+  njun2 = jsno - njun1;
+  jsnop = jsno;
+  if (jsnop >= cap) { set_err("tbf capacity"); return -1; }
+  jco[jsnop] = i;
+  d = PI_T * sl[ix];
+  sdh = sin(d);
+  cdh = cos(d);
+  sd = 2.0 * sdh * cdh;
+  cd = cdh * cdh - sdh * sdh;
+  omc = omc_of(d, sdh, cdh, 1, cd);
+  ap = log_weight(sr[ix]);
+  aj = ap;
+  if (njun1 == 0) {
+    if (njun2 == 0) {
+      bx[jsnop] = 0.;
+      if (icap == 0) xxi = 0.;
+      else { qp = PI_T * sr[ix]; xxi = qp * qp; xxi = qp * (1. - .5 * xxi) / (1. - xxi); }
+      cx[jsnop] = 1. / (cdh - xxi * sdh);
+      ax[jsnop] = -1.;
+      return (int)(jsnop + 1);
+    }
+    if (icap == 0) xxi = 0.;
+    else { qp = PI_T * sr[ix]; xxi = qp * qp; xxi = qp * (1. - .5 * xxi) / (1. - xxi); }
+    qp = -(omc + xxi * sd) / (sd * (ap + xxi * pp) + cd * (xxi * ap - pp));
+    d = cd - xxi * sd;
+    bx[jsnop] = (sdh + ap * qp * (cdh - xxi * sdh)) / d;
+    cx[jsnop] = (cdh + ap * qp * (sdh + xxi * cdh)) / d;
+    for (int64_t e = 0; e < njun2; e++) {
+      ax[e] = -ax[e] * qp;
+      bx[e] = bx[e] * qp;
+      cx[e] = -cx[e] * qp;
+    }
+    ax[jsnop] = -1.;
+    return (int)(jsnop + 1);
+  }
+  if (njun2 == 0) {
+    if (icap == 0) xxi = 0.;
+    else { qm = PI_T * sr[ix]; xxi = qm * qm; xxi = qm * (1. - .5 * xxi) / (1. - xxi); }
+    qm = (omc + xxi * sd) / (sd * (aj - xxi * pm) + cd * (pm + xxi * aj));
+    d = cd - xxi * sd;
+    bx[jsnop] = (aj * qm * (cdh - xxi * sdh) - sdh) / d;
+    cx[jsnop] = (cdh - aj * qm * (sdh + xxi * cdh)) / d;
+    for (int64_t e = 0; e < njun1; e++) {
+      ax[e] = ax[e] * qm;
+      bx[e] = bx[e] * qm;
+      cx[e] = cx[e] * qm;
+    }
+    ax[jsnop] = -1.;
+    return (int)(jsnop + 1);
+  }
+  qp = sd * (pm * pp + aj * ap) + cd * (pm * ap - pp * aj);
+  qm = (ap * omc - pp * sd) / qp;
+  qp = -(aj * omc + pm * sd) / qp;
+  bx[jsnop] = (aj * qm + ap * qp) * sdh / sd;
+  cx[jsnop] = (aj * qm - ap * qp) * cdh / sd;
+  for (int64_t e = 0; e < njun1; e++) {
+    ax[e] = ax[e] * qm;
+    bx[e] = bx[e] * qm;
+    cx[e] = cx[e] * qm;
+  }
+  for (int64_t e = njun1; e < jsno; e++) {
+    ax[e] = -ax[e] * qp;
+    bx[e] = bx[e] * qp;
+    cx[e] = -cx[e] * qp;
+  }
+  ax[jsnop] = -1.;
+  return (int)(jsnop + 1);
+}
+
+/* ------------------------------------------------------------------ */
+/* Fill kernels                                                        */
+/* ------------------------------------------------------------------ */
+typedef struct {
+  /* source segment (nec_context members m_s, m_b, xj..salpj, ind1, ind2) */
+  double s, b, xj, yj, zj, cabj, sabj, salpj;
+  int ind1, ind2;
+  double rkh;
+  int use_ek;
+  /* gf state (ija, zpk, rkb2) */
+  int ija;
+  double zpk, rkb2;
+  ora_fill_counts* cnt;
+} fill_state;
+
+/* gf, src/nec_context.cpp:3553-3576. */
+static void gf(fill_state* st, double zk, double* co, double* si) {
+  static const double c0 = -1.38888889e-3, c1 = 4.16666667e-2;
+  st->cnt->gf_calls++;
+  double zdk = zk - st->zpk;
+  double rk = sqrt(st->rkb2 + zdk * zdk);
+  *si = sin(rk) / rk;
+  if (st->ija != 0) { *co = cos(rk) / rk; return; }
+  if (rk >= .2) { *co = (cos(rk) - 1.0) / rk; return; }
+  double rks = rk * rk;
+  *co = ((c0 * rks + c1) * rks - 0.5) * rk;
+}
+
+/* test, src/c_evlcom.cpp:845-868. */
+static void conv_test(double f1r, double f2r, double* tr, double f1i, double f2i, double* ti, double dmin) {
+  double den = fabs(f2r), t = fabs(f2i);
+  if (den < t) den = t;
+  if (den < dmin) den = dmin;
+  if (den < 1.0e-37) { *tr = 0.0; *ti = 0.0; return; }
+  *tr = fabs((f1r - f2r) / den);
+  *ti = fabs((f1i - f2i) / den);
+}
+
+/* intx, src/nec_context.cpp:4135-4297: adaptive Romberg integration of the
+ * gf integrand over the source segment. */
+static void intx(fill_state* st, double el1, double el2, double b, int ij, double* sgr, double* sgi) {
+  int ns, nt, nx = 1, nma = 65536, nts = 4, flag = 1;
+  double z, s, ze, fnm, ep, zend, fns, dz = 0., zp, dzot = 0.;
+  double t00r, g1r, g5r, t00i, g1i, g5i, t01r, g3r, t01i, g3i, t10r, t10i, te1i, te1r, t02r;
+  double g2r, g4r, t02i, g2i, g4i, t11r, t11i, t20r, t20i, te2i, te2r;
+  const double rx = 1.0e-4;
+  st->cnt->intx_calls++;
+  z = el1;
+  ze = el2;
+  if (ij == 0) ze = 0.;
+  s = ze - z;
+  fnm = nma;
+  ep = s / (10. * fnm);
+  zend = ze - ep;
+  *sgr = 0.;
+  *sgi = 0.;
+  ns = nx;
+  nt = 0;
+  gf(st, z, &g1r, &g1i);
+  for (;;) {
+    if (flag) {
+      fns = ns;
+      dz = s / fns;
+      zp = z + dz;
+      if (zp > ze) {
+        dz = ze - z;
+        if (fabs(dz) <= ep) goto done;
+      }
+      dzot = dz * .5;
+      zp = z + dzot;
+      gf(st, zp, &g3r, &g3i);
+      zp = z + dz;
+      gf(st, zp, &g5r, &g5i);
+    }
+    t00r = (g1r + g5r) * dzot;
+    t00i = (g1i + g5i) * dzot;
+    t01r = (t00r + dz * g3r) * 0.5;
+    t01i = (t00i + dz * g3i) * 0.5;
+    t10r = (4.0 * t01r - t00r) / 3.0;
+    t10i = (4.0 * t01i - t00i) / 3.0;
+    conv_test(t01r, t10r, &te1r, t01i, t10i, &te1i, 0.);
+    if ((te1i <= rx) && (te1r <= rx)) {
+      *sgr = *sgr + t10r;
+      *sgi = *sgi + t10i;
+      nt += 2;
+      z += dz;
+      if (z >= zend) goto done;
+      g1r = g5r;
+      g1i = g5i;
+      if (nt >= nts && ns > nx) { ns = ns / 2; nt = 1; }
+      flag = 1;
+      continue;
+    }
+    zp = z + dz * 0.25;
+    gf(st, zp, &g2r, &g2i);
+    zp = z + dz * 0.75;
+    gf(st, zp, &g4r, &g4i);
+    t02r = (t01r + dzot * (g2r + g4r)) * 0.5;
+    t02i = (t01i + dzot * (g2i + g4i)) * 0.5;
+    t11r = (4.0 * t02r - t01r) / 3.0;
+    t11i = (4.0 * t02i - t01i) / 3.0;
+    t20r = (16.0 * t11r - t10r) / 15.0;
+    t20i = (16.0 * t11i - t10i) / 15.0;
+    conv_test(t11r, t20r, &te2r, t11i, t20i, &te2i, 0.);
+    if ((te2i > rx) || (te2r > rx)) {
+      nt = 0;
+      if (ns < nma) {
+        ns = ns * 2;
+        fns = ns;
+        dz = s / fns;
+        dzot = dz * 0.5;
+        g5r = g3r; g5i = g3i;
+        g3r = g2r; g3i = g2i;
+        flag = 0;
+        continue;
+      }
+      /* step size limited: accept the 5-point result */
+    }
+    *sgr = *sgr + t20r;
+    *sgi = *sgi + t20i;
+    nt++;
+    z += dz;
+    if (z >= zend) goto done;
+    g1r = g5r;
+    g1i = g5i;
+    if (nt >= nts && ns > nx) { ns = ns / 2; nt = 1; }
+    flag = 1;
+  }
+done:
+  if (ij == 0) {
+    *sgr = 2. * (*sgr + log((sqrt(b * b + s * s) + s) / b));
+    *sgi = 2. * *sgi;
+  }
+}
+
+/* gx, src/nec_context.cpp:3605-3616. */
+static void gx(fill_state* st, double zz, double rh, double xk, cx* gz, cx* gzp) {
+  st->cnt->gx_calls++;
+  double r2 = zz * zz + rh * rh;
+  double r = sqrt(r2);
+  double rkz = xk * r;
+  cx temp = mk(cos(rkz), -sin(rkz));
+  temp = cdivr(temp, r);
+  *gz = temp;
+  *gzp = cdivr(cmul(cneg(mk(1.0, rkz)), temp), r2);
+}
+
+/* gxx, src/nec_context.cpp:3623-3673. */
+static void gxx(fill_state* st, double zz, double rh, double a, double a2, double xk, int ira,
+                cx* g1, cx* g1p, cx* g2, cx* g2p, cx* g3, cx* gzp) {
+  st->cnt->gxx_calls++;
+  double r2 = zz * zz + rh * rh;
+  double r = sqrt(r2);
+  double r4 = r2 * r2;
+  double rk = xk * r;
+  double rk2 = rk * rk;
+  double rh2 = rh * rh;
+  double t1 = .25 * a2 * rh2 / r4;
+  double t2 = .5 * a2 / r2;
+  cx c1 = mk(1.0, rk);
+  cx c2 = cmulr(c1, 3.);
+  c2.re = c2.re - rk2;
+  cx c3 = csub(cmulr(mk(6.0, rk), rk2), cmulr(c1, 15.));
+  cx gz = cdivr(mk(cos(rk), -sin(rk)), r);
+  cx tc = cmulr(c2, t1);
+  *g2 = cmul(gz, mk(1. + tc.re, tc.im));
+  *g1 = csub(*g2, cmul(cmulr(c1, t2), gz));
+  gz = cdivr(gz, r2);
+  *g2p = cmul(gz, csub(cmulr(c3, t1), c1));
+  *gzp = cmul(cmulr(c2, t2), gz);
+  *g3 = cadd(*g2p, *gzp);
+  *g1p = cmulr(*g3, zz);
+  if (ira != 1) {
+    *g3 = cmulr(cadd(*g3, *gzp), rh);
+    *gzp = cmul(cmulr(c1, -zz), gz);
+    if (rh <= 1.0e-10) { *g2 = mk(0., 0.); *g2p = mk(0., 0.); return; }
+    *g2 = cdivr(*g2, rh);
+    *g2p = cdivr(cmulr(*g2p, zz), rh);
+    return;
+  }
+  t2 = .5 * a;
+  *g2 = cmul(cmulr(c1, -t2), gz);
+  *g2p = cdivr(cmul(cmulr(gz, t2), c2), r2);
+  *g3 = csub(cmulr(*g2p, rh2), cmul(cmulr(gz, a), c1));
+  *g2p = cmulr(*g2p, zz);
+  *gzp = cmul(cmulr(c1, -zz), gz);
+}
+
+static const cx CONST1 = {0.0, 4.771341189};
+
+/* eksc, src/nec_context.cpp:3051-3100: thin-wire fields of the sine, cosine
+ * and constant filaments. */
+static void eksc(fill_state* st, double s, double z, double rh, double xk, int ij,
+                 cx* ezs, cx* ers, cx* ezc, cx* erc, cx* ezk, cx* erk) {
+  st->ija = ij;
+  st->zpk = xk * z;
+  double rhk = xk * rh;
+  st->rkb2 = rhk * rhk;
+  double sh = .5 * s;
+  double shk = xk * sh;
+  double ss = sin(shk);
+  double cs = cos(shk);
+  double z2a = sh - z;
+  double z1a = -(sh + z);
+  cx gz1, gz2, gp1, gp2;
+  gx(st, z1a, rh, xk, &gz1, &gp1);
+  gx(st, z2a, rh, xk, &gz2, &gp2);
+  cx gzp1 = cmulr(gp1, z1a);
+  cx gzp2 = cmulr(gp2, z2a);
+  cx mc1 = cneg(CONST1);
+  *ezs = cmul(CONST1, csub(cmulr(cmulr(csub(gz2, gz1), cs), xk), cmulr(cadd(gzp2, gzp1), ss)));
+  *ezc = cmul(mc1, cadd(cmulr(cmulr(cadd(gz2, gz1), ss), xk), cmulr(csub(gzp2, gzp1), cs)));
+  *erk = cmulr(cmul(CONST1, csub(gp2, gp1)), rh);
+  double cint, sint;
+  intx(st, -shk, shk, rhk, ij, &cint, &sint);
+  *ezk = cmul(mc1, cadd(csub(gzp2, gzp1), cmulr(mk(cint, -sint), xk * xk)));
+  if (rh >= 1.0e-10) {
+    gzp1 = cmulr(gzp1, z1a);
+    gzp2 = cmulr(gzp2, z2a);
+    cx t1 = cmulr(cadd(cadd(cadd(gzp2, gzp1), gz2), gz1), ss);
+    cx t2 = cmulr(cmulr(csub(cmulr(gz2, z2a), cmulr(gz1, z1a)), cs), xk);
+    *ers = cdivr(cmul(mc1, csub(t1, t2)), rh);
+    cx u1 = cmulr(csub(cadd(csub(gzp2, gzp1), gz2), gz1), cs);
+    cx u2 = cmulr(cmulr(cadd(cmulr(gz2, z2a), cmulr(gz1, z1a)), ss), xk);
+    *erc = cdivr(cmul(mc1, cadd(u1, u2)), rh);
+    return;
+  }
+  *ers = mk(0., 0.);
+  *erc = mk(0., 0.);
+}
+
+/* ekscx, src/nec_context.cpp:3108-3188: extended thin-wire kernel. */
+static void ekscx(fill_state* st, double bx, double s, double z, double rhx, double xk, int ij, int inx1, int inx2,
+                  cx* ezs, cx* ers, cx* ezc, cx* erc, cx* ezk, cx* erk) {
+  int ira;
+  double b, rh;
+  if (rhx >= bx) { rh = rhx; b = bx; ira = 0; }
+  else { rh = bx; b = rhx; ira = 1; }
+  double sh = .5 * s;
+  st->ija = ij;
+  st->zpk = xk * z;
+  double rhk = xk * rh;
+  st->rkb2 = rhk * rhk;
+  double shk = xk * sh;
+  double ss = sin(shk);
+  double cs = cos(shk);
+  double z2a = sh - z;
+  double z1a = -(sh + z);
+  double a2 = b * b;
+  cx gz1, gz2, gzp1, gzp2, gr1, gr2, grp1, grp2, grk1, grk2, gzz1, gzz2;
+  if (inx1 != 2) gxx(st, z1a, rh, b, a2, xk, ira, &gz1, &gzp1, &gr1, &grp1, &grk1, &gzz1);
+  else {
+    gx(st, z1a, rhx, xk, &gz1, &grk1);
+    gzp1 = cmulr(grk1, z1a);
+    gr1 = cdivr(gz1, rhx);
+    grp1 = cdivr(gzp1, rhx);
+    grk1 = cmulr(grk1, rhx);
+    gzz1 = mk(0., 0.);
+  }
+  if (inx2 != 2) gxx(st, z2a, rh, b, a2, xk, ira, &gz2, &gzp2, &gr2, &grp2, &grk2, &gzz2);
+  else {
+    gx(st, z2a, rhx, xk, &gz2, &grk2);
+    gzp2 = cmulr(grk2, z2a);
+    gr2 = cdivr(gz2, rhx);
+    grp2 = cdivr(gzp2, rhx);
+    grk2 = cmulr(grk2, rhx);
+    gzz2 = mk(0., 0.);
+  }
+  cx mc1 = cneg(CONST1);
+  *ezs = cmul(CONST1, csub(cmulr(cmulr(csub(gz2, gz1), cs), xk), cmulr(cadd(gzp2, gzp1), ss)));
+  *ezc = cmul(mc1, cadd(cmulr(cmulr(cadd(gz2, gz1), ss), xk), cmulr(csub(gzp2, gzp1), cs)));
+  {
+    cx t1 = cmulr(cadd(cadd(cadd(cmulr(grp2, z2a), cmulr(grp1, z1a)), gr2), gr1), ss);
+    cx t2 = cmulr(cmulr(csub(cmulr(gr2, z2a), cmulr(gr1, z1a)), cs), xk);
+    *ers = cmul(mc1, csub(t1, t2));
+    cx u1 = cmulr(csub(cadd(csub(cmulr(grp2, z2a), cmulr(grp1, z1a)), gr2), gr1), cs);
+    cx u2 = cmulr(cmulr(cadd(cmulr(gr2, z2a), cmulr(gr1, z1a)), ss), xk);
+    *erc = cmul(mc1, cadd(u1, u2));
+  }
+  *erk = cmul(CONST1, csub(grk2, grk1));
+  double cint, sint;
+  intx(st, -shk, shk, rhk, ij, &cint, &sint);
+  double bk = b * xk;
+  double bk2 = bk * bk * .25;
+  cx term = cmulr(mk(cint, -sint), (xk * xk) * (1. - bk2));
+  *ezk = cmul(mc1, csub(cadd(csub(gzp2, gzp1), term), cmulr(csub(gzz2, gzz1), bk2)));
+}
+
+/* efld_compute, src/nec_context.cpp:2777-2837 (direct field only). */
+static double efld_compute(fill_state* st, double xij, double yij, double ai, double salpr, double zij, int ijx, cx egnd[9]) {
+  cx tezs = mk(0, 0), ters = mk(0, 0), tezc, terc, tezk, terk;
+  double cabj = st->cabj, sabj = st->sabj;
+  double zp = xij * cabj + yij * sabj + zij * salpr;
+  double rhox = xij - cabj * zp;
+  double rhoy = yij - sabj * zp;
+  double rhoz = zij - salpr * zp;
+  double rh = sqrt(rhox * rhox + rhoy * rhoy + rhoz * rhoz + ai * ai);
+  if (rh <= 1.e-10) { rhox = 0.0; rhoy = 0.0; rhoz = 0.0; }
+  else { rhox /= rh; rhoy /= rh; rhoz /= rh; }
+  double r = sqrt(zp * zp + rh * rh);
+  if (r >= st->rkh) {
+    st->cnt->far_pairs++;
+    double eta = ora_impedance();
+    double rmag = TWO_PI * r;
+    double cth = zp / r;
+    double px = rh / r;
+    cx e0 = mk(cos(rmag), -sin(rmag));
+    double py = TWO_PI * r * r;
+    cx e1 = cdivr(cmul(cmulr(e0, eta * cth), mk(1.0, -1.0 / rmag)), py);
+    cx e2 = cdivr(cmul(cmulr(e0, eta * px), mk(1.0, rmag - 1.0 / rmag)), 2. * py);
+    tezk = csub(cmulr(e1, cth), cmulr(e2, px));
+    terk = cadd(cmulr(e1, px), cmulr(e2, cth));
+    rmag = sin(PI_T * st->s) / PI_T;
+    tezc = cmulr(tezk, rmag);
+    terc = cmulr(terk, rmag);
+    tezk = cmulr(tezk, st->s);
+    terk = cmulr(terk, st->s);
+    egnd[3] = mk(0, 0); egnd[4] = mk(0, 0); egnd[5] = mk(0, 0);
+  } else {
+    st->cnt->near_pairs++;
+    if (!st->use_ek)
+      eksc(st, st->s, zp, rh, TWO_PI, ijx, &tezs, &ters, &tezc, &terc, &tezk, &terk);
+    else
+      ekscx(st, st->b, st->s, zp, rh, TWO_PI, ijx, st->ind1, st->ind2, &tezs, &ters, &tezc, &terc, &tezk, &terk);
+    egnd[3] = cadd(cmulr(tezs, cabj), cmulr(ters, rhox));
+    egnd[4] = cadd(cmulr(tezs, sabj), cmulr(ters, rhoy));
+    egnd[5] = cadd(cmulr(tezs, salpr), cmulr(ters, rhoz));
+  }
+  egnd[0] = cadd(cmulr(tezk, cabj), cmulr(terk, rhox));
+  egnd[1] = cadd(cmulr(tezk, sabj), cmulr(terk, rhoy));
+  egnd[2] = cadd(cmulr(tezk, salpr), cmulr(terk, rhoz));
+  egnd[6] = cadd(cmulr(tezc, cabj), cmulr(terc, rhox));
+  egnd[7] = cadd(cmulr(tezc, sabj), cmulr(terc, rhoy));
+  egnd[8] = cadd(cmulr(tezc, salpr), cmulr(terc, rhoz));
+  return r;
+}
+
+/* test_ek_approximation, src/c_geometry.cpp:3449-3458. */
+static int test_ek(const ora_geom* g, int64_t s1, int64_t s2) {
+  double ratio = g->bs[s2] / g->bs[s1];
+  double xi = fabs(g->cab[s1] * g->cab[s2] + g->sab[s1] * g->sab[s2] + g->salp[s1] * g->salp[s2]);
+  return ((xi < 0.999999) || (fabs(ratio - 1.0) > 1.e-6)) ? 2 : 0;
+}
+
+/* End-condition indices of cmww for the extended kernel,
+ * src/nec_context.cpp:2535-2601 (j 0-based). */
+static void ek_ends(const ora_geom* g, int64_t j, int* ind1, int* ind2) {
+  int64_t jx = j + 1;
+  double cabj = g->cab[j], sabj = g->sab[j];
+  int64_t ipr = g->icon1[j];
+  int64_t iprx = llabs(ipr) - 1;
+  *ind1 = 2;
+  if (ipr > PCHCON) *ind1 = 0;
+  else if (ipr == 0) *ind1 = 1;
+  else if ((ipr == jx) && (cabj * cabj + sabj * sabj <= 1.e-8)) *ind1 = 0;
+  else if ((ipr > 0) && (g->icon2[iprx] == jx)) *ind1 = test_ek(g, j, ipr - 1);
+  else if ((ipr < 0) && (g->icon1[iprx] == -jx)) *ind1 = test_ek(g, j, -ipr - 1);
+  ipr = g->icon2[j];
+  if (ipr > PCHCON) *ind2 = 2;
+  else if (ipr == 0) *ind2 = 1;
+  else if (ipr < 0) {
+    iprx = -ipr - 1;
+    if (-g->icon2[iprx] != jx) *ind2 = 2; else *ind2 = test_ek(g, j, iprx);
+  } else {
+    if (ipr != jx) {
+      iprx = ipr - 1;
+      if (g->icon1[iprx] != jx) *ind2 = 2; else *ind2 = test_ek(g, j, iprx);
+    } else if ((cabj * cabj + sabj * sabj) > 1.e-8) *ind2 = 2;
+    else *ind2 = 0;
+  }
+}
+
+#define TRIO_CAP 64
+
+/* cmset + cmww, src/nec_context.cpp:2077-2203 and :2514-2654, transposed
+ * fill (itrp = 1): Z[i][k] += etk*ax + ets*bx + etc*cx for each basis k in
+ * trio(j), sources j in ascending order. */
+int ora_fill(const ora_geom* g, double rkh, int use_ek, double* Z, int nthreads, ora_fill_counts* counts) {
+  int64_t n = g->n;
+  memset(Z, 0, (size_t)n * (size_t)n * 2 * sizeof(double));
+  /* trio tables of every source segment */
+  int32_t* tj = (int32_t*)malloc((size_t)n * TRIO_CAP * sizeof(int32_t));
+  double* ta = (double*)malloc((size_t)n * TRIO_CAP * sizeof(double));
+  double* tb = (double*)malloc((size_t)n * TRIO_CAP * sizeof(double));
+  double* tc = (double*)malloc((size_t)n * TRIO_CAP * sizeof(double));
+  int* tn = (int*)malloc((size_t)n * sizeof(int));
+  int bad = 0;
+  for (int64_t j = 0; j < n; j++) {
+    tn[j] = ora_trio(g, (int)(j + 1), tj + j * TRIO_CAP, ta + j * TRIO_CAP, tb + j * TRIO_CAP, tc + j * TRIO_CAP, TRIO_CAP);
+    if (tn[j] < 0) { bad = 1; break; }
+  }
+  ora_fill_counts total;
+  memset(&total, 0, sizeof total);
+  if (!bad) {
+    if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+    {
+      ora_fill_counts local;
+      memset(&local, 0, sizeof local);
+      fill_state st;
+      memset(&st, 0, sizeof st);
+      st.cnt = &local;
+      st.rkh = rkh;
+      st.use_ek = use_ek;
+#pragma omp for schedule(dynamic, 4)
+      for (int64_t i = 0; i < n; i++) {
+        double xi = g->xs[i], yi = g->ys[i], zi = g->zs[i], ai = g->bs[i];
+        double cabi = g->cab[i], sabi = g->sab[i], salpi = g->salp[i];
+        double* row = Z + 2 * (size_t)i * (size_t)n;
+        for (int64_t j = 0; j < n; j++) {
+          st.s = g->ss[j]; st.b = g->bs[j];
+          st.xj = g->xs[j]; st.yj = g->ys[j]; st.zj = g->zs[j];
+          st.cabj = g->cab[j]; st.sabj = g->sab[j]; st.salpj = g->salp[j];
+          if (use_ek) ek_ends(g, j, &st.ind1, &st.ind2);
+          cx e[9];
+          local.pairs++;
+          efld_compute(&st, xi - st.xj, yi - st.yj, ai, st.salpj, zi - st.zj, i != j, e);
+          cx etk = cadd(cadd(cmulr(e[0], cabi), cmulr(e[1], sabi)), cmulr(e[2], salpi));
+          cx ets = cadd(cadd(cmulr(e[3], cabi), cmulr(e[4], sabi)), cmulr(e[5], salpi));
+          cx etc = cadd(cadd(cmulr(e[6], cabi), cmulr(e[7], sabi)), cmulr(e[8], salpi));
+          const int32_t* jj = tj + j * TRIO_CAP;
+          for (int q = 0; q < tn[j]; q++) {
+            int64_t k = jj[q] - 1;
+            cx t = cadd(cadd(cmulr(etk, ta[j * TRIO_CAP + q]), cmulr(ets, tb[j * TRIO_CAP + q])),
+                        cmulr(etc, tc[j * TRIO_CAP + q]));
+            row[2 * k] += t.re;
+            row[2 * k + 1] += t.im;
+            local.scatter_terms++;
+          }
+        }
+      }
+#pragma omp critical
+      {
+        total.pairs += local.pairs; total.far_pairs += local.far_pairs; total.near_pairs += local.near_pairs;
+        total.gx_calls += local.gx_calls; total.gxx_calls += local.gxx_calls; total.intx_calls += local.intx_calls;
+        total.gf_calls += local.gf_calls; total.scatter_terms += local.scatter_terms;
+      }
+    }
+  }
+  if (counts) *counts = total;
+  free(tj); free(ta); free(tb); free(tc); free(tn);
+  return bad;
+}
+
+/* ------------------------------------------------------------------ */
+/* LU and solve                                                        */
+/* ------------------------------------------------------------------ */
+#define LU_NB 64
+
+/* Right-looking partial-pivot LU on row-major complex Z (row pivoting; the
+ * pivot is the first maximum of |z| = hypot(re, im) as in Eigen's
+ * PartialPivLU, src/eigen/Eigen/src/LU/PartialPivLU.h:338-385).  Blocked for
+ * speed, but each element still receives its rank-1 updates in pivot order,
+ * so the result equals the unblocked algorithm bit for bit. */
+int64_t ora_lu(int64_t n, double* Z, int32_t* piv, int nthreads) {
+  int64_t first_zero = -1;
+  if (nthreads < 1) nthreads = 1;
+  cx* A = (cx*)Z;
+  for (int64_t k0 = 0; k0 < n; k0 += LU_NB) {
+    int64_t nb = (n - k0 < LU_NB) ? (n - k0) : LU_NB;
+    int64_t kend = k0 + nb;
+    /* panel: columns k0..kend-1 over rows k0..n-1 */
+    for (int64_t k = k0; k < kend; k++) {
+      int64_t p = k;
+      double best = hypot(A[k * n + k].re, A[k * n + k].im);
+      for (int64_t i = k + 1; i < n; i++) {
+        double v = hypot(A[i * n + k].re, A[i * n + k].im);
+        if (v > best) { best = v; p = i; }
+      }
+      piv[k] = (int32_t)p;
+      if (best != 0.0) {
+        if (p != k) {
+          cx* rk = A + k * n;
+          cx* rp = A + p * n;
+          for (int64_t c = 0; c < n; c++) { cx t = rk[c]; rk[c] = rp[c]; rp[c] = t; }
+        }
+        cx d = A[k * n + k];
+        for (int64_t i = k + 1; i < n; i++) A[i * n + k] = cdiv(A[i * n + k], d);
+      } else if (first_zero < 0) {
+        first_zero = k;
+      }
+      /* rank-1 update restricted to the panel columns */
+#pragma omp parallel for num_threads(nthreads) schedule(static) if (n - k > 256)
+      for (int64_t i = k + 1; i < n; i++) {
+        cx l = A[i * n + k];
+        cx* ri = A + i * n;
+        const cx* rk = A + k * n;
+        for (int64_t c = k + 1; c < kend; c++) ri[c] = csub(ri[c], cmul(l, rk[c]));
+      }
+    }
+    if (kend >= n) break;
+    /* U12 = L11^-1 A12 (rows k0..kend-1, columns kend..n-1) */
+    for (int64_t r = k0 + 1; r < kend; r++) {
+      cx* rr = A + r * n;
+      for (int64_t q = k0; q < r; q++) {
+        cx l = rr[q];
+        const cx* rq = A + q * n;
+        for (int64_t c = kend; c < n; c++) rr[c] = csub(rr[c], cmul(l, rq[c]));
+      }
+    }
+    /* A22 -= L21 U12 */
+#pragma omp parallel for num_threads(nthreads) schedule(static)
+    for (int64_t i = kend; i < n; i++) {
+      cx* ri = A + i * n;
+      for (int64_t q = k0; q < kend; q++) {
+        cx l = ri[q];
+        const cx* rq = A + q * n;
+        for (int64_t c = kend; c < n; c++) ri[c] = csub(ri[c], cmul(l, rq[c]));
+      }
+    }
+  }
+  return first_zero;
+}
+
+/* matrix_algebra.cpp solve (:271-307): permute, unit-lower forward
+ * (column-oriented), upper backward (row-oriented dot, then divide). */
+void ora_lu_solve(int64_t n, const double* LU, const int32_t* piv, double* b) {
+  const cx* A = (const cx*)LU;
+  cx* x = (cx*)b;
+  for (int64_t k = 0; k < n; k++) {
+    int64_t p = piv[k];
+    if (p != k) { cx t = x[k]; x[k] = x[p]; x[p] = t; }
+  }
+  for (int64_t i = 0; i < n; i++)
+    for (int64_t j = i + 1; j < n; j++) x[j] = csub(x[j], cmul(A[j * n + i], x[i]));
+  for (int64_t k = 0; k < n; k++) {
+    int64_t i = n - k - 1;
+    cx sum = mk(0.0, 0.0);
+    for (int64_t j = i + 1; j < n; j++) sum = cadd(sum, cmul(A[i * n + j], x[j]));
+    x[i] = cdiv(csub(x[i], sum), A[i * n + i]);
+  }
+}
+
+/* c_geometry::get_current_coefficients, src/c_geometry.cpp:3299-3386. */
+int ora_current_coefficients(const ora_geom* g, double* cur, double* abc) {
+  int64_t n = g->n;
+  double *air = abc, *aii = abc + n, *bir = abc + 2 * n, *bii = abc + 3 * n, *cir = abc + 4 * n, *cii = abc + 5 * n;
+  memset(abc, 0, (size_t)6 * n * sizeof(double));
+  int32_t jco[TRIO_CAP];
+  double ax[TRIO_CAP], bx[TRIO_CAP], cxx[TRIO_CAP];
+  for (int64_t i = 0; i < n; i++) {
+    double ar = cur[2 * i], ai = cur[2 * i + 1];
+    int jsno = ora_tbf(g, (int)(i + 1), 1, jco, ax, bx, cxx, TRIO_CAP);
+    if (jsno < 0) return 1;
+    for (int jx = 0; jx < jsno; jx++) {
+      int64_t j = jco[jx] - 1;
+      air[j] += ax[jx] * ar;
+      aii[j] += ax[jx] * ai;
+      bir[j] += bx[jx] * ar;
+      bii[j] += bx[jx] * ai;
+      cir[j] += cxx[jx] * ar;
+      cii[j] += cxx[jx] * ai;
+    }
+  }
+  for (int64_t i = 0; i < n; i++) {
+    cur[2 * i] = air[i] + cir[i];
+    cur[2 * i + 1] = aii[i] + cii[i];
+  }
+  return 0;
+}
+
+/* ------------------------------------------------------------------ */
+/* Far field                                                           */
+/* ------------------------------------------------------------------ */
+
+/* nec_context::ffld, src/nec_context.cpp:6639-6887, free space, no patches. */
+void ora_ffld(const ora_geom* g, const double* abc, double sth, double cth, double sph, double cph,
+              double* eth, double* eph) {
+  int64_t n = g->n;
+  const double *air = abc, *aii = abc + n, *bir = abc + 2 * n, *bii = abc + 3 * n, *cir = abc + 4 * n, *cii = abc + 5 * n;
+  cx const3 = mk(0.0, -ora_impedance() / FOUR_PI);
+  double phx = -sph;
+  double phy = cph;
+  double roz = cth;
+  double thx = roz * phy;
+  double thy = -roz * phx;
+  double thz = -sth;
+  double rox = -thz * phy;
+  double roy = thz * phx;
+  cx cix = mk(0, 0), ciy = mk(0, 0), ciz = mk(0, 0);
+  for (int64_t i = 0; i < n; i++) {
+    double omega = -(rox * g->cab[i] + roy * g->sab[i] + roz * g->salp[i]);
+    double el = PI_T * g->ss[i];
+    double sill = omega * el;
+    double top = el + sill;
+    double bot = el - sill;
+    double a, too, boo;
+    if (fabs(omega) >= 1.0e-7) a = 2. * sin(sill) / omega;
+    else a = (2. - omega * omega * el * el / 3.) * el;
+    if (fabs(top) >= 1.0e-7) too = sin(top) / top;
+    else too = 1. - top * top / 6.;
+    if (fabs(bot) >= 1.0e-7) boo = sin(bot) / bot;
+    else boo = 1. - bot * bot / 6.;
+    double b = el * (boo - too);
+    double c = el * (boo + too);
+    double rr = a * air[i] + b * bii[i] + c * cir[i];
+    double ri = a * aii[i] - b * bir[i] + c * cii[i];
+    double arg = TWO_PI * (g->xs[i] * rox + g->ys[i] * roy + g->zs[i] * roz);
+    cx exa = cmul(mk(cos(arg), sin(arg)), mk(rr, ri));
+    cix = cadd(cix, cmulr(exa, g->cab[i]));
+    ciy = cadd(ciy, cmulr(exa, g->sab[i]));
+    ciz = cadd(ciz, cmulr(exa, g->salp[i]));
+  }
+  cx et = cmul(cadd(cadd(cmulr(cix, thx), cmulr(ciy, thy)), cmulr(ciz, thz)), const3);
+  cx ep = cmul(cadd(cmulr(cix, phx), cmulr(ciy, phy)), const3);
+  eth[0] = et.re; eth[1] = et.im;
+  eph[0] = ep.re; eph[1] = ep.im;
+}
+
+/* nec_radiation_pattern::analyze (src/nec_radiation_pattern.cpp:209-450),
+ * voltage excitation, ifar = 0, free space. */
+int ora_pattern(const ora_geom* g, const double* abc, double wavelength, double pinr, double pnlr,
+                const ora_rp_spec* rp, ora_rp_out* out, int nthreads) {
+  int nth = rp->n_theta, nph = rp->n_phi;
+  double eta = ora_impedance();
+  double gcop = wavelength * wavelength * 2.0 * PI_T / (eta * pinr);
+  double prad = pinr - 0.0 - pnlr;
+  double gcon = gcop;
+  if (rp->ipd != 0) gcon = gcon * pinr / prad;
+  double exrm = 0., exra = 0.;
+  if (rp->rfld >= 1.0e-20) {
+    exrm = 1. / rp->rfld;
+    exra = rp->rfld / wavelength;
+    exra = -360. * (exra - floor(exra));
+  }
+  /* angle sequences accumulate exactly as the reference's loops do */
+  double* thv = (double*)malloc((size_t)nth * sizeof(double));
+  double* phv = (double*)malloc((size_t)nph * sizeof(double));
+  double phi = rp->phi0 - rp->dphi;
+  for (int k = 0; k < nph; k++) { phi += rp->dphi; phv[k] = phi; }
+  double thet = rp->theta0 - rp->dtheta;
+  for (int k = 0; k < nth; k++) { thet += rp->dtheta; thv[k] = thet; }
+  double delta_phi_rad = deg_to_rad(rp->dphi);
+  double tmp2 = 0.5 * deg_to_rad(rp->dtheta);
+  double pint = 0.0;
+  if (nthreads < 1) nthreads = 1;
+  double* gainbuf = out->gain ? out->gain : (double*)malloc((size_t)nth * nph * sizeof(double));
+  double* totbuf = out->pg_tot ? out->pg_tot : (double*)malloc((size_t)nth * nph * sizeof(double));
+  double* rhbuf = out->pg_rhcp ? out->pg_rhcp : (double*)malloc((size_t)nth * nph * sizeof(double));
+  double* lhbuf = out->pg_lhcp ? out->pg_lhcp : (double*)malloc((size_t)nth * nph * sizeof(double));
+  double* scbuf = out->avg_scale ? out->avg_scale : (double*)malloc((size_t)nth * nph * sizeof(double));
+  double* pintv = (double*)calloc((size_t)nth * nph, sizeof(double));
+#pragma omp parallel for num_threads(nthreads) schedule(dynamic, 16)
+  for (int64_t idx = 0; idx < (int64_t)nth * nph; idx++) {
+    int kph = (int)(idx / nth), kth = (int)(idx % nth);
+    double pha = deg_to_rad(phv[kph]);
+    double th = thv[kth];
+    double tha = deg_to_rad(th);
+    double e_t[2], e_p[2];
+    ora_ffld(g, abc, sin(tha), cos(tha), sin(pha), cos(pha), e_t, e_p);
+    double ethm2 = e_t[0] * e_t[0] + e_t[1] * e_t[1];
+    double ethm = sqrt(ethm2);
+    double etha = rad_to_deg(atan2(e_t[1], e_t[0]));
+    double ephm2 = e_p[0] * e_p[0] + e_p[1] * e_p[1];
+    double ephm = sqrt(ephm2);
+    double epha = rad_to_deg(atan2(e_p[1], e_p[0]));
+    double tilta, emajr2, eminr2, axrat;
+    int sense;
+    if ((ethm2 <= 1.0e-20) && (ephm2 <= 1.0e-20)) {
+      tilta = 0.; emajr2 = 0.; eminr2 = 0.; axrat = 0.; sense = 3;
+    } else {
+      double dfaz = epha - etha, dfaz2;
+      if (epha >= 0.) dfaz2 = dfaz - 360.; else dfaz2 = dfaz + 360.;
+      if (fabs(dfaz) > fabs(dfaz2)) dfaz = dfaz2;
+      double cdfaz = cos(deg_to_rad(dfaz));
+      double tstor1 = ethm2 - ephm2;
+      double tstor2 = 2. * ephm * ethm * cdfaz;
+      tilta = .5 * atan2(tstor2, tstor1);
+      double stilta = sin(tilta);
+      tstor1 = tstor1 * stilta * stilta;
+      tstor2 = tstor2 * stilta * cos(tilta);
+      emajr2 = -tstor1 + tstor2 + ethm2;
+      eminr2 = tstor1 - tstor2 + ephm2;
+      if (eminr2 < 0.) eminr2 = 0.;
+      axrat = sqrt(eminr2 / emajr2);
+      tilta = rad_to_deg(tilta);
+      if (axrat <= 1.0e-5) sense = 0;       /* POL_LINEAR */
+      else if (dfaz <= 0.) sense = 1;       /* POL_RIGHT */
+      else sense = 2;                       /* POL_LEFT */
+    }
+    double gnmj = db10(gcon * emajr2);
+    double gnmn = db10(gcon * eminr2);
+    double gnv = db10(gcon * ethm2);
+    double gnh = db10(gcon * ephm2);
+    double gtot = db10(gcon * (ethm2 + ephm2));
+    double tg;
+    switch (rp->normalization) {
+      case 1: tg = gnmj; break;
+      case 2: tg = gnmn; break;
+      case 3: tg = gnv; break;
+      case 4: tg = gnh; break;
+      default: tg = gtot; break;
+    }
+    gainbuf[idx] = tg;
+    if (rp->iavp != 0) {
+      double ts = gcop * (ethm2 + ephm2);
+      double tmp3 = tha - tmp2, tmp4 = tha + tmp2;
+      if (kth == 0) tmp3 = tha;
+      else if (kth == nth - 1) tmp4 = tha;
+      double da = fabs(delta_phi_rad * (cos(tmp3) - cos(tmp4)));
+      if ((kph == 0) || (kph == nph - 1)) da *= .5;
+      pintv[idx] = ts * da;
+      if (rp->iavp == 2) continue;
+    }
+    if (out->pg_vert) out->pg_vert[idx] = (rp->output_format != 1) ? gnmj : gnv;
+    if (out->pg_horiz) out->pg_horiz[idx] = (rp->output_format != 1) ? gnmn : gnh;
+    ethm *= wavelength;
+    ephm *= wavelength;
+    if (rp->rfld >= 1.0e-20) {
+      ethm = ethm * exrm; etha = etha + exra;
+      ephm = ephm * exrm; epha = epha + exra;
+    }
+    if (out->e_theta) {
+      double r = deg_to_rad(etha);
+      out->e_theta[2 * idx] = ethm * cos(r);
+      out->e_theta[2 * idx + 1] = ethm * sin(r);
+    }
+    if (out->e_phi) {
+      double r = deg_to_rad(epha);
+      out->e_phi[2 * idx] = ephm * cos(r);
+      out->e_phi[2 * idx + 1] = ephm * sin(r);
+    }
+    totbuf[idx] = gtot;
+    {
+      double a = axrat;
+      if (sense == 1) a = -a;
+      double gain = from_db10(gtot);
+      double lf = (1 + 2 * a + a * a) / (2 * (1 + a * a));
+      double rf = (1 - 2 * a + a * a) / (2 * (1 + a * a));
+      rhbuf[idx] = db10(gain * rf);
+      lhbuf[idx] = db10(gain * lf);
+    }
+    if (out->axial_ratio) out->axial_ratio[idx] = axrat;
+    if (out->tilt) out->tilt[idx] = tilta;
+    if (out->sense) out->sense[idx] = sense;
+    scbuf[idx] = (sin(tha) * (nth - 1) / nth) + 1.0 / nth;
+  }
+  /* pint accumulates in the reference's loop order (phi outer, theta inner) */
+  for (int64_t idx = 0; idx < (int64_t)nth * nph; idx++) pint += pintv[idx];
+  if (rp->iavp != 0) {
+    double ts = deg_to_rad(rp->theta0);
+    double tr = deg_to_rad(rp->dtheta) * (double)(nth - 1);
+    double pr = deg_to_rad(rp->dphi) * (double)(nph - 1);
+    double tt = ts + tr;
+    double sa = fabs(pr * (cos(ts) - cos(tt)));
+    out->average_power_gain = pint / sa;
+    out->average_power_solid_angle = sa / PI_T;
+  } else {
+    out->average_power_gain = 0.0;
+    out->average_power_solid_angle = 0.0;
+  }
+  double mx = gainbuf[0];
+  for (int64_t idx = 1; idx < (int64_t)nth * nph; idx++) if (gainbuf[idx] > mx) mx = gainbuf[idx];
+  out->max_gain = mx;
+  /* statistics, nec_radiation_pattern.h:252-333 (rows = theta, cols = phi) */
+  const double* arrs[3] = {totbuf, rhbuf, lhbuf};
+  for (int a = 0; a < 3; a++) {
+    const double* p = arrs[a];
+    double mxv = p[0], mnv = p[0];
+    for (int64_t idx = 1; idx < (int64_t)nth * nph; idx++) {
+      if (p[idx] > mxv) mxv = p[idx];
+      if (p[idx] < mnv) mnv = p[idx];
+    }
+    double sum = 0.0;
+    for (int i = 0; i < nth; i++)
+      for (int j = 0; j < nph; j++) sum += p[(int64_t)j * nth + i] * scbuf[(int64_t)j * nth + i];
+    int64_t len = (int64_t)nth * nph;
+    double mean = sum / ((double)(len * 2) / PI_T);
+    double s2 = 0.0;
+    for (int i = 0; i < nth; i++)
+      for (int j = 0; j < nph; j++) {
+        double diff = p[(int64_t)j * nth + i] - mean;
+        s2 += diff * diff * scbuf[(int64_t)j * nth + i];
+      }
+    double sdv = sqrt(s2 / ((double)(len * 2) / PI_T));
+    out->stats[4 * a + 0] = mxv;
+    out->stats[4 * a + 1] = mnv;
+    out->stats[4 * a + 2] = mean;
+    out->stats[4 * a + 3] = sdv;
+  }
+  if (!out->gain) free(gainbuf);
+  if (!out->pg_tot) free(totbuf);
+  if (!out->pg_rhcp) free(rhbuf);
+  if (!out->pg_lhcp) free(lhbuf);
+  if (!out->avg_scale) free(scbuf);
+  free(pintv); free(thv); free(phv);
+  return 0;
+}
+
+static double now_s(void) {
+  struct timespec ts;
+  clock_gettime(CLOCK_MONOTONIC, &ts);
+  return ts.tv_sec + 1e-9 * ts.tv_nsec;
+}
+
+/* One voltage-source run at one frequency: simulate() states
+ * MEMORY_ALLOC -> STRUCTURE_LOADING -> EXCITATION (src/nec_context.cpp:1031-1159,
+ * :1523-1558, :1622-1791, :3193-3210, :4850-5047). */
+int ora_run_voltage(ora_geom* g, double freq_mhz, double rkh, int use_ek,
+                    int nsrc, const int64_t* src_seg, const double* v,
+                    const ora_rp_spec* rp, ora_rp_out* rp_out,
+                    double* zin, double* current, double* input_power, double* abc_out,
+                    int nthreads, ora_timing* timing) {
+  int64_t n = g->n;
+  double wl = ora_wavelength(freq_mhz);
+  ora_frequency_scale(g, freq_mhz);
+  double t0 = now_s();
+  double* Z = (double*)malloc((size_t)n * n * 2 * sizeof(double));
+  if (!Z) { set_err("oracle: cannot allocate Z"); return 1; }
+  if (ora_fill(g, rkh, use_ek, Z, nthreads, NULL)) { free(Z); return 1; }
+  double t1 = now_s();
+  int32_t* piv = (int32_t*)malloc((size_t)n * sizeof(int32_t));
+  ora_lu(n, Z, piv, nthreads);
+  double t2 = now_s();
+  double* e = (double*)calloc((size_t)2 * n, sizeof(double));
+  for (int s = 0; s < nsrc; s++) {
+    int64_t k = src_seg[s] - 1;
+    cx val = cdivr(cneg(mk(v[2 * s], v[2 * s + 1])), g->ss[k] * wl);
+    e[2 * k] = val.re;
+    e[2 * k + 1] = val.im;
+  }
+  ora_lu_solve(n, Z, piv, e);
+  double* abc = abc_out ? abc_out : (double*)malloc((size_t)6 * n * sizeof(double));
+  if (ora_current_coefficients(g, e, abc)) { free(Z); free(piv); free(e); return 1; }
+  double pin = 0.0;
+  for (int s = 0; s < nsrc; s++) {
+    int64_t k = src_seg[s] - 1;
+    cx volt = mk(v[2 * s], v[2 * s + 1]);
+    cx cur = cmulr(mk(e[2 * k], e[2 * k + 1]), wl);
+    cx z = cdiv(volt, cur);
+    if (zin) { zin[2 * s] = z.re; zin[2 * s + 1] = z.im; }
+    double p = 0.5 * cmul(volt, mk(cur.re, -cur.im)).re;
+    pin += p;
+  }
+  if (input_power) *input_power = pin;
+  if (current)
+    for (int64_t i = 0; i < n; i++) { current[2 * i] = e[2 * i] * wl; current[2 * i + 1] = e[2 * i + 1] * wl; }
+  double t3 = now_s();
+  if (rp && rp_out) ora_pattern(g, abc, wl, pin, 0.0, rp, rp_out, nthreads);
+  double t4 = now_s();
+  if (timing) { timing->fill_s = t1 - t0; timing->factor_s = t2 - t1; timing->solve_s = t3 - t2; timing->pattern_s = t4 - t3; }
+  if (!abc_out) free(abc);
+  free(Z); free(piv); free(e);
+  return 0;
+}

@@ -1,0 +1,175 @@
+/*
+ * necpp_oracle.h -- CPU restatement of nec2++'s free-space thin-wire
+ * method-of-moments path (geometry -> basis -> fill -> LU -> solve ->
+ * currents / Z_in -> far-field pattern).
+ *
+ * THIS IS TEST INFRASTRUCTURE. It exists only so that tests/, the smoke
+ * check and the bench's cpu_baseline leg can check and time the HIP product
+ * path against an independent CPU statement of the reference algorithm.
+ * Nothing in the product library links, loads or calls it.
+ *
+ * Each function cites the reference routine it follows (paths relative to
+ * the nec2++ source tree).  Arithmetic follows the reference's operation
+ * order with IEEE binary64 and no FMA contraction (build with
+ * -ffp-contract=off), so geometry, basis coefficients and matrix entries are
+ * bit-identical to nec2++ compiled with g++ on x86-64.  The LU is a plain
+ * right-looking partial-pivot factorisation (same pivot rule as Eigen's
+ * PartialPivLU: first maximum of |z|), so its rounding differs from Eigen's
+ * blocked kernel at the 1e-15 * cond level.
+ */
+#ifndef NECPP_ORACLE_H
+#define NECPP_ORACLE_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ora_geom ora_geom;
+
+/* Last error text (static storage, overwritten by the next error). */
+const char* ora_last_error(void);
+
+ora_geom* ora_geom_new(void);
+void ora_geom_free(ora_geom* g);
+
+/* c_geometry::wire (src/c_geometry.cpp:614-683).  0 = ok. */
+int ora_wire(ora_geom* g, int tag, int ns, double x1, double y1, double z1,
+             double x2, double y2, double z2, double rad, double rdel, double rrad);
+
+/* c_geometry::geometry_complete (src/c_geometry.cpp:454-604) with
+ * build_connections (:1705-1915) and resolve_junctions (:1918-2074).
+ * 0 = ok, 1 = geometry error (message in ora_last_error). */
+int ora_geometry_complete(ora_geom* g, int gpflag);
+
+int64_t ora_nseg(const ora_geom* g);
+
+/* Per-segment unscaled data after geometry_complete: out[8*i + 0..7] =
+ * x, y, z (centre), length, radius, cab, sab, salp.  icon1/icon2/tags may be
+ * NULL. */
+void ora_segments(const ora_geom* g, double* out, int32_t* icon1, int32_t* icon2, int32_t* tags);
+
+/* c_geometry::get_segment_number (src/c_geometry.cpp:76-99): 1-based segment
+ * number, or -1 when no such tag (error text set). */
+int64_t ora_segment_number(const ora_geom* g, int tag, int m);
+
+/* em::get_wavelength(1e6 * f) with the reference's truncated constants
+ * (src/electromag.cpp:25-35, src/math_util.h:72-74). */
+double ora_wavelength(double freq_mhz);
+double ora_speed_of_light(void);
+double ora_impedance(void);
+
+/* c_geometry::frequency_scale (src/c_geometry.cpp:3389-3418). */
+void ora_frequency_scale(ora_geom* g, double freq_mhz);
+
+/* c_geometry::trio + sbf (src/c_geometry.cpp:2982-3289) on the scaled
+ * geometry: the basis functions with support on segment j (1-based).
+ * Returns jsno (or -1 on a connection error); jco is 1-based. */
+int ora_trio(const ora_geom* g, int j, int32_t* jco, double* ax, double* bx, double* cx, int cap);
+
+/* c_geometry::tbf (src/c_geometry.cpp:2763-2977). */
+int ora_tbf(const ora_geom* g, int i, int icap, int32_t* jco, double* ax, double* bx, double* cx, int cap);
+
+/* Counters of the last ora_fill call (for algorithmic-work accounting). */
+typedef struct {
+  int64_t pairs;        /* (observation, source) pairs evaluated         */
+  int64_t far_pairs;    /* r >= rkh lumped-element branch                */
+  int64_t near_pairs;   /* eksc / ekscx branch                            */
+  int64_t gx_calls;
+  int64_t gxx_calls;
+  int64_t intx_calls;
+  int64_t gf_calls;
+  int64_t scatter_terms;/* sum over pairs of jsno(j)                      */
+} ora_fill_counts;
+
+/* nec_context::cmset / cmww / efld / efld_compute / eksc / ekscx / gf / gx /
+ * gxx / intx (src/nec_context.cpp:2077-4297) for free space, no loads, no
+ * symmetry.  Z is N x N complex (interleaved re,im) in the reference's
+ * "cm" layout: Z[i][k] at 2*(i*N + k), i = observation segment, k = basis.
+ * nthreads > 1 splits observation rows over OpenMP threads (same values). */
+int ora_fill(const ora_geom* g, double rkh, int use_ek, double* Z, int nthreads, ora_fill_counts* counts);
+
+/* lu_decompose + Eigen PartialPivLU (src/matrix_algebra.cpp:217-266,
+ * src/eigen/Eigen/src/LU/PartialPivLU.h:338-465) on the row-major Z above
+ * (row pivoting on Z = column-major Z^T of the reference's cm): in place
+ * L\U, piv[k] = row swapped with k at step k (0-based transpositions).
+ * Returns -1, or the first exactly-zero pivot index. */
+int64_t ora_lu(int64_t n, double* Z, int32_t* piv, int nthreads);
+
+/* matrix_algebra.cpp solve (:271-307): b <- Z^-1 b. */
+void ora_lu_solve(int64_t n, const double* LU, const int32_t* piv, double* b);
+
+/* c_geometry::get_current_coefficients (src/c_geometry.cpp:3299-3386) for
+ * nqds = 0, no patches.  cur: in = solution vector, out = curx (A+C).
+ * abc: 6*N = air, aii, bir, bii, cir, cii. */
+int ora_current_coefficients(const ora_geom* g, double* cur, double* abc);
+
+/* nec_context::ffld (src/nec_context.cpp:6639-6887), free space: E_theta and
+ * E_phi for direction (thet, phi) given in radians through their sines and
+ * cosines. */
+void ora_ffld(const ora_geom* g, const double* abc, double sth, double cth, double sph, double cph,
+              double* eth, double* eph);
+
+/* Pattern specification as accepted by nec_rp_card (ifar must be 0). */
+typedef struct {
+  int n_theta, n_phi;
+  double theta0, phi0, dtheta, dphi;  /* degrees */
+  int output_format;                  /* XNDA X: 0 major/minor, 1 vert/horiz */
+  int normalization;                  /* XNDA N */
+  int ipd;                            /* XNDA D */
+  int iavp;                           /* XNDA A */
+  double rfld;                        /* radial distance (m) */
+  double gnor;
+} ora_rp_spec;
+
+/* Pattern results; arrays are n_theta*n_phi, index phi_i*n_theta + theta_i
+ * (nec_radiation_pattern.cpp:28-35). Any pointer may be NULL. */
+typedef struct {
+  double* gain;        /* _gain (normalisation-selected dB)        */
+  double* pg_vert;     /* _power_gain_vert                         */
+  double* pg_horiz;    /* _power_gain_horiz                        */
+  double* pg_tot;      /* _power_gain_tot                          */
+  double* pg_rhcp;
+  double* pg_lhcp;
+  double* axial_ratio;
+  double* tilt;
+  int32_t* sense;
+  double* avg_scale;   /* _averaging_scales                        */
+  double* e_theta;     /* 2 per point (re, im) after lambda / range scaling */
+  double* e_phi;
+  /* scalars */
+  double max_gain;     /* _gain.maxCoeff()                         */
+  double average_power_gain;
+  double average_power_solid_angle;
+  double stats[12];    /* tot max,min,mean,sd; rhcp max,min,mean,sd; lhcp ... */
+} ora_rp_out;
+
+/* nec_radiation_pattern::analyze (src/nec_radiation_pattern.cpp:209-450) for
+ * ifar = 0 and voltage excitation, plus the statistics of
+ * nec_radiation_pattern.h:252-333.  pinr = input power, pnlr = network loss.
+ * nthreads > 1 splits directions over OpenMP threads. */
+int ora_pattern(const ora_geom* g, const double* abc, double wavelength, double pinr, double pnlr,
+                const ora_rp_spec* rp, ora_rp_out* out, int nthreads);
+
+/* One complete voltage-source run at one frequency, as XQ/RP would do it:
+ * frequency_scale, fill, factor, RHS (etmns_voltage_source), solve, current
+ * coefficients, Z_in per source (netwk_compute_inputs) and, if rp != NULL,
+ * the pattern.  src_seg are 1-based segment numbers, v = 2*nsrc doubles.
+ * Outputs (any may be NULL): zin (2*nsrc), current (2*N, curx * lambda),
+ * input_power, abc (6*N). */
+typedef struct {
+  double fill_s, factor_s, solve_s, pattern_s;
+} ora_timing;
+
+int ora_run_voltage(ora_geom* g, double freq_mhz, double rkh, int use_ek,
+                    int nsrc, const int64_t* src_seg, const double* v,
+                    const ora_rp_spec* rp, ora_rp_out* rp_out,
+                    double* zin, double* current, double* input_power, double* abc,
+                    int nthreads, ora_timing* timing);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

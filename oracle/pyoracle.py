@@ -1,0 +1,272 @@
+"""ctypes binding of the CPU restatement in necpp_oracle.c.
+
+Test infrastructure only: tests/, the smoke check and the bench's
+cpu_baseline leg load it as the checker.  The product path never imports it.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+
+def build():
+    """Compile liboracle.so with the recipe in oracle/Makefile."""
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
+        L = ctypes.CDLL(_LIB_PATH)
+        d, i, i64, p = ctypes.c_double, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
+        L.ora_last_error.restype = ctypes.c_char_p
+        L.ora_geom_new.restype = p
+        L.ora_geom_free.argtypes = [p]
+        L.ora_wire.argtypes = [p, i, i, d, d, d, d, d, d, d, d, d]
+        L.ora_geometry_complete.argtypes = [p, i]
+        L.ora_nseg.argtypes = [p]
+        L.ora_nseg.restype = i64
+        L.ora_segments.argtypes = [p, p, p, p, p]
+        L.ora_segment_number.argtypes = [p, i, i]
+        L.ora_segment_number.restype = i64
+        L.ora_wavelength.argtypes = [d]
+        L.ora_wavelength.restype = d
+        L.ora_speed_of_light.restype = d
+        L.ora_impedance.restype = d
+        L.ora_frequency_scale.argtypes = [p, d]
+        L.ora_trio.argtypes = [p, i, p, p, p, p, i]
+        L.ora_tbf.argtypes = [p, i, i, p, p, p, p, i]
+        L.ora_fill.argtypes = [p, d, i, p, i, p]
+        L.ora_lu.argtypes = [i64, p, p, i]
+        L.ora_lu.restype = i64
+        L.ora_lu_solve.argtypes = [i64, p, p, p]
+        L.ora_current_coefficients.argtypes = [p, p, p]
+        L.ora_ffld.argtypes = [p, p, d, d, d, d, p, p]
+        L.ora_pattern.argtypes = [p, p, d, d, d, p, p, i]
+        L.ora_run_voltage.argtypes = [p, d, d, i, i, p, p, p, p, p, p, p, p, i, p]
+        _lib = L
+    return _lib
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class FillCounts(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in
+                ("pairs", "far_pairs", "near_pairs", "gx_calls", "gxx_calls",
+                 "intx_calls", "gf_calls", "scatter_terms")]
+
+
+class RpSpec(ctypes.Structure):
+    _fields_ = [("n_theta", ctypes.c_int), ("n_phi", ctypes.c_int),
+                ("theta0", ctypes.c_double), ("phi0", ctypes.c_double),
+                ("dtheta", ctypes.c_double), ("dphi", ctypes.c_double),
+                ("output_format", ctypes.c_int), ("normalization", ctypes.c_int),
+                ("ipd", ctypes.c_int), ("iavp", ctypes.c_int),
+                ("rfld", ctypes.c_double), ("gnor", ctypes.c_double)]
+
+
+class RpOut(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in
+                ("gain", "pg_vert", "pg_horiz", "pg_tot", "pg_rhcp", "pg_lhcp",
+                 "axial_ratio", "tilt", "sense", "avg_scale", "e_theta", "e_phi")] + [
+        ("max_gain", ctypes.c_double), ("average_power_gain", ctypes.c_double),
+        ("average_power_solid_angle", ctypes.c_double), ("stats", ctypes.c_double * 12)]
+
+
+class Timing(ctypes.Structure):
+    _fields_ = [("fill_s", ctypes.c_double), ("factor_s", ctypes.c_double),
+                ("solve_s", ctypes.c_double), ("pattern_s", ctypes.c_double)]
+
+
+def speed_of_light():
+    return lib().ora_speed_of_light()
+
+
+def wavelength(f_mhz):
+    return lib().ora_wavelength(f_mhz)
+
+
+class Geometry:
+    """Wire geometry held by the oracle (c_geometry restated)."""
+
+    def __init__(self):
+        self._L = lib()
+        self._g = self._L.ora_geom_new()
+
+    def __del__(self):
+        if getattr(self, "_g", None):
+            self._L.ora_geom_free(self._g)
+            self._g = None
+
+    def wire(self, tag, ns, x1, y1, z1, x2, y2, z2, rad, rdel=1.0, rrad=1.0):
+        self._L.ora_wire(self._g, tag, ns, x1, y1, z1, x2, y2, z2, rad, rdel, rrad)
+
+    def complete(self, gpflag=0):
+        if self._L.ora_geometry_complete(self._g, gpflag) != 0:
+            raise RuntimeError(self._L.ora_last_error().decode())
+
+    @property
+    def n(self):
+        return int(self._L.ora_nseg(self._g))
+
+    def segments(self):
+        n = self.n
+        out = np.zeros((n, 8))
+        ic1 = np.zeros(n, np.int32)
+        ic2 = np.zeros(n, np.int32)
+        tags = np.zeros(n, np.int32)
+        self._L.ora_segments(self._g, _ptr(out), _ptr(ic1), _ptr(ic2), _ptr(tags))
+        return out, ic1, ic2, tags
+
+    def segment_number(self, tag, m):
+        r = self._L.ora_segment_number(self._g, tag, m)
+        if r < 0:
+            raise RuntimeError(self._L.ora_last_error().decode())
+        return int(r)
+
+    def scale(self, f_mhz):
+        self._L.ora_frequency_scale(self._g, f_mhz)
+
+    def trio(self, j):
+        cap = 64
+        jco = np.zeros(cap, np.int32)
+        a, b, c = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+        k = self._L.ora_trio(self._g, j, _ptr(jco), _ptr(a), _ptr(b), _ptr(c), cap)
+        if k < 0:
+            raise RuntimeError(self._L.ora_last_error().decode())
+        return jco[:k].copy(), a[:k].copy(), b[:k].copy(), c[:k].copy()
+
+    def tbf(self, i, icap=1):
+        cap = 64
+        jco = np.zeros(cap, np.int32)
+        a, b, c = np.zeros(cap), np.zeros(cap), np.zeros(cap)
+        k = self._L.ora_tbf(self._g, i, icap, _ptr(jco), _ptr(a), _ptr(b), _ptr(c), cap)
+        if k < 0:
+            raise RuntimeError(self._L.ora_last_error().decode())
+        return jco[:k].copy(), a[:k].copy(), b[:k].copy(), c[:k].copy()
+
+    def fill(self, rkh=1.0, use_ek=False, nthreads=1):
+        """Z[i, k] (observation i, basis k) at the current scaling."""
+        n = self.n
+        Z = np.zeros((n, n), np.complex128)
+        cnt = FillCounts()
+        if self._L.ora_fill(self._g, rkh, int(use_ek), _ptr(Z), nthreads, ctypes.byref(cnt)) != 0:
+            raise RuntimeError(self._L.ora_last_error().decode())
+        return Z, {k: getattr(cnt, k) for k, _ in FillCounts._fields_}
+
+    def current_coefficients(self, x):
+        n = self.n
+        cur = np.ascontiguousarray(x, np.complex128).copy()
+        abc = np.zeros(6 * n)
+        if self._L.ora_current_coefficients(self._g, _ptr(cur), _ptr(abc)) != 0:
+            raise RuntimeError(self._L.ora_last_error().decode())
+        return cur, abc
+
+    def ffld(self, abc, theta_rad, phi_rad):
+        et = np.zeros(2)
+        ep = np.zeros(2)
+        self._L.ora_ffld(self._g, _ptr(np.ascontiguousarray(abc, np.float64)),
+                         np.sin(theta_rad), np.cos(theta_rad), np.sin(phi_rad), np.cos(phi_rad),
+                         _ptr(et), _ptr(ep))
+        return complex(et[0], et[1]), complex(ep[0], ep[1])
+
+    def run_voltage(self, f_mhz, src, rkh=1.0, use_ek=False, rp=None, nthreads=1):
+        """One frequency: fill, LU, solve, Z_in and (optionally) the pattern.
+
+        src: list of (segment_number_1based, complex_voltage).  rp: dict with
+        the nec_rp_card fields n_theta, n_phi, theta0, phi0, dtheta, dphi and
+        optionally output_format, normalization, ipd, iavp, rfld, gnor.
+        """
+        n = self.n
+        segs = np.array([s for s, _ in src], np.int64)
+        v = np.array([[complex(vv).real, complex(vv).imag] for _, vv in src], np.float64).ravel()
+        zin = np.zeros(2 * len(src))
+        cur = np.zeros(2 * n)
+        pin = ctypes.c_double()
+        abc = np.zeros(6 * n)
+        tim = Timing()
+        spec = out = None
+        res = {}
+        if rp is not None:
+            spec = RpSpec(rp["n_theta"], rp["n_phi"], rp["theta0"], rp["phi0"], rp["dtheta"], rp["dphi"],
+                          rp.get("output_format", 0), rp.get("normalization", 0), rp.get("ipd", 0),
+                          rp.get("iavp", 0), rp.get("rfld", 0.0), rp.get("gnor", 0.0))
+            if spec.n_theta < 2 or spec.n_phi < 2:
+                spec.iavp = 0
+            npt = spec.n_theta * spec.n_phi
+            arrs = {k: np.zeros(npt) for k in ("gain", "pg_vert", "pg_horiz", "pg_tot", "pg_rhcp",
+                                               "pg_lhcp", "axial_ratio", "tilt", "avg_scale")}
+            arrs["sense"] = np.zeros(npt, np.int32)
+            arrs["e_theta"] = np.zeros(2 * npt)
+            arrs["e_phi"] = np.zeros(2 * npt)
+            out = RpOut()
+            for k, a in arrs.items():
+                setattr(out, k, a.ctypes.data)
+            res["pattern"] = arrs
+        rc = self._L.ora_run_voltage(self._g, f_mhz, rkh, int(use_ek), len(src), _ptr(segs), _ptr(v),
+                                     ctypes.byref(spec) if spec is not None else None,
+                                     ctypes.byref(out) if out is not None else None,
+                                     _ptr(zin), _ptr(cur), ctypes.byref(pin), _ptr(abc), nthreads,
+                                     ctypes.byref(tim))
+        if rc != 0:
+            raise RuntimeError(self._L.ora_last_error().decode())
+        res["zin"] = zin[0::2] + 1j * zin[1::2]
+        res["current"] = cur[0::2] + 1j * cur[1::2]
+        res["input_power"] = pin.value
+        res["abc"] = abc
+        res["timing"] = {k: getattr(tim, k) for k, _ in Timing._fields_}
+        if out is not None:
+            pat = res["pattern"]
+            pat["e_theta"] = pat["e_theta"][0::2] + 1j * pat["e_theta"][1::2]
+            pat["e_phi"] = pat["e_phi"][0::2] + 1j * pat["e_phi"][1::2]
+            res["max_gain"] = out.max_gain
+            res["average_power_gain"] = out.average_power_gain
+            res["stats"] = list(out.stats)
+        return res
+
+
+def lu(Z, nthreads=1):
+    """Partial-pivot LU of Z (row pivoting); returns (LU, piv, first_zero)."""
+    A = np.ascontiguousarray(Z, np.complex128).copy()
+    n = A.shape[0]
+    piv = np.zeros(n, np.int32)
+    fz = lib().ora_lu(n, _ptr(A), _ptr(piv), nthreads)
+    return A, piv, int(fz)
+
+
+def lu_solve(LU, piv, b):
+    x = np.ascontiguousarray(b, np.complex128).copy()
+    lib().ora_lu_solve(LU.shape[0], _ptr(np.ascontiguousarray(LU)), _ptr(piv), _ptr(x))
+    return x
+
+
+def geometry_from_deck(deck):
+    g = Geometry()
+    for w in deck["wires"]:
+        g.wire(*w)
+    g.complete(deck["gpflag"])
+    return g
+
+
+def run_deck(deck, nthreads=1, with_rp=True, rkh=1.0, use_ek=False):
+    """Oracle run of a necpp_amd.decks deck: returns run_voltage's dict."""
+    g = geometry_from_deck(deck)
+    seg = g.segment_number(*deck["source"])
+    rp = None
+    if with_rp and "rp" in deck:
+        (_, nth, nph, fmt, norm, ipd, iavp, th0, ph0, dth, dph, rfld, gnor) = deck["rp"]
+        rp = dict(n_theta=nth, n_phi=nph, theta0=th0, phi0=ph0, dtheta=dth, dphi=dph, output_format=fmt,
+                  normalization=norm, ipd=ipd, iavp=iavp, rfld=rfld, gnor=gnor)
+    res = g.run_voltage(deck["freq_mhz"], [(seg, deck["voltage"])], rkh=rkh, use_ek=use_ek, rp=rp,
+                        nthreads=nthreads)
+    res["geometry"] = g
+    return res

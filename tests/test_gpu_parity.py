@@ -1,0 +1,256 @@
+"""GPU parity: the HIP path through the C ABI against the CPU oracle.
+
+Every test runs the product library (libnecpp_amd.so) on the GPU and checks
+it against oracle/necpp_oracle.c on the same inputs, and against the
+reference's known answers where they exist.  Bar: FP64 results within 1e-10
+relative; the fill keeps the reference's
+operation order, so it is held to 1e-12.
+"""
+import ctypes
+import os
+
+import numpy as np
+import pytest
+import torch  # noqa: F401  (imported at collection: the first import on a fresh box is slow)
+
+from necpp_amd import decks
+from necpp_amd import necpp as api
+from oracle import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-10
+
+
+def make(deck, ek=False):
+    ctx = api.nec_create()
+    assert ctx
+    for w in deck["wires"]:
+        assert api.nec_wire(ctx, *w) == 0
+    assert api.nec_geometry_complete(ctx, deck["gpflag"]) == 0, api.nec_error_message()
+    if ek:
+        assert api.nec_ek_card(ctx, 0) == 0
+    return ctx
+
+
+def run(deck, with_rp=True, ek=False):
+    ctx = make(deck, ek)
+    assert api.nec_fr_card(ctx, 0, 1, deck["freq_mhz"], 0.0) == 0
+    s = deck["source"]
+    v = deck["voltage"]
+    assert api.nec_ex_card(ctx, 0, s[0], s[1], 0, v.real, v.imag, 0, 0, 0, 0) == 0, api.nec_error_message()
+    if with_rp and "rp" in deck:
+        assert api.nec_rp_card(ctx, *deck["rp"]) == 0, api.nec_error_message()
+    else:
+        assert api.nec_xq_card(ctx, 0) == 0, api.nec_error_message()
+    return ctx
+
+
+def zin(ctx, k=0):
+    return complex(api.nec_impedance_real(ctx, k), api.nec_impedance_imag(ctx, k))
+
+
+def rel(a, b):
+    return abs(a - b) / abs(b)
+
+
+# ---------------------------------------------------------------- LU / solve
+
+def _torch():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a visible MI355X"
+    return torch
+
+
+@pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 127, 130, 257, 300])
+def test_zgetrf_zgetrs_random(n):
+    torch = _torch()
+    rng = np.random.default_rng(n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dA = torch.from_numpy(A.copy()).to("cuda")
+    piv = torch.zeros(2 * n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    assert api.necgpu_zgetrf(dA.data_ptr(), n, n, piv.data_ptr(), info.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    LU = dA.cpu().numpy()
+    # oracle factorisation of the same matrix: same pivots, same factors
+    oLU, opiv, _ = po.lu(A)
+    p = piv.cpu().numpy()
+    assert np.array_equal(p[:n], opiv)
+    scale = np.max(np.abs(oLU))
+    assert np.max(np.abs(LU - oLU)) <= 1e-12 * scale * max(1, n / 64)
+    # the composed permutation agrees with the transpositions
+    perm = np.arange(n)
+    for k in range(n):
+        perm[[k, opiv[k]]] = perm[[opiv[k], k]]
+    assert np.array_equal(p[n:], perm)
+    ws = torch.zeros(int(api.necgpu_zgetrs_workspace(n)), dtype=torch.uint8, device="cuda")
+    for epoch in (1, 2):
+        x = torch.from_numpy(b.copy()).to("cuda")
+        assert api.necgpu_zgetrs(dA.data_ptr(), n, n, piv.data_ptr(), x.data_ptr(), ws.data_ptr(), epoch, None) == 0
+        torch.cuda.synchronize()
+        xs = x.cpu().numpy()
+        ref = np.linalg.solve(A, b)
+        assert np.linalg.norm(xs - ref) <= TOL * np.linalg.norm(ref)
+    assert np.linalg.norm(A @ xs - b) <= 1e-12 * np.linalg.norm(b) * n
+
+
+def test_zgetrf_reports_zero_pivot():
+    torch = _torch()
+    n = 70
+    A = np.zeros((n, n), np.complex128)
+    A[:, :3] = np.random.default_rng(1).standard_normal((n, 3))
+    dA = torch.from_numpy(A).to("cuda")
+    piv = torch.zeros(2 * n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    assert api.necgpu_zgetrf(dA.data_ptr(), n, n, piv.data_ptr(), info.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    assert int(info.item()) == 4  # 1 + first exactly-zero pivot (Eigen's first_zero_pivot)
+
+
+# ---------------------------------------------------------------- fill
+
+FILL_DECKS = [
+    ("dipole7", decks.dipole(7, rp=None), False),
+    ("dipole51", decks.dipole(51, rp=None), False),
+    ("grid9", decks.grid(9, rp=None), False),
+    ("yagi4", decks.yagi([0.5, 0.48, 0.45, 0.44], [0.4, 0.3, 0.35], 0.003, 11, rp=None), False),
+    ("dipole51_ek", decks.dipole(51, rp=None), True),
+    ("yagi4_ek", decks.yagi([0.5, 0.48, 0.45, 0.44], [0.4, 0.3, 0.35], 0.003, 11, rp=None), True),
+]
+
+
+@pytest.mark.parametrize("name,deck,ek", FILL_DECKS, ids=[d[0] for d in FILL_DECKS])
+def test_fill_matches_oracle(name, deck, ek):
+    ctx = make(deck, ek)
+    try:
+        Z = api.gpu_fill_matrix(ctx, deck["freq_mhz"])
+        g = po.geometry_from_deck(deck)
+        g.scale(deck["freq_mhz"])
+        oZ, counts = g.fill(use_ek=ek, nthreads=4)
+        assert counts["near_pairs"] > 0
+        err = np.abs(Z - oZ)
+        bound = 1e-12 * np.abs(oZ) + 1e-15 * np.max(np.abs(oZ))
+        assert np.all(err <= bound), (np.max(err / (np.abs(oZ) + 1e-300)), np.count_nonzero(err > bound))
+    finally:
+        api.nec_delete(ctx)
+
+
+# ---------------------------------------------------------------- end to end
+
+def test_example1_kat_gpu(kats):
+    k = kats["example1_dipole_7seg"]
+    ctx = api.nec_create()
+    try:
+        assert api.nec_wire(ctx, *k["deck"]["wire"]) == 0
+        assert api.nec_geometry_complete(ctx, 1) == 0
+        assert api.nec_ex_card(ctx, 0, 0, 4, 0, 1.0, 0.0, 0.0, 0.0, 0.0, 0.0) == 0
+        assert api.nec_xq_card(ctx, 0) == 0, api.nec_error_message()
+        z = zin(ctx)
+        assert rel(z, complex(*k["zin"])) < 1e-12
+        assert rel(z, complex(*k["zin_reference_now"])) < 1e-12
+    finally:
+        api.nec_delete(ctx)
+
+
+def test_c1_dipole_gpu(kats):
+    k = kats["c1_dipole_51seg"]
+    deck = decks.dipole()
+    ctx = run(deck)
+    try:
+        o = po.run_deck(deck, nthreads=8)
+        z = zin(ctx)
+        assert rel(z, o["zin"][0]) < TOL
+        assert rel(z, complex(*k["zin"])) < TOL
+        cur = api.gpu_currents(ctx, 0)
+        assert np.linalg.norm(cur - o["current"]) <= TOL * np.linalg.norm(o["current"])
+        assert abs(api.nec_gain(ctx, 0, 90, 0) - k["gain_theta90_phi0"]) < 1e-9
+        assert abs(api.nec_gain_max(ctx, 0) - k["gain_max"]) < 1e-9
+        npt = 181 * 361
+        for name in ("gain", "pg_vert", "pg_horiz", "pg_tot", "pg_rhcp", "pg_lhcp"):
+            a = api.gpu_pattern_array(ctx, 0, name, npt)
+            b = o["pattern"][name]
+            finite = b > -999
+            assert np.array_equal(a > -999, finite), name
+            # dB values: absolute agreement; deep nulls amplify relative field errors
+            assert np.max(np.abs(a[finite] - b[finite])) < 1e-8, name
+        for name in ("e_theta", "e_phi"):
+            a = api.gpu_pattern_array(ctx, 0, name, npt)
+            b = o["pattern"][name]
+            assert np.max(np.abs(a - b)) <= TOL * np.max(np.abs(b)), name
+        stats = [api.nec_gain_max(ctx, 0), api.nec_gain_min(ctx, 0), api.nec_gain_mean(ctx, 0),
+                 api.nec_gain_sd(ctx, 0)]
+        assert np.allclose(stats[:1] + stats[2:], [o["stats"][0]] + o["stats"][2:4], rtol=1e-10, atol=1e-9)
+    finally:
+        api.nec_delete(ctx)
+
+
+def test_c2_grid_gpu(kats):
+    k = kats["c2_grid_3280"]
+    deck = decks.grid(41)
+    ctx = run(deck)
+    try:
+        z = zin(ctx)
+        assert rel(z, complex(*k["zin"])) < k["tol_rel_print"]
+        o = po.run_deck(deck, nthreads=16, with_rp=False)
+        assert rel(z, o["zin"][0]) < TOL
+        cur = api.gpu_currents(ctx, 0)
+        assert np.linalg.norm(cur - o["current"]) <= TOL * np.linalg.norm(o["current"])
+        t = api.gpu_stage_times(ctx)
+        print("C2 stage times (ms):", t)
+    finally:
+        api.nec_delete(ctx)
+
+
+def test_frequency_sweep_and_reuse():
+    deck = decks.dipole(21, rp=(19, 5))
+    ctx = make(deck)
+    try:
+        assert api.nec_fr_card(ctx, 0, 3, 250.0, 25.0) == 0
+        assert api.nec_ex_card(ctx, 0, 1, 11, 0, 1.0, 0.0, 0, 0, 0, 0) == 0
+        assert api.nec_rp_card(ctx, 0, 19, 5, 1, 0, 0, 1, 0.0, 0.0, 10.0, 90.0, 0, 0) == 0, api.nec_error_message()
+        g = po.geometry_from_deck(deck)
+        for m, f in enumerate((250.0, 275.0, 300.0)):
+            o = g.run_voltage(f, [(11, 1.0)], rp=dict(n_theta=19, n_phi=5, theta0=0, phi0=0, dtheta=10,
+                                                         dphi=90, output_format=1, iavp=1))
+            assert rel(zin(ctx, m), o["zin"][0]) < TOL
+            assert abs(api.nec_gain_max(ctx, m) - o["stats"][0]) < 1e-9
+            assert abs(api.nec_gain(ctx, m, 9, 2) - o["pattern"]["gain"][2 * 19 + 9]) < 1e-9
+        assert api.nec_impedance_real(ctx, 3) == -999.0
+        # a new source at the last frequency re-uses the factors
+        assert api.nec_ex_card(ctx, 0, 1, 5, 0, 0.5, 0.25, 0, 0, 0, 0) == 0
+        assert api.nec_xq_card(ctx, 0) == 0, api.nec_error_message()
+        o = g.run_voltage(300.0, [(5, 0.5 + 0.25j)])
+        assert rel(zin(ctx, 3), o["zin"][0]) < TOL
+    finally:
+        api.nec_delete(ctx)
+
+
+def test_two_feeds_indexed():
+    deck = decks.dipole(31, rp=None)
+    ctx = make(deck)
+    try:
+        assert api.nec_fr_card(ctx, 0, 1, 280.0, 0) == 0
+        assert api.nec_ex_card(ctx, 0, 1, 10, 0, 1.0, 0.0, 0, 0, 0, 0) == 0
+        assert api.nec_ex_card(ctx, 0, 1, 22, 0, 0.0, 1.0, 0, 0, 0, 0) == 0
+        assert api.nec_xq_card(ctx, 0) == 0
+        o = po.geometry_from_deck(deck).run_voltage(280.0, [(10, 1.0), (22, 1j)])
+        for f in range(2):
+            z = complex(api.nec_impedance_real_indexed(ctx, 0, f), api.nec_impedance_imag_indexed(ctx, 0, f))
+            assert rel(z, o["zin"][f]) < TOL
+    finally:
+        api.nec_delete(ctx)
+
+
+def test_device_allocation_failure_is_reported():
+    deck = decks.dipole(51, rp=None)
+    ctx = make(deck)
+    os.environ["NECGPU_FAIL_ALLOC"] = str(40000)
+    try:
+        assert api.nec_xq_card(ctx, 0) == 1
+        assert "out of memory" in api.nec_error_message()
+    finally:
+        del os.environ["NECGPU_FAIL_ALLOC"]
+        api.nec_delete(ctx)
