@@ -74,8 +74,12 @@ long nec_gpu_pattern_average(nec_context* ctx, int freq_index, double* avg_gain,
 /* In-place partial-pivot LU of the n x n matrix at device pointer a.
  * piv: device int buffer of 2*n entries (transpositions, then the composed
  * row permutation).  info: device int, set to 1 + first zero pivot or 0.
- * stream: hipStream_t (NULL = default stream). */
-long necgpu_zgetrf(double* a, long long lda, int n, int* piv, int* info, void* stream);
+ * work: device scratch of necgpu_zgetrf_workspace(n) bytes (no
+ * initialisation needed).  stream: hipStream_t (NULL = default stream).
+ * Returns 1 if n exceeds the largest order the panel kernel supports
+ * (28,672). */
+long long necgpu_zgetrf_workspace(int n);
+long necgpu_zgetrf(double* a, long long lda, int n, int* piv, int* info, void* work, void* stream);
 
 /* Solve with the factors of necgpu_zgetrf: x (device, n complex) is
  * overwritten.  work: device buffer of necgpu_zgetrs_workspace(n) bytes,
@@ -84,6 +88,11 @@ long necgpu_zgetrf(double* a, long long lda, int n, int* piv, int* info, void* s
 long long necgpu_zgetrs_workspace(int n);
 long necgpu_zgetrs(const double* lu, long long lda, int n, const int* piv, double* x, void* work, unsigned epoch,
                    void* stream);
+
+/* Self-check of the FP64 matrix-core lane layout the LU relies on: one wave
+ * feeds a[l], b[l] (device, 64 doubles each) to v_mfma_f64_16x16x4f64 and
+ * writes the 4 result registers of every lane to d (device, 256 doubles). */
+long necgpu_check_mfma_f64(const double* a, const double* b, double* d, void* stream);
 
 #ifdef __cplusplus
 }

@@ -325,10 +325,20 @@ long nec_gpu_pattern_average(nec_context* ctx, int freq_index, double* avg_gain,
   });
 }
 
-long necgpu_zgetrf(double* a, long long lda, int n, int* piv, int* info, void* stream) {
+long long necgpu_zgetrf_workspace(int n) { return static_cast<long long>(necgpu::lu_workspace_bytes(n)); }
+
+long necgpu_zgetrf(double* a, long long lda, int n, int* piv, int* info, void* work, void* stream) {
   return guarded([&] {
-    if (necgpu::launch_lu_fp64(a, lda, n, piv, info, static_cast<hipStream_t>(stream)))
-      necgpu::hip_check(hipGetLastError(), "necgpu_zgetrf");
+    const int rc = necgpu::launch_lu_fp64(a, lda, n, piv, info, work, static_cast<hipStream_t>(stream));
+    if (rc == 2) throw NecError("necgpu_zgetrf: n too large for the panel exchange area");
+    if (rc) necgpu::hip_check(hipGetLastError(), "necgpu_zgetrf");
+  });
+}
+
+long necgpu_check_mfma_f64(const double* a, const double* b, double* d, void* stream) {
+  return guarded([&] {
+    if (necgpu::launch_mfma_probe(a, b, d, static_cast<hipStream_t>(stream)))
+      necgpu::hip_check(hipGetLastError(), "necgpu_check_mfma_f64");
   });
 }
 

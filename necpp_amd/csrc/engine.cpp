@@ -238,9 +238,11 @@ void DeviceSolver::enqueue_factor() {
   const int n = n_;
   piv_.ensure((size_t)2 * n * sizeof(int));
   info_.ensure(sizeof(int));
+  lu_work_.ensure(lu_workspace_bytes(n));
   hip_check(hipMemsetAsync(info_.as<void>(), 0, sizeof(int), stream_), "memset");
-  if (launch_lu_fp64(Z_.as<double>(), ld_, n, piv_.as<int>(), info_.as<int>(), stream_))
-    hip_check(hipGetLastError(), "lu launch");
+  const int rc = launch_lu_fp64(Z_.as<double>(), ld_, n, piv_.as<int>(), info_.as<int>(), lu_work_.as<void>(), stream_);
+  if (rc == 2) throw NecError("LU: matrix order " + std::to_string(n) + " exceeds the panel exchange area");
+  if (rc) hip_check(hipGetLastError(), "lu launch");
 }
 
 void DeviceSolver::enqueue_solve(const void* rhs_dev, hipMemcpyKind kind, const void* rhs_host) {
@@ -306,9 +308,12 @@ void DeviceSolver::factor() {
   hip_check(hipEventRecord(ev_[0], stream_), "event");
   enqueue_factor();
   hip_check(hipEventRecord(ev_[1], stream_), "event");
-  int info = 0;
+  int info = 0, status = 0;
   hip_check(hipMemcpyAsync(&info, info_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
+  hip_check(hipMemcpyAsync(&status, lu_status(lu_work_.as<void>(), n_), sizeof(int), hipMemcpyDeviceToHost, stream_),
+            "status");
   hip_check(hipStreamSynchronize(stream_), "factor");
+  if (status) throw NecError("LU: a panel exchange timed out");
   times_.factor_ms = elapsed(ev_[0], ev_[1]);
   lu_info_ = info;
   factored_ = true;

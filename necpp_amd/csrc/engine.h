@@ -112,7 +112,7 @@ class DeviceSolver {
   StageTimes times_;
 
   DevBuf seg_, ekind_, colptr_, colj_, colabc_, segptr_, segbasis_, segabc_;
-  DevBuf Z_, E_, piv_, info_, x_, abc_, curx_, solve_work_, rhs_, src_, zin_;
+  DevBuf Z_, E_, piv_, info_, lu_work_, x_, abc_, curx_, solve_work_, rhs_, src_, zin_;
   DevBuf trig_, pat_part_, pat_out_, pat_sense_;
 };
 

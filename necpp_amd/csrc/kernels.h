@@ -19,9 +19,16 @@ int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int u
                 hipStream_t stream);
 
 // lu.hip: in-place partial-pivot LU of the row-major complex n x n matrix A
-// (leading dimension lda complex).  piv[k] = row exchanged with row k at step
-// k.  info (device int) receives 1 + the first exactly-zero pivot, or stays 0.
-int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, hipStream_t stream);
+// (leading dimension lda complex).  piv (2n ints): piv[k] = row exchanged
+// with row k at step k, then the composed permutation.  info (device int)
+// receives 1 + the first exactly-zero pivot, or stays 0.  work: device
+// scratch of lu_workspace_bytes(n); its first int after the panel counters
+// (lu_status) is set to 1 if a panel exchange timed out.
+size_t lu_workspace_bytes(int n);
+int* lu_status(void* work, int n);
+int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream);
+// one-wave check of the FP64 MFMA operand / result lane layout (64 a, 64 b in, 256 d out)
+int launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t stream);
 
 // solve.hip: x <- A^-1 x with the LU factors of launch_lu_fp64.  work: device
 // scratch of solve_workspace_bytes(n) bytes; epoch: a value that increases

@@ -5,18 +5,24 @@
 // row-major Z (row i = observation segment), which is the reference's "cm"
 // storage; pivoting exchanges whole rows, which are contiguous here.  Pivot
 // rule: first maximum of |z| = hypot(re, im) in the column, as Eigen's
-// scalar_score_coeff_op.
+// scalar_score_coeff_op; ties go to the lower row.
 //
 // Per panel of NB = 64 columns:
-//   lu_panel   one workgroup (16 waves): column by column pivot search
-//              (fused into the previous column's update), full-row swap,
-//              column scaling by complex division, rank-1 update of the
-//              panel; every wave walks whole rows so accesses are coalesced.
-//   lu_trsm    U12 = L11^-1 A12 with L11 and a 64-column block of A12 in LDS.
-//   zgemm_nt   A22 -= L21 * U12 on FP64 MFMA (v_mfma_f64_16x16x4f64), the
-//              complex product in 4M form; 64x64 complex tile per workgroup,
-//              operands staged in LDS as split re/im planes.
+//   lu_panel   G cooperating workgroups, each holding up to ~112 rows of
+//              the panel in LDS.  Per column: local pivot candidate, one
+//              grid-wide exchange (candidate rows published to global
+//              memory, one atomic arrival counter), row exchange between
+//              the two owners, complex-division scaling and rank-1 update
+//              in LDS.  When G <= 32 the active workgroups are every 8th of
+//              the launch, which places them on one XCD (round-robin
+//              dispatch) so the exchange stays within one L2.
+//   lu_laswp   the panel's row exchanges applied to the columns outside it.
+//   lu_trsm    U12 = L11^-1 A12 with L11 and a 64-column block in LDS.
+//   zgemm      A22 -= L21 * U12 on FP64 MFMA (v_mfma_f64_16x16x4f64), the
+//              complex product in 4M form; 64x64 complex tile per
+//              workgroup, operands staged in LDS as split re/im planes.
 #include <climits>
+#include <cstdlib>
 
 #include "kernels.h"
 #include "nec_math.h"
@@ -26,136 +32,253 @@ namespace necgpu {
 namespace {
 
 constexpr int NB = 64;
-constexpr int PANEL_THREADS = 1024;
+constexpr int PANEL_THREADS = 256;
+constexpr int GMAX = 256;     // most cooperating panel workgroups
+constexpr int RMAX = 112;     // most panel rows per workgroup (LDS: RMAX*(NB+1)*16 B)
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ void better(double& best, int& idx, double ob, int oi) {
-  if (ob > best || (ob == best && oi < idx)) {
-    best = ob;
-    idx = oi;
-  }
+// Exchange area of the cooperative panel (double-buffered by column parity).
+struct PanelXchg {
+  double cand_val[2][GMAX];
+  int cand_idx[2][GMAX];
+  double cand_row[2][GMAX][2 * NB];
+  double diag_row[2][2 * NB];
+};
+
+struct LuWork {
+  unsigned* arrivals;  // one counter per panel
+  PanelXchg* x;
+  int* status;
+};
+
+__device__ __forceinline__ bool better(double ob, int oi, double best, int idx) {
+  return ob > best || (ob == best && oi < idx);
 }
 
-__global__ void init_perm_kernel(int* perm, int n) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+__global__ void lu_init_kernel(int* perm, int n, unsigned* arrivals, int npanels, int* status) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) perm[i] = i;
+  if (i < npanels) arrivals[i] = 0u;
+  if (i == 0) *status = 0;
 }
 
-// Factor columns [k0, k0+nb) over rows [k0, n).  Row exchanges are applied to
-// the whole row (all n columns) and recorded in piv (transpositions) and in
-// perm (the composed permutation, perm[i] = original row now at i).
+// Factor panel columns [k0, k0+nb) over rows [k0, n) with G workgroups; the
+// active workgroup g is launch block g*stride.  Rows owned by g:
+// [k0 + g*R, min(n, k0 + (g+1)*R)).
 __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restrict__ A, long long lda, int n, int k0,
-                                                                  int nb, int* __restrict__ piv, int* __restrict__ perm,
-                                                                  int* __restrict__ info) {
+                                                                  int nb, int G, int R, int stride, int* piv, int* perm,
+                                                                  int* info, unsigned* arrival, PanelXchg* X,
+                                                                  int* status) {
+  extern __shared__ double lds[];
+  if (blockIdx.x % stride != 0) return;
+  const int g = blockIdx.x / stride;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int LDR = NB + 1;  // padded complex row length in LDS
+  double* rows = lds;                                  // [R][LDR] complex
+  double* su = lds + 2 * (size_t)RMAX * LDR;           // pivot row, nb complex
   __shared__ double s_best[PANEL_THREADS / 64];
   __shared__ int s_idx[PANEL_THREADS / 64];
-  __shared__ int s_piv;
-  __shared__ double s_prow[2 * NB];
-  const int t = threadIdx.x;
-  const int lane = t & 63;
-  const int wave = t >> 6;
-  const int nwaves = PANEL_THREADS / 64;
-
-  // pivot candidates of the first panel column
-  double best = -1.0;
-  int bidx = INT_MAX;
-  for (int i = k0 + t; i < n; i += PANEL_THREADS) {
-    const double* v = A + 2 * ((size_t)i * lda + k0);
-    const double h = hypot(v[0], v[1]);
-    if (h > best) {
-      best = h;
-      bidx = i;
-    }
+  __shared__ int s_win[2];  // winner row, winner group
+  const int r0 = k0 + g * R;
+  const int r1 = min(n, r0 + R);
+  const int nrow = max(0, r1 - r0);
+  for (int e = t; e < nrow * nb; e += PANEL_THREADS) {
+    const int r = e / nb, c = e % nb;
+    const double* v = A + 2 * ((size_t)(r0 + r) * lda + k0 + c);
+    rows[2 * (r * LDR + c)] = v[0];
+    rows[2 * (r * LDR + c) + 1] = v[1];
   }
+  __syncthreads();
   for (int c = 0; c < nb; ++c) {
     const int k = k0 + c;
-    // reduce (best, bidx) over the workgroup: larger wins, ties to lower row
+    const int par = c & 1;
+    // 1. local candidate: first maximum of |A[i][k]| over owned rows i >= k
+    double best = -1.0;
+    int bidx = INT_MAX;
+    for (int r = max(0, k - r0) + t; r < nrow; r += PANEL_THREADS) {
+      const double h = hypot(rows[2 * (r * LDR + c)], rows[2 * (r * LDR + c) + 1]);
+      if (h > best) {
+        best = h;
+        bidx = r0 + r;
+      }
+    }
     for (int off = 32; off > 0; off >>= 1) {
       const double ob = __shfl_down(best, off, 64);
       const int oi = __shfl_down(bidx, off, 64);
-      better(best, bidx, ob, oi);
+      if (better(ob, oi, best, bidx)) {
+        best = ob;
+        bidx = oi;
+      }
     }
     if (lane == 0) {
       s_best[wave] = best;
       s_idx[wave] = bidx;
     }
     __syncthreads();
-    if (t == 0) {
-      double b = s_best[0];
-      int p = s_idx[0];
-      for (int w = 1; w < nwaves; ++w) better(b, p, s_best[w], s_idx[w]);
-      if (p == INT_MAX || !(b > 0.0)) {
-        if (b == 0.0 && *info == 0) *info = k + 1;
-        p = k;
-      }
-      piv[k] = p;
-      if (p != k) {
-        const int tmp = perm[k];
-        perm[k] = perm[p];
-        perm[p] = tmp;
-      }
-      s_piv = p;
-    }
-    __syncthreads();
-    const int p = s_piv;
-    if (p != k) {
-      double* rk = A + 2 * (size_t)k * lda;
-      double* rp = A + 2 * (size_t)p * lda;
-      for (int col = t; col < n; col += PANEL_THREADS) {
-        const double ar = rk[2 * col], ai = rk[2 * col + 1];
-        rk[2 * col] = rp[2 * col];
-        rk[2 * col + 1] = rp[2 * col + 1];
-        rp[2 * col] = ar;
-        rp[2 * col + 1] = ai;
-      }
-    }
-    __syncthreads();
-    const int width = nb - c;  // pivot row entries k .. k0+nb-1
-    if (t < width) {
-      const double* v = A + 2 * ((size_t)k * lda + k + t);
-      s_prow[2 * t] = v[0];
-      s_prow[2 * t + 1] = v[1];
-    }
-    __syncthreads();
-    const zc d = zmk(s_prow[0], s_prow[1]);
-    const bool scale = (d.re != 0.0) || (d.im != 0.0);
-    // scale column k and apply the rank-1 update to the rest of the panel;
-    // each wave takes whole rows, lane q <-> panel column k + q.
-    best = -1.0;
-    bidx = INT_MAX;
-    const zc u = (lane < width) ? zmk(s_prow[2 * lane], s_prow[2 * lane + 1]) : zmk(0.0, 0.0);
-    for (int i = k + 1 + wave; i < n; i += nwaves) {
-      double* row = A + 2 * ((size_t)i * lda + k);
-      double vr = 0.0, vi = 0.0;
-      if (lane < width) {
-        vr = row[2 * lane];
-        vi = row[2 * lane + 1];
-      }
-      zc l = zmk(__shfl(vr, 0, 64), __shfl(vi, 0, 64));
-      if (scale) l = zdiv(l, d);
+    // 2. publish (wave 0 writes, fences, and arrives)
+    if (wave == 0) {
+      best = s_best[0];
+      bidx = s_idx[0];
+      for (int w = 1; w < PANEL_THREADS / 64; ++w)
+        if (better(s_best[w], s_idx[w], best, bidx)) {
+          best = s_best[w];
+          bidx = s_idx[w];
+        }
       if (lane == 0) {
-        row[0] = l.re;
-        row[1] = l.im;
-      } else if (lane < width) {
-        const zc prod = zmul(l, u);
-        vr = vr - prod.re;
-        vi = vi - prod.im;
-        row[2 * lane] = vr;
-        row[2 * lane + 1] = vi;
-        if (lane == 1) {
-          const double h = hypot(vr, vi);
-          if (h > best) {
-            best = h;
-            bidx = i;
+        X->cand_val[par][g] = best;
+        X->cand_idx[par][g] = bidx;
+      }
+      if (bidx != INT_MAX) {
+        const int r = bidx - r0;
+        for (int q = lane; q < nb; q += 64) {
+          X->cand_row[par][g][2 * q] = rows[2 * (r * LDR + q)];
+          X->cand_row[par][g][2 * q + 1] = rows[2 * (r * LDR + q) + 1];
+        }
+      }
+      if (k >= r0 && k < r1) {
+        const int r = k - r0;
+        for (int q = lane; q < nb; q += 64) {
+          X->diag_row[par][2 * q] = rows[2 * (r * LDR + q)];
+          X->diag_row[par][2 * q + 1] = rows[2 * (r * LDR + q) + 1];
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      if (lane == 0) {
+        __hip_atomic_fetch_add(arrival, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        // bounded wait; a timeout is sticky so every later wait returns at once
+        const unsigned target = (unsigned)G * (unsigned)(c + 1);
+        long spins = 0;
+        while (__hip_atomic_load(arrival, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
+          if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+          if (++spins > (1L << 22)) {
+            __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
+        }
+      }
+      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+      // 3. winner over all groups (every group computes the same)
+      double wv = -1.0;
+      int wi = INT_MAX, wg = 0;
+      for (int h = lane; h < G; h += 64) {
+        const double v = __hip_atomic_load(&X->cand_val[par][h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const int ix = __hip_atomic_load(&X->cand_idx[par][h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (better(v, ix, wv, wi)) {
+          wv = v;
+          wi = ix;
+          wg = h;
+        }
+      }
+      for (int off = 32; off > 0; off >>= 1) {
+        const double ov = __shfl_down(wv, off, 64);
+        const int oi = __shfl_down(wi, off, 64);
+        const int og = __shfl_down(wg, off, 64);
+        if (better(ov, oi, wv, wi)) {
+          wv = ov;
+          wi = oi;
+          wg = og;
+        }
+      }
+      if (lane == 0) {
+        int p = wi;
+        if (p == INT_MAX || !(wv > 0.0)) {
+          if (wv == 0.0 && g == 0 && *info == 0) *info = k + 1;
+          p = k;
+          wg = -1;
+        }
+        s_win[0] = p;
+        s_win[1] = wg;
+        if (g == 0) {
+          piv[k] = p;
+          if (p != k) {
+            const int tmp = perm[k];
+            perm[k] = perm[p];
+            perm[p] = tmp;
           }
         }
       }
     }
-    // candidates for the next column live in lane 1 of every wave
-    best = __shfl(best, 1, 64);
-    bidx = __shfl(bidx, 1, 64);
     __syncthreads();
+    const int p = s_win[0];
+    const int wg = s_win[1];
+    // 4. the pivot row (whole panel row) and the exchange of rows k and p
+    if (t < nb) {
+      double ur, ui;
+      if (p != k) {
+        ur = X->cand_row[par][wg][2 * t];
+        ui = X->cand_row[par][wg][2 * t + 1];
+      } else if (k >= r0 && k < r1) {
+        ur = rows[2 * ((k - r0) * LDR + t)];
+        ui = rows[2 * ((k - r0) * LDR + t) + 1];
+      } else {
+        ur = X->diag_row[par][2 * t];
+        ui = X->diag_row[par][2 * t + 1];
+      }
+      su[2 * t] = ur;
+      su[2 * t + 1] = ui;
+    }
+    __syncthreads();
+    if (p != k) {
+      if (t < nb && p >= r0 && p < r1) {
+        rows[2 * ((p - r0) * LDR + t)] = X->diag_row[par][2 * t];
+        rows[2 * ((p - r0) * LDR + t) + 1] = X->diag_row[par][2 * t + 1];
+      }
+      if (t < nb && k >= r0 && k < r1) {
+        rows[2 * ((k - r0) * LDR + t)] = su[2 * t];
+        rows[2 * ((k - r0) * LDR + t) + 1] = su[2 * t + 1];
+      }
+    }
+    __syncthreads();
+    // 5. scale column c and update columns c+1.. of owned rows below k;
+    //    wave w takes rows w, w+4, ...; lane q <-> panel column q.
+    const zc d = zmk(su[2 * c], su[2 * c + 1]);
+    const bool scale = (d.re != 0.0) || (d.im != 0.0);
+    const zc u = (lane < nb) ? zmk(su[2 * lane], su[2 * lane + 1]) : zmk(0.0, 0.0);
+    for (int r = max(0, k + 1 - r0) + wave; r < nrow; r += PANEL_THREADS / 64) {
+      double* row = rows + 2 * (size_t)r * LDR;
+      zc l = zmk(row[2 * c], row[2 * c + 1]);
+      if (scale) l = zdiv(l, d);
+      if (lane == c) {
+        row[2 * c] = l.re;
+        row[2 * c + 1] = l.im;
+      } else if (lane > c && lane < nb) {
+        const zc prod = zmul(l, u);
+        row[2 * lane] -= prod.re;
+        row[2 * lane + 1] -= prod.im;
+      }
+    }
+    __syncthreads();
+  }
+  for (int e = t; e < nrow * nb; e += PANEL_THREADS) {
+    const int r = e / nb, c = e % nb;
+    double* v = A + 2 * ((size_t)(r0 + r) * lda + k0 + c);
+    v[0] = rows[2 * (r * LDR + c)];
+    v[1] = rows[2 * (r * LDR + c) + 1];
+  }
+}
+
+// Apply the exchanges piv[k0 .. k0+nb) to columns [0, k0) and [k0+nb, n).
+__global__ void __launch_bounds__(256) lu_laswp_kernel(double* __restrict__ A, long long lda, int n, int k0, int nb,
+                                                       const int* __restrict__ piv) {
+  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  const int outside = n - nb;
+  if (idx >= outside) return;
+  const int col = idx < k0 ? idx : idx + nb;
+  for (int q = 0; q < nb; ++q) {
+    const int k = k0 + q;
+    const int p = piv[k];
+    if (p != k) {
+      double* a = A + 2 * ((size_t)k * lda + col);
+      double* b = A + 2 * ((size_t)p * lda + col);
+      const double ar = a[0], ai = a[1];
+      a[0] = b[0];
+      a[1] = b[1];
+      b[0] = ar;
+      b[1] = ai;
+    }
   }
 }
 
@@ -267,7 +390,8 @@ __global__ void __launch_bounds__(256) zgemm_update_kernel(double* __restrict__ 
   for (int a = 0; a < 2; ++a)
     for (int b = 0; b < 2; ++b)
       for (int v = 0; v < 4; ++v) {
-        const int i = row0 + wr * 32 + a * 16 + lk * 4 + v;
+        // result register v of lane l holds D[(l / 16) + 4 v][l % 16]
+        const int i = row0 + wr * 32 + a * 16 + v * 4 + lk;
         const int j = col0 + wc * 32 + b * 16 + li;
         if (i < M && j < N) {
           double* c = A + 2 * ((size_t)(m0 + i) * lda + n0 + j);
@@ -277,15 +401,82 @@ __global__ void __launch_bounds__(256) zgemm_update_kernel(double* __restrict__ 
       }
 }
 
+// One wave: lane l feeds a[l] as the A operand and b[l] as the B operand of
+// v_mfma_f64_16x16x4f64 and writes its four result registers to d[4*l..].
+__global__ void mfma_probe_kernel(const double* a, const double* b, double* d) {
+  const int l = threadIdx.x;
+  double4_t acc = (double4_t){0.0, 0.0, 0.0, 0.0};
+  acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a[l], b[l], acc, 0, 0, 0);
+  for (int v = 0; v < 4; ++v) d[4 * l + v] = acc[v];
+}
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+LuWork carve(void* work, int n) {
+  const int npanels = (n + NB - 1) / NB;
+  char* p = static_cast<char*>(work);
+  LuWork w;
+  w.arrivals = reinterpret_cast<unsigned*>(p);
+  p += align256((size_t)npanels * sizeof(unsigned));
+  w.status = reinterpret_cast<int*>(p);
+  p += 256;
+  w.x = reinterpret_cast<PanelXchg*>(p);
+  return w;
+}
+
+int panel_stride_override() {
+  const char* v = std::getenv("NECGPU_PANEL_STRIDE");
+  return (v && *v) ? std::atoi(v) : 0;
+}
+
 }  // namespace
 
-int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, hipStream_t stream) {
-  // perm lives right after piv in the caller's int buffer (2*n ints).
+int* lu_status(void* work, int n) { return carve(work, n).status; }
+
+size_t lu_workspace_bytes(int n) {
+  const int npanels = (n + NB - 1) / NB;
+  return align256((size_t)npanels * sizeof(unsigned)) + 256 + align256(sizeof(PanelXchg));
+}
+
+int launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t stream) {
+  hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, stream, a, b, d);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream) {
+  static bool attr_set = false;
+  static int cu_count = 0;
+  const size_t lds_bytes = (size_t)(2 * RMAX * (NB + 1) + 2 * NB) * sizeof(double);
+  if (!attr_set) {
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(lu_panel_kernel),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
+      return 1;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+      return 1;
+    attr_set = true;
+  }
   int* perm = piv + n;
-  hipLaunchKernelGGL(init_perm_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, perm, n);
-  for (int k0 = 0; k0 < n; k0 += NB) {
+  const LuWork w = carve(work, n);
+  const int npanels = (n + NB - 1) / NB;
+  hipLaunchKernelGGL(lu_init_kernel, dim3((std::max(n, npanels) + 255) / 256), dim3(256), 0, stream, perm, n,
+                     w.arrivals, npanels, w.status);
+  const int forced = panel_stride_override();
+  for (int k0 = 0, pi = 0; k0 < n; k0 += NB, ++pi) {
     const int nb = (n - k0 < NB) ? (n - k0) : NB;
-    hipLaunchKernelGGL(lu_panel_kernel, dim3(1), dim3(PANEL_THREADS), 0, stream, A, lda, n, k0, nb, piv, perm, info);
+    const int rows = n - k0;
+    int G = (rows + RMAX - 1) / RMAX;
+    // every active workgroup needs its own CU at the same time (one per CU
+    // by LDS), so G may not exceed the CU count
+    if (G > GMAX || G > cu_count) return 2;
+    const int R = (rows + G - 1) / G;
+    int stride = (G <= 32 && 8 * G <= cu_count) ? 8 : 1;
+    if (forced > 0 && forced * G <= cu_count) stride = forced;
+    hipLaunchKernelGGL(lu_panel_kernel, dim3(G * stride), dim3(PANEL_THREADS), lds_bytes, stream, A, lda, n, k0, nb, G,
+                       R, stride, piv, perm, info, w.arrivals + pi, w.x, w.status);
+    if (n > nb)
+      hipLaunchKernelGGL(lu_laswp_kernel, dim3((n - nb + 255) / 256), dim3(256), 0, stream, A, lda, n, k0, nb, piv);
     const int rest = n - k0 - nb;
     if (rest <= 0) break;
     hipLaunchKernelGGL(lu_trsm_kernel, dim3((rest + 63) / 64), dim3(256), 0, stream, A, lda, n, k0, nb);

@@ -21,8 +21,9 @@ __device__ __forceinline__ unsigned load_flag(const unsigned* f) {
 }
 
 __global__ void permute_kernel(const double* __restrict__ b, const int* __restrict__ perm, double* __restrict__ x,
-                               int n) {
+                               int n, int* status) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *status = 0;
   if (i < n) {
     const int p = perm[i];
     x[2 * i] = b[2 * p];
@@ -58,10 +59,12 @@ __global__ void __launch_bounds__(256) trsv_blocked_kernel(const double* __restr
   for (int step = 0; step < (lower ? b : nblk - 1 - b); ++step) {
     const int c = lower ? step : nblk - 1 - step;
     if (t == 0) {
+      // bounded wait; a timeout is sticky so every later wait returns at once
       long spins = 0;
       while (load_flag(flags + c) != epoch) {
+        if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
         if (++spins > (1L << 22)) {
-          atomicExch(status, 1);
+          __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
           break;
         }
         __builtin_amdgcn_s_sleep(1);
@@ -149,7 +152,7 @@ int launch_lu_solve_fp64(const double* LU, long long lda, int n, const int* piv,
   int* status = reinterpret_cast<int*>(flags_b + nblk);
   double* tmp = reinterpret_cast<double*>(static_cast<char*>(work) + (((size_t)2 * nblk * sizeof(unsigned) + 64 + 255) & ~(size_t)255));
   const int* perm = piv + n;
-  hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, x, perm, tmp, n);
+  hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, x, perm, tmp, n, status);
   hipLaunchKernelGGL(trsv_blocked_kernel<true>, dim3(nblk), dim3(256), 0, stream, LU, lda, n, tmp, flags_f, epoch,
                      status);
   hipLaunchKernelGGL(trsv_blocked_kernel<false>, dim3(nblk), dim3(256), 0, stream, LU, lda, n, tmp, flags_b, epoch,

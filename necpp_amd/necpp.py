@@ -66,7 +66,9 @@ _SIGS = [
     ("nec_gpu_fill_matrix", _L, [_P, _D, _P, _L]),
     ("nec_gpu_pattern_array", _L, [_P, _I, _I, _P, _L]),
     ("nec_gpu_pattern_average", _L, [_P, _I, _P, _P]),
-    ("necgpu_zgetrf", _L, [_P, _LL, _I, _P, _P, _P]),
+    ("necgpu_zgetrf_workspace", _LL, [_I]),
+    ("necgpu_zgetrf", _L, [_P, _LL, _I, _P, _P, _P, _P]),
+    ("necgpu_check_mfma_f64", _L, [_P, _P, _P, _P]),
     ("necgpu_zgetrs_workspace", _LL, [_I]),
     ("necgpu_zgetrs", _L, [_P, _LL, _I, _P, _P, _P, _U, _P]),
 ]

@@ -1023,6 +1023,58 @@ static void ek_ends(const ora_geom* g, int64_t j, int* ind1, int* ind2) {
   }
 }
 
+/* Test hooks (see necpp_oracle.h). */
+static void put_cx(double* out, int k, cx v) { out[2 * k] = v.re; out[2 * k + 1] = v.im; }
+
+void ora_eksc(double s, double z, double rh, int ij, double* out) {
+  ora_fill_counts cnt;
+  fill_state st;
+  memset(&st, 0, sizeof st);
+  st.cnt = &cnt;
+  cx ezs, ers, ezc, erc, ezk, erk;
+  eksc(&st, s, z, rh, TWO_PI, ij, &ezs, &ers, &ezc, &erc, &ezk, &erk);
+  put_cx(out, 0, ezs); put_cx(out, 1, ers); put_cx(out, 2, ezc);
+  put_cx(out, 3, erc); put_cx(out, 4, ezk); put_cx(out, 5, erk);
+}
+
+void ora_ekscx(double b, double s, double z, double rh, int ij, int inx1, int inx2, double* out) {
+  ora_fill_counts cnt;
+  fill_state st;
+  memset(&st, 0, sizeof st);
+  st.cnt = &cnt;
+  cx ezs, ers, ezc, erc, ezk, erk;
+  ekscx(&st, b, s, z, rh, TWO_PI, ij, inx1, inx2, &ezs, &ers, &ezc, &erc, &ezk, &erk);
+  put_cx(out, 0, ezs); put_cx(out, 1, ers); put_cx(out, 2, ezc);
+  put_cx(out, 3, erc); put_cx(out, 4, ezk); put_cx(out, 5, erk);
+}
+
+void ora_intx(double zpk, double rkb2, int ija, double el1, double el2, double b, int ij, double* out) {
+  ora_fill_counts cnt;
+  fill_state st;
+  memset(&st, 0, sizeof st);
+  st.cnt = &cnt;
+  st.zpk = zpk;
+  st.rkb2 = rkb2;
+  st.ija = ija;
+  intx(&st, el1, el2, b, ij, &out[0], &out[1]);
+}
+
+double ora_efld(double xj, double yj, double zj, double cabj, double sabj, double salpj, double s, double b,
+                double xi, double yi, double zi, double ai, int not_self, double rkh, int use_ek, int ind1, int ind2,
+                double* out) {
+  ora_fill_counts cnt;
+  fill_state st;
+  memset(&st, 0, sizeof st);
+  st.cnt = &cnt;
+  st.s = s; st.b = b; st.xj = xj; st.yj = yj; st.zj = zj;
+  st.cabj = cabj; st.sabj = sabj; st.salpj = salpj;
+  st.rkh = rkh; st.use_ek = use_ek; st.ind1 = ind1; st.ind2 = ind2;
+  cx e[9];
+  double r = efld_compute(&st, xi - xj, yi - yj, ai, salpj, zi - zj, not_self, e);
+  for (int k = 0; k < 9; k++) put_cx(out, k, e[k]);
+  return r;
+}
+
 #define TRIO_CAP 64
 
 /* cmset + cmww, src/nec_context.cpp:2077-2203 and :2514-2654, transposed

@@ -111,6 +111,22 @@ int ora_current_coefficients(const ora_geom* g, double* cur, double* abc);
 void ora_ffld(const ora_geom* g, const double* abc, double sth, double cth, double sph, double cph,
               double* eth, double* eph);
 
+/* Test hooks exposing single kernels of the fill (free space, k = 2*pi):
+ * ora_eksc / ora_ekscx: out[12] = ezs, ers, ezc, erc, ezk, erk (re, im) of a
+ * source segment of length s (wavelengths) at axial offset z and radial
+ * distance rh; ij = 0 for the self term.
+ * ora_intx: out[2] = (cint, sint) of intx over [el1, el2] with the gf state
+ * (zpk, rkb2, ija).
+ * ora_efld: out[18] = the 9 complex field components egnd of efld_compute
+ * for a source segment (centre xj.., cosines, length s, radius b) at the
+ * observation point (xi, yi, zi) with observation radius ai. */
+void ora_eksc(double s, double z, double rh, int ij, double* out);
+void ora_ekscx(double b, double s, double z, double rh, int ij, int inx1, int inx2, double* out);
+void ora_intx(double zpk, double rkb2, int ija, double el1, double el2, double b, int ij, double* out);
+double ora_efld(double xj, double yj, double zj, double cabj, double sabj, double salpj, double s, double b,
+                double xi, double yi, double zi, double ai, int not_self, double rkh, int use_ek, int ind1, int ind2,
+                double* out);
+
 /* Pattern specification as accepted by nec_rp_card (ifar must be 0). */
 typedef struct {
   int n_theta, n_phi;

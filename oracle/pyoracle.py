@@ -51,6 +51,11 @@ def lib():
         L.ora_ffld.argtypes = [p, p, d, d, d, d, p, p]
         L.ora_pattern.argtypes = [p, p, d, d, d, p, p, i]
         L.ora_run_voltage.argtypes = [p, d, d, i, i, p, p, p, p, p, p, p, p, i, p]
+        L.ora_eksc.argtypes = [d, d, d, i, p]
+        L.ora_ekscx.argtypes = [d, d, d, d, i, i, i, p]
+        L.ora_intx.argtypes = [d, d, i, d, d, d, i, p]
+        L.ora_efld.argtypes = [d] * 12 + [i, d, i, i, i, p]
+        L.ora_efld.restype = d
         _lib = L
     return _lib
 
@@ -89,6 +94,37 @@ class Timing(ctypes.Structure):
 
 def speed_of_light():
     return lib().ora_speed_of_light()
+
+
+def impedance():
+    return lib().ora_impedance()
+
+
+def eksc(s, z, rh, ij=1):
+    """(ezs, ers, ezc, erc, ezk, erk) of the thin-wire kernel."""
+    out = np.zeros(12)
+    lib().ora_eksc(s, z, rh, ij, _ptr(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def ekscx(b, s, z, rh, ij=1, inx1=2, inx2=2):
+    out = np.zeros(12)
+    lib().ora_ekscx(b, s, z, rh, ij, inx1, inx2, _ptr(out))
+    return out[0::2] + 1j * out[1::2]
+
+
+def intx(zpk, rkb2, ija, el1, el2, b, ij):
+    out = np.zeros(2)
+    lib().ora_intx(zpk, rkb2, ija, el1, el2, b, ij, _ptr(out))
+    return out[0], out[1]
+
+
+def efld(src, obs, ai, not_self=1, rkh=1.0, use_ek=False, ind=(2, 2)):
+    """9 complex field components of efld_compute.  src = (x, y, z, cab, sab,
+    salp, s, b) of the source segment, obs = (x, y, z)."""
+    out = np.zeros(18)
+    r = lib().ora_efld(*src, *obs, ai, not_self, rkh, int(use_ek), ind[0], ind[1], _ptr(out))
+    return out[0::2] + 1j * out[1::2], r
 
 
 def wavelength(f_mhz):

@@ -62,6 +62,32 @@ def _torch():
     return torch
 
 
+def test_mfma_f64_lane_layout():
+    # The LU's trailing update assumes, for v_mfma_f64_16x16x4f64:
+    # A[i][k] from lane i + 16k, B[k][j] from lane j + 16k, and result
+    # register v of lane l holding D[l/16 + 4*v][l % 16].
+    torch = _torch()
+    rng = np.random.default_rng(11)
+    A = rng.standard_normal((16, 4))
+    B = rng.standard_normal((4, 16))
+    lanes = np.arange(64)
+    a = torch.from_numpy(A[lanes % 16, lanes // 16].copy()).to("cuda")
+    b = torch.from_numpy(B[lanes // 16, lanes % 16].copy()).to("cuda")
+    d = torch.zeros(256, dtype=torch.float64, device="cuda")
+    assert api.necgpu_check_mfma_f64(a.data_ptr(), b.data_ptr(), d.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    got = d.cpu().numpy().reshape(64, 4)
+    D = A @ B
+    found = {}
+    for l in range(64):
+        for v in range(4):
+            hit = np.argwhere(np.abs(D - got[l, v]) < 1e-12)
+            found[(l, v)] = tuple(hit[0]) if len(hit) == 1 else None
+    print("MFMA f64 16x16x4 layout (lane, reg) -> (i, j):", {k: found[k] for k in list(found)[:20]})
+    expect = {(l, v): (l // 16 + 4 * v, l % 16) for l in range(64) for v in range(4)}
+    assert found == expect
+
+
 @pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 127, 130, 257, 300])
 def test_zgetrf_zgetrs_random(n):
     torch = _torch()
@@ -71,7 +97,8 @@ def test_zgetrf_zgetrs_random(n):
     dA = torch.from_numpy(A.copy()).to("cuda")
     piv = torch.zeros(2 * n, dtype=torch.int32, device="cuda")
     info = torch.zeros(1, dtype=torch.int32, device="cuda")
-    assert api.necgpu_zgetrf(dA.data_ptr(), n, n, piv.data_ptr(), info.data_ptr(), None) == 0
+    lw = torch.empty(int(api.necgpu_zgetrf_workspace(n)), dtype=torch.uint8, device="cuda")
+    assert api.necgpu_zgetrf(dA.data_ptr(), n, n, piv.data_ptr(), info.data_ptr(), lw.data_ptr(), None) == 0
     torch.cuda.synchronize()
     assert int(info.item()) == 0
     LU = dA.cpu().numpy()
@@ -105,7 +132,8 @@ def test_zgetrf_reports_zero_pivot():
     dA = torch.from_numpy(A).to("cuda")
     piv = torch.zeros(2 * n, dtype=torch.int32, device="cuda")
     info = torch.zeros(1, dtype=torch.int32, device="cuda")
-    assert api.necgpu_zgetrf(dA.data_ptr(), n, n, piv.data_ptr(), info.data_ptr(), None) == 0
+    lw = torch.empty(int(api.necgpu_zgetrf_workspace(n)), dtype=torch.uint8, device="cuda")
+    assert api.necgpu_zgetrf(dA.data_ptr(), n, n, piv.data_ptr(), info.data_ptr(), lw.data_ptr(), None) == 0
     torch.cuda.synchronize()
     assert int(info.item()) == 4  # 1 + first exactly-zero pivot (Eigen's first_zero_pivot)
 
@@ -175,7 +203,8 @@ def test_c1_dipole_gpu(kats):
             finite = b > -999
             assert np.array_equal(a > -999, finite), name
             # dB values: absolute agreement; deep nulls amplify relative field errors
-            assert np.max(np.abs(a[finite] - b[finite])) < 1e-8, name
+            if finite.any():
+                assert np.max(np.abs(a[finite] - b[finite])) < 1e-8, name
         for name in ("e_theta", "e_phi"):
             a = api.gpu_pattern_array(ctx, 0, name, npt)
             b = o["pattern"][name]
