@@ -13,9 +13,7 @@
 //              grid-wide exchange (candidate rows published to global
 //              memory, one atomic arrival counter), row exchange between
 //              the two owners, complex-division scaling and rank-1 update
-//              in LDS.  When G <= 32 the active workgroups are every 8th of
-//              the launch, which places them on one XCD (round-robin
-//              dispatch) so the exchange stays within one L2.
+//              in LDS.
 //   lu_laswp   the panel's row exchanges applied to the columns outside it.
 //   lu_trsm    U12 = L11^-1 A12 with L11 and a 64-column block in LDS.
 //   zgemm      A22 -= L21 * U12 on FP64 MFMA (v_mfma_f64_16x16x4f64), the
@@ -32,7 +30,7 @@ namespace necgpu {
 namespace {
 
 constexpr int NB = 64;
-constexpr int PANEL_THREADS = 256;
+constexpr int PANEL_THREADS = 512;
 constexpr int GMAX = 256;     // most cooperating panel workgroups
 constexpr int RMAX = 112;     // most panel rows per workgroup (LDS: RMAX*(NB+1)*16 B)
 
@@ -52,6 +50,13 @@ struct LuWork {
   int* status;
 };
 
+__device__ __forceinline__ void xstore(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double xload(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ bool better(double ob, int oi, double best, int idx) {
   return ob > best || (ob == best && oi < idx);
 }
@@ -66,20 +71,32 @@ __global__ void lu_init_kernel(int* perm, int n, unsigned* arrivals, int npanels
 // Factor panel columns [k0, k0+nb) over rows [k0, n) with G workgroups; the
 // active workgroup g is launch block g*stride.  Rows owned by g:
 // [k0 + g*R, min(n, k0 + (g+1)*R)).
+//
+// Column c (k = k0 + c):
+//  A  block reduction of the column-c candidates (|z| from step E).
+//  B  wave 0 applies the update still pending from column c-1 (columns
+//     c+1..) to the candidate row and to row k, publishes both with the
+//     candidate and runs the exchange; the other waves meanwhile apply that
+//     pending update to every other owned row, which takes the bulk of the
+//     rank-1 work off the exchange's critical path.
+//  D  the pivot row is fetched; rows k and p are exchanged.
+//  E  one thread per row: scale column c, update column c+1 only, and form
+//     the column-(c+1) candidate; the update of columns c+2.. is left to B.
 __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restrict__ A, long long lda, int n, int k0,
                                                                   int nb, int G, int R, int stride, int* piv, int* perm,
                                                                   int* info, unsigned* arrival, PanelXchg* X,
                                                                   int* status) {
   extern __shared__ double lds[];
   if (blockIdx.x % stride != 0) return;
+  constexpr int NW = PANEL_THREADS / 64;  // waves
   const int g = blockIdx.x / stride;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int LDR = NB + 1;  // padded complex row length in LDS
-  double* rows = lds;                                  // [R][LDR] complex
-  double* su = lds + 2 * (size_t)RMAX * LDR;           // pivot row, nb complex
-  __shared__ double s_best[PANEL_THREADS / 64];
-  __shared__ int s_idx[PANEL_THREADS / 64];
+  double* rows = lds;                         // [R][LDR] complex
+  double* su = lds + 2 * (size_t)RMAX * LDR;  // current pivot row, nb complex
+  __shared__ double s_best[NW];
+  __shared__ int s_idx[NW];
   __shared__ int s_win[2];  // winner row, winner group
   const int r0 = k0 + g * R;
   const int r1 = min(n, r0 + R);
@@ -91,19 +108,20 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
     rows[2 * (r * LDR + c) + 1] = v[1];
   }
   __syncthreads();
+  // candidates of column 0
+  double best = -1.0;
+  int bidx = INT_MAX;
+  for (int r = t; r < nrow; r += PANEL_THREADS) {
+    const double h = hypot(rows[2 * (r * LDR)], rows[2 * (r * LDR) + 1]);
+    if (h > best) {
+      best = h;
+      bidx = r0 + r;
+    }
+  }
   for (int c = 0; c < nb; ++c) {
     const int k = k0 + c;
     const int par = c & 1;
-    // 1. local candidate: first maximum of |A[i][k]| over owned rows i >= k
-    double best = -1.0;
-    int bidx = INT_MAX;
-    for (int r = max(0, k - r0) + t; r < nrow; r += PANEL_THREADS) {
-      const double h = hypot(rows[2 * (r * LDR + c)], rows[2 * (r * LDR + c) + 1]);
-      if (h > best) {
-        best = h;
-        bidx = r0 + r;
-      }
-    }
+    // A. block reduction of the candidates
     for (int off = 32; off > 0; off >>= 1) {
       const double ob = __shfl_down(best, off, 64);
       const int oi = __shfl_down(bidx, off, 64);
@@ -117,36 +135,51 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
       s_idx[wave] = bidx;
     }
     __syncthreads();
-    // 2. publish (wave 0 writes, fences, and arrives)
+    best = s_best[0];
+    bidx = s_idx[0];
+    for (int w = 1; w < NW; ++w)
+      if (better(s_best[w], s_idx[w], best, bidx)) {
+        best = s_best[w];
+        bidx = s_idx[w];
+      }
+    const int rcand = (bidx != INT_MAX) ? bidx - r0 : -1;  // local candidate row
+    const int rk = (k >= r0 && k < r1) ? k - r0 : -1;       // local row k
+    // columns c+1.. still carry the pending update of column c-1 with
+    // multipliers rows[r][c-1] (rows below k-1, i.e. local rows >= k - r0)
+    const int pend_lo = c + 1;
+    const bool pending = (c > 0) && (pend_lo < nb);
     if (wave == 0) {
-      best = s_best[0];
-      bidx = s_idx[0];
-      for (int w = 1; w < PANEL_THREADS / 64; ++w)
-        if (better(s_best[w], s_idx[w], best, bidx)) {
-          best = s_best[w];
-          bidx = s_idx[w];
+      // B0. candidate row and row k brought up to date, then published
+      if (pending && lane >= pend_lo && lane < nb) {
+        const zc u = zmk(su[2 * lane], su[2 * lane + 1]);
+        for (int which = 0; which < 2; ++which) {
+          const int r = which == 0 ? rcand : rk;
+          if (r < 0 || (which == 1 && r == rcand)) continue;
+          const zc l = zmk(rows[2 * (r * LDR + c - 1)], rows[2 * (r * LDR + c - 1) + 1]);
+          const zc prod = zmul(l, u);
+          rows[2 * (r * LDR + lane)] -= prod.re;
+          rows[2 * (r * LDR + lane) + 1] -= prod.im;
         }
+      }
+      // The exchange area is only touched with agent-scope atomics, which
+      // the hardware performs at the device coherence point, so no cache
+      // write-back / invalidate fences are needed: the data stores are
+      // complete (s_waitcnt vmcnt(0)) before the arrival is counted.
       if (lane == 0) {
-        X->cand_val[par][g] = best;
-        X->cand_idx[par][g] = bidx;
+        xstore(&X->cand_val[par][g], best);
+        __hip_atomic_store(&X->cand_idx[par][g], bidx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       }
-      if (bidx != INT_MAX) {
-        const int r = bidx - r0;
-        for (int q = lane; q < nb; q += 64) {
-          X->cand_row[par][g][2 * q] = rows[2 * (r * LDR + q)];
-          X->cand_row[par][g][2 * q + 1] = rows[2 * (r * LDR + q) + 1];
-        }
+      if (rcand >= 0 && lane < nb) {
+        xstore(&X->cand_row[par][g][2 * lane], rows[2 * (rcand * LDR + lane)]);
+        xstore(&X->cand_row[par][g][2 * lane + 1], rows[2 * (rcand * LDR + lane) + 1]);
       }
-      if (k >= r0 && k < r1) {
-        const int r = k - r0;
-        for (int q = lane; q < nb; q += 64) {
-          X->diag_row[par][2 * q] = rows[2 * (r * LDR + q)];
-          X->diag_row[par][2 * q + 1] = rows[2 * (r * LDR + q) + 1];
-        }
+      if (rk >= 0 && lane < nb) {
+        xstore(&X->diag_row[par][2 * lane], rows[2 * (rk * LDR + lane)]);
+        xstore(&X->diag_row[par][2 * lane + 1], rows[2 * (rk * LDR + lane) + 1]);
       }
-      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       if (lane == 0) {
-        __hip_atomic_fetch_add(arrival, 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_fetch_add(arrival, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         // bounded wait; a timeout is sticky so every later wait returns at once
         const unsigned target = (unsigned)G * (unsigned)(c + 1);
         long spins = 0;
@@ -159,12 +192,11 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
           __builtin_amdgcn_s_sleep(1);
         }
       }
-      __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-      // 3. winner over all groups (every group computes the same)
+      // winner over all groups (every group computes the same)
       double wv = -1.0;
       int wi = INT_MAX, wg = 0;
       for (int h = lane; h < G; h += 64) {
-        const double v = __hip_atomic_load(&X->cand_val[par][h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const double v = xload(&X->cand_val[par][h]);
         const int ix = __hip_atomic_load(&X->cand_idx[par][h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         if (better(v, ix, wv, wi)) {
           wv = v;
@@ -200,54 +232,106 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
           }
         }
       }
+    } else if (pending) {
+      // B1. the pending update for every other owned row below k - 1
+      const int w = nb - pend_lo;
+      const zc u = (lane < w) ? zmk(su[2 * (pend_lo + lane)], su[2 * (pend_lo + lane) + 1]) : zmk(0.0, 0.0);
+      const int rstart = max(0, k - r0);
+      for (int r = rstart + (wave - 1); r < nrow; r += 4 * (NW - 1)) {
+        double lr[4], li[4], ar[4], ai[4];
+        bool on[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rr = r + q * (NW - 1);
+          on[q] = rr < nrow && rr != rcand && rr != rk && lane < w;
+          lr[q] = li[q] = ar[q] = ai[q] = 0.0;
+          if (on[q]) {
+            lr[q] = rows[2 * (rr * LDR + c - 1)];
+            li[q] = rows[2 * (rr * LDR + c - 1) + 1];
+            ar[q] = rows[2 * (rr * LDR + pend_lo + lane)];
+            ai[q] = rows[2 * (rr * LDR + pend_lo + lane) + 1];
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+          const int rr = r + q * (NW - 1);
+          if (on[q]) {
+            const zc prod = zmul(zmk(lr[q], li[q]), u);
+            rows[2 * (rr * LDR + pend_lo + lane)] = ar[q] - prod.re;
+            rows[2 * (rr * LDR + pend_lo + lane) + 1] = ai[q] - prod.im;
+          }
+        }
+      }
     }
     __syncthreads();
     const int p = s_win[0];
     const int wg = s_win[1];
-    // 4. the pivot row (whole panel row) and the exchange of rows k and p
+    // D. the pivot row (whole panel row) and the exchange of rows k and p
     if (t < nb) {
       double ur, ui;
       if (p != k) {
-        ur = X->cand_row[par][wg][2 * t];
-        ui = X->cand_row[par][wg][2 * t + 1];
-      } else if (k >= r0 && k < r1) {
-        ur = rows[2 * ((k - r0) * LDR + t)];
-        ui = rows[2 * ((k - r0) * LDR + t) + 1];
+        ur = xload(&X->cand_row[par][wg][2 * t]);
+        ui = xload(&X->cand_row[par][wg][2 * t + 1]);
+      } else if (rk >= 0) {
+        ur = rows[2 * (rk * LDR + t)];
+        ui = rows[2 * (rk * LDR + t) + 1];
       } else {
-        ur = X->diag_row[par][2 * t];
-        ui = X->diag_row[par][2 * t + 1];
+        ur = xload(&X->diag_row[par][2 * t]);
+        ui = xload(&X->diag_row[par][2 * t + 1]);
       }
       su[2 * t] = ur;
       su[2 * t + 1] = ui;
     }
     __syncthreads();
-    if (p != k) {
-      if (t < nb && p >= r0 && p < r1) {
-        rows[2 * ((p - r0) * LDR + t)] = X->diag_row[par][2 * t];
-        rows[2 * ((p - r0) * LDR + t) + 1] = X->diag_row[par][2 * t + 1];
+    if (p != k && t < nb) {
+      if (p >= r0 && p < r1) {
+        rows[2 * ((p - r0) * LDR + t)] = xload(&X->diag_row[par][2 * t]);
+        rows[2 * ((p - r0) * LDR + t) + 1] = xload(&X->diag_row[par][2 * t + 1]);
       }
-      if (t < nb && k >= r0 && k < r1) {
-        rows[2 * ((k - r0) * LDR + t)] = su[2 * t];
-        rows[2 * ((k - r0) * LDR + t) + 1] = su[2 * t + 1];
+      if (rk >= 0) {
+        rows[2 * (rk * LDR + t)] = su[2 * t];
+        rows[2 * (rk * LDR + t) + 1] = su[2 * t + 1];
       }
     }
     __syncthreads();
-    // 5. scale column c and update columns c+1.. of owned rows below k;
-    //    wave w takes rows w, w+4, ...; lane q <-> panel column q.
+    // E. one thread per row below k: scale column c (Smith's division as
+    //    g++'s complex divide, with the pivot's ratio and denominator formed
+    //    once), update column c+1, and keep the column-(c+1) candidate.
     const zc d = zmk(su[2 * c], su[2 * c + 1]);
     const bool scale = (d.re != 0.0) || (d.im != 0.0);
-    const zc u = (lane < nb) ? zmk(su[2 * lane], su[2 * lane + 1]) : zmk(0.0, 0.0);
-    for (int r = max(0, k + 1 - r0) + wave; r < nrow; r += PANEL_THREADS / 64) {
-      double* row = rows + 2 * (size_t)r * LDR;
-      zc l = zmk(row[2 * c], row[2 * c + 1]);
-      if (scale) l = zdiv(l, d);
-      if (lane == c) {
-        row[2 * c] = l.re;
-        row[2 * c + 1] = l.im;
-      } else if (lane > c && lane < nb) {
-        const zc prod = zmul(l, u);
-        row[2 * lane] -= prod.re;
-        row[2 * lane + 1] -= prod.im;
+    const bool swp = fabs(d.re) < fabs(d.im);
+    const double ratio = swp ? d.re / d.im : d.im / d.re;
+    const double denom = swp ? (d.re * ratio) + d.im : (d.im * ratio) + d.re;
+    const bool next = c + 1 < nb;
+    const zc u1 = next ? zmk(su[2 * (c + 1)], su[2 * (c + 1) + 1]) : zmk(0.0, 0.0);
+    best = -1.0;
+    bidx = INT_MAX;
+    for (int r = max(0, k + 1 - r0) + t; r < nrow; r += PANEL_THREADS) {
+      double* v = rows + 2 * (r * LDR + c);
+      double lr = v[0], li = v[1];
+      if (scale) {
+        const double ar = lr, ai = li;
+        if (swp) {
+          lr = ((ar * ratio) + ai) / denom;
+          li = ((ai * ratio) - ar) / denom;
+        } else {
+          lr = ((ai * ratio) + ar) / denom;
+          li = (ai - (ar * ratio)) / denom;
+        }
+        v[0] = lr;
+        v[1] = li;
+      }
+      if (next) {
+        const zc prod = zmul(zmk(lr, li), u1);
+        const double nr = v[2] - prod.re;
+        const double ni = v[3] - prod.im;
+        v[2] = nr;
+        v[3] = ni;
+        const double h = hypot(nr, ni);
+        if (h > best) {
+          best = h;
+          bidx = r0 + r;
+        }
       }
     }
     __syncthreads();
@@ -471,7 +555,9 @@ int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* w
     // by LDS), so G may not exceed the CU count
     if (G > GMAX || G > cu_count) return 2;
     const int R = (rows + G - 1) / G;
-    int stride = (G <= 32 && 8 * G <= cu_count) ? 8 : 1;
+    // Packing the workgroups onto one XCD (stride 8) measured slower than
+    // spreading them, so the default is 1; NECGPU_PANEL_STRIDE overrides.
+    int stride = 1;
     if (forced > 0 && forced * G <= cu_count) stride = forced;
     hipLaunchKernelGGL(lu_panel_kernel, dim3(G * stride), dim3(PANEL_THREADS), lds_bytes, stream, A, lda, n, k0, nb, G,
                        R, stride, piv, perm, info, w.arrivals + pi, w.x, w.status);
