@@ -21,6 +21,13 @@ sys.path.insert(0, ROOT)
 METRIC = "solves/s (fill+LU+RP) at N=3,280 FP64; mixed-vs-FP64 speedup @1e-10 rel Zin"
 PEAK_FP64_TFLOPS = 78.6
 PEAK_HBM_GBS = 8000.0
+NB = 64  # LU panel width (lu.hip)
+
+# rocprofv3 kernel names of the kernel families nec_gpu_kernel_times reports
+ROCPROF_NAME = {"lu_panel": "lu_panel_kernel", "zgemm": "zgemm_update_kernel", "lu_trsm": "lu_trsm_kernel",
+                "lu_laswp": "lu_laswp_kernel", "fill_fields": "fill_fields_kernel",
+                "fill_assemble": "fill_assemble_kernel", "pattern_partial": "pattern_partial_kernel",
+                "trsv_lower": "trsv_blocked_kernel<true>", "trsv_upper": "trsv_blocked_kernel<false>"}
 
 
 def parse():
@@ -30,7 +37,100 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--nn", type=int, default=41, help="grid nodes per side (41 -> N=3,280)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--summarize", metavar="DIR", help="print a rocprofv3 --stats CSV summary and exit")
+    p.add_argument("--summarize-pmc", nargs=2, metavar=("DIR", "COUNTER"),
+                   help="print per-kernel means of one rocprofv3 --pmc counter and exit")
+    p.add_argument("--copy-probe", type=int, metavar="MIB", help="run only a plain MIB-MiB device copy and exit")
     return p.parse_args()
+
+
+def lu_work(n):
+    """Algorithmic work of one factorisation, per kernel family.
+
+    Complex multiply-add = 8 real flops, complex scaling by the pivot = 6;
+    row exchanges move 16-byte entries (read + write)."""
+    panel = gemm = trsm = 0.0
+    laswp_bytes = 0.0
+    for k0 in range(0, n, NB):
+        nb = min(NB, n - k0)
+        m = n - k0
+        for c in range(nb):
+            below = m - c - 1
+            panel += 8.0 * below * (nb - c - 1) + 6.0 * below
+        rest = n - k0 - nb
+        laswp_bytes += 2.0 * 2.0 * 16.0 * nb * (n - nb)
+        if rest > 0:
+            trsm += 8.0 * rest * nb * (nb - 1) / 2.0
+            gemm += 8.0 * rest * rest * nb
+    return {"lu_panel": ("flop", panel), "zgemm": ("flop", gemm), "lu_trsm": ("flop", trsm),
+            "lu_laswp": ("byte", laswp_bytes)}
+
+
+def read_traffic(kernel, workload_tag):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, or None."""
+    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_tag}_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    for line in open(path):
+        if line.startswith("traffic_bytes_per_launch="):
+            return float(line.split("=", 1)[1])
+    return None
+
+
+def short_name(name):
+    """Kernel name without namespaces, return type and parameter list."""
+    name = name.replace("(anonymous namespace)::", "").replace("necgpu::", "")
+    if name.startswith("void "):
+        name = name[5:]
+    return name.split("(")[0]
+
+
+def summarize(prof_dir):
+    """Print the per-kernel statistics of a rocprofv3 --kernel-trace --stats -f csv run."""
+    import csv
+    import glob
+    stats = sorted(glob.glob(os.path.join(prof_dir, "**", "*kernel_stats.csv"), recursive=True))
+    if not stats:
+        raise SystemExit(f"no *kernel_stats.csv under {prof_dir}")
+    rows = list(csv.DictReader(open(stats[0])))
+    print(f"{'kernel':44s} {'calls':>7s} {'total ms':>10s} {'avg us':>10s} {'%':>6s}")
+    for r in rows:
+        name = short_name(r["Name"])
+        print(f"{name[:44]:44s} {int(r['Calls']):7d} {float(r['TotalDurationNs']) / 1e6:10.3f} "
+              f"{float(r['AverageNs']) / 1e3:10.2f} {float(r['Percentage']):6.2f}")
+
+
+def summarize_pmc(prof_dir, counter):
+    """Per-kernel mean of one rocprofv3 --pmc counter (CSV output), in the counter's unit."""
+    import csv
+    import glob
+    files = sorted(glob.glob(os.path.join(prof_dir, "**", "*counter_collection.csv"), recursive=True))
+    if not files:
+        raise SystemExit(f"no *counter_collection.csv under {prof_dir}")
+    acc = {}
+    for r in csv.DictReader(open(files[0])):
+        if r.get("Counter_Name") != counter:
+            continue
+        name = short_name(r["Kernel_Name"])
+        key = (name, r.get("Dispatch_Id", r.get("Correlation_Id")))
+        acc[key] = acc.get(key, 0.0) + float(r["Counter_Value"])
+    per = {}
+    for (name, _), v in acc.items():
+        per.setdefault(name, []).append(v)
+    print(f"{'kernel':44s} {'dispatches':>10s} {counter + ' mean':>16s}")
+    for name, v in sorted(per.items(), key=lambda kv: -sum(kv[1])):
+        print(f"{name[:44]:44s} {len(v):10d} {sum(v) / len(v):16.1f}")
+
+
+def copy_probe(mib):
+    """A plain device-to-device copy of known size (the --pmc scaling reference)."""
+    import torch
+    a = torch.ones(mib * 1024 * 1024 // 8, dtype=torch.float64, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": b.numel() * 8}))
 
 
 def cpu_baseline(deck):
@@ -48,6 +148,15 @@ def cpu_baseline(deck):
 
 def main():
     a = parse()
+    if a.summarize:
+        summarize(a.summarize)
+        return
+    if a.summarize_pmc:
+        summarize_pmc(*a.summarize_pmc)
+        return
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     if a.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported (see DESIGN.md "
                                    "for how the solve would shard)", "n_gpus": a.gpus}))
@@ -67,10 +176,43 @@ def main():
     ms = float(out[0])
     fill_ms, lu_ms, solve_ms, pat_ms = (float(v) for v in out[1:5])
     z_bench = complex(out[5], out[6])
+    ktimes = api.gpu_kernel_times(ctx)
     api.nec_delete(ctx)
 
     lu_flops = 8.0 * n ** 3 / 3.0
     lu_tf = lu_flops / (lu_ms * 1e-3) / 1e12
+    work = lu_work(n)
+    kernels = {}
+    for name, (launches, total) in ktimes.items():
+        if launches == 0:
+            continue
+        e = {"launches_per_step": launches / a.steps, "ms_per_step": total / a.steps,
+             "avg_us": 1000.0 * total / launches}
+        if name in work:
+            kind, amount = work[name]
+            per_launch = amount / (launches / a.steps)
+            if kind == "flop":
+                e["achieved_tflops"] = per_launch / (total / launches * 1e-3) / 1e12
+            else:
+                e["achieved_gbs"] = per_launch / (total / launches * 1e-3) / 1e9
+        kernels[name] = e
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
+    dk = kernels[dom]
+    workload_tag = f"grid{a.nn}"
+    if dom in work and work[dom][0] == "flop":
+        achieved = dk["achieved_tflops"]
+        roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "frac": achieved / PEAK_FP64_TFLOPS}
+    elif dom in work:
+        achieved = dk["achieved_gbs"]
+        roof = {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                "frac": achieved / PEAK_HBM_GBS}
+    else:
+        roof = {"bound": None, "achieved": None, "peak": None, "unit": None, "frac": None}
+    roof["traffic"] = read_traffic(dom, workload_tag)
+    roof["kernel"] = ROCPROF_NAME.get(dom, dom)
+    roof["avg_launch_us"] = dk["avg_us"]
+    roof["launches_per_step"] = dk["launches_per_step"]
     result = {
         "metric": METRIC,
         "value": 1000.0 / ms,
@@ -89,9 +231,11 @@ def main():
         "stages_ms": {"fill": fill_ms, "factor": lu_ms, "solve": solve_ms, "pattern": pat_ms},
         "zin": [z_bench.real, z_bench.imag],
         "zin_api_rel_diff": abs(z_bench - z_api) / abs(z_api),
-        "roofline": {"bound": "mfma", "achieved": lu_tf, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": lu_tf / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "LU stage (8N^3/3 flops per factorisation)"},
+        "roofline": roof,
+        "lu_stage": {"achieved_tflops": lu_tf, "frac_fp64_peak": lu_tf / PEAK_FP64_TFLOPS,
+                     "flops": lu_flops},
+        "kernels": kernels,
+        "mixed_vs_fp64": None,
     }
     if not a.no_cpu:
         cb, r = cpu_baseline(deck)

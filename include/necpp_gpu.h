@@ -49,6 +49,15 @@ long nec_gpu_stage_times(nec_context* ctx, double* out);
  * out[6] = Z_in of the first feed from the last repetition. */
 long nec_gpu_benchmark_solve(nec_context* ctx, int warmup, int steps, double* out);
 
+/* Per-kernel times of the timed steps of the last nec_gpu_benchmark_solve:
+ * every launch there carries its own start/stop HIP events
+ * (hipExtLaunchKernelGGL) on the engine's stream.  For kernel family k
+ * (0 <= k < the return value): out[2k] = launches, out[2k+1] = summed
+ * milliseconds.  n = length of out in doubles.  Returns the number of
+ * families, or -1 on error.  nec_gpu_kernel_name(k) names family k. */
+long nec_gpu_kernel_times(nec_context* ctx, double* out, long n);
+const char* nec_gpu_kernel_name(int kind);
+
 /* Impedance matrix of the completed geometry at freq_mhz (FP64 fill only,
  * no factorisation): out receives 2*n*n doubles, Z[i][k] (observation
  * segment i, basis k) at out[2*(i*n+k)], i.e. nec2++'s "cm" layout.  Leaves

@@ -269,6 +269,22 @@ long nec_gpu_stage_times(nec_context* ctx, double* out) {
   });
 }
 
+long nec_gpu_kernel_times(nec_context* ctx, double* out, long n) {
+  long rc = -1;
+  const long ok = guarded([&] {
+    if (n < 2L * necgpu::K_NKINDS) throw NecError("nec_gpu_kernel_times: out too short");
+    const necgpu::KernelClock& c = ctx->device().kernel_clock();
+    for (int k = 0; k < necgpu::K_NKINDS; ++k) {
+      out[2 * k] = (double)c.launches[k];
+      out[2 * k + 1] = c.total_ms[k];
+    }
+    rc = necgpu::K_NKINDS;
+  });
+  return ok == 0 ? rc : -1;
+}
+
+const char* nec_gpu_kernel_name(int kind) { return necgpu::kernel_kind_name(kind); }
+
 long nec_gpu_benchmark_solve(nec_context* ctx, int warmup, int steps, double* out) {
   return guarded([&] {
     if (warmup < 0 || steps < 1) throw NecError("benchmark: need warmup >= 0 and steps >= 1");
@@ -330,7 +346,7 @@ long long necgpu_zgetrf_workspace(int n) { return static_cast<long long>(necgpu:
 long necgpu_zgetrf(double* a, long long lda, int n, int* piv, int* info, void* work, void* stream) {
   return guarded([&] {
     const int rc = necgpu::launch_lu_fp64(a, lda, n, piv, info, work, static_cast<hipStream_t>(stream));
-    if (rc == 2) throw NecError("necgpu_zgetrf: n too large for the panel exchange area");
+    if (rc == 2) throw NecError("necgpu_zgetrf: n too large (the panel needs one CU per 112 rows, at most 256)");
     if (rc) necgpu::hip_check(hipGetLastError(), "necgpu_zgetrf");
   });
 }

@@ -70,13 +70,13 @@ __global__ void zin_kernel(int nsrc, const int* __restrict__ seg, const double* 
 int launch_zin(int nsrc, const int* seg, const double* volt, const double* curx, double wavelength, double* out,
                hipStream_t stream) {
   if (nsrc <= 0) return 0;
-  hipLaunchKernelGGL(zin_kernel, dim3((nsrc + 63) / 64), dim3(64), 0, stream, nsrc, seg, volt, curx, wavelength, out);
+  launch_k(K_ZIN, zin_kernel, dim3((nsrc + 63) / 64), dim3(64), 0, stream, nsrc, seg, volt, curx, wavelength, out);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 int launch_currents(int n, const double* x, const int* segptr, const int* segbasis, const double* segabc, double* abc,
                     double* curx, hipStream_t stream) {
-  hipLaunchKernelGGL(currents_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, x, segptr, segbasis, segabc, abc,
+  launch_k(K_CURRENTS, currents_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, n, x, segptr, segbasis, segabc, abc,
                      curx);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }

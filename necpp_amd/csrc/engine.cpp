@@ -403,33 +403,54 @@ void DeviceSolver::benchmark_pipeline(double rkh, const std::vector<zc>& e, cons
     upload(trig_, trig, stream_);
     po = pattern_buffers(spec->n_theta * spec->n_phi);
   }
-  StageTimes acc;
-  hipEvent_t t0 = ev_[4], t1 = ev_[5];
-  for (int it = 0; it < warmup + steps; ++it) {
-    const bool timed = it >= warmup;
-    if (it == warmup) hip_check(hipEventRecord(t0, stream_), "event");
-    if (timed) hip_check(hipEventRecord(ev_[0], stream_), "event");
+  // Stage boundaries get their own events per timed step and every kernel
+  // launch carries a start/stop pair (KernelClock), so nothing waits on the
+  // host between steps.  The warmup steps run with the clock installed too,
+  // so that the timed steps find all the events they need already created.
+  struct ClockGuard {
+    explicit ClockGuard(KernelClock* c) { g_kernel_clock = c; }
+    ~ClockGuard() { g_kernel_clock = nullptr; }
+  } guard(&kclock_);
+  kclock_.reset();
+  auto one_step = [&](hipEvent_t* stage) {
+    if (stage) hip_check(hipEventRecord(stage[0], stream_), "event");
     enqueue_fill(rkh);
-    if (timed) hip_check(hipEventRecord(ev_[1], stream_), "event");
+    if (stage) hip_check(hipEventRecord(stage[1], stream_), "event");
     enqueue_factor();
-    if (timed) hip_check(hipEventRecord(ev_[2], stream_), "event");
+    if (stage) hip_check(hipEventRecord(stage[2], stream_), "event");
     enqueue_solve(rhs_.as<void>(), hipMemcpyDeviceToDevice, nullptr);
     if (launch_zin(nsrc, dseg, dvolt, curx_.as<double>(), wavelength, zin_.as<double>(), stream_))
       hip_check(hipGetLastError(), "zin launch");
-    if (timed) hip_check(hipEventRecord(ev_[3], stream_), "event");
+    if (stage) hip_check(hipEventRecord(stage[3], stream_), "event");
     if (spec) enqueue_pattern(*spec, po);
-    if (timed) {
-      // per-stage times of this step (the events are reused, so wait here;
-      // the wait sits outside the stage intervals)
-      hip_check(hipEventSynchronize(ev_[3]), "event");
-      acc.fill_ms += elapsed(ev_[0], ev_[1]);
-      acc.factor_ms += elapsed(ev_[1], ev_[2]);
-      acc.solve_ms += elapsed(ev_[2], ev_[3]);
+  };
+  for (int it = 0; it < warmup; ++it) one_step(nullptr);
+  hip_check(hipStreamSynchronize(stream_), "warmup");
+  const size_t per_step = kclock_.pending() / (size_t)std::max(1, warmup);
+  kclock_.reset();
+  if (!kclock_.reserve(per_step * (size_t)steps)) throw NecError("cannot create kernel timing events");
+  std::vector<hipEvent_t> stage(4 * (size_t)std::max(0, steps));
+  for (auto& e : stage) hip_check(hipEventCreate(&e), "hipEventCreate");
+  struct EventsGuard {
+    std::vector<hipEvent_t>& v;
+    ~EventsGuard() {
+      for (hipEvent_t e : v) (void)hipEventDestroy(e);
     }
-  }
+  } events_guard{stage};
+  hipEvent_t t0 = ev_[4], t1 = ev_[5];
+  hip_check(hipEventRecord(t0, stream_), "event");
+  for (int it = 0; it < steps; ++it) one_step(&stage[4 * (size_t)it]);
   hip_check(hipEventRecord(t1, stream_), "event");
   hip_check(hipEventSynchronize(t1), "event");
   const double total = elapsed(t0, t1);
+  StageTimes acc;
+  for (int it = 0; it < steps; ++it) {
+    const hipEvent_t* s = &stage[4 * (size_t)it];
+    acc.fill_ms += elapsed(s[0], s[1]);
+    acc.factor_ms += elapsed(s[1], s[2]);
+    acc.solve_ms += elapsed(s[2], s[3]);
+  }
+  if (kclock_.collect() != 0) throw NecError("kernel timing events failed");
   *ms_per_solve = steps > 0 ? total / steps : 0.0;
   if (stage_ms) {
     stage_ms->fill_ms = acc.fill_ms / std::max(1, steps);

@@ -89,6 +89,8 @@ class DeviceSolver {
   int n() const { return n_; }
   int lu_info() const { return lu_info_; }
   const StageTimes& times() const { return times_; }
+  // per-kernel launch times of the timed steps of the last benchmark_pipeline
+  const KernelClock& kernel_clock() const { return kclock_; }
   hipStream_t stream() const { return stream_; }
 
  private:
@@ -110,6 +112,7 @@ class DeviceSolver {
   hipStream_t stream_ = nullptr;
   hipEvent_t ev_[6] = {};
   StageTimes times_;
+  KernelClock kclock_;
 
   DevBuf seg_, ekind_, colptr_, colj_, colabc_, segptr_, segbasis_, segabc_;
   DevBuf Z_, E_, piv_, info_, lu_work_, x_, abc_, curx_, solve_work_, rhs_, src_, zin_;

@@ -474,8 +474,8 @@ int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int u
   if (n <= 0 || rows <= 0) return 0;
   dim3 block(256);
   dim3 grid((n + 255) / 256, rows);
-  hipLaunchKernelGGL(fill_fields_kernel, grid, block, 0, stream, g, n, i0, rows, rkh, use_ek, ekind, E);
-  hipLaunchKernelGGL(fill_assemble_kernel, grid, block, 0, stream, E, n, i0, rows, colptr, colj, colabc, Z, ldz);
+  launch_k(K_FILL_FIELDS, fill_fields_kernel, grid, block, 0, stream, g, n, i0, rows, rkh, use_ek, ekind, E);
+  launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel, grid, block, 0, stream, E, n, i0, rows, colptr, colj, colabc, Z, ldz);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

@@ -1,11 +1,69 @@
 // kernels.h -- launch interface of the HIP kernels (host-callable).
 #pragma once
 
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <vector>
 
 namespace necgpu {
+
+// ------------------------------------------------------------ kernel clock
+// Kernel families whose launches can be timed one by one.
+enum KernelKind {
+  K_FILL_FIELDS,
+  K_FILL_ASSEMBLE,
+  K_LU_INIT,
+  K_LU_PANEL,
+  K_LU_LASWP,
+  K_LU_TRSM,
+  K_ZGEMM,
+  K_PERMUTE,
+  K_TRSV_LOWER,
+  K_TRSV_UPPER,
+  K_CURRENTS,
+  K_ZIN,
+  K_PATTERN_PARTIAL,
+  K_PATTERN_FINISH,
+  K_NKINDS
+};
+const char* kernel_kind_name(int kind);
+
+// While a KernelClock is installed on the calling thread, every launch made
+// through launch_k carries a start/stop event pair (hipExtLaunchKernelGGL),
+// so the events bracket exactly that kernel's execution on its stream.
+class KernelClock {
+ public:
+  KernelClock() = default;
+  KernelClock(const KernelClock&) = delete;
+  KernelClock& operator=(const KernelClock&) = delete;
+  ~KernelClock();
+  bool next(int kind, hipEvent_t* start, hipEvent_t* stop);
+  // After the timed work has completed: add every pending pair to the totals.
+  int collect();
+  void reset();
+  size_t pending() const { return used_; }
+  // pre-create event pairs so that later launches create none
+  bool reserve(size_t slots);
+  double total_ms[K_NKINDS] = {};
+  long launches[K_NKINDS] = {};
+
+ private:
+  std::vector<hipEvent_t> ev_;  // 2 per slot
+  std::vector<int> kind_;
+  size_t used_ = 0;
+};
+extern thread_local KernelClock* g_kernel_clock;
+
+template <typename F, typename... Args>
+inline void launch_k(int kind, F kernel, dim3 grid, dim3 block, uint32_t shmem, hipStream_t stream, Args... args) {
+  hipEvent_t a, b;
+  if (g_kernel_clock && g_kernel_clock->next(kind, &a, &b))
+    hipExtLaunchKernelGGL(kernel, grid, block, shmem, stream, a, b, 0u, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
+}
 
 // Segment tables on the device, in wavelengths (structure of arrays).
 struct DevSegments {

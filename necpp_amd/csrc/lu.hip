@@ -544,7 +544,7 @@ int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* w
   int* perm = piv + n;
   const LuWork w = carve(work, n);
   const int npanels = (n + NB - 1) / NB;
-  hipLaunchKernelGGL(lu_init_kernel, dim3((std::max(n, npanels) + 255) / 256), dim3(256), 0, stream, perm, n,
+  launch_k(K_LU_INIT, lu_init_kernel, dim3((std::max(n, npanels) + 255) / 256), dim3(256), 0, stream, perm, n,
                      w.arrivals, npanels, w.status);
   const int forced = panel_stride_override();
   for (int k0 = 0, pi = 0; k0 < n; k0 += NB, ++pi) {
@@ -559,14 +559,14 @@ int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* w
     // spreading them, so the default is 1; NECGPU_PANEL_STRIDE overrides.
     int stride = 1;
     if (forced > 0 && forced * G <= cu_count) stride = forced;
-    hipLaunchKernelGGL(lu_panel_kernel, dim3(G * stride), dim3(PANEL_THREADS), lds_bytes, stream, A, lda, n, k0, nb, G,
+    launch_k(K_LU_PANEL, lu_panel_kernel, dim3(G * stride), dim3(PANEL_THREADS), lds_bytes, stream, A, lda, n, k0, nb, G,
                        R, stride, piv, perm, info, w.arrivals + pi, w.x, w.status);
     if (n > nb)
-      hipLaunchKernelGGL(lu_laswp_kernel, dim3((n - nb + 255) / 256), dim3(256), 0, stream, A, lda, n, k0, nb, piv);
+      launch_k(K_LU_LASWP, lu_laswp_kernel, dim3((n - nb + 255) / 256), dim3(256), 0, stream, A, lda, n, k0, nb, piv);
     const int rest = n - k0 - nb;
     if (rest <= 0) break;
-    hipLaunchKernelGGL(lu_trsm_kernel, dim3((rest + 63) / 64), dim3(256), 0, stream, A, lda, n, k0, nb);
-    hipLaunchKernelGGL(zgemm_update_kernel, dim3((rest + 63) / 64, (rest + 63) / 64), dim3(256), 0, stream, A, lda,
+    launch_k(K_LU_TRSM, lu_trsm_kernel, dim3((rest + 63) / 64), dim3(256), 0, stream, A, lda, n, k0, nb);
+    launch_k(K_ZGEMM, zgemm_update_kernel, dim3((rest + 63) / 64, (rest + 63) / 64), dim3(256), 0, stream, A, lda,
                        k0 + nb, k0 + nb, rest, rest, k0, nb);
   }
   return hipGetLastError() == hipSuccess ? 0 : 1;

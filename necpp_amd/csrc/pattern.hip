@@ -232,9 +232,9 @@ int launch_pattern(const DevSegments& g, int n, const double* abc, const Pattern
   const int dir_blocks = (n_dir + 255) / 256;
   const int slices = pattern_slices(n, n_dir);
   const int per = (n + slices - 1) / slices;
-  hipLaunchKernelGGL(pattern_partial_kernel, dim3(dir_blocks, slices), dim3(256), 0, stream, g, n, abc, spec.n_theta,
+  launch_k(K_PATTERN_PARTIAL, pattern_partial_kernel, dim3(dir_blocks, slices), dim3(256), 0, stream, g, n, abc, spec.n_theta,
                      n_dir, trig, per, part);
-  hipLaunchKernelGGL(pattern_finish_kernel, dim3(dir_blocks), dim3(256), 0, stream, n_dir, slices, part, trig, spec,
+  launch_k(K_PATTERN_FINISH, pattern_finish_kernel, dim3(dir_blocks), dim3(256), 0, stream, n_dir, slices, part, trig, spec,
                      out);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }

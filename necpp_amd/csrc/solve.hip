@@ -152,12 +152,12 @@ int launch_lu_solve_fp64(const double* LU, long long lda, int n, const int* piv,
   int* status = reinterpret_cast<int*>(flags_b + nblk);
   double* tmp = reinterpret_cast<double*>(static_cast<char*>(work) + (((size_t)2 * nblk * sizeof(unsigned) + 64 + 255) & ~(size_t)255));
   const int* perm = piv + n;
-  hipLaunchKernelGGL(permute_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, x, perm, tmp, n, status);
-  hipLaunchKernelGGL(trsv_blocked_kernel<true>, dim3(nblk), dim3(256), 0, stream, LU, lda, n, tmp, flags_f, epoch,
+  launch_k(K_PERMUTE, permute_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, x, perm, tmp, n, status);
+  launch_k(K_TRSV_LOWER, trsv_blocked_kernel<true>, dim3(nblk), dim3(256), 0, stream, LU, lda, n, tmp, flags_f, epoch,
                      status);
-  hipLaunchKernelGGL(trsv_blocked_kernel<false>, dim3(nblk), dim3(256), 0, stream, LU, lda, n, tmp, flags_b, epoch,
+  launch_k(K_TRSV_UPPER, trsv_blocked_kernel<false>, dim3(nblk), dim3(256), 0, stream, LU, lda, n, tmp, flags_b, epoch,
                      status);
-  hipMemcpyAsync(x, tmp, (size_t)n * 16, hipMemcpyDeviceToDevice, stream);
+  if (hipMemcpyAsync(x, tmp, (size_t)n * 16, hipMemcpyDeviceToDevice, stream) != hipSuccess) return 1;
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

@@ -63,6 +63,8 @@ _SIGS = [
     ("nec_gpu_currents", _L, [_P, _I, _P, _L]),
     ("nec_gpu_stage_times", _L, [_P, _P]),
     ("nec_gpu_benchmark_solve", _L, [_P, _I, _I, _P]),
+    ("nec_gpu_kernel_times", _L, [_P, _P, _L]),
+    ("nec_gpu_kernel_name", ctypes.c_char_p, [_I]),
     ("nec_gpu_fill_matrix", _L, [_P, _D, _P, _L]),
     ("nec_gpu_pattern_array", _L, [_P, _I, _I, _P, _L]),
     ("nec_gpu_pattern_average", _L, [_P, _I, _P, _P]),
@@ -158,6 +160,15 @@ def gpu_pattern_array(ctx, freq_index, name, npoints):
     if lib().nec_gpu_pattern_array(ctx, freq_index, which, _ptr(out), out.size) != 0:
         raise RuntimeError(nec_error_message())
     return out[0::2] + 1j * out[1::2] if cplx else out
+
+
+def gpu_kernel_times(ctx):
+    """{kernel family: (launches, total ms)} of the last nec_gpu_benchmark_solve's timed steps."""
+    out = np.zeros(64)
+    k = lib().nec_gpu_kernel_times(ctx, _ptr(out), out.size)
+    if k < 0:
+        raise RuntimeError(nec_error_message())
+    return {lib().nec_gpu_kernel_name(i).decode(): (int(out[2 * i]), float(out[2 * i + 1])) for i in range(k)}
 
 
 def gpu_info():
