@@ -80,8 +80,8 @@ int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int u
 // (leading dimension lda complex).  piv (2n ints): piv[k] = row exchanged
 // with row k at step k, then the composed permutation.  info (device int)
 // receives 1 + the first exactly-zero pivot, or stays 0.  work: device
-// scratch of lu_workspace_bytes(n); its first int after the panel counters
-// (lu_status) is set to 1 if a panel exchange timed out.
+// scratch of lu_workspace_bytes(n) (no initialisation needed); lu_status
+// points at its status int, set to 1 if a panel exchange timed out.
 size_t lu_workspace_bytes(int n);
 int* lu_status(void* work, int n);
 int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream);

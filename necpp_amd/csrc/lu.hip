@@ -36,19 +36,49 @@ constexpr int RMAX = 112;     // most panel rows per workgroup (LDS: RMAX*(NB+1)
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
 
-// Exchange area of the cooperative panel (double-buffered by column parity).
-struct PanelXchg {
-  double cand_val[2][GMAX];
-  int cand_idx[2][GMAX];
-  double cand_row[2][GMAX][2 * NB];
-  double diag_row[2][2 * NB];
+// Exchange area of the cooperative panel.  Every slot is written once per
+// panel, so a reader knows a value is current when it differs from the
+// sentinel the slot was cleared to (all-ones bits: a NaN payload that no
+// arithmetic produces from the canonical NaNs the panel loads).  Two areas
+// alternate by panel parity; each panel clears the other one for its
+// successor, which runs only after it has finished.
+// Layout of one area (Gcap = most panel workgroups of this n):
+//   val [NB][Gcap] doubles   candidate |z| of group g for column c
+//   idx [NB][Gcap] ints      candidate row (INT_MAX: none)
+//   row [NB][Gcap][2 NB]     candidate row (whole panel row)
+//   diag[NB][2 NB]           row k, published by its owner
+struct Xchg {
+  double* val;
+  int* idx;
+  double* row;
+  double* diag;
 };
 
+__host__ __device__ inline size_t xchg_area_bytes(int Gcap) {
+  return ((size_t)NB * Gcap * 8 + 255) / 256 * 256 + ((size_t)NB * Gcap * 4 + 255) / 256 * 256 +
+         (size_t)NB * Gcap * 2 * NB * 8 + (size_t)NB * 2 * NB * 8;
+}
+
+__host__ __device__ inline Xchg xchg_area(char* base, int Gcap, int parity) {
+  char* p = base + (size_t)parity * xchg_area_bytes(Gcap);
+  Xchg x;
+  x.val = reinterpret_cast<double*>(p);
+  p += ((size_t)NB * Gcap * 8 + 255) / 256 * 256;
+  x.idx = reinterpret_cast<int*>(p);
+  p += ((size_t)NB * Gcap * 4 + 255) / 256 * 256;
+  x.row = reinterpret_cast<double*>(p);
+  p += (size_t)NB * Gcap * 2 * NB * 8;
+  x.diag = reinterpret_cast<double*>(p);
+  return x;
+}
+
 struct LuWork {
-  unsigned* arrivals;  // one counter per panel
-  PanelXchg* x;
   int* status;
+  char* xbase;  // two exchange areas
+  int Gcap;
 };
+
+constexpr long long kSentinelBits = -1LL;
 
 __device__ __forceinline__ void xstore(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -56,36 +86,81 @@ __device__ __forceinline__ void xstore(double* p, double v) {
 __device__ __forceinline__ double xload(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ void xstore_i(int* p, int v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ int xload_i(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ bool is_sentinel(double v) { return __double_as_longlong(v) == kSentinelBits; }
+
+// Clear the exchange slots that a panel of G groups will use (threads
+// tid = 0 .. nthreads-1 share the work).
+__device__ void clear_xchg(const Xchg& x, int Gcap, int G, int tid, int nthreads) {
+  const double s = __longlong_as_double(kSentinelBits);
+  const int nv = NB * G;
+  for (int e = tid; e < nv; e += nthreads) {
+    const int c = e / G, g = e % G;
+    xstore(&x.val[c * Gcap + g], s);
+    xstore_i(&x.idx[c * Gcap + g], -1);
+  }
+  const int nr = NB * G * 2 * NB;
+  for (int e = tid; e < nr; e += nthreads) {
+    const int c = e / (G * 2 * NB), rem = e % (G * 2 * NB);
+    xstore(&x.row[(size_t)c * Gcap * 2 * NB + rem], s);
+  }
+  for (int e = tid; e < NB * 2 * NB; e += nthreads) xstore(&x.diag[e], s);
+}
+
+// Bounded poll of one exchange word; a timeout sets the sticky status.
+__device__ __forceinline__ double poll(const double* p, int* status) {
+  double v = xload(p);
+  long spins = 0;
+  while (is_sentinel(v)) {
+    if (++spins > (1L << 22) || (spins % 64 == 0 && xload_i(status) != 0)) {
+      __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return 0.0;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    v = xload(p);
+  }
+  return v;
+}
 
 __device__ __forceinline__ bool better(double ob, int oi, double best, int idx) {
   return ob > best || (ob == best && oi < idx);
 }
 
-__global__ void lu_init_kernel(int* perm, int n, unsigned* arrivals, int npanels, int* status) {
+__device__ __forceinline__ double canonical(double v) {
+  return (v != v) ? __longlong_as_double(0x7FF8000000000000LL) : v;
+}
+
+__global__ void lu_init_kernel(int* perm, int n, int* status, char* xbase, int Gcap, int G0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) perm[i] = i;
-  if (i < npanels) arrivals[i] = 0u;
   if (i == 0) *status = 0;
+  clear_xchg(xchg_area(xbase, Gcap, 0), Gcap, G0, i, gridDim.x * blockDim.x);
 }
 
 // Factor panel columns [k0, k0+nb) over rows [k0, n) with G workgroups; the
 // active workgroup g is launch block g*stride.  Rows owned by g:
-// [k0 + g*R, min(n, k0 + (g+1)*R)).
+// [k0 + g*R, min(n, k0 + (g+1)*R)), at most RMAX, one thread per row.
 //
 // Column c (k = k0 + c):
-//  A  block reduction of the column-c candidates (|z| from step E).
+//  A  reduction of the column-c candidates (|z| from step E): wave max,
+//     first lane by ballot (rows follow thread order), then two waves.
 //  B  wave 0 applies the update still pending from column c-1 (columns
 //     c+1..) to the candidate row and to row k, publishes both with the
-//     candidate and runs the exchange; the other waves meanwhile apply that
-//     pending update to every other owned row, which takes the bulk of the
-//     rank-1 work off the exchange's critical path.
+//     candidate, and polls the other groups' candidates; the other waves
+//     meanwhile apply that pending update to every other owned row, which
+//     takes the bulk of the rank-1 work off the exchange's critical path.
 //  D  the pivot row is fetched; rows k and p are exchanged.
 //  E  one thread per row: scale column c, update column c+1 only, and form
 //     the column-(c+1) candidate; the update of columns c+2.. is left to B.
 __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restrict__ A, long long lda, int n, int k0,
                                                                   int nb, int G, int R, int stride, int* piv, int* perm,
-                                                                  int* info, unsigned* arrival, PanelXchg* X,
-                                                                  int* status) {
+                                                                  int* info, char* xbase, int Gcap, int parity,
+                                                                  int Gnext, int* status) {
   extern __shared__ double lds[];
   if (blockIdx.x % stride != 0) return;
   constexpr int NW = PANEL_THREADS / 64;  // waves
@@ -95,53 +170,68 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
   const int LDR = NB + 1;  // padded complex row length in LDS
   double* rows = lds;                         // [R][LDR] complex
   double* su = lds + 2 * (size_t)RMAX * LDR;  // current pivot row, nb complex
-  __shared__ double s_best[NW];
-  __shared__ int s_idx[NW];
+  __shared__ double s_best[2];
+  __shared__ int s_idx[2];
   __shared__ int s_win[2];  // winner row, winner group
+  const Xchg X = xchg_area(xbase, Gcap, parity);
+  // the next panel's area: this panel's predecessor used it and has ended
+  if (Gnext > 0) clear_xchg(xchg_area(xbase, Gcap, parity ^ 1), Gcap, Gnext, g * PANEL_THREADS + t, G * PANEL_THREADS);
   const int r0 = k0 + g * R;
   const int r1 = min(n, r0 + R);
   const int nrow = max(0, r1 - r0);
-  for (int e = t; e < nrow * nb; e += PANEL_THREADS) {
-    const int r = e / nb, c = e % nb;
-    const double* v = A + 2 * ((size_t)(r0 + r) * lda + k0 + c);
-    rows[2 * (r * LDR + c)] = v[0];
-    rows[2 * (r * LDR + c) + 1] = v[1];
+  {
+    // every load of a thread in flight at once, then the LDS stores
+    constexpr int PER = (RMAX * NB + PANEL_THREADS - 1) / PANEL_THREADS;
+    double vr[PER], vi[PER];
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = t + PANEL_THREADS * u;
+      vr[u] = vi[u] = 0.0;
+      if (e < nrow * nb) {
+        const int r = e / nb, c = e % nb;
+        const double* v = A + 2 * ((size_t)(r0 + r) * lda + k0 + c);
+        vr[u] = v[0];
+        vi[u] = v[1];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < PER; ++u) {
+      const int e = t + PANEL_THREADS * u;
+      if (e < nrow * nb) {
+        const int r = e / nb, c = e % nb;
+        rows[2 * (r * LDR + c)] = canonical(vr[u]);
+        rows[2 * (r * LDR + c) + 1] = canonical(vi[u]);
+      }
+    }
   }
   __syncthreads();
-  // candidates of column 0
+  // candidate of column 0 (thread t holds row t)
   double best = -1.0;
-  int bidx = INT_MAX;
-  for (int r = t; r < nrow; r += PANEL_THREADS) {
-    const double h = hypot(rows[2 * (r * LDR)], rows[2 * (r * LDR) + 1]);
-    if (h > best) {
-      best = h;
-      bidx = r0 + r;
-    }
-  }
+  if (t < nrow) best = hypot(rows[2 * (t * LDR)], rows[2 * (t * LDR) + 1]);
+  bool has = t < nrow;
   for (int c = 0; c < nb; ++c) {
     const int k = k0 + c;
-    const int par = c & 1;
-    // A. block reduction of the candidates
-    for (int off = 32; off > 0; off >>= 1) {
-      const double ob = __shfl_down(best, off, 64);
-      const int oi = __shfl_down(bidx, off, 64);
-      if (better(ob, oi, best, bidx)) {
-        best = ob;
-        bidx = oi;
+    // A. candidate reduction (rows are held by waves 0 and 1 only)
+    if (wave < 2) {
+      double m = has ? best : -1.0;
+      for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+      const unsigned long long hit = __ballot(has && best == m);
+      if (lane == 0) {
+        s_best[wave] = m;
+        s_idx[wave] = hit ? r0 + wave * 64 + (__ffsll((long long)hit) - 1) : INT_MAX;
       }
-    }
-    if (lane == 0) {
-      s_best[wave] = best;
-      s_idx[wave] = bidx;
     }
     __syncthreads();
-    best = s_best[0];
-    bidx = s_idx[0];
-    for (int w = 1; w < NW; ++w)
-      if (better(s_best[w], s_idx[w], best, bidx)) {
-        best = s_best[w];
-        bidx = s_idx[w];
-      }
+    int bidx;
+    double bval;
+    if (s_idx[0] != INT_MAX && !(s_idx[1] != INT_MAX && s_best[1] > s_best[0])) {
+      bval = s_best[0];
+      bidx = s_idx[0];
+    } else {
+      bval = s_best[1];
+      bidx = s_idx[1];
+    }
+    if (bidx == INT_MAX) bval = -1.0;
     const int rcand = (bidx != INT_MAX) ? bidx - r0 : -1;  // local candidate row
     const int rk = (k >= r0 && k < r1) ? k - r0 : -1;       // local row k
     // columns c+1.. still carry the pending update of column c-1 with
@@ -161,43 +251,46 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
           rows[2 * (r * LDR + lane) + 1] -= prod.im;
         }
       }
-      // The exchange area is only touched with agent-scope atomics, which
-      // the hardware performs at the device coherence point, so no cache
-      // write-back / invalidate fences are needed: the data stores are
-      // complete (s_waitcnt vmcnt(0)) before the arrival is counted.
-      if (lane == 0) {
-        xstore(&X->cand_val[par][g], best);
-        __hip_atomic_store(&X->cand_idx[par][g], bidx, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      }
+      // Slots go straight to the device coherence point (agent-scope
+      // atomics); each is written once per panel, so no ordering between
+      // them is needed: readers poll until a slot leaves the sentinel.
       if (rcand >= 0 && lane < nb) {
-        xstore(&X->cand_row[par][g][2 * lane], rows[2 * (rcand * LDR + lane)]);
-        xstore(&X->cand_row[par][g][2 * lane + 1], rows[2 * (rcand * LDR + lane) + 1]);
+        double* dst = X.row + ((size_t)c * Gcap + g) * 2 * NB;
+        xstore(&dst[2 * lane], rows[2 * (rcand * LDR + lane)]);
+        xstore(&dst[2 * lane + 1], rows[2 * (rcand * LDR + lane) + 1]);
       }
       if (rk >= 0 && lane < nb) {
-        xstore(&X->diag_row[par][2 * lane], rows[2 * (rk * LDR + lane)]);
-        xstore(&X->diag_row[par][2 * lane + 1], rows[2 * (rk * LDR + lane) + 1]);
+        xstore(&X.diag[(size_t)c * 2 * NB + 2 * lane], rows[2 * (rk * LDR + lane)]);
+        xstore(&X.diag[(size_t)c * 2 * NB + 2 * lane + 1], rows[2 * (rk * LDR + lane) + 1]);
       }
-      asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
       if (lane == 0) {
-        __hip_atomic_fetch_add(arrival, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        // bounded wait; a timeout is sticky so every later wait returns at once
-        const unsigned target = (unsigned)G * (unsigned)(c + 1);
-        long spins = 0;
-        while (__hip_atomic_load(arrival, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) < target) {
-          if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-          if (++spins > (1L << 22)) {
-            __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            break;
-          }
-          __builtin_amdgcn_s_sleep(1);
-        }
+        xstore(&X.val[c * Gcap + g], bval);
+        xstore_i(&X.idx[c * Gcap + g], bidx);
       }
       // winner over all groups (every group computes the same)
       double wv = -1.0;
       int wi = INT_MAX, wg = 0;
-      for (int h = lane; h < G; h += 64) {
-        const double v = xload(&X->cand_val[par][h]);
-        const int ix = __hip_atomic_load(&X->cand_idx[par][h], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      for (int h0 = 0; h0 < G; h0 += 64) {
+        const int h = h0 + lane;
+        double v = -1.0;
+        int ix = INT_MAX;
+        if (h < G) {
+          // bounded poll of this group's (val, idx); sticky timeout
+          long spins = 0;
+          v = xload(&X.val[c * Gcap + h]);
+          ix = xload_i(&X.idx[c * Gcap + h]);
+          while (is_sentinel(v) || ix == -1) {
+            if (++spins > (1L << 22) || (spins % 64 == 0 && xload_i(status) != 0)) {
+              __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+              v = -1.0;
+              ix = INT_MAX;
+              break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+            v = xload(&X.val[c * Gcap + h]);
+            ix = xload_i(&X.idx[c * Gcap + h]);
+          }
+        }
         if (better(v, ix, wv, wi)) {
           wv = v;
           wi = ix;
@@ -270,14 +363,15 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
     if (t < nb) {
       double ur, ui;
       if (p != k) {
-        ur = xload(&X->cand_row[par][wg][2 * t]);
-        ui = xload(&X->cand_row[par][wg][2 * t + 1]);
+        const double* src = X.row + ((size_t)c * Gcap + wg) * 2 * NB;
+        ur = poll(&src[2 * t], status);
+        ui = poll(&src[2 * t + 1], status);
       } else if (rk >= 0) {
         ur = rows[2 * (rk * LDR + t)];
         ui = rows[2 * (rk * LDR + t) + 1];
       } else {
-        ur = xload(&X->diag_row[par][2 * t]);
-        ui = xload(&X->diag_row[par][2 * t + 1]);
+        ur = poll(&X.diag[(size_t)c * 2 * NB + 2 * t], status);
+        ui = poll(&X.diag[(size_t)c * 2 * NB + 2 * t + 1], status);
       }
       su[2 * t] = ur;
       su[2 * t + 1] = ui;
@@ -285,8 +379,8 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
     __syncthreads();
     if (p != k && t < nb) {
       if (p >= r0 && p < r1) {
-        rows[2 * ((p - r0) * LDR + t)] = xload(&X->diag_row[par][2 * t]);
-        rows[2 * ((p - r0) * LDR + t) + 1] = xload(&X->diag_row[par][2 * t + 1]);
+        rows[2 * ((p - r0) * LDR + t)] = poll(&X.diag[(size_t)c * 2 * NB + 2 * t], status);
+        rows[2 * ((p - r0) * LDR + t) + 1] = poll(&X.diag[(size_t)c * 2 * NB + 2 * t + 1], status);
       }
       if (rk >= 0) {
         rows[2 * (rk * LDR + t)] = su[2 * t];
@@ -304,9 +398,9 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
     const double denom = swp ? (d.re * ratio) + d.im : (d.im * ratio) + d.re;
     const bool next = c + 1 < nb;
     const zc u1 = next ? zmk(su[2 * (c + 1)], su[2 * (c + 1) + 1]) : zmk(0.0, 0.0);
-    best = -1.0;
-    bidx = INT_MAX;
-    for (int r = max(0, k + 1 - r0) + t; r < nrow; r += PANEL_THREADS) {
+    has = false;
+    const int r = t;
+    if (r < nrow && r0 + r > k) {
       double* v = rows + 2 * (r * LDR + c);
       double lr = v[0], li = v[1];
       if (scale) {
@@ -327,11 +421,8 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
         const double ni = v[3] - prod.im;
         v[2] = nr;
         v[3] = ni;
-        const double h = hypot(nr, ni);
-        if (h > best) {
-          best = h;
-          bidx = r0 + r;
-        }
+        best = hypot(nr, ni);
+        has = true;
       }
     }
     __syncthreads();
@@ -345,144 +436,360 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
 }
 
 // Apply the exchanges piv[k0 .. k0+nb) to columns [0, k0) and [k0+nb, n).
-__global__ void __launch_bounds__(256) lu_laswp_kernel(double* __restrict__ A, long long lda, int n, int k0, int nb,
-                                                       const int* __restrict__ piv) {
-  const int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  const int outside = n - nb;
-  if (idx >= outside) return;
-  const int col = idx < k0 ? idx : idx + nb;
-  for (int q = 0; q < nb; ++q) {
-    const int k = k0 + q;
-    const int p = piv[k];
-    if (p != k) {
-      double* a = A + 2 * ((size_t)k * lda + col);
-      double* b = A + 2 * ((size_t)p * lda + col);
-      const double ar = a[0], ai = a[1];
-      a[0] = b[0];
-      a[1] = b[1];
-      b[0] = ar;
-      b[1] = ai;
-    }
-  }
-}
-
-// U12 = L11^-1 A12: rows [k0, k0+nb), columns [c0, c0+64) of the trailing part.
-__global__ void __launch_bounds__(256) lu_trsm_kernel(double* __restrict__ A, long long lda, int n, int k0, int nb) {
-  __shared__ double sL[NB * NB * 2];  // L11[r][q], complex
-  __shared__ double sX[NB * 64 * 2];  // X[r][c], complex
+// Row exchanges of one panel (piv[k0 .. k0+nb)) applied to the columns
+// [c0, c1) and [c2, c3).  The nb sequential exchanges touch at most 2 nb
+// rows; wave 0 composes them into a list of (destination row, source row)
+// moves (a ballot search over the rows outside the panel's first nb), then
+// every move is done in one gather: all touched rows of the workgroup's 32
+// columns are read into registers, then written to their destinations.
+constexpr int LASWP_COLS = 32;
+__global__ void __launch_bounds__(256) lu_laswp_kernel(double* __restrict__ A, long long lda, int k0, int nb,
+                                                       const int* __restrict__ piv, int c0, int c1, int c2, int c3) {
+  __shared__ int s_dst[2 * NB], s_src[2 * NB];
+  __shared__ int s_moves;
   const int t = threadIdx.x;
-  const int c0 = k0 + nb + blockIdx.x * 64;
-  for (int e = t; e < nb * nb; e += 256) {
-    const int r = e / nb, q = e % nb;
-    const double* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + q);
-    sL[2 * (r * NB + q)] = v[0];
-    sL[2 * (r * NB + q) + 1] = v[1];
-  }
-  for (int e = t; e < nb * 64; e += 256) {
-    const int r = e / 64, c = e % 64;
-    double vr = 0.0, vi = 0.0;
-    if (c0 + c < n) {
-      const double* v = A + 2 * ((size_t)(k0 + r) * lda + c0 + c);
-      vr = v[0];
-      vi = v[1];
+  if (t < 64) {
+    // slot q < nb: row k0 + q; slot nb + e: e-th row outside [k0, k0+nb)
+    // touched (held by lane e).  cont = original row whose content the
+    // slot's row holds after the exchanges so far.
+    const int lane = t;
+    int cont_in = (lane < nb) ? k0 + lane : -1;  // panel rows
+    int ext_row = -1, ext_cont = -1;              // outside rows
+    int next_ext = 0;
+    const int my_piv = (lane < nb) ? piv[k0 + lane] : 0;  // one load per lane, up front
+    for (int q = 0; q < nb; ++q) {
+      const int k = k0 + q;
+      const int p = __shfl(my_piv, q, 64);
+      if (p == k) continue;
+      int ck = __shfl(cont_in, q, 64);
+      int cp;
+      if (p < k0 + nb) {
+        cp = __shfl(cont_in, p - k0, 64);
+        if (lane == p - k0) cont_in = ck;
+      } else {
+        const unsigned long long hit = __ballot(ext_row == p);
+        int e;
+        if (hit) {
+          e = __ffsll((long long)hit) - 1;
+        } else {
+          e = next_ext++;
+          if (lane == e) {
+            ext_row = p;
+            ext_cont = p;
+          }
+        }
+        cp = __shfl(ext_cont, e, 64);
+        if (lane == e) ext_cont = ck;
+      }
+      if (lane == q) cont_in = cp;
     }
-    sX[2 * (r * 64 + c)] = vr;
-    sX[2 * (r * 64 + c) + 1] = vi;
+    // moves: slot rows whose content changed
+    const bool m1 = lane < nb && cont_in != k0 + lane;
+    const bool m2 = ext_row >= 0 && ext_cont != ext_row;
+    const unsigned long long b1 = __ballot(m1), b2 = __ballot(m2);
+    const unsigned long long below = (lane == 0) ? 0ull : (~0ull >> (64 - lane));
+    const int n1 = __popcll(b1);
+    if (m1) {
+      const int i = __popcll(b1 & below);
+      s_dst[i] = k0 + lane;
+      s_src[i] = cont_in;
+    }
+    if (m2) {
+      const int i = n1 + __popcll(b2 & below);
+      s_dst[i] = ext_row;
+      s_src[i] = ext_cont;
+    }
+    if (lane == 0) s_moves = n1 + __popcll(b2);
   }
   __syncthreads();
-  const int c = t & 63;
-  const int rg = t >> 6;  // row group 0..3
-  for (int q = 0; q < nb - 1; ++q) {
-    const zc xq = zmk(sX[2 * (q * 64 + c)], sX[2 * (q * 64 + c) + 1]);
-    for (int r = q + 1 + ((rg - (q + 1)) & 3); r < nb; r += 4) {
-      const zc l = zmk(sL[2 * (r * NB + q)], sL[2 * (r * NB + q) + 1]);
-      const zc prod = zmul(l, xq);
-      sX[2 * (r * 64 + c)] -= prod.re;
-      sX[2 * (r * 64 + c) + 1] -= prod.im;
+  const int moves = s_moves;
+  const int w1 = c1 - c0;
+  const int ncols = w1 + (c3 - c2);
+  const int cbase = blockIdx.x * LASWP_COLS;
+  const int cl = t % LASWP_COLS, rg = t / LASWP_COLS;  // 8 row groups
+  const int idx = cbase + cl;
+  const bool on = idx < ncols;
+  const int col = idx < w1 ? c0 + idx : c2 + idx - w1;
+  // at most 2 nb = 128 moves: 16 per row group, loads issued together
+  {
+    constexpr int RG = 256 / LASWP_COLS;
+    double vr[2 * NB / RG], vi[2 * NB / RG];
+#pragma unroll
+    for (int u = 0; u < 2 * NB / RG; ++u) {
+      const int m = rg + RG * u;
+      vr[u] = vi[u] = 0.0;
+      if (on && m < moves) {
+        const double* v = A + 2 * ((size_t)s_src[m] * lda + col);
+        vr[u] = v[0];
+        vi[u] = v[1];
+      }
     }
+    // every source row of this workgroup's columns is read before any
+    // destination is written (destinations are sources of other moves)
     __syncthreads();
-  }
-  for (int e = t; e < nb * 64; e += 256) {
-    const int r = e / 64, cc = e % 64;
-    if (c0 + cc < n) {
-      double* v = A + 2 * ((size_t)(k0 + r) * lda + c0 + cc);
-      v[0] = sX[2 * (r * 64 + cc)];
-      v[1] = sX[2 * (r * 64 + cc) + 1];
+#pragma unroll
+    for (int u = 0; u < 2 * NB / RG; ++u) {
+      const int m = rg + RG * u;
+      if (on && m < moves) {
+        double* v = A + 2 * ((size_t)s_dst[m] * lda + col);
+        v[0] = vr[u];
+        v[1] = vi[u];
+      }
     }
   }
 }
 
-// C[m0+i][n0+j] -= sum_q A[m0+i][k0+q] * A[k0+q][n0+j], i < M, j < N, q < K.
+// U12 = L11^-1 A12: rows [k0, k0+nb), columns [cb, min(cb + 64, cend))
+// with cb = cbeg + 64 * blockIdx.x, on FP64 MFMA.  L11 (unit lower) is cut
+// into 16 x 16 blocks; wave w inverts diagonal block w, then every wave
+// runs the blocked forward substitution for its own 16 columns,
+//   X_b = D_b^-1 (A_b - sum_{j<b} L_bj X_j),   b = 0..3,
+// with the 16 x 16 complex products as 4 real MFMAs (4M form).  Operand
+// layout of v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16 k, B[k][j] from
+// lane j + 16 k; result register v of lane l holds D[l / 16 + 4 v][l % 16].
+__global__ void __launch_bounds__(256) lu_trsm_kernel(double* __restrict__ A, long long lda, int cend, int k0, int nb,
+                                                      int cbeg) {
+  __shared__ double sL[NB * NB * 2];       // L11[r][q], complex, zero outside the strict lower part
+  __shared__ double sX[NB * 64 * 2];       // X[r][col], complex
+  __shared__ double sDi[4 * 16 * 16 * 2];  // D_b^-1 [b][r][q], complex
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int cb = cbeg + blockIdx.x * 64;
+  {
+    // all loads of a thread in flight at once, then the LDS stores
+    double vr[NB * NB / 256], vi[NB * NB / 256];
+#pragma unroll
+    for (int u = 0; u < NB * NB / 256; ++u) {
+      const int e = t + 256 * u;
+      const int r = e / NB, q = e % NB;
+      vr[u] = vi[u] = 0.0;
+      if (r < nb && q < r) {
+        const double* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + q);
+        vr[u] = v[0];
+        vi[u] = v[1];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < NB * NB / 256; ++u) {
+      sL[2 * (t + 256 * u)] = vr[u];
+      sL[2 * (t + 256 * u) + 1] = vi[u];
+    }
+  }
+  // right-hand side blocks of this wave's columns, in result layout
+  const int li = lane & 15, lk = lane >> 4;
+  const int col = cb + wave * 16 + li;
+  const bool on = col < cend;
+  double4_t tr[4], ti[4];
+#pragma unroll
+  for (int b = 0; b < 4; ++b)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * b + lk + 4 * v;
+      double xr = 0.0, xi = 0.0;
+      if (on && r < nb) {
+        const double* p = A + 2 * ((size_t)(k0 + r) * lda + col);
+        xr = p[0];
+        xi = p[1];
+      }
+      tr[b][v] = xr;
+      ti[b][v] = xi;
+    }
+  __syncthreads();
+  // D_w^-1: lane j < 16 forward-substitutes unit vector e_j
+  if (lane < 16) {
+    double yr[16], yi[16];
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      yr[r] = (r == lane) ? 1.0 : 0.0;
+      yi[r] = 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 15; ++q) {
+#pragma unroll
+      for (int r = q + 1; r < 16; ++r) {
+        const int gr = 16 * wave + r, gq = 16 * wave + q;
+        const double lr = sL[2 * (gr * NB + gq)], lim = sL[2 * (gr * NB + gq) + 1];
+        yr[r] -= lr * yr[q] - lim * yi[q];
+        yi[r] -= lr * yi[q] + lim * yr[q];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      sDi[2 * ((wave * 16 + r) * 16 + lane)] = yr[r];
+      sDi[2 * ((wave * 16 + r) * 16 + lane) + 1] = yi[r];
+    }
+  }
+  __syncthreads();
+  const int xc = wave * 16 + li;  // this lane's column within sX
+#pragma unroll
+  for (int b = 0; b < 4; ++b) {
+    // T_b -= L_bj X_j for finished blocks j < b
+#pragma unroll
+    for (int j = 0; j < b; ++j)
+#pragma unroll
+      for (int kk = 0; kk < 16; kk += 4) {
+        const int lr_ = 16 * b + li, lq = 16 * j + kk + lk;
+        const double ar = sL[2 * (lr_ * NB + lq)], ai = sL[2 * (lr_ * NB + lq) + 1];
+        const double xr = sX[2 * ((16 * j + kk + lk) * 64 + xc)], xi = sX[2 * ((16 * j + kk + lk) * 64 + xc) + 1];
+        tr[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar, xr, tr[b], 0, 0, 0);
+        tr[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, xi, tr[b], 0, 0, 0);
+        ti[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ar, xi, ti[b], 0, 0, 0);
+        ti[b] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, xr, ti[b], 0, 0, 0);
+      }
+    // T_b into sX rows [16 b, 16 b + 16) (plain layout), then X_b = D_b^-1 T_b
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      sX[2 * ((16 * b + lk + 4 * v) * 64 + xc)] = tr[b][v];
+      sX[2 * ((16 * b + lk + 4 * v) * 64 + xc) + 1] = ti[b][v];
+    }
+    // the wave's LDS writes precede its reads (in-order LDS); keep the
+    // compiler from moving the reads above the writes
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    double4_t xr4 = (double4_t){0.0, 0.0, 0.0, 0.0}, xi4 = (double4_t){0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+      const double dr = sDi[2 * ((b * 16 + li) * 16 + kk + lk)], di = sDi[2 * ((b * 16 + li) * 16 + kk + lk) + 1];
+      const double yr = sX[2 * ((16 * b + kk + lk) * 64 + xc)], yi = sX[2 * ((16 * b + kk + lk) * 64 + xc) + 1];
+      xr4 = __builtin_amdgcn_mfma_f64_16x16x4f64(dr, yr, xr4, 0, 0, 0);
+      xr4 = __builtin_amdgcn_mfma_f64_16x16x4f64(-di, yi, xr4, 0, 0, 0);
+      xi4 = __builtin_amdgcn_mfma_f64_16x16x4f64(dr, yi, xi4, 0, 0, 0);
+      xi4 = __builtin_amdgcn_mfma_f64_16x16x4f64(di, yr, xi4, 0, 0, 0);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const int r = 16 * b + lk + 4 * v;
+      sX[2 * (r * 64 + xc)] = xr4[v];
+      sX[2 * (r * 64 + xc) + 1] = xi4[v];
+      if (on && r < nb) {
+        double* p = A + 2 * ((size_t)(k0 + r) * lda + col);
+        p[0] = xr4[v];
+        p[1] = xi4[v];
+      }
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
+// C[m0.., n0..] (M x N) -= L[m0.., k0..k0+K) * U[k0..k0+K, n0..] on FP64
+// MFMA, complex product in 4M form.  Tiles of 64 x 64 complex are taken in
+// turn by the workgroups of a 1-D grid; within a tile, 4 waves each own a
+// 32 x 32 block (2 x 2 MFMA blocks of 16 x 16, re and im accumulators).  K
+// is staged through LDS 16 at a time (split re / im planes, 32 KB), the
+// next chunk's global loads issued before the current chunk's MFMAs.
+// Workgroups are numbered so that each XCD (launch index mod 8) walks a
+// contiguous run of tiles, which keeps the shared L and U rows in its L2.
+constexpr int GK = 16;
 __global__ void __launch_bounds__(256) zgemm_update_kernel(double* __restrict__ A, long long lda, int m0, int n0, int M,
                                                            int N, int k0, int K) {
-  __shared__ double sLr[NB * 64], sLi[NB * 64];  // [q][row]
-  __shared__ double sUr[NB * 64], sUi[NB * 64];  // [q][col]
+  __shared__ double sAr[GK * 64], sAi[GK * 64];  // [k][row]
+  __shared__ double sBr[GK * 64], sBi[GK * 64];  // [k][col]
   const int t = threadIdx.x;
-  const int row0 = blockIdx.y * 64, col0 = blockIdx.x * 64;
-  const int Kp = (K + 3) & ~3;
-  for (int e = t; e < 64 * Kp; e += 256) {
-    const int r = e / Kp, q = e % Kp;
-    double vr = 0.0, vi = 0.0;
-    if (row0 + r < M && q < K) {
-      const double* v = A + 2 * ((size_t)(m0 + row0 + r) * lda + k0 + q);
-      vr = v[0];
-      vi = v[1];
-    }
-    sLr[q * 64 + r] = vr;
-    sLi[q * 64 + r] = vi;
-  }
-  for (int e = t; e < 64 * Kp; e += 256) {
-    const int q = e / 64, c = e % 64;
-    double vr = 0.0, vi = 0.0;
-    if (col0 + c < N && q < K) {
-      const double* v = A + 2 * ((size_t)(k0 + q) * lda + n0 + col0 + c);
-      vr = v[0];
-      vi = v[1];
-    }
-    sUr[q * 64 + c] = vr;
-    sUi[q * 64 + c] = vi;
-  }
-  __syncthreads();
   const int lane = t & 63, wave = t >> 6;
   const int wr = wave >> 1, wc = wave & 1;
   const int li = lane & 15, lk = lane >> 4;
-  double4_t accr[2][2], acci[2][2];
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b) {
-      accr[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
-      acci[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
-    }
-  for (int kk = 0; kk < Kp; kk += 4) {
-    const int q = kk + lk;
-    double ar[2], ai[2], br[2], bi[2];
-    for (int a = 0; a < 2; ++a) {
-      ar[a] = sLr[q * 64 + wr * 32 + a * 16 + li];
-      ai[a] = sLi[q * 64 + wr * 32 + a * 16 + li];
-    }
-    for (int b = 0; b < 2; ++b) {
-      br[b] = sUr[q * 64 + wc * 32 + b * 16 + li];
-      bi[b] = sUi[q * 64 + wc * 32 + b * 16 + li];
-    }
-    for (int a = 0; a < 2; ++a)
-      for (int b = 0; b < 2; ++b) {
-        accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], br[b], accr[a][b], 0, 0, 0);
-        accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], -bi[b], accr[a][b], 0, 0, 0);
-        acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[a], bi[b], acci[a][b], 0, 0, 0);
-        acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[a], br[b], acci[a][b], 0, 0, 0);
-      }
-  }
-  for (int a = 0; a < 2; ++a)
-    for (int b = 0; b < 2; ++b)
-      for (int v = 0; v < 4; ++v) {
-        // result register v of lane l holds D[(l / 16) + 4 v][l % 16]
-        const int i = row0 + wr * 32 + a * 16 + v * 4 + lk;
-        const int j = col0 + wc * 32 + b * 16 + li;
-        if (i < M && j < N) {
-          double* c = A + 2 * ((size_t)(m0 + i) * lda + n0 + j);
-          c[0] -= accr[a][b][v];
-          c[1] -= acci[a][b][v];
+  const int tiles_x = (N + 63) / 64;
+  const int ntiles = tiles_x * ((M + 63) / 64);
+  const int G = gridDim.x;
+  // XCD-contiguous numbering (dispatch places workgroup b on XCD b % 8)
+  const int per = G / 8, extra = G % 8, x = blockIdx.x % 8, y = blockIdx.x / 8;
+  const int logical = (x < extra) ? x * (per + 1) + y : extra * (per + 1) + (x - extra) * per + y;
+  const int nchunks = (K + GK - 1) / GK;
+  // loader coordinates: A chunk 64 rows x 16 k (thread: row t/4, k 4*(t%4)..+4)
+  const int ar_ = t >> 2, ak = (t & 3) * 4;
+  // B chunk 16 k x 64 cols (thread: k t/16, cols 4*(t%16)..+4)
+  const int bq = t >> 4, bc = (t & 15) * 4;
+  for (int tile = logical; tile < ntiles; tile += G) {
+    const int row0 = (tile / tiles_x) * 64, col0 = (tile % tiles_x) * 64;
+    double pa[8], pb[8];
+    auto fetch = [&](int ch) {
+      const int kb = ch * GK;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int q = kb + ak + j;
+        pa[2 * j] = pa[2 * j + 1] = 0.0;
+        if (row0 + ar_ < M && q < K) {
+          const double* v = A + 2 * ((size_t)(m0 + row0 + ar_) * lda + k0 + q);
+          pa[2 * j] = v[0];
+          pa[2 * j + 1] = v[1];
+        }
+        const int qb = kb + bq;
+        pb[2 * j] = pb[2 * j + 1] = 0.0;
+        if (col0 + bc + j < N && qb < K) {
+          const double* v = A + 2 * ((size_t)(k0 + qb) * lda + n0 + col0 + bc + j);
+          pb[2 * j] = v[0];
+          pb[2 * j + 1] = v[1];
         }
       }
+    };
+    double4_t accr[2][2], acci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        accr[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
+        acci[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
+      }
+    fetch(0);
+    for (int ch = 0; ch < nchunks; ++ch) {
+      __syncthreads();  // previous chunk's operands are no longer read
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        sAr[(ak + j) * 64 + ar_] = pa[2 * j];
+        sAi[(ak + j) * 64 + ar_] = pa[2 * j + 1];
+        sBr[bq * 64 + bc + j] = pb[2 * j];
+        sBi[bq * 64 + bc + j] = pb[2 * j + 1];
+      }
+      __syncthreads();
+      if (ch + 1 < nchunks) fetch(ch + 1);
+#pragma unroll
+      for (int kk = 0; kk < GK; kk += 4) {
+        const int q = kk + lk;
+        double xr[2], xi[2], yr[2], yi[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          xr[a] = sAr[q * 64 + wr * 32 + a * 16 + li];
+          xi[a] = sAi[q * 64 + wr * 32 + a * 16 + li];
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          yr[b] = sBr[q * 64 + wc * 32 + b * 16 + li];
+          yi[b] = sBi[q * 64 + wc * 32 + b * 16 + li];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[a], yr[b], accr[a][b], 0, 0, 0);
+            accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[a], -yi[b], accr[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[a], yi[b], acci[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[a], yr[b], acci[a][b], 0, 0, 0);
+          }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          // result register v of lane l holds D[(l / 16) + 4 v][l % 16]
+          const int i = row0 + wr * 32 + a * 16 + v * 4 + lk;
+          const int j = col0 + wc * 32 + b * 16 + li;
+          if (i < M && j < N) {
+            double* c = A + 2 * ((size_t)(m0 + i) * lda + n0 + j);
+            c[0] -= accr[a][b][v];
+            c[1] -= acci[a][b][v];
+          }
+        }
+  }
 }
 
 // One wave: lane l feeds a[l] as the A operand and b[l] as the B operand of
@@ -496,15 +803,14 @@ __global__ void mfma_probe_kernel(const double* a, const double* b, double* d) {
 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+int gcap_for(int n) { return std::max(1, (n + RMAX - 1) / RMAX); }
+
 LuWork carve(void* work, int n) {
-  const int npanels = (n + NB - 1) / NB;
   char* p = static_cast<char*>(work);
   LuWork w;
-  w.arrivals = reinterpret_cast<unsigned*>(p);
-  p += align256((size_t)npanels * sizeof(unsigned));
   w.status = reinterpret_cast<int*>(p);
-  p += 256;
-  w.x = reinterpret_cast<PanelXchg*>(p);
+  w.xbase = p + 256;
+  w.Gcap = gcap_for(n);
   return w;
 }
 
@@ -513,14 +819,49 @@ int panel_stride_override() {
   return (v && *v) ? std::atoi(v) : 0;
 }
 
+// Workgroups per free CU for the trailing update that runs beside a panel
+// (NECGPU_GEMM_OCC overrides, for measurements).
+int gemm_occupancy() {
+  const char* v = std::getenv("NECGPU_GEMM_OCC");
+  const int o = (v && *v) ? std::atoi(v) : 0;
+  return o > 0 ? o : 2;  // 2 measured faster than 1 and 4 at N = 3,280
+}
+
+bool lookahead_enabled() {
+  const char* v = std::getenv("NECGPU_LU_LOOKAHEAD");
+  return !(v && v[0] == '0');
+}
+
+// Second stream (default priority) and two events per device for the
+// look-ahead: the next panel runs there while the caller's stream finishes
+// the trailing update.  Created once, kept for the life of the process.
+struct Lookahead {
+  hipStream_t side = nullptr;
+  hipEvent_t next_ready = nullptr, panel_done = nullptr;
+};
+
+Lookahead* lookahead_for_device() {
+  static Lookahead la[64];
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) return nullptr;
+  Lookahead& l = la[dev];
+  if (!l.side) {
+    if (hipStreamCreateWithFlags(&l.side, hipStreamNonBlocking) != hipSuccess) return nullptr;
+    if (hipEventCreateWithFlags(&l.next_ready, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&l.panel_done, hipEventDisableTiming) != hipSuccess) {
+      (void)hipStreamDestroy(l.side);
+      l.side = nullptr;
+      return nullptr;
+    }
+  }
+  return &l;
+}
+
 }  // namespace
 
 int* lu_status(void* work, int n) { return carve(work, n).status; }
 
-size_t lu_workspace_bytes(int n) {
-  const int npanels = (n + NB - 1) / NB;
-  return align256((size_t)npanels * sizeof(unsigned)) + 256 + align256(sizeof(PanelXchg));
-}
+size_t lu_workspace_bytes(int n) { return 256 + 2 * xchg_area_bytes(gcap_for(n)); }
 
 int launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t stream) {
   hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, stream, a, b, d);
@@ -541,33 +882,84 @@ int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* w
       return 1;
     attr_set = true;
   }
+  if (n <= 0) return 0;
+  // every active panel workgroup needs its own CU at the same time (one per
+  // CU by LDS), so the first (largest) panel bounds n
+  const int G0 = (n + RMAX - 1) / RMAX;
+  if (G0 > GMAX || G0 > cu_count) return 2;
   int* perm = piv + n;
   const LuWork w = carve(work, n);
   const int npanels = (n + NB - 1) / NB;
-  launch_k(K_LU_INIT, lu_init_kernel, dim3((std::max(n, npanels) + 255) / 256), dim3(256), 0, stream, perm, n,
-                     w.arrivals, npanels, w.status);
+  launch_k(K_LU_INIT, lu_init_kernel, dim3(std::max((n + 255) / 256, 64)), dim3(256), 0, stream, perm, n, w.status,
+           w.xbase, w.Gcap, G0);
   const int forced = panel_stride_override();
-  for (int k0 = 0, pi = 0; k0 < n; k0 += NB, ++pi) {
-    const int nb = (n - k0 < NB) ? (n - k0) : NB;
+  Lookahead* la = (npanels > 1 && lookahead_enabled()) ? lookahead_for_device() : nullptr;
+  const int occ = gemm_occupancy();
+  auto panel = [&](int k0, hipStream_t s) {
+    const int nb = std::min(NB, n - k0);
     const int rows = n - k0;
-    int G = (rows + RMAX - 1) / RMAX;
-    // every active workgroup needs its own CU at the same time (one per CU
-    // by LDS), so G may not exceed the CU count
-    if (G > GMAX || G > cu_count) return 2;
+    const int G = (rows + RMAX - 1) / RMAX;
     const int R = (rows + G - 1) / G;
     // Packing the workgroups onto one XCD (stride 8) measured slower than
     // spreading them, so the default is 1; NECGPU_PANEL_STRIDE overrides.
     int stride = 1;
     if (forced > 0 && forced * G <= cu_count) stride = forced;
-    launch_k(K_LU_PANEL, lu_panel_kernel, dim3(G * stride), dim3(PANEL_THREADS), lds_bytes, stream, A, lda, n, k0, nb, G,
-                       R, stride, piv, perm, info, w.arrivals + pi, w.x, w.status);
-    if (n > nb)
-      launch_k(K_LU_LASWP, lu_laswp_kernel, dim3((n - nb + 255) / 256), dim3(256), 0, stream, A, lda, n, k0, nb, piv);
-    const int rest = n - k0 - nb;
-    if (rest <= 0) break;
-    launch_k(K_LU_TRSM, lu_trsm_kernel, dim3((rest + 63) / 64), dim3(256), 0, stream, A, lda, n, k0, nb);
-    launch_k(K_ZGEMM, zgemm_update_kernel, dim3((rest + 63) / 64, (rest + 63) / 64), dim3(256), 0, stream, A, lda,
-                       k0 + nb, k0 + nb, rest, rest, k0, nb);
+    const int Gnext = (k0 + nb < n) ? (n - k0 - nb + RMAX - 1) / RMAX : 0;
+    launch_k(K_LU_PANEL, lu_panel_kernel, dim3(G * stride), dim3(PANEL_THREADS), (uint32_t)lds_bytes, s, A, lda, n,
+             k0, nb, G, R, stride, piv, perm, info, w.xbase, w.Gcap, (k0 / NB) & 1, Gnext, w.status);
+  };
+  auto laswp = [&](int k0, int nb, int c0, int c1, int c2, int c3, hipStream_t s) {
+    const int cols = (c1 - c0) + (c3 - c2);
+    if (cols > 0)
+      launch_k(K_LU_LASWP, lu_laswp_kernel, dim3((cols + LASWP_COLS - 1) / LASWP_COLS), dim3(256), 0, s, A, lda, k0, nb,
+               (const int*)piv, c0, c1, c2, c3);
+  };
+  auto update = [&](int k0, int nb, int cbeg, int cend, int max_groups, hipStream_t s) {
+    // trailing rows [k0+nb, n), columns [cbeg, cend)
+    const int cols = cend - cbeg, rows = n - k0 - nb;
+    if (cols <= 0) return;
+    launch_k(K_LU_TRSM, lu_trsm_kernel, dim3((cols + 63) / 64), dim3(256), 0, s, A, lda, cend, k0, nb, cbeg);
+    if (rows <= 0) return;
+    const int tiles = ((rows + 63) / 64) * ((cols + 63) / 64);
+    const int grid = std::max(1, std::min(tiles, max_groups));
+    launch_k(K_ZGEMM, zgemm_update_kernel, dim3(grid), dim3(256), 0, s, A, lda, k0 + nb, cbeg, rows, cols, k0, nb);
+  };
+  // Right-looking LU with a look-ahead of one panel.  Per panel p at k0:
+  //   caller stream: [wait for panel p] row exchanges, U12 and trailing
+  //   update of the next panel's columns only -> next_ready;
+  //   side stream:   [wait next_ready] factor panel p+1 -> panel_done;
+  //   caller stream: row exchanges, U12 and trailing update of the columns
+  //   beyond the next panel, on at most (CUs - panel workgroups) workgroups
+  //   so that the panel's workgroups find free CUs.
+  // The two streams never touch the same columns at the same time: the
+  // side stream only works on the next panel's columns while the caller's
+  // stream updates the columns beyond them.
+  panel(0, stream);
+  for (int k0 = 0; k0 < n; k0 += NB) {
+    const int nb = std::min(NB, n - k0);
+    const int next = k0 + nb;
+    // panel p ran on the side stream (p > 0): wait for it
+    if (la && k0 > 0) {
+      if (hipStreamWaitEvent(stream, la->panel_done, 0) != hipSuccess) return 1;
+    }
+    if (next >= n) {
+      laswp(k0, nb, 0, k0, n, n, stream);
+      break;
+    }
+    const int nb2 = std::min(NB, n - next);
+    const int far = next + nb2;
+    laswp(k0, nb, next, far, far, far, stream);
+    update(k0, nb, next, far, INT_MAX, stream);
+    if (la) {
+      if (hipEventRecord(la->next_ready, stream) != hipSuccess) return 1;
+      if (hipStreamWaitEvent(la->side, la->next_ready, 0) != hipSuccess) return 1;
+      panel(next, la->side);
+      if (hipEventRecord(la->panel_done, la->side) != hipSuccess) return 1;
+    }
+    laswp(k0, nb, 0, k0, far, n, stream);
+    const int Gnext = (n - next + RMAX - 1) / RMAX;
+    update(k0, nb, far, n, la ? std::max(1, occ * (cu_count - Gnext)) : INT_MAX, stream);
+    if (!la) panel(next, stream);
   }
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
