@@ -172,7 +172,9 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
   double* su = lds + 2 * (size_t)RMAX * LDR;  // current pivot row, nb complex
   __shared__ double s_best[2];
   __shared__ int s_idx[2];
-  __shared__ int s_win[2];  // winner row, winner group
+  __shared__ int s_win[2];     // winner row, winner group
+  __shared__ double s_piv[4];  // pivot row, columns c and c+1
+  __shared__ double s_dg[4];   // row k, columns c and c+1
   const Xchg X = xchg_area(xbase, Gcap, parity);
   // the next panel's area: this panel's predecessor used it and has ended
   if (Gnext > 0) clear_xchg(xchg_area(xbase, Gcap, parity ^ 1), Gcap, Gnext, g * PANEL_THREADS + t, G * PANEL_THREADS);
@@ -267,46 +269,82 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
         xstore(&X.val[c * Gcap + g], bval);
         xstore_i(&X.idx[c * Gcap + g], bidx);
       }
-      // winner over all groups (every group computes the same)
-      double wv = -1.0;
-      int wi = INT_MAX, wg = 0;
+      // a group without a candidate still fills the pivot words that every
+      // reader polls (columns c and c+1 of its row slot)
+      if (rcand < 0 && 2 * c + lane < 2 * min(c + 2, nb) && lane < 4)
+        xstore(X.row + ((size_t)c * Gcap + g) * 2 * NB + 2 * c + lane, 0.0);
+      // Winner over all groups (every group computes the same).  Each lane
+      // polls one group's (val, idx) and, with them, columns c and c+1 of
+      // that group's candidate row and of row k, so that the scaling step
+      // can start before the whole pivot row has arrived.
+      const int c1 = (c + 1 < nb) ? c + 1 : c;
+      const double* dsrc = X.diag + (size_t)c * 2 * NB;
+      double wv = -1.0, e0 = 0.0, e1 = 0.0, e2 = 0.0, e3 = 0.0;
+      double d0 = 0.0, d1 = 0.0, d2 = 0.0, d3 = 0.0;
+      int wi = INT_MAX, wgl = 0;
       for (int h0 = 0; h0 < G; h0 += 64) {
         const int h = h0 + lane;
-        double v = -1.0;
+        const bool act = h < G;
+        const double* rs = X.row + ((size_t)c * Gcap + (act ? h : 0)) * 2 * NB;
+        double v = -1.0, a0 = 0.0, a1 = 0.0, a2 = 0.0, a3 = 0.0;
         int ix = INT_MAX;
-        if (h < G) {
-          // bounded poll of this group's (val, idx); sticky timeout
-          long spins = 0;
-          v = xload(&X.val[c * Gcap + h]);
-          ix = xload_i(&X.idx[c * Gcap + h]);
-          while (is_sentinel(v) || ix == -1) {
-            if (++spins > (1L << 22) || (spins % 64 == 0 && xload_i(status) != 0)) {
-              __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-              v = -1.0;
-              ix = INT_MAX;
-              break;
-            }
-            __builtin_amdgcn_s_sleep(1);
+        long spins = 0;
+        for (;;) {
+          bool ok = true;
+          if (act) {
             v = xload(&X.val[c * Gcap + h]);
             ix = xload_i(&X.idx[c * Gcap + h]);
+            a0 = xload(&rs[2 * c]);
+            a1 = xload(&rs[2 * c + 1]);
+            a2 = xload(&rs[2 * c1]);
+            a3 = xload(&rs[2 * c1 + 1]);
+            ok = !(is_sentinel(v) || ix == -1 || is_sentinel(a0) || is_sentinel(a1) || is_sentinel(a2) ||
+                   is_sentinel(a3));
           }
+          if (h0 == 0) {
+            // row k's columns c, c+1 (same address in every lane)
+            d0 = xload(&dsrc[2 * c]);
+            d1 = xload(&dsrc[2 * c + 1]);
+            d2 = xload(&dsrc[2 * c1]);
+            d3 = xload(&dsrc[2 * c1 + 1]);
+            ok = ok && !(is_sentinel(d0) || is_sentinel(d1) || is_sentinel(d2) || is_sentinel(d3));
+          }
+          if (ok) break;
+          // bounded wait; a timeout is sticky so every later wait returns at once
+          if (++spins > (1L << 22) || (spins % 64 == 0 && xload_i(status) != 0)) {
+            __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            v = -1.0;
+            ix = INT_MAX;
+            break;
+          }
+          __builtin_amdgcn_s_sleep(1);
         }
-        if (better(v, ix, wv, wi)) {
+        if (act && better(v, ix, wv, wi)) {
           wv = v;
           wi = ix;
-          wg = h;
+          wgl = h;
+          e0 = a0;
+          e1 = a1;
+          e2 = a2;
+          e3 = a3;
         }
       }
-      for (int off = 32; off > 0; off >>= 1) {
-        const double ov = __shfl_down(wv, off, 64);
-        const int oi = __shfl_down(wi, off, 64);
-        const int og = __shfl_down(wg, off, 64);
-        if (better(ov, oi, wv, wi)) {
-          wv = ov;
-          wi = oi;
-          wg = og;
-        }
-      }
+      // first maximum: wave max, then the lowest row among the lanes at it
+      double m = wv;
+      for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+      const bool at = wi != INT_MAX && wv == m;
+      int mi = at ? wi : INT_MAX;
+      if (G > 64)  // lanes hold several groups: order by row, not by lane
+        for (int off = 32; off > 0; off >>= 1) mi = min(mi, __shfl_xor(mi, off, 64));
+      const unsigned long long hit = __ballot(at && wi == mi);
+      const int wl = hit ? (__ffsll((long long)hit) - 1) : 0;
+      wv = hit ? m : -1.0;
+      wi = hit ? __shfl(wi, wl, 64) : INT_MAX;
+      int wg = __shfl(wgl, wl, 64);
+      e0 = __shfl(e0, wl, 64);
+      e1 = __shfl(e1, wl, 64);
+      e2 = __shfl(e2, wl, 64);
+      e3 = __shfl(e3, wl, 64);
       if (lane == 0) {
         int p = wi;
         if (p == INT_MAX || !(wv > 0.0)) {
@@ -316,6 +354,22 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
         }
         s_win[0] = p;
         s_win[1] = wg;
+        // pivot row columns c, c+1: the winner's candidate row, or row k
+        if (p != k) {
+          s_piv[0] = e0;
+          s_piv[1] = e1;
+          s_piv[2] = e2;
+          s_piv[3] = e3;
+        } else {
+          s_piv[0] = d0;
+          s_piv[1] = d1;
+          s_piv[2] = d2;
+          s_piv[3] = d3;
+        }
+        s_dg[0] = d0;
+        s_dg[1] = d1;
+        s_dg[2] = d2;
+        s_dg[3] = d3;
         if (g == 0) {
           piv[k] = p;
           if (p != k) {
@@ -359,50 +413,47 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
     __syncthreads();
     const int p = s_win[0];
     const int wg = s_win[1];
-    // D. the pivot row (whole panel row) and the exchange of rows k and p
+    const bool mine_p = (p != k) && p >= r0 && p < r1;
+    // D. loads of the whole pivot row (and, in the winner's group, of row
+    //    k's content for row p) are issued now and used after step E
+    double fsr = 0.0, fsi = 0.0, fpr = 0.0, fpi = 0.0;
+    const double* fsrc = nullptr;
     if (t < nb) {
-      double ur, ui;
       if (p != k) {
-        const double* src = X.row + ((size_t)c * Gcap + wg) * 2 * NB;
-        ur = poll(&src[2 * t], status);
-        ui = poll(&src[2 * t + 1], status);
-      } else if (rk >= 0) {
-        ur = rows[2 * (rk * LDR + t)];
-        ui = rows[2 * (rk * LDR + t) + 1];
+        fsrc = X.row + ((size_t)c * Gcap + wg) * 2 * NB;
+      } else if (rk < 0) {
+        fsrc = X.diag + (size_t)c * 2 * NB;
+      }
+      if (fsrc) {
+        fsr = xload(&fsrc[2 * t]);
+        fsi = xload(&fsrc[2 * t + 1]);
       } else {
-        ur = poll(&X.diag[(size_t)c * 2 * NB + 2 * t], status);
-        ui = poll(&X.diag[(size_t)c * 2 * NB + 2 * t + 1], status);
+        fsr = rows[2 * (rk * LDR + t)];
+        fsi = rows[2 * (rk * LDR + t) + 1];
       }
-      su[2 * t] = ur;
-      su[2 * t + 1] = ui;
-    }
-    __syncthreads();
-    if (p != k && t < nb) {
-      if (p >= r0 && p < r1) {
-        rows[2 * ((p - r0) * LDR + t)] = poll(&X.diag[(size_t)c * 2 * NB + 2 * t], status);
-        rows[2 * ((p - r0) * LDR + t) + 1] = poll(&X.diag[(size_t)c * 2 * NB + 2 * t + 1], status);
-      }
-      if (rk >= 0) {
-        rows[2 * (rk * LDR + t)] = su[2 * t];
-        rows[2 * (rk * LDR + t) + 1] = su[2 * t + 1];
+      if (mine_p) {
+        fpr = xload(&X.diag[(size_t)c * 2 * NB + 2 * t]);
+        fpi = xload(&X.diag[(size_t)c * 2 * NB + 2 * t + 1]);
       }
     }
-    __syncthreads();
     // E. one thread per row below k: scale column c (Smith's division as
     //    g++'s complex divide, with the pivot's ratio and denominator formed
     //    once), update column c+1, and keep the column-(c+1) candidate.
-    const zc d = zmk(su[2 * c], su[2 * c + 1]);
+    //    Row p (winner's group) takes row k's columns c, c+1 from the poll.
+    const zc d = zmk(s_piv[0], s_piv[1]);
     const bool scale = (d.re != 0.0) || (d.im != 0.0);
     const bool swp = fabs(d.re) < fabs(d.im);
     const double ratio = swp ? d.re / d.im : d.im / d.re;
     const double denom = swp ? (d.re * ratio) + d.im : (d.im * ratio) + d.re;
     const bool next = c + 1 < nb;
-    const zc u1 = next ? zmk(su[2 * (c + 1)], su[2 * (c + 1) + 1]) : zmk(0.0, 0.0);
+    const zc u1 = next ? zmk(s_piv[2], s_piv[3]) : zmk(0.0, 0.0);
     has = false;
     const int r = t;
     if (r < nrow && r0 + r > k) {
       double* v = rows + 2 * (r * LDR + c);
-      double lr = v[0], li = v[1];
+      const bool is_p = (r0 + r == p);
+      double lr = is_p ? s_dg[0] : v[0];
+      double li = is_p ? s_dg[1] : v[1];
       if (scale) {
         const double ar = lr, ai = li;
         if (swp) {
@@ -412,17 +463,39 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
           lr = ((ai * ratio) + ar) / denom;
           li = (ai - (ar * ratio)) / denom;
         }
-        v[0] = lr;
-        v[1] = li;
       }
+      v[0] = lr;
+      v[1] = li;
       if (next) {
         const zc prod = zmul(zmk(lr, li), u1);
-        const double nr = v[2] - prod.re;
-        const double ni = v[3] - prod.im;
+        const double nr = (is_p ? s_dg[2] : v[2]) - prod.re;
+        const double ni = (is_p ? s_dg[3] : v[3]) - prod.im;
         v[2] = nr;
         v[3] = ni;
         best = hypot(nr, ni);
         has = true;
+      }
+    }
+    // the whole pivot row into su; rows k and p exchanged (row p keeps the
+    // columns c, c+1 step E has just written)
+    if (t < nb) {
+      if (fsrc && (is_sentinel(fsr) || is_sentinel(fsi))) {
+        fsr = poll(&fsrc[2 * t], status);
+        fsi = poll(&fsrc[2 * t + 1], status);
+      }
+      su[2 * t] = fsr;
+      su[2 * t + 1] = fsi;
+      if (p != k && rk >= 0) {
+        rows[2 * (rk * LDR + t)] = fsr;
+        rows[2 * (rk * LDR + t) + 1] = fsi;
+      }
+      if (mine_p && t != c && !(next && t == c + 1)) {
+        if (is_sentinel(fpr) || is_sentinel(fpi)) {
+          fpr = poll(&X.diag[(size_t)c * 2 * NB + 2 * t], status);
+          fpi = poll(&X.diag[(size_t)c * 2 * NB + 2 * t + 1], status);
+        }
+        rows[2 * ((p - r0) * LDR + t)] = fpr;
+        rows[2 * ((p - r0) * LDR + t) + 1] = fpi;
       }
     }
     __syncthreads();
