@@ -243,6 +243,18 @@ void DeviceSolver::enqueue_factor() {
   const int rc = launch_lu_fp64(Z_.as<double>(), ld_, n, piv_.as<int>(), info_.as<int>(), lu_work_.as<void>(), stream_);
   if (rc == 2) throw NecError("LU: matrix order " + std::to_string(n) + " exceeds the panel exchange area");
   if (rc) hip_check(hipGetLastError(), "lu launch");
+  ensure_solve_work();
+  if (launch_lu_solve_prepare(Z_.as<double>(), ld_, n, solve_work_.as<void>(), stream_))
+    hip_check(hipGetLastError(), "solve prepare launch");
+}
+
+void DeviceSolver::ensure_solve_work() {
+  const size_t wbytes = solve_workspace_bytes(n_);
+  if (solve_work_.size() < wbytes) {
+    solve_work_.ensure(wbytes);
+    hip_check(hipMemsetAsync(solve_work_.as<void>(), 0, wbytes, stream_), "memset");
+    epoch_ = 0;
+  }
 }
 
 void DeviceSolver::enqueue_solve(const void* rhs_dev, hipMemcpyKind kind, const void* rhs_host) {
@@ -250,16 +262,11 @@ void DeviceSolver::enqueue_solve(const void* rhs_dev, hipMemcpyKind kind, const 
   x_.ensure((size_t)n * 16);
   abc_.ensure((size_t)6 * n * sizeof(double));
   curx_.ensure((size_t)n * 16);
-  const size_t wbytes = solve_workspace_bytes(n);
-  if (solve_work_.size() < wbytes) {
-    solve_work_.ensure(wbytes);
-    hip_check(hipMemsetAsync(solve_work_.as<void>(), 0, wbytes, stream_), "memset");
-    epoch_ = 0;
-  }
+  ensure_solve_work();
   ++epoch_;
   hip_check(hipMemcpyAsync(x_.as<void>(), rhs_dev ? rhs_dev : rhs_host, (size_t)n * 16, kind, stream_), "rhs");
-  if (launch_lu_solve_fp64(Z_.as<double>(), ld_, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(), epoch_,
-                           stream_))
+  if (launch_lu_solve_apply(Z_.as<double>(), ld_, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(), epoch_,
+                            stream_))
     hip_check(hipGetLastError(), "solve launch");
   if (launch_currents(n, x_.as<double>(), segptr_.as<int>(), segbasis_.as<int>(), segabc_.as<double>(),
                       abc_.as<double>(), curx_.as<double>(), stream_))
@@ -334,9 +341,7 @@ void DeviceSolver::solve(const std::vector<zc>& e, std::vector<zc>* curx, std::v
     hip_check(hipMemcpyAsync(abc->data(), abc_.as<void>(), (size_t)6 * n * 8, hipMemcpyDeviceToHost, stream_), "abc");
   }
   int status = 0;
-  const int nblk = (n + 63) / 64;
-  hip_check(hipMemcpyAsync(&status, static_cast<char*>(solve_work_.as<void>()) + 2 * nblk * sizeof(unsigned),
-                           sizeof(int), hipMemcpyDeviceToHost, stream_),
+  hip_check(hipMemcpyAsync(&status, solve_status(solve_work_.as<void>(), n), sizeof(int), hipMemcpyDeviceToHost, stream_),
             "status");
   hip_check(hipStreamSynchronize(stream_), "solve");
   if (status) throw NecError("triangular solve: block wait timed out");

@@ -98,6 +98,7 @@ class DeviceSolver {
   DevSegments dev_segments() const;
   void enqueue_fill(double rkh);
   void enqueue_factor();
+  void ensure_solve_work();
   void enqueue_solve(const void* rhs_dev, hipMemcpyKind kind, const void* rhs_host);
   PatternOut pattern_buffers(int nd);
   void enqueue_pattern(const PatternSpec& spec, const PatternOut& po);

@@ -19,7 +19,8 @@ enum KernelKind {
   K_LU_LASWP,
   K_LU_TRSM,
   K_ZGEMM,
-  K_PERMUTE,
+  K_SOLVE_RESET,
+  K_DIAG_INVERSE,
   K_TRSV_LOWER,
   K_TRSV_UPPER,
   K_CURRENTS,
@@ -88,10 +89,20 @@ int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* w
 // one-wave check of the FP64 MFMA operand / result lane layout (64 a, 64 b in, 256 d out)
 int launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t stream);
 
-// solve.hip: x <- A^-1 x with the LU factors of launch_lu_fp64.  work: device
-// scratch of solve_workspace_bytes(n) bytes; epoch: a value that increases
-// with every call on the same workspace.
+// solve.hip: x <- A^-1 x with the LU factors of launch_lu_fp64 (or the FP32
+// factors of the mixed path; x stays FP64).  work: device scratch of
+// solve_workspace_bytes(n) bytes, zeroed once; epoch: a value that increases
+// with every apply on the same workspace.  prepare (inverts the diagonal
+// blocks) runs once per factorisation, before any apply.
 size_t solve_workspace_bytes(int n);
+int* solve_status(void* work, int n);
+int launch_lu_solve_prepare(const double* LU, long long lda, int n, void* work, hipStream_t stream);
+int launch_lu_solve_prepare_f32(const float* LU, long long lda, int n, void* work, hipStream_t stream);
+int launch_lu_solve_apply(const double* LU, long long lda, int n, const int* piv, double* x, void* work,
+                          unsigned epoch, hipStream_t stream);
+int launch_lu_solve_apply_f32(const float* LU, long long lda, int n, const int* piv, double* x, void* work,
+                              unsigned epoch, hipStream_t stream);
+// prepare + apply
 int launch_lu_solve_fp64(const double* LU, long long lda, int n, const int* piv, double* x, void* work,
                          unsigned epoch, hipStream_t stream);
 

@@ -1,12 +1,20 @@
-// solve.hip -- forward / backward substitution with the LU factors (FP64).
+// solve.hip -- forward / backward substitution with the LU factors.
 //
 // Replaces matrix_algebra.cpp solve (:271-307): x <- P b, L y = x (unit
-// lower), U x = y.  Each triangular sweep is one launch of one workgroup per
-// 64-row block.  Block b accumulates the products with every finished block
-// as soon as that block's flag is raised (flags carry an epoch, so no reset
-// is needed between calls), then solves its own 64 x 64 diagonal block in LDS
-// and raises its flag.  Workgroups are dispatched in block order, so every
-// block a workgroup waits on is already resident or done.
+// lower), U x = y.  The factors are FP64 (the FP64 path) or FP32 (the
+// mixed-precision path); the right-hand side and all arithmetic are FP64.
+//
+// Once per factorisation, the 64 x 64 diagonal blocks of L and U are
+// inverted (one wave per block, all blocks at once).  Each triangular sweep
+// is then one launch of one workgroup per 64-row block: block b subtracts
+// the products with every finished block as soon as that block's flag is
+// raised (flags carry an epoch, so no reset is needed between calls), then
+// applies its diagonal block's inverse (a 64 x 64 matrix-vector product
+// instead of 64 dependent steps) and raises its flag.  The block that
+// finishes last before b (the neighbour) and the inverse are staged in LDS
+// while b waits, so that b's critical path is two small products.
+// Workgroups are dispatched in block order, so every block a workgroup
+// waits on is already resident or done.
 #include "kernels.h"
 #include "nec_math.h"
 
@@ -16,119 +24,193 @@ namespace {
 
 constexpr int SB = 64;
 
+__device__ __forceinline__ double ld(const double* p) { return *p; }
+__device__ __forceinline__ double ld(const float* p) { return static_cast<double>(*p); }
+
 __device__ __forceinline__ unsigned load_flag(const unsigned* f) {
   return __hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-__global__ void permute_kernel(const double* __restrict__ b, const int* __restrict__ perm, double* __restrict__ x,
-                               int n, int* status) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i == 0) *status = 0;
-  if (i < n) {
-    const int p = perm[i];
-    x[2 * i] = b[2 * p];
-    x[2 * i + 1] = b[2 * p + 1];
+// Bounded wait of thread 0 for flags[c] == epoch; a timeout is sticky so
+// every later wait returns at once.
+__device__ void wait_flag(const unsigned* flags, int c, unsigned epoch, int* status) {
+  long spins = 0;
+  while (load_flag(flags + c) != epoch) {
+    if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+    if (++spins > (1L << 22)) {
+      __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      break;
+    }
+    __builtin_amdgcn_s_sleep(1);
   }
 }
 
-// lower = true: L y = x, unit diagonal, blocks in increasing order.
-// lower = false: U x = y, blocks in decreasing order.
-template <bool lower>
-__global__ void __launch_bounds__(256) trsv_blocked_kernel(const double* __restrict__ LU, long long lda, int n,
-                                                           double* x, unsigned* flags, unsigned epoch,
-                                                           int* status) {
-  __shared__ double sD[SB * SB * 2];
-  __shared__ double sAcc[SB * 2];
+// Inverse of diagonal block b of L (blockIdx.y == 0, unit lower) or U
+// (blockIdx.y == 1, upper with its diagonal), padded to 64 x 64 with the
+// identity.  Lane j computes column j by substitution on e_j.
+template <typename TF>
+__global__ void __launch_bounds__(64) diag_inverse_kernel(const TF* __restrict__ LU, long long lda, int n,
+                                                          double* __restrict__ inv) {
+  __shared__ double sT[SB * SB * 2];  // the block, [r][q]
+  __shared__ double sX[SB * SB * 2];  // its inverse, [r][j]
+  const int b = blockIdx.x;
+  const bool upper = blockIdx.y == 1;
+  const int nblk = gridDim.x;
+  const int r0 = b * SB;
+  const int m = min(SB, n - r0);
+  const int j = threadIdx.x;
+  for (int e = j; e < SB * SB; e += 64) {
+    const int r = e / SB, q = e % SB;
+    double vr = 0.0, vi = 0.0;
+    if (r < m && q < m) {
+      const TF* v = LU + 2 * ((size_t)(r0 + r) * lda + r0 + q);
+      vr = ld(v);
+      vi = ld(v + 1);
+    }
+    sT[2 * e] = vr;
+    sT[2 * e + 1] = vi;
+  }
+  __syncthreads();
+  if (!upper) {
+    for (int r = 0; r < SB; ++r) {
+      double sr = (r == j) ? 1.0 : 0.0, si = 0.0;
+      if (r < m) {
+        for (int q = j; q < r; ++q) {
+          const double lr = sT[2 * (r * SB + q)], li = sT[2 * (r * SB + q) + 1];
+          const double xr = sX[2 * (q * SB + j)], xi = sX[2 * (q * SB + j) + 1];
+          sr -= lr * xr - li * xi;
+          si -= lr * xi + li * xr;
+        }
+      }
+      sX[2 * (r * SB + j)] = sr;
+      sX[2 * (r * SB + j) + 1] = si;
+    }
+  } else {
+    for (int r = SB - 1; r >= 0; --r) {
+      double sr = (r == j) ? 1.0 : 0.0, si = 0.0;
+      if (r < m && r <= j) {
+        for (int q = r + 1; q <= j && q < m; ++q) {
+          const double ur = sT[2 * (r * SB + q)], ui = sT[2 * (r * SB + q) + 1];
+          const double xr = sX[2 * (q * SB + j)], xi = sX[2 * (q * SB + j) + 1];
+          sr -= ur * xr - ui * xi;
+          si -= ur * xi + ui * xr;
+        }
+        const zc x = zdiv(zmk(sr, si), zmk(sT[2 * (r * SB + r)], sT[2 * (r * SB + r) + 1]));
+        sr = x.re;
+        si = x.im;
+      } else if (r < m) {
+        sr = 0.0;  // the inverse of an upper triangle is upper
+      }
+      sX[2 * (r * SB + j)] = sr;
+      sX[2 * (r * SB + j) + 1] = si;
+    }
+  }
+  double* dst = inv + ((size_t)(upper ? nblk : 0) + b) * SB * SB * 2;
+  for (int e = j; e < SB * SB * 2; e += 64) dst[e] = sX[e];
+}
+
+// One triangular sweep.  lower: out[r] = y_r with L y = P rhs (rhs read
+// through perm).  upper: out[r] = x_r with U x = rhs.
+template <typename TF, bool lower>
+__global__ void __launch_bounds__(256) trsv_kernel(const TF* __restrict__ LU, long long lda, int n,
+                                                   const int* __restrict__ perm, const double* rhs, double* out,
+                                                   const double* __restrict__ inv_all, unsigned* flags,
+                                                   unsigned epoch, int* status) {
+  __shared__ double sInv[SB * SB * 2];   // inverse of the diagonal block
+  __shared__ double sNb[SB * SB * 2];    // the neighbour block
   __shared__ double sPart[4][SB * 2];
+  __shared__ double sAcc[SB * 2];
   const int nblk = (n + SB - 1) / SB;
   const int b = lower ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
+  const int nb = lower ? b - 1 : b + 1;  // neighbour block (last to finish)
+  const bool has_nb = lower ? (b > 0) : (b < nblk - 1);
   const int r0 = b * SB;
-  const int rows = (n - r0 < SB) ? (n - r0) : SB;
+  const int rows = min(SB, n - r0);
   const int t = threadIdx.x;
-  const int r = t & 63;    // row within the block
-  const int part = t >> 6; // column quarter handled by this thread
-  // diagonal block into LDS
-  for (int e = t; e < rows * rows; e += 256) {
-    const int i = e / rows, j = e % rows;
-    const double* v = LU + 2 * ((size_t)(r0 + i) * lda + r0 + j);
-    sD[2 * (i * SB + j)] = v[0];
-    sD[2 * (i * SB + j) + 1] = v[1];
-  }
-  double accr = 0.0, acci = 0.0;
-  // off-diagonal blocks in the order they become available
-  for (int step = 0; step < (lower ? b : nblk - 1 - b); ++step) {
-    const int c = lower ? step : nblk - 1 - step;
-    if (t == 0) {
-      // bounded wait; a timeout is sticky so every later wait returns at once
-      long spins = 0;
-      while (load_flag(flags + c) != epoch) {
-        if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-        if (++spins > (1L << 22)) {
-          __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
+  const int r = t & 63;     // row within the block
+  const int part = t >> 6;  // column quarter of this thread
+  // stage the inverse and the neighbour block while earlier blocks finish
+  {
+    const double* src = inv_all + ((size_t)(lower ? 0 : nblk) + b) * SB * SB * 2;
+    for (int e = t; e < SB * SB * 2; e += 256) sInv[e] = src[e];
+    if (has_nb) {
+      const int c0 = nb * SB;
+      const int cols = min(SB, n - c0);
+      for (int e = t; e < SB * SB; e += 256) {
+        const int i = e / SB, q = e % SB;
+        double vr = 0.0, vi = 0.0;
+        if (i < rows && q < cols) {
+          const TF* v = LU + 2 * ((size_t)(r0 + i) * lda + c0 + q);
+          vr = ld(v);
+          vi = ld(v + 1);
         }
-        __builtin_amdgcn_s_sleep(1);
+        sNb[2 * e] = vr;
+        sNb[2 * e + 1] = vi;
       }
     }
+  }
+  double accr = 0.0, acci = 0.0;
+  // blocks other than the neighbour, in the order they finish
+  const int others = lower ? max(0, b - 1) : max(0, nblk - 2 - b);
+  for (int step = 0; step < others; ++step) {
+    const int c = lower ? step : nblk - 1 - step;
+    if (t == 0) wait_flag(flags, c, epoch, status);
     __syncthreads();
     const int c0 = c * SB;
-    const int cols = (n - c0 < SB) ? (n - c0) : SB;
+    const int cols = min(SB, n - c0);
     if (r < rows) {
-      const double* lrow = LU + 2 * ((size_t)(r0 + r) * lda + c0);
+      const TF* lrow = LU + 2 * ((size_t)(r0 + r) * lda + c0);
       for (int j = part; j < cols; j += 4) {
-        const double yr = __builtin_nontemporal_load(x + 2 * (c0 + j));
-        const double yi = __builtin_nontemporal_load(x + 2 * (c0 + j) + 1);
-        const double lr = lrow[2 * j], li = lrow[2 * j + 1];
+        const double yr = __builtin_nontemporal_load(out + 2 * (c0 + j));
+        const double yi = __builtin_nontemporal_load(out + 2 * (c0 + j) + 1);
+        const double lr = ld(lrow + 2 * j), li = ld(lrow + 2 * j + 1);
         accr += lr * yr - li * yi;
         acci += lr * yi + li * yr;
       }
     }
   }
+  if (has_nb) {
+    if (t == 0) wait_flag(flags, nb, epoch, status);
+    __syncthreads();
+    const int c0 = nb * SB;
+    for (int j = part; j < SB; j += 4) {
+      const double yr = (c0 + j < n) ? __builtin_nontemporal_load(out + 2 * (c0 + j)) : 0.0;
+      const double yi = (c0 + j < n) ? __builtin_nontemporal_load(out + 2 * (c0 + j) + 1) : 0.0;
+      const double lr = sNb[2 * (r * SB + j)], li = sNb[2 * (r * SB + j) + 1];
+      accr += lr * yr - li * yi;
+      acci += lr * yi + li * yr;
+    }
+  }
   sPart[part][2 * r] = accr;
   sPart[part][2 * r + 1] = acci;
   __syncthreads();
-  if (t < rows) {
+  if (t < SB) {
+    double vr = 0.0, vi = 0.0;
+    if (t < rows) {
+      const int src = lower ? perm[r0 + t] : r0 + t;
+      vr = rhs[2 * src];
+      vi = rhs[2 * src + 1];
+    }
     const double sr = ((sPart[0][2 * t] + sPart[1][2 * t]) + sPart[2][2 * t]) + sPart[3][2 * t];
     const double si = ((sPart[0][2 * t + 1] + sPart[1][2 * t + 1]) + sPart[2][2 * t + 1]) + sPart[3][2 * t + 1];
-    sAcc[2 * t] = x[2 * (r0 + t)] - sr;
-    sAcc[2 * t + 1] = x[2 * (r0 + t) + 1] - si;
+    sAcc[2 * t] = (t < rows) ? vr - sr : 0.0;
+    sAcc[2 * t + 1] = (t < rows) ? vi - si : 0.0;
   }
   __syncthreads();
-  // diagonal block: one wave, sequential over rows, lanes over rows
-  if (t < 64) {
-    if (lower) {
-      for (int q = 0; q < rows; ++q) {
-        const zc yq = zmk(sAcc[2 * q], sAcc[2 * q + 1]);
-        if (t > q && t < rows) {
-          const zc l = zmk(sD[2 * (t * SB + q)], sD[2 * (t * SB + q) + 1]);
-          const zc p = zmul(l, yq);
-          sAcc[2 * t] -= p.re;
-          sAcc[2 * t + 1] -= p.im;
-        }
-        __builtin_amdgcn_wave_barrier();
-      }
-    } else {
-      for (int q = rows - 1; q >= 0; --q) {
-        if (t == q) {
-          const zc v = zdiv(zmk(sAcc[2 * q], sAcc[2 * q + 1]), zmk(sD[2 * (q * SB + q)], sD[2 * (q * SB + q) + 1]));
-          sAcc[2 * q] = v.re;
-          sAcc[2 * q + 1] = v.im;
-        }
-        __builtin_amdgcn_wave_barrier();
-        const zc xq = zmk(sAcc[2 * q], sAcc[2 * q + 1]);
-        if (t < q) {
-          const zc u = zmk(sD[2 * (t * SB + q)], sD[2 * (t * SB + q) + 1]);
-          const zc p = zmul(u, xq);
-          sAcc[2 * t] -= p.re;
-          sAcc[2 * t + 1] -= p.im;
-        }
-        __builtin_amdgcn_wave_barrier();
-      }
-    }
-    if (t < rows) {
-      x[2 * (r0 + t)] = sAcc[2 * t];
-      x[2 * (r0 + t) + 1] = sAcc[2 * t + 1];
-    }
+  // x_b = inverse * acc
+  double xr = 0.0, xi = 0.0;
+  for (int j = part; j < SB; j += 4) {
+    const double ir = sInv[2 * (r * SB + j)], ii = sInv[2 * (r * SB + j) + 1];
+    const double ar = sAcc[2 * j], ai = sAcc[2 * j + 1];
+    xr += ir * ar - ii * ai;
+    xi += ir * ai + ii * ar;
+  }
+  sPart[part][2 * r] = xr;
+  sPart[part][2 * r + 1] = xi;
+  __syncthreads();
+  if (t < rows) {
+    out[2 * (r0 + t)] = ((sPart[0][2 * t] + sPart[1][2 * t]) + sPart[2][2 * t]) + sPart[3][2 * t];
+    out[2 * (r0 + t) + 1] = ((sPart[0][2 * t + 1] + sPart[1][2 * t + 1]) + sPart[2][2 * t + 1]) + sPart[3][2 * t + 1];
   }
   __syncthreads();
   if (t == 0) {
@@ -137,28 +219,87 @@ __global__ void __launch_bounds__(256) trsv_blocked_kernel(const double* __restr
   }
 }
 
+__global__ void reset_status_kernel(int* status) {
+  if (threadIdx.x == 0) *status = 0;
+}
+
+struct SolveWork {
+  unsigned *flags_f, *flags_b;
+  int* status;
+  double* tmp;
+  double* inv;
+};
+
+size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
+
+SolveWork carve_solve(void* work, int n) {
+  const int nblk = (n + SB - 1) / SB;
+  char* p = static_cast<char*>(work);
+  SolveWork w;
+  w.flags_f = reinterpret_cast<unsigned*>(p);
+  w.flags_b = w.flags_f + nblk;
+  p += align256((size_t)2 * nblk * sizeof(unsigned));
+  w.status = reinterpret_cast<int*>(p);
+  p += 256;
+  w.tmp = reinterpret_cast<double*>(p);
+  p += align256((size_t)n * 16);
+  w.inv = reinterpret_cast<double*>(p);
+  return w;
+}
+
+template <typename TF>
+int prepare(const TF* LU, long long lda, int n, void* work, hipStream_t stream) {
+  if (n <= 0) return 0;
+  const int nblk = (n + SB - 1) / SB;
+  const SolveWork w = carve_solve(work, n);
+  launch_k(K_DIAG_INVERSE, diag_inverse_kernel<TF>, dim3(nblk, 2), dim3(64), 0u, stream, LU, lda, n, w.inv);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+template <typename TF>
+int apply(const TF* LU, long long lda, int n, const int* piv, double* x, void* work, unsigned epoch,
+          hipStream_t stream) {
+  if (n <= 0) return 0;
+  const int nblk = (n + SB - 1) / SB;
+  const SolveWork w = carve_solve(work, n);
+  const int* perm = piv + n;
+  launch_k(K_SOLVE_RESET, reset_status_kernel, dim3(1), dim3(64), 0u, stream, w.status);
+  launch_k(K_TRSV_LOWER, trsv_kernel<TF, true>, dim3(nblk), dim3(256), 0u, stream, LU, lda, n, perm,
+           (const double*)x, w.tmp, (const double*)w.inv, w.flags_f, epoch, w.status);
+  launch_k(K_TRSV_UPPER, trsv_kernel<TF, false>, dim3(nblk), dim3(256), 0u, stream, LU, lda, n, perm,
+           (const double*)w.tmp, x, (const double*)w.inv, w.flags_b, epoch, w.status);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 }  // namespace
 
 size_t solve_workspace_bytes(int n) {
   const int nblk = (n + SB - 1) / SB;
-  return (size_t)2 * nblk * sizeof(unsigned) + 64 + (size_t)n * 16 + 256;
+  return align256((size_t)2 * nblk * sizeof(unsigned)) + 256 + align256((size_t)n * 16) +
+         (size_t)2 * nblk * SB * SB * 16;
+}
+
+int* solve_status(void* work, int n) { return carve_solve(work, n).status; }
+
+int launch_lu_solve_prepare(const double* LU, long long lda, int n, void* work, hipStream_t stream) {
+  return prepare<double>(LU, lda, n, work, stream);
+}
+int launch_lu_solve_prepare_f32(const float* LU, long long lda, int n, void* work, hipStream_t stream) {
+  return prepare<float>(LU, lda, n, work, stream);
+}
+int launch_lu_solve_apply(const double* LU, long long lda, int n, const int* piv, double* x, void* work,
+                          unsigned epoch, hipStream_t stream) {
+  return apply<double>(LU, lda, n, piv, x, work, epoch, stream);
+}
+int launch_lu_solve_apply_f32(const float* LU, long long lda, int n, const int* piv, double* x, void* work,
+                              unsigned epoch, hipStream_t stream) {
+  return apply<float>(LU, lda, n, piv, x, work, epoch, stream);
 }
 
 int launch_lu_solve_fp64(const double* LU, long long lda, int n, const int* piv, double* x, void* work,
                          unsigned epoch, hipStream_t stream) {
-  const int nblk = (n + SB - 1) / SB;
-  unsigned* flags_f = static_cast<unsigned*>(work);
-  unsigned* flags_b = flags_f + nblk;
-  int* status = reinterpret_cast<int*>(flags_b + nblk);
-  double* tmp = reinterpret_cast<double*>(static_cast<char*>(work) + (((size_t)2 * nblk * sizeof(unsigned) + 64 + 255) & ~(size_t)255));
-  const int* perm = piv + n;
-  launch_k(K_PERMUTE, permute_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, x, perm, tmp, n, status);
-  launch_k(K_TRSV_LOWER, trsv_blocked_kernel<true>, dim3(nblk), dim3(256), 0, stream, LU, lda, n, tmp, flags_f, epoch,
-                     status);
-  launch_k(K_TRSV_UPPER, trsv_blocked_kernel<false>, dim3(nblk), dim3(256), 0, stream, LU, lda, n, tmp, flags_b, epoch,
-                     status);
-  if (hipMemcpyAsync(x, tmp, (size_t)n * 16, hipMemcpyDeviceToDevice, stream) != hipSuccess) return 1;
-  return hipGetLastError() == hipSuccess ? 0 : 1;
+  if (prepare<double>(LU, lda, n, work, stream)) return 1;
+  return apply<double>(LU, lda, n, piv, x, work, epoch, stream);
 }
 
 }  // namespace necgpu

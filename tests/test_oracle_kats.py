@@ -4,6 +4,8 @@ The oracle (oracle/necpp_oracle.c) is the checker for every GPU parity test,
 so it is itself checked here against nec2++'s own known-answer tests and the
 values measured through the reference's C API (tests/golden/kats.json).
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -123,3 +125,23 @@ def test_c2_grid_kat(kats):
     assert abs(z.real - k["zin"][0]) < 0.51e-4 * 10
     assert abs(z.imag - k["zin"][1]) < 0.51e-2 * 10
     assert rel(z, complex(*k["zin"])) < k["tol_rel_print"]
+
+
+def test_c5_fixture_matches_reference_print(kats):
+    # The N = 16,020 oracle fixture (tests/golden/make_c5_oracle.py) against
+    # the reference's printed Z_in (5 significant digits); the deck's source
+    # is the reference's tag 3961.
+    import json
+    k = kats["c5_grid_16020"]
+    f = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "c5_oracle.json")))
+    d = decks.grid(90, rp=None)
+    assert d["source"] == (k["source_tag"], 1)
+    assert f["n"] == len(d["wires"]) == 16020
+    z = complex(*f["zin"])
+    assert rel(z, complex(*k["zin"])) < k["tol_rel_print"]
+    cur = np.load(os.path.join(os.path.dirname(__file__), "golden", "c5_oracle_current.npy"))
+    assert cur.shape == (16020,)
+    assert abs(np.linalg.norm(cur) - f["current_l2"]) < 1e-15
+    # Z_in = V / I at the source segment (1 V source; segment numbers 1-based)
+    assert rel(z, 1.0 / cur[f["source_segment_number"] - 1]) < 1e-12
+    assert rel(complex(*f["current_src"]), cur[f["source_segment_number"] - 1]) == 0
