@@ -37,6 +37,8 @@ def parse():
     p.add_argument("--warmup", type=int, default=2)
     p.add_argument("--nn", type=int, default=41, help="grid nodes per side (41 -> N=3,280)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
+    p.add_argument("--no-mixed", action="store_true", help="skip the N=16,020 mixed-vs-FP64 leg")
+    p.add_argument("--mixed-nn", type=int, default=90, help="grid nodes per side of the mixed leg (90 -> N=16,020)")
     p.add_argument("--summarize", metavar="DIR", help="print a rocprofv3 --stats CSV summary and exit")
     p.add_argument("--summarize-pmc", nargs=2, metavar=("DIR", "COUNTER"),
                    help="print per-kernel means of one rocprofv3 --pmc counter and exit")
@@ -131,6 +133,54 @@ def copy_probe(mib):
         b.copy_(a)
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes_read": a.numel() * 8, "copy_probe_bytes_written": b.numel() * 8}))
+
+
+def solve_times(deck, precision, steps):
+    """(ms per solve, Z_in, stage ms, refinement) of the on-device pipeline."""
+    from necpp_amd import decks
+    from necpp_amd import necpp as api
+    import numpy as np
+    ctx = api.nec_create()
+    try:
+        for w in deck["wires"]:
+            api.nec_wire(ctx, *w)
+        api.nec_geometry_complete(ctx, deck["gpflag"])
+        if api.nec_gpu_set_precision(ctx, precision) != 0:
+            raise RuntimeError(api.nec_error_message())
+        api.nec_fr_card(ctx, 0, 1, deck["freq_mhz"], 0.0)
+        s, v = deck["source"], deck["voltage"]
+        api.nec_ex_card(ctx, 0, s[0], s[1], 0, v.real, v.imag, 0, 0, 0, 0)
+        if api.nec_xq_card(ctx, 0) != 0:
+            raise RuntimeError(api.nec_error_message())
+        out = np.zeros(8)
+        if api.nec_gpu_benchmark_solve(ctx, 0, steps, out.ctypes.data) != 0:
+            raise RuntimeError(api.nec_error_message())
+        ref = api.gpu_refinement(ctx) if precision == 1 else None
+        return float(out[0]), complex(out[5], out[6]), [float(x) for x in out[1:5]], ref
+    finally:
+        api.nec_delete(ctx)
+
+
+def mixed_leg(nn, steps=1):
+    """FP64 vs mixed precision on the nn x nn grid, Z_in checked against the oracle fixture."""
+    from necpp_amd import decks
+    deck = decks.grid(nn, rp=None)
+    n = len(deck["wires"])
+    ms64, z64, st64, _ = solve_times(deck, 0, steps)
+    ms32, z32, st32, (isteps, ires) = solve_times(deck, 1, steps)
+    fixture = os.path.join(ROOT, "tests", "golden", "c5_oracle.json")
+    zref = None
+    if n == 16020 and os.path.exists(fixture):
+        zref = complex(*json.load(open(fixture))["zin"])
+    return {"workload": f"wire grid {nn}x{nn} nodes, N={n}, fill+LU+solve+Z_in (no pattern)",
+            "fp64_ms_per_solve": ms64, "mixed_ms_per_solve": ms32, "speedup": ms64 / ms32,
+            "fp64_stages_ms": dict(zip(("fill", "factor", "solve", "pattern"), st64)),
+            "mixed_stages_ms": dict(zip(("fill", "factor", "solve", "pattern"), st32)),
+            "refinement_steps": isteps, "refinement_rel_residual": ires,
+            "zin_fp64": [z64.real, z64.imag], "zin_mixed": [z32.real, z32.imag],
+            "zin_mixed_vs_fp64_rel": abs(z32 - z64) / abs(z64),
+            "zin_mixed_vs_oracle_rel": None if zref is None else abs(z32 - zref) / abs(zref),
+            "zin_fp64_vs_oracle_rel": None if zref is None else abs(z64 - zref) / abs(zref)}
 
 
 def cpu_baseline(deck):
@@ -237,6 +287,8 @@ def main():
         "kernels": kernels,
         "mixed_vs_fp64": None,
     }
+    if not a.no_mixed:
+        result["mixed_vs_fp64"] = mixed_leg(a.mixed_nn)
     if not a.no_cpu:
         cb, r = cpu_baseline(deck)
         result["cpu_baseline"] = cb

@@ -32,6 +32,12 @@ long nec_gpu_segments(nec_context* ctx, double* out, int* conn, long n);
  * refinement against the FP64 matrix. */
 long nec_gpu_set_precision(nec_context* ctx, int mode);
 
+/* Mixed precision: relative residual target of the refinement (default
+ * 1e-12, i.e. |b - Z x| <= tol |b| with the FP64 Z), and, after a solve,
+ * the refinement steps taken and the final relative residual. */
+long nec_gpu_set_refine_tolerance(nec_context* ctx, double tol);
+long nec_gpu_refinement(nec_context* ctx, int* steps, double* rel_residual);
+
 /* Segment currents (amperes, complex interleaved re,im) of antenna-input
  * result freq_index: writes 2*n doubles, n = nec_gpu_segment_count(). */
 long nec_gpu_currents(nec_context* ctx, int freq_index, double* out, long n);
@@ -102,6 +108,9 @@ long necgpu_zgetrs(const double* lu, long long lda, int n, const int* piv, doubl
  * feeds a[l], b[l] (device, 64 doubles each) to v_mfma_f64_16x16x4f64 and
  * writes the 4 result registers of every lane to d (device, 256 doubles). */
 long necgpu_check_mfma_f64(const double* a, const double* b, double* d, void* stream);
+/* The same for v_mfma_f32_16x16x4f32 (64 floats in each, 256 out), the
+ * instruction of the FP32 trailing update of the mixed-precision path. */
+long necgpu_check_mfma_f32(const float* a, const float* b, float* d, void* stream);
 
 #ifdef __cplusplus
 }

@@ -240,6 +240,20 @@ long nec_gpu_segments(nec_context* ctx, double* out, int* conn, long n) {
   });
 }
 
+long nec_gpu_set_refine_tolerance(nec_context* ctx, double tol) {
+  return guarded([&] {
+    if (!(tol > 0.0)) throw NecError("refinement tolerance must be positive");
+    ctx->device().set_refine_tol(tol);
+  });
+}
+
+long nec_gpu_refinement(nec_context* ctx, int* steps, double* rel_residual) {
+  return guarded([&] {
+    *steps = ctx->device().refine_steps();
+    *rel_residual = ctx->device().refine_residual();
+  });
+}
+
 long nec_gpu_set_precision(nec_context* ctx, int mode) {
   return guarded([&] {
     if (mode != 0 && mode != 1) throw NecError("precision mode must be 0 (FP64) or 1 (mixed)");
@@ -355,6 +369,13 @@ long necgpu_check_mfma_f64(const double* a, const double* b, double* d, void* st
   return guarded([&] {
     if (necgpu::launch_mfma_probe(a, b, d, static_cast<hipStream_t>(stream)))
       necgpu::hip_check(hipGetLastError(), "necgpu_check_mfma_f64");
+  });
+}
+
+long necgpu_check_mfma_f32(const float* a, const float* b, float* d, void* stream) {
+  return guarded([&] {
+    if (necgpu::launch_mfma_probe_f32(a, b, d, static_cast<hipStream_t>(stream)))
+      necgpu::hip_check(hipGetLastError(), "necgpu_check_mfma_f32");
   });
 }
 

@@ -159,6 +159,7 @@ void Context::simulate(bool far_field_flag) {
       state_ = STRUCTURE_LOADING;
     }
     if (igox <= STRUCTURE_LOADING) {
+      device().set_precision(precision_);
       device().fill(rkh_);
       device().factor();
       times_.fill_ms = device().times().fill_ms;

@@ -60,7 +60,12 @@ class Context {
   int pattern_count() const { return static_cast<int>(patterns_.size()); }
 
   // GPU extras
-  void set_precision(int mode) { precision_ = mode; }
+  // takes effect at the next XQ / RP, which refills and refactors
+  void set_precision(int mode) {
+    if (mode != precision_) state_ = MEMORY_ALLOC;
+    precision_ = mode;
+  }
+  int precision() const { return precision_; }
   const StageTimes& last_times() const { return times_; }
   const Geometry& geometry() const { return geo_; }
   double wavelength() const { return wavelength_; }

@@ -240,12 +240,58 @@ void DeviceSolver::enqueue_factor() {
   info_.ensure(sizeof(int));
   lu_work_.ensure(lu_workspace_bytes(n));
   hip_check(hipMemsetAsync(info_.as<void>(), 0, sizeof(int), stream_), "memset");
+  if (precision_ == 1) {
+    // mixed: factor an FP32 copy; Z keeps the FP64 matrix for refinement
+    float* z32 = static_cast<float*>(Z32_.ensure((size_t)n * n * 8));
+    if (launch_to_f32(Z_.as<double>(), ld_, z32, n, n, stream_)) hip_check(hipGetLastError(), "copy launch");
+    const int rc = launch_lu_fp32(z32, n, n, piv_.as<int>(), info_.as<int>(), lu_work_.as<void>(), stream_);
+    if (rc == 2) throw NecError("LU: matrix order " + std::to_string(n) + " exceeds the panel exchange area");
+    if (rc) hip_check(hipGetLastError(), "lu launch");
+    ensure_solve_work();
+    if (launch_lu_solve_prepare_f32(z32, n, n, solve_work_.as<void>(), stream_))
+      hip_check(hipGetLastError(), "solve prepare launch");
+    return;
+  }
   const int rc = launch_lu_fp64(Z_.as<double>(), ld_, n, piv_.as<int>(), info_.as<int>(), lu_work_.as<void>(), stream_);
   if (rc == 2) throw NecError("LU: matrix order " + std::to_string(n) + " exceeds the panel exchange area");
   if (rc) hip_check(hipGetLastError(), "lu launch");
   ensure_solve_work();
   if (launch_lu_solve_prepare(Z_.as<double>(), ld_, n, solve_work_.as<void>(), stream_))
     hip_check(hipGetLastError(), "solve prepare launch");
+}
+
+void DeviceSolver::refine_solve() {
+  // x_ holds b on entry.  x = LU32 \ b, then x += LU32 \ (b - Z x) until
+  // |b - Z x| <= refine_tol_ |b|.  The norms come back to the host after
+  // every residual, so this stage synchronises once per step.
+  const int n = n_;
+  double* b = static_cast<double*>(b_.ensure((size_t)n * 16));
+  double* r = static_cast<double*>(r_.ensure((size_t)n * 16));
+  double* acc = static_cast<double*>(acc_.ensure(64));
+  const float* z32 = Z32_.as<float>();
+  hip_check(hipMemcpyAsync(b, x_.as<void>(), (size_t)n * 16, hipMemcpyDeviceToDevice, stream_), "rhs copy");
+  ++epoch_;
+  if (launch_lu_solve_apply_f32(z32, n, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(), epoch_, stream_))
+    hip_check(hipGetLastError(), "solve launch");
+  double norms[2] = {0.0, 0.0};
+  refine_steps_ = 0;
+  for (int it = 0;; ++it) {
+    if (launch_residual(Z_.as<double>(), ld_, n, x_.as<double>(), b, r, acc, it == 0, stream_))
+      hip_check(hipGetLastError(), "residual launch");
+    hip_check(hipMemcpyAsync(norms, acc, it == 0 ? 16 : 8, hipMemcpyDeviceToHost, stream_), "norms");
+    hip_check(hipStreamSynchronize(stream_), "refinement");
+    const double bn = std::sqrt(norms[1]);
+    refine_residual_ = bn > 0.0 ? std::sqrt(norms[0]) / bn : std::sqrt(norms[0]);
+    if (!(refine_residual_ > refine_tol_)) break;  // converged (or no right-hand side)
+    if (it >= kMaxRefine)
+      throw NecError("mixed-precision refinement did not reach the tolerance in " + std::to_string(kMaxRefine) +
+                     " steps (relative residual " + std::to_string(refine_residual_) + ")");
+    ++epoch_;
+    if (launch_lu_solve_apply_f32(z32, n, n, piv_.as<int>(), r, solve_work_.as<void>(), epoch_, stream_))
+      hip_check(hipGetLastError(), "solve launch");
+    if (launch_update(x_.as<double>(), r, n, stream_)) hip_check(hipGetLastError(), "update launch");
+    ++refine_steps_;
+  }
 }
 
 void DeviceSolver::ensure_solve_work() {
@@ -263,11 +309,15 @@ void DeviceSolver::enqueue_solve(const void* rhs_dev, hipMemcpyKind kind, const 
   abc_.ensure((size_t)6 * n * sizeof(double));
   curx_.ensure((size_t)n * 16);
   ensure_solve_work();
-  ++epoch_;
   hip_check(hipMemcpyAsync(x_.as<void>(), rhs_dev ? rhs_dev : rhs_host, (size_t)n * 16, kind, stream_), "rhs");
-  if (launch_lu_solve_apply(Z_.as<double>(), ld_, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(), epoch_,
-                            stream_))
-    hip_check(hipGetLastError(), "solve launch");
+  if (precision_ == 1) {
+    refine_solve();
+  } else {
+    ++epoch_;
+    if (launch_lu_solve_apply(Z_.as<double>(), ld_, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(),
+                              epoch_, stream_))
+      hip_check(hipGetLastError(), "solve launch");
+  }
   if (launch_currents(n, x_.as<double>(), segptr_.as<int>(), segbasis_.as<int>(), segabc_.as<double>(),
                       abc_.as<double>(), curx_.as<double>(), stream_))
     hip_check(hipGetLastError(), "currents launch");

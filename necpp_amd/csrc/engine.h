@@ -91,6 +91,14 @@ class DeviceSolver {
   const StageTimes& times() const { return times_; }
   // per-kernel launch times of the timed steps of the last benchmark_pipeline
   const KernelClock& kernel_clock() const { return kclock_; }
+  // 0: FP64 LU.  1: mixed (FP32 LU of a copy of Z + FP64 iterative
+  // refinement against Z until |b - Z x| <= refine_tol |b|).
+  void set_precision(int mode) { precision_ = mode; }
+  int precision() const { return precision_; }
+  void set_refine_tol(double tol) { refine_tol_ = tol; }
+  // refinement steps and final relative residual of the last mixed solve
+  int refine_steps() const { return refine_steps_; }
+  double refine_residual() const { return refine_residual_; }
   hipStream_t stream() const { return stream_; }
 
  private:
@@ -114,10 +122,17 @@ class DeviceSolver {
   hipEvent_t ev_[6] = {};
   StageTimes times_;
   KernelClock kclock_;
+  int precision_ = 0;
+  double refine_tol_ = 1e-12;
+  int refine_steps_ = 0;
+  double refine_residual_ = 0.0;
+  static constexpr int kMaxRefine = 30;
+  void refine_solve();
 
   DevBuf seg_, ekind_, colptr_, colj_, colabc_, segptr_, segbasis_, segabc_;
   DevBuf Z_, E_, piv_, info_, lu_work_, x_, abc_, curx_, solve_work_, rhs_, src_, zin_;
   DevBuf trig_, pat_part_, pat_out_, pat_sense_;
+  DevBuf Z32_, b_, r_, acc_;  // mixed path: FP32 factors, right-hand side, residual, norms
 };
 
 }  // namespace necgpu

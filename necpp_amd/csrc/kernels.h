@@ -27,6 +27,8 @@ enum KernelKind {
   K_ZIN,
   K_PATTERN_PARTIAL,
   K_PATTERN_FINISH,
+  K_REFINE,
+  K_RESIDUAL,
   K_NKINDS
 };
 const char* kernel_kind_name(int kind);
@@ -86,8 +88,13 @@ int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int u
 size_t lu_workspace_bytes(int n);
 int* lu_status(void* work, int n);
 int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream);
+// The same factorisation of an FP32 matrix (the mixed-precision path): FP32
+// storage and trailing update on v_mfma_f32_16x16x4f32; same pivot rule.
+// work: lu_workspace_bytes(n) is large enough for either.
+int launch_lu_fp32(float* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream);
 // one-wave check of the FP64 MFMA operand / result lane layout (64 a, 64 b in, 256 d out)
 int launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t stream);
+int launch_mfma_probe_f32(const float* a, const float* b, float* d, hipStream_t stream);
 
 // solve.hip: x <- A^-1 x with the LU factors of launch_lu_fp64 (or the FP32
 // factors of the mixed path; x stays FP64).  work: device scratch of
@@ -105,6 +112,13 @@ int launch_lu_solve_apply_f32(const float* LU, long long lda, int n, const int* 
 // prepare + apply
 int launch_lu_solve_fp64(const double* LU, long long lda, int n, const int* piv, double* x, void* work,
                          unsigned epoch, hipStream_t stream);
+
+// refine.hip: the mixed path's FP32 copy of Z, FP64 residual r = b - Z x
+// (acc[0] = |r|^2, acc[1] = |b|^2 when with_b) and update x += d.
+int launch_to_f32(const double* Z, long long ldz, float* Z32, long long ld32, int n, hipStream_t stream);
+int launch_residual(const double* Z, long long ldz, int n, const double* x, const double* b, double* r, double* acc,
+                    bool with_b, hipStream_t stream);
+int launch_update(double* x, const double* d, int n, hipStream_t stream);
 
 // current.hip: basis amplitudes x (complex) -> per-segment coefficients
 // abc (6*n: air, aii, bir, bii, cir, cii) and segment-centre currents curx.

@@ -32,9 +32,16 @@ namespace {
 constexpr int NB = 64;
 constexpr int PANEL_THREADS = 512;
 constexpr int GMAX = 256;     // most cooperating panel workgroups
-constexpr int RMAX = 112;     // most panel rows per workgroup (LDS: RMAX*(NB+1)*16 B)
+constexpr int RMAX = 112;     // most FP64 panel rows per workgroup (LDS: RMAX*(NB+1)*16 B)
+
+// Panel rows per workgroup for element type T (FP32 rows take half the LDS).
+template <typename T>
+constexpr int rmax_of() {
+  return sizeof(T) == 8 ? RMAX : 2 * RMAX;
+}
 
 typedef double double4_t __attribute__((ext_vector_type(4)));
+typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // Exchange area of the cooperative panel.  Every slot is written once per
 // panel, so a reader knows a value is current when it differs from the
@@ -144,7 +151,8 @@ __global__ void lu_init_kernel(int* perm, int n, int* status, char* xbase, int G
 
 // Factor panel columns [k0, k0+nb) over rows [k0, n) with G workgroups; the
 // active workgroup g is launch block g*stride.  Rows owned by g:
-// [k0 + g*R, min(n, k0 + (g+1)*R)), at most RMAX, one thread per row.
+// [k0 + g*R, min(n, k0 + (g+1)*R)), at most rmax_of<T>(), one thread per
+// row.  Rows are held in LDS as T; arithmetic is FP64.
 //
 // Column c (k = k0 + c):
 //  A  reduction of the column-c candidates (|z| from step E): wave max,
@@ -157,21 +165,24 @@ __global__ void lu_init_kernel(int* perm, int n, int* status, char* xbase, int G
 //  D  the pivot row is fetched; rows k and p are exchanged.
 //  E  one thread per row: scale column c, update column c+1 only, and form
 //     the column-(c+1) candidate; the update of columns c+2.. is left to B.
-__global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restrict__ A, long long lda, int n, int k0,
+template <typename T>
+__global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(T* __restrict__ A, long long lda, int n, int k0,
                                                                   int nb, int G, int R, int stride, int* piv, int* perm,
                                                                   int* info, char* xbase, int Gcap, int parity,
                                                                   int Gnext, int* status) {
-  extern __shared__ double lds[];
+  extern __shared__ double lds_raw[];
+  constexpr int RMAX_T = rmax_of<T>();
+  constexpr int NRW = (RMAX_T + 63) / 64;  // waves that hold rows
   if (blockIdx.x % stride != 0) return;
   constexpr int NW = PANEL_THREADS / 64;  // waves
   const int g = blockIdx.x / stride;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const int LDR = NB + 1;  // padded complex row length in LDS
-  double* rows = lds;                         // [R][LDR] complex
-  double* su = lds + 2 * (size_t)RMAX * LDR;  // current pivot row, nb complex
-  __shared__ double s_best[2];
-  __shared__ int s_idx[2];
+  T* rows = reinterpret_cast<T*>(lds_raw);  // [R][LDR] complex
+  T* su = rows + 2 * (size_t)RMAX_T * LDR;   // current pivot row, nb complex
+  __shared__ double s_best[NRW];
+  __shared__ int s_idx[NRW];
   __shared__ int s_win[2];     // winner row, winner group
   __shared__ double s_piv[4];  // pivot row, columns c and c+1
   __shared__ double s_dg[4];   // row k, columns c and c+1
@@ -183,7 +194,7 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
   const int nrow = max(0, r1 - r0);
   {
     // every load of a thread in flight at once, then the LDS stores
-    constexpr int PER = (RMAX * NB + PANEL_THREADS - 1) / PANEL_THREADS;
+    constexpr int PER = (RMAX_T * NB + PANEL_THREADS - 1) / PANEL_THREADS;
     double vr[PER], vi[PER];
 #pragma unroll
     for (int u = 0; u < PER; ++u) {
@@ -191,7 +202,7 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
       vr[u] = vi[u] = 0.0;
       if (e < nrow * nb) {
         const int r = e / nb, c = e % nb;
-        const double* v = A + 2 * ((size_t)(r0 + r) * lda + k0 + c);
+        const T* v = A + 2 * ((size_t)(r0 + r) * lda + k0 + c);
         vr[u] = v[0];
         vi[u] = v[1];
       }
@@ -201,8 +212,8 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
       const int e = t + PANEL_THREADS * u;
       if (e < nrow * nb) {
         const int r = e / nb, c = e % nb;
-        rows[2 * (r * LDR + c)] = canonical(vr[u]);
-        rows[2 * (r * LDR + c) + 1] = canonical(vi[u]);
+        rows[2 * (r * LDR + c)] = static_cast<T>(canonical(vr[u]));
+        rows[2 * (r * LDR + c) + 1] = static_cast<T>(canonical(vi[u]));
       }
     }
   }
@@ -213,8 +224,8 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
   bool has = t < nrow;
   for (int c = 0; c < nb; ++c) {
     const int k = k0 + c;
-    // A. candidate reduction (rows are held by waves 0 and 1 only)
-    if (wave < 2) {
+    // A. candidate reduction (rows are held by waves 0 .. NRW-1 only)
+    if (wave < NRW) {
       double m = has ? best : -1.0;
       for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
       const unsigned long long hit = __ballot(has && best == m);
@@ -224,16 +235,15 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
       }
     }
     __syncthreads();
-    int bidx;
-    double bval;
-    if (s_idx[0] != INT_MAX && !(s_idx[1] != INT_MAX && s_best[1] > s_best[0])) {
-      bval = s_best[0];
-      bidx = s_idx[0];
-    } else {
-      bval = s_best[1];
-      bidx = s_idx[1];
-    }
-    if (bidx == INT_MAX) bval = -1.0;
+    // first maximum over the waves in row order
+    int bidx = INT_MAX;
+    double bval = -1.0;
+#pragma unroll
+    for (int w = 0; w < NRW; ++w)
+      if (s_idx[w] != INT_MAX && (bidx == INT_MAX || s_best[w] > bval)) {
+        bval = s_best[w];
+        bidx = s_idx[w];
+      }
     const int rcand = (bidx != INT_MAX) ? bidx - r0 : -1;  // local candidate row
     const int rk = (k >= r0 && k < r1) ? k - r0 : -1;       // local row k
     // columns c+1.. still carry the pending update of column c-1 with
@@ -249,8 +259,8 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
           if (r < 0 || (which == 1 && r == rcand)) continue;
           const zc l = zmk(rows[2 * (r * LDR + c - 1)], rows[2 * (r * LDR + c - 1) + 1]);
           const zc prod = zmul(l, u);
-          rows[2 * (r * LDR + lane)] -= prod.re;
-          rows[2 * (r * LDR + lane) + 1] -= prod.im;
+          rows[2 * (r * LDR + lane)] = static_cast<T>(rows[2 * (r * LDR + lane)] - prod.re);
+          rows[2 * (r * LDR + lane) + 1] = static_cast<T>(rows[2 * (r * LDR + lane) + 1] - prod.im);
         }
       }
       // Slots go straight to the device coherence point (agent-scope
@@ -404,8 +414,8 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
           const int rr = r + q * (NW - 1);
           if (on[q]) {
             const zc prod = zmul(zmk(lr[q], li[q]), u);
-            rows[2 * (rr * LDR + pend_lo + lane)] = ar[q] - prod.re;
-            rows[2 * (rr * LDR + pend_lo + lane) + 1] = ai[q] - prod.im;
+            rows[2 * (rr * LDR + pend_lo + lane)] = static_cast<T>(ar[q] - prod.re);
+            rows[2 * (rr * LDR + pend_lo + lane) + 1] = static_cast<T>(ai[q] - prod.im);
           }
         }
       }
@@ -450,7 +460,7 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
     has = false;
     const int r = t;
     if (r < nrow && r0 + r > k) {
-      double* v = rows + 2 * (r * LDR + c);
+      T* v = rows + 2 * (r * LDR + c);
       const bool is_p = (r0 + r == p);
       double lr = is_p ? s_dg[0] : v[0];
       double li = is_p ? s_dg[1] : v[1];
@@ -464,15 +474,16 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
           li = (ai - (ar * ratio)) / denom;
         }
       }
-      v[0] = lr;
-      v[1] = li;
+      v[0] = static_cast<T>(lr);
+      v[1] = static_cast<T>(li);
       if (next) {
-        const zc prod = zmul(zmk(lr, li), u1);
+        // the multiplier as stored (FP32 rows round it)
+        const zc prod = zmul(zmk(v[0], v[1]), u1);
         const double nr = (is_p ? s_dg[2] : v[2]) - prod.re;
         const double ni = (is_p ? s_dg[3] : v[3]) - prod.im;
-        v[2] = nr;
-        v[3] = ni;
-        best = hypot(nr, ni);
+        v[2] = static_cast<T>(nr);
+        v[3] = static_cast<T>(ni);
+        best = hypot(static_cast<double>(v[2]), static_cast<double>(v[3]));
         has = true;
       }
     }
@@ -483,26 +494,26 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
         fsr = poll(&fsrc[2 * t], status);
         fsi = poll(&fsrc[2 * t + 1], status);
       }
-      su[2 * t] = fsr;
-      su[2 * t + 1] = fsi;
+      su[2 * t] = static_cast<T>(fsr);
+      su[2 * t + 1] = static_cast<T>(fsi);
       if (p != k && rk >= 0) {
-        rows[2 * (rk * LDR + t)] = fsr;
-        rows[2 * (rk * LDR + t) + 1] = fsi;
+        rows[2 * (rk * LDR + t)] = static_cast<T>(fsr);
+        rows[2 * (rk * LDR + t) + 1] = static_cast<T>(fsi);
       }
       if (mine_p && t != c && !(next && t == c + 1)) {
         if (is_sentinel(fpr) || is_sentinel(fpi)) {
           fpr = poll(&X.diag[(size_t)c * 2 * NB + 2 * t], status);
           fpi = poll(&X.diag[(size_t)c * 2 * NB + 2 * t + 1], status);
         }
-        rows[2 * ((p - r0) * LDR + t)] = fpr;
-        rows[2 * ((p - r0) * LDR + t) + 1] = fpi;
+        rows[2 * ((p - r0) * LDR + t)] = static_cast<T>(fpr);
+        rows[2 * ((p - r0) * LDR + t) + 1] = static_cast<T>(fpi);
       }
     }
     __syncthreads();
   }
   for (int e = t; e < nrow * nb; e += PANEL_THREADS) {
     const int r = e / nb, c = e % nb;
-    double* v = A + 2 * ((size_t)(r0 + r) * lda + k0 + c);
+    T* v = A + 2 * ((size_t)(r0 + r) * lda + k0 + c);
     v[0] = rows[2 * (r * LDR + c)];
     v[1] = rows[2 * (r * LDR + c) + 1];
   }
@@ -516,7 +527,8 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(double* __restr
 // every move is done in one gather: all touched rows of the workgroup's 32
 // columns are read into registers, then written to their destinations.
 constexpr int LASWP_COLS = 32;
-__global__ void __launch_bounds__(256) lu_laswp_kernel(double* __restrict__ A, long long lda, int k0, int nb,
+template <typename T>
+__global__ void __launch_bounds__(256) lu_laswp_kernel(T* __restrict__ A, long long lda, int k0, int nb,
                                                        const int* __restrict__ piv, int c0, int c1, int c2, int c3) {
   __shared__ int s_dst[2 * NB], s_src[2 * NB];
   __shared__ int s_moves;
@@ -586,13 +598,13 @@ __global__ void __launch_bounds__(256) lu_laswp_kernel(double* __restrict__ A, l
   // at most 2 nb = 128 moves: 16 per row group, loads issued together
   {
     constexpr int RG = 256 / LASWP_COLS;
-    double vr[2 * NB / RG], vi[2 * NB / RG];
+    T vr[2 * NB / RG], vi[2 * NB / RG];
 #pragma unroll
     for (int u = 0; u < 2 * NB / RG; ++u) {
       const int m = rg + RG * u;
-      vr[u] = vi[u] = 0.0;
+      vr[u] = vi[u] = T(0);
       if (on && m < moves) {
-        const double* v = A + 2 * ((size_t)s_src[m] * lda + col);
+        const T* v = A + 2 * ((size_t)s_src[m] * lda + col);
         vr[u] = v[0];
         vi[u] = v[1];
       }
@@ -604,7 +616,7 @@ __global__ void __launch_bounds__(256) lu_laswp_kernel(double* __restrict__ A, l
     for (int u = 0; u < 2 * NB / RG; ++u) {
       const int m = rg + RG * u;
       if (on && m < moves) {
-        double* v = A + 2 * ((size_t)s_dst[m] * lda + col);
+        T* v = A + 2 * ((size_t)s_dst[m] * lda + col);
         v[0] = vr[u];
         v[1] = vi[u];
       }
@@ -620,7 +632,8 @@ __global__ void __launch_bounds__(256) lu_laswp_kernel(double* __restrict__ A, l
 // with the 16 x 16 complex products as 4 real MFMAs (4M form).  Operand
 // layout of v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16 k, B[k][j] from
 // lane j + 16 k; result register v of lane l holds D[l / 16 + 4 v][l % 16].
-__global__ void __launch_bounds__(256) lu_trsm_kernel(double* __restrict__ A, long long lda, int cend, int k0, int nb,
+template <typename T>
+__global__ void __launch_bounds__(256) lu_trsm_kernel(T* __restrict__ A, long long lda, int cend, int k0, int nb,
                                                       int cbeg) {
   __shared__ double sL[NB * NB * 2];       // L11[r][q], complex, zero outside the strict lower part
   __shared__ double sX[NB * 64 * 2];       // X[r][col], complex
@@ -637,7 +650,7 @@ __global__ void __launch_bounds__(256) lu_trsm_kernel(double* __restrict__ A, lo
       const int r = e / NB, q = e % NB;
       vr[u] = vi[u] = 0.0;
       if (r < nb && q < r) {
-        const double* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + q);
+        const T* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + q);
         vr[u] = v[0];
         vi[u] = v[1];
       }
@@ -660,7 +673,7 @@ __global__ void __launch_bounds__(256) lu_trsm_kernel(double* __restrict__ A, lo
       const int r = 16 * b + lk + 4 * v;
       double xr = 0.0, xi = 0.0;
       if (on && r < nb) {
-        const double* p = A + 2 * ((size_t)(k0 + r) * lda + col);
+        const T* p = A + 2 * ((size_t)(k0 + r) * lda + col);
         xr = p[0];
         xi = p[1];
       }
@@ -740,9 +753,9 @@ __global__ void __launch_bounds__(256) lu_trsm_kernel(double* __restrict__ A, lo
       sX[2 * (r * 64 + xc)] = xr4[v];
       sX[2 * (r * 64 + xc) + 1] = xi4[v];
       if (on && r < nb) {
-        double* p = A + 2 * ((size_t)(k0 + r) * lda + col);
-        p[0] = xr4[v];
-        p[1] = xi4[v];
+        T* p = A + 2 * ((size_t)(k0 + r) * lda + col);
+        p[0] = static_cast<T>(xr4[v]);
+        p[1] = static_cast<T>(xi4[v]);
       }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
@@ -865,6 +878,114 @@ __global__ void __launch_bounds__(256) zgemm_update_kernel(double* __restrict__ 
   }
 }
 
+// The mixed-precision path's FP32 trailing update: zgemm_update_kernel with
+// FP32 operands on v_mfma_f32_16x16x4f32 (FP32 accumulation), whose result
+// register v of lane l holds D[4 (l / 16) + v][l % 16].
+__global__ void __launch_bounds__(256) zgemm_update_f32_kernel(float* __restrict__ A, long long lda, int m0, int n0, int M,
+                                                           int N, int k0, int K) {
+  __shared__ float sAr[GK * 64], sAi[GK * 64];  // [k][row]
+  __shared__ float sBr[GK * 64], sBi[GK * 64];  // [k][col]
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  const int tiles_x = (N + 63) / 64;
+  const int ntiles = tiles_x * ((M + 63) / 64);
+  const int G = gridDim.x;
+  // XCD-contiguous numbering (dispatch places workgroup b on XCD b % 8)
+  const int per = G / 8, extra = G % 8, x = blockIdx.x % 8, y = blockIdx.x / 8;
+  const int logical = (x < extra) ? x * (per + 1) + y : extra * (per + 1) + (x - extra) * per + y;
+  const int nchunks = (K + GK - 1) / GK;
+  // loader coordinates: A chunk 64 rows x 16 k (thread: row t/4, k 4*(t%4)..+4)
+  const int ar_ = t >> 2, ak = (t & 3) * 4;
+  // B chunk 16 k x 64 cols (thread: k t/16, cols 4*(t%16)..+4)
+  const int bq = t >> 4, bc = (t & 15) * 4;
+  for (int tile = logical; tile < ntiles; tile += G) {
+    const int row0 = (tile / tiles_x) * 64, col0 = (tile % tiles_x) * 64;
+    float pa[8], pb[8];
+    auto fetch = [&](int ch) {
+      const int kb = ch * GK;
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int q = kb + ak + j;
+        pa[2 * j] = pa[2 * j + 1] = 0.0f;
+        if (row0 + ar_ < M && q < K) {
+          const float* v = A + 2 * ((size_t)(m0 + row0 + ar_) * lda + k0 + q);
+          pa[2 * j] = v[0];
+          pa[2 * j + 1] = v[1];
+        }
+        const int qb = kb + bq;
+        pb[2 * j] = pb[2 * j + 1] = 0.0f;
+        if (col0 + bc + j < N && qb < K) {
+          const float* v = A + 2 * ((size_t)(k0 + qb) * lda + n0 + col0 + bc + j);
+          pb[2 * j] = v[0];
+          pb[2 * j + 1] = v[1];
+        }
+      }
+    };
+    float4_t accr[2][2], acci[2][2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        accr[a][b] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+        acci[a][b] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+      }
+    fetch(0);
+    for (int ch = 0; ch < nchunks; ++ch) {
+      __syncthreads();  // previous chunk's operands are no longer read
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        sAr[(ak + j) * 64 + ar_] = pa[2 * j];
+        sAi[(ak + j) * 64 + ar_] = pa[2 * j + 1];
+        sBr[bq * 64 + bc + j] = pb[2 * j];
+        sBi[bq * 64 + bc + j] = pb[2 * j + 1];
+      }
+      __syncthreads();
+      if (ch + 1 < nchunks) fetch(ch + 1);
+#pragma unroll
+      for (int kk = 0; kk < GK; kk += 4) {
+        const int q = kk + lk;
+        float xr[2], xi[2], yr[2], yi[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) {
+          xr[a] = sAr[q * 64 + wr * 32 + a * 16 + li];
+          xi[a] = sAi[q * 64 + wr * 32 + a * 16 + li];
+        }
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          yr[b] = sBr[q * 64 + wc * 32 + b * 16 + li];
+          yi[b] = sBi[q * 64 + wc * 32 + b * 16 + li];
+        }
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[a], yr[b], accr[a][b], 0, 0, 0);
+            accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[a], -yi[b], accr[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[a], yi[b], acci[a][b], 0, 0, 0);
+            acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[a], yr[b], acci[a][b], 0, 0, 0);
+          }
+      }
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+          // result register v of lane l holds D[4 (l / 16) + v][l % 16]
+          const int i = row0 + wr * 32 + a * 16 + 4 * lk + v;
+          const int j = col0 + wc * 32 + b * 16 + li;
+          if (i < M && j < N) {
+            float* c = A + 2 * ((size_t)(m0 + i) * lda + n0 + j);
+            c[0] -= accr[a][b][v];
+            c[1] -= acci[a][b][v];
+          }
+        }
+  }
+}
+
 // One wave: lane l feeds a[l] as the A operand and b[l] as the B operand of
 // v_mfma_f64_16x16x4f64 and writes its four result registers to d[4*l..].
 __global__ void mfma_probe_kernel(const double* a, const double* b, double* d) {
@@ -874,8 +995,17 @@ __global__ void mfma_probe_kernel(const double* a, const double* b, double* d) {
   for (int v = 0; v < 4; ++v) d[4 * l + v] = acc[v];
 }
 
+// The same for v_mfma_f32_16x16x4f32.
+__global__ void mfma_probe_f32_kernel(const float* a, const float* b, float* d) {
+  const int l = threadIdx.x;
+  float4_t acc = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+  acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a[l], b[l], acc, 0, 0, 0);
+  for (int v = 0; v < 4; ++v) d[4 * l + v] = acc[v];
+}
+
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
+// exchange-area capacity: the FP64 panel's group count, the larger one
 int gcap_for(int n) { return std::max(1, (n + RMAX - 1) / RMAX); }
 
 LuWork carve(void* work, int n) {
@@ -941,12 +1071,21 @@ int launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t s
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream) {
+int launch_mfma_probe_f32(const float* a, const float* b, float* d, hipStream_t stream) {
+  hipLaunchKernelGGL(mfma_probe_f32_kernel, dim3(1), dim3(64), 0, stream, a, b, d);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+namespace {
+
+template <typename T>
+int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream) {
+  constexpr int RMAX_T = rmax_of<T>();
   static bool attr_set = false;
   static int cu_count = 0;
-  const size_t lds_bytes = (size_t)(2 * RMAX * (NB + 1) + 2 * NB) * sizeof(double);
+  const size_t lds_bytes = (size_t)(2 * RMAX_T * (NB + 1) + 2 * NB) * sizeof(T);
   if (!attr_set) {
-    if (hipFuncSetAttribute(reinterpret_cast<const void*>(lu_panel_kernel),
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(lu_panel_kernel<T>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
       return 1;
     int dev = 0;
@@ -958,7 +1097,7 @@ int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* w
   if (n <= 0) return 0;
   // every active panel workgroup needs its own CU at the same time (one per
   // CU by LDS), so the first (largest) panel bounds n
-  const int G0 = (n + RMAX - 1) / RMAX;
+  const int G0 = (n + RMAX_T - 1) / RMAX_T;
   if (G0 > GMAX || G0 > cu_count) return 2;
   int* perm = piv + n;
   const LuWork w = carve(work, n);
@@ -971,31 +1110,35 @@ int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* w
   auto panel = [&](int k0, hipStream_t s) {
     const int nb = std::min(NB, n - k0);
     const int rows = n - k0;
-    const int G = (rows + RMAX - 1) / RMAX;
+    const int G = (rows + RMAX_T - 1) / RMAX_T;
     const int R = (rows + G - 1) / G;
     // Packing the workgroups onto one XCD (stride 8) measured slower than
     // spreading them, so the default is 1; NECGPU_PANEL_STRIDE overrides.
     int stride = 1;
     if (forced > 0 && forced * G <= cu_count) stride = forced;
-    const int Gnext = (k0 + nb < n) ? (n - k0 - nb + RMAX - 1) / RMAX : 0;
-    launch_k(K_LU_PANEL, lu_panel_kernel, dim3(G * stride), dim3(PANEL_THREADS), (uint32_t)lds_bytes, s, A, lda, n,
+    const int Gnext = (k0 + nb < n) ? (n - k0 - nb + RMAX_T - 1) / RMAX_T : 0;
+    launch_k(K_LU_PANEL, lu_panel_kernel<T>, dim3(G * stride), dim3(PANEL_THREADS), (uint32_t)lds_bytes, s, A, lda, n,
              k0, nb, G, R, stride, piv, perm, info, w.xbase, w.Gcap, (k0 / NB) & 1, Gnext, w.status);
   };
   auto laswp = [&](int k0, int nb, int c0, int c1, int c2, int c3, hipStream_t s) {
     const int cols = (c1 - c0) + (c3 - c2);
     if (cols > 0)
-      launch_k(K_LU_LASWP, lu_laswp_kernel, dim3((cols + LASWP_COLS - 1) / LASWP_COLS), dim3(256), 0, s, A, lda, k0, nb,
+      launch_k(K_LU_LASWP, lu_laswp_kernel<T>, dim3((cols + LASWP_COLS - 1) / LASWP_COLS), dim3(256), 0, s, A, lda, k0, nb,
                (const int*)piv, c0, c1, c2, c3);
   };
   auto update = [&](int k0, int nb, int cbeg, int cend, int max_groups, hipStream_t s) {
     // trailing rows [k0+nb, n), columns [cbeg, cend)
     const int cols = cend - cbeg, rows = n - k0 - nb;
     if (cols <= 0) return;
-    launch_k(K_LU_TRSM, lu_trsm_kernel, dim3((cols + 63) / 64), dim3(256), 0, s, A, lda, cend, k0, nb, cbeg);
+    launch_k(K_LU_TRSM, lu_trsm_kernel<T>, dim3((cols + 63) / 64), dim3(256), 0, s, A, lda, cend, k0, nb, cbeg);
     if (rows <= 0) return;
     const int tiles = ((rows + 63) / 64) * ((cols + 63) / 64);
     const int grid = std::max(1, std::min(tiles, max_groups));
-    launch_k(K_ZGEMM, zgemm_update_kernel, dim3(grid), dim3(256), 0, s, A, lda, k0 + nb, cbeg, rows, cols, k0, nb);
+    if constexpr (sizeof(T) == 8)
+      launch_k(K_ZGEMM, zgemm_update_kernel, dim3(grid), dim3(256), 0, s, A, lda, k0 + nb, cbeg, rows, cols, k0, nb);
+    else
+      launch_k(K_ZGEMM, zgemm_update_f32_kernel, dim3(grid), dim3(256), 0, s, A, lda, k0 + nb, cbeg, rows, cols, k0,
+               nb);
   };
   // Right-looking LU with a look-ahead of one panel.  Per panel p at k0:
   //   caller stream: [wait for panel p] row exchanges, U12 and trailing
@@ -1030,11 +1173,21 @@ int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* w
       if (hipEventRecord(la->panel_done, la->side) != hipSuccess) return 1;
     }
     laswp(k0, nb, 0, k0, far, n, stream);
-    const int Gnext = (n - next + RMAX - 1) / RMAX;
+    const int Gnext = (n - next + RMAX_T - 1) / RMAX_T;
     update(k0, nb, far, n, la ? std::max(1, occ * (cu_count - Gnext)) : INT_MAX, stream);
     if (!la) panel(next, stream);
   }
   return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+}  // namespace
+
+int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream) {
+  return launch_lu<double>(A, lda, n, piv, info, work, stream);
+}
+
+int launch_lu_fp32(float* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream) {
+  return launch_lu<float>(A, lda, n, piv, info, work, stream);
 }
 
 }  // namespace necgpu

@@ -60,6 +60,8 @@ _SIGS = [
     ("nec_gpu_segment_count", _L, [_P]),
     ("nec_gpu_segments", _L, [_P, _P, _P, _L]),
     ("nec_gpu_set_precision", _L, [_P, _I]),
+    ("nec_gpu_set_refine_tolerance", _L, [_P, _D]),
+    ("nec_gpu_refinement", _L, [_P, _P, _P]),
     ("nec_gpu_currents", _L, [_P, _I, _P, _L]),
     ("nec_gpu_stage_times", _L, [_P, _P]),
     ("nec_gpu_benchmark_solve", _L, [_P, _I, _I, _P]),
@@ -71,6 +73,7 @@ _SIGS = [
     ("necgpu_zgetrf_workspace", _LL, [_I]),
     ("necgpu_zgetrf", _L, [_P, _LL, _I, _P, _P, _P, _P]),
     ("necgpu_check_mfma_f64", _L, [_P, _P, _P, _P]),
+    ("necgpu_check_mfma_f32", _L, [_P, _P, _P, _P]),
     ("necgpu_zgetrs_workspace", _LL, [_I]),
     ("necgpu_zgetrs", _L, [_P, _LL, _I, _P, _P, _P, _U, _P]),
 ]
@@ -169,6 +172,15 @@ def gpu_kernel_times(ctx):
     if k < 0:
         raise RuntimeError(nec_error_message())
     return {lib().nec_gpu_kernel_name(i).decode(): (int(out[2 * i]), float(out[2 * i + 1])) for i in range(k)}
+
+
+def gpu_refinement(ctx):
+    """(steps, relative residual) of the last mixed-precision solve."""
+    steps = ctypes.c_int(0)
+    res = ctypes.c_double(0.0)
+    if lib().nec_gpu_refinement(ctx, ctypes.byref(steps), ctypes.byref(res)) != 0:
+        raise RuntimeError(nec_error_message())
+    return steps.value, res.value
 
 
 def gpu_info():

@@ -88,6 +88,32 @@ def test_mfma_f64_lane_layout():
     assert found == expect
 
 
+def test_mfma_f32_lane_layout():
+    # The mixed-precision trailing update assumes, for v_mfma_f32_16x16x4f32:
+    # A[i][k] from lane i + 16k, B[k][j] from lane j + 16k, and result
+    # register v of lane l holding D[4*(l/16) + v][l % 16].
+    torch = _torch()
+    rng = np.random.default_rng(12)
+    A = rng.standard_normal((16, 4)).astype(np.float32)
+    B = rng.standard_normal((4, 16)).astype(np.float32)
+    lanes = np.arange(64)
+    a = torch.from_numpy(A[lanes % 16, lanes // 16].copy()).to("cuda")
+    b = torch.from_numpy(B[lanes // 16, lanes % 16].copy()).to("cuda")
+    d = torch.zeros(256, dtype=torch.float32, device="cuda")
+    assert api.necgpu_check_mfma_f32(a.data_ptr(), b.data_ptr(), d.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    got = d.cpu().numpy().reshape(64, 4).astype(np.float64)
+    D = A.astype(np.float64) @ B.astype(np.float64)
+    found = {}
+    for l in range(64):
+        for v in range(4):
+            hit = np.argwhere(np.abs(D - got[l, v]) < 1e-5)
+            found[(l, v)] = tuple(int(x) for x in hit[0]) if len(hit) == 1 else None
+    print("MFMA f32 16x16x4 layout (lane, reg) -> (i, j):", {k: found[k] for k in list(found)[:20]})
+    expect = {(l, v): (4 * (l // 16) + v, l % 16) for l in range(64) for v in range(4)}
+    assert found == expect
+
+
 @pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 127, 130, 257, 300])
 def test_zgetrf_zgetrs_random(n):
     torch = _torch()
