@@ -8,7 +8,8 @@
 // inverted (one wave per block, all blocks at once).  Each triangular sweep
 // is then one launch of one workgroup per 64-row block: block b subtracts
 // the products with every finished block as soon as that block's flag is
-// raised (flags carry an epoch, so no reset is needed between calls), then
+// raised (flags carry an epoch, so no reset is needed between calls; one
+// poll of 64 consecutive flags takes every block finished so far), then
 // applies its diagonal block's inverse (a 64 x 64 matrix-vector product
 // instead of 64 dependent steps) and raises its flag.  The block that
 // finishes last before b (the neighbour) and the inverse are staged in LDS
@@ -26,24 +27,6 @@ constexpr int SB = 64;
 
 __device__ __forceinline__ double ld(const double* p) { return *p; }
 __device__ __forceinline__ double ld(const float* p) { return static_cast<double>(*p); }
-
-__device__ __forceinline__ unsigned load_flag(const unsigned* f) {
-  return __hip_atomic_load(f, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-
-// Bounded wait of thread 0 for flags[c] == epoch; a timeout is sticky so
-// every later wait returns at once.
-__device__ void wait_flag(const unsigned* flags, int c, unsigned epoch, int* status) {
-  long spins = 0;
-  while (load_flag(flags + c) != epoch) {
-    if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-    if (++spins > (1L << 22)) {
-      __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      break;
-    }
-    __builtin_amdgcn_s_sleep(1);
-  }
-}
 
 // Inverse of diagonal block b of L (blockIdx.y == 0, unit lower) or U
 // (blockIdx.y == 1, upper with its diagonal), padded to 64 x 64 with the
@@ -109,8 +92,21 @@ __global__ void __launch_bounds__(64) diag_inverse_kernel(const TF* __restrict__
   for (int e = j; e < SB * SB * 2; e += 64) dst[e] = sX[e];
 }
 
+// x values and flags travel through the device coherence point: written
+// with agent-scope atomic stores (write-through), the flag after the data
+// stores have completed, and read with agent-scope atomic loads, so no
+// cache write-back or invalidate is needed.
+__device__ __forceinline__ void xstore(double* p, double v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double xload(const double* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 // One triangular sweep.  lower: out[r] = y_r with L y = P rhs (rhs read
-// through perm).  upper: out[r] = x_r with U x = rhs.
+// through perm).  upper: out[r] = x_r with U x = rhs.  Blocks finish in
+// order, so one poll of 64 consecutive flags tells how many more finished
+// blocks can be taken at once.
 template <typename TF, bool lower>
 __global__ void __launch_bounds__(256) trsv_kernel(const TF* __restrict__ LU, long long lda, int n,
                                                    const int* __restrict__ perm, const double* rhs, double* out,
@@ -120,6 +116,7 @@ __global__ void __launch_bounds__(256) trsv_kernel(const TF* __restrict__ LU, lo
   __shared__ double sNb[SB * SB * 2];    // the neighbour block
   __shared__ double sPart[4][SB * 2];
   __shared__ double sAcc[SB * 2];
+  __shared__ int s_ready;
   const int nblk = (n + SB - 1) / SB;
   const int b = lower ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
   const int nb = lower ? b - 1 : b + 1;  // neighbour block (last to finish)
@@ -150,32 +147,72 @@ __global__ void __launch_bounds__(256) trsv_kernel(const TF* __restrict__ LU, lo
     }
   }
   double accr = 0.0, acci = 0.0;
-  // blocks other than the neighbour, in the order they finish
+  // blocks other than the neighbour, taken in finished batches
   const int others = lower ? max(0, b - 1) : max(0, nblk - 2 - b);
-  for (int step = 0; step < others; ++step) {
-    const int c = lower ? step : nblk - 1 - step;
-    if (t == 0) wait_flag(flags, c, epoch, status);
-    __syncthreads();
-    const int c0 = c * SB;
-    const int cols = min(SB, n - c0);
-    if (r < rows) {
-      const TF* lrow = LU + 2 * ((size_t)(r0 + r) * lda + c0);
-      for (int j = part; j < cols; j += 4) {
-        const double yr = __builtin_nontemporal_load(out + 2 * (c0 + j));
-        const double yi = __builtin_nontemporal_load(out + 2 * (c0 + j) + 1);
-        const double lr = ld(lrow + 2 * j), li = ld(lrow + 2 * j + 1);
-        accr += lr * yr - li * yi;
-        acci += lr * yi + li * yr;
+  int done = 0;
+  long spins = 0;
+  while (done < others) {
+    if (t < 64) {
+      const int step = done + t;
+      bool ok = false;
+      if (step < others) {
+        const int c = lower ? step : nblk - 1 - step;
+        ok = __hip_atomic_load(flags + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
+      }
+      const unsigned long long m = __ballot(ok);
+      // count of consecutive finished blocks from `done`
+      const int cnt = (~m == 0ull) ? 64 : __ffsll((long long)~m) - 1;
+      if (t == 0) {
+        int ready = cnt;
+        if (cnt == 0) {
+          // bounded wait; a timeout is sticky so every later wait returns at once
+          if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > (1L << 22)) {
+            __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            ready = others - done;  // give up waiting: finish with what is there
+          } else {
+            __builtin_amdgcn_s_sleep(1);
+          }
+        }
+        s_ready = ready;
       }
     }
+    __syncthreads();
+    const int cnt = s_ready;
+    __syncthreads();
+    for (int step = done; step < done + cnt; ++step) {
+      const int c = lower ? step : nblk - 1 - step;
+      const int c0 = c * SB;
+      const int cols = min(SB, n - c0);
+      if (r < rows) {
+        const TF* lrow = LU + 2 * ((size_t)(r0 + r) * lda + c0);
+        for (int j = part; j < cols; j += 4) {
+          const double yr = xload(out + 2 * (c0 + j));
+          const double yi = xload(out + 2 * (c0 + j) + 1);
+          const double lr = ld(lrow + 2 * j), li = ld(lrow + 2 * j + 1);
+          accr += lr * yr - li * yi;
+          acci += lr * yi + li * yr;
+        }
+      }
+    }
+    done += cnt;
   }
   if (has_nb) {
-    if (t == 0) wait_flag(flags, nb, epoch, status);
+    if (t == 0) {
+      long sp = 0;
+      while (__hip_atomic_load(flags + nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
+        if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
+        if (++sp > (1L << 22)) {
+          __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+          break;
+        }
+        __builtin_amdgcn_s_sleep(1);
+      }
+    }
     __syncthreads();
     const int c0 = nb * SB;
     for (int j = part; j < SB; j += 4) {
-      const double yr = (c0 + j < n) ? __builtin_nontemporal_load(out + 2 * (c0 + j)) : 0.0;
-      const double yi = (c0 + j < n) ? __builtin_nontemporal_load(out + 2 * (c0 + j) + 1) : 0.0;
+      const double yr = (c0 + j < n) ? xload(out + 2 * (c0 + j)) : 0.0;
+      const double yi = (c0 + j < n) ? xload(out + 2 * (c0 + j) + 1) : 0.0;
       const double lr = sNb[2 * (r * SB + j)], li = sNb[2 * (r * SB + j) + 1];
       accr += lr * yr - li * yi;
       acci += lr * yi + li * yr;
@@ -209,14 +246,14 @@ __global__ void __launch_bounds__(256) trsv_kernel(const TF* __restrict__ LU, lo
   sPart[part][2 * r + 1] = xi;
   __syncthreads();
   if (t < rows) {
-    out[2 * (r0 + t)] = ((sPart[0][2 * t] + sPart[1][2 * t]) + sPart[2][2 * t]) + sPart[3][2 * t];
-    out[2 * (r0 + t) + 1] = ((sPart[0][2 * t + 1] + sPart[1][2 * t + 1]) + sPart[2][2 * t + 1]) + sPart[3][2 * t + 1];
+    xstore(out + 2 * (r0 + t), ((sPart[0][2 * t] + sPart[1][2 * t]) + sPart[2][2 * t]) + sPart[3][2 * t]);
+    xstore(out + 2 * (r0 + t) + 1,
+           ((sPart[0][2 * t + 1] + sPart[1][2 * t + 1]) + sPart[2][2 * t + 1]) + sPart[3][2 * t + 1]);
   }
+  // the data stores have completed before the flag is raised
+  asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
-  if (t == 0) {
-    __threadfence();
-    __hip_atomic_store(flags + b, epoch, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-  }
+  if (t == 0) __hip_atomic_store(flags + b, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 __global__ void reset_status_kernel(int* status) {
