@@ -82,6 +82,40 @@ long nec_gpu_pattern_array(nec_context* ctx, int freq_index, int which, double* 
 long nec_gpu_pattern_average(nec_context* ctx, int freq_index, double* avg_gain, double* solid_angle);
 
 /* ------------------------------------------------------------------ */
+/* Batched solves: many small structures (straight wires in free space,   */
+/* at most 256 segments each), each at n_freq frequencies, one voltage    */
+/* source each, with an optional pattern cut.  No counterpart in nec2++,  */
+/* which runs one structure per nec_context; each (design, frequency) is  */
+/* what an EX 0 + FR + XQ/RP deck would compute.                          */
+/* ------------------------------------------------------------------ */
+typedef struct nec_gpu_batch {
+  int n_designs;
+  const int* wire_ptr;      /* n_designs + 1: wires of design d are [wire_ptr[d], wire_ptr[d+1]) */
+  const int* wire_tag;      /* per wire (GW tag) */
+  const int* wire_ns;       /* per wire (GW segment count) */
+  const double* wire_geom;  /* 7 per wire: x1 y1 z1 x2 y2 z2 radius, metres */
+  const int* src_tag;       /* per design: EX 0 tag */
+  const int* src_seg;       /* per design: EX 0 segment within the tag (1-based) */
+  const double* src_volt;   /* 2 per design: source voltage */
+  int n_freq;               /* frequencies per design */
+  const double* freq_mhz;   /* n_designs * n_freq, design-major */
+  int n_dir;                /* pattern directions (0: no pattern) */
+  const double* theta_deg;  /* n_dir */
+  const double* phi_deg;    /* n_dir */
+  double rkh;               /* KH card distance in wavelengths (0: default 1) */
+  int panel_width;          /* batched-LU panel width 8, 16 or 32; 0: autotune */
+  int repeats;              /* timed replays of the captured pipeline graph */
+} nec_gpu_batch;
+
+/* Runs the batch: zin receives 2 doubles per solve (solve s = d * n_freq + f),
+ * gain n_dir doubles per solve (total power gain, dB; may be NULL when
+ * n_dir == 0), stats 12 doubles: solves, segments, host ms, fill ms, LU ms,
+ * solve+pattern ms, ms per graph replay, chosen panel width, autotune ms for
+ * widths 8 / 16 / 32, graph replays equal to the eager pass (1/0/-1).
+ * Returns 0, or 1 with nec_error_message() set. */
+long nec_gpu_batch_solve(const nec_gpu_batch* batch, double* zin, double* gain, double* stats);
+
+/* ------------------------------------------------------------------ */
 /* Device-pointer entry points (no host round trip).  Matrices are      */
 /* row-major complex128 (interleaved), leading dimension lda entries.   */
 /* ------------------------------------------------------------------ */

@@ -9,6 +9,7 @@
 
 #include "../../include/libnecpp.h"
 #include "../../include/necpp_gpu.h"
+#include "batch.h"
 #include "context.h"
 
 using necgpu::Context;
@@ -236,6 +237,39 @@ long nec_gpu_segments(nec_context* ctx, double* out, int* conn, long n) {
         conn[n + i] = g.icon2[i];
         conn[2 * n + i] = g.tags[i];
       }
+    }
+  });
+}
+
+long nec_gpu_batch_solve(const nec_gpu_batch* b, double* zin, double* gain, double* stats) {
+  return guarded([&] {
+    if (!b) throw NecError("nec_gpu_batch_solve: no batch");
+    necgpu::BatchInput in;
+    in.n_designs = b->n_designs;
+    in.wire_ptr = b->wire_ptr;
+    in.wire_tag = b->wire_tag;
+    in.wire_ns = b->wire_ns;
+    in.wire_geom = b->wire_geom;
+    in.src_tag = b->src_tag;
+    in.src_seg = b->src_seg;
+    in.src_volt = b->src_volt;
+    in.n_freq = b->n_freq;
+    in.freq_mhz = b->freq_mhz;
+    in.n_dir = b->n_dir;
+    in.theta_deg = b->theta_deg;
+    in.phi_deg = b->phi_deg;
+    in.rkh = b->rkh;
+    in.panel_width = b->panel_width;
+    in.repeats = b->repeats;
+    necgpu::BatchOutput out;
+    necgpu::run_batch(in, &out);
+    std::copy(out.zin.begin(), out.zin.end(), zin);
+    if (gain && !out.gain.empty()) std::copy(out.gain.begin(), out.gain.end(), gain);
+    if (stats) {
+      const double v[12] = {(double)out.solves, (double)out.segments, out.host_ms, out.fill_ms, out.lu_ms,
+                            out.solve_pattern_ms, out.graph_ms, (double)out.panel_width, out.autotune_ms[0],
+                            out.autotune_ms[1], out.autotune_ms[2], (double)out.graph_matches_eager};
+      std::copy(v, v + 12, stats);
     }
   });
 }

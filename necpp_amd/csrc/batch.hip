@@ -1,0 +1,380 @@
+// batch.hip -- many small solves at once (frequency sweeps, design
+// populations): LU, solve + current coefficients + Z_in, and a pattern cut,
+// one workgroup per solve.  The fill is batch_fill_kernel in fill.hip.
+//
+// The algorithms are those of the single-structure path (the reference's
+// lu_decompose / solve, matrix_algebra.cpp:217-307; get_current_coefficients,
+// c_geometry.cpp:3299-3386; netwk_compute_inputs, nec_context.cpp:4937-5009;
+// ffld, nec_context.cpp:6639-6887, and the total gain of
+// nec_radiation_pattern.cpp:282-334).  What differs is the layout: a
+// structure of at most 256 segments is factored by one workgroup, with the
+// panel in LDS and the pivot rows tracked by a permutation instead of being
+// moved.
+#include <climits>
+
+#include "kernels.h"
+#include "nec_math.h"
+
+namespace necgpu {
+
+namespace {
+
+constexpr int BT = 256;  // threads per workgroup = largest n
+
+// First maximum over the block: thread t offers value v for row t
+// (negative = no candidate); returns the winning row or -1.  Every thread
+// must call it.
+__device__ int block_first_max(double v, double* s_val, int* s_row) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  double m = v;
+  for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+  const unsigned long long hit = __ballot(v >= 0.0 && v == m);
+  if (lane == 0) {
+    s_val[wave] = m;
+    s_row[wave] = hit ? wave * 64 + (__ffsll((long long)hit) - 1) : -1;
+  }
+  __syncthreads();
+  int best = -1;
+  double bv = -1.0;
+  for (int w = 0; w < BT / 64; ++w)
+    if (s_row[w] >= 0 && (best < 0 || s_val[w] > bv)) {
+      bv = s_val[w];
+      best = s_row[w];
+    }
+  __syncthreads();
+  return best;
+}
+
+// In-place LU with partial pivoting (first maximum of |z|, as the
+// single-structure path) of each solve's n x n matrix, n <= 256.
+template <int P, int CW>
+__global__ void __launch_bounds__(BT) batch_lu_kernel(const BatchSolve* __restrict__ sol, double* __restrict__ Zall,
+                                                      int* __restrict__ rpall, int* __restrict__ info) {
+  __shared__ double sP[BT * P * 2];  // panel, logical rows k0.. (row r at r * P)
+  __shared__ double sU[P * CW * 2];  // U12 chunk [c][jj]
+  __shared__ int s_rp[BT];
+  __shared__ double s_val[BT / 64];
+  __shared__ int s_row[BT / 64];
+  const BatchSolve d = sol[blockIdx.x];
+  const int n = d.n;
+  const int t = threadIdx.x;
+  double* Z = Zall + 2 * d.z0;
+  if (t < n) s_rp[t] = t;
+  int first_zero = 0;
+  __syncthreads();
+  for (int k0 = 0; k0 < n; k0 += P) {
+    const int pw = min(P, n - k0);
+    const int m = n - k0;
+    // panel: logical rows k0 .. n-1, columns k0 .. k0+pw-1
+    if (t < m)
+      for (int c = 0; c < P; ++c) {
+        double vr = 0.0, vi = 0.0;
+        if (c < pw) {
+          const double* v = Z + 2 * ((long long)s_rp[k0 + t] * n + k0 + c);
+          vr = v[0];
+          vi = v[1];
+        }
+        sP[2 * (t * P + c)] = vr;
+        sP[2 * (t * P + c) + 1] = vi;
+      }
+    __syncthreads();
+    for (int c = 0; c < pw; ++c) {
+      const int k = k0 + c;
+      double v = -1.0;
+      if (t >= c && t < m) v = hypot(sP[2 * (t * P + c)], sP[2 * (t * P + c) + 1]);
+      const int pr = block_first_max(v, s_val, s_row);
+      if (pr >= 0 && pr != c) {
+        if (t < P) {
+          const double ar = sP[2 * (c * P + t)], ai = sP[2 * (c * P + t) + 1];
+          sP[2 * (c * P + t)] = sP[2 * (pr * P + t)];
+          sP[2 * (c * P + t) + 1] = sP[2 * (pr * P + t) + 1];
+          sP[2 * (pr * P + t)] = ar;
+          sP[2 * (pr * P + t) + 1] = ai;
+        }
+        if (t == 0) {
+          const int tmp = s_rp[k0 + c];
+          s_rp[k0 + c] = s_rp[k0 + pr];
+          s_rp[k0 + pr] = tmp;
+        }
+      }
+      __syncthreads();
+      const zc dpiv = zmk(sP[2 * (c * P + c)], sP[2 * (c * P + c) + 1]);
+      const bool zero = (dpiv.re == 0.0 && dpiv.im == 0.0);
+      if (zero && first_zero == 0) first_zero = k + 1;
+      if (!zero && t > c && t < m) {
+        const zc l = zdiv(zmk(sP[2 * (t * P + c)], sP[2 * (t * P + c) + 1]), dpiv);
+        sP[2 * (t * P + c)] = l.re;
+        sP[2 * (t * P + c) + 1] = l.im;
+        for (int cc = c + 1; cc < pw; ++cc) {
+          const zc u = zmk(sP[2 * (c * P + cc)], sP[2 * (c * P + cc) + 1]);
+          const zc pu = zmul(l, u);
+          sP[2 * (t * P + cc)] -= pu.re;
+          sP[2 * (t * P + cc) + 1] -= pu.im;
+        }
+      }
+      __syncthreads();
+    }
+    if (t < m)
+      for (int c = 0; c < pw; ++c) {
+        double* v = Z + 2 * ((long long)s_rp[k0 + t] * n + k0 + c);
+        v[0] = sP[2 * (t * P + c)];
+        v[1] = sP[2 * (t * P + c) + 1];
+      }
+    // trailing columns in chunks of CW: U12 = L11^-1 A12, A22 -= L21 U12
+    for (int j0 = k0 + pw; j0 < n; j0 += CW) {
+      const int cw = min(CW, n - j0);
+      for (int e = t; e < P * CW; e += BT) {
+        const int c = e / CW, jj = e % CW;
+        double vr = 0.0, vi = 0.0;
+        if (c < pw && jj < cw) {
+          const double* v = Z + 2 * ((long long)s_rp[k0 + c] * n + j0 + jj);
+          vr = v[0];
+          vi = v[1];
+        }
+        sU[2 * e] = vr;
+        sU[2 * e + 1] = vi;
+      }
+      __syncthreads();
+      if (t < cw) {
+        for (int c = 1; c < pw; ++c) {
+          double ur = sU[2 * (c * CW + t)], ui = sU[2 * (c * CW + t) + 1];
+          for (int c2 = 0; c2 < c; ++c2) {
+            const double lr = sP[2 * (c * P + c2)], li = sP[2 * (c * P + c2) + 1];
+            const double xr = sU[2 * (c2 * CW + t)], xi = sU[2 * (c2 * CW + t) + 1];
+            ur -= lr * xr - li * xi;
+            ui -= lr * xi + li * xr;
+          }
+          sU[2 * (c * CW + t)] = ur;
+          sU[2 * (c * CW + t) + 1] = ui;
+        }
+        for (int c = 0; c < pw; ++c) {
+          double* v = Z + 2 * ((long long)s_rp[k0 + c] * n + j0 + t);
+          v[0] = sU[2 * (c * CW + t)];
+          v[1] = sU[2 * (c * CW + t) + 1];
+        }
+      }
+      __syncthreads();
+      if (t >= pw && t < m) {
+        double* row = Z + 2 * ((long long)s_rp[k0 + t] * n + j0);
+        for (int jj = 0; jj < cw; ++jj) {
+          double ar = row[2 * jj], ai = row[2 * jj + 1];
+          for (int c = 0; c < pw; ++c) {
+            const double lr = sP[2 * (t * P + c)], li = sP[2 * (t * P + c) + 1];
+            const double ur = sU[2 * (c * CW + jj)], ui = sU[2 * (c * CW + jj) + 1];
+            ar -= lr * ur - li * ui;
+            ai -= lr * ui + li * ur;
+          }
+          row[2 * jj] = ar;
+          row[2 * jj + 1] = ai;
+        }
+      }
+      __syncthreads();
+    }
+  }
+  if (t < n) rpall[d.row0 + t] = s_rp[t];
+  if (t == 0) info[blockIdx.x] = first_zero;
+}
+
+// Forward / back substitution with the factors and permutation, then the
+// current coefficients (gather over the segment CSR in ascending basis order)
+// and the antenna input: zin[4 s ..] = Z_in (2), input power, 0.
+__global__ void __launch_bounds__(BT) batch_solve_kernel(const BatchSolve* __restrict__ sol,
+                                                         const double* __restrict__ Zall, const int* __restrict__ rpall,
+                                                         const double* __restrict__ seg, const int* __restrict__ segptr,
+                                                         const int* __restrict__ segbasis,
+                                                         const double* __restrict__ segabc, double* __restrict__ xall,
+                                                         double* __restrict__ abcall, double* __restrict__ zin) {
+  __shared__ double sx[2 * BT];
+  __shared__ int s_rp[BT];
+  const BatchSolve d = sol[blockIdx.x];
+  const int n = d.n;
+  const int t = threadIdx.x;
+  const double* Z = Zall + 2 * d.z0;
+  if (t < n) {
+    s_rp[t] = rpall[d.row0 + t];
+  }
+  __syncthreads();
+  // (P b)[i] = b[rp[i]]; b is zero except at the source
+  if (t < n) {
+    const bool src = s_rp[t] == d.src;
+    sx[2 * t] = src ? d.er : 0.0;
+    sx[2 * t + 1] = src ? d.ei : 0.0;
+  }
+  // L y = P b (unit lower), column by column as the reference's solve
+  for (int q = 0; q < n; ++q) {
+    __syncthreads();
+    if (t > q && t < n) {
+      const double yr = sx[2 * q], yi = sx[2 * q + 1];
+      const double* l = Z + 2 * ((long long)s_rp[t] * n + q);
+      sx[2 * t] -= l[0] * yr - l[1] * yi;
+      sx[2 * t + 1] -= l[0] * yi + l[1] * yr;
+    }
+  }
+  // U x = y
+  for (int q = n - 1; q >= 0; --q) {
+    __syncthreads();
+    if (t == q) {
+      const double* u = Z + 2 * ((long long)s_rp[q] * n + q);
+      const zc x = zdiv(zmk(sx[2 * q], sx[2 * q + 1]), zmk(u[0], u[1]));
+      sx[2 * q] = x.re;
+      sx[2 * q + 1] = x.im;
+    }
+    __syncthreads();
+    if (t < q) {
+      const double xr = sx[2 * q], xi = sx[2 * q + 1];
+      const double* u = Z + 2 * ((long long)s_rp[t] * n + q);
+      sx[2 * t] -= u[0] * xr - u[1] * xi;
+      sx[2 * t + 1] -= u[0] * xi + u[1] * xr;
+    }
+  }
+  __syncthreads();
+  double* x = xall + 2 * d.row0;
+  if (t < n) {
+    x[2 * t] = sx[2 * t];
+    x[2 * t + 1] = sx[2 * t + 1];
+  }
+  // current coefficients of segment t
+  const int* sp = segptr + d.sptr0;
+  const int* sb = segbasis + d.ssr0;
+  const double* sv = segabc + 3 * d.ssr0;
+  double* abc = abcall + 6 * d.row0;
+  if (t < n) {
+    double air = 0.0, aii = 0.0, bir = 0.0, bii = 0.0, cir = 0.0, cii = 0.0;
+    for (int q = sp[t]; q < sp[t + 1]; ++q) {
+      const int i = sb[q];
+      const double ar = sx[2 * i], ai = sx[2 * i + 1];
+      const double a = sv[3 * q], b = sv[3 * q + 1], c = sv[3 * q + 2];
+      air += a * ar;
+      aii += a * ai;
+      bir += b * ar;
+      bii += b * ai;
+      cir += c * ar;
+      cii += c * ai;
+    }
+    abc[t] = air;
+    abc[n + t] = aii;
+    abc[2 * n + t] = bir;
+    abc[3 * n + t] = bii;
+    abc[4 * n + t] = cir;
+    abc[5 * n + t] = cii;
+    if (t == d.src) {
+      const zc v = zmk(d.vr, d.vi);
+      const zc cur = zmulr(zmk(air + cir, aii + cii), d.wavelength);
+      const zc z = zdiv(v, cur);
+      double* o = zin + 4 * (long long)blockIdx.x;
+      o[0] = z.re;
+      o[1] = z.im;
+      o[2] = 0.5 * zmul(v, zconj(cur)).re;
+      o[3] = 0.0;
+    }
+  }
+}
+
+// Total power gain (dB) of each solve in n_dir directions: the ffld sum over
+// segments staged in LDS, then db10(gcon (|E_theta|^2 + |E_phi|^2)) with
+// gcon = lambda^2 2 pi / (eta P_in).
+__global__ void __launch_bounds__(BT) batch_pattern_kernel(const BatchSolve* __restrict__ sol,
+                                                           const double* __restrict__ seg,
+                                                           const double* __restrict__ abcall,
+                                                           const double* __restrict__ zin, int n_dir,
+                                                           const double* __restrict__ dir, double* __restrict__ gain) {
+  constexpr int F = 13;
+  __shared__ double ss[kBatchMaxN * F];
+  const BatchSolve d = sol[blockIdx.x];
+  const int n = d.n;
+  const double* p = seg + d.seg0;
+  const double* abc = abcall + 6 * d.row0;
+  for (int e = threadIdx.x; e < n * F; e += blockDim.x) {
+    const int i = e / F, f = e % F;
+    double v;
+    switch (f) {
+      case 0: v = p[5 * n + i]; break;       // cab
+      case 1: v = p[6 * n + i]; break;       // sab
+      case 2: v = p[7 * n + i]; break;       // salp
+      case 3: v = kPi * p[3 * n + i]; break; // pi * s
+      case 4: v = p[i]; break;               // x
+      case 5: v = p[n + i]; break;           // y
+      case 6: v = p[2 * n + i]; break;       // z
+      default: v = abc[(size_t)(f - 7) * n + i]; break;
+    }
+    ss[e] = v;
+  }
+  __syncthreads();
+  const double pin = zin[4 * (long long)blockIdx.x + 2];
+  const double gcon = d.wavelength * d.wavelength * 2.0 * kPi / (em_impedance() * pin);
+  for (int q = threadIdx.x; q < n_dir; q += blockDim.x) {
+    const double sth = dir[4 * q], cth = dir[4 * q + 1], sph = dir[4 * q + 2], cph = dir[4 * q + 3];
+    const double phx = -sph, phy = cph, roz = cth, thz = -sth;
+    const double rox = -thz * phy, roy = thz * phx;
+    double cxr = 0, cxi = 0, cyr = 0, cyi = 0, czr = 0, czi = 0;
+    for (int i = 0; i < n; ++i) {
+      const double* sg = ss + i * F;
+      const double cab = sg[0], sab = sg[1], salp = sg[2], el = sg[3];
+      const double omega = -(rox * cab + roy * sab + roz * salp);
+      const double sill = omega * el;
+      const double top = el + sill;
+      const double bot = el - sill;
+      double a, too, boo;
+      if (fabs(omega) >= 1.0e-7) a = 2. * sin(sill) / omega;
+      else a = (2. - omega * omega * el * el / 3.) * el;
+      if (fabs(top) >= 1.0e-7) too = sin(top) / top;
+      else too = 1. - top * top / 6.;
+      if (fabs(bot) >= 1.0e-7) boo = sin(bot) / bot;
+      else boo = 1. - bot * bot / 6.;
+      const double b = el * (boo - too);
+      const double c = el * (boo + too);
+      const double rr = a * sg[7] + b * sg[10] + c * sg[11];
+      const double ri = a * sg[8] - b * sg[9] + c * sg[12];
+      const double arg = kTwoPi * (sg[4] * rox + sg[5] * roy + sg[6] * roz);
+      double sa, ca;
+      sincos(arg, &sa, &ca);
+      const zc exa = zmul(zmk(ca, sa), zmk(rr, ri));
+      cxr = cxr + exa.re * cab;
+      cxi = cxi + exa.im * cab;
+      cyr = cyr + exa.re * sab;
+      cyi = cyi + exa.im * sab;
+      czr = czr + exa.re * salp;
+      czi = czi + exa.im * salp;
+    }
+    const double thx = roz * phy, thy = -roz * phx;
+    const zc const3 = zmk(0.0, -em_impedance() / kFourPi);
+    const zc cix = zmk(cxr, cxi), ciy = zmk(cyr, cyi), ciz = zmk(czr, czi);
+    const zc eth = zmul(zadd(zadd(zmulr(cix, thx), zmulr(ciy, thy)), zmulr(ciz, thz)), const3);
+    const zc eph = zmul(zadd(zmulr(cix, phx), zmulr(ciy, phy)), const3);
+    const double ethm2 = eth.re * eth.re + eth.im * eth.im;
+    const double ephm2 = eph.re * eph.re + eph.im * eph.im;
+    gain[(long long)blockIdx.x * n_dir + q] = db10(gcon * (ethm2 + ephm2));
+  }
+}
+
+}  // namespace
+
+int launch_batch_lu(const BatchSolve* sol, int S, double* Z, int* rp, int* info, int P, hipStream_t stream) {
+  if (S <= 0) return 0;
+  if (P == 32)
+    launch_k(K_BATCH_LU, batch_lu_kernel<32, 32>, dim3(S), dim3(BT), 0u, stream, sol, Z, rp, info);
+  else if (P == 8)
+    launch_k(K_BATCH_LU, batch_lu_kernel<8, 32>, dim3(S), dim3(BT), 0u, stream, sol, Z, rp, info);
+  else
+    launch_k(K_BATCH_LU, batch_lu_kernel<16, 32>, dim3(S), dim3(BT), 0u, stream, sol, Z, rp, info);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_batch_solve(const BatchSolve* sol, int S, const double* Z, const int* rp, const double* seg,
+                       const int* segptr, const int* segbasis, const double* segabc, double* x, double* abc,
+                       double* zin, hipStream_t stream) {
+  if (S <= 0) return 0;
+  launch_k(K_BATCH_SOLVE, batch_solve_kernel, dim3(S), dim3(BT), 0u, stream, sol, Z, rp, seg, segptr, segbasis,
+           segabc, x, abc, zin);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_batch_pattern(const BatchSolve* sol, int S, const double* seg, const double* abc, const double* zin,
+                         int n_dir, const double* dir, double* gain, hipStream_t stream) {
+  if (S <= 0 || n_dir <= 0) return 0;
+  launch_k(K_BATCH_PATTERN, batch_pattern_kernel, dim3(S), dim3(BT), 0u, stream, sol, seg, abc, zin, n_dir, dir,
+           gain);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+}  // namespace necgpu

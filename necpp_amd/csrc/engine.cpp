@@ -69,6 +69,77 @@ void upload(DevBuf& buf, const std::vector<T>& v, hipStream_t s) {
 
 }  // namespace
 
+void build_basis_tables(const Geometry& geo, const ScaledSegments& seg, BasisTables* out) {
+  const int n = static_cast<int>(geo.size());
+  // Basis tables.  trio(j) gives the bases with support on source j; the
+  // fill gathers per column k in ascending j.  tbf(i) gives the support of
+  // basis i; the current coefficients gather per segment in ascending i.
+  std::vector<int> icon1(geo.icon1.begin(), geo.icon1.end()), icon2(geo.icon2.begin(), geo.icon2.end());
+  SegView sv{icon1.data(), icon2.data(), seg.s.data(), seg.b.data()};
+  std::vector<int> ccount(n + 1, 0), scount(n + 1, 0);
+  std::vector<int> tj, tk, sj, si;
+  std::vector<double> tv, sv3;
+  tj.reserve(8 * (size_t)n);
+  int jco[kTableCap];
+  double ax[kTableCap], bx[kTableCap], cx[kTableCap];
+  for (int j = 0; j < n; ++j) {
+    const int cnt = trio(sv, j + 1, jco, ax, bx, cx, kTableCap);
+    if (cnt < 0) throw NecError("TRIO - SEGMENT CONNECTION ERROR FOR SEGMENT " + std::to_string(j + 1));
+    for (int q = 0; q < cnt; ++q) {
+      tj.push_back(j);
+      tk.push_back(jco[q] - 1);
+      tv.push_back(ax[q]);
+      tv.push_back(bx[q]);
+      tv.push_back(cx[q]);
+      ++ccount[jco[q]];
+    }
+  }
+  for (int i = 0; i < n; ++i) {
+    const int cnt = tbf(sv, i + 1, 1, jco, ax, bx, cx, kTableCap);
+    if (cnt < 0) throw NecError("TBF - SEGMENT CONNECTION ERROR FOR SEGMENT " + std::to_string(i + 1));
+    for (int q = 0; q < cnt; ++q) {
+      si.push_back(i);
+      sj.push_back(jco[q] - 1);
+      sv3.push_back(ax[q]);
+      sv3.push_back(bx[q]);
+      sv3.push_back(cx[q]);
+      ++scount[jco[q]];
+    }
+  }
+  // counting sort keeps the source order (ascending j, resp. ascending i)
+  std::vector<int>& cptr = out->colptr;
+  std::vector<int>& sptr = out->segptr;
+  cptr.assign(n + 1, 0);
+  sptr.assign(n + 1, 0);
+  for (int k = 0; k < n; ++k) {
+    cptr[k + 1] = cptr[k] + ccount[k + 1];
+    sptr[k + 1] = sptr[k] + scount[k + 1];
+  }
+  std::vector<int>& cj = out->colj;
+  std::vector<int>& sb = out->segbasis;
+  std::vector<double>& cv = out->colabc;
+  std::vector<double>& sval = out->segabc;
+  cj.assign(tj.size(), 0);
+  sb.assign(si.size(), 0);
+  cv.assign(tv.size(), 0.0);
+  sval.assign(sv3.size(), 0.0);
+  std::vector<int> cfill(cptr.begin(), cptr.end() - 1), sfill(sptr.begin(), sptr.end() - 1);
+  for (size_t e = 0; e < tj.size(); ++e) {
+    const int p = cfill[tk[e]]++;
+    cj[p] = tj[e];
+    cv[3 * p] = tv[3 * e];
+    cv[3 * p + 1] = tv[3 * e + 1];
+    cv[3 * p + 2] = tv[3 * e + 2];
+  }
+  for (size_t e = 0; e < si.size(); ++e) {
+    const int p = sfill[sj[e]]++;
+    sb[p] = si[e];
+    sval[3 * p] = sv3[3 * e];
+    sval[3 * p + 1] = sv3[3 * e + 1];
+    sval[3 * p + 2] = sv3[3 * e + 2];
+  }
+}
+
 DevBuf::~DevBuf() { release(); }
 
 void DevBuf::release() {
@@ -148,74 +219,19 @@ void DeviceSolver::load(const Geometry& geo, const ScaledSegments& seg, bool use
     segv[7 * n + i] = seg.salp[i];
   }
 
-  // Basis tables.  trio(j) gives the bases with support on source j; the
-  // fill gathers per column k in ascending j.  tbf(i) gives the support of
-  // basis i; the current coefficients gather per segment in ascending i.
-  std::vector<int> icon1(geo.icon1.begin(), geo.icon1.end()), icon2(geo.icon2.begin(), geo.icon2.end());
-  SegView sv{icon1.data(), icon2.data(), seg.s.data(), seg.b.data()};
-  std::vector<int> ccount(n + 1, 0), scount(n + 1, 0);
-  std::vector<int> tj, tk, sj, si;
-  std::vector<double> tv, sv3;
-  tj.reserve(8 * (size_t)n);
-  int jco[kTableCap];
-  double ax[kTableCap], bx[kTableCap], cx[kTableCap];
-  for (int j = 0; j < n; ++j) {
-    const int cnt = trio(sv, j + 1, jco, ax, bx, cx, kTableCap);
-    if (cnt < 0) throw NecError("TRIO - SEGMENT CONNECTION ERROR FOR SEGMENT " + std::to_string(j + 1));
-    for (int q = 0; q < cnt; ++q) {
-      tj.push_back(j);
-      tk.push_back(jco[q] - 1);
-      tv.push_back(ax[q]);
-      tv.push_back(bx[q]);
-      tv.push_back(cx[q]);
-      ++ccount[jco[q]];
-    }
-  }
-  for (int i = 0; i < n; ++i) {
-    const int cnt = tbf(sv, i + 1, 1, jco, ax, bx, cx, kTableCap);
-    if (cnt < 0) throw NecError("TBF - SEGMENT CONNECTION ERROR FOR SEGMENT " + std::to_string(i + 1));
-    for (int q = 0; q < cnt; ++q) {
-      si.push_back(i);
-      sj.push_back(jco[q] - 1);
-      sv3.push_back(ax[q]);
-      sv3.push_back(bx[q]);
-      sv3.push_back(cx[q]);
-      ++scount[jco[q]];
-    }
-  }
-  // counting sort keeps the source order (ascending j, resp. ascending i)
-  std::vector<int> cptr(n + 1, 0), sptr(n + 1, 0);
-  for (int k = 0; k < n; ++k) {
-    cptr[k + 1] = cptr[k] + ccount[k + 1];
-    sptr[k + 1] = sptr[k] + scount[k + 1];
-  }
-  std::vector<int> cj(tj.size()), sb(si.size()), cfill(cptr.begin(), cptr.end() - 1), sfill(sptr.begin(), sptr.end() - 1);
-  std::vector<double> cv(tv.size()), sval(sv3.size());
-  for (size_t e = 0; e < tj.size(); ++e) {
-    const int p = cfill[tk[e]]++;
-    cj[p] = tj[e];
-    cv[3 * p] = tv[3 * e];
-    cv[3 * p + 1] = tv[3 * e + 1];
-    cv[3 * p + 2] = tv[3 * e + 2];
-  }
-  for (size_t e = 0; e < si.size(); ++e) {
-    const int p = sfill[sj[e]]++;
-    sb[p] = si[e];
-    sval[3 * p] = sv3[3 * e];
-    sval[3 * p + 1] = sv3[3 * e + 1];
-    sval[3 * p + 2] = sv3[3 * e + 2];
-  }
+  BasisTables t;
+  build_basis_tables(geo, seg, &t);
   std::vector<int> ek(2 * (size_t)n, 2);
   if (use_ek)
     for (int j = 0; j < n; ++j) ek_ends(geo, seg, j, &ek[2 * j], &ek[2 * j + 1]);
 
   upload(seg_, segv, stream_);
-  upload(colptr_, cptr, stream_);
-  upload(colj_, cj, stream_);
-  upload(colabc_, cv, stream_);
-  upload(segptr_, sptr, stream_);
-  upload(segbasis_, sb, stream_);
-  upload(segabc_, sval, stream_);
+  upload(colptr_, t.colptr, stream_);
+  upload(colj_, t.colj, stream_);
+  upload(colabc_, t.colabc, stream_);
+  upload(segptr_, t.segptr, stream_);
+  upload(segbasis_, t.segbasis, stream_);
+  upload(segabc_, t.segabc, stream_);
   upload(ekind_, ek, stream_);
   hip_check(hipStreamSynchronize(stream_), "load");
 }

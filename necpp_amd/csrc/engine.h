@@ -57,6 +57,17 @@ struct HostPattern {
   std::vector<double> e_theta, e_phi;  // interleaved complex
 };
 
+// Basis tables of one structure at one frequency.  Column CSR (from trio):
+// for basis k, the source segments j with support, ascending j, and their
+// (A, B, C) coefficients -- the fill's gather order.  Segment CSR (from tbf):
+// for segment i, the bases with support on it, ascending, and coefficients --
+// the current coefficients' gather order.  Indices are 0-based.
+struct BasisTables {
+  std::vector<int> colptr, colj, segptr, segbasis;
+  std::vector<double> colabc, segabc;
+};
+void build_basis_tables(const Geometry& geo, const ScaledSegments& seg, BasisTables* out);
+
 class DeviceSolver {
  public:
   DeviceSolver();

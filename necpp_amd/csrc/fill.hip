@@ -466,7 +466,74 @@ __global__ void __launch_bounds__(256) fill_assemble_kernel(const double* __rest
   zp[1] = acci;
 }
 
+// Batched fill: one workgroup per (solve, row i).  The fields of every
+// source j at i go to LDS, then each entry Z[i][k] gathers them over basis
+// k's support in ascending j, as fill_assemble does.
+__global__ void __launch_bounds__(256) batch_fill_kernel(const BatchSolve* __restrict__ sol, int S,
+                                                         const double* __restrict__ seg,
+                                                         const int* __restrict__ colptr,
+                                                         const int* __restrict__ colj,
+                                                         const double* __restrict__ colabc, double* __restrict__ Zall,
+                                                         double rkh) {
+  __shared__ double sE[kBatchMaxN * 6];
+  const long long blk = blockIdx.x;
+  int lo = 0, hi = S - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sol[mid].row0 <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  const BatchSolve d = sol[lo];
+  const int n = d.n;
+  const int i = static_cast<int>(blk - d.row0);
+  const double* p = seg + d.seg0;
+  DevSegments g;
+  g.x = p;
+  g.y = p + n;
+  g.z = p + 2 * n;
+  g.s = p + 3 * n;
+  g.b = p + 4 * n;
+  g.cab = p + 5 * n;
+  g.sab = p + 6 * n;
+  g.salp = p + 7 * n;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    zc etk, ets, etc;
+    pair_fields(g, i, j, rkh, 0, 2, 2, &etk, &ets, &etc);
+    double* o = sE + 6 * j;
+    o[0] = etk.re;
+    o[1] = etk.im;
+    o[2] = ets.re;
+    o[3] = ets.im;
+    o[4] = etc.re;
+    o[5] = etc.im;
+  }
+  __syncthreads();
+  const int* cp = colptr + d.cptr0;
+  const int* cj = colj + d.csr0;
+  const double* cv = colabc + 3 * d.csr0;
+  double* zrow = Zall + 2 * (d.z0 + (long long)i * n);
+  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+    double accr = 0.0, acci = 0.0;
+    for (int q = cp[k]; q < cp[k + 1]; ++q) {
+      const double* e = sE + 6 * cj[q];
+      const double a = cv[3 * q], b = cv[3 * q + 1], c = cv[3 * q + 2];
+      accr += ((e[0] * a) + (e[2] * b)) + (e[4] * c);
+      acci += ((e[1] * a) + (e[3] * b)) + (e[5] * c);
+    }
+    zrow[2 * k] = accr;
+    zrow[2 * k + 1] = acci;
+  }
+}
+
 }  // namespace
+
+int launch_batch_fill(const BatchSolve* sol, int S, long long rows_total, const double* seg, const int* colptr,
+                      const int* colj, const double* colabc, double* Z, double rkh, hipStream_t stream) {
+  if (S <= 0 || rows_total <= 0) return 0;
+  launch_k(K_BATCH_FILL, batch_fill_kernel, dim3((unsigned)rows_total), dim3(256), 0u, stream, sol, S, seg, colptr,
+           colj, colabc, Z, rkh);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
 
 int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind, double* E,
                 const int* colptr, const int* colj, const double* colabc, double* Z, long long ldz,

@@ -9,7 +9,8 @@ const char* kernel_kind_name(int kind) {
   static const char* names[K_NKINDS] = {"fill_fields",   "fill_assemble", "lu_init",    "lu_panel",
                                         "lu_laswp",      "lu_trsm",       "zgemm",      "solve_reset", "diag_inverse",
                                         "trsv_lower",    "trsv_upper",    "currents",   "zin",
-                                        "pattern_partial", "pattern_finish", "refine", "residual"};
+                                        "pattern_partial", "pattern_finish", "refine", "residual",
+                                        "batch_fill", "batch_lu", "batch_solve", "batch_pattern"};
   return (kind >= 0 && kind < K_NKINDS) ? names[kind] : "?";
 }
 

@@ -29,6 +29,10 @@ enum KernelKind {
   K_PATTERN_FINISH,
   K_REFINE,
   K_RESIDUAL,
+  K_BATCH_FILL,
+  K_BATCH_LU,
+  K_BATCH_SOLVE,
+  K_BATCH_PATTERN,
   K_NKINDS
 };
 const char* kernel_kind_name(int kind);
@@ -72,6 +76,41 @@ inline void launch_k(int kind, F kernel, dim3 grid, dim3 block, uint32_t shmem, 
 struct DevSegments {
   const double *x, *y, *z, *s, *b, *cab, *sab, *salp;
 };
+
+// ------------------------------------------------------------ batches
+// One solve of a batch of small structures: offsets into the batch's
+// concatenated arrays (segment tables, basis CSRs, matrices, vectors).
+struct BatchSolve {
+  int n;            // segments
+  int src;          // source segment (0-based)
+  long long row0;   // sum of n over earlier solves (x, rp, abc offsets)
+  long long seg0;   // into seg: 8 doubles per segment, SoA per solve
+  long long cptr0;  // into colptr (n + 1 entries per solve)
+  long long csr0;   // into colj (and 3x into colabc)
+  long long sptr0;  // into segptr (n + 1 entries per solve)
+  long long ssr0;   // into segbasis (and 3x into segabc)
+  long long z0;     // into Z, complex entries (n x n row-major)
+  double wavelength;
+  double er, ei;    // right-hand side at the source: -V / (s lambda)
+  double vr, vi;    // source voltage
+};
+constexpr int kBatchMaxN = 256;  // largest structure of the batched path
+
+// fill.hip: one workgroup per (solve, row); rows_total = sum of n.
+int launch_batch_fill(const BatchSolve* sol, int S, long long rows_total, const double* seg, const int* colptr,
+                      const int* colj, const double* colabc, double* Z, double rkh, hipStream_t stream);
+
+// batch.hip: one workgroup per solve.  LU with row permutation rp (logical
+// row i of the factors is physical row rp[i]; no rows are moved), panel
+// width P (16 or 32); then the solve, current coefficients abc (6 per
+// segment), Z_in and input power (zin: 4 per solve); then the total gain
+// (dB) in n_dir directions (dir: sin th, cos th, sin ph, cos ph each).
+int launch_batch_lu(const BatchSolve* sol, int S, double* Z, int* rp, int* info, int P, hipStream_t stream);
+int launch_batch_solve(const BatchSolve* sol, int S, const double* Z, const int* rp, const double* seg,
+                       const int* segptr, const int* segbasis, const double* segabc, double* x, double* abc,
+                       double* zin, hipStream_t stream);
+int launch_batch_pattern(const BatchSolve* sol, int S, const double* seg, const double* abc, const double* zin,
+                         int n_dir, const double* dir, double* gain, hipStream_t stream);
 
 // fill.hip: fields of rows [i0, i0+rows) into E (rows*n*6 doubles), then
 // gather into Z (row-major complex, leading dimension ldz complex entries).

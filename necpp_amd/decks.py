@@ -91,6 +91,36 @@ def yagi(half_lengths, spacings, rad, seg_per_element=21, freq_mhz=145.0, rp=(1,
     return deck
 
 
+# speed of light of nec2++ (electromag.cpp: 1 / sqrt(mu0 eps0), eps0 = 8.854e-12,
+# mu0 = 4 pi 1e-7 with pi = 3.1415926536)
+C_NEC = 1.0 / (4.0 * 3.1415926536e-7 * 8.854e-12) ** 0.5
+BATCH_SEED = 20261203
+
+
+def yagi_population(count=512, seed=BATCH_SEED, n_freq=64, f_lo=140.0, f_hi=150.0, seg_per_element=21):
+    """The batching workload: `count` Yagi-Uda designs drawn from numpy's
+    PCG64 generator seeded with `seed`.  Per design, in draw order: the
+    element count k ~ U{3..12}, k half-lengths ~ U(0.20, 0.28) lambda0 and
+    k-1 spacings ~ U(0.10, 0.35) lambda0 (lambda0 at 145 MHz); radius 0.003
+    lambda0; source on the centre segment of element 2.  Each design is swept
+    over n_freq frequencies evenly spaced over [f_lo, f_hi] MHz and cut in
+    its E-plane (theta = 90 deg, phi = 0..360 deg in 1 deg steps).
+    Returns (decks, freqs, theta_deg, phi_deg)."""
+    import numpy as np
+    lam0 = C_NEC / 145.0e6
+    rng = np.random.default_rng(seed)
+    designs = []
+    for _ in range(count):
+        k = int(rng.integers(3, 13))
+        half = rng.uniform(0.20, 0.28, k) * lam0
+        spac = rng.uniform(0.10, 0.35, k - 1) * lam0
+        designs.append(yagi(list(half), list(spac), 0.003 * lam0, seg_per_element, 145.0))
+    freqs = np.linspace(f_lo, f_hi, n_freq)
+    phi = np.arange(361, dtype=float)
+    theta = np.full(361, 90.0)
+    return designs, freqs, theta, phi
+
+
 def apply_to_api(deck, api, ctx, with_rp=True):
     """Run the deck through a libnecpp-style API object (module or wrapper)."""
     for w in deck["wires"]:

@@ -60,6 +60,7 @@ _SIGS = [
     ("nec_gpu_segment_count", _L, [_P]),
     ("nec_gpu_segments", _L, [_P, _P, _P, _L]),
     ("nec_gpu_set_precision", _L, [_P, _I]),
+    ("nec_gpu_batch_solve", _L, [_P, _P, _P, _P]),
     ("nec_gpu_set_refine_tolerance", _L, [_P, _D]),
     ("nec_gpu_refinement", _L, [_P, _P, _P]),
     ("nec_gpu_currents", _L, [_P, _I, _P, _L]),
@@ -181,6 +182,54 @@ def gpu_refinement(ctx):
     if lib().nec_gpu_refinement(ctx, ctypes.byref(steps), ctypes.byref(res)) != 0:
         raise RuntimeError(nec_error_message())
     return steps.value, res.value
+
+
+class BatchDesc(ctypes.Structure):
+    """struct nec_gpu_batch of necpp_gpu.h."""
+    _fields_ = [("n_designs", ctypes.c_int), ("wire_ptr", _P), ("wire_tag", _P), ("wire_ns", _P),
+                ("wire_geom", _P), ("src_tag", _P), ("src_seg", _P), ("src_volt", _P), ("n_freq", ctypes.c_int),
+                ("freq_mhz", _P), ("n_dir", ctypes.c_int), ("theta_deg", _P), ("phi_deg", _P),
+                ("rkh", ctypes.c_double), ("panel_width", ctypes.c_int), ("repeats", ctypes.c_int)]
+
+
+BATCH_STATS = ("solves", "segments", "host_ms", "fill_ms", "lu_ms", "solve_pattern_ms", "graph_ms_per_replay",
+               "panel_width", "autotune_ms_p8", "autotune_ms_p16", "autotune_ms_p32", "graph_matches_eager")
+
+
+def gpu_batch_solve(decks, freqs, theta_deg=(), phi_deg=(), repeats=0, panel_width=0, rkh=1.0):
+    """Batched solves of necpp_amd.decks structures (GW wires, one EX 0
+    source each).  freqs: one list for all designs, or one row per design.
+    Returns (zin complex [design, freq], gain [design, freq, dir], stats)."""
+    nd = len(decks)
+    freqs = np.asarray(freqs, dtype=float)
+    if freqs.ndim == 1:
+        freqs = np.tile(freqs, (nd, 1))
+    nf = freqs.shape[1]
+    wire_ptr, tags, ns, geom = [0], [], [], []
+    for d in decks:
+        for w in d["wires"]:
+            tags.append(w[0])
+            ns.append(w[1])
+            geom.extend(w[2:9])
+        wire_ptr.append(len(tags))
+    a = {"wire_ptr": np.asarray(wire_ptr, np.int32), "wire_tag": np.asarray(tags, np.int32),
+         "wire_ns": np.asarray(ns, np.int32), "wire_geom": np.asarray(geom, float),
+         "src_tag": np.asarray([d["source"][0] for d in decks], np.int32),
+         "src_seg": np.asarray([d["source"][1] for d in decks], np.int32),
+         "src_volt": np.asarray([[complex(d["voltage"]).real, complex(d["voltage"]).imag] for d in decks], float),
+         "freq_mhz": np.ascontiguousarray(freqs), "theta_deg": np.asarray(theta_deg, float),
+         "phi_deg": np.asarray(phi_deg, float)}
+    ndir = len(a["theta_deg"])
+    b = BatchDesc(nd, _ptr(a["wire_ptr"]), _ptr(a["wire_tag"]), _ptr(a["wire_ns"]), _ptr(a["wire_geom"]),
+                  _ptr(a["src_tag"]), _ptr(a["src_seg"]), _ptr(a["src_volt"]), nf, _ptr(a["freq_mhz"]), ndir,
+                  _ptr(a["theta_deg"]), _ptr(a["phi_deg"]), rkh, panel_width, repeats)
+    zin = np.zeros(2 * nd * nf)
+    gain = np.zeros(max(1, nd * nf * ndir))
+    stats = np.zeros(12)
+    if lib().nec_gpu_batch_solve(ctypes.byref(b), _ptr(zin), _ptr(gain) if ndir else None, _ptr(stats)) != 0:
+        raise RuntimeError(nec_error_message())
+    z = (zin[0::2] + 1j * zin[1::2]).reshape(nd, nf)
+    return z, gain[:nd * nf * ndir].reshape(nd, nf, ndir), dict(zip(BATCH_STATS, stats.tolist()))
 
 
 def gpu_info():
