@@ -38,6 +38,7 @@ def parse():
     p.add_argument("--nn", type=int, default=41, help="grid nodes per side (41 -> N=3,280)")
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-mixed", action="store_true", help="skip the N=16,020 mixed-vs-FP64 leg")
+    p.add_argument("--no-batch", action="store_true", help="skip the batched Yagi-population leg")
     p.add_argument("--mixed-nn", type=int, default=90, help="grid nodes per side of the mixed leg (90 -> N=16,020)")
     p.add_argument("--summarize", metavar="DIR", help="print a rocprofv3 --stats CSV summary and exit")
     p.add_argument("--summarize-pmc", nargs=2, metavar=("DIR", "COUNTER"),
@@ -183,6 +184,27 @@ def mixed_leg(nn, steps=1):
             "zin_fp64_vs_oracle_rel": None if zref is None else abs(z64 - zref) / abs(zref)}
 
 
+def batch_leg(repeats=3):
+    """512 seeded Yagi designs x 64 frequencies with 361-point E-plane cuts, one batch."""
+    from necpp_amd import decks
+    from necpp_amd import necpp as api
+    import numpy as np
+    designs, freqs, theta, phi = decks.yagi_population()
+    t = time.perf_counter()
+    zin, gain, st = api.gpu_batch_solve(designs, freqs, theta, phi, repeats=repeats)
+    wall = time.perf_counter() - t
+    n = np.array([sum(w[1] for w in d["wires"]) for d in designs])
+    lu_flops = float(np.sum(8.0 * n.astype(float) ** 3 / 3.0)) * len(freqs)
+    return {"workload": "512 Yagi-Uda designs (seed %d, N = 63..252) x 64 frequencies 140-150 MHz, "
+                        "fill+LU+solve+Z_in+361-point E-plane cut per solve" % decks.BATCH_SEED,
+            "solves": int(st["solves"]), "ms_per_batch": st["graph_ms_per_replay"],
+            "solves_per_s": st["solves"] / (st["graph_ms_per_replay"] * 1e-3),
+            "stages_ms": {"fill": st["fill_ms"], "lu": st["lu_ms"], "solve_pattern": st["solve_pattern_ms"]},
+            "lu_tflops": lu_flops / (st["lu_ms"] * 1e-3) / 1e12,
+            "panel_width": int(st["panel_width"]), "graph_replays_equal_eager": st["graph_matches_eager"] == 1,
+            "host_prep_ms": st["host_ms"], "call_wall_s": wall}
+
+
 def cpu_baseline(deck):
     """Time the oracle (CPU restatement) on one full solve of the same deck."""
     from oracle import pyoracle as po
@@ -289,6 +311,8 @@ def main():
     }
     if not a.no_mixed:
         result["mixed_vs_fp64"] = mixed_leg(a.mixed_nn)
+    if not a.no_batch:
+        result["batch"] = batch_leg()
     if not a.no_cpu:
         cb, r = cpu_baseline(deck)
         result["cpu_baseline"] = cb
