@@ -39,6 +39,7 @@ def parse():
     p.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     p.add_argument("--no-mixed", action="store_true", help="skip the N=16,020 mixed-vs-FP64 leg")
     p.add_argument("--no-batch", action="store_true", help="skip the batched Yagi-population leg")
+    p.add_argument("--no-adjoint", action="store_true", help="skip the 256-design adjoint-gradient leg")
     p.add_argument("--mixed-nn", type=int, default=90, help="grid nodes per side of the mixed leg (90 -> N=16,020)")
     p.add_argument("--summarize", metavar="DIR", help="print a rocprofv3 --stats CSV summary and exit")
     p.add_argument("--summarize-pmc", nargs=2, metavar=("DIR", "COUNTER"),
@@ -205,6 +206,45 @@ def batch_leg(repeats=3):
             "host_prep_ms": st["host_ms"], "call_wall_s": wall}
 
 
+def adjoint_leg(count=256):
+    """J and dJ/dp (44 parameters) of 256 seeded 15-element Yagis (N = 465)
+    by the adjoint method, one design after another through the C API."""
+    from necpp_amd import adjoint as adj
+    from necpp_amd import decks
+    import numpy as np
+    P = decks.c4_params(count=count)
+    keys = ("solve_wall_ms", "adjoint_wall_ms", "device_ms", "host_ms", "fill_ms", "factor_ms", "solve_ms")
+    acc = {k: 0.0 for k in keys}
+    Js, Gs = [], []
+    t = time.perf_counter()
+    for p in P:
+        J, G, info = adj.objective_and_gradient(p)
+        Js.append(J)
+        Gs.append(G)
+        for k in keys:
+            acc[k] += info[k]
+    wall = time.perf_counter() - t
+    out = {"workload": "256 Yagi-Uda designs (15 elements x 31 segments, N = 465, 144 MHz, seed %d): "
+                       "J = -G(90, 0) + 0.01 |Z_in - 50|^2 and its gradient in 44 parameters" % decks.ADJOINT_SEED,
+           "designs": count, "wall_s": wall, "designs_per_s": count / wall,
+           "mean_ms": {k: v / count for k, v in acc.items()}}
+    fx = os.path.join(ROOT, "tests", "golden", "c4_fd_oracle.npz")
+    if os.path.exists(fx):
+        z = np.load(fx)
+        m = min(count, len(z["J"]))
+        G = np.array(Gs[:m])
+        nf = np.linalg.norm(z["fd"][:m], axis=1)
+        err = np.linalg.norm(G - z["fd"][:m], axis=1) / nf
+        cons = np.linalg.norm(z["fd"][:m] - z["fd2"][:m], axis=1) / nf
+        smooth = cons <= 1e-6
+        out["vs_oracle_central_differences"] = {
+            "step": "1e-6 |p|", "designs": m, "smooth_at_step": int(smooth.sum()),
+            "median_rel_err": float(np.median(err)), "max_rel_err_smooth": float(err[smooth].max()),
+            "max_rel_err_all": float(err.max()),
+            "J_max_rel_diff": float(np.max(np.abs(np.array(Js[:m]) - z["J"][:m]) / np.abs(z["J"][:m])))}
+    return out
+
+
 def cpu_baseline(deck):
     """Time the oracle (CPU restatement) on one full solve of the same deck."""
     from oracle import pyoracle as po
@@ -313,6 +353,8 @@ def main():
         result["mixed_vs_fp64"] = mixed_leg(a.mixed_nn)
     if not a.no_batch:
         result["batch"] = batch_leg()
+    if not a.no_adjoint:
+        result["adjoint"] = adjoint_leg()
     if not a.no_cpu:
         cb, r = cpu_baseline(deck)
         result["cpu_baseline"] = cb

@@ -38,6 +38,22 @@ long nec_gpu_set_precision(nec_context* ctx, int mode);
 long nec_gpu_set_refine_tolerance(nec_context* ctx, double tol);
 long nec_gpu_refinement(nec_context* ctx, int* steps, double* rel_residual);
 
+/* Adjoint gradient of the last solve (XQ or RP card; one voltage source,
+ * FP64 path, thin-wire kernel) for the design objective
+ *   J = -G(theta, phi) + weight * |Z_in - z0|^2,
+ * G the total power gain (dBi) in direction (theta, phi) and Z_in the input
+ * impedance of the source.  The transposed solve Z^T lam = dJ/dx reuses the
+ * LU factors; the fill derivative lam^T (dZ/dp) x is contracted on the GPU
+ * without storing dZ/dp.  Writes J to *J_out, dJ with respect to every
+ * segment's endpoints and radius to grad_out (7 doubles per segment: x1 y1 z1
+ * x2 y2 z2 radius, per metre; the endpoints after junction resolution), and
+ * dJ/d(frequency in MHz) to *dfreq_out (may be NULL).  info (may be NULL)
+ * receives 6 doubles: gain (dBi), Z_in (re, im), host ms, device ms,
+ * dJ/d(wavelength) at fixed wavelength-scaled geometry.  No counterpart in
+ * the reference (new entry point). */
+long nec_gpu_adjoint_gradient(nec_context* ctx, double theta_deg, double phi_deg, double weight, double z0_re,
+                              double z0_im, double* J_out, double* grad_out, double* dfreq_out, double* info);
+
 /* Segment currents (amperes, complex interleaved re,im) of antenna-input
  * result freq_index: writes 2*n doubles, n = nec_gpu_segment_count(). */
 long nec_gpu_currents(nec_context* ctx, int freq_index, double* out, long n);

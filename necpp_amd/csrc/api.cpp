@@ -281,6 +281,29 @@ long nec_gpu_set_refine_tolerance(nec_context* ctx, double tol) {
   });
 }
 
+long nec_gpu_adjoint_gradient(nec_context* ctx, double theta_deg, double phi_deg, double weight, double z0_re,
+                              double z0_im, double* J_out, double* grad_out, double* dfreq_out, double* info) {
+  return guarded([&] {
+    if (!ctx) throw NecError("nec_gpu_adjoint_gradient: no context");
+    necgpu::AdjointSpec spec;
+    spec.theta_deg = theta_deg;
+    spec.phi_deg = phi_deg;
+    spec.weight = weight;
+    spec.z0 = necgpu::zmk(z0_re, z0_im);
+    necgpu::AdjointResult r;
+    std::vector<double> g7;
+    double df = 0.0;
+    ctx->adjoint(spec, &r, &g7, &df);
+    if (J_out) *J_out = r.J;
+    if (grad_out) std::copy(g7.begin(), g7.end(), grad_out);
+    if (dfreq_out) *dfreq_out = df;
+    if (info) {
+      const double v[6] = {r.gain_db, r.zin.re, r.zin.im, r.host_ms, r.device_ms, r.dlambda};
+      std::copy(v, v + 6, info);
+    }
+  });
+}
+
 long nec_gpu_refinement(nec_context* ctx, int* steps, double* rel_residual) {
   return guarded([&] {
     *steps = ctx->device().refine_steps();

@@ -12,42 +12,52 @@
 // returns.
 #pragma once
 
+#include "dual.h"
 #include "nec_math.h"
 
 namespace necgpu {
 
-struct SegView {
+template <class T>
+struct SegViewT {
   const int* icon1;
   const int* icon2;
-  const double* s;  // lengths in wavelengths
-  const double* b;  // radii in wavelengths
+  const T* s;  // lengths in wavelengths
+  const T* b;  // radii in wavelengths
 };
+using SegView = SegViewT<double>;
 
 namespace basis_detail {
-NEC_HD double omc_series_or(double d, double exact) {
-  if (d <= 0.015) {
-    double omc = 4. * d * d;
+template <class T>
+NEC_HD T omc_series_or(const T& d, const T& exact) {
+  if (val(d) <= 0.015) {
+    T omc = 4. * d * d;
     return ((1.3888889e-3 * omc - 4.1666666667e-2) * omc + .5) * omc;
   }
   return exact;
 }
-NEC_HD double log_weight(double b) { return 1. / (log(1. / (kPi * b)) - kEuler); }
-NEC_HD double end_cap(double b) {
-  double q = kPi * b;
-  double x = q * q;
+template <class T>
+NEC_HD T log_weight(const T& b) {
+  return 1. / (log(1. / (kPi * b)) - kEuler);
+}
+template <class T>
+NEC_HD T end_cap(const T& b) {
+  T q = kPi * b;
+  T x = q * q;
   return q * (1. - .5 * x) / (1. - x);
 }
 }  // namespace basis_detail
 
 // sbf(i, is): coefficients of basis i on segment is (1-based).  Returns 0,
 // or 1 on a connection error.
-NEC_HD int sbf(const SegView& g, int i, int is, double* aa, double* bb, double* cc) {
+template <class T>
+NEC_HD int sbf(const SegViewT<T>& g, int i, int is, T* aa, T* bb, T* cc) {
   using namespace basis_detail;
-  double pp = 0., pm = 0., sig = -1.;
+  T pp = T(0.), pm = T(0.);
+  double sig = -1.;
   int count = 0, njun1 = 0, june = 0;
-  *aa = 0.;
-  *bb = 0.;
-  *cc = 0.;
+  *aa = T(0.);
+  *bb = T(0.);
+  *cc = T(0.);
   const int ix = i - 1;
   int jcox = g.icon1[ix];
   if (jcox > kPchcon) jcox = i;
@@ -61,12 +71,12 @@ NEC_HD int sbf(const SegView& g, int i, int is, double* aa, double* bb, double* 
       }
       const int k = jcox - 1;
       ++count;
-      const double d = kPi * g.s[k];
-      const double sdh = sin(d), cdh = cos(d);
-      const double sd = 2. * sdh * cdh;
-      const double omc = omc_series_or(d, 1. - cdh * cdh + sdh * sdh);
-      const double aj = log_weight(g.b[k]);
-      pp -= omc / sd * aj;
+      const T d = kPi * g.s[k];
+      const T sdh = sin(d), cdh = cos(d);
+      const T sd = 2. * sdh * cdh;
+      const T omc = omc_series_or(d, 1. - cdh * cdh + sdh * sdh);
+      const T aj = log_weight(g.b[k]);
+      pp = pp - omc / sd * aj;
       if (jcox == is) {
         *aa = aj / sd * sig;
         *bb = aj / (2. * cdh);
@@ -85,7 +95,7 @@ NEC_HD int sbf(const SegView& g, int i, int is, double* aa, double* bb, double* 
       if (iend == 1) break;
     }
     pm = -pp;
-    pp = 0.;
+    pp = T(0.);
     njun1 = count;
     jcox = g.icon2[ix];
     if (jcox > kPchcon) jcox = i;
@@ -94,50 +104,50 @@ NEC_HD int sbf(const SegView& g, int i, int is, double* aa, double* bb, double* 
     sig = -1.;
   } while (jcox != 0);
   const int njun2 = count - njun1;
-  const double d = kPi * g.s[ix];
-  const double sdh = sin(d), cdh = cos(d);
-  const double sd = 2. * sdh * cdh;
-  const double cd = cdh * cdh - sdh * sdh;
-  const double omc = omc_series_or(d, 1. - cd);
-  const double ap = log_weight(g.b[ix]);
-  const double aj = ap;
+  const T d = kPi * g.s[ix];
+  const T sdh = sin(d), cdh = cos(d);
+  const T sd = 2. * sdh * cdh;
+  const T cd = cdh * cdh - sdh * sdh;
+  const T omc = omc_series_or(d, 1. - cd);
+  const T ap = log_weight(g.b[ix]);
+  const T aj = ap;
   if (njun1 == 0) {
-    const double xxi = end_cap(g.b[ix]);
+    const T xxi = end_cap(g.b[ix]);
     if (njun2 == 0) {
-      *aa = -1.;
+      *aa = T(-1.);
       *cc = 1. / (cdh - xxi * sdh);
       return 0;
     }
-    const double qp = -(omc + xxi * sd) / (sd * (ap + xxi * pp) + cd * (xxi * ap - pp));
+    const T qp = -(omc + xxi * sd) / (sd * (ap + xxi * pp) + cd * (xxi * ap - pp));
     if (june == 1) {
       *aa = -*aa * qp;
       *bb = *bb * qp;
       *cc = -*cc * qp;
       if (i != is) return 0;
     }
-    *aa -= 1.;
-    const double den = cd - xxi * sd;
-    *bb += (sdh + ap * qp * (cdh - xxi * sdh)) / den;
-    *cc += (cdh + ap * qp * (sdh + xxi * cdh)) / den;
+    *aa = *aa - 1.;
+    const T den = cd - xxi * sd;
+    *bb = *bb + (sdh + ap * qp * (cdh - xxi * sdh)) / den;
+    *cc = *cc + (cdh + ap * qp * (sdh + xxi * cdh)) / den;
     return 0;
   }
   if (njun2 == 0) {
-    const double xxi = end_cap(g.b[ix]);
-    const double qm = (omc + xxi * sd) / (sd * (aj - xxi * pm) + cd * (pm + xxi * aj));
+    const T xxi = end_cap(g.b[ix]);
+    const T qm = (omc + xxi * sd) / (sd * (aj - xxi * pm) + cd * (pm + xxi * aj));
     if (june == -1) {
       *aa = *aa * qm;
       *bb = *bb * qm;
       *cc = *cc * qm;
       if (i != is) return 0;
     }
-    *aa -= 1.;
-    const double den = cd - xxi * sd;
-    *bb += (aj * qm * (cdh - xxi * sdh) - sdh) / den;
-    *cc += (cdh - aj * qm * (sdh + xxi * cdh)) / den;
+    *aa = *aa - 1.;
+    const T den = cd - xxi * sd;
+    *bb = *bb + (aj * qm * (cdh - xxi * sdh) - sdh) / den;
+    *cc = *cc + (cdh - aj * qm * (sdh + xxi * cdh)) / den;
     return 0;
   }
-  double qp = sd * (pm * pp + aj * ap) + cd * (pm * ap - pp * aj);
-  const double qm = (ap * omc - pp * sd) / qp;
+  T qp = sd * (pm * pp + aj * ap) + cd * (pm * ap - pp * aj);
+  const T qm = (ap * omc - pp * sd) / qp;
   qp = -(aj * omc + pm * sd) / qp;
   if (june != 0) {
     if (june < 0) {
@@ -151,16 +161,17 @@ NEC_HD int sbf(const SegView& g, int i, int is, double* aa, double* bb, double* 
     }
     if (i != is) return 0;
   }
-  *aa -= 1.;
-  *bb += (aj * qm + ap * qp) * sdh / sd;
-  *cc += (aj * qm - ap * qp) * cdh / sd;
+  *aa = *aa - 1.;
+  *bb = *bb + (aj * qm + ap * qp) * sdh / sd;
+  *cc = *cc + (aj * qm - ap * qp) * cdh / sd;
   return 0;
 }
 
 // trio(j): the basis functions with support on segment j (1-based), in the
 // reference's order (segments around end 1, then around end 2, then j
 // itself).  Returns the count, or -1 on a connection error / overflow.
-NEC_HD int trio(const SegView& g, int j, int* jco, double* ax, double* bx, double* cx, int cap) {
+template <class T>
+NEC_HD int trio(const SegViewT<T>& g, int j, int* jco, T* ax, T* bx, T* cx, int cap) {
   int count = 0, jend = 0, iend = 0;
   const int jx = j - 1;
   int jcox = g.icon1[jx];

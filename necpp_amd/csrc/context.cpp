@@ -143,6 +143,16 @@ void Context::rp_card(int calc_mode, int n_theta, int n_phi, int output_format, 
   simulate(true);
 }
 
+void Context::adjoint(const AdjointSpec& spec, AdjointResult* r, std::vector<double>* grad7, double* dfreq_mhz) {
+  if (inputs_.empty() || state_ < EXCITATION_LOOP) throw NecError("adjoint: no solve to differentiate (XQ or RP first)");
+  if (src_seg_.size() != 1) throw NecError("adjoint: the objective needs exactly one voltage source");
+  if (precision_ != 0) throw NecError("adjoint: needs the FP64 path (precision mode 0)");
+  const ScaledSegments seg = geo_.scaled(freq_mhz_);
+  adjoint_gradient(geo_, seg, static_cast<int>(src_seg_[0] - 1), src_v_[0], wavelength_, rkh_, abc_, device(), spec,
+                   r);
+  adjoint_endpoints(seg, wavelength_, freq_mhz_, *r, grad7, dfreq_mhz);
+}
+
 void Context::simulate(bool far_field_flag) {
   // src/nec_context.cpp:1031-1159 for voltage excitation.
   if (!geo_.is_complete()) throw NecError("geometry is not complete");

@@ -11,6 +11,7 @@
 #include <string>
 #include <vector>
 
+#include "adjoint.h"
 #include "engine.h"
 #include "geometry.h"
 
@@ -78,6 +79,10 @@ class Context {
   // the last card was RP, the pattern) on the device.  out: ms per solve,
   // fill / factor / solve / pattern ms, Z_in of feed 0 (re, im).
   void benchmark_solve(int warmup, int steps, double* out);
+  // Adjoint gradient of the last solve (one voltage source, FP64 path) for
+  // the objective of `spec`: the segment endpoint and radius gradient (7
+  // per segment, per metre) and dJ/d(frequency in MHz).
+  void adjoint(const AdjointSpec& spec, AdjointResult* r, std::vector<double>* grad7, double* dfreq_mhz);
   DeviceSolver& device();
 
  private:

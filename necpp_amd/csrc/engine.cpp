@@ -414,6 +414,46 @@ void DeviceSolver::solve(const std::vector<zc>& e, std::vector<zc>* curx, std::v
   times_.solve_ms = elapsed(ev_[0], ev_[1]);
 }
 
+void DeviceSolver::adjoint(double rkh, const std::vector<zc>& g, std::vector<zc>* lam, std::vector<zc>* x,
+                           std::vector<double>* gi, std::vector<double>* gj) {
+  if (!factored_) throw NecError("adjoint requested before the matrix was factored");
+  if (precision_ != 0) throw NecError("the adjoint needs the FP64 factors (precision mode 0)");
+  if (use_ek_) throw NecError("the adjoint supports the thin-wire kernel only (no EK card)");
+  const int n = n_;
+  if ((int)g.size() != n) throw NecError("adjoint: right-hand side has the wrong length");
+  adj_g_.ensure((size_t)n * 16);
+  adj_lam_.ensure((size_t)n * 16);
+  adj_acc_.ensure((size_t)n * 16);
+  adj_gi_.ensure((size_t)n * 7 * 8);
+  adj_gj_.ensure((size_t)n * 13 * 8);
+  const int rows = adjoint_rows_per_chunk(n, kFieldBudget / 4);
+  adj_pj_.ensure((size_t)rows * n * 13 * 8);
+  hip_check(hipMemcpyAsync(adj_g_.as<void>(), g.data(), (size_t)n * 16, hipMemcpyHostToDevice, stream_), "adjoint g");
+  hip_check(hipMemsetAsync(adj_gj_.as<void>(), 0, (size_t)n * 13 * 8, stream_), "adjoint zero");
+  hip_check(hipEventRecord(ev_[0], stream_), "event");
+  if (launch_lu_solve_transposed(Z_.as<double>(), ld_, n, piv_.as<int>(), adj_g_.as<double>(), adj_lam_.as<double>(),
+                                 adj_acc_.as<double>(), stream_))
+    hip_check(hipGetLastError(), "transposed solve launch");
+  const DevSegments sg = dev_segments();
+  for (int i0 = 0; i0 < n; i0 += rows) {
+    const int r = std::min(rows, n - i0);
+    if (launch_adjoint_fill(sg, n, i0, r, rkh, adj_lam_.as<double>(), abc_.as<double>(), adj_gi_.as<double>(),
+                            adj_pj_.as<double>(), adj_gj_.as<double>(), stream_))
+      hip_check(hipGetLastError(), "adjoint fill launch");
+  }
+  hip_check(hipEventRecord(ev_[1], stream_), "event");
+  lam->resize(n);
+  x->resize(n);
+  gi->resize((size_t)n * 7);
+  gj->resize((size_t)n * 13);
+  hip_check(hipMemcpyAsync(lam->data(), adj_lam_.as<void>(), (size_t)n * 16, hipMemcpyDeviceToHost, stream_), "lam");
+  hip_check(hipMemcpyAsync(x->data(), x_.as<void>(), (size_t)n * 16, hipMemcpyDeviceToHost, stream_), "x");
+  hip_check(hipMemcpyAsync(gi->data(), adj_gi_.as<void>(), (size_t)n * 7 * 8, hipMemcpyDeviceToHost, stream_), "gi");
+  hip_check(hipMemcpyAsync(gj->data(), adj_gj_.as<void>(), (size_t)n * 13 * 8, hipMemcpyDeviceToHost, stream_), "gj");
+  hip_check(hipStreamSynchronize(stream_), "adjoint");
+  adjoint_ms_ = elapsed(ev_[0], ev_[1]);
+}
+
 void DeviceSolver::pattern(const PatternSpec& spec, const std::vector<double>& trig, HostPattern* out) {
   const int nd = spec.n_theta * spec.n_phi;
   upload(trig_, trig, stream_);

@@ -96,6 +96,15 @@ class DeviceSolver {
                           const std::vector<double>& trig, int warmup, int steps, double* ms_per_solve,
                           StageTimes* stage_ms, std::vector<double>* zin_out);
 
+  // Adjoint pass of the last solve, with the FP64 factors: lam = Z^-T g
+  // (g: n complex), then the fill's adjoint contraction (launch_adjoint_fill):
+  // gi (7 per segment) and gj (13 per segment).  x receives the solution of
+  // the last solve (n complex).
+  void adjoint(double rkh, const std::vector<zc>& g, std::vector<zc>* lam, std::vector<zc>* x,
+               std::vector<double>* gi, std::vector<double>* gj);
+  // time of the last adjoint pass (transposed solve + contraction), ms
+  double adjoint_ms() const { return adjoint_ms_; }
+
   void download_matrix(std::vector<double>* Z);  // current content of Z (2*n*n)
   int n() const { return n_; }
   int lu_info() const { return lu_info_; }
@@ -144,6 +153,8 @@ class DeviceSolver {
   DevBuf Z_, E_, piv_, info_, lu_work_, x_, abc_, curx_, solve_work_, rhs_, src_, zin_;
   DevBuf trig_, pat_part_, pat_out_, pat_sense_;
   DevBuf Z32_, b_, r_, acc_;  // mixed path: FP32 factors, right-hand side, residual, norms
+  DevBuf adj_g_, adj_lam_, adj_acc_, adj_gi_, adj_gj_, adj_pj_;
+  double adjoint_ms_ = 0.0;
 };
 
 }  // namespace necgpu

@@ -18,6 +18,9 @@
 // Both are compiled with -ffp-contract=off and follow the reference's
 // operation order, so entries agree with nec2++ to the ulp-level differences
 // of the device sin/cos/log.
+#include <type_traits>
+
+#include "dual.h"
 #include "kernels.h"
 #include "nec_math.h"
 
@@ -25,38 +28,46 @@ namespace necgpu {
 
 namespace {
 
-struct GfState {
+// The near-field routines below are templates over the scalar type T:
+// double for the fill, Dn<NV> (forward-mode derivatives) for the adjoint.
+// The double instantiation performs exactly the operations of the reference
+// routine it cites, in its order.
+template <class T>
+struct GfStateT {
   int ija;
-  double zpk, rkb2;
+  T zpk, rkb2;
 };
+using GfState = GfStateT<double>;
 
 // gf, src/nec_context.cpp:3553-3576
-__device__ __forceinline__ void gf(const GfState& st, double zk, double* co, double* si) {
-  const double zdk = zk - st.zpk;
-  const double rk = sqrt(st.rkb2 + zdk * zdk);
-  double s, c;
+template <class T>
+__device__ __forceinline__ void gf(const GfStateT<T>& st, const T& zk, T* co, T* si) {
+  const T zdk = zk - st.zpk;
+  const T rk = sqrt(st.rkb2 + zdk * zdk);
+  T s, c;
   sincos(rk, &s, &c);
   *si = s / rk;
   if (st.ija != 0) {
     *co = c / rk;
     return;
   }
-  if (rk >= .2) {
+  if (val(rk) >= .2) {
     *co = (c - 1.0) / rk;
     return;
   }
-  const double rks = rk * rk;
+  const T rks = rk * rk;
   *co = ((-1.38888889e-3 * rks + 4.16666667e-2) * rks - 0.5) * rk;
 }
 
 // test, src/c_evlcom.cpp:845-868 (dmin = 0)
-__device__ __forceinline__ void conv_test(double f1r, double f2r, double* tr, double f1i, double f2i, double* ti) {
-  double den = fabs(f2r);
-  const double t = fabs(f2i);
-  if (den < t) den = t;
-  if (den < 1.0e-37) {
-    *tr = 0.0;
-    *ti = 0.0;
+template <class T>
+__device__ __forceinline__ void conv_test(const T& f1r, const T& f2r, T* tr, const T& f1i, const T& f2i, T* ti) {
+  T den = fabs(f2r);
+  const T t = fabs(f2i);
+  if (val(den) < val(t)) den = t;
+  if (val(den) < 1.0e-37) {
+    *tr = T(0.0);
+    *ti = T(0.0);
     return;
   }
   *tr = fabs((f1r - f2r) / den);
@@ -65,27 +76,29 @@ __device__ __forceinline__ void conv_test(double f1r, double f2r, double* tr, do
 
 // intx, src/nec_context.cpp:4135-4297: variable-step Romberg integration of
 // the gf integrand; for the self term (ij == 0) half the segment plus the
-// analytic log term.
-__device__ void intx(const GfState& st, double el1, double el2, double b, int ij, double* sgr, double* sgi) {
+// analytic log term.  The step decisions use values only, so a Dn
+// instantiation differentiates the quadrature the double one computes.
+template <class T>
+__device__ void intx(const GfStateT<T>& st, const T& el1, const T& el2, const T& b, int ij, T* sgr, T* sgi) {
   const int nx = 1, nma = 65536, nts = 4;
   const double rx = 1.0e-4;
-  double z = el1;
-  double ze = el2;
-  if (ij == 0) ze = 0.;
-  const double s = ze - z;
+  T z = el1;
+  T ze = el2;
+  if (ij == 0) ze = T(0.);
+  const T s = ze - z;
   const double fnm = nma;
-  const double ep = s / (10. * fnm);
-  const double zend = ze - ep;
-  double accr = 0., acci = 0.;
+  const T ep = s / (10. * fnm);
+  const T zend = ze - ep;
+  T accr = T(0.), acci = T(0.);
   int ns = nx, nt = 0;
   bool fresh = true;
-  double dz = 0., dzot = 0.;
-  double g1r, g1i, g2r, g2i, g3r = 0, g3i = 0, g4r, g4i, g5r = 0, g5i = 0;
+  T dz = T(0.), dzot = T(0.);
+  T g1r, g1i, g2r, g2i, g3r = T(0), g3i = T(0), g4r, g4i, g5r = T(0), g5i = T(0);
   gf(st, z, &g1r, &g1i);
   // The step count is bounded by 65536 subdivisions of a finite interval;
   // the guard only stops a non-finite interval from spinning forever.
   long guard = 0;
-  if (!(fabs(s) < 1.0e300)) {
+  if (!(fabs(val(s)) < 1.0e300)) {
     *sgr = s;
     *sgi = s;
     return;
@@ -94,10 +107,10 @@ __device__ void intx(const GfState& st, double el1, double el2, double b, int ij
     if (fresh) {
       const double fns = ns;
       dz = s / fns;
-      double zp = z + dz;
-      if (zp > ze) {
+      T zp = z + dz;
+      if (val(zp) > val(ze)) {
         dz = ze - z;
-        if (fabs(dz) <= ep) break;
+        if (fabs(val(dz)) <= val(ep)) break;
       }
       dzot = dz * .5;
       zp = z + dzot;
@@ -105,20 +118,20 @@ __device__ void intx(const GfState& st, double el1, double el2, double b, int ij
       zp = z + dz;
       gf(st, zp, &g5r, &g5i);
     }
-    const double t00r = (g1r + g5r) * dzot;
-    const double t00i = (g1i + g5i) * dzot;
-    const double t01r = (t00r + dz * g3r) * 0.5;
-    const double t01i = (t00i + dz * g3i) * 0.5;
-    const double t10r = (4.0 * t01r - t00r) / 3.0;
-    const double t10i = (4.0 * t01i - t00i) / 3.0;
-    double te1r, te1i;
+    const T t00r = (g1r + g5r) * dzot;
+    const T t00i = (g1i + g5i) * dzot;
+    const T t01r = (t00r + dz * g3r) * 0.5;
+    const T t01i = (t00i + dz * g3i) * 0.5;
+    const T t10r = (4.0 * t01r - t00r) / 3.0;
+    const T t10i = (4.0 * t01i - t00i) / 3.0;
+    T te1r, te1i;
     conv_test(t01r, t10r, &te1r, t01i, t10i, &te1i);
-    if ((te1i <= rx) && (te1r <= rx)) {
+    if ((val(te1i) <= rx) && (val(te1r) <= rx)) {
       accr = accr + t10r;
       acci = acci + t10i;
       nt += 2;
-      z += dz;
-      if (z >= zend) break;
+      z = z + dz;
+      if (val(z) >= val(zend)) break;
       g1r = g5r;
       g1i = g5i;
       if (nt >= nts && ns > nx) {
@@ -128,19 +141,19 @@ __device__ void intx(const GfState& st, double el1, double el2, double b, int ij
       fresh = true;
       continue;
     }
-    double zp = z + dz * 0.25;
+    T zp = z + dz * 0.25;
     gf(st, zp, &g2r, &g2i);
     zp = z + dz * 0.75;
     gf(st, zp, &g4r, &g4i);
-    const double t02r = (t01r + dzot * (g2r + g4r)) * 0.5;
-    const double t02i = (t01i + dzot * (g2i + g4i)) * 0.5;
-    const double t11r = (4.0 * t02r - t01r) / 3.0;
-    const double t11i = (4.0 * t02i - t01i) / 3.0;
-    const double t20r = (16.0 * t11r - t10r) / 15.0;
-    const double t20i = (16.0 * t11i - t10i) / 15.0;
-    double te2r, te2i;
+    const T t02r = (t01r + dzot * (g2r + g4r)) * 0.5;
+    const T t02i = (t01i + dzot * (g2i + g4i)) * 0.5;
+    const T t11r = (4.0 * t02r - t01r) / 3.0;
+    const T t11i = (4.0 * t02i - t01i) / 3.0;
+    const T t20r = (16.0 * t11r - t10r) / 15.0;
+    const T t20i = (16.0 * t11i - t10i) / 15.0;
+    T te2r, te2i;
     conv_test(t11r, t20r, &te2r, t11i, t20i, &te2i);
-    if ((te2i > rx) || (te2r > rx)) {
+    if ((val(te2i) > rx) || (val(te2r) > rx)) {
       nt = 0;
       if (ns < nma) {
         ns = ns * 2;
@@ -159,8 +172,8 @@ __device__ void intx(const GfState& st, double el1, double el2, double b, int ij
     accr = accr + t20r;
     acci = acci + t20i;
     nt++;
-    z += dz;
-    if (z >= zend) break;
+    z = z + dz;
+    if (val(z) >= val(zend)) break;
     g1r = g5r;
     g1i = g5i;
     if (nt >= nts && ns > nx) {
@@ -178,15 +191,22 @@ __device__ void intx(const GfState& st, double el1, double el2, double b, int ij
 }
 
 // gx, src/nec_context.cpp:3605-3616
-__device__ __forceinline__ void gx(double zz, double rh, double xk, zc* gz, zc* gzp) {
-  const double r2 = zz * zz + rh * rh;
-  const double r = sqrt(r2);
-  const double rkz = xk * r;
-  double s, c;
+template <class T>
+__device__ __forceinline__ void gx_t(const T& zz, const T& rh, double xk, cx<T>* gz, cx<T>* gzp) {
+  const T r2 = zz * zz + rh * rh;
+  const T r = sqrt(r2);
+  const T rkz = xk * r;
+  T s, c;
   sincos(rkz, &s, &c);
-  const zc temp = zdivr(zmk(c, -s), r);
+  const cx<T> temp = cdivr(cmk<T>(c, -s), r);
   *gz = temp;
-  *gzp = zdivr(zmul(zneg(zmk(1.0, rkz)), temp), r2);
+  *gzp = cdivr(cmul(cneg(cmk<T>(T(1.0), rkz)), temp), r2);
+}
+__device__ __forceinline__ void gx(double zz, double rh, double xk, zc* gz, zc* gzp) {
+  cx<double> a, b;
+  gx_t<double>(zz, rh, xk, &a, &b);
+  *gz = zmk(a.re, a.im);
+  *gzp = zmk(b.re, b.im);
 }
 
 // gxx, src/nec_context.cpp:3623-3673
@@ -238,45 +258,51 @@ __device__ void gxx(double zz, double rh, double a, double a2, double xk, int ir
 struct NearOut {
   zc ezs, ers, ezc, erc, ezk, erk;
 };
+template <class T>
+struct NearOutT {
+  cx<T> ezs, ers, ezc, erc, ezk, erk;
+};
 
 // eksc, src/nec_context.cpp:3051-3100
-__device__ void eksc(double s, double z, double rh, double xk, int ij, NearOut* o) {
+template <class T>
+__device__ void eksc(const T& s, const T& z, const T& rh, double xk, int ij, NearOutT<T>* o) {
   const zc k1 = zmk(0.0, kNearConst), mk1 = zneg(k1);
-  GfState st;
+  GfStateT<T> st;
   st.ija = ij;
   st.zpk = xk * z;
-  const double rhk = xk * rh;
+  const T rhk = xk * rh;
   st.rkb2 = rhk * rhk;
-  const double sh = .5 * s;
-  const double shk = xk * sh;
-  double ss, cs;
+  const T sh = .5 * s;
+  const T shk = xk * sh;
+  T ss, cs;
   sincos(shk, &ss, &cs);
-  const double z2a = sh - z;
-  const double z1a = -(sh + z);
-  zc gz1, gz2, gp1, gp2;
-  gx(z1a, rh, xk, &gz1, &gp1);
-  gx(z2a, rh, xk, &gz2, &gp2);
-  zc gzp1 = zmulr(gp1, z1a);
-  zc gzp2 = zmulr(gp2, z2a);
-  o->ezs = zmul(k1, zsub(zmulr(zmulr(zsub(gz2, gz1), cs), xk), zmulr(zadd(gzp2, gzp1), ss)));
-  o->ezc = zmul(mk1, zadd(zmulr(zmulr(zadd(gz2, gz1), ss), xk), zmulr(zsub(gzp2, gzp1), cs)));
-  o->erk = zmulr(zmul(k1, zsub(gp2, gp1)), rh);
-  double cint, sint;
+  const T z2a = sh - z;
+  const T z1a = -(sh + z);
+  cx<T> gz1, gz2, gp1, gp2;
+  gx_t(z1a, rh, xk, &gz1, &gp1);
+  gx_t(z2a, rh, xk, &gz2, &gp2);
+  cx<T> gzp1 = cmulr(gp1, z1a);
+  cx<T> gzp2 = cmulr(gp2, z2a);
+  const T txk = T(xk);
+  o->ezs = cmulc(k1, csub(cmulr(cmulr(csub(gz2, gz1), cs), txk), cmulr(cadd(gzp2, gzp1), ss)));
+  o->ezc = cmulc(mk1, cadd(cmulr(cmulr(cadd(gz2, gz1), ss), txk), cmulr(csub(gzp2, gzp1), cs)));
+  o->erk = cmulr(cmulc(k1, csub(gp2, gp1)), rh);
+  T cint, sint;
   intx(st, -shk, shk, rhk, ij, &cint, &sint);
-  o->ezk = zmul(mk1, zadd(zsub(gzp2, gzp1), zmulr(zmk(cint, -sint), xk * xk)));
-  if (rh >= 1.0e-10) {
-    gzp1 = zmulr(gzp1, z1a);
-    gzp2 = zmulr(gzp2, z2a);
-    const zc t1 = zmulr(zadd(zadd(zadd(gzp2, gzp1), gz2), gz1), ss);
-    const zc t2 = zmulr(zmulr(zsub(zmulr(gz2, z2a), zmulr(gz1, z1a)), cs), xk);
-    o->ers = zdivr(zmul(mk1, zsub(t1, t2)), rh);
-    const zc u1 = zmulr(zsub(zadd(zsub(gzp2, gzp1), gz2), gz1), cs);
-    const zc u2 = zmulr(zmulr(zadd(zmulr(gz2, z2a), zmulr(gz1, z1a)), ss), xk);
-    o->erc = zdivr(zmul(mk1, zadd(u1, u2)), rh);
+  o->ezk = cmulc(mk1, cadd(csub(gzp2, gzp1), cmulr(cmk<T>(cint, -sint), T(xk * xk))));
+  if (val(rh) >= 1.0e-10) {
+    gzp1 = cmulr(gzp1, z1a);
+    gzp2 = cmulr(gzp2, z2a);
+    const cx<T> t1 = cmulr(cadd(cadd(cadd(gzp2, gzp1), gz2), gz1), ss);
+    const cx<T> t2 = cmulr(cmulr(csub(cmulr(gz2, z2a), cmulr(gz1, z1a)), cs), txk);
+    o->ers = cdivr(cmulc(mk1, csub(t1, t2)), rh);
+    const cx<T> u1 = cmulr(csub(cadd(csub(gzp2, gzp1), gz2), gz1), cs);
+    const cx<T> u2 = cmulr(cmulr(cadd(cmulr(gz2, z2a), cmulr(gz1, z1a)), ss), txk);
+    o->erc = cdivr(cmulc(mk1, cadd(u1, u2)), rh);
     return;
   }
-  o->ers = zmk(0., 0.);
-  o->erc = zmk(0., 0.);
+  o->ers = cmk<T>(T(0.), T(0.));
+  o->erc = cmk<T>(T(0.), T(0.));
 }
 
 // ekscx, src/nec_context.cpp:3108-3188
@@ -348,75 +374,106 @@ __device__ void ekscx(double bx, double s, double z, double rhx, double xk, int 
 // Fields of the constant (k), sine (s) and cosine (c) currents of source
 // segment j at observation segment i, projected on i's direction
 // (efld_compute, src/nec_context.cpp:2777-2837, and the projection of cmww,
-// :2617-2619).
-__device__ void pair_fields(const DevSegments& g, int i, int j, double rkh, int use_ek, int ind1, int ind2, zc* etk,
-                            zc* ets, zc* etc) {
-  const double cabj = g.cab[j], sabj = g.sab[j], salpj = g.salp[j];
-  const double sj = g.s[j];
-  const double xij = g.x[i] - g.x[j];
-  const double yij = g.y[i] - g.y[j];
-  const double zij = g.z[i] - g.z[j];
-  const double ai = g.b[i];
-  const double zp = xij * cabj + yij * sabj + zij * salpj;
-  double rhox = xij - cabj * zp;
-  double rhoy = yij - sabj * zp;
-  double rhoz = zij - salpj * zp;
-  const double rh = sqrt(rhox * rhox + rhoy * rhoy + rhoz * rhoz + ai * ai);
-  if (rh <= 1.e-10) {
-    rhox = 0.0;
-    rhoy = 0.0;
-    rhoz = 0.0;
+// :2617-2619).  Segment data by value, so that a Dn instantiation
+// differentiates with respect to the centres, the observation radius, the
+// source length and the source direction (the observation direction enters
+// only the final projection; evec, when given, receives the nine unprojected
+// field components e0..e8 for its derivative).  The extended kernel (EK
+// card) exists for double only.
+template <class T>
+__device__ void pair_fields_v(const T& xi_, const T& yi_, const T& zi_, const T& ai, double cabi, double sabi,
+                              double salpi, const T& xj_, const T& yj_, const T& zj_, const T& sj, const T& bj,
+                              const T& cabj, const T& sabj, const T& salpj, bool self, double rkh, int use_ek,
+                              int ind1, int ind2, cx<T>* etk, cx<T>* ets, cx<T>* etc, cx<double>* evec = nullptr) {
+  const T xij = xi_ - xj_;
+  const T yij = yi_ - yj_;
+  const T zij = zi_ - zj_;
+  const T zp = xij * cabj + yij * sabj + zij * salpj;
+  T rhox = xij - cabj * zp;
+  T rhoy = yij - sabj * zp;
+  T rhoz = zij - salpj * zp;
+  const T rh = sqrt(rhox * rhox + rhoy * rhoy + rhoz * rhoz + ai * ai);
+  if (val(rh) <= 1.e-10) {
+    rhox = T(0.0);
+    rhoy = T(0.0);
+    rhoz = T(0.0);
   } else {
-    rhox /= rh;
-    rhoy /= rh;
-    rhoz /= rh;
+    rhox = rhox / rh;
+    rhoy = rhoy / rh;
+    rhoz = rhoz / rh;
   }
-  const double r = sqrt(zp * zp + rh * rh);
-  zc tezk, terk, tezc, terc;
-  zc e3 = zmk(0, 0), e4 = zmk(0, 0), e5 = zmk(0, 0);
-  if (r >= rkh) {
+  const T r = sqrt(zp * zp + rh * rh);
+  cx<T> tezk, terk, tezc, terc;
+  const cx<T> zero = cmk<T>(T(0), T(0));
+  cx<T> e3 = zero, e4 = zero, e5 = zero;
+  if (val(r) >= rkh) {
     // lumped current element, src/nec_context.cpp:2798-2816
     const double eta = em_impedance();
-    double rmag = kTwoPi * r;
-    const double cth = zp / r;
-    const double px = rh / r;
-    double sr, cr;
+    T rmag = kTwoPi * r;
+    const T cth = zp / r;
+    const T px = rh / r;
+    T sr, cr;
     sincos(rmag, &sr, &cr);
-    const zc e0 = zmk(cr, -sr);
-    const double py = kTwoPi * r * r;
-    const zc e1 = zdivr(zmul(zmulr(e0, eta * cth), zmk(1.0, -1.0 / rmag)), py);
-    const zc e2 = zdivr(zmul(zmulr(e0, eta * px), zmk(1.0, rmag - 1.0 / rmag)), 2. * py);
-    tezk = zsub(zmulr(e1, cth), zmulr(e2, px));
-    terk = zadd(zmulr(e1, px), zmulr(e2, cth));
+    const cx<T> e0 = cmk<T>(cr, -sr);
+    const T py = kTwoPi * r * r;
+    const cx<T> e1 = cdivr(cmul(cmulr(e0, eta * cth), cmk<T>(T(1.0), -1.0 / rmag)), py);
+    const cx<T> e2 = cdivr(cmul(cmulr(e0, eta * px), cmk<T>(T(1.0), rmag - 1.0 / rmag)), 2. * py);
+    tezk = csub(cmulr(e1, cth), cmulr(e2, px));
+    terk = cadd(cmulr(e1, px), cmulr(e2, cth));
     rmag = sin(kPi * sj) / kPi;
-    tezc = zmulr(tezk, rmag);
-    terc = zmulr(terk, rmag);
-    tezk = zmulr(tezk, sj);
-    terk = zmulr(terk, sj);
+    tezc = cmulr(tezk, rmag);
+    terc = cmulr(terk, rmag);
+    tezk = cmulr(tezk, sj);
+    terk = cmulr(terk, sj);
   } else {
-    NearOut o;
-    if (!use_ek)
-      eksc(sj, zp, rh, kTwoPi, i != j, &o);
-    else
-      ekscx(g.b[j], sj, zp, rh, kTwoPi, i != j, ind1, ind2, &o);
+    NearOutT<T> o;
+    if constexpr (std::is_same<T, double>::value) {
+      if (use_ek) {
+        NearOut ox;
+        ekscx(val(bj), val(sj), val(zp), val(rh), kTwoPi, !self, ind1, ind2, &ox);
+        o.ezs = cmk(ox.ezs.re, ox.ezs.im);
+        o.ers = cmk(ox.ers.re, ox.ers.im);
+        o.ezc = cmk(ox.ezc.re, ox.ezc.im);
+        o.erc = cmk(ox.erc.re, ox.erc.im);
+        o.ezk = cmk(ox.ezk.re, ox.ezk.im);
+        o.erk = cmk(ox.erk.re, ox.erk.im);
+      } else {
+        eksc(sj, zp, rh, kTwoPi, !self, &o);
+      }
+    } else {
+      eksc(sj, zp, rh, kTwoPi, !self, &o);
+    }
     tezk = o.ezk;
     terk = o.erk;
     tezc = o.ezc;
     terc = o.erc;
-    e3 = zadd(zmulr(o.ezs, cabj), zmulr(o.ers, rhox));
-    e4 = zadd(zmulr(o.ezs, sabj), zmulr(o.ers, rhoy));
-    e5 = zadd(zmulr(o.ezs, salpj), zmulr(o.ers, rhoz));
+    e3 = cadd(cmulr(o.ezs, cabj), cmulr(o.ers, rhox));
+    e4 = cadd(cmulr(o.ezs, sabj), cmulr(o.ers, rhoy));
+    e5 = cadd(cmulr(o.ezs, salpj), cmulr(o.ers, rhoz));
   }
-  const zc e0 = zadd(zmulr(tezk, cabj), zmulr(terk, rhox));
-  const zc e1 = zadd(zmulr(tezk, sabj), zmulr(terk, rhoy));
-  const zc e2 = zadd(zmulr(tezk, salpj), zmulr(terk, rhoz));
-  const zc e6 = zadd(zmulr(tezc, cabj), zmulr(terc, rhox));
-  const zc e7 = zadd(zmulr(tezc, sabj), zmulr(terc, rhoy));
-  const zc e8 = zadd(zmulr(tezc, salpj), zmulr(terc, rhoz));
-  const double cabi = g.cab[i], sabi = g.sab[i], salpi = g.salp[i];
-  *etk = zadd(zadd(zmulr(e0, cabi), zmulr(e1, sabi)), zmulr(e2, salpi));
-  *ets = zadd(zadd(zmulr(e3, cabi), zmulr(e4, sabi)), zmulr(e5, salpi));
-  *etc = zadd(zadd(zmulr(e6, cabi), zmulr(e7, sabi)), zmulr(e8, salpi));
+  const cx<T> e0 = cadd(cmulr(tezk, cabj), cmulr(terk, rhox));
+  const cx<T> e1 = cadd(cmulr(tezk, sabj), cmulr(terk, rhoy));
+  const cx<T> e2 = cadd(cmulr(tezk, salpj), cmulr(terk, rhoz));
+  const cx<T> e6 = cadd(cmulr(tezc, cabj), cmulr(terc, rhox));
+  const cx<T> e7 = cadd(cmulr(tezc, sabj), cmulr(terc, rhoy));
+  const cx<T> e8 = cadd(cmulr(tezc, salpj), cmulr(terc, rhoz));
+  if (evec) {
+    const cx<T>* ek[9] = {&e0, &e1, &e2, &e3, &e4, &e5, &e6, &e7, &e8};
+    for (int q = 0; q < 9; ++q) evec[q] = cmk<double>(val(ek[q]->re), val(ek[q]->im));
+  }
+  *etk = cadd(cadd(cmulr(e0, T(cabi)), cmulr(e1, T(sabi))), cmulr(e2, T(salpi)));
+  *ets = cadd(cadd(cmulr(e3, T(cabi)), cmulr(e4, T(sabi))), cmulr(e5, T(salpi)));
+  *etc = cadd(cadd(cmulr(e6, T(cabi)), cmulr(e7, T(sabi))), cmulr(e8, T(salpi)));
+}
+
+__device__ void pair_fields(const DevSegments& g, int i, int j, double rkh, int use_ek, int ind1, int ind2, zc* etk,
+                            zc* ets, zc* etc) {
+  cx<double> a, b, c;
+  pair_fields_v<double>(g.x[i], g.y[i], g.z[i], g.b[i], g.cab[i], g.sab[i], g.salp[i], g.x[j], g.y[j], g.z[j],
+                        g.s[j], g.b[j], g.cab[j], g.sab[j], g.salp[j], i == j, rkh, use_ek, ind1, ind2, &a, &b, &c);
+  *etk = zmk(a.re, a.im);
+  *ets = zmk(b.re, b.im);
+  *etc = zmk(c.re, c.im);
 }
 
 __global__ void __launch_bounds__(256) fill_fields_kernel(DevSegments g, int n, int i0, int rows, double rkh,
@@ -525,7 +582,139 @@ __global__ void __launch_bounds__(256) batch_fill_kernel(const BatchSolve* __res
   }
 }
 
+// Adjoint contraction of the fill for one structure.  With lam the adjoint
+// vector (Z^T lam = dJ/dx) and U_j = (A_j, B_j, C_j) the current
+// coefficients of segment j (abc), pair (i, j) adds
+//   -2 Re( lam_i * d(E_ij . U_j) / dv )
+// to the gradient of each geometry variable v, where E_ij = (etk, ets, etc)
+// are the fields of pair_fields_v.  The fields are differentiated on the fly
+// (Dn, no dZ/dv is stored): with respect to the centre difference c_i - c_j
+// (the fields depend on the centres through it only), the observation
+// radius, the source length and the source direction; the observation
+// direction enters through the projection of the unprojected components.
+// One workgroup per observation row i (rows i0 .. i0+rows-1), threads over
+// source segments j.  Row-side sums go to gi (7 per row: d/dc_i x y z,
+// d/db_i, d/d(cab, sab, salp)_i); source-side terms of each pair go to pj
+// (13 per pair: d/dc_j x y z, d/ds_j, d/d(cab, sab, salp)_j, and
+// lam_i E_ij as 3 complex) and are summed over rows by adjoint_reduce in
+// row order, so the result does not depend on scheduling.
+constexpr int ADJ_NV = 8;  // dx, dy, dz (c_i - c_j), b_i, s_j, cab_j, sab_j, salp_j
+constexpr int ADJ_PJ = 13;
+constexpr int ADJ_GI = 7;
+constexpr int ADJ_THREADS = 256;
+__global__ void __launch_bounds__(ADJ_THREADS) adjoint_fill_kernel(DevSegments g, int n, int i0, double rkh,
+                                                                   const double* __restrict__ lam,
+                                                                   const double* __restrict__ abc,
+                                                                   double* __restrict__ gi, double* __restrict__ pj) {
+  typedef Dn<ADJ_NV> D;
+  __shared__ double sred[ADJ_GI][ADJ_THREADS / 64];
+  const int r = blockIdx.x;
+  const int i = i0 + r;
+  const double lr = lam[2 * i], li = lam[2 * i + 1];
+  const double cabi = g.cab[i], sabi = g.sab[i], salpi = g.salp[i];
+  double acc[ADJ_GI] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    const bool self = (i == j);
+    // positions enter as c_i = c_j + (c_i - c_j): seed the difference; a
+    // segment's own pair has no difference to vary
+    D dx(g.x[i] - g.x[j]), dy(g.y[i] - g.y[j]), dz(g.z[i] - g.z[j]);
+    D ai(g.b[i]), sj(g.s[j]), bj(g.b[j]), cj(g.cab[j]), sbj(g.sab[j]), slj(g.salp[j]);
+    if (!self) {
+      dx.d[0] = 1.0;
+      dy.d[1] = 1.0;
+      dz.d[2] = 1.0;
+    }
+    ai.d[3] = 1.0;
+    sj.d[4] = 1.0;
+    cj.d[5] = 1.0;
+    sbj.d[6] = 1.0;
+    slj.d[7] = 1.0;
+    const D zero(0.0);
+    cx<D> etk, ets, etc;
+    cx<double> ev[9];
+    pair_fields_v<D>(dx, dy, dz, ai, cabi, sabi, salpi, zero, zero, zero, sj, bj, cj, sbj, slj, self, rkh, 0, 2, 2,
+                     &etk, &ets, &etc, ev);
+    const double Ar = abc[j], Ai = abc[n + j], Br = abc[2 * n + j], Bi = abc[3 * n + j], Cr = abc[4 * n + j],
+                 Ci = abc[5 * n + j];
+    // -2 Re(lam * (dk A + ds B + dc C)) for complex derivative triples
+    auto contract = [&](double kr, double ki, double sr, double si, double cr, double ci) {
+      const double fr = ((kr * Ar - ki * Ai) + (sr * Br - si * Bi)) + (cr * Cr - ci * Ci);
+      const double fi = ((kr * Ai + ki * Ar) + (sr * Bi + si * Br)) + (cr * Ci + ci * Cr);
+      return -2.0 * (lr * fr - li * fi);
+    };
+    double cv[ADJ_NV];
+    for (int v = 0; v < ADJ_NV; ++v)
+      cv[v] = contract(etk.re.d[v], etk.im.d[v], ets.re.d[v], ets.im.d[v], etc.re.d[v], etc.im.d[v]);
+    // observation direction: etk = e0 cab + e1 sab + e2 salp (e3..e5: ets,
+    // e6..e8: etc)
+    double cdi[3];
+    for (int q = 0; q < 3; ++q)
+      cdi[q] = contract(ev[q].re, ev[q].im, ev[3 + q].re, ev[3 + q].im, ev[6 + q].re, ev[6 + q].im);
+    acc[0] += cv[0];
+    acc[1] += cv[1];
+    acc[2] += cv[2];
+    acc[3] += cv[3];
+    acc[4] += cdi[0];
+    acc[5] += cdi[1];
+    acc[6] += cdi[2];
+    double* o = pj + ((size_t)r * n + j) * ADJ_PJ;
+    o[0] = -cv[0];
+    o[1] = -cv[1];
+    o[2] = -cv[2];
+    o[3] = cv[4];
+    o[4] = cv[5];
+    o[5] = cv[6];
+    o[6] = cv[7];
+    o[7] = lr * etk.re.v - li * etk.im.v;
+    o[8] = lr * etk.im.v + li * etk.re.v;
+    o[9] = lr * ets.re.v - li * ets.im.v;
+    o[10] = lr * ets.im.v + li * ets.re.v;
+    o[11] = lr * etc.re.v - li * etc.im.v;
+    o[12] = lr * etc.im.v + li * etc.re.v;
+  }
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  for (int v = 0; v < ADJ_GI; ++v) {
+    double x = acc[v];
+    for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
+    if (lane == 0) sred[v][wave] = x;
+  }
+  __syncthreads();
+  if (threadIdx.x < ADJ_GI) {
+    double x = 0.0;
+    for (int w = 0; w < ADJ_THREADS / 64; ++w) x += sred[threadIdx.x][w];
+    gi[(size_t)i * ADJ_GI + threadIdx.x] = x;
+  }
+}
+
+// gj[j][c] += sum over the rows of a chunk, in row order.
+__global__ void __launch_bounds__(256) adjoint_reduce_kernel(const double* __restrict__ pj, int n, int rows,
+                                                             double* __restrict__ gj) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n * ADJ_PJ) return;
+  double x = gj[t];
+  for (int r = 0; r < rows; ++r) x += pj[(size_t)r * n * ADJ_PJ + t];
+  gj[t] = x;
+}
+
 }  // namespace
+
+int adjoint_rows_per_chunk(int n, size_t budget_bytes) {
+  const size_t per_row = (size_t)n * ADJ_PJ * sizeof(double);
+  size_t rows = budget_bytes / (per_row ? per_row : 1);
+  if (rows < 1) rows = 1;
+  if (rows > (size_t)n) rows = n;
+  return static_cast<int>(rows);
+}
+
+int launch_adjoint_fill(const DevSegments& g, int n, int i0, int rows, double rkh, const double* lam,
+                        const double* abc, double* gi, double* pj, double* gj, hipStream_t stream) {
+  if (n <= 0 || rows <= 0) return 0;
+  launch_k(K_ADJOINT_FILL, adjoint_fill_kernel, dim3(rows), dim3(ADJ_THREADS), 0u, stream, g, n, i0, rkh, lam, abc,
+           gi, pj);
+  launch_k(K_ADJOINT_REDUCE, adjoint_reduce_kernel, dim3((n * ADJ_PJ + 255) / 256), dim3(256), 0u, stream, pj, n,
+           rows, gj);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
 
 int launch_batch_fill(const BatchSolve* sol, int S, long long rows_total, const double* seg, const int* colptr,
                       const int* colj, const double* colabc, double* Z, double rkh, hipStream_t stream) {

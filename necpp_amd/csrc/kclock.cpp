@@ -10,7 +10,8 @@ const char* kernel_kind_name(int kind) {
                                         "lu_laswp",      "lu_trsm",       "zgemm",      "solve_reset", "diag_inverse",
                                         "trsv_lower",    "trsv_upper",    "currents",   "zin",
                                         "pattern_partial", "pattern_finish", "refine", "residual",
-                                        "batch_fill", "batch_lu", "batch_solve", "batch_pattern"};
+                                        "batch_fill", "batch_lu", "batch_solve", "batch_pattern",
+                                        "adjoint_fill", "adjoint_solve", "adjoint_reduce"};
   return (kind >= 0 && kind < K_NKINDS) ? names[kind] : "?";
 }
 

@@ -33,6 +33,9 @@ enum KernelKind {
   K_BATCH_LU,
   K_BATCH_SOLVE,
   K_BATCH_PATTERN,
+  K_ADJOINT_FILL,
+  K_ADJOINT_SOLVE,
+  K_ADJOINT_REDUCE,
   K_NKINDS
 };
 const char* kernel_kind_name(int kind);
@@ -151,6 +154,17 @@ int launch_lu_solve_apply_f32(const float* LU, long long lda, int n, const int* 
 // prepare + apply
 int launch_lu_solve_fp64(const double* LU, long long lda, int n, const int* piv, double* x, void* work,
                          unsigned epoch, hipStream_t stream);
+
+// Adjoint (gradient) path.  solve.hip: Z^T lam = g with the FP64 factors
+// (acc: n complex of scratch).  fill.hip: the fill's adjoint contraction
+// for observation rows i0 .. i0+rows-1: gi (7 per segment, written for
+// those rows) and gj (13 per segment, accumulated: zero it before the
+// first chunk); pj holds rows * n * 13 doubles of pair terms.
+int launch_lu_solve_transposed(const double* LU, long long lda, int n, const int* piv, const double* g, double* lam,
+                               double* acc, hipStream_t stream);
+int adjoint_rows_per_chunk(int n, size_t budget_bytes);
+int launch_adjoint_fill(const DevSegments& g, int n, int i0, int rows, double rkh, const double* lam,
+                        const double* abc, double* gi, double* pj, double* gj, hipStream_t stream);
 
 // refine.hip: the mixed path's FP32 copy of Z, FP64 residual r = b - Z x
 // (acc[0] = |r|^2, acc[1] = |b|^2 when with_b) and update x += d.

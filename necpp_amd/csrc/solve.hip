@@ -256,6 +256,53 @@ __global__ void __launch_bounds__(256) trsv_kernel(const TF* __restrict__ LU, lo
   if (t == 0) __hip_atomic_store(flags + b, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
+// Z^T lam = g with the FP64 factors (P Z = L U, perm = piv + n):
+// U^T w = g (forward, U^T lower), L^T v = w (backward, unit), lam[perm[i]]
+// = v[i].  One workgroup; each step reads one contiguous row of the factors.
+// The adjoint's transposed solve for small structures.
+__global__ void __launch_bounds__(1024) lu_solve_transposed_kernel(const double* __restrict__ LU, long long lda,
+                                                                    int n, const int* __restrict__ perm,
+                                                                    const double* __restrict__ g,
+                                                                    double* __restrict__ lam, double* acc) {
+  const int t = threadIdx.x;
+  for (int i = t; i < n; i += blockDim.x) {
+    acc[2 * i] = g[2 * i];
+    acc[2 * i + 1] = g[2 * i + 1];
+  }
+  __syncthreads();
+  for (int q = 0; q < n; ++q) {
+    if (t == 0) {
+      const double* u = LU + 2 * ((size_t)q * lda + q);
+      const zc w = zdiv(zmk(acc[2 * q], acc[2 * q + 1]), zmk(u[0], u[1]));
+      acc[2 * q] = w.re;
+      acc[2 * q + 1] = w.im;
+    }
+    __syncthreads();
+    const double wr = acc[2 * q], wi = acc[2 * q + 1];
+    const double* row = LU + 2 * (size_t)q * lda;
+    for (int i = q + 1 + t; i < n; i += blockDim.x) {
+      const double ur = row[2 * i], ui = row[2 * i + 1];
+      acc[2 * i] -= ur * wr - ui * wi;
+      acc[2 * i + 1] -= ur * wi + ui * wr;
+    }
+    __syncthreads();
+  }
+  for (int q = n - 1; q >= 0; --q) {
+    const double vr = acc[2 * q], vi = acc[2 * q + 1];
+    const double* row = LU + 2 * (size_t)q * lda;
+    for (int i = t; i < q; i += blockDim.x) {
+      const double lr = row[2 * i], li = row[2 * i + 1];
+      acc[2 * i] -= lr * vr - li * vi;
+      acc[2 * i + 1] -= lr * vi + li * vr;
+    }
+    __syncthreads();
+  }
+  for (int i = t; i < n; i += blockDim.x) {
+    lam[2 * perm[i]] = acc[2 * i];
+    lam[2 * perm[i] + 1] = acc[2 * i + 1];
+  }
+}
+
 __global__ void reset_status_kernel(int* status) {
   if (threadIdx.x == 0) *status = 0;
 }
@@ -317,6 +364,14 @@ size_t solve_workspace_bytes(int n) {
 }
 
 int* solve_status(void* work, int n) { return carve_solve(work, n).status; }
+
+int launch_lu_solve_transposed(const double* LU, long long lda, int n, const int* piv, const double* g, double* lam,
+                               double* acc, hipStream_t stream) {
+  if (n <= 0) return 0;
+  launch_k(K_ADJOINT_SOLVE, lu_solve_transposed_kernel, dim3(1), dim3(1024), 0u, stream, LU, lda, n, piv + n, g, lam,
+           acc);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
 
 int launch_lu_solve_prepare(const double* LU, long long lda, int n, void* work, hipStream_t stream) {
   return prepare<double>(LU, lda, n, work, stream);

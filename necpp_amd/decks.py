@@ -121,6 +121,57 @@ def yagi_population(count=512, seed=BATCH_SEED, n_freq=64, f_lo=140.0, f_hi=150.
     return designs, freqs, theta, phi
 
 
+ADJOINT_SEED = 20261204
+ADJOINT_FREQ_MHZ = 144.0
+
+
+def c4_lambda(freq_mhz=ADJOINT_FREQ_MHZ):
+    """Wavelength (m) of nec2++ at freq_mhz."""
+    return C_NEC / (freq_mhz * 1.0e6)
+
+
+def c4_params(count=256, seed=ADJOINT_SEED, n_elem=15):
+    """The differentiation workload's design parameters: `count` rows of
+    3 n_elem - 1 values in wavelengths, drawn from numpy's PCG64 generator
+    seeded with `seed`, per design in order: n_elem half-lengths ~ U(0.20,
+    0.26), n_elem - 1 spacings ~ U(0.10, 0.30), n_elem radii ~ U(0.002,
+    0.005)."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    rows = []
+    for _ in range(count):
+        h = rng.uniform(0.20, 0.26, n_elem)
+        d = rng.uniform(0.10, 0.30, n_elem - 1)
+        r = rng.uniform(0.002, 0.005, n_elem)
+        rows.append(np.concatenate([h, d, r]))
+    return np.array(rows)
+
+
+def c4_deck(p, freq_mhz=ADJOINT_FREQ_MHZ, seg_per_element=31):
+    """Yagi-Uda deck of one parameter row (c4_params): elements parallel to
+    y, boom along x, element k at x = sum of the first k spacings, source on
+    the centre segment of element 2; pattern cut: the single direction
+    theta = 90, phi = 0 (forward, +x)."""
+    lam = c4_lambda(freq_mhz)
+    ne = (len(p) + 1) // 3
+    h, d, r = p[:ne], p[ne:2 * ne - 1], p[2 * ne - 1:]
+    wires = []
+    x = 0.0
+    for e in range(ne):
+        if e > 0:
+            x += d[e - 1] * lam
+        wires.append((e + 1, seg_per_element, x, -h[e] * lam, 0.0, x, h[e] * lam, 0.0, r[e] * lam, 1.0, 1.0))
+    return {
+        "name": f"yagi{ne}x{seg_per_element}-adjoint",
+        "wires": wires,
+        "gpflag": 0,
+        "freq_mhz": freq_mhz,
+        "source": (2, seg_per_element // 2 + 1),
+        "voltage": 1.0 + 0.0j,
+        "rp": (0, 1, 1, 1, 0, 0, 0, 90.0, 0.0, 0.0, 0.0, 0.0, 0.0),
+    }
+
+
 def apply_to_api(deck, api, ctx, with_rp=True):
     """Run the deck through a libnecpp-style API object (module or wrapper)."""
     for w in deck["wires"]:

@@ -63,6 +63,7 @@ _SIGS = [
     ("nec_gpu_batch_solve", _L, [_P, _P, _P, _P]),
     ("nec_gpu_set_refine_tolerance", _L, [_P, _D]),
     ("nec_gpu_refinement", _L, [_P, _P, _P]),
+    ("nec_gpu_adjoint_gradient", _L, [_P, _D, _D, _D, _D, _D, _P, _P, _P, _P]),
     ("nec_gpu_currents", _L, [_P, _I, _P, _L]),
     ("nec_gpu_stage_times", _L, [_P, _P]),
     ("nec_gpu_benchmark_solve", _L, [_P, _I, _I, _P]),
@@ -182,6 +183,25 @@ def gpu_refinement(ctx):
     if lib().nec_gpu_refinement(ctx, ctypes.byref(steps), ctypes.byref(res)) != 0:
         raise RuntimeError(nec_error_message())
     return steps.value, res.value
+
+
+ADJOINT_INFO = ("gain_db", "zin_re", "zin_im", "host_ms", "device_ms", "dJ_dlambda")
+
+
+def gpu_adjoint_gradient(ctx, theta_deg=90.0, phi_deg=0.0, weight=0.01, z0=50.0):
+    """Adjoint gradient of J = -G(theta, phi) + weight |Z_in - z0|^2 for the
+    last solve of ctx.  Returns (J, grad [n, 7] per metre: x1 y1 z1 x2 y2 z2
+    radius of every segment, dJ/d(frequency MHz), info dict)."""
+    n = lib().nec_gpu_segment_count(ctx)
+    J = ctypes.c_double(0.0)
+    df = ctypes.c_double(0.0)
+    g = np.zeros(7 * max(n, 1))
+    info = np.zeros(6)
+    z0 = complex(z0)
+    if lib().nec_gpu_adjoint_gradient(ctx, theta_deg, phi_deg, weight, z0.real, z0.imag, ctypes.byref(J), _ptr(g),
+                                      ctypes.byref(df), _ptr(info)) != 0:
+        raise RuntimeError(nec_error_message())
+    return J.value, g[:7 * n].reshape(n, 7), df.value, dict(zip(ADJOINT_INFO, info.tolist()))
 
 
 class BatchDesc(ctypes.Structure):
