@@ -149,6 +149,7 @@ def solve_times(deck, precision, steps):
         api.nec_geometry_complete(ctx, deck["gpflag"])
         if api.nec_gpu_set_precision(ctx, precision) != 0:
             raise RuntimeError(api.nec_error_message())
+        api.nec_gpu_set_kernel_timing(ctx, 0)
         api.nec_fr_card(ctx, 0, 1, deck["freq_mhz"], 0.0)
         s, v = deck["source"], deck["voltage"]
         api.nec_ex_card(ctx, 0, s[0], s[1], 0, v.real, v.imag, 0, 0, 0, 0)
@@ -282,13 +283,20 @@ def main():
     ctx = api.nec_create()
     decks.apply_to_api(deck, api, ctx)  # one solve through the C API (loads HBM)
     z_api = complex(api.nec_impedance_real(ctx, 0), api.nec_impedance_imag(ctx, 0))
+    # per-kernel durations (HIP events on every launch) from a few steps of
+    # their own; the timed steps then run without those events
+    kt_steps = max(1, min(a.steps, 5))
     out = np.zeros(8)
+    if api.nec_gpu_benchmark_solve(ctx, 1, kt_steps, out.ctypes.data) != 0:
+        raise RuntimeError(api.nec_error_message())
+    ms_with_kernel_events = float(out[0])
+    ktimes = api.gpu_kernel_times(ctx)
+    api.nec_gpu_set_kernel_timing(ctx, 0)
     if api.nec_gpu_benchmark_solve(ctx, a.warmup, a.steps, out.ctypes.data) != 0:
         raise RuntimeError(api.nec_error_message())
     ms = float(out[0])
     fill_ms, lu_ms, solve_ms, pat_ms = (float(v) for v in out[1:5])
     z_bench = complex(out[5], out[6])
-    ktimes = api.gpu_kernel_times(ctx)
     api.nec_delete(ctx)
 
     lu_flops = 8.0 * n ** 3 / 3.0
@@ -298,20 +306,22 @@ def main():
     for name, (launches, total) in ktimes.items():
         if launches == 0:
             continue
-        e = {"launches_per_step": launches / a.steps, "ms_per_step": total / a.steps,
+        e = {"launches_per_step": launches / kt_steps, "ms_per_step": total / kt_steps,
              "avg_us": 1000.0 * total / launches}
         if name in work:
             kind, amount = work[name]
-            per_launch = amount / (launches / a.steps)
+            per_launch = amount / (launches / kt_steps)
             if kind == "flop":
                 e["achieved_tflops"] = per_launch / (total / launches * 1e-3) / 1e12
             else:
                 e["achieved_gbs"] = per_launch / (total / launches * 1e-3) / 1e9
         kernels[name] = e
-    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"])
-    dk = kernels[dom]
+    dom = max(kernels, key=lambda k: kernels[k]["ms_per_step"]) if kernels else None
+    dk = kernels.get(dom, {"avg_us": None, "launches_per_step": None})
     workload_tag = f"grid{a.nn}"
-    if dom in work and work[dom][0] == "flop":
+    if dom is None:
+        roof = {"bound": None, "achieved": None, "peak": None, "unit": None, "frac": None}
+    elif dom in work and work[dom][0] == "flop":
         achieved = dk["achieved_tflops"]
         roof = {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
                 "frac": achieved / PEAK_FP64_TFLOPS}
@@ -347,6 +357,7 @@ def main():
         "lu_stage": {"achieved_tflops": lu_tf, "frac_fp64_peak": lu_tf / PEAK_FP64_TFLOPS,
                      "flops": lu_flops},
         "kernels": kernels,
+        "kernel_timing": {"steps": kt_steps, "ms_per_step_with_kernel_events": ms_with_kernel_events},
         "mixed_vs_fp64": None,
     }
     if not a.no_mixed:

@@ -78,6 +78,12 @@ long nec_gpu_benchmark_solve(nec_context* ctx, int warmup, int steps, double* ou
  * milliseconds.  n = length of out in doubles.  Returns the number of
  * families, or -1 on error.  nec_gpu_kernel_name(k) names family k. */
 long nec_gpu_kernel_times(nec_context* ctx, double* out, long n);
+
+/* Per-kernel events on (1, the default) or off (0) for the following
+ * nec_gpu_benchmark_solve calls.  Each event pair adds marker packets to the
+ * queue (about 5 % of an N = 3,280 solve), so a wall-clock measurement of
+ * the solve runs with them off and nec_gpu_kernel_times reports nothing new. */
+long nec_gpu_set_kernel_timing(nec_context* ctx, int on);
 const char* nec_gpu_kernel_name(int kind);
 
 /* Impedance matrix of the completed geometry at freq_mhz (FP64 fill only,

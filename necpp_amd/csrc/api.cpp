@@ -340,6 +340,10 @@ long nec_gpu_stage_times(nec_context* ctx, double* out) {
   });
 }
 
+long nec_gpu_set_kernel_timing(nec_context* ctx, int on) {
+  return guarded([&] { ctx->device().set_kernel_timing(on != 0); });
+}
+
 long nec_gpu_kernel_times(nec_context* ctx, double* out, long n) {
   long rc = -1;
   const long ok = guarded([&] {

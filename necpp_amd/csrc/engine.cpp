@@ -521,7 +521,8 @@ void DeviceSolver::benchmark_pipeline(double rkh, const std::vector<zc>& e, cons
   struct ClockGuard {
     explicit ClockGuard(KernelClock* c) { g_kernel_clock = c; }
     ~ClockGuard() { g_kernel_clock = nullptr; }
-  } guard(&kclock_);
+  };
+  ClockGuard guard(kernel_timing_ ? &kclock_ : nullptr);
   kclock_.reset();
   auto one_step = [&](hipEvent_t* stage) {
     if (stage) hip_check(hipEventRecord(stage[0], stream_), "event");

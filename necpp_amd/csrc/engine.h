@@ -111,6 +111,10 @@ class DeviceSolver {
   const StageTimes& times() const { return times_; }
   // per-kernel launch times of the timed steps of the last benchmark_pipeline
   const KernelClock& kernel_clock() const { return kclock_; }
+  // per-launch start/stop events in benchmark_pipeline (default on); their
+  // marker packets cost about 5 % of an N = 3,280 solve, so headline
+  // timings run with them off
+  void set_kernel_timing(bool on) { kernel_timing_ = on; }
   // 0: FP64 LU.  1: mixed (FP32 LU of a copy of Z + FP64 iterative
   // refinement against Z until |b - Z x| <= refine_tol |b|).
   void set_precision(int mode) { precision_ = mode; }
@@ -143,6 +147,7 @@ class DeviceSolver {
   StageTimes times_;
   KernelClock kclock_;
   int precision_ = 0;
+  bool kernel_timing_ = true;
   double refine_tol_ = 1e-12;
   int refine_steps_ = 0;
   double refine_residual_ = 0.0;

@@ -11,7 +11,8 @@ const char* kernel_kind_name(int kind) {
                                         "trsv_lower",    "trsv_upper",    "currents",   "zin",
                                         "pattern_partial", "pattern_finish", "refine", "residual",
                                         "batch_fill", "batch_lu", "batch_solve", "batch_pattern",
-                                        "adjoint_fill", "adjoint_solve", "adjoint_reduce"};
+                                        "adjoint_fill", "adjoint_solve", "adjoint_reduce",
+                                        "lu_next"};
   return (kind >= 0 && kind < K_NKINDS) ? names[kind] : "?";
 }
 

@@ -36,6 +36,7 @@ enum KernelKind {
   K_ADJOINT_FILL,
   K_ADJOINT_SOLVE,
   K_ADJOINT_REDUCE,
+  K_LU_NEXT,
   K_NKINDS
 };
 const char* kernel_kind_name(int kind);

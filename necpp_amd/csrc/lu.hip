@@ -81,6 +81,7 @@ __host__ __device__ inline Xchg xchg_area(char* base, int Gcap, int parity) {
 
 struct LuWork {
   int* status;
+  int* ticket;  // arrival count of lu_next_update_kernel (zero between launches)
   char* xbase;  // two exchange areas
   int Gcap;
 };
@@ -142,10 +143,13 @@ __device__ __forceinline__ double canonical(double v) {
   return (v != v) ? __longlong_as_double(0x7FF8000000000000LL) : v;
 }
 
-__global__ void lu_init_kernel(int* perm, int n, int* status, char* xbase, int Gcap, int G0) {
+__global__ void lu_init_kernel(int* perm, int n, int* status, int* ticket, char* xbase, int Gcap, int G0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) perm[i] = i;
-  if (i == 0) *status = 0;
+  if (i == 0) {
+    *status = 0;
+    *ticket = 0;
+  }
   clear_xchg(xchg_area(xbase, Gcap, 0), Gcap, G0, i, gridDim.x * blockDim.x);
 }
 
@@ -632,15 +636,14 @@ __global__ void __launch_bounds__(256) lu_laswp_kernel(T* __restrict__ A, long l
 // with the 16 x 16 complex products as 4 real MFMAs (4M form).  Operand
 // layout of v_mfma_f64_16x16x4f64: A[i][k] from lane i + 16 k, B[k][j] from
 // lane j + 16 k; result register v of lane l holds D[l / 16 + 4 v][l % 16].
-template <typename T>
-__global__ void __launch_bounds__(256) lu_trsm_kernel(T* __restrict__ A, long long lda, int cend, int k0, int nb,
-                                                      int cbeg) {
-  __shared__ double sL[NB * NB * 2];       // L11[r][q], complex, zero outside the strict lower part
-  __shared__ double sX[NB * 64 * 2];       // X[r][col], complex
-  __shared__ double sDi[4 * 16 * 16 * 2];  // D_b^-1 [b][r][q], complex
+// One workgroup of 256 threads: columns [cb, min(cb + 64, cend)).  sL
+// (NB x NB complex), sX (NB x 64 complex) and sDi (4 x 16 x 16 complex)
+// are LDS; on return sX holds X.  WRITE: X also goes to A as it is formed.
+template <typename T, bool WRITE>
+__device__ __forceinline__ void trsm_block64(T* __restrict__ A, long long lda, int cend, int k0, int nb, int cb,
+                                             double* sL, double* sX, double* sDi) {
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
-  const int cb = cbeg + blockIdx.x * 64;
   {
     // all loads of a thread in flight at once, then the LDS stores
     double vr[NB * NB / 256], vi[NB * NB / 256];
@@ -752,7 +755,7 @@ __global__ void __launch_bounds__(256) lu_trsm_kernel(T* __restrict__ A, long lo
       const int r = 16 * b + lk + 4 * v;
       sX[2 * (r * 64 + xc)] = xr4[v];
       sX[2 * (r * 64 + xc) + 1] = xi4[v];
-      if (on && r < nb) {
+      if (WRITE && on && r < nb) {
         T* p = A + 2 * ((size_t)(k0 + r) * lda + col);
         p[0] = static_cast<T>(xr4[v]);
         p[1] = static_cast<T>(xi4[v]);
@@ -762,6 +765,144 @@ __global__ void __launch_bounds__(256) lu_trsm_kernel(T* __restrict__ A, long lo
     __builtin_amdgcn_wave_barrier();
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) lu_trsm_kernel(T* __restrict__ A, long long lda, int cend, int k0, int nb,
+                                                      int cbeg) {
+  __shared__ double sL[NB * NB * 2];       // L11[r][q], complex, zero outside the strict lower part
+  __shared__ double sX[NB * 64 * 2];       // X[r][col], complex
+  __shared__ double sDi[4 * 16 * 16 * 2];  // D_b^-1 [b][r][q], complex
+  trsm_block64<T, true>(A, lda, cend, k0, nb, cbeg + blockIdx.x * 64, sL, sX, sDi);
+}
+
+// The look-ahead update of the next panel's columns [cb, cend) (at most
+// 64) after panel [k0, k0+nb), in one launch: every workgroup forms
+// U12 = L11^-1 A12 itself (trsm_block64, no global writes), then updates
+// its 64-row tile of rows [k0+nb, n): C -= L21 U12 on FP64 MFMA (4M form,
+// each wave a 32 x 32 block).  A12 is read by every workgroup, so U12 is
+// written over it only by the workgroup that arrives last at the ticket
+// (all others have read A12 by then); the ticket returns to zero.
+template <typename T>
+__global__ void __launch_bounds__(256) lu_next_update_kernel(T* __restrict__ A, long long lda, int n, int k0, int nb,
+                                                             int cb, int cend, int* ticket) {
+  __shared__ double sL[NB * NB * 2];       // L11, then this tile's L21 rows
+  __shared__ double sX[NB * 64 * 2];       // U12[r][col]
+  __shared__ double sDi[4 * 16 * 16 * 2];
+  __shared__ int s_last;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int row0 = k0 + nb + 64 * blockIdx.x;
+  const int cols = cend - cb;
+  // the tile's C block and L21 rows: loads issued before the trsm
+  constexpr int PER = 64 * 64 / 256;
+  double cr[PER], ci[PER], lr[PER], li[PER];
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int e = t + 256 * u;
+    const int r = e / 64, q = e % 64;
+    cr[u] = ci[u] = lr[u] = li[u] = 0.0;
+    if (row0 + r < n) {
+      if (q < cols) {
+        const T* v = A + 2 * ((size_t)(row0 + r) * lda + cb + q);
+        cr[u] = v[0];
+        ci[u] = v[1];
+      }
+      if (q < nb) {
+        const T* v = A + 2 * ((size_t)(row0 + r) * lda + k0 + q);
+        lr[u] = v[0];
+        li[u] = v[1];
+      }
+    }
+  }
+  trsm_block64<T, false>(A, lda, cend, k0, nb, cb, sL, sX, sDi);
+  __syncthreads();  // every read of A12 and L11 has completed
+  if (t == 0) {
+    __threadfence();
+    const int old = atomicAdd(ticket, 1);
+    s_last = (old == (int)gridDim.x - 1);
+    if (s_last) atomicExch(ticket, 0);
+  }
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int e = t + 256 * u;
+    sL[2 * e] = lr[u];
+    sL[2 * e + 1] = li[u];
+  }
+  __syncthreads();
+  if (s_last) {
+    for (int e = t; e < nb * 64; e += 256) {
+      const int r = e / 64, q = e % 64;
+      if (q < cols) {
+        T* v = A + 2 * ((size_t)(k0 + r) * lda + cb + q);
+        v[0] = static_cast<T>(sX[2 * e]);
+        v[1] = static_cast<T>(sX[2 * e + 1]);
+      }
+    }
+  }
+  if (row0 >= n) return;
+  // C (64 x 64) -= L21 (64 x nb) U12 (nb x 64); wave (wr, wc) owns rows
+  // 32 wr.., columns 32 wc..
+  const int wr = wave >> 1, wc = wave & 1;
+  const int lx = lane & 15, lk = lane >> 4;
+  double4_t accr[2][2], acci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      accr[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
+      acci[a][b] = (double4_t){0.0, 0.0, 0.0, 0.0};
+    }
+  for (int kk = 0; kk < NB; kk += 4) {
+    const int q = kk + lk;
+    double xr[2], xi[2], yr[2], yi[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) {
+      const int r = wr * 32 + a * 16 + lx;
+      xr[a] = sL[2 * (r * 64 + q)];
+      xi[a] = sL[2 * (r * 64 + q) + 1];
+    }
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int c = wc * 32 + b * 16 + lx;
+      yr[b] = sX[2 * (q * 64 + c)];
+      yi[b] = sX[2 * (q * 64 + c) + 1];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[a], yr[b], accr[a][b], 0, 0, 0);
+        accr[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[a], -yi[b], accr[a][b], 0, 0, 0);
+        acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xr[a], yi[b], acci[a][b], 0, 0, 0);
+        acci[a][b] = __builtin_amdgcn_mfma_f64_16x16x4f64(xi[a], yr[b], acci[a][b], 0, 0, 0);
+      }
+  }
+  // C from registers (element e = t + 256 u is row e / 64, column e % 64)
+  // through LDS (sL is free again) into the result layout
+  __syncthreads();
+#pragma unroll
+  for (int u = 0; u < PER; ++u) {
+    const int e = t + 256 * u;
+    sL[2 * e] = cr[u];
+    sL[2 * e + 1] = ci[u];
+  }
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        // result register v of lane l holds D[(l / 16) + 4 v][l % 16]
+        const int r = wr * 32 + a * 16 + v * 4 + lk;
+        const int c = wc * 32 + b * 16 + lx;
+        if (row0 + r < n && c < cols) {
+          T* o = A + 2 * ((size_t)(row0 + r) * lda + cb + c);
+          o[0] = static_cast<T>(sL[2 * (r * 64 + c)] - accr[a][b][v]);
+          o[1] = static_cast<T>(sL[2 * (r * 64 + c) + 1] - acci[a][b][v]);
+        }
+      }
 }
 
 // C[m0.., n0..] (M x N) -= L[m0.., k0..k0+K) * U[k0..k0+K, n0..] on FP64
@@ -1012,6 +1153,7 @@ LuWork carve(void* work, int n) {
   char* p = static_cast<char*>(work);
   LuWork w;
   w.status = reinterpret_cast<int*>(p);
+  w.ticket = reinterpret_cast<int*>(p + 64);
   w.xbase = p + 256;
   w.Gcap = gcap_for(n);
   return w;
@@ -1030,6 +1172,13 @@ int gemm_occupancy() {
   return o > 0 ? o : 2;  // 2 measured faster than 1 and 4 at N = 3,280
 }
 
+// Most 64-row tiles for the fused next-panel update (NECGPU_LU_NEXT_TILES
+// overrides, for measurements; 0 turns it off).
+int next_fused_max_tiles() {
+  const char* v = std::getenv("NECGPU_LU_NEXT_TILES");
+  return (v && *v) ? std::atoi(v) : 64;
+}
+
 bool lookahead_enabled() {
   const char* v = std::getenv("NECGPU_LU_LOOKAHEAD");
   return !(v && v[0] == '0');
@@ -1040,7 +1189,7 @@ bool lookahead_enabled() {
 // the trailing update.  Created once, kept for the life of the process.
 struct Lookahead {
   hipStream_t side = nullptr;
-  hipEvent_t next_ready = nullptr, panel_done = nullptr;
+  hipEvent_t start = nullptr, panel_done = nullptr, rest_done = nullptr;
 };
 
 Lookahead* lookahead_for_device() {
@@ -1050,8 +1199,9 @@ Lookahead* lookahead_for_device() {
   Lookahead& l = la[dev];
   if (!l.side) {
     if (hipStreamCreateWithFlags(&l.side, hipStreamNonBlocking) != hipSuccess) return nullptr;
-    if (hipEventCreateWithFlags(&l.next_ready, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&l.panel_done, hipEventDisableTiming) != hipSuccess) {
+    if (hipEventCreateWithFlags(&l.start, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&l.panel_done, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&l.rest_done, hipEventDisableTiming) != hipSuccess) {
       (void)hipStreamDestroy(l.side);
       l.side = nullptr;
       return nullptr;
@@ -1103,7 +1253,7 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
   const LuWork w = carve(work, n);
   const int npanels = (n + NB - 1) / NB;
   launch_k(K_LU_INIT, lu_init_kernel, dim3(std::max((n + 255) / 256, 64)), dim3(256), 0, stream, perm, n, w.status,
-           w.xbase, w.Gcap, G0);
+           w.ticket, w.xbase, w.Gcap, G0);
   const int forced = panel_stride_override();
   Lookahead* la = (npanels > 1 && lookahead_enabled()) ? lookahead_for_device() : nullptr;
   const int occ = gemm_occupancy();
@@ -1140,42 +1290,64 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
       launch_k(K_ZGEMM, zgemm_update_f32_kernel, dim3(grid), dim3(256), 0, s, A, lda, k0 + nb, cbeg, rows, cols, k0,
                nb);
   };
-  // Right-looking LU with a look-ahead of one panel.  Per panel p at k0:
-  //   caller stream: [wait for panel p] row exchanges, U12 and trailing
-  //   update of the next panel's columns only -> next_ready;
-  //   side stream:   [wait next_ready] factor panel p+1 -> panel_done;
-  //   caller stream: row exchanges, U12 and trailing update of the columns
-  //   beyond the next panel, on at most (CUs - panel workgroups) workgroups
-  //   so that the panel's workgroups find free CUs.
-  // The two streams never touch the same columns at the same time: the
-  // side stream only works on the next panel's columns while the caller's
-  // stream updates the columns beyond them.
-  panel(0, stream);
+  // The next panel's columns: one fused launch while its 64-row tiles fit
+  // the CUs the trailing update leaves free (each tile workgroup takes a
+  // CU's LDS), else the trsm and the MFMA update as two launches.
+  const int fused_max = next_fused_max_tiles();
+  auto next_update = [&](int k0, int nb, int cbeg, int cend, hipStream_t s) {
+    const int rows = n - k0 - nb;
+    const int tiles = std::max(1, (rows + 63) / 64);
+    if (tiles <= fused_max)
+      launch_k(K_LU_NEXT, lu_next_update_kernel<T>, dim3(tiles), dim3(256), 0, s, A, lda, n, k0, nb, cbeg, cend,
+               w.ticket);
+    else
+      update(k0, nb, cbeg, cend, INT_MAX, s);
+  };
+  if (!la) {
+    // right-looking LU, one stream
+    panel(0, stream);
+    for (int k0 = 0; k0 < n; k0 += NB) {
+      const int nb = std::min(NB, n - k0);
+      const int next = k0 + nb;
+      laswp(k0, nb, 0, k0, next, n, stream);
+      if (next >= n) break;
+      update(k0, nb, next, n, INT_MAX, stream);
+      panel(next, stream);
+    }
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+  }
+  // Right-looking LU with a look-ahead of one panel, on two streams:
+  //   side stream:   panel p -> [rest of p-1 done] row exchanges of the next
+  //                  panel's columns -> their U12 and trailing update (one
+  //                  launch) -> panel p+1 -> ...
+  //   caller stream: [panel p done] row exchanges, U12 and trailing update
+  //                  of the columns beyond the next panel, on at most
+  //                  occupancy x (CUs - next panel's workgroups) workgroups
+  //                  so that the panel finds free CUs -> rest_done.
+  // The critical path is panels and the next-columns work on one stream;
+  // the two streams never touch the same columns at the same time.
+  if (hipEventRecord(la->start, stream) != hipSuccess) return 1;
+  if (hipStreamWaitEvent(la->side, la->start, 0) != hipSuccess) return 1;
+  panel(0, la->side);
+  if (hipEventRecord(la->panel_done, la->side) != hipSuccess) return 1;
   for (int k0 = 0; k0 < n; k0 += NB) {
     const int nb = std::min(NB, n - k0);
     const int next = k0 + nb;
-    // panel p ran on the side stream (p > 0): wait for it
-    if (la && k0 > 0) {
-      if (hipStreamWaitEvent(stream, la->panel_done, 0) != hipSuccess) return 1;
-    }
+    if (hipStreamWaitEvent(stream, la->panel_done, 0) != hipSuccess) return 1;
     if (next >= n) {
       laswp(k0, nb, 0, k0, n, n, stream);
       break;
     }
-    const int nb2 = std::min(NB, n - next);
-    const int far = next + nb2;
-    laswp(k0, nb, next, far, far, far, stream);
-    update(k0, nb, next, far, INT_MAX, stream);
-    if (la) {
-      if (hipEventRecord(la->next_ready, stream) != hipSuccess) return 1;
-      if (hipStreamWaitEvent(la->side, la->next_ready, 0) != hipSuccess) return 1;
-      panel(next, la->side);
-      if (hipEventRecord(la->panel_done, la->side) != hipSuccess) return 1;
-    }
+    const int far = next + std::min(NB, n - next);
+    if (k0 > 0 && hipStreamWaitEvent(la->side, la->rest_done, 0) != hipSuccess) return 1;
+    laswp(k0, nb, next, far, far, far, la->side);
+    next_update(k0, nb, next, far, la->side);
+    panel(next, la->side);
+    if (hipEventRecord(la->panel_done, la->side) != hipSuccess) return 1;
     laswp(k0, nb, 0, k0, far, n, stream);
     const int Gnext = (n - next + RMAX_T - 1) / RMAX_T;
-    update(k0, nb, far, n, la ? std::max(1, occ * (cu_count - Gnext)) : INT_MAX, stream);
-    if (!la) panel(next, stream);
+    update(k0, nb, far, n, std::max(1, occ * (cu_count - Gnext)), stream);
+    if (hipEventRecord(la->rest_done, stream) != hipSuccess) return 1;
   }
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }

@@ -68,6 +68,7 @@ _SIGS = [
     ("nec_gpu_stage_times", _L, [_P, _P]),
     ("nec_gpu_benchmark_solve", _L, [_P, _I, _I, _P]),
     ("nec_gpu_kernel_times", _L, [_P, _P, _L]),
+    ("nec_gpu_set_kernel_timing", _L, [_P, _I]),
     ("nec_gpu_kernel_name", ctypes.c_char_p, [_I]),
     ("nec_gpu_fill_matrix", _L, [_P, _D, _P, _L]),
     ("nec_gpu_pattern_array", _L, [_P, _I, _I, _P, _L]),
