@@ -137,6 +137,22 @@ typedef struct nec_gpu_batch {
  * Returns 0, or 1 with nec_error_message() set. */
 long nec_gpu_batch_solve(const nec_gpu_batch* batch, double* zin, double* gain, double* stats);
 
+/* Adjoint gradients of a whole batch (the structures and frequencies of
+ * `batch`; its pattern and repeat fields are not used): for every (design,
+ * frequency) solve s, in the order of nec_gpu_batch_solve, the objective of
+ * nec_gpu_adjoint_gradient, J[s], its gradient with respect to every
+ * segment's endpoints and radius (grad: 7 doubles per segment, per metre,
+ * solves one after another) and dJ/d(frequency in MHz) (dfreq[s]).  One
+ * batched forward pass and one batched adjoint pass on the GPU (transposed
+ * solves with the batch's LU factors, fill contraction); the per-solve host
+ * work runs on the host's cores.  info (may be NULL): 3 doubles per solve,
+ * gain (dBi), Z_in (re, im).  stats (may be NULL): 10 doubles, solves,
+ * segments, host preparation ms, fill ms, LU ms, solve ms, host objective ms,
+ * device adjoint ms, host gradient ms, total ms.  N <= 512 per structure.
+ * No counterpart in the reference (new entry point). */
+long nec_gpu_batch_adjoint(const nec_gpu_batch* batch, double theta_deg, double phi_deg, double weight, double z0_re,
+                           double z0_im, double* J, double* grad, double* dfreq, double* info, double* stats);
+
 /* ------------------------------------------------------------------ */
 /* Device-pointer entry points (no host round trip).  Matrices are      */
 /* row-major complex128 (interleaved), leading dimension lda entries.   */

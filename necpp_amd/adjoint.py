@@ -74,6 +74,18 @@ def objective_and_gradient(p, theta_deg=90.0, phi_deg=0.0, weight=0.01, z0=50.0,
     return J, params_gradient(p, g7, freq_mhz, seg_per_element), info
 
 
+def batch_objective_and_gradient(P, theta_deg=90.0, phi_deg=0.0, weight=0.01, z0=50.0,
+                                 freq_mhz=decks.ADJOINT_FREQ_MHZ, seg_per_element=31):
+    """(J [B], dJ/dp [B, 44], stats) of a batch of parameter rows in one call
+    of nec_gpu_batch_adjoint: one batched solve and one batched adjoint on
+    the GPU for all rows."""
+    P = np.atleast_2d(np.asarray(P, dtype=float))
+    ds = [decks.c4_deck(p, freq_mhz, seg_per_element) for p in P]
+    J, grads, _, _, stats = api.gpu_batch_adjoint(ds, [freq_mhz], theta_deg, phi_deg, weight, z0)
+    G = np.array([params_gradient(p, g7, freq_mhz, seg_per_element) for p, g7 in zip(P, grads)])
+    return J[:, 0], G, stats
+
+
 def central_differences(p, rel_step=1e-6, per_component=False, fn=None, **kw):
     """Central finite differences of J (objective(), or fn) with the step
     rel_step * |p| (the Euclidean norm of the parameter row) for every
@@ -95,18 +107,15 @@ def _torch_function():
     import torch
 
     class YagiObjective(torch.autograd.Function):
-        """J for a batch of parameter rows [B, 3 ne - 1] (float64); the
-        backward pass returns the adjoint gradient computed with J."""
+        """J for a batch of parameter rows [B, 3 ne - 1] (float64), all rows
+        in one batched solve and adjoint; the backward pass returns the
+        adjoint gradient computed with J."""
 
         @staticmethod
         def forward(ctx, params, weight=0.01, z0=50.0):
             P = params.detach().to("cpu", torch.float64).numpy()
             squeeze = P.ndim == 1
-            P = np.atleast_2d(P)
-            J = np.zeros(P.shape[0])
-            G = np.zeros(P.shape)
-            for b in range(P.shape[0]):
-                J[b], G[b], _ = objective_and_gradient(P[b], weight=weight, z0=z0)
+            J, G, _ = batch_objective_and_gradient(np.atleast_2d(P), weight=weight, z0=z0)
             Gt = torch.from_numpy(G).to(params.device, params.dtype)
             ctx.save_for_backward(Gt[0] if squeeze else Gt)
             Jt = torch.from_numpy(J).to(params.device, params.dtype)

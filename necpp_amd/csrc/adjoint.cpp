@@ -133,13 +133,10 @@ struct EndGroups {
 
 }  // namespace
 
-void adjoint_gradient(const Geometry& geo, const ScaledSegments& seg, int src, zc volt, double wavelength,
-                      double rkh, const std::vector<double>& abc, DeviceSolver& dev, const AdjointSpec& spec,
-                      AdjointResult* out) {
-  const auto t0 = std::chrono::steady_clock::now();
-  const int n = static_cast<int>(geo.size());
+void adjoint_front(const ScaledSegments& seg, const BasisTables& t, int src, zc volt, double wavelength,
+                   const double* abc, const AdjointSpec& spec, AdjointFront* f) {
+  const int n = static_cast<int>(seg.x.size());
   if (src < 0 || src >= n) throw NecError("adjoint: the source segment is out of range");
-  if ((int)abc.size() != 6 * n) throw NecError("adjoint: no current coefficients of the last solve");
   const double kdb = 10.0 / std::log(10.0);
   const Direction dr = direction(spec.theta_deg, spec.phi_deg);
   const zc const3 = zmk(0.0, -em_impedance() / kFourPi);
@@ -156,59 +153,65 @@ void adjoint_gradient(const Geometry& geo, const ScaledSegments& seg, int src, z
     Cy = zadd(Cy, zmk(oy.re, oy.im));
     Cz = zadd(Cz, zmk(oz.re, oz.im));
   }
-  const zc eth = zmul(zadd(zadd(zmulr(Cx, dr.thx), zmulr(Cy, dr.thy)), zmulr(Cz, dr.thz)), const3);
-  const zc eph = zmul(zadd(zmulr(Cx, dr.phx), zmulr(Cy, dr.phy)), const3);
+  f->eth = zmul(zadd(zadd(zmulr(Cx, dr.thx), zmulr(Cy, dr.thy)), zmulr(Cz, dr.thz)), const3);
+  f->eph = zmul(zadd(zmulr(Cx, dr.phx), zmulr(Cy, dr.phy)), const3);
+  const zc eth = f->eth, eph = f->eph;
   const double S = (eth.re * eth.re + eth.im * eth.im) + (eph.re * eph.re + eph.im * eph.im);
-  const zc usrc = zadd(U(src, 0), U(src, 2));  // current at the segment centre: A + C
-  const zc I = zmulr(usrc, wavelength);
+  f->usrc = zadd(U(src, 0), U(src, 2));  // current at the segment centre: A + C
+  const zc I = zmulr(f->usrc, wavelength);
   const zc zin = zdiv(volt, I);
   const double P = 0.5 * zmul(volt, zconj(I)).re;
   if (!(S > 0.0) || !(P > 0.0)) throw NecError("adjoint: the gain is undefined (no radiated field or input power)");
   const double gcon = wavelength * wavelength * 2.0 * kPi / (em_impedance() * P);
   const double G = 10.0 * std::log10(gcon * S);
   const zc dz = zsub(zin, spec.z0);
-  out->gain_db = G;
-  out->zin = zin;
-  out->J = -G + spec.weight * (dz.re * dz.re + dz.im * dz.im);
+  f->gain_db = G;
+  f->zin = zin;
+  f->J = -G + spec.weight * (dz.re * dz.re + dz.im * dz.im);
 
   // ---- Wirtinger derivatives dJ/dU_j (3 complex per segment)
-  std::vector<zc> dU(3 * (size_t)n);
+  std::vector<zc>& dU = f->dU;
+  dU.assign(3 * (size_t)n, zmk(0.0, 0.0));
   const zc ceth = zconj(eth), ceph = zconj(eph);
-  const double dJdS = -kdb / S;
+  f->dJdS = -kdb / S;
   for (int j = 0; j < n; ++j) {
     const zc ph = zmk(std::cos(farg[j]), std::sin(farg[j]));
     const zc pth = zmul(const3, zmulr(ph, (seg.cab[j] * dr.thx + seg.sab[j] * dr.thy) + seg.salp[j] * dr.thz));
     const zc pph = zmul(const3, zmulr(ph, seg.cab[j] * dr.phx + seg.sab[j] * dr.phy));
-    const zc Q = zmulr(zadd(zmul(ceth, pth), zmul(ceph, pph)), dJdS);
+    const zc Q = zmulr(zadd(zmul(ceth, pth), zmul(ceph, pph)), f->dJdS);
     dU[3 * j + 0] = zmulr(Q, fa[j]);
     dU[3 * j + 1] = zmul(Q, zmk(0.0, -fb[j]));
     dU[3 * j + 2] = zmulr(Q, fc[j]);
   }
   // Z_in = V / I and P = Re(V conj(I)) / 2, I = lambda (A + C)_src
-  const zc dJdI = zadd(zmulr(zmul(zconj(dz), zneg(zdiv(volt, zmul(I, I)))), spec.weight),
-                       zmulr(zconj(volt), 0.25 * kdb / P));
-  dU[3 * src + 0] = zadd(dU[3 * src + 0], zmulr(dJdI, wavelength));
-  dU[3 * src + 2] = zadd(dU[3 * src + 2], zmulr(dJdI, wavelength));
+  f->dJdI = zadd(zmulr(zmul(zconj(dz), zneg(zdiv(volt, zmul(I, I)))), spec.weight),
+                 zmulr(zconj(volt), 0.25 * kdb / P));
+  dU[3 * src + 0] = zadd(dU[3 * src + 0], zmulr(f->dJdI, wavelength));
+  dU[3 * src + 2] = zadd(dU[3 * src + 2], zmulr(f->dJdI, wavelength));
 
   // ---- adjoint right-hand side through the current coefficients
-  BasisTables t;
-  build_basis_tables(geo, seg, &t);
-  std::vector<zc> g(n, zmk(0.0, 0.0));
+  f->g.assign(n, zmk(0.0, 0.0));
   for (int j = 0; j < n; ++j)
     for (int q = t.segptr[j]; q < t.segptr[j + 1]; ++q) {
       const int k = t.segbasis[q];
       const double* c = &t.segabc[3 * (size_t)q];
-      g[k] = zadd(g[k], zadd(zadd(zmulr(dU[3 * j], c[0]), zmulr(dU[3 * j + 1], c[1])), zmulr(dU[3 * j + 2], c[2])));
+      f->g[k] = zadd(f->g[k],
+                     zadd(zadd(zmulr(dU[3 * j], c[0]), zmulr(dU[3 * j + 1], c[1])), zmulr(dU[3 * j + 2], c[2])));
     }
-  const auto t1 = std::chrono::steady_clock::now();
+}
 
-  // ---- device: lam = Z^-T g and the fill contraction
-  std::vector<zc> lam, x;
-  std::vector<double> gi, gj;
-  dev.adjoint(rkh, g, &lam, &x, &gi, &gj);
-  out->device_ms = dev.adjoint_ms();
-  const auto t2 = std::chrono::steady_clock::now();
-
+void adjoint_back(const Geometry& geo, const ScaledSegments& seg, int src, zc volt, double wavelength,
+                  const double* abc, const AdjointSpec& spec, const AdjointFront& f, const zc* lam, const zc* x,
+                  const double* gi, const double* gj, AdjointResult* out) {
+  const int n = static_cast<int>(seg.x.size());
+  const double kdb = 10.0 / std::log(10.0);
+  const Direction dr = direction(spec.theta_deg, spec.phi_deg);
+  const zc const3 = zmk(0.0, -em_impedance() / kFourPi);
+  auto U = [&](int j, int c) { return cplx(abc[(size_t)(2 * c) * n + j], abc[(size_t)(2 * c + 1) * n + j]); };
+  const zc ceth = zconj(f.eth), ceph = zconj(f.eph);
+  out->J = f.J;
+  out->gain_db = f.gain_db;
+  out->zin = f.zin;
   std::vector<double>& D = out->dseg;
   D.assign(8 * (size_t)n, 0.0);
   for (int m = 0; m < n; ++m) {
@@ -239,7 +242,7 @@ void adjoint_gradient(const Geometry& geo, const ScaledSegments& seg, int src, z
       const zc deth = zmul(zadd(zadd(zmulr(dcx, dr.thx), zmulr(dcy, dr.thy)), zmulr(dcz, dr.thz)), const3);
       const zc deph = zmul(zadd(zmulr(dcx, dr.phx), zmulr(dcy, dr.phy)), const3);
       const double dS = 2.0 * (zmul(ceth, deth).re + zmul(ceph, deph).re);
-      D[8 * (size_t)j + slot[q]] += dJdS * dS;
+      D[8 * (size_t)j + slot[q]] += f.dJdS * dS;
     }
   }
 
@@ -271,7 +274,7 @@ void adjoint_gradient(const Geometry& geo, const ScaledSegments& seg, int src, z
       zc H[3];
       for (int c = 0; c < 3; ++c) {
         const double* w = &gj[13 * (size_t)j + 7 + 2 * c];
-        H[c] = zsub(dU[3 * j + c], zmk(w[0], w[1]));
+        H[c] = zsub(f.dU[3 * j + c], zmk(w[0], w[1]));
       }
       // segments read by trio(j): each basis on j, and every segment with an
       // end at either end of that basis' centre segment
@@ -309,8 +312,27 @@ void adjoint_gradient(const Geometry& geo, const ScaledSegments& seg, int src, z
 
   // ---- explicit wavelength dependence (for the frequency derivative):
   // I = lambda (A + C)_src, gcon ~ lambda^2, e_src ~ 1 / lambda
-  out->dlambda = 2.0 * zmul(dJdI, usrc).re - kdb * 2.0 / wavelength +
+  out->dlambda = 2.0 * zmul(f.dJdI, f.usrc).re - kdb * 2.0 / wavelength +
                  2.0 * zmul(lam[src], zdivr(zneg(esrc), wavelength)).re;
+}
+
+void adjoint_gradient(const Geometry& geo, const ScaledSegments& seg, int src, zc volt, double wavelength,
+                      double rkh, const std::vector<double>& abc, DeviceSolver& dev, const AdjointSpec& spec,
+                      AdjointResult* out) {
+  const auto t0 = std::chrono::steady_clock::now();
+  const int n = static_cast<int>(geo.size());
+  if ((int)abc.size() != 6 * n) throw NecError("adjoint: no current coefficients of the last solve");
+  BasisTables t;
+  build_basis_tables(geo, seg, &t);
+  AdjointFront f;
+  adjoint_front(seg, t, src, volt, wavelength, abc.data(), spec, &f);
+  const auto t1 = std::chrono::steady_clock::now();
+  std::vector<zc> lam, x;
+  std::vector<double> gi, gj;
+  dev.adjoint(rkh, f.g, &lam, &x, &gi, &gj);
+  out->device_ms = dev.adjoint_ms();
+  const auto t2 = std::chrono::steady_clock::now();
+  adjoint_back(geo, seg, src, volt, wavelength, abc.data(), spec, f, lam.data(), x.data(), gi.data(), gj.data(), out);
   const auto t3 = std::chrono::steady_clock::now();
   out->host_ms = std::chrono::duration<double, std::milli>((t1 - t0) + (t3 - t2)).count();
 }

@@ -30,6 +30,22 @@ struct AdjointResult {
   double host_ms = 0.0, device_ms = 0.0;
 };
 
+// The objective's value and the adjoint right-hand side g of one solve
+// (current coefficients abc: 6n doubles), and what the back half reuses.
+struct AdjointFront {
+  double J = 0.0, gain_db = 0.0, dJdS = 0.0;
+  zc zin = {0.0, 0.0}, usrc = {0.0, 0.0}, dJdI = {0.0, 0.0}, eth = {0.0, 0.0}, eph = {0.0, 0.0};
+  std::vector<zc> dU;  // dJ/dU_j, 3 per segment
+  std::vector<zc> g;   // adjoint right-hand side, n
+};
+void adjoint_front(const ScaledSegments& seg, const BasisTables& t, int src, zc volt, double wavelength,
+                   const double* abc, const AdjointSpec& spec, AdjointFront* f);
+// The gradient from the device's terms: lam (Z^T lam = g), x (the solution),
+// gi (7 per segment) and gj (13 per segment) of the fill contraction.
+void adjoint_back(const Geometry& geo, const ScaledSegments& seg, int src, zc volt, double wavelength,
+                  const double* abc, const AdjointSpec& spec, const AdjointFront& f, const zc* lam, const zc* x,
+                  const double* gi, const double* gj, AdjointResult* out);
+
 // The last solve of `dev` must be the FP64 solve of this geometry at this
 // wavelength with one voltage source `volt` on segment `src` (0-based) and
 // current coefficients `abc` (6n: A, B, C real and imaginary parts).

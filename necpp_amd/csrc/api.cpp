@@ -304,6 +304,47 @@ long nec_gpu_adjoint_gradient(nec_context* ctx, double theta_deg, double phi_deg
   });
 }
 
+long nec_gpu_batch_adjoint(const nec_gpu_batch* b, double theta_deg, double phi_deg, double weight, double z0_re,
+                           double z0_im, double* J, double* grad, double* dfreq, double* info, double* stats) {
+  return guarded([&] {
+    if (!b) throw NecError("nec_gpu_batch_adjoint: no batch");
+    necgpu::BatchInput in;
+    in.n_designs = b->n_designs;
+    in.wire_ptr = b->wire_ptr;
+    in.wire_tag = b->wire_tag;
+    in.wire_ns = b->wire_ns;
+    in.wire_geom = b->wire_geom;
+    in.src_tag = b->src_tag;
+    in.src_seg = b->src_seg;
+    in.src_volt = b->src_volt;
+    in.n_freq = b->n_freq;
+    in.freq_mhz = b->freq_mhz;
+    in.rkh = b->rkh;
+    in.panel_width = b->panel_width;
+    necgpu::AdjointSpec spec;
+    spec.theta_deg = theta_deg;
+    spec.phi_deg = phi_deg;
+    spec.weight = weight;
+    spec.z0 = necgpu::zmk(z0_re, z0_im);
+    necgpu::BatchAdjointOutput out;
+    necgpu::run_batch_adjoint(in, spec, &out);
+    if (J) std::copy(out.J.begin(), out.J.end(), J);
+    if (grad) std::copy(out.grad.begin(), out.grad.end(), grad);
+    if (dfreq) std::copy(out.dfreq.begin(), out.dfreq.end(), dfreq);
+    if (info)
+      for (int s = 0; s < out.solves; ++s) {
+        info[3 * s] = out.gain_db[s];
+        info[3 * s + 1] = out.zin[2 * s];
+        info[3 * s + 2] = out.zin[2 * s + 1];
+      }
+    if (stats) {
+      const double v[10] = {(double)out.solves, (double)out.segments, out.host_prep_ms, out.fill_ms, out.lu_ms,
+                            out.solve_ms, out.front_ms, out.adjoint_device_ms, out.back_ms, out.total_ms};
+      std::copy(v, v + 10, stats);
+    }
+  });
+}
+
 long nec_gpu_refinement(nec_context* ctx, int* steps, double* rel_residual) {
   return guarded([&] {
     *steps = ctx->device().refine_steps();

@@ -3,6 +3,8 @@
 
 #include <vector>
 
+#include "adjoint.h"
+
 namespace necgpu {
 
 struct BatchInput {
@@ -37,5 +39,19 @@ struct BatchOutput {
 };
 
 void run_batch(const BatchInput& in, BatchOutput* out);
+
+// Adjoint gradients of every (design, frequency) of a batch: one batched
+// forward pass (fill, LU, solve), the objective and adjoint right-hand side
+// of each solve on the host's cores, batched transposed solves and fill
+// contraction on the device, then each solve's gradient on the host.
+struct BatchAdjointOutput {
+  int solves = 0;
+  long long segments = 0;
+  std::vector<double> J, gain_db, zin, dfreq;  // per solve (zin: 2)
+  std::vector<double> grad;                    // 7 per segment, solves in order
+  double host_prep_ms = 0, fill_ms = 0, lu_ms = 0, solve_ms = 0, front_ms = 0, adjoint_device_ms = 0, back_ms = 0,
+         total_ms = 0;
+};
+void run_batch_adjoint(const BatchInput& in, const AdjointSpec& spec, BatchAdjointOutput* out);
 
 }  // namespace necgpu

@@ -7,9 +7,9 @@
 // c_geometry.cpp:3299-3386; netwk_compute_inputs, nec_context.cpp:4937-5009;
 // ffld, nec_context.cpp:6639-6887, and the total gain of
 // nec_radiation_pattern.cpp:282-334).  What differs is the layout: a
-// structure of at most 256 segments is factored by one workgroup, with the
-// panel in LDS and the pivot rows tracked by a permutation instead of being
-// moved.
+// structure of at most 512 segments is factored by one workgroup (BT = 256
+// or 512 threads, one per row), with the panel in LDS and the pivot rows
+// tracked by a permutation instead of being moved.
 #include <climits>
 
 #include "kernels.h"
@@ -19,11 +19,10 @@ namespace necgpu {
 
 namespace {
 
-constexpr int BT = 256;  // threads per workgroup = largest n
-
 // First maximum over the block: thread t offers value v for row t
 // (negative = no candidate); returns the winning row or -1.  Every thread
 // must call it.
+template <int BT>
 __device__ int block_first_max(double v, double* s_val, int* s_row) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   double m = v;
@@ -46,8 +45,8 @@ __device__ int block_first_max(double v, double* s_val, int* s_row) {
 }
 
 // In-place LU with partial pivoting (first maximum of |z|, as the
-// single-structure path) of each solve's n x n matrix, n <= 256.
-template <int P, int CW>
+// single-structure path) of each solve's n x n matrix, n <= BT.
+template <int BT, int P, int CW>
 __global__ void __launch_bounds__(BT) batch_lu_kernel(const BatchSolve* __restrict__ sol, double* __restrict__ Zall,
                                                       int* __restrict__ rpall, int* __restrict__ info) {
   __shared__ double sP[BT * P * 2];  // panel, logical rows k0.. (row r at r * P)
@@ -82,7 +81,7 @@ __global__ void __launch_bounds__(BT) batch_lu_kernel(const BatchSolve* __restri
       const int k = k0 + c;
       double v = -1.0;
       if (t >= c && t < m) v = hypot(sP[2 * (t * P + c)], sP[2 * (t * P + c) + 1]);
-      const int pr = block_first_max(v, s_val, s_row);
+      const int pr = block_first_max<BT>(v, s_val, s_row);
       if (pr >= 0 && pr != c) {
         if (t < P) {
           const double ar = sP[2 * (c * P + t)], ai = sP[2 * (c * P + t) + 1];
@@ -178,6 +177,7 @@ __global__ void __launch_bounds__(BT) batch_lu_kernel(const BatchSolve* __restri
 // Forward / back substitution with the factors and permutation, then the
 // current coefficients (gather over the segment CSR in ascending basis order)
 // and the antenna input: zin[4 s ..] = Z_in (2), input power, 0.
+template <int BT>
 __global__ void __launch_bounds__(BT) batch_solve_kernel(const BatchSolve* __restrict__ sol,
                                                          const double* __restrict__ Zall, const int* __restrict__ rpall,
                                                          const double* __restrict__ seg, const int* __restrict__ segptr,
@@ -270,9 +270,65 @@ __global__ void __launch_bounds__(BT) batch_solve_kernel(const BatchSolve* __res
   }
 }
 
+// Z^T lam = g for each solve, with the factors and permutation of
+// batch_lu_kernel (P Z = L U, row i of P Z = physical row rp[i]):
+// U^T w = g (forward), L^T v = w (backward, unit), lam[rp[i]] = v[i].
+// Every step reads one contiguous factor row.
+template <int BT>
+__global__ void __launch_bounds__(BT) batch_solve_transposed_kernel(const BatchSolve* __restrict__ sol,
+                                                                    const double* __restrict__ Zall,
+                                                                    const int* __restrict__ rpall,
+                                                                    const double* __restrict__ gall,
+                                                                    double* __restrict__ lamall) {
+  __shared__ double sx[2 * BT];
+  __shared__ int s_rp[BT];
+  const BatchSolve d = sol[blockIdx.x];
+  const int n = d.n;
+  const int t = threadIdx.x;
+  const double* Z = Zall + 2 * d.z0;
+  const double* g = gall + 2 * d.row0;
+  if (t < n) {
+    s_rp[t] = rpall[d.row0 + t];
+    sx[2 * t] = g[2 * t];
+    sx[2 * t + 1] = g[2 * t + 1];
+  }
+  for (int q = 0; q < n; ++q) {
+    __syncthreads();
+    if (t == q) {
+      const double* u = Z + 2 * ((long long)s_rp[q] * n + q);
+      const zc w = zdiv(zmk(sx[2 * q], sx[2 * q + 1]), zmk(u[0], u[1]));
+      sx[2 * q] = w.re;
+      sx[2 * q + 1] = w.im;
+    }
+    __syncthreads();
+    if (t > q && t < n) {
+      const double wr = sx[2 * q], wi = sx[2 * q + 1];
+      const double* u = Z + 2 * ((long long)s_rp[q] * n + t);
+      sx[2 * t] -= u[0] * wr - u[1] * wi;
+      sx[2 * t + 1] -= u[0] * wi + u[1] * wr;
+    }
+  }
+  for (int q = n - 1; q >= 0; --q) {
+    __syncthreads();
+    if (t < q) {
+      const double vr = sx[2 * q], vi = sx[2 * q + 1];
+      const double* l = Z + 2 * ((long long)s_rp[q] * n + t);
+      sx[2 * t] -= l[0] * vr - l[1] * vi;
+      sx[2 * t + 1] -= l[0] * vi + l[1] * vr;
+    }
+  }
+  __syncthreads();
+  double* lam = lamall + 2 * d.row0;
+  if (t < n) {
+    lam[2 * s_rp[t]] = sx[2 * t];
+    lam[2 * s_rp[t] + 1] = sx[2 * t + 1];
+  }
+}
+
 // Total power gain (dB) of each solve in n_dir directions: the ffld sum over
 // segments staged in LDS, then db10(gcon (|E_theta|^2 + |E_phi|^2)) with
 // gcon = lambda^2 2 pi / (eta P_in).
+template <int BT>
 __global__ void __launch_bounds__(BT) batch_pattern_kernel(const BatchSolve* __restrict__ sol,
                                                            const double* __restrict__ seg,
                                                            const double* __restrict__ abcall,
@@ -349,31 +405,56 @@ __global__ void __launch_bounds__(BT) batch_pattern_kernel(const BatchSolve* __r
 
 }  // namespace
 
-int launch_batch_lu(const BatchSolve* sol, int S, double* Z, int* rp, int* info, int P, hipStream_t stream) {
+int launch_batch_lu(const BatchSolve* sol, int S, double* Z, int* rp, int* info, int P, int bt, hipStream_t stream) {
   if (S <= 0) return 0;
-  if (P == 32)
-    launch_k(K_BATCH_LU, batch_lu_kernel<32, 32>, dim3(S), dim3(BT), 0u, stream, sol, Z, rp, info);
-  else if (P == 8)
-    launch_k(K_BATCH_LU, batch_lu_kernel<8, 32>, dim3(S), dim3(BT), 0u, stream, sol, Z, rp, info);
-  else
-    launch_k(K_BATCH_LU, batch_lu_kernel<16, 32>, dim3(S), dim3(BT), 0u, stream, sol, Z, rp, info);
+  if (bt == 512) {
+    // the 512-row panel takes P * 16 KB of LDS: P = 8 or 16
+    if (P == 8)
+      launch_k(K_BATCH_LU, batch_lu_kernel<512, 8, 32>, dim3(S), dim3(512), 0u, stream, sol, Z, rp, info);
+    else
+      launch_k(K_BATCH_LU, batch_lu_kernel<512, 16, 32>, dim3(S), dim3(512), 0u, stream, sol, Z, rp, info);
+  } else if (P == 32) {
+    launch_k(K_BATCH_LU, batch_lu_kernel<256, 32, 32>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, info);
+  } else if (P == 8) {
+    launch_k(K_BATCH_LU, batch_lu_kernel<256, 8, 32>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, info);
+  } else {
+    launch_k(K_BATCH_LU, batch_lu_kernel<256, 16, 32>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, info);
+  }
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 int launch_batch_solve(const BatchSolve* sol, int S, const double* Z, const int* rp, const double* seg,
                        const int* segptr, const int* segbasis, const double* segabc, double* x, double* abc,
-                       double* zin, hipStream_t stream) {
+                       double* zin, int bt, hipStream_t stream) {
   if (S <= 0) return 0;
-  launch_k(K_BATCH_SOLVE, batch_solve_kernel, dim3(S), dim3(BT), 0u, stream, sol, Z, rp, seg, segptr, segbasis,
-           segabc, x, abc, zin);
+  if (bt == 512)
+    launch_k(K_BATCH_SOLVE, batch_solve_kernel<512>, dim3(S), dim3(512), 0u, stream, sol, Z, rp, seg, segptr,
+             segbasis, segabc, x, abc, zin);
+  else
+    launch_k(K_BATCH_SOLVE, batch_solve_kernel<256>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, seg, segptr,
+             segbasis, segabc, x, abc, zin);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_batch_solve_transposed(const BatchSolve* sol, int S, const double* Z, const int* rp, const double* g,
+                                  double* lam, int bt, hipStream_t stream) {
+  if (S <= 0) return 0;
+  if (bt == 512)
+    launch_k(K_ADJOINT_SOLVE, batch_solve_transposed_kernel<512>, dim3(S), dim3(512), 0u, stream, sol, Z, rp, g, lam);
+  else
+    launch_k(K_ADJOINT_SOLVE, batch_solve_transposed_kernel<256>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, g, lam);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
 int launch_batch_pattern(const BatchSolve* sol, int S, const double* seg, const double* abc, const double* zin,
-                         int n_dir, const double* dir, double* gain, hipStream_t stream) {
+                         int n_dir, const double* dir, double* gain, int bt, hipStream_t stream) {
   if (S <= 0 || n_dir <= 0) return 0;
-  launch_k(K_BATCH_PATTERN, batch_pattern_kernel, dim3(S), dim3(BT), 0u, stream, sol, seg, abc, zin, n_dir, dir,
-           gain);
+  if (bt == 512)
+    launch_k(K_BATCH_PATTERN, batch_pattern_kernel<512>, dim3(S), dim3(512), 0u, stream, sol, seg, abc, zin, n_dir,
+             dir, gain);
+  else
+    launch_k(K_BATCH_PATTERN, batch_pattern_kernel<256>, dim3(S), dim3(256), 0u, stream, sol, seg, abc, zin, n_dir,
+             dir, gain);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

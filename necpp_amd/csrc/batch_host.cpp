@@ -12,10 +12,14 @@
 #include "batch.h"
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
+#include <mutex>
+#include <thread>
 #include <cmath>
 #include <cstring>
 
+#include "adjoint.h"
 #include "engine.h"
 #include "nec_math.h"
 
@@ -56,22 +60,68 @@ struct EventPair {
   }
 };
 
-}  // namespace
 
-void run_batch(const BatchInput& in, BatchOutput* out) {
+// Host tables of a batch: per (design, frequency) the segment table in
+// wavelengths, the basis CSRs and the BatchSolve offsets, concatenated; the
+// designs' geometries are kept for the adjoint.
+struct BatchPrep {
+  int S = 0, maxn = 0;
+  std::vector<BatchSolve> sol;
+  std::vector<double> seg;
+  std::vector<int> colptr, colj, segptr, segbasis;
+  std::vector<double> colabc, segabc;
+  std::vector<Geometry> geo;  // per design
+  std::vector<double> freq;   // per solve, MHz
+  long long rows = 0, zents = 0;
+};
+
+// body(s) for s = 0 .. count-1 on the host's cores; the first exception is
+// rethrown after every worker has stopped.
+template <class F>
+void parallel_for(int count, F body) {
+  const unsigned hw = std::max(1u, std::thread::hardware_concurrency());
+  const int nthreads = static_cast<int>(std::min<unsigned>(std::min(hw, 16u), (unsigned)std::max(1, count)));
+  std::atomic<int> next(0);
+  std::mutex mu;
+  std::string err;
+  auto work = [&]() {
+    for (;;) {
+      const int s = next.fetch_add(1);
+      if (s >= count) return;
+      try {
+        body(s);
+      } catch (const std::exception& e) {
+        std::lock_guard<std::mutex> lk(mu);
+        if (err.empty()) err = e.what();
+        next.store(count);
+      }
+    }
+  };
+  std::vector<std::thread> pool;
+  for (int k = 1; k < nthreads; ++k) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  if (!err.empty()) throw NecError(err);
+}
+
+void prepare_batch(const BatchInput& in, BatchPrep* p) {
   int count = 0;
   if (hipGetDeviceCount(&count) != hipSuccess || count == 0)
     throw NecError("no HIP device available: the GPU path needs an MI355X (gfx950)");
   if (in.n_designs <= 0 || in.n_freq <= 0) throw NecError("batch: no designs or no frequencies");
-  const auto t_host0 = std::chrono::steady_clock::now();
   const int S = in.n_designs * in.n_freq;
-  std::vector<BatchSolve> sol(S);
-  std::vector<double> seg;
-  std::vector<int> colptr, colj, segptr, segbasis;
-  std::vector<double> colabc, segabc;
-  long long rows = 0, zents = 0;
-  for (int d = 0; d < in.n_designs; ++d) {
-    Geometry geo;
+  p->S = S;
+  p->sol.assign(S, BatchSolve{});
+  p->geo.assign(in.n_designs, Geometry{});
+  p->freq.assign(S, 0.0);
+  // every design's geometry and every solve's tables, on the host's cores
+  struct Part {
+    ScaledSegments sc;
+    BasisTables t;
+  };
+  std::vector<Part> part(S);
+  parallel_for(in.n_designs, [&](int d) {
+    Geometry& geo = p->geo[d];
     for (int w = in.wire_ptr[d]; w < in.wire_ptr[d + 1]; ++w) {
       const double* gm = in.wire_geom + 7 * (size_t)w;
       geo.add_wire(in.wire_tag[w], in.wire_ns[w], gm[0], gm[1], gm[2], gm[3], gm[4], gm[5], gm[6], 1.0, 1.0);
@@ -85,39 +135,94 @@ void run_batch(const BatchInput& in, BatchOutput* out) {
     const int src = static_cast<int>(srcno - 1);
     const zc v = zmk(in.src_volt[2 * d], in.src_volt[2 * d + 1]);
     for (int f = 0; f < in.n_freq; ++f) {
-      const double fmhz = in.freq_mhz[(size_t)d * in.n_freq + f];
-      const ScaledSegments sc = geo.scaled(fmhz);
-      BasisTables t;
-      build_basis_tables(geo, sc, &t);
-      BatchSolve& b = sol[(size_t)d * in.n_freq + f];
+      const size_t s = (size_t)d * in.n_freq + f;
+      const double fmhz = in.freq_mhz[s];
+      Part& q = part[s];
+      q.sc = geo.scaled(fmhz);
+      build_basis_tables(geo, q.sc, &q.t);
+      BatchSolve& b = p->sol[s];
+      p->freq[s] = fmhz;
       b.n = n;
       b.src = src;
-      b.row0 = rows;
-      b.seg0 = (long long)seg.size();
-      b.cptr0 = (long long)colptr.size();
-      b.csr0 = (long long)colj.size();
-      b.sptr0 = (long long)segptr.size();
-      b.ssr0 = (long long)segbasis.size();
-      b.z0 = zents;
       b.wavelength = em_speed_of_light() / (1.0e6 * fmhz);
       // e[src] = -V / (s lambda), s the scaled segment length
-      const zc e = zdivr(zneg(v), sc.s[src] * b.wavelength);
+      const zc e = zdivr(zneg(v), q.sc.s[src] * b.wavelength);
       b.er = e.re;
       b.ei = e.im;
       b.vr = v.re;
       b.vi = v.im;
-      for (const std::vector<double>* a : {&sc.x, &sc.y, &sc.z, &sc.s, &sc.b, &sc.cab, &sc.sab, &sc.salp})
-        seg.insert(seg.end(), a->begin(), a->end());
-      colptr.insert(colptr.end(), t.colptr.begin(), t.colptr.end());
-      colj.insert(colj.end(), t.colj.begin(), t.colj.end());
-      colabc.insert(colabc.end(), t.colabc.begin(), t.colabc.end());
-      segptr.insert(segptr.end(), t.segptr.begin(), t.segptr.end());
-      segbasis.insert(segbasis.end(), t.segbasis.begin(), t.segbasis.end());
-      segabc.insert(segabc.end(), t.segabc.begin(), t.segabc.end());
-      rows += n;
-      zents += (long long)n * n;
     }
+  });
+  // offsets and concatenation, in solve order
+  long long& rows = p->rows;
+  long long& zents = p->zents;
+  for (int s = 0; s < S; ++s) {
+    BatchSolve& b = p->sol[s];
+    const Part& q = part[s];
+    const int n = b.n;
+    p->maxn = std::max(p->maxn, n);
+    b.row0 = rows;
+    b.seg0 = (long long)p->seg.size();
+    b.cptr0 = (long long)p->colptr.size();
+    b.csr0 = (long long)p->colj.size();
+    b.sptr0 = (long long)p->segptr.size();
+    b.ssr0 = (long long)p->segbasis.size();
+    b.z0 = zents;
+    for (const std::vector<double>* a : {&q.sc.x, &q.sc.y, &q.sc.z, &q.sc.s, &q.sc.b, &q.sc.cab, &q.sc.sab, &q.sc.salp})
+      p->seg.insert(p->seg.end(), a->begin(), a->end());
+    p->colptr.insert(p->colptr.end(), q.t.colptr.begin(), q.t.colptr.end());
+    p->colj.insert(p->colj.end(), q.t.colj.begin(), q.t.colj.end());
+    p->colabc.insert(p->colabc.end(), q.t.colabc.begin(), q.t.colabc.end());
+    p->segptr.insert(p->segptr.end(), q.t.segptr.begin(), q.t.segptr.end());
+    p->segbasis.insert(p->segbasis.end(), q.t.segbasis.begin(), q.t.segbasis.end());
+    p->segabc.insert(p->segabc.end(), q.t.segabc.begin(), q.t.segabc.end());
+    rows += n;
+    zents += (long long)n * n;
   }
+}
+
+// Device copies of a batch's tables and the solve buffers.
+struct BatchDev {
+  DevBuf sol, seg, colptr, colj, colabc, segptr, segbasis, segabc;
+  DevBuf Z, rp, info, x, abc, zin;
+  void upload_all(const BatchPrep& p, hipStream_t st) {
+    upload(sol, p.sol, st);
+    upload(seg, p.seg, st);
+    upload(colptr, p.colptr, st);
+    upload(colj, p.colj, st);
+    upload(colabc, p.colabc, st);
+    upload(segptr, p.segptr, st);
+    upload(segbasis, p.segbasis, st);
+    upload(segabc, p.segabc, st);
+    Z.ensure((size_t)p.zents * 16);
+    rp.ensure((size_t)p.rows * sizeof(int));
+    info.ensure((size_t)p.S * sizeof(int));
+    x.ensure((size_t)p.rows * 16);
+    abc.ensure((size_t)p.rows * 48);
+    zin.ensure((size_t)p.S * 32);
+  }
+};
+
+// threads per batched workgroup for structures of up to maxn segments
+int batch_threads(int maxn) { return maxn <= 256 ? 256 : 512; }
+
+void check_info(const std::vector<int>& hinfo) {
+  for (size_t s = 0; s < hinfo.size(); ++s)
+    if (hinfo[s] != 0)
+      throw NecError("batch: solve " + std::to_string(s) + " has a zero pivot at column " +
+                     std::to_string(hinfo[s]));
+}
+
+}  // namespace
+
+void run_batch(const BatchInput& in, BatchOutput* out) {
+  const auto t_host0 = std::chrono::steady_clock::now();
+  BatchPrep prep;
+  prepare_batch(in, &prep);
+  const int S = prep.S;
+  const long long rows = prep.rows;
+  const long long zents = prep.zents;
+  const int bt = batch_threads(prep.maxn);
   // pattern directions (degrees -> sin / cos, as the RP card's tables)
   std::vector<double> dir(4 * (size_t)std::max(0, in.n_dir));
   for (int q = 0; q < in.n_dir; ++q) {
@@ -132,26 +237,26 @@ void run_batch(const BatchInput& in, BatchOutput* out) {
   StreamGuard sg;
   hip_check(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking), "stream");
   const hipStream_t st = sg.s;
-  DevBuf d_sol, d_seg, d_colptr, d_colj, d_colabc, d_segptr, d_segbasis, d_segabc, d_dir;
-  DevBuf d_Z, d_Zbak, d_rp, d_info, d_x, d_abc, d_zin, d_gain;
-  upload(d_sol, sol, st);
-  upload(d_seg, seg, st);
-  upload(d_colptr, colptr, st);
-  upload(d_colj, colj, st);
-  upload(d_colabc, colabc, st);
-  upload(d_segptr, segptr, st);
-  upload(d_segbasis, segbasis, st);
-  upload(d_segabc, segabc, st);
+  BatchDev dv;
+  dv.upload_all(prep, st);
+  DevBuf d_dir, d_Zbak, d_gain;
   upload(d_dir, dir, st);
-  double* Z = static_cast<double*>(d_Z.ensure((size_t)zents * 16));
-  int* rp = static_cast<int*>(d_rp.ensure((size_t)rows * sizeof(int)));
-  int* info = static_cast<int*>(d_info.ensure((size_t)S * sizeof(int)));
-  double* x = static_cast<double*>(d_x.ensure((size_t)rows * 16));
-  double* abc = static_cast<double*>(d_abc.ensure((size_t)rows * 48));
-  double* zin = static_cast<double*>(d_zin.ensure((size_t)S * 32));
+  double* Z = dv.Z.as<double>();
+  int* rp = dv.rp.as<int>();
+  int* info = dv.info.as<int>();
+  double* x = dv.x.as<double>();
+  double* abc = dv.abc.as<double>();
+  double* zin = dv.zin.as<double>();
   double* gain = static_cast<double*>(d_gain.ensure(std::max<size_t>(8, (size_t)S * std::max(0, in.n_dir) * 8)));
-  const BatchSolve* dsol = d_sol.as<BatchSolve>();
+  const BatchSolve* dsol = dv.sol.as<BatchSolve>();
   const double rkh = in.rkh > 0.0 ? in.rkh : 1.0;
+  DevBuf& d_seg = dv.seg;
+  DevBuf& d_colptr = dv.colptr;
+  DevBuf& d_colj = dv.colj;
+  DevBuf& d_colabc = dv.colabc;
+  DevBuf& d_segptr = dv.segptr;
+  DevBuf& d_segbasis = dv.segbasis;
+  DevBuf& d_segabc = dv.segabc;
 
   auto fill = [&]() {
     if (launch_batch_fill(dsol, S, rows, d_seg.as<double>(), d_colptr.as<int>(), d_colj.as<int>(),
@@ -159,14 +264,14 @@ void run_batch(const BatchInput& in, BatchOutput* out) {
       hip_check(hipGetLastError(), "batch fill");
   };
   auto lu = [&](int P) {
-    if (launch_batch_lu(dsol, S, Z, rp, info, P, st)) hip_check(hipGetLastError(), "batch lu");
+    if (launch_batch_lu(dsol, S, Z, rp, info, P, bt, st)) hip_check(hipGetLastError(), "batch lu");
   };
   auto rest = [&]() {
     if (launch_batch_solve(dsol, S, Z, rp, d_seg.as<double>(), d_segptr.as<int>(), d_segbasis.as<int>(),
-                           d_segabc.as<double>(), x, abc, zin, st))
+                           d_segabc.as<double>(), x, abc, zin, bt, st))
       hip_check(hipGetLastError(), "batch solve");
     if (in.n_dir > 0 &&
-        launch_batch_pattern(dsol, S, d_seg.as<double>(), abc, zin, in.n_dir, d_dir.as<double>(), gain, st))
+        launch_batch_pattern(dsol, S, d_seg.as<double>(), abc, zin, in.n_dir, d_dir.as<double>(), gain, bt, st))
       hip_check(hipGetLastError(), "batch pattern");
   };
 
@@ -176,11 +281,13 @@ void run_batch(const BatchInput& in, BatchOutput* out) {
   fill();
   hip_check(hipEventRecord(e1.b, st), "event");
   int bestP = in.panel_width;
+  if (bt == 512 && bestP == 32) bestP = 16;
   if (bestP != 8 && bestP != 16 && bestP != 32) {
     void* bak = d_Zbak.ensure((size_t)zents * 16);
     hip_check(hipMemcpyAsync(bak, Z, (size_t)zents * 16, hipMemcpyDeviceToDevice, st), "copy");
     float best = 1e30f;
     for (int P : {8, 16, 32}) {
+      if (bt == 512 && P == 32) continue;  // the 512-row panel fits LDS at P <= 16
       hip_check(hipMemcpyAsync(Z, bak, (size_t)zents * 16, hipMemcpyDeviceToDevice, st), "copy");
       hip_check(hipEventRecord(e2.a, st), "event");
       lu(P);
@@ -248,10 +355,8 @@ void run_batch(const BatchInput& in, BatchOutput* out) {
   hip_check(hipStreamSynchronize(st), "batch results");
   out->zin.resize(2 * (size_t)S);
   out->pin.resize(S);
+  check_info(hinfo);
   for (int s = 0; s < S; ++s) {
-    if (hinfo[s] != 0)
-      throw NecError("batch: solve " + std::to_string(s) + " has a zero pivot at column " +
-                     std::to_string(hinfo[s]));
     out->zin[2 * s] = hzin[4 * s];
     out->zin[2 * s + 1] = hzin[4 * s + 1];
     out->pin[s] = hzin[4 * s + 2];
@@ -262,6 +367,124 @@ void run_batch(const BatchInput& in, BatchOutput* out) {
          (in.n_dir <= 0 || std::memcmp(eager_gain.data(), out->gain.data(), out->gain.size() * 8) == 0))
             ? 1
             : 0;
+  out->solves = S;
+  out->segments = rows;
+  (void)zents;
+}
+
+void run_batch_adjoint(const BatchInput& in, const AdjointSpec& spec, BatchAdjointOutput* out) {
+  using clk = std::chrono::steady_clock;
+  const auto t0 = clk::now();
+  BatchPrep prep;
+  prepare_batch(in, &prep);
+  const int S = prep.S;
+  const long long rows = prep.rows;
+  const int bt = batch_threads(prep.maxn);
+  const double rkh = in.rkh > 0.0 ? in.rkh : 1.0;
+  int P = in.panel_width;
+  if (P != 8 && P != 16 && P != 32) P = 16;
+  if (bt == 512 && P == 32) P = 16;
+  out->host_prep_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
+
+  StreamGuard sg;
+  hip_check(hipStreamCreateWithFlags(&sg.s, hipStreamNonBlocking), "stream");
+  const hipStream_t st = sg.s;
+  BatchDev dv;
+  dv.upload_all(prep, st);
+  const BatchSolve* dsol = dv.sol.as<BatchSolve>();
+  EventPair e1, e2, e3, e4;
+  hip_check(hipEventRecord(e1.a, st), "event");
+  if (launch_batch_fill(dsol, S, rows, dv.seg.as<double>(), dv.colptr.as<int>(), dv.colj.as<int>(),
+                        dv.colabc.as<double>(), dv.Z.as<double>(), rkh, st))
+    hip_check(hipGetLastError(), "batch fill");
+  hip_check(hipEventRecord(e1.b, st), "event");
+  hip_check(hipEventRecord(e2.a, st), "event");
+  if (launch_batch_lu(dsol, S, dv.Z.as<double>(), dv.rp.as<int>(), dv.info.as<int>(), P, bt, st))
+    hip_check(hipGetLastError(), "batch lu");
+  hip_check(hipEventRecord(e2.b, st), "event");
+  hip_check(hipEventRecord(e3.a, st), "event");
+  if (launch_batch_solve(dsol, S, dv.Z.as<double>(), dv.rp.as<int>(), dv.seg.as<double>(), dv.segptr.as<int>(),
+                         dv.segbasis.as<int>(), dv.segabc.as<double>(), dv.x.as<double>(), dv.abc.as<double>(),
+                         dv.zin.as<double>(), bt, st))
+    hip_check(hipGetLastError(), "batch solve");
+  hip_check(hipEventRecord(e3.b, st), "event");
+  std::vector<int> hinfo(S);
+  std::vector<double> habc(6 * (size_t)rows);
+  std::vector<zc> hx(rows);
+  hip_check(hipMemcpyAsync(hinfo.data(), dv.info.as<void>(), (size_t)S * sizeof(int), hipMemcpyDeviceToHost, st),
+            "info");
+  hip_check(hipMemcpyAsync(habc.data(), dv.abc.as<void>(), habc.size() * 8, hipMemcpyDeviceToHost, st), "abc");
+  hip_check(hipMemcpyAsync(hx.data(), dv.x.as<void>(), (size_t)rows * 16, hipMemcpyDeviceToHost, st), "x");
+  hip_check(hipStreamSynchronize(st), "batch forward");
+  check_info(hinfo);
+  out->fill_ms = e1.ms();
+  out->lu_ms = e2.ms();
+  out->solve_ms = e3.ms();
+
+  // host: objective and adjoint right-hand side of every solve
+  const auto t1 = clk::now();
+  std::vector<AdjointFront> front(S);
+  std::vector<zc> g(rows);
+  auto solve_design = [&](int s) { return s / in.n_freq; };
+  parallel_for(S, [&](int s) {
+    const BatchSolve& b = prep.sol[s];
+    const Geometry& geo = prep.geo[solve_design(s)];
+    const ScaledSegments sc = geo.scaled(prep.freq[s]);
+    BasisTables t;
+    build_basis_tables(geo, sc, &t);
+    adjoint_front(sc, t, b.src, zmk(b.vr, b.vi), b.wavelength, habc.data() + 6 * b.row0, spec, &front[s]);
+    std::copy(front[s].g.begin(), front[s].g.end(), g.begin() + b.row0);
+  });
+  out->front_ms = std::chrono::duration<double, std::milli>(clk::now() - t1).count();
+
+  // device: transposed solves and the fill contraction
+  DevBuf d_g, d_lam, d_gi, d_gj;
+  upload(d_g, g, st);
+  d_lam.ensure((size_t)rows * 16);
+  d_gi.ensure((size_t)rows * 7 * 8);
+  d_gj.ensure((size_t)rows * 13 * 8);
+  hip_check(hipEventRecord(e4.a, st), "event");
+  if (launch_batch_solve_transposed(dsol, S, dv.Z.as<double>(), dv.rp.as<int>(), d_g.as<double>(), d_lam.as<double>(),
+                                    bt, st))
+    hip_check(hipGetLastError(), "batch transposed solve");
+  if (launch_batch_adjoint_fill(dsol, S, rows, dv.seg.as<double>(), d_lam.as<double>(), dv.abc.as<double>(), rkh,
+                                d_gi.as<double>(), d_gj.as<double>(), st))
+    hip_check(hipGetLastError(), "batch adjoint fill");
+  hip_check(hipEventRecord(e4.b, st), "event");
+  std::vector<zc> lam(rows);
+  std::vector<double> gi(7 * (size_t)rows), gj(13 * (size_t)rows);
+  hip_check(hipMemcpyAsync(lam.data(), d_lam.as<void>(), (size_t)rows * 16, hipMemcpyDeviceToHost, st), "lam");
+  hip_check(hipMemcpyAsync(gi.data(), d_gi.as<void>(), gi.size() * 8, hipMemcpyDeviceToHost, st), "gi");
+  hip_check(hipMemcpyAsync(gj.data(), d_gj.as<void>(), gj.size() * 8, hipMemcpyDeviceToHost, st), "gj");
+  hip_check(hipStreamSynchronize(st), "batch adjoint");
+  out->adjoint_device_ms = e4.ms();
+
+  // host: gradients
+  const auto t2 = clk::now();
+  out->J.assign(S, 0.0);
+  out->gain_db.assign(S, 0.0);
+  out->zin.assign(2 * (size_t)S, 0.0);
+  out->dfreq.assign(S, 0.0);
+  out->grad.assign(7 * (size_t)rows, 0.0);
+  parallel_for(S, [&](int s) {
+    const BatchSolve& b = prep.sol[s];
+    const Geometry& geo = prep.geo[solve_design(s)];
+    const ScaledSegments sc = geo.scaled(prep.freq[s]);
+    AdjointResult r;
+    adjoint_back(geo, sc, b.src, zmk(b.vr, b.vi), b.wavelength, habc.data() + 6 * b.row0, spec, front[s],
+                 lam.data() + b.row0, hx.data() + b.row0, gi.data() + 7 * b.row0, gj.data() + 13 * b.row0, &r);
+    std::vector<double> g7;
+    double df = 0.0;
+    adjoint_endpoints(sc, b.wavelength, prep.freq[s], r, &g7, &df);
+    std::copy(g7.begin(), g7.end(), out->grad.begin() + 7 * b.row0);
+    out->J[s] = r.J;
+    out->gain_db[s] = r.gain_db;
+    out->zin[2 * s] = r.zin.re;
+    out->zin[2 * s + 1] = r.zin.im;
+    out->dfreq[s] = df;
+  });
+  out->back_ms = std::chrono::duration<double, std::milli>(clk::now() - t2).count();
+  out->total_ms = std::chrono::duration<double, std::milli>(clk::now() - t0).count();
   out->solves = S;
   out->segments = rows;
 }

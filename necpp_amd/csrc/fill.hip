@@ -602,88 +602,165 @@ constexpr int ADJ_NV = 8;  // dx, dy, dz (c_i - c_j), b_i, s_j, cab_j, sab_j, sa
 constexpr int ADJ_PJ = 13;
 constexpr int ADJ_GI = 7;
 constexpr int ADJ_THREADS = 256;
-__global__ void __launch_bounds__(ADJ_THREADS) adjoint_fill_kernel(DevSegments g, int n, int i0, double rkh,
-                                                                   const double* __restrict__ lam,
-                                                                   const double* __restrict__ abc,
-                                                                   double* __restrict__ gi, double* __restrict__ pj) {
+
+// One pair's adjoint terms: oi (7, observation side: d/dc_i x y z, d/db_i,
+// d/d(cab, sab, salp)_i) and oj (13, source side: d/dc_j x y z, d/ds_j,
+// d/d(cab, sab, salp)_j, lam_i E_ij as 3 complex).  abc: 6 n doubles of the
+// solve (A, B, C real and imaginary planes).
+__device__ __forceinline__ void adjoint_pair(const DevSegments& g, int n, int i, int j, double rkh, double lr,
+                                             double li, const double* __restrict__ abc, double* oi, double* oj) {
   typedef Dn<ADJ_NV> D;
-  __shared__ double sred[ADJ_GI][ADJ_THREADS / 64];
-  const int r = blockIdx.x;
-  const int i = i0 + r;
-  const double lr = lam[2 * i], li = lam[2 * i + 1];
-  const double cabi = g.cab[i], sabi = g.sab[i], salpi = g.salp[i];
-  double acc[ADJ_GI] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
-    const bool self = (i == j);
-    // positions enter as c_i = c_j + (c_i - c_j): seed the difference; a
-    // segment's own pair has no difference to vary
-    D dx(g.x[i] - g.x[j]), dy(g.y[i] - g.y[j]), dz(g.z[i] - g.z[j]);
-    D ai(g.b[i]), sj(g.s[j]), bj(g.b[j]), cj(g.cab[j]), sbj(g.sab[j]), slj(g.salp[j]);
-    if (!self) {
-      dx.d[0] = 1.0;
-      dy.d[1] = 1.0;
-      dz.d[2] = 1.0;
-    }
-    ai.d[3] = 1.0;
-    sj.d[4] = 1.0;
-    cj.d[5] = 1.0;
-    sbj.d[6] = 1.0;
-    slj.d[7] = 1.0;
-    const D zero(0.0);
-    cx<D> etk, ets, etc;
-    cx<double> ev[9];
-    pair_fields_v<D>(dx, dy, dz, ai, cabi, sabi, salpi, zero, zero, zero, sj, bj, cj, sbj, slj, self, rkh, 0, 2, 2,
-                     &etk, &ets, &etc, ev);
-    const double Ar = abc[j], Ai = abc[n + j], Br = abc[2 * n + j], Bi = abc[3 * n + j], Cr = abc[4 * n + j],
-                 Ci = abc[5 * n + j];
-    // -2 Re(lam * (dk A + ds B + dc C)) for complex derivative triples
-    auto contract = [&](double kr, double ki, double sr, double si, double cr, double ci) {
-      const double fr = ((kr * Ar - ki * Ai) + (sr * Br - si * Bi)) + (cr * Cr - ci * Ci);
-      const double fi = ((kr * Ai + ki * Ar) + (sr * Bi + si * Br)) + (cr * Ci + ci * Cr);
-      return -2.0 * (lr * fr - li * fi);
-    };
-    double cv[ADJ_NV];
-    for (int v = 0; v < ADJ_NV; ++v)
-      cv[v] = contract(etk.re.d[v], etk.im.d[v], ets.re.d[v], ets.im.d[v], etc.re.d[v], etc.im.d[v]);
-    // observation direction: etk = e0 cab + e1 sab + e2 salp (e3..e5: ets,
-    // e6..e8: etc)
-    double cdi[3];
-    for (int q = 0; q < 3; ++q)
-      cdi[q] = contract(ev[q].re, ev[q].im, ev[3 + q].re, ev[3 + q].im, ev[6 + q].re, ev[6 + q].im);
-    acc[0] += cv[0];
-    acc[1] += cv[1];
-    acc[2] += cv[2];
-    acc[3] += cv[3];
-    acc[4] += cdi[0];
-    acc[5] += cdi[1];
-    acc[6] += cdi[2];
-    double* o = pj + ((size_t)r * n + j) * ADJ_PJ;
-    o[0] = -cv[0];
-    o[1] = -cv[1];
-    o[2] = -cv[2];
-    o[3] = cv[4];
-    o[4] = cv[5];
-    o[5] = cv[6];
-    o[6] = cv[7];
-    o[7] = lr * etk.re.v - li * etk.im.v;
-    o[8] = lr * etk.im.v + li * etk.re.v;
-    o[9] = lr * ets.re.v - li * ets.im.v;
-    o[10] = lr * ets.im.v + li * ets.re.v;
-    o[11] = lr * etc.re.v - li * etc.im.v;
-    o[12] = lr * etc.im.v + li * etc.re.v;
+  const bool self = (i == j);
+  // positions enter as c_i = c_j + (c_i - c_j): seed the difference; a
+  // segment's own pair has no difference to vary
+  D dx(g.x[i] - g.x[j]), dy(g.y[i] - g.y[j]), dz(g.z[i] - g.z[j]);
+  D ai(g.b[i]), sj(g.s[j]), bj(g.b[j]), cj(g.cab[j]), sbj(g.sab[j]), slj(g.salp[j]);
+  if (!self) {
+    dx.d[0] = 1.0;
+    dy.d[1] = 1.0;
+    dz.d[2] = 1.0;
   }
+  ai.d[3] = 1.0;
+  sj.d[4] = 1.0;
+  cj.d[5] = 1.0;
+  sbj.d[6] = 1.0;
+  slj.d[7] = 1.0;
+  const D zero(0.0);
+  cx<D> etk, ets, etc;
+  cx<double> ev[9];
+  pair_fields_v<D>(dx, dy, dz, ai, g.cab[i], g.sab[i], g.salp[i], zero, zero, zero, sj, bj, cj, sbj, slj, self, rkh, 0,
+                   2, 2, &etk, &ets, &etc, ev);
+  const double Ar = abc[j], Ai = abc[n + j], Br = abc[2 * n + j], Bi = abc[3 * n + j], Cr = abc[4 * n + j],
+               Ci = abc[5 * n + j];
+  // -2 Re(lam * (dk A + ds B + dc C)) for complex derivative triples
+  auto contract = [&](double kr, double ki, double sr, double si, double cr, double ci) {
+    const double fr = ((kr * Ar - ki * Ai) + (sr * Br - si * Bi)) + (cr * Cr - ci * Ci);
+    const double fi = ((kr * Ai + ki * Ar) + (sr * Bi + si * Br)) + (cr * Ci + ci * Cr);
+    return -2.0 * (lr * fr - li * fi);
+  };
+  double cv[ADJ_NV];
+  for (int v = 0; v < ADJ_NV; ++v)
+    cv[v] = contract(etk.re.d[v], etk.im.d[v], ets.re.d[v], ets.im.d[v], etc.re.d[v], etc.im.d[v]);
+  oi[0] = cv[0];
+  oi[1] = cv[1];
+  oi[2] = cv[2];
+  oi[3] = cv[3];
+  // observation direction: etk = e0 cab + e1 sab + e2 salp (e3..e5: ets,
+  // e6..e8: etc)
+  for (int q = 0; q < 3; ++q)
+    oi[4 + q] = contract(ev[q].re, ev[q].im, ev[3 + q].re, ev[3 + q].im, ev[6 + q].re, ev[6 + q].im);
+  oj[0] = -cv[0];
+  oj[1] = -cv[1];
+  oj[2] = -cv[2];
+  oj[3] = cv[4];
+  oj[4] = cv[5];
+  oj[5] = cv[6];
+  oj[6] = cv[7];
+  oj[7] = lr * etk.re.v - li * etk.im.v;
+  oj[8] = lr * etk.im.v + li * etk.re.v;
+  oj[9] = lr * ets.re.v - li * ets.im.v;
+  oj[10] = lr * ets.im.v + li * ets.re.v;
+  oj[11] = lr * etc.re.v - li * etc.im.v;
+  oj[12] = lr * etc.im.v + li * etc.re.v;
+}
+
+// Sum of K per-thread values over the workgroup (ADJ_THREADS threads);
+// thread c < K writes total c to out[c].
+template <int K>
+__device__ __forceinline__ void block_sum_out(const double* acc, double (*sred)[ADJ_THREADS / 64], double* out) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
-  for (int v = 0; v < ADJ_GI; ++v) {
+  for (int v = 0; v < K; ++v) {
     double x = acc[v];
     for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off, 64);
     if (lane == 0) sred[v][wave] = x;
   }
   __syncthreads();
-  if (threadIdx.x < ADJ_GI) {
+  if (threadIdx.x < K) {
     double x = 0.0;
     for (int w = 0; w < ADJ_THREADS / 64; ++w) x += sred[threadIdx.x][w];
-    gi[(size_t)i * ADJ_GI + threadIdx.x] = x;
+    out[threadIdx.x] = x;
   }
+}
+
+// Adjoint contraction of the fill for one structure.  With lam the adjoint
+// vector (Z^T lam = dJ/dx) and U_j = (A_j, B_j, C_j) the current
+// coefficients of segment j (abc), pair (i, j) adds
+//   -2 Re( lam_i * d(E_ij . U_j) / dv )
+// to the gradient of each geometry variable v, where E_ij = (etk, ets, etc)
+// are the fields of pair_fields_v.  The fields are differentiated on the fly
+// (Dn, no dZ/dv is stored): with respect to the centre difference c_i - c_j
+// (the fields depend on the centres through it only), the observation
+// radius, the source length and the source direction; the observation
+// direction enters through the projection of the unprojected components.
+// One workgroup per observation row i (rows i0 .. i0+rows-1), threads over
+// source segments j.  Row-side sums go to gi (7 per row); source-side terms
+// of each pair go to pj (13 per pair) and are summed over rows by
+// adjoint_reduce in row order, so the result does not depend on scheduling.
+__global__ void __launch_bounds__(ADJ_THREADS) adjoint_fill_kernel(DevSegments g, int n, int i0, double rkh,
+                                                                   const double* __restrict__ lam,
+                                                                   const double* __restrict__ abc,
+                                                                   double* __restrict__ gi, double* __restrict__ pj) {
+  __shared__ double sred[ADJ_PJ][ADJ_THREADS / 64];
+  const int r = blockIdx.x;
+  const int i = i0 + r;
+  const double lr = lam[2 * i], li = lam[2 * i + 1];
+  double acc[ADJ_GI] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    double oi[ADJ_GI];
+    double* o = pj + ((size_t)r * n + j) * ADJ_PJ;
+    double oj[ADJ_PJ];
+    adjoint_pair(g, n, i, j, rkh, lr, li, abc, oi, oj);
+    for (int v = 0; v < ADJ_GI; ++v) acc[v] += oi[v];
+    for (int v = 0; v < ADJ_PJ; ++v) o[v] = oj[v];
+  }
+  block_sum_out<ADJ_GI>(acc, sred, gi + (size_t)i * ADJ_GI);
+}
+
+// Batched contraction: one workgroup per (solve, segment a) over all rows of
+// the concatenated solves (the solve found by row0, as batch_fill_kernel).
+// ROWSIDE: a = observation row i, threads over sources j, gi of row i;
+// otherwise a = source j, threads over observation rows i, gj of segment j.
+// Every pair is evaluated once in each pass; the sums run over threads in a
+// fixed order, so the result does not depend on scheduling.
+template <bool ROWSIDE>
+__global__ void __launch_bounds__(ADJ_THREADS) batch_adjoint_kernel(const BatchSolve* __restrict__ sol, int S,
+                                                                    const double* __restrict__ seg,
+                                                                    const double* __restrict__ lamall,
+                                                                    const double* __restrict__ abcall, double rkh,
+                                                                    double* __restrict__ out) {
+  __shared__ double sred[ADJ_PJ][ADJ_THREADS / 64];
+  const long long blk = blockIdx.x;
+  int lo = 0, hi = S - 1;
+  while (lo < hi) {
+    const int mid = (lo + hi + 1) >> 1;
+    if (sol[mid].row0 <= blk) lo = mid;
+    else hi = mid - 1;
+  }
+  const BatchSolve d = sol[lo];
+  const int n = d.n;
+  const int a = static_cast<int>(blk - d.row0);
+  const double* p = seg + d.seg0;
+  DevSegments g;
+  g.x = p;
+  g.y = p + n;
+  g.z = p + 2 * n;
+  g.s = p + 3 * n;
+  g.b = p + 4 * n;
+  g.cab = p + 5 * n;
+  g.sab = p + 6 * n;
+  g.salp = p + 7 * n;
+  const double* lam = lamall + 2 * d.row0;
+  const double* abc = abcall + 6 * d.row0;
+  constexpr int K = ROWSIDE ? ADJ_GI : ADJ_PJ;
+  double acc[K];
+  for (int v = 0; v < K; ++v) acc[v] = 0.0;
+  for (int b = threadIdx.x; b < n; b += blockDim.x) {
+    const int i = ROWSIDE ? a : b, j = ROWSIDE ? b : a;
+    double oi[ADJ_GI], oj[ADJ_PJ];
+    adjoint_pair(g, n, i, j, rkh, lam[2 * i], lam[2 * i + 1], abc, oi, oj);
+    for (int v = 0; v < K; ++v) acc[v] += ROWSIDE ? oi[v] : oj[v];
+  }
+  block_sum_out<K>(acc, sred, out + (size_t)(d.row0 + a) * K);
 }
 
 // gj[j][c] += sum over the rows of a chunk, in row order.
@@ -713,6 +790,17 @@ int launch_adjoint_fill(const DevSegments& g, int n, int i0, int rows, double rk
            gi, pj);
   launch_k(K_ADJOINT_REDUCE, adjoint_reduce_kernel, dim3((n * ADJ_PJ + 255) / 256), dim3(256), 0u, stream, pj, n,
            rows, gj);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+int launch_batch_adjoint_fill(const BatchSolve* sol, int S, long long rows_total, const double* seg,
+                              const double* lam, const double* abc, double rkh, double* gi, double* gj,
+                              hipStream_t stream) {
+  if (S <= 0 || rows_total <= 0) return 0;
+  launch_k(K_ADJOINT_FILL, batch_adjoint_kernel<true>, dim3((unsigned)rows_total), dim3(ADJ_THREADS), 0u, stream, sol,
+           S, seg, lam, abc, rkh, gi);
+  launch_k(K_ADJOINT_FILL, batch_adjoint_kernel<false>, dim3((unsigned)rows_total), dim3(ADJ_THREADS), 0u, stream,
+           sol, S, seg, lam, abc, rkh, gj);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

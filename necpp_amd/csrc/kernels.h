@@ -98,23 +98,33 @@ struct BatchSolve {
   double er, ei;    // right-hand side at the source: -V / (s lambda)
   double vr, vi;    // source voltage
 };
-constexpr int kBatchMaxN = 256;  // largest structure of the batched path
+constexpr int kBatchMaxN = 512;  // largest structure of the batched path
 
 // fill.hip: one workgroup per (solve, row); rows_total = sum of n.
 int launch_batch_fill(const BatchSolve* sol, int S, long long rows_total, const double* seg, const int* colptr,
                       const int* colj, const double* colabc, double* Z, double rkh, hipStream_t stream);
 
-// batch.hip: one workgroup per solve.  LU with row permutation rp (logical
-// row i of the factors is physical row rp[i]; no rows are moved), panel
-// width P (16 or 32); then the solve, current coefficients abc (6 per
+// batch.hip: one workgroup per solve, bt = 256 or 512 threads (bt >= the
+// largest n).  LU with row permutation rp (logical row i of the factors is
+// physical row rp[i]; no rows are moved), panel width P (8, 16 or 32; 8 or
+// 16 for bt = 512); then the solve, current coefficients abc (6 per
 // segment), Z_in and input power (zin: 4 per solve); then the total gain
 // (dB) in n_dir directions (dir: sin th, cos th, sin ph, cos ph each).
-int launch_batch_lu(const BatchSolve* sol, int S, double* Z, int* rp, int* info, int P, hipStream_t stream);
+int launch_batch_lu(const BatchSolve* sol, int S, double* Z, int* rp, int* info, int P, int bt, hipStream_t stream);
 int launch_batch_solve(const BatchSolve* sol, int S, const double* Z, const int* rp, const double* seg,
                        const int* segptr, const int* segbasis, const double* segabc, double* x, double* abc,
-                       double* zin, hipStream_t stream);
+                       double* zin, int bt, hipStream_t stream);
 int launch_batch_pattern(const BatchSolve* sol, int S, const double* seg, const double* abc, const double* zin,
-                         int n_dir, const double* dir, double* gain, hipStream_t stream);
+                         int n_dir, const double* dir, double* gain, int bt, hipStream_t stream);
+// Batched adjoint.  batch.hip: Z^T lam = g per solve with the batched
+// factors (g, lam: 2 per row).  fill.hip: the fill's adjoint contraction per
+// solve, gi (7 per row: the observation side) and gj (13 per row: the
+// source side and lam_i E_ij), both written (no accumulation).
+int launch_batch_solve_transposed(const BatchSolve* sol, int S, const double* Z, const int* rp, const double* g,
+                                  double* lam, int bt, hipStream_t stream);
+int launch_batch_adjoint_fill(const BatchSolve* sol, int S, long long rows_total, const double* seg,
+                              const double* lam, const double* abc, double rkh, double* gi, double* gj,
+                              hipStream_t stream);
 
 // fill.hip: fields of rows [i0, i0+rows) into E (rows*n*6 doubles), then
 // gather into Z (row-major complex, leading dimension ldz complex entries).

@@ -61,6 +61,7 @@ _SIGS = [
     ("nec_gpu_segments", _L, [_P, _P, _P, _L]),
     ("nec_gpu_set_precision", _L, [_P, _I]),
     ("nec_gpu_batch_solve", _L, [_P, _P, _P, _P]),
+    ("nec_gpu_batch_adjoint", _L, [_P, _D, _D, _D, _D, _D, _P, _P, _P, _P, _P]),
     ("nec_gpu_set_refine_tolerance", _L, [_P, _D]),
     ("nec_gpu_refinement", _L, [_P, _P, _P]),
     ("nec_gpu_adjoint_gradient", _L, [_P, _D, _D, _D, _D, _D, _P, _P, _P, _P]),
@@ -217,10 +218,9 @@ BATCH_STATS = ("solves", "segments", "host_ms", "fill_ms", "lu_ms", "solve_patte
                "panel_width", "autotune_ms_p8", "autotune_ms_p16", "autotune_ms_p32", "graph_matches_eager")
 
 
-def gpu_batch_solve(decks, freqs, theta_deg=(), phi_deg=(), repeats=0, panel_width=0, rkh=1.0):
-    """Batched solves of necpp_amd.decks structures (GW wires, one EX 0
-    source each).  freqs: one list for all designs, or one row per design.
-    Returns (zin complex [design, freq], gain [design, freq, dir], stats)."""
+def _batch_desc(decks, freqs, theta_deg=(), phi_deg=(), repeats=0, panel_width=0, rkh=1.0):
+    """nec_gpu_batch of necpp_amd.decks structures (and the arrays it points
+    to, which must stay alive while it is used)."""
     nd = len(decks)
     freqs = np.asarray(freqs, dtype=float)
     if freqs.ndim == 1:
@@ -244,6 +244,43 @@ def gpu_batch_solve(decks, freqs, theta_deg=(), phi_deg=(), repeats=0, panel_wid
     b = BatchDesc(nd, _ptr(a["wire_ptr"]), _ptr(a["wire_tag"]), _ptr(a["wire_ns"]), _ptr(a["wire_geom"]),
                   _ptr(a["src_tag"]), _ptr(a["src_seg"]), _ptr(a["src_volt"]), nf, _ptr(a["freq_mhz"]), ndir,
                   _ptr(a["theta_deg"]), _ptr(a["phi_deg"]), rkh, panel_width, repeats)
+    return b, a, nd, nf, ndir
+
+
+BATCH_ADJOINT_STATS = ("solves", "segments", "host_prep_ms", "fill_ms", "lu_ms", "solve_ms", "host_objective_ms",
+                       "device_adjoint_ms", "host_gradient_ms", "total_ms")
+
+
+def gpu_batch_adjoint(decks, freqs, theta_deg=90.0, phi_deg=0.0, weight=0.01, z0=50.0, panel_width=0, rkh=1.0):
+    """nec_gpu_batch_adjoint over necpp_amd.decks structures.  Returns (J
+    [design, freq], grads: list over solves of [n, 7] arrays, dJ/dfreq
+    [design, freq], info [design, freq, 3], stats)."""
+    b, arrs, nd, nf, _ = _batch_desc(decks, freqs, panel_width=panel_width, rkh=rkh)
+    nseg = [sum(w[1] for w in d["wires"]) for d in decks]
+    total = sum(nseg) * nf
+    J = np.zeros(nd * nf)
+    grad = np.zeros(7 * max(total, 1))
+    df = np.zeros(nd * nf)
+    info = np.zeros(3 * nd * nf)
+    stats = np.zeros(10)
+    z0 = complex(z0)
+    if lib().nec_gpu_batch_adjoint(ctypes.byref(b), theta_deg, phi_deg, weight, z0.real, z0.imag, _ptr(J), _ptr(grad),
+                                   _ptr(df), _ptr(info), _ptr(stats)) != 0:
+        raise RuntimeError(nec_error_message())
+    grads, off = [], 0
+    for d in range(nd):
+        for _ in range(nf):
+            grads.append(grad[7 * off:7 * (off + nseg[d])].reshape(nseg[d], 7))
+            off += nseg[d]
+    return (J.reshape(nd, nf), grads, df.reshape(nd, nf), info.reshape(nd, nf, 3),
+            dict(zip(BATCH_ADJOINT_STATS, stats.tolist())))
+
+
+def gpu_batch_solve(decks, freqs, theta_deg=(), phi_deg=(), repeats=0, panel_width=0, rkh=1.0):
+    """Batched solves of necpp_amd.decks structures (GW wires, one EX 0
+    source each).  freqs: one list for all designs, or one row per design.
+    Returns (zin complex [design, freq], gain [design, freq, dir], stats)."""
+    b, arrs, nd, nf, ndir = _batch_desc(decks, freqs, theta_deg, phi_deg, repeats, panel_width, rkh)
     zin = np.zeros(2 * nd * nf)
     gain = np.zeros(max(1, nd * nf * ndir))
     stats = np.zeros(12)

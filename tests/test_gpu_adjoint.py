@@ -188,16 +188,60 @@ def test_adjoint_torch_function():
     P = torch.tensor(decks.c4_params(count=2), dtype=torch.float64, requires_grad=True)
     J = adj.yagi_objective(P)
     J.sum().backward()
-    for b in range(2):
-        Jb, Gb, _ = adj.objective_and_gradient(P[b].detach().numpy())
-        assert float(J[b].detach()) == Jb
-        assert np.array_equal(P.grad[b].numpy(), Gb)
+    Jb, Gb, _ = adj.batch_objective_and_gradient(P.detach().numpy())
+    assert np.array_equal(J.detach().numpy(), Jb)
+    assert np.array_equal(P.grad.numpy(), Gb)
     # a single row keeps its shape
     p = torch.tensor(decks.c4_params(count=1)[0], dtype=torch.float64, requires_grad=True)
     j = adj.yagi_objective(p)
     assert j.ndim == 0
     (2.0 * j).backward()
-    assert np.allclose(p.grad.numpy(), 2.0 * adj.objective_and_gradient(p.detach().numpy())[1], rtol=0, atol=0)
+    assert np.array_equal(p.grad.numpy(), 2.0 * adj.batch_objective_and_gradient(p.detach().numpy()[None])[1][0])
+
+
+def test_batch_adjoint_matches_single_path():
+    """The batched adjoint (one call for all designs) against the
+    single-structure adjoint: the batched LU rounds differently, so J to
+    1e-12 and the gradient to 1e-9 normwise."""
+    P = decks.c4_params(count=4)
+    J, G, stats = adj.batch_objective_and_gradient(P)
+    print("batch adjoint stats", stats)
+    assert stats["solves"] == 4
+    for d, p in enumerate(P):
+        Js, Gs, _ = adj.objective_and_gradient(p)
+        assert abs(J[d] - Js) <= 1e-12 * abs(Js)
+        assert np.linalg.norm(G[d] - Gs) <= 1e-9 * np.linalg.norm(Gs), d
+
+
+def test_batch_adjoint_junction_two_frequencies():
+    theta, phi, weight, z0 = 60.0, 30.0, 0.02, 60.0 + 10.0j
+    P = np.array([[0.0, 0.0, -0.3], [0.0, 0.01, 0.0], [0.2, 0.0, 0.2], [-0.15, 0.1, 0.25]])
+    radii = np.array([0.002, 0.003, 0.0025])
+    deck = junction_deck(P, radii, 310.0, theta, phi)
+    freqs = [300.0, 320.0]
+    J, grads, df, info, stats = api.gpu_batch_adjoint([deck], freqs, theta, phi, weight, z0)
+    for f, fm in enumerate(freqs):
+        Js, g7, dfs, _ = deck_adjoint(dict(deck, freq_mhz=fm), theta, phi, weight, z0)
+        assert abs(J[0, f] - Js) <= 1e-12 * abs(Js)
+        assert np.linalg.norm(grads[f] - g7) <= 1e-9 * np.linalg.norm(g7)
+        assert abs(df[0, f] - dfs) <= 1e-9 * abs(dfs)
+
+
+def test_batch_adjoint_all_designs_match_oracle_central_differences():
+    """The 256 designs in one batched call, held to the same bar as the
+    single-structure test above."""
+    z = np.load(FIXTURE)
+    P = decks.c4_params(count=len(z["J"]))
+    J, G, stats = adj.batch_objective_and_gradient(P)
+    print("batch adjoint of 256 designs", stats)
+    assert np.all(np.abs(J - z["J"]) <= 1e-10 * np.abs(z["J"]))
+    nf = np.linalg.norm(z["fd"], axis=1)
+    errs = np.linalg.norm(G - z["fd"], axis=1) / nf
+    cons = np.linalg.norm(z["fd"] - z["fd2"], axis=1) / nf
+    ok = errs <= 1e-6 + 4.0 * cons
+    assert np.all(ok), [(int(d), errs[d], cons[d]) for d in np.nonzero(~ok)[0]]
+    assert np.median(errs) <= 1e-6
+    assert (errs <= 1e-6).sum() >= 0.8 * len(errs)
 
 
 def test_adjoint_errors():

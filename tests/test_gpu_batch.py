@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 TOL = 1e-10
 
 
-def single(deck, f):
+def single(deck, f, npoints=361):
     """Z_in and E-plane gains of one deck at f MHz through the C API."""
     ctx = api.nec_create()
     try:
@@ -25,7 +25,7 @@ def single(deck, f):
         d["freq_mhz"] = f
         decks.apply_to_api(d, api, ctx, with_rp=True)
         z = complex(api.nec_impedance_real(ctx, 0), api.nec_impedance_imag(ctx, 0))
-        g = api.gpu_pattern_array(ctx, 0, "pg_tot", 361)
+        g = api.gpu_pattern_array(ctx, 0, "pg_tot", npoints)
         return z, g
     finally:
         api.nec_delete(ctx)
@@ -68,9 +68,27 @@ def test_batch_without_pattern_and_errors():
     zin, gain, stats = api.gpu_batch_solve(designs, freqs)
     assert zin.shape == (3, 2) and gain.shape == (3, 2, 0)
     assert stats["panel_width"] in (8, 16, 32)
-    big = decks.yagi([0.5] * 13, [0.3] * 12, 0.003, 21)  # 273 segments
+    big = decks.yagi([0.5] * 25, [0.3] * 24, 0.003, 21)  # 525 segments
     with pytest.raises(RuntimeError, match="batched path takes"):
         api.gpu_batch_solve([big], freqs)
+
+
+def test_batch_large_structures_use_512_thread_workgroups():
+    # 273 and 441 segments: the 512-thread kernels, against the single path
+    rng = np.random.default_rng(4)
+    lam0 = decks.C_NEC / 145.0e6
+    designs = [decks.yagi(list(rng.uniform(0.2, 0.26, k) * lam0), list(rng.uniform(0.1, 0.3, k - 1) * lam0),
+                          0.003 * lam0, 21) for k in (13, 21)]
+    freqs = np.array([144.0, 146.0])
+    theta, phi = np.full(5, 90.0), np.arange(5) * 45.0
+    for p in (8, 16):
+        zin, gain, stats = api.gpu_batch_solve(designs, freqs, theta, phi, panel_width=p)
+        assert stats["panel_width"] == p
+        for d, deck in enumerate(designs):
+            for f, fm in enumerate(freqs):
+                z, g = single(dict(deck, rp=(0, 1, 5, 1, 0, 0, 0, 90.0, 0.0, 0.0, 45.0, 0.0, 0.0)), fm, 5)
+                assert abs(zin[d, f] - z) <= TOL * abs(z), (p, d, f)
+                assert np.max(np.abs(gain[d, f] - g)) < 1e-9
 
 
 def test_batch_full_population_sample():
