@@ -30,7 +30,7 @@ namespace necgpu {
 namespace {
 
 constexpr int NB = 64;
-constexpr int PANEL_THREADS = 512;
+constexpr int PANEL_THREADS = 1024;
 constexpr int GMAX = 256;     // most cooperating panel workgroups
 constexpr int RMAX = 112;     // most FP64 panel rows per workgroup (LDS: RMAX*(NB+1)*16 B)
 
