@@ -1164,12 +1164,15 @@ int panel_stride_override() {
   return (v && *v) ? std::atoi(v) : 0;
 }
 
-// Workgroups per free CU for the trailing update that runs beside a panel
-// (NECGPU_GEMM_OCC overrides, for measurements).
-int gemm_occupancy() {
+// Workgroups per free CU for the trailing update that runs beside a panel:
+// 1 while the update has fewer than 16 tiles per free CU (the panel beside
+// it then runs faster, measured at N = 3,280), else 2 (measured faster at
+// N = 16,020).  NECGPU_GEMM_OCC overrides, for measurements.
+int gemm_occupancy(int tiles, int free_cus) {
   const char* v = std::getenv("NECGPU_GEMM_OCC");
   const int o = (v && *v) ? std::atoi(v) : 0;
-  return o > 0 ? o : 2;  // 2 measured faster than 1 and 4 at N = 3,280
+  if (o > 0) return o;
+  return tiles < 16 * free_cus ? 1 : 2;
 }
 
 // Most 64-row tiles for the fused next-panel update (NECGPU_LU_NEXT_TILES
@@ -1256,7 +1259,6 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
            w.ticket, w.xbase, w.Gcap, G0);
   const int forced = panel_stride_override();
   Lookahead* la = (npanels > 1 && lookahead_enabled()) ? lookahead_for_device() : nullptr;
-  const int occ = gemm_occupancy();
   auto panel = [&](int k0, hipStream_t s) {
     const int nb = std::min(NB, n - k0);
     const int rows = n - k0;
@@ -1346,7 +1348,9 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
     if (hipEventRecord(la->panel_done, la->side) != hipSuccess) return 1;
     laswp(k0, nb, 0, k0, far, n, stream);
     const int Gnext = (n - next + RMAX_T - 1) / RMAX_T;
-    update(k0, nb, far, n, std::max(1, occ * (cu_count - Gnext)), stream);
+    const int free_cus = std::max(1, cu_count - Gnext);
+    const int tiles = ((n - next + 63) / 64) * ((n - far + 63) / 64);
+    update(k0, nb, far, n, gemm_occupancy(tiles, free_cus) * free_cus, stream);
     if (hipEventRecord(la->rest_done, stream) != hipSuccess) return 1;
   }
   return hipGetLastError() == hipSuccess ? 0 : 1;
