@@ -370,13 +370,10 @@ __global__ void __launch_bounds__(BT) batch_pattern_kernel(const BatchSolve* __r
       const double sill = omega * el;
       const double top = el + sill;
       const double bot = el - sill;
-      double a, too, boo;
-      if (fabs(omega) >= 1.0e-7) a = 2. * sin(sill) / omega;
-      else a = (2. - omega * omega * el * el / 3.) * el;
-      if (fabs(top) >= 1.0e-7) too = sin(top) / top;
-      else too = 1. - top * top / 6.;
-      if (fabs(bot) >= 1.0e-7) boo = sin(bot) / bot;
-      else boo = 1. - bot * bot / 6.;
+      // 2 sin(sill) / omega = 2 el sinc(sill); sin(top) / top; sin(bot) / bot
+      const double a = 2. * el * sinc(sill);
+      const double too = sinc(top);
+      const double boo = sinc(bot);
       const double b = el * (boo - too);
       const double c = el * (boo + too);
       const double rr = a * sg[7] + b * sg[10] + c * sg[11];

@@ -65,6 +65,24 @@ NEC_HD double em_impedance() { return sqrt(em_permeability() / em_permittivity()
 
 NEC_HD double deg_to_rad(double d) { return d * ((2 * kPi) / 360.0); }
 NEC_HD double rad_to_deg(double r) { return r * (360.0 / (2 * kPi)); }
+// sin(x) / x.  For |x| <= 0.5 its Taylor series to x^16 (truncation below
+// 1e-19 relative), which replaces the library sine in the far-field shape
+// factors of short segments; the reference's own small-argument branch
+// (|x| < 1e-7: 1 - x^2 / 6) is the series' first two terms.
+NEC_HD double sinc(double x) {
+  if (fabs(x) > 0.5) return sin(x) / x;
+  const double x2 = x * x;
+  double p = 1.0 / 355687428096000.0;        // 1 / 17!
+  p = p * x2 - 1.0 / 1307674368000.0;        // 1 / 15!
+  p = p * x2 + 1.0 / 6227020800.0;           // 1 / 13!
+  p = p * x2 - 1.0 / 39916800.0;             // 1 / 11!
+  p = p * x2 + 1.0 / 362880.0;               // 1 / 9!
+  p = p * x2 - 1.0 / 5040.0;                 // 1 / 7!
+  p = p * x2 + 1.0 / 120.0;                  // 1 / 5!
+  p = p * x2 - 1.0 / 6.0;                    // 1 / 3!
+  return p * x2 + 1.0;
+}
+
 NEC_HD double db10(double x) { return (x < 1.0e-20) ? -999.99 : 10.0 * log10(x); }
 NEC_HD double from_db10(double x) { return (x < -99.9) ? 0.0 : pow(10.0, x / 10.0); }
 
