@@ -46,49 +46,49 @@ __device__ int block_first_max(double v, double* s_val, int* s_row) {
 
 // In-place LU with partial pivoting (first maximum of |z|, as the
 // single-structure path) of each solve's n x n matrix, n <= BT.
-template <int BT, int P, int CW>
+//
+// The panel (logical rows k0 .. n-1, columns k0 .. k0+P-1) is held in LDS
+// column-major, so a wave reading one column over 64 rows touches 64
+// consecutive 16-byte entries (no bank conflicts) and a wave reading one row
+// broadcasts.  The trailing update maps lanes to columns: thread t of row
+// group g owns column k0 + pw + t % W and rows pw + g, pw + g + G, ... of the
+// block, so every global access of a wave is one contiguous row segment.  Each
+// thread forms its column of U12 = L11^-1 A12 in registers (the G groups
+// repeat that small triangular solve rather than exchange it), then
+// subtracts L21 U12 from its rows, four rows at a time.
+template <int BT, int P>
 __global__ void __launch_bounds__(BT) batch_lu_kernel(const BatchSolve* __restrict__ sol, double* __restrict__ Zall,
                                                       int* __restrict__ rpall, int* __restrict__ info) {
-  __shared__ double sP[BT * P * 2];  // panel, logical rows k0.. (row r at r * P)
-  __shared__ double sU[P * CW * 2];  // U12 chunk [c][jj]
+  __shared__ double2 sP[P * BT];  // sP[c * BT + r]: logical row k0 + r, column k0 + c
   __shared__ int s_rp[BT];
   __shared__ double s_val[BT / 64];
   __shared__ int s_row[BT / 64];
   const BatchSolve d = sol[blockIdx.x];
   const int n = d.n;
   const int t = threadIdx.x;
-  double* Z = Zall + 2 * d.z0;
+  double2* Z = reinterpret_cast<double2*>(Zall) + d.z0;
   if (t < n) s_rp[t] = t;
   int first_zero = 0;
   __syncthreads();
   for (int k0 = 0; k0 < n; k0 += P) {
     const int pw = min(P, n - k0);
     const int m = n - k0;
-    // panel: logical rows k0 .. n-1, columns k0 .. k0+pw-1
-    if (t < m)
-      for (int c = 0; c < P; ++c) {
-        double vr = 0.0, vi = 0.0;
-        if (c < pw) {
-          const double* v = Z + 2 * ((long long)s_rp[k0 + t] * n + k0 + c);
-          vr = v[0];
-          vi = v[1];
-        }
-        sP[2 * (t * P + c)] = vr;
-        sP[2 * (t * P + c) + 1] = vi;
-      }
+    if (t < m) {
+      const double2* src = Z + (long long)s_rp[k0 + t] * n + k0;
+#pragma unroll
+      for (int c = 0; c < P; ++c) sP[c * BT + t] = c < pw ? src[c] : make_double2(0.0, 0.0);
+    }
     __syncthreads();
     for (int c = 0; c < pw; ++c) {
       const int k = k0 + c;
       double v = -1.0;
-      if (t >= c && t < m) v = hypot(sP[2 * (t * P + c)], sP[2 * (t * P + c) + 1]);
+      if (t >= c && t < m) v = hypot(sP[c * BT + t].x, sP[c * BT + t].y);
       const int pr = block_first_max<BT>(v, s_val, s_row);
       if (pr >= 0 && pr != c) {
         if (t < P) {
-          const double ar = sP[2 * (c * P + t)], ai = sP[2 * (c * P + t) + 1];
-          sP[2 * (c * P + t)] = sP[2 * (pr * P + t)];
-          sP[2 * (c * P + t) + 1] = sP[2 * (pr * P + t) + 1];
-          sP[2 * (pr * P + t)] = ar;
-          sP[2 * (pr * P + t) + 1] = ai;
+          const double2 a = sP[t * BT + c];
+          sP[t * BT + c] = sP[t * BT + pr];
+          sP[t * BT + pr] = a;
         }
         if (t == 0) {
           const int tmp = s_rp[k0 + c];
@@ -97,78 +97,103 @@ __global__ void __launch_bounds__(BT) batch_lu_kernel(const BatchSolve* __restri
         }
       }
       __syncthreads();
-      const zc dpiv = zmk(sP[2 * (c * P + c)], sP[2 * (c * P + c) + 1]);
+      const double2 dp = sP[c * BT + c];
+      const zc dpiv = zmk(dp.x, dp.y);
       const bool zero = (dpiv.re == 0.0 && dpiv.im == 0.0);
       if (zero && first_zero == 0) first_zero = k + 1;
       if (!zero && t > c && t < m) {
-        const zc l = zdiv(zmk(sP[2 * (t * P + c)], sP[2 * (t * P + c) + 1]), dpiv);
-        sP[2 * (t * P + c)] = l.re;
-        sP[2 * (t * P + c) + 1] = l.im;
+        const double2 a = sP[c * BT + t];
+        const zc l = zdiv(zmk(a.x, a.y), dpiv);
+        sP[c * BT + t] = make_double2(l.re, l.im);
         for (int cc = c + 1; cc < pw; ++cc) {
-          const zc u = zmk(sP[2 * (c * P + cc)], sP[2 * (c * P + cc) + 1]);
-          const zc pu = zmul(l, u);
-          sP[2 * (t * P + cc)] -= pu.re;
-          sP[2 * (t * P + cc) + 1] -= pu.im;
+          const double2 u = sP[cc * BT + c];
+          const zc pu = zmul(l, zmk(u.x, u.y));
+          double2 e = sP[cc * BT + t];
+          e.x -= pu.re;
+          e.y -= pu.im;
+          sP[cc * BT + t] = e;
         }
       }
       __syncthreads();
     }
-    if (t < m)
-      for (int c = 0; c < pw; ++c) {
-        double* v = Z + 2 * ((long long)s_rp[k0 + t] * n + k0 + c);
-        v[0] = sP[2 * (t * P + c)];
-        v[1] = sP[2 * (t * P + c) + 1];
-      }
-    // trailing columns in chunks of CW: U12 = L11^-1 A12, A22 -= L21 U12
-    for (int j0 = k0 + pw; j0 < n; j0 += CW) {
-      const int cw = min(CW, n - j0);
-      for (int e = t; e < P * CW; e += BT) {
-        const int c = e / CW, jj = e % CW;
-        double vr = 0.0, vi = 0.0;
-        if (c < pw && jj < cw) {
-          const double* v = Z + 2 * ((long long)s_rp[k0 + c] * n + j0 + jj);
-          vr = v[0];
-          vi = v[1];
-        }
-        sU[2 * e] = vr;
-        sU[2 * e + 1] = vi;
-      }
-      __syncthreads();
-      if (t < cw) {
-        for (int c = 1; c < pw; ++c) {
-          double ur = sU[2 * (c * CW + t)], ui = sU[2 * (c * CW + t) + 1];
-          for (int c2 = 0; c2 < c; ++c2) {
-            const double lr = sP[2 * (c * P + c2)], li = sP[2 * (c * P + c2) + 1];
-            const double xr = sU[2 * (c2 * CW + t)], xi = sU[2 * (c2 * CW + t) + 1];
-            ur -= lr * xr - li * xi;
-            ui -= lr * xi + li * xr;
-          }
-          sU[2 * (c * CW + t)] = ur;
-          sU[2 * (c * CW + t) + 1] = ui;
-        }
-        for (int c = 0; c < pw; ++c) {
-          double* v = Z + 2 * ((long long)s_rp[k0 + c] * n + j0 + t);
-          v[0] = sU[2 * (c * CW + t)];
-          v[1] = sU[2 * (c * CW + t) + 1];
-        }
-      }
-      __syncthreads();
-      if (t >= pw && t < m) {
-        double* row = Z + 2 * ((long long)s_rp[k0 + t] * n + j0);
-        for (int jj = 0; jj < cw; ++jj) {
-          double ar = row[2 * jj], ai = row[2 * jj + 1];
-          for (int c = 0; c < pw; ++c) {
-            const double lr = sP[2 * (t * P + c)], li = sP[2 * (t * P + c) + 1];
-            const double ur = sU[2 * (c * CW + jj)], ui = sU[2 * (c * CW + jj) + 1];
-            ar -= lr * ur - li * ui;
-            ai -= lr * ui + li * ur;
-          }
-          row[2 * jj] = ar;
-          row[2 * jj + 1] = ai;
-        }
-      }
-      __syncthreads();
+    if (t < m) {
+      double2* dst = Z + (long long)s_rp[k0 + t] * n + k0;
+      for (int c = 0; c < pw; ++c) dst[c] = sP[c * BT + t];
     }
+    const int w = m - pw;  // trailing columns
+    if (w > 0) {
+      const int W = (w + 63) & ~63;
+      const int G = BT / W;  // row groups (>= 1: w < n <= BT)
+      const int col = t % W, grp = t / W;
+      if (col < w && grp < G) {
+        const int j = k0 + pw + col;
+        double ur[P], ui[P];
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+          double2 a = make_double2(0.0, 0.0);
+          if (c < pw) a = Z[(long long)s_rp[k0 + c] * n + j];
+          ur[c] = a.x;
+          ui[c] = a.y;
+        }
+#pragma unroll
+        for (int c = 1; c < P; ++c) {
+          if (c < pw) {
+#pragma unroll
+            for (int c2 = 0; c2 < c; ++c2) {
+              const double2 l = sP[c2 * BT + c];
+              ur[c] -= l.x * ur[c2] - l.y * ui[c2];
+              ui[c] -= l.x * ui[c2] + l.y * ur[c2];
+            }
+          }
+        }
+        if (grp == 0) {
+#pragma unroll
+          for (int c = 0; c < P; ++c)
+            if (c < pw) Z[(long long)s_rp[k0 + c] * n + j] = make_double2(ur[c], ui[c]);
+        }
+        int r = pw + grp;
+        for (; r + 3 * G < m; r += 4 * G) {
+          double2* z0 = Z + (long long)s_rp[k0 + r] * n + j;
+          double2* z1 = Z + (long long)s_rp[k0 + r + G] * n + j;
+          double2* z2 = Z + (long long)s_rp[k0 + r + 2 * G] * n + j;
+          double2* z3 = Z + (long long)s_rp[k0 + r + 3 * G] * n + j;
+          double2 a0 = *z0, a1 = *z1, a2 = *z2, a3 = *z3;
+#pragma unroll
+          for (int c = 0; c < P; ++c) {
+            if (c < pw) {
+              const double2 l0 = sP[c * BT + r], l1 = sP[c * BT + r + G];
+              const double2 l2 = sP[c * BT + r + 2 * G], l3 = sP[c * BT + r + 3 * G];
+              a0.x -= l0.x * ur[c] - l0.y * ui[c];
+              a0.y -= l0.x * ui[c] + l0.y * ur[c];
+              a1.x -= l1.x * ur[c] - l1.y * ui[c];
+              a1.y -= l1.x * ui[c] + l1.y * ur[c];
+              a2.x -= l2.x * ur[c] - l2.y * ui[c];
+              a2.y -= l2.x * ui[c] + l2.y * ur[c];
+              a3.x -= l3.x * ur[c] - l3.y * ui[c];
+              a3.y -= l3.x * ui[c] + l3.y * ur[c];
+            }
+          }
+          *z0 = a0;
+          *z1 = a1;
+          *z2 = a2;
+          *z3 = a3;
+        }
+        for (; r < m; r += G) {
+          double2* z0 = Z + (long long)s_rp[k0 + r] * n + j;
+          double2 a0 = *z0;
+#pragma unroll
+          for (int c = 0; c < P; ++c) {
+            if (c < pw) {
+              const double2 l0 = sP[c * BT + r];
+              a0.x -= l0.x * ur[c] - l0.y * ui[c];
+              a0.y -= l0.x * ui[c] + l0.y * ur[c];
+            }
+          }
+          *z0 = a0;
+        }
+      }
+    }
+    __syncthreads();
   }
   if (t < n) rpall[d.row0 + t] = s_rp[t];
   if (t == 0) info[blockIdx.x] = first_zero;
@@ -407,15 +432,15 @@ int launch_batch_lu(const BatchSolve* sol, int S, double* Z, int* rp, int* info,
   if (bt == 512) {
     // the 512-row panel takes P * 16 KB of LDS: P = 8 or 16
     if (P == 8)
-      launch_k(K_BATCH_LU, batch_lu_kernel<512, 8, 32>, dim3(S), dim3(512), 0u, stream, sol, Z, rp, info);
+      launch_k(K_BATCH_LU, batch_lu_kernel<512, 8>, dim3(S), dim3(512), 0u, stream, sol, Z, rp, info);
     else
-      launch_k(K_BATCH_LU, batch_lu_kernel<512, 16, 32>, dim3(S), dim3(512), 0u, stream, sol, Z, rp, info);
+      launch_k(K_BATCH_LU, batch_lu_kernel<512, 16>, dim3(S), dim3(512), 0u, stream, sol, Z, rp, info);
   } else if (P == 32) {
-    launch_k(K_BATCH_LU, batch_lu_kernel<256, 32, 32>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, info);
+    launch_k(K_BATCH_LU, batch_lu_kernel<256, 32>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, info);
   } else if (P == 8) {
-    launch_k(K_BATCH_LU, batch_lu_kernel<256, 8, 32>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, info);
+    launch_k(K_BATCH_LU, batch_lu_kernel<256, 8>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, info);
   } else {
-    launch_k(K_BATCH_LU, batch_lu_kernel<256, 16, 32>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, info);
+    launch_k(K_BATCH_LU, batch_lu_kernel<256, 16>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, info);
   }
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
