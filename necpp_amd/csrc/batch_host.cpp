@@ -73,6 +73,8 @@ struct BatchPrep {
   std::vector<Geometry> geo;  // per design
   std::vector<double> freq;   // per solve, MHz
   long long rows = 0, zents = 0;
+  int fill_rows = 1;          // rows per fill workgroup
+  std::vector<int> fill_blk;  // (solve, first row) per fill workgroup
 };
 
 // body(s) for s = 0 .. count-1 on the host's cores; the first exception is
@@ -179,14 +181,25 @@ void prepare_batch(const BatchInput& in, BatchPrep* p) {
     rows += n;
     zents += (long long)n * n;
   }
+  p->fill_rows = batch_fill_rows(p->maxn);
+  for (int s = 0; s < S; ++s)
+    for (int r0 = 0; r0 < p->sol[s].n; r0 += p->fill_rows) {
+      p->fill_blk.push_back(s);
+      p->fill_blk.push_back(r0);
+    }
 }
 
 // Device copies of a batch's tables and the solve buffers.
 struct BatchDev {
-  DevBuf sol, seg, colptr, colj, colabc, segptr, segbasis, segabc;
+  DevBuf sol, seg, colptr, colj, colabc, segptr, segbasis, segabc, fill_blk;
   DevBuf Z, rp, info, x, abc, zin;
+  int fill_rows = 1, fill_nblk = 0, maxn = 0;
   void upload_all(const BatchPrep& p, hipStream_t st) {
     upload(sol, p.sol, st);
+    upload(fill_blk, p.fill_blk, st);
+    fill_rows = p.fill_rows;
+    fill_nblk = static_cast<int>(p.fill_blk.size() / 2);
+    maxn = p.maxn;
     upload(seg, p.seg, st);
     upload(colptr, p.colptr, st);
     upload(colj, p.colj, st);
@@ -259,8 +272,8 @@ void run_batch(const BatchInput& in, BatchOutput* out) {
   DevBuf& d_segabc = dv.segabc;
 
   auto fill = [&]() {
-    if (launch_batch_fill(dsol, S, rows, d_seg.as<double>(), d_colptr.as<int>(), d_colj.as<int>(),
-                          d_colabc.as<double>(), Z, rkh, st))
+    if (launch_batch_fill(dsol, dv.fill_blk.as<int>(), dv.fill_nblk, dv.fill_rows, dv.maxn, d_seg.as<double>(),
+                          d_colptr.as<int>(), d_colj.as<int>(), d_colabc.as<double>(), Z, rkh, st))
       hip_check(hipGetLastError(), "batch fill");
   };
   auto lu = [&](int P) {
@@ -394,8 +407,8 @@ void run_batch_adjoint(const BatchInput& in, const AdjointSpec& spec, BatchAdjoi
   const BatchSolve* dsol = dv.sol.as<BatchSolve>();
   EventPair e1, e2, e3, e4;
   hip_check(hipEventRecord(e1.a, st), "event");
-  if (launch_batch_fill(dsol, S, rows, dv.seg.as<double>(), dv.colptr.as<int>(), dv.colj.as<int>(),
-                        dv.colabc.as<double>(), dv.Z.as<double>(), rkh, st))
+  if (launch_batch_fill(dsol, dv.fill_blk.as<int>(), dv.fill_nblk, dv.fill_rows, dv.maxn, dv.seg.as<double>(),
+                        dv.colptr.as<int>(), dv.colj.as<int>(), dv.colabc.as<double>(), dv.Z.as<double>(), rkh, st))
     hip_check(hipGetLastError(), "batch fill");
   hip_check(hipEventRecord(e1.b, st), "event");
   hip_check(hipEventRecord(e2.a, st), "event");

@@ -18,6 +18,7 @@
 // Both are compiled with -ffp-contract=off and follow the reference's
 // operation order, so entries agree with nec2++ to the ulp-level differences
 // of the device sin/cos/log.
+#include <algorithm>
 #include <type_traits>
 
 #include "dual.h"
@@ -523,26 +524,28 @@ __global__ void __launch_bounds__(256) fill_assemble_kernel(const double* __rest
   zp[1] = acci;
 }
 
-// Batched fill: one workgroup per (solve, row i).  The fields of every
-// source j at i go to LDS, then each entry Z[i][k] gathers them over basis
-// k's support in ascending j, as fill_assemble does.
-__global__ void __launch_bounds__(256) batch_fill_kernel(const BatchSolve* __restrict__ sol, int S,
+// Batched fill: one workgroup per (solve, block of R consecutive rows), the
+// block's solve and first row from the table blk (2 ints per workgroup, made
+// on the host).  Threads run over the block's R x n (row, source) pairs, so
+// lanes stay busy when n is not a multiple of the workgroup size; the fields
+// go to LDS (R n 6 doubles, dynamic), then each entry Z[i][k] gathers them
+// over basis k's support in ascending j, as fill_assemble does.  The
+// register budget is capped at three waves per SIMD (167 VGPRs, a few
+// spilled): C3-like batches fill faster than at two waves (173 VGPRs) or four
+// (128 VGPRs, heavy spilling).
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) batch_fill_kernel(const BatchSolve* __restrict__ sol,
+                                                         const int* __restrict__ blk, int R,
                                                          const double* __restrict__ seg,
                                                          const int* __restrict__ colptr,
                                                          const int* __restrict__ colj,
                                                          const double* __restrict__ colabc, double* __restrict__ Zall,
                                                          double rkh) {
-  __shared__ double sE[kBatchMaxN * 6];
-  const long long blk = blockIdx.x;
-  int lo = 0, hi = S - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (sol[mid].row0 <= blk) lo = mid;
-    else hi = mid - 1;
-  }
-  const BatchSolve d = sol[lo];
+  extern __shared__ double sE[];
+  const BatchSolve d = sol[blk[2 * blockIdx.x]];
+  const int r0 = blk[2 * blockIdx.x + 1];
   const int n = d.n;
-  const int i = static_cast<int>(blk - d.row0);
+  const int rows = min(R, n - r0);
+  const int np = rows * n;
   const double* p = seg + d.seg0;
   DevSegments g;
   g.x = p;
@@ -553,10 +556,11 @@ __global__ void __launch_bounds__(256) batch_fill_kernel(const BatchSolve* __res
   g.cab = p + 5 * n;
   g.sab = p + 6 * n;
   g.salp = p + 7 * n;
-  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+  for (int q = threadIdx.x; q < np; q += blockDim.x) {
+    const int r = q / n, j = q - r * n;
     zc etk, ets, etc;
-    pair_fields(g, i, j, rkh, 0, 2, 2, &etk, &ets, &etc);
-    double* o = sE + 6 * j;
+    pair_fields(g, r0 + r, j, rkh, 0, 2, 2, &etk, &ets, &etc);
+    double* o = sE + 6 * q;
     o[0] = etk.re;
     o[1] = etk.im;
     o[2] = ets.re;
@@ -568,17 +572,19 @@ __global__ void __launch_bounds__(256) batch_fill_kernel(const BatchSolve* __res
   const int* cp = colptr + d.cptr0;
   const int* cj = colj + d.csr0;
   const double* cv = colabc + 3 * d.csr0;
-  double* zrow = Zall + 2 * (d.z0 + (long long)i * n);
-  for (int k = threadIdx.x; k < n; k += blockDim.x) {
+  double* Z = Zall + 2 * (d.z0 + (long long)r0 * n);
+  for (int q = threadIdx.x; q < np; q += blockDim.x) {
+    const int r = q / n, k = q - r * n;
+    const double* er = sE + 6 * (r * n);
     double accr = 0.0, acci = 0.0;
-    for (int q = cp[k]; q < cp[k + 1]; ++q) {
-      const double* e = sE + 6 * cj[q];
-      const double a = cv[3 * q], b = cv[3 * q + 1], c = cv[3 * q + 2];
-      accr += ((e[0] * a) + (e[2] * b)) + (e[4] * c);
-      acci += ((e[1] * a) + (e[3] * b)) + (e[5] * c);
+    for (int e = cp[k]; e < cp[k + 1]; ++e) {
+      const double* f = er + 6 * cj[e];
+      const double a = cv[3 * e], b = cv[3 * e + 1], c = cv[3 * e + 2];
+      accr += ((f[0] * a) + (f[2] * b)) + (f[4] * c);
+      acci += ((f[1] * a) + (f[3] * b)) + (f[5] * c);
     }
-    zrow[2 * k] = accr;
-    zrow[2 * k + 1] = acci;
+    Z[2 * q] = accr;
+    Z[2 * q + 1] = acci;
   }
 }
 
@@ -804,10 +810,14 @@ int launch_batch_adjoint_fill(const BatchSolve* sol, int S, long long rows_total
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-int launch_batch_fill(const BatchSolve* sol, int S, long long rows_total, const double* seg, const int* colptr,
-                      const int* colj, const double* colabc, double* Z, double rkh, hipStream_t stream) {
-  if (S <= 0 || rows_total <= 0) return 0;
-  launch_k(K_BATCH_FILL, batch_fill_kernel, dim3((unsigned)rows_total), dim3(256), 0u, stream, sol, S, seg, colptr,
+int batch_fill_rows(int maxn) { return std::max(1, std::min(8, 1024 / std::max(1, maxn))); }
+
+int launch_batch_fill(const BatchSolve* sol, const int* blk, int nblk, int R, int maxn, const double* seg,
+                      const int* colptr, const int* colj, const double* colabc, double* Z, double rkh,
+                      hipStream_t stream) {
+  if (nblk <= 0) return 0;
+  const uint32_t lds = (uint32_t)R * (uint32_t)maxn * 6u * sizeof(double);
+  launch_k(K_BATCH_FILL, batch_fill_kernel, dim3((unsigned)nblk), dim3(256), lds, stream, sol, blk, R, seg, colptr,
            colj, colabc, Z, rkh);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }

@@ -100,9 +100,13 @@ struct BatchSolve {
 };
 constexpr int kBatchMaxN = 512;  // largest structure of the batched path
 
-// fill.hip: one workgroup per (solve, row); rows_total = sum of n.
-int launch_batch_fill(const BatchSolve* sol, int S, long long rows_total, const double* seg, const int* colptr,
-                      const int* colj, const double* colabc, double* Z, double rkh, hipStream_t stream);
+// fill.hip: one workgroup per block of R consecutive rows of one solve
+// (R = batch_fill_rows(largest n)); blk holds (solve, first row) of each of
+// the nblk blocks.  LDS: R * maxn * 48 bytes.
+int batch_fill_rows(int maxn);
+int launch_batch_fill(const BatchSolve* sol, const int* blk, int nblk, int R, int maxn, const double* seg,
+                      const int* colptr, const int* colj, const double* colabc, double* Z, double rkh,
+                      hipStream_t stream);
 
 // batch.hip: one workgroup per solve, bt = 256 or 512 threads (bt >= the
 // largest n).  LU with row permutation rp (logical row i of the factors is
