@@ -451,17 +451,33 @@ void run_batch_adjoint(const BatchInput& in, const AdjointSpec& spec, BatchAdjoi
   out->front_ms = std::chrono::duration<double, std::milli>(clk::now() - t1).count();
 
   // device: transposed solves and the fill contraction
-  DevBuf d_g, d_lam, d_gi, d_gj;
+  DevBuf d_g, d_lam, d_gi, d_gj, d_ablk, d_afb, d_part;
   upload(d_g, g, st);
   d_lam.ensure((size_t)rows * 16);
   d_gi.ensure((size_t)rows * 7 * 8);
   d_gj.ensure((size_t)rows * 13 * 8);
+  // contraction blocks: (solve, first row) and the first block of each solve
+  std::vector<int> ablk, afb(S + 1, 0);
+  const int arows = batch_adjoint_rows();
+  for (int s = 0; s < S; ++s) {
+    afb[s] = static_cast<int>(ablk.size() / 2);
+    for (int r0 = 0; r0 < prep.sol[s].n; r0 += arows) {
+      ablk.push_back(s);
+      ablk.push_back(r0);
+    }
+  }
+  const int anblk = static_cast<int>(ablk.size() / 2);
+  afb[S] = anblk;
+  upload(d_ablk, ablk, st);
+  upload(d_afb, afb, st);
+  d_part.ensure((size_t)anblk * 13 * prep.maxn * 8);
   hip_check(hipEventRecord(e4.a, st), "event");
   if (launch_batch_solve_transposed(dsol, S, dv.Z.as<double>(), dv.rp.as<int>(), d_g.as<double>(), d_lam.as<double>(),
                                     bt, st))
     hip_check(hipGetLastError(), "batch transposed solve");
-  if (launch_batch_adjoint_fill(dsol, S, rows, dv.seg.as<double>(), d_lam.as<double>(), dv.abc.as<double>(), rkh,
-                                d_gi.as<double>(), d_gj.as<double>(), st))
+  if (launch_batch_adjoint_fill(dsol, S, d_ablk.as<int>(), d_afb.as<int>(), anblk, prep.maxn, dv.seg.as<double>(),
+                                d_lam.as<double>(), dv.abc.as<double>(), rkh, d_gi.as<double>(), d_part.as<double>(),
+                                d_gj.as<double>(), st))
     hip_check(hipGetLastError(), "batch adjoint fill");
   hip_check(hipEventRecord(e4.b, st), "event");
   std::vector<zc> lam(rows);

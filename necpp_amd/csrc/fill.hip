@@ -531,15 +531,13 @@ __global__ void __launch_bounds__(256) fill_assemble_kernel(const double* __rest
 // go to LDS (R n 6 doubles, dynamic), then each entry Z[i][k] gathers them
 // over basis k's support in ascending j, as fill_assemble does.  The
 // register budget is capped at three waves per SIMD (167 VGPRs, a few
-// spilled): C3-like batches fill faster than at two waves (173 VGPRs) or four
+// spilled): batches of Yagi-sized structures (n = 63 .. 252) fill faster
+// than at two waves (173 VGPRs) or four
 // (128 VGPRs, heavy spilling).
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))) batch_fill_kernel(const BatchSolve* __restrict__ sol,
-                                                         const int* __restrict__ blk, int R,
-                                                         const double* __restrict__ seg,
-                                                         const int* __restrict__ colptr,
-                                                         const int* __restrict__ colj,
-                                                         const double* __restrict__ colabc, double* __restrict__ Zall,
-                                                         double rkh) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8)))
+batch_fill_kernel(const BatchSolve* __restrict__ sol, const int* __restrict__ blk, int R,
+                  const double* __restrict__ seg, const int* __restrict__ colptr, const int* __restrict__ colj,
+                  const double* __restrict__ colabc, double* __restrict__ Zall, double rkh) {
   extern __shared__ double sE[];
   const BatchSolve d = sol[blk[2 * blockIdx.x]];
   const int r0 = blk[2 * blockIdx.x + 1];
@@ -604,10 +602,40 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 8))
 // (13 per pair: d/dc_j x y z, d/ds_j, d/d(cab, sab, salp)_j, and
 // lam_i E_ij as 3 complex) and are summed over rows by adjoint_reduce in
 // row order, so the result does not depend on scheduling.
-constexpr int ADJ_NV = 8;  // dx, dy, dz (c_i - c_j), b_i, s_j, cab_j, sab_j, salp_j
 constexpr int ADJ_PJ = 13;
 constexpr int ADJ_GI = 7;
 constexpr int ADJ_THREADS = 256;
+
+// Fields of pair (i, j) with forward derivatives in one group of four of the
+// eight variables: G = 0: dx, dy, dz (c_i - c_j; a segment's own pair has no
+// difference to vary) and b_i; G = 1: s_j, cab_j, sab_j, salp_j.  Each
+// derivative component is computed from the values and its own seed only, so
+// two passes of four give the same numbers as one of eight, with half the
+// registers live.
+template <int G>
+__device__ __forceinline__ void adjoint_fields(const DevSegments& g, int i, int j, double rkh, cx<Dn<4>>* etk,
+                                               cx<Dn<4>>* ets, cx<Dn<4>>* etc, cx<double>* ev) {
+  typedef Dn<4> D;
+  const bool self = (i == j);
+  D dx(g.x[i] - g.x[j]), dy(g.y[i] - g.y[j]), dz(g.z[i] - g.z[j]);
+  D ai(g.b[i]), sj(g.s[j]), bj(g.b[j]), cj(g.cab[j]), sbj(g.sab[j]), slj(g.salp[j]);
+  if (G == 0) {
+    if (!self) {
+      dx.d[0] = 1.0;
+      dy.d[1] = 1.0;
+      dz.d[2] = 1.0;
+    }
+    ai.d[3] = 1.0;
+  } else {
+    sj.d[0] = 1.0;
+    cj.d[1] = 1.0;
+    sbj.d[2] = 1.0;
+    slj.d[3] = 1.0;
+  }
+  const D zero(0.0);
+  pair_fields_v<D>(dx, dy, dz, ai, g.cab[i], g.sab[i], g.salp[i], zero, zero, zero, sj, bj, cj, sbj, slj, self, rkh, 0,
+                   2, 2, etk, ets, etc, ev);
+}
 
 // One pair's adjoint terms: oi (7, observation side: d/dc_i x y z, d/db_i,
 // d/d(cab, sab, salp)_i) and oj (13, source side: d/dc_j x y z, d/ds_j,
@@ -615,27 +643,6 @@ constexpr int ADJ_THREADS = 256;
 // solve (A, B, C real and imaginary planes).
 __device__ __forceinline__ void adjoint_pair(const DevSegments& g, int n, int i, int j, double rkh, double lr,
                                              double li, const double* __restrict__ abc, double* oi, double* oj) {
-  typedef Dn<ADJ_NV> D;
-  const bool self = (i == j);
-  // positions enter as c_i = c_j + (c_i - c_j): seed the difference; a
-  // segment's own pair has no difference to vary
-  D dx(g.x[i] - g.x[j]), dy(g.y[i] - g.y[j]), dz(g.z[i] - g.z[j]);
-  D ai(g.b[i]), sj(g.s[j]), bj(g.b[j]), cj(g.cab[j]), sbj(g.sab[j]), slj(g.salp[j]);
-  if (!self) {
-    dx.d[0] = 1.0;
-    dy.d[1] = 1.0;
-    dz.d[2] = 1.0;
-  }
-  ai.d[3] = 1.0;
-  sj.d[4] = 1.0;
-  cj.d[5] = 1.0;
-  sbj.d[6] = 1.0;
-  slj.d[7] = 1.0;
-  const D zero(0.0);
-  cx<D> etk, ets, etc;
-  cx<double> ev[9];
-  pair_fields_v<D>(dx, dy, dz, ai, g.cab[i], g.sab[i], g.salp[i], zero, zero, zero, sj, bj, cj, sbj, slj, self, rkh, 0,
-                   2, 2, &etk, &ets, &etc, ev);
   const double Ar = abc[j], Ai = abc[n + j], Br = abc[2 * n + j], Bi = abc[3 * n + j], Cr = abc[4 * n + j],
                Ci = abc[5 * n + j];
   // -2 Re(lam * (dk A + ds B + dc C)) for complex derivative triples
@@ -644,17 +651,34 @@ __device__ __forceinline__ void adjoint_pair(const DevSegments& g, int n, int i,
     const double fi = ((kr * Ai + ki * Ar) + (sr * Bi + si * Br)) + (cr * Ci + ci * Cr);
     return -2.0 * (lr * fr - li * fi);
   };
-  double cv[ADJ_NV];
-  for (int v = 0; v < ADJ_NV; ++v)
-    cv[v] = contract(etk.re.d[v], etk.im.d[v], ets.re.d[v], ets.im.d[v], etc.re.d[v], etc.im.d[v]);
+  double cv[8];
+  {
+    cx<Dn<4>> etk, ets, etc;
+    cx<double> ev[9];
+    adjoint_fields<0>(g, i, j, rkh, &etk, &ets, &etc, ev);
+    for (int v = 0; v < 4; ++v)
+      cv[v] = contract(etk.re.d[v], etk.im.d[v], ets.re.d[v], ets.im.d[v], etc.re.d[v], etc.im.d[v]);
+    // observation direction: etk = e0 cab + e1 sab + e2 salp (e3..e5: ets,
+    // e6..e8: etc)
+    for (int q = 0; q < 3; ++q)
+      oi[4 + q] = contract(ev[q].re, ev[q].im, ev[3 + q].re, ev[3 + q].im, ev[6 + q].re, ev[6 + q].im);
+    oj[7] = lr * etk.re.v - li * etk.im.v;
+    oj[8] = lr * etk.im.v + li * etk.re.v;
+    oj[9] = lr * ets.re.v - li * ets.im.v;
+    oj[10] = lr * ets.im.v + li * ets.re.v;
+    oj[11] = lr * etc.re.v - li * etc.im.v;
+    oj[12] = lr * etc.im.v + li * etc.re.v;
+  }
+  {
+    cx<Dn<4>> etk, ets, etc;
+    adjoint_fields<1>(g, i, j, rkh, &etk, &ets, &etc, nullptr);
+    for (int v = 0; v < 4; ++v)
+      cv[4 + v] = contract(etk.re.d[v], etk.im.d[v], ets.re.d[v], ets.im.d[v], etc.re.d[v], etc.im.d[v]);
+  }
   oi[0] = cv[0];
   oi[1] = cv[1];
   oi[2] = cv[2];
   oi[3] = cv[3];
-  // observation direction: etk = e0 cab + e1 sab + e2 salp (e3..e5: ets,
-  // e6..e8: etc)
-  for (int q = 0; q < 3; ++q)
-    oi[4 + q] = contract(ev[q].re, ev[q].im, ev[3 + q].re, ev[3 + q].im, ev[6 + q].re, ev[6 + q].im);
   oj[0] = -cv[0];
   oj[1] = -cv[1];
   oj[2] = -cv[2];
@@ -662,12 +686,6 @@ __device__ __forceinline__ void adjoint_pair(const DevSegments& g, int n, int i,
   oj[4] = cv[5];
   oj[5] = cv[6];
   oj[6] = cv[7];
-  oj[7] = lr * etk.re.v - li * etk.im.v;
-  oj[8] = lr * etk.im.v + li * etk.re.v;
-  oj[9] = lr * ets.re.v - li * ets.im.v;
-  oj[10] = lr * ets.im.v + li * ets.re.v;
-  oj[11] = lr * etc.re.v - li * etc.im.v;
-  oj[12] = lr * etc.im.v + li * etc.re.v;
 }
 
 // Sum of K per-thread values over the workgroup (ADJ_THREADS threads);
@@ -722,29 +740,28 @@ __global__ void __launch_bounds__(ADJ_THREADS) adjoint_fill_kernel(DevSegments g
   block_sum_out<ADJ_GI>(acc, sred, gi + (size_t)i * ADJ_GI);
 }
 
-// Batched contraction: one workgroup per (solve, segment a) over all rows of
-// the concatenated solves (the solve found by row0, as batch_fill_kernel).
-// ROWSIDE: a = observation row i, threads over sources j, gi of row i;
-// otherwise a = source j, threads over observation rows i, gj of segment j.
-// Every pair is evaluated once in each pass; the sums run over threads in a
-// fixed order, so the result does not depend on scheduling.
-template <bool ROWSIDE>
-__global__ void __launch_bounds__(ADJ_THREADS) batch_adjoint_kernel(const BatchSolve* __restrict__ sol, int S,
+// Batched contraction: one workgroup per block of ADJ_ROWS consecutive
+// observation rows of one solve (table blk: solve, first row), threads over
+// the sources j.  Each pair is evaluated once.  Its observation-side terms
+// are summed over the workgroup per row (gi, 7 per row); its source-side
+// terms are summed per source over the block's rows, in row order, in LDS and
+// written to the block's slice of part (13 x maxn, [term][j]);
+// batch_adjoint_reduce adds the slices of a solve in block order.  Every sum
+// runs in a fixed order, so the result does not depend on scheduling.
+constexpr int ADJ_ROWS = 16;
+__global__ void __launch_bounds__(ADJ_THREADS) batch_adjoint_kernel(const BatchSolve* __restrict__ sol,
+                                                                    const int* __restrict__ blk, int maxn,
                                                                     const double* __restrict__ seg,
                                                                     const double* __restrict__ lamall,
                                                                     const double* __restrict__ abcall, double rkh,
-                                                                    double* __restrict__ out) {
-  __shared__ double sred[ADJ_PJ][ADJ_THREADS / 64];
-  const long long blk = blockIdx.x;
-  int lo = 0, hi = S - 1;
-  while (lo < hi) {
-    const int mid = (lo + hi + 1) >> 1;
-    if (sol[mid].row0 <= blk) lo = mid;
-    else hi = mid - 1;
-  }
-  const BatchSolve d = sol[lo];
+                                                                    double* __restrict__ gi,
+                                                                    double* __restrict__ part) {
+  __shared__ double sacc[ADJ_PJ * kBatchMaxN];  // [term][j]
+  __shared__ double sred[2][ADJ_PJ][ADJ_THREADS / 64];
+  const BatchSolve d = sol[blk[2 * blockIdx.x]];
+  const int r0 = blk[2 * blockIdx.x + 1];
   const int n = d.n;
-  const int a = static_cast<int>(blk - d.row0);
+  const int rows = min(ADJ_ROWS, n - r0);
   const double* p = seg + d.seg0;
   DevSegments g;
   g.x = p;
@@ -757,16 +774,41 @@ __global__ void __launch_bounds__(ADJ_THREADS) batch_adjoint_kernel(const BatchS
   g.salp = p + 7 * n;
   const double* lam = lamall + 2 * d.row0;
   const double* abc = abcall + 6 * d.row0;
-  constexpr int K = ROWSIDE ? ADJ_GI : ADJ_PJ;
-  double acc[K];
-  for (int v = 0; v < K; ++v) acc[v] = 0.0;
-  for (int b = threadIdx.x; b < n; b += blockDim.x) {
-    const int i = ROWSIDE ? a : b, j = ROWSIDE ? b : a;
-    double oi[ADJ_GI], oj[ADJ_PJ];
-    adjoint_pair(g, n, i, j, rkh, lam[2 * i], lam[2 * i + 1], abc, oi, oj);
-    for (int v = 0; v < K; ++v) acc[v] += ROWSIDE ? oi[v] : oj[v];
+  // each thread owns the sacc entries of its own sources: no barrier needed
+  for (int j = threadIdx.x; j < n; j += ADJ_THREADS)
+    for (int v = 0; v < ADJ_PJ; ++v) sacc[v * kBatchMaxN + j] = 0.0;
+  for (int r = 0; r < rows; ++r) {
+    const int i = r0 + r;
+    const double lr = lam[2 * i], li = lam[2 * i + 1];
+    double acc[ADJ_GI] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int j = threadIdx.x; j < n; j += ADJ_THREADS) {
+      double oi[ADJ_GI], oj[ADJ_PJ];
+      adjoint_pair(g, n, i, j, rkh, lr, li, abc, oi, oj);
+      for (int v = 0; v < ADJ_GI; ++v) acc[v] += oi[v];
+      for (int v = 0; v < ADJ_PJ; ++v) sacc[v * kBatchMaxN + j] += oj[v];
+    }
+    // alternate reduction buffers: row r + 1 may write while row r is read
+    block_sum_out<ADJ_GI>(acc, sred[r & 1], gi + (size_t)(d.row0 + i) * ADJ_GI);
   }
-  block_sum_out<K>(acc, sred, out + (size_t)(d.row0 + a) * K);
+  double* o = part + (size_t)blockIdx.x * ADJ_PJ * maxn;
+  for (int j = threadIdx.x; j < n; j += ADJ_THREADS)
+    for (int v = 0; v < ADJ_PJ; ++v) o[(size_t)v * maxn + j] = sacc[v * kBatchMaxN + j];
+}
+
+// gj[row0 + j][v] = sum over the blocks fb[s] .. fb[s+1]-1 of solve s of
+// their part slices, in block order.  Grid: (solve, chunk of 13 n).
+__global__ void __launch_bounds__(256) batch_adjoint_reduce_kernel(const BatchSolve* __restrict__ sol,
+                                                                   const int* __restrict__ fb, int maxn,
+                                                                   const double* __restrict__ part,
+                                                                   double* __restrict__ gj) {
+  const int s = blockIdx.x;
+  const int n = sol[s].n;
+  const int e = blockIdx.y * blockDim.x + threadIdx.x;
+  if (e >= ADJ_PJ * n) return;
+  const int v = e / n, j = e - v * n;
+  double x = 0.0;
+  for (int b = fb[s]; b < fb[s + 1]; ++b) x += part[((size_t)b * ADJ_PJ + v) * maxn + j];
+  gj[(size_t)(sol[s].row0 + j) * ADJ_PJ + v] = x;
 }
 
 // gj[j][c] += sum over the rows of a chunk, in row order.
@@ -799,14 +841,17 @@ int launch_adjoint_fill(const DevSegments& g, int n, int i0, int rows, double rk
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-int launch_batch_adjoint_fill(const BatchSolve* sol, int S, long long rows_total, const double* seg,
-                              const double* lam, const double* abc, double rkh, double* gi, double* gj,
-                              hipStream_t stream) {
-  if (S <= 0 || rows_total <= 0) return 0;
-  launch_k(K_ADJOINT_FILL, batch_adjoint_kernel<true>, dim3((unsigned)rows_total), dim3(ADJ_THREADS), 0u, stream, sol,
-           S, seg, lam, abc, rkh, gi);
-  launch_k(K_ADJOINT_FILL, batch_adjoint_kernel<false>, dim3((unsigned)rows_total), dim3(ADJ_THREADS), 0u, stream,
-           sol, S, seg, lam, abc, rkh, gj);
+int batch_adjoint_rows() { return ADJ_ROWS; }
+
+int launch_batch_adjoint_fill(const BatchSolve* sol, int S, const int* blk, const int* fb, int nblk, int maxn,
+                              const double* seg, const double* lam, const double* abc, double rkh, double* gi,
+                              double* part, double* gj, hipStream_t stream) {
+  if (S <= 0 || nblk <= 0) return 0;
+  if (maxn > kBatchMaxN) return 1;
+  launch_k(K_ADJOINT_FILL, batch_adjoint_kernel, dim3((unsigned)nblk), dim3(ADJ_THREADS), 0u, stream, sol, blk, maxn,
+           seg, lam, abc, rkh, gi, part);
+  launch_k(K_ADJOINT_REDUCE, batch_adjoint_reduce_kernel, dim3((unsigned)S, (ADJ_PJ * maxn + 255) / 256), dim3(256),
+           0u, stream, sol, fb, maxn, part, gj);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

@@ -122,13 +122,17 @@ int launch_batch_pattern(const BatchSolve* sol, int S, const double* seg, const 
                          int n_dir, const double* dir, double* gain, int bt, hipStream_t stream);
 // Batched adjoint.  batch.hip: Z^T lam = g per solve with the batched
 // factors (g, lam: 2 per row).  fill.hip: the fill's adjoint contraction per
-// solve, gi (7 per row: the observation side) and gj (13 per row: the
-// source side and lam_i E_ij), both written (no accumulation).
+// solve over blocks of batch_adjoint_rows() observation rows (blk: solve and
+// first row of each of the nblk blocks; fb: S + 1 entries, the first block
+// of each solve), gi (7 per row: the observation side) and gj (13 per row:
+// the source side and lam_i E_ij), both written; part: device scratch of
+// nblk * 13 * maxn doubles.
 int launch_batch_solve_transposed(const BatchSolve* sol, int S, const double* Z, const int* rp, const double* g,
                                   double* lam, int bt, hipStream_t stream);
-int launch_batch_adjoint_fill(const BatchSolve* sol, int S, long long rows_total, const double* seg,
-                              const double* lam, const double* abc, double rkh, double* gi, double* gj,
-                              hipStream_t stream);
+int batch_adjoint_rows();
+int launch_batch_adjoint_fill(const BatchSolve* sol, int S, const int* blk, const int* fb, int nblk, int maxn,
+                              const double* seg, const double* lam, const double* abc, double rkh, double* gi,
+                              double* part, double* gj, hipStream_t stream);
 
 // fill.hip: fields of rows [i0, i0+rows) into E (rows*n*6 doubles), then
 // gather into Z (row-major complex, leading dimension ldz complex entries).
