@@ -80,9 +80,9 @@ const char* nec_error_message(void) { return g_err_message.c_str(); }
 
 long nec_medium_parameters(nec_context*, double, double) { return unsupported("MP"); }
 
-long nec_gn_card(nec_context*, int iperf, int, double, double, double, double, double, double) {
-  if (iperf == -1) return 0;  // GN -1: free space, which is the only ground here
-  return unsupported("GN");
+long nec_gn_card(nec_context* ctx, int iperf, int nradl, double epsr, double sig, double tmp3, double tmp4,
+                 double tmp5, double tmp6) {
+  return guarded([&] { ctx->gn_card(iperf, nradl, epsr, sig, tmp3, tmp4, tmp5, tmp6); });
 }
 
 long nec_fr_card(nec_context* ctx, int in_ifrq, int in_nfrq, double in_freq_mhz, double in_del_freq) {

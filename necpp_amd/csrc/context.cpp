@@ -2,6 +2,7 @@
 #include "context.h"
 
 #include <cmath>
+#include <complex>
 
 namespace necgpu {
 
@@ -19,6 +20,7 @@ void Context::calc_prepare() {
   state_ = MEMORY_ALLOC;
   rkh_ = 1.;
   use_ek_ = false;
+  gn_iperf_ = -1;  // ground.default_values(): free space
   ifar_ = -1;
   freq_mhz_ = em_speed_of_light() / 1.0e6;
   nfrq_ = 1;
@@ -84,6 +86,47 @@ void Context::kh_card(double rkh) {
   iflow_ = 1;
 }
 
+void Context::gn_card(int iperf, int nradl, double epsr, double sig, double t3, double t4, double t5, double t6) {
+  // nec_ground::parse_gn, src/nec_ground.cpp:75-121
+  (void)t3;
+  (void)t4;
+  (void)t5;
+  (void)t6;  // cliff parameters: used by RP modes 2-5 only, which are not supported
+  if (iperf == -1) {
+    gn_iperf_ = -1;
+  } else {
+    if (iperf == 2) throw NecError("GN: the Sommerfeld ground (type 2) is not supported by the GPU path");
+    if (iperf != 0 && iperf != 1) throw NecError("GN: ground type " + std::to_string(iperf) + " is not supported");
+    if (nradl != 0) throw NecError("GN: the radial-wire ground screen is not supported by the GPU path");
+    if (iperf == 0) {
+      if (epsr == 0.0) throw NecError("GROUND DIELECTRIC CONSTANT IS ZERO");
+      if (sig == 0.0) throw NecError("GROUND CONDUCTIVITY IS ZERO");
+    }
+    gn_iperf_ = iperf;
+    gn_epsr_ = epsr;
+    gn_sig_ = sig;
+  }
+  reset_to_structure_loading();
+}
+
+GroundSpec Context::ground_spec() {
+  // nec_ground::calculate_antenna_environment, src/nec_ground.cpp:176-197:
+  // frati = 1; the finite ground's zrati = 1 / sqrt(epsc), epsc = epsr -
+  // j sig lambda eta / (2 pi); a negative sig is converted once and kept
+  GroundSpec g;
+  if (gn_iperf_ == -1) return g;
+  g.type = (gn_iperf_ == 1) ? 1 : 2;
+  if (g.type == 2) {
+    const double eta2pi = em_impedance() / (2.0 * kPi);
+    if (gn_sig_ < 0.) gn_sig_ = -gn_sig_ / (eta2pi * wavelength_);
+    const std::complex<double> epsc(gn_epsr_, -gn_sig_ * wavelength_ * eta2pi);
+    const std::complex<double> zr = 1.0 / std::sqrt(epsc);
+    g.zr_re = zr.real();
+    g.zr_im = zr.imag();
+  }
+  return g;
+}
+
 void Context::ek_card(bool on) {
   use_ek_ = on;
   reset_to_structure_loading();
@@ -147,6 +190,7 @@ void Context::adjoint(const AdjointSpec& spec, AdjointResult* r, std::vector<dou
   if (inputs_.empty() || state_ < EXCITATION_LOOP) throw NecError("adjoint: no solve to differentiate (XQ or RP first)");
   if (src_seg_.size() != 1) throw NecError("adjoint: the objective needs exactly one voltage source");
   if (precision_ != 0) throw NecError("adjoint: needs the FP64 path (precision mode 0)");
+  if (ground_now_.type != 0) throw NecError("adjoint: free space only (no GN card)");
   const ScaledSegments seg = geo_.scaled(freq_mhz_);
   adjoint_gradient(geo_, seg, static_cast<int>(src_seg_[0] - 1), src_v_[0], wavelength_, rkh_, abc_, device(), spec,
                    r);
@@ -170,6 +214,8 @@ void Context::simulate(bool far_field_flag) {
     }
     if (igox <= STRUCTURE_LOADING) {
       device().set_precision(precision_);
+      ground_now_ = ground_spec();
+      device().set_ground(ground_now_);
       device().fill(rkh_);
       device().factor();
       times_.fill_ms = device().times().fill_ms;
@@ -240,6 +286,9 @@ PatternSpec Context::pattern_spec(std::vector<double>* trig_out) const {
   PatternSpec sp;
   sp.n_theta = nth_;
   sp.n_phi = nph_;
+  sp.ground = ground_now_.type;
+  sp.zr_re = ground_now_.zr_re;
+  sp.zr_im = ground_now_.zr_im;
   sp.output_format = rp_format_;
   sp.normalization = rp_norm_;
   sp.ipd = ipd_;
@@ -260,7 +309,7 @@ PatternSpec Context::pattern_spec(std::vector<double>* trig_out) const {
   sp.dphi_rad = deg_to_rad(dph_);
   sp.half_dtheta_rad = 0.5 * deg_to_rad(dth_);
   // the reference accumulates the angles by repeated addition
-  std::vector<double> trig(3 * (size_t)nth_ + 2 * (size_t)nph_);
+  std::vector<double> trig(3 * (size_t)nth_ + 2 * (size_t)nph_ + (sp.ground ? (size_t)nth_ : 0));
   double thet = thets_ - dth_;
   for (int k = 0; k < nth_; ++k) {
     thet += dth_;
@@ -268,6 +317,9 @@ PatternSpec Context::pattern_spec(std::vector<double>* trig_out) const {
     trig[k] = std::sin(tha);
     trig[nth_ + k] = std::cos(tha);
     trig[2 * nth_ + k] = tha;
+    // over a ground, directions below the horizon are skipped
+    // (src/nec_radiation_pattern.cpp:263-264)
+    if (sp.ground) trig[3 * (size_t)nth_ + 2 * (size_t)nph_ + k] = (thet > 90.01) ? 1.0 : 0.0;
   }
   double phi = phis_ - dph_;
   for (int k = 0; k < nph_; ++k) {
@@ -388,6 +440,10 @@ void Context::fill_matrix(double freq_mhz, std::vector<double>* Z) {
   if (!geo_.is_complete()) throw NecError("geometry is not complete");
   DeviceSolver& d = device();
   d.load(geo_, geo_.scaled(freq_mhz), use_ek_);
+  const double wl = wavelength_;
+  wavelength_ = em_speed_of_light() / (1.0e6 * freq_mhz);
+  d.set_ground(ground_spec());
+  wavelength_ = wl;
   d.fill(rkh_);
   d.download_matrix(Z);
   state_ = MEMORY_ALLOC;  // the device matrix no longer holds factors

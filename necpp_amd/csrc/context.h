@@ -52,6 +52,9 @@ class Context {
                double theta0, double phi0, double delta_theta, double delta_phi, double radial_distance,
                double gain_norm);
   void kh_card(double rkh);
+  // GN: iperf -1 free space, 0 finite ground (reflection coefficients), 1
+  // perfect ground; no radial-wire screen, no Sommerfeld ground
+  void gn_card(int iperf, int nradl, double epsr, double sig, double t3, double t4, double t5, double t6);
   void ek_card(bool on);
 
   // results
@@ -104,6 +107,11 @@ class Context {
   double freq_mhz_ = 0, delfrq_ = 0, wavelength_ = 0;
   double rkh_ = 1.0;
   bool use_ek_ = false;
+  int gn_iperf_ = -1;  // -1: free space
+  double gn_epsr_ = 0.0, gn_sig_ = 0.0;
+  // ground of the current frequency (nec_ground::calculate_antenna_environment)
+  GroundSpec ground_spec();
+  GroundSpec ground_now_;  // the ground of the last fill
   int precision_ = 0;
   // voltage sources (EX 0)
   std::vector<int64_t> src_seg_;

@@ -244,8 +244,8 @@ void DeviceSolver::enqueue_fill(double rkh) {
   const DevSegments g = dev_segments();
   for (int i0 = 0; i0 < n; i0 += rows) {
     const int r = std::min(rows, n - i0);
-    if (launch_fill(g, n, i0, r, rkh, use_ek_ ? 1 : 0, ekind_.as<int>(), E_.as<double>(), colptr_.as<int>(),
-                    colj_.as<int>(), colabc_.as<double>(), Z_.as<double>(), ld_, stream_))
+    if (launch_fill(g, n, i0, r, rkh, use_ek_ ? 1 : 0, ekind_.as<int>(), ground_, E_.as<double>(),
+                    colptr_.as<int>(), colj_.as<int>(), colabc_.as<double>(), Z_.as<double>(), ld_, stream_))
       hip_check(hipGetLastError(), "fill launch");
   }
 }
@@ -362,7 +362,7 @@ PatternOut DeviceSolver::pattern_buffers(int nd) {
 
 void DeviceSolver::enqueue_pattern(const PatternSpec& spec, const PatternOut& po) {
   const int nd = spec.n_theta * spec.n_phi;
-  pat_part_.ensure((size_t)pattern_slices(n_, nd) * nd * 6 * sizeof(double));
+  pat_part_.ensure((size_t)pattern_slices(n_, nd) * nd * (spec.ground ? 12 : 6) * sizeof(double));
   if (launch_pattern(dev_segments(), n_, abc_.as<double>(), spec, trig_.as<double>(), pat_part_.as<double>(), po,
                      stream_))
     hip_check(hipGetLastError(), "pattern launch");
@@ -419,6 +419,7 @@ void DeviceSolver::adjoint(double rkh, const std::vector<zc>& g, std::vector<zc>
   if (!factored_) throw NecError("adjoint requested before the matrix was factored");
   if (precision_ != 0) throw NecError("the adjoint needs the FP64 factors (precision mode 0)");
   if (use_ek_) throw NecError("the adjoint supports the thin-wire kernel only (no EK card)");
+  if (ground_.type != 0) throw NecError("the adjoint supports free space only (no GN card)");
   const int n = n_;
   if ((int)g.size() != n) throw NecError("adjoint: right-hand side has the wrong length");
   adj_g_.ensure((size_t)n * 16);

@@ -117,6 +117,9 @@ class DeviceSolver {
   void set_kernel_timing(bool on) { kernel_timing_ = on; }
   // 0: FP64 LU.  1: mixed (FP32 LU of a copy of Z + FP64 iterative
   // refinement against Z until |b - Z x| <= refine_tol |b|).
+  // ground of the next fills (the image term of efld)
+  void set_ground(const GroundSpec& g) { ground_ = g; }
+  const GroundSpec& ground() const { return ground_; }
   void set_precision(int mode) { precision_ = mode; }
   int precision() const { return precision_; }
   void set_refine_tol(double tol) { refine_tol_ = tol; }
@@ -139,6 +142,7 @@ class DeviceSolver {
   int n_ = 0;
   long long ld_ = 0;
   bool use_ek_ = false;
+  GroundSpec ground_;
   bool factored_ = false;
   int lu_info_ = 0;
   unsigned epoch_ = 0;

@@ -385,7 +385,8 @@ template <class T>
 __device__ void pair_fields_v(const T& xi_, const T& yi_, const T& zi_, const T& ai, double cabi, double sabi,
                               double salpi, const T& xj_, const T& yj_, const T& zj_, const T& sj, const T& bj,
                               const T& cabj, const T& sabj, const T& salpj, bool self, double rkh, int use_ek,
-                              int ind1, int ind2, cx<T>* etk, cx<T>* ets, cx<T>* etc, cx<double>* evec = nullptr) {
+                              int ind1, int ind2, cx<T>* etk, cx<T>* ets, cx<T>* etc, cx<double>* evec = nullptr,
+                              double* rout = nullptr) {
   const T xij = xi_ - xj_;
   const T yij = yi_ - yj_;
   const T zij = zi_ - zj_;
@@ -404,6 +405,7 @@ __device__ void pair_fields_v(const T& xi_, const T& yi_, const T& zi_, const T&
     rhoz = rhoz / rh;
   }
   const T r = sqrt(zp * zp + rh * rh);
+  if (rout) *rout = val(r);
   cx<T> tezk, terk, tezc, terc;
   const cx<T> zero = cmk<T>(T(0), T(0));
   cx<T> e3 = zero, e4 = zero, e5 = zero;
@@ -477,8 +479,74 @@ __device__ void pair_fields(const DevSegments& g, int i, int j, double rkh, int 
   *etc = zmk(c.re, c.im);
 }
 
+// Pair (i, j) over a ground, efld src/nec_context.cpp:2845-2956: the direct
+// fields e and the image fields t (efld_compute with zij = zi + zj, the
+// source direction's z cosine negated, never the self term), t weighted for
+// the finite ground by the reflection coefficients at the specular point,
+// then e - t * frati projected on the observation direction.
+__device__ void pair_fields_ground(const DevSegments& g, int i, int j, double rkh, int use_ek, int ind1, int ind2,
+                                   const GroundSpec& gs, zc* etk, zc* ets, zc* etc) {
+  cx<double> a, b, c, e[9], t[9];
+  pair_fields_v<double>(g.x[i], g.y[i], g.z[i], g.b[i], g.cab[i], g.sab[i], g.salp[i], g.x[j], g.y[j], g.z[j],
+                        g.s[j], g.b[j], g.cab[j], g.sab[j], g.salp[j], i == j, rkh, use_ek, ind1, ind2, &a, &b, &c, e);
+  double r;
+  // zi - (-zj) == zi + zj exactly
+  pair_fields_v<double>(g.x[i], g.y[i], g.z[i], g.b[i], g.cab[i], g.sab[i], g.salp[i], g.x[j], g.y[j], -g.z[j],
+                        g.s[j], g.b[j], g.cab[j], g.sab[j], -g.salp[j], false, rkh, use_ek, ind1, ind2, &a, &b, &c, t,
+                        &r);
+  if (gs.type == 2) {
+    const zc zratx = zmk(gs.zr_re, gs.zr_im);
+    const double xij = g.x[i] - g.x[j], yij = g.y[i] - g.y[j], zij = g.z[i] + g.z[j];
+    const double rmag = r;
+    const double xymag = sqrt(xij * xij + yij * yij);
+    double px, py, cth;
+    zc zrsin;
+    if (xymag <= 1.0e-6) {
+      px = 0.;
+      py = 0.;
+      cth = 1.;
+      zrsin = zmk(1.0, 0.0);
+    } else {
+      px = -yij / xymag;
+      py = xij / xymag;
+      cth = zij / rmag;
+      // 1 - zratx^2 (1 - cth^2)
+      const zc q = zmulr(zmul(zratx, zratx), 1.0 - cth * cth);
+      zrsin = zsqrt(zmk(1.0 - q.re, -q.im));
+    }
+    const zc zz = zmul(zratx, zrsin);
+    const zc refs = zdiv(zmk(cth - zz.re, -zz.im), zmk(cth + zz.re, zz.im));
+    const zc zc1 = zmulr(zratx, cth);
+    zc refps = zdiv(zneg(zsub(zc1, zrsin)), zadd(zc1, zrsin));
+    refps = zsub(refps, refs);
+    for (int q = 0; q < 9; q += 3) {
+      const zc tx = zmk(t[q].re, t[q].im), ty = zmk(t[q + 1].re, t[q + 1].im), tz = zmk(t[q + 2].re, t[q + 2].im);
+      zc epy = zadd(zmulr(tx, px), zmulr(ty, py));
+      const zc epx = zmulr(epy, px);
+      epy = zmulr(epy, py);
+      const zc nx = zadd(zmul(refs, tx), zmul(refps, epx));
+      const zc ny = zadd(zmul(refs, ty), zmul(refps, epy));
+      const zc nz = zmul(refs, tz);
+      t[q] = cmk<double>(nx.re, nx.im);
+      t[q + 1] = cmk<double>(ny.re, ny.im);
+      t[q + 2] = cmk<double>(nz.re, nz.im);
+    }
+  }
+  const zc fr = zmk(gs.fr_re, gs.fr_im);
+  zc d[9];
+  for (int q = 0; q < 9; ++q) d[q] = zsub(zmk(e[q].re, e[q].im), zmul(zmk(t[q].re, t[q].im), fr));
+  const double ci = g.cab[i], si = g.sab[i], li = g.salp[i];
+  *etk = zadd(zadd(zmulr(d[0], ci), zmulr(d[1], si)), zmulr(d[2], li));
+  *ets = zadd(zadd(zmulr(d[3], ci), zmulr(d[4], si)), zmulr(d[5], li));
+  *etc = zadd(zadd(zmulr(d[6], ci), zmulr(d[7], si)), zmulr(d[8], li));
+}
+
+// GROUND: the image term of every pair (a separate instantiation keeps the
+// free-space kernel's register use as it is).
+template <bool GROUND>
 __global__ void __launch_bounds__(256) fill_fields_kernel(DevSegments g, int n, int i0, int rows, double rkh,
-                                                          int use_ek, const int* ekind, double* __restrict__ E) {
+                                                          int use_ek, const int* ekind, GroundSpec gs,
+                                                          double* __restrict__ E) {
   const int j = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;
   if (j >= n || r >= rows) return;
@@ -489,7 +557,8 @@ __global__ void __launch_bounds__(256) fill_fields_kernel(DevSegments g, int n, 
     ind2 = ekind[2 * j + 1];
   }
   zc etk, ets, etc;
-  pair_fields(g, i, j, rkh, use_ek, ind1, ind2, &etk, &ets, &etc);
+  if (GROUND) pair_fields_ground(g, i, j, rkh, use_ek, ind1, ind2, gs, &etk, &ets, &etc);
+  else pair_fields(g, i, j, rkh, use_ek, ind1, ind2, &etk, &ets, &etc);
   double* o = E + ((size_t)r * n + j) * 6;
   o[0] = etk.re;
   o[1] = etk.im;
@@ -867,13 +936,18 @@ int launch_batch_fill(const BatchSolve* sol, const int* blk, int nblk, int R, in
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
-int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind, double* E,
-                const int* colptr, const int* colj, const double* colabc, double* Z, long long ldz,
-                hipStream_t stream) {
+int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind,
+                const GroundSpec& ground, double* E, const int* colptr, const int* colj, const double* colabc,
+                double* Z, long long ldz, hipStream_t stream) {
   if (n <= 0 || rows <= 0) return 0;
   dim3 block(256);
   dim3 grid((n + 255) / 256, rows);
-  launch_k(K_FILL_FIELDS, fill_fields_kernel, grid, block, 0, stream, g, n, i0, rows, rkh, use_ek, ekind, E);
+  if (ground.type != 0)
+    launch_k(K_FILL_FIELDS, fill_fields_kernel<true>, grid, block, 0, stream, g, n, i0, rows, rkh, use_ek, ekind,
+             ground, E);
+  else
+    launch_k(K_FILL_FIELDS, fill_fields_kernel<false>, grid, block, 0, stream, g, n, i0, rows, rkh, use_ek, ekind,
+             ground, E);
   launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel, grid, block, 0, stream, E, n, i0, rows, colptr, colj, colabc, Z, ldz);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }

@@ -76,6 +76,15 @@ inline void launch_k(int kind, F kernel, dim3 grid, dim3 block, uint32_t shmem, 
     hipLaunchKernelGGL(kernel, grid, block, shmem, stream, args...);
 }
 
+// Ground of the fill and the pattern (nec_ground): type 0 free space, 1
+// perfect ground, 2 finite ground by reflection coefficients; zr = zrati
+// (1 / sqrt of the complex relative permittivity), fr = frati (1 for both).
+struct GroundSpec {
+  int type = 0;
+  double zr_re = 0.0, zr_im = 0.0;
+  double fr_re = 1.0, fr_im = 0.0;
+};
+
 // Segment tables on the device, in wavelengths (structure of arrays).
 struct DevSegments {
   const double *x, *y, *z, *s, *b, *cab, *sab, *salp;
@@ -136,9 +145,10 @@ int launch_batch_adjoint_fill(const BatchSolve* sol, int S, const int* blk, cons
 
 // fill.hip: fields of rows [i0, i0+rows) into E (rows*n*6 doubles), then
 // gather into Z (row-major complex, leading dimension ldz complex entries).
-int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind, double* E,
-                const int* colptr, const int* colj, const double* colabc, double* Z, long long ldz,
-                hipStream_t stream);
+// With a ground each pair adds the image term of efld.
+int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind,
+                const GroundSpec& ground, double* E, const int* colptr, const int* colj, const double* colabc,
+                double* Z, long long ldz, hipStream_t stream);
 
 // lu.hip: in-place partial-pivot LU of the row-major complex n x n matrix A
 // (leading dimension lda complex).  piv (2n ints): piv[k] = row exchanged
@@ -205,6 +215,11 @@ int launch_zin(int nsrc, const int* seg, const double* volt, const double* curx,
 // pattern.hip
 struct PatternSpec {
   int n_theta, n_phi;
+  // ground: 0 free space, 1 perfect, 2 finite (zrati zr); with a ground the
+  // structure image is added and directions flagged below the horizon in
+  // trig (theta > 90.01 deg) read as no field
+  int ground;
+  double zr_re, zr_im;
   int output_format, normalization, ipd, iavp;
   double wavelength, gcon, gcop, exrm, exra, rfld;
   double dphi_rad, half_dtheta_rad;
@@ -214,10 +229,12 @@ struct PatternOut {
   int* sense;
   double *e_theta, *e_phi, *pint;  // e_*: 2 doubles per direction
 };
-// Directions d = kph * n_theta + kth; trig holds per-theta (sin, cos) then
-// per-phi (sin, cos) of the reference's accumulated angles, and the angles
-// themselves (radians) for theta.
-// Number of segment slices launch_pattern uses (scratch = slices*n_dir*6 doubles).
+// Directions d = kph * n_theta + kth.  trig (of the reference's accumulated
+// angles): sin theta, cos theta, theta (radians) (n_theta each), sin phi,
+// cos phi (n_phi each), then, with a ground, a below-horizon flag per theta
+// (1.0 where theta > 90.01 deg, else 0.0).
+// Number of segment slices launch_pattern uses (scratch = slices*n_dir*6
+// doubles, 12 with a ground).
 int pattern_slices(int n, int n_dir);
 int launch_pattern(const DevSegments& g, int n, const double* abc, const PatternSpec& spec, const double* trig,
                    double* eth_eph_scratch, const PatternOut& out, hipStream_t stream);

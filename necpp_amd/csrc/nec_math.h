@@ -58,6 +58,28 @@ constexpr double kEuler = .577215664;  // src/c_geometry.cpp:2810
 constexpr double kNearConst = 4.771341189;  // src/nec_context.cpp:3055
 
 // em::constants (src/electromag.cpp:22-47).
+// Principal complex square root (the branch libm's csqrt takes: cut on the
+// negative real axis, Im of the result with the sign of Im z).  For Im z != 0
+// it uses 2 Re(w) Im(w) = Im(z) for the smaller part, so neither part loses
+// digits to the cancellation in |z| -/+ Re z.  No rescaling: arguments here
+// are of order one (ground impedance ratios).
+NEC_HD zc zsqrt(zc z) {
+  if (z.im == 0.0) {
+    if (z.re < 0.0) return zmk(0.0, copysign(sqrt(-z.re), z.im));
+    return zmk(fabs(sqrt(z.re)), copysign(0.0, z.im));
+  }
+  const double d = hypot(z.re, z.im);
+  double r, s;
+  if (z.re > 0.0) {
+    r = sqrt(0.5 * (d + z.re));
+    s = 0.5 * (z.im / r);
+  } else {
+    s = sqrt(0.5 * (d - z.re));
+    r = fabs(0.5 * (z.im / s));
+  }
+  return zmk(r, copysign(s, z.im));
+}
+
 NEC_HD double em_permittivity() { return 8.854e-12; }
 NEC_HD double em_permeability() { return kFourPi * 1.0e-7; }
 NEC_HD double em_speed_of_light() { return 1.0 / sqrt(em_permeability() * em_permittivity()); }

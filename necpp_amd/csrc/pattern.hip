@@ -1,4 +1,4 @@
-// pattern.hip -- far-field radiation pattern (free space).
+// pattern.hip -- far-field radiation pattern (free space or over a ground).
 //
 // nec_context::ffld (src/nec_context.cpp:6639-6887) for every direction of
 // an RP card, then the per-direction part of
@@ -8,9 +8,12 @@
 //
 // pattern_partial  grid (direction blocks, segment slices); every thread sums
 //                  the segment terms of its slice for one direction, with
-//                  the slice's segment data staged through LDS.
+//                  the slice's segment data staged through LDS; over a
+//                  ground also the sum of the structure image (roz -> -roz).
 // pattern_finish   one thread per direction: adds the slice partials in
-//                  slice order, forms E_theta / E_phi and the gains.
+//                  slice order, adds the image with the reflection
+//                  coefficients (ffld :6666-6700, :6817-6822), forms
+//                  E_theta / E_phi and the gains.
 #include "kernels.h"
 #include "nec_math.h"
 
@@ -21,6 +24,31 @@ namespace {
 constexpr int SEG_CHUNK = 64;
 constexpr int SEG_FIELDS = 13;
 
+// The segment term of ffld (src/nec_context.cpp:6706-6742) for one segment
+// (staged data sg) and direction cosines (rox, roy, roz).
+__device__ __forceinline__ zc ffld_term(const double* sg, double rox, double roy, double roz) {
+  const double cab = sg[0], sab = sg[1], salp = sg[2], el = sg[3];
+  const double omega = -(rox * cab + roy * sab + roz * salp);
+  const double sill = omega * el;
+  const double top = el + sill;
+  const double bot = el - sill;
+  // 2 sin(sill) / omega = 2 el sinc(sill); sin(top) / top; sin(bot) / bot
+  const double a = 2. * el * sinc(sill);
+  const double too = sinc(top);
+  const double boo = sinc(bot);
+  const double b = el * (boo - too);
+  const double c = el * (boo + too);
+  const double rr = a * sg[7] + b * sg[10] + c * sg[11];
+  const double ri = a * sg[8] - b * sg[9] + c * sg[12];
+  const double arg = kTwoPi * (sg[4] * rox + sg[5] * roy + sg[6] * roz);
+  double sa, ca;
+  sincos(arg, &sa, &ca);
+  return zmul(zmk(ca, sa), zmk(rr, ri));
+}
+
+// GROUND: also the image sum (roz -> -roz); part holds 6 doubles per
+// (slice, direction), 12 with the image (direct then image).
+template <bool GROUND>
 __global__ void __launch_bounds__(256) pattern_partial_kernel(DevSegments g, int n, const double* __restrict__ abc,
                                                               int n_theta, int n_dir, const double* __restrict__ trig,
                                                               int seg_per_slice, double* __restrict__ part) {
@@ -43,6 +71,9 @@ __global__ void __launch_bounds__(256) pattern_partial_kernel(DevSegments g, int
     roy = thz * phx;
   }
   double cxr = 0, cxi = 0, cyr = 0, cyi = 0, czr = 0, czi = 0;
+  double ixr = 0, ixi = 0, iyr = 0, iyi = 0, izr = 0, izi = 0;
+  bool below = false;
+  if (GROUND && d < n_dir) below = trig[3 * n_theta + 2 * n_phi + d % n_theta] != 0.0;
   for (int c0 = s0; c0 < s1; c0 += SEG_CHUNK) {
     const int cnt = min(SEG_CHUNK, s1 - c0);
     __syncthreads();
@@ -63,43 +94,45 @@ __global__ void __launch_bounds__(256) pattern_partial_kernel(DevSegments g, int
       sseg[q * SEG_FIELDS + f] = v;
     }
     __syncthreads();
-    if (d < n_dir) {
+    if (d < n_dir && !below) {
       for (int q = 0; q < cnt; ++q) {
         const double* sg = sseg + q * SEG_FIELDS;
-        const double cab = sg[0], sab = sg[1], salp = sg[2], el = sg[3];
-        const double omega = -(rox * cab + roy * sab + roz * salp);
-        const double sill = omega * el;
-        const double top = el + sill;
-        const double bot = el - sill;
-        // 2 sin(sill) / omega = 2 el sinc(sill); sin(top) / top; sin(bot) / bot
-        const double a = 2. * el * sinc(sill);
-        const double too = sinc(top);
-        const double boo = sinc(bot);
-        const double b = el * (boo - too);
-        const double c = el * (boo + too);
-        const double rr = a * sg[7] + b * sg[10] + c * sg[11];
-        const double ri = a * sg[8] - b * sg[9] + c * sg[12];
-        const double arg = kTwoPi * (sg[4] * rox + sg[5] * roy + sg[6] * roz);
-        double sa, ca;
-        sincos(arg, &sa, &ca);
-        const zc exa = zmul(zmk(ca, sa), zmk(rr, ri));
+        const double cab = sg[0], sab = sg[1], salp = sg[2];
+        const zc exa = ffld_term(sg, rox, roy, roz);
         cxr = cxr + exa.re * cab;
         cxi = cxi + exa.im * cab;
         cyr = cyr + exa.re * sab;
         cyi = cyi + exa.im * sab;
         czr = czr + exa.re * salp;
         czi = czi + exa.im * salp;
+        if (GROUND) {
+          const zc exi = ffld_term(sg, rox, roy, -roz);
+          ixr = ixr + exi.re * cab;
+          ixi = ixi + exi.im * cab;
+          iyr = iyr + exi.re * sab;
+          iyi = iyi + exi.im * sab;
+          izr = izr + exi.re * salp;
+          izi = izi + exi.im * salp;
+        }
       }
     }
   }
   if (d < n_dir) {
-    double* o = part + ((size_t)slice * n_dir + d) * 6;
+    double* o = part + ((size_t)slice * n_dir + d) * (GROUND ? 12 : 6);
     o[0] = cxr;
     o[1] = cxi;
     o[2] = cyr;
     o[3] = cyi;
     o[4] = czr;
     o[5] = czi;
+    if (GROUND) {
+      o[6] = ixr;
+      o[7] = ixi;
+      o[8] = iyr;
+      o[9] = iyi;
+      o[10] = izr;
+      o[11] = izi;
+    }
   }
 }
 
@@ -110,17 +143,66 @@ __global__ void __launch_bounds__(256) pattern_finish_kernel(int n_dir, int slic
   if (d >= n_dir) return;
   const int n_theta = sp.n_theta, n_phi = sp.n_phi;
   const int kth = d % n_theta, kph = d / n_theta;
+  if (sp.ground != 0 && trig[3 * n_theta + 2 * n_phi + kth] != 0.0) {
+    // below the horizon over a ground: the reference skips the direction
+    // (nec_radiation_pattern.cpp:263-264) and leaves its entries unset;
+    // here they read as no field
+    out.gain[d] = -999.99;
+    out.pg_vert[d] = -999.99;
+    out.pg_horiz[d] = -999.99;
+    out.pg_tot[d] = -999.99;
+    out.pg_rhcp[d] = -999.99;
+    out.pg_lhcp[d] = -999.99;
+    out.e_theta[2 * d] = 0.0;
+    out.e_theta[2 * d + 1] = 0.0;
+    out.e_phi[2 * d] = 0.0;
+    out.e_phi[2 * d + 1] = 0.0;
+    out.axial[d] = 0.0;
+    out.tilt[d] = 0.0;
+    out.sense[d] = 3;
+    out.avg_scale[d] = 0.0;
+    if (sp.iavp != 0) out.pint[d] = 0.0;
+    return;
+  }
+  const int pw = sp.ground ? 12 : 6;
   zc cix = zmk(0, 0), ciy = zmk(0, 0), ciz = zmk(0, 0);
+  zc iix = zmk(0, 0), iiy = zmk(0, 0), iiz = zmk(0, 0);
   for (int s = 0; s < slices; ++s) {
-    const double* p = part + ((size_t)s * n_dir + d) * 6;
+    const double* p = part + ((size_t)s * n_dir + d) * pw;
     cix = zadd(cix, zmk(p[0], p[1]));
     ciy = zadd(ciy, zmk(p[2], p[3]));
     ciz = zadd(ciz, zmk(p[4], p[5]));
+    if (sp.ground) {
+      iix = zadd(iix, zmk(p[6], p[7]));
+      iiy = zadd(iiy, zmk(p[8], p[9]));
+      iiz = zadd(iiz, zmk(p[10], p[11]));
+    }
   }
   const double sth = trig[kth], cth = trig[n_theta + kth], tha = trig[2 * n_theta + kth];
   const double sph = trig[3 * n_theta + kph], cph = trig[3 * n_theta + n_phi + kph];
   const double phx = -sph, phy = cph, roz = cth;
   const double thx = roz * phy, thy = -roz * phx, thz = -sth;
+  if (sp.ground) {
+    // reflection coefficients (ffld :6666-6678), then the image combination
+    // (:6817-6822)
+    zc rrv, rrh;
+    if (sp.ground == 1) {
+      rrv = zmk(-1.0, -0.0);
+      rrh = zmk(-1.0, -0.0);
+    } else {
+      const zc zr = zmk(sp.zr_re, sp.zr_im);
+      const zc q = zmulr(zmulr(zmul(zr, zr), thz), thz);
+      const zc zrsin = zsqrt(zmk(1. - q.re, -q.im));
+      const zc zz = zmul(zr, zrsin);
+      rrv = zdiv(zneg(zmk(roz - zz.re, -zz.im)), zmk(zz.re + roz, zz.im));
+      const zc zro = zmulr(zr, roz);
+      rrh = zdiv(zsub(zro, zrsin), zadd(zro, zrsin));
+    }
+    const zc cdp = zmul(zadd(zmulr(iix, phx), zmulr(iiy, phy)), zsub(rrh, rrv));
+    cix = zadd(zadd(cix, zmul(iix, rrv)), zmulr(cdp, phx));
+    ciy = zadd(zadd(ciy, zmul(iiy, rrv)), zmulr(cdp, phy));
+    ciz = zsub(ciz, zmul(iiz, rrv));
+  }
   const zc const3 = zmk(0.0, -em_impedance() / kFourPi);
   const zc eth = zmul(zadd(zadd(zmulr(cix, thx), zmulr(ciy, thy)), zmulr(ciz, thz)), const3);
   const zc eph = zmul(zadd(zmulr(cix, phx), zmulr(ciy, phy)), const3);
@@ -229,8 +311,12 @@ int launch_pattern(const DevSegments& g, int n, const double* abc, const Pattern
   const int dir_blocks = (n_dir + 255) / 256;
   const int slices = pattern_slices(n, n_dir);
   const int per = (n + slices - 1) / slices;
-  launch_k(K_PATTERN_PARTIAL, pattern_partial_kernel, dim3(dir_blocks, slices), dim3(256), 0, stream, g, n, abc, spec.n_theta,
-                     n_dir, trig, per, part);
+  if (spec.ground)
+    launch_k(K_PATTERN_PARTIAL, pattern_partial_kernel<true>, dim3(dir_blocks, slices), dim3(256), 0, stream, g, n,
+             abc, spec.n_theta, n_dir, trig, per, part);
+  else
+    launch_k(K_PATTERN_PARTIAL, pattern_partial_kernel<false>, dim3(dir_blocks, slices), dim3(256), 0, stream, g, n,
+             abc, spec.n_theta, n_dir, trig, per, part);
   launch_k(K_PATTERN_FINISH, pattern_finish_kernel, dim3(dir_blocks), dim3(256), 0, stream, n_dir, slices, part, trig, spec,
                      out);
   return hipGetLastError() == hipSuccess ? 0 : 1;

@@ -12,6 +12,7 @@
  */
 #include "necpp_oracle.h"
 
+#include <complex.h>
 #include <math.h>
 #include <stdarg.h>
 #include <stdio.h>
@@ -96,6 +97,12 @@ struct ora_geom {
   double *xu, *yu, *zu, *su, *bu;   /* unscaled copies */
   double *xs, *ys, *zs, *ss, *bs;   /* frequency-scaled */
   int complete;
+  /* ground (nec_ground): 0 free space, 1 perfect (GN 1), 2 finite ground by
+   * reflection coefficients (GN 0); zrati and frati are set by
+   * ora_frequency_scale */
+  int gtype;
+  double epsr, sig;
+  double zr_re, zr_im, fr_re, fr_im;
 };
 
 static void* xrealloc(void* p, size_t n) {
@@ -351,6 +358,35 @@ void ora_frequency_scale(ora_geom* g, double freq_mhz) {
     g->xs[i] = g->xu[i] * fr; g->ys[i] = g->yu[i] * fr; g->zs[i] = g->zu[i] * fr;
     g->ss[i] = g->su[i] * fr; g->bs[i] = g->bu[i] * fr;
   }
+  /* nec_ground::calculate_antenna_environment, src/nec_ground.cpp:176-197:
+   * frati = 1; for the finite ground zrati = 1 / sqrt(epsc),
+   * epsc = epsr - j sig lambda eta / (2 pi); a negative sig is replaced by
+   * -sig / (eta lambda / (2 pi)) and kept. */
+  g->fr_re = 1.0; g->fr_im = 0.0;
+  if (g->gtype == 2) {
+    double wavelength = ora_speed_of_light() / (freq_mhz * 1.0e6);
+    double eta2pi = ora_impedance() / TWO_PI;
+    if (g->sig < 0.) g->sig = -g->sig / (eta2pi * wavelength);
+    double complex epsc = CMPLX(g->epsr, -g->sig * wavelength * eta2pi);
+    double complex zr = 1.0 / csqrt(epsc);
+    g->zr_re = creal(zr); g->zr_im = cimag(zr);
+  }
+}
+
+/* nec_ground::parse_gn, src/nec_ground.cpp:75-121, for iperf -1 (free
+ * space), 0 (finite ground, reflection-coefficient approximation) and 1
+ * (perfect ground), no radial wires.  0 = ok. */
+int ora_set_ground(ora_geom* g, int iperf, double epsr, double sig) {
+  if (iperf == -1) { g->gtype = 0; return 0; }
+  if (iperf == 1) { g->gtype = 1; g->epsr = epsr; g->sig = sig; return 0; }
+  if (iperf == 0) {
+    if (epsr == 0.0) { set_err("GROUND DIELECTRIC CONSTANT IS ZERO"); return 1; }
+    if (sig == 0.0) { set_err("GROUND CONDUCTIVITY IS ZERO"); return 1; }
+    g->gtype = 2; g->epsr = epsr; g->sig = sig;
+    return 0;
+  }
+  set_err("ground type %d is not restated", iperf);
+  return 1;
 }
 
 /* ------------------------------------------------------------------ */
@@ -1080,6 +1116,45 @@ double ora_efld(double xj, double yj, double zj, double cabj, double sabj, doubl
 /* cmset + cmww, src/nec_context.cpp:2077-2203 and :2514-2654, transposed
  * fill (itrp = 1): Z[i][k] += etk*ax + ets*bx + etc*cx for each basis k in
  * trio(j), sources j in ascending order. */
+static double complex to_c(cx a) { return CMPLX(a.re, a.im); }
+static cx from_c(double complex a) { return mk(creal(a), cimag(a)); }
+
+/* The reflected field of efld, src/nec_context.cpp:2880-2956: t holds the
+ * image fields (efld_compute with zij = zi + zj and -salpj); for the finite
+ * ground they are weighted by the reflection coefficients of the specular
+ * point (no radial-wire screen), then subtracted from e times frati. */
+static void ground_image(const ora_geom* g, double xij, double yij, double zij, double r, cx t[9], cx e[9]) {
+  if (g->gtype == 2) {
+    double complex zratx = CMPLX(g->zr_re, g->zr_im);
+    double rmag = r;
+    double xymag = sqrt(xij * xij + yij * yij);
+    double px, py, cth;
+    double complex zrsin;
+    if (xymag <= 1.0e-6) {
+      px = 0.; py = 0.; cth = 1.; zrsin = CMPLX(1.0, 0.0);
+    } else {
+      px = -yij / xymag;
+      py = xij / xymag;
+      cth = zij / rmag;
+      zrsin = csqrt(1.0 - zratx * zratx * (1.0 - cth * cth));
+    }
+    double complex refs = (cth - zratx * zrsin) / (cth + zratx * zrsin);
+    double complex refps = -(zratx * cth - zrsin) / (zratx * cth + zrsin);
+    refps = refps - refs;
+    for (int q = 0; q < 9; q += 3) {
+      double complex tx = to_c(t[q]), ty = to_c(t[q + 1]), tz = to_c(t[q + 2]);
+      double complex epy = px * tx + py * ty;
+      double complex epx = px * epy;
+      epy = py * epy;
+      t[q] = from_c(refs * tx + refps * epx);
+      t[q + 1] = from_c(refs * ty + refps * epy);
+      t[q + 2] = from_c(refs * tz);
+    }
+  }
+  cx fr = mk(g->fr_re, g->fr_im);
+  for (int q = 0; q < 9; q++) e[q] = csub(e[q], cmul(t[q], fr));
+}
+
 int ora_fill(const ora_geom* g, double rkh, int use_ek, double* Z, int nthreads, ora_fill_counts* counts) {
   int64_t n = g->n;
   memset(Z, 0, (size_t)n * (size_t)n * 2 * sizeof(double));
@@ -1120,6 +1195,12 @@ int ora_fill(const ora_geom* g, double rkh, int use_ek, double* Z, int nthreads,
           cx e[9];
           local.pairs++;
           efld_compute(&st, xi - st.xj, yi - st.yj, ai, st.salpj, zi - st.zj, i != j, e);
+          if (g->gtype != 0) {
+            cx t[9];
+            double zij = zi + st.zj;
+            double r = efld_compute(&st, xi - st.xj, yi - st.yj, ai, -st.salpj, zij, 1, t);
+            ground_image(g, xi - st.xj, yi - st.yj, zij, r, t, e);
+          }
           cx etk = cadd(cadd(cmulr(e[0], cabi), cmulr(e[1], sabi)), cmulr(e[2], salpi));
           cx ets = cadd(cadd(cmulr(e[3], cabi), cmulr(e[4], sabi)), cmulr(e[5], salpi));
           cx etc = cadd(cadd(cmulr(e[6], cabi), cmulr(e[7], sabi)), cmulr(e[8], salpi));
@@ -1268,21 +1349,13 @@ int ora_current_coefficients(const ora_geom* g, double* cur, double* abc) {
 /* Far field                                                           */
 /* ------------------------------------------------------------------ */
 
-/* nec_context::ffld, src/nec_context.cpp:6639-6887, free space, no patches. */
-void ora_ffld(const ora_geom* g, const double* abc, double sth, double cth, double sph, double cph,
-              double* eth, double* eph) {
+/* The segment sum of ffld, src/nec_context.cpp:6706-6742, for the
+ * direction cosines (rox, roy, roz). */
+static void ffld_sum(const ora_geom* g, const double* abc, double rox, double roy, double roz, cx* cix, cx* ciy,
+                     cx* ciz) {
   int64_t n = g->n;
   const double *air = abc, *aii = abc + n, *bir = abc + 2 * n, *bii = abc + 3 * n, *cir = abc + 4 * n, *cii = abc + 5 * n;
-  cx const3 = mk(0.0, -ora_impedance() / FOUR_PI);
-  double phx = -sph;
-  double phy = cph;
-  double roz = cth;
-  double thx = roz * phy;
-  double thy = -roz * phx;
-  double thz = -sth;
-  double rox = -thz * phy;
-  double roy = thz * phx;
-  cx cix = mk(0, 0), ciy = mk(0, 0), ciz = mk(0, 0);
+  *cix = mk(0, 0); *ciy = mk(0, 0); *ciz = mk(0, 0);
   for (int64_t i = 0; i < n; i++) {
     double omega = -(rox * g->cab[i] + roy * g->sab[i] + roz * g->salp[i]);
     double el = PI_T * g->ss[i];
@@ -1302,9 +1375,48 @@ void ora_ffld(const ora_geom* g, const double* abc, double sth, double cth, doub
     double ri = a * aii[i] - b * bir[i] + c * cii[i];
     double arg = TWO_PI * (g->xs[i] * rox + g->ys[i] * roy + g->zs[i] * roz);
     cx exa = cmul(mk(cos(arg), sin(arg)), mk(rr, ri));
-    cix = cadd(cix, cmulr(exa, g->cab[i]));
-    ciy = cadd(ciy, cmulr(exa, g->sab[i]));
-    ciz = cadd(ciz, cmulr(exa, g->salp[i]));
+    *cix = cadd(*cix, cmulr(exa, g->cab[i]));
+    *ciy = cadd(*ciy, cmulr(exa, g->sab[i]));
+    *ciz = cadd(*ciz, cmulr(exa, g->salp[i]));
+  }
+}
+
+/* nec_context::ffld, src/nec_context.cpp:6639-6887, no patches; with a
+ * ground (ifar 0 or 1) the structure image (roz -> -roz) is added with the
+ * reflection coefficients rrv, rrh (-1 for the perfect ground,
+ * :6666-6700, :6817-6822). */
+void ora_ffld(const ora_geom* g, const double* abc, double sth, double cth, double sph, double cph,
+              double* eth, double* eph) {
+  cx const3 = mk(0.0, -ora_impedance() / FOUR_PI);
+  double phx = -sph;
+  double phy = cph;
+  double roz = cth;
+  double thx = roz * phy;
+  double thy = -roz * phx;
+  double thz = -sth;
+  double rox = -thz * phy;
+  double roy = thz * phx;
+  cx cix, ciy, ciz;
+  ffld_sum(g, abc, rox, roy, roz, &cix, &ciy, &ciz);
+  if (g->gtype != 0) {
+    double complex rrv, rrh;
+    if (g->gtype == 1) {
+      rrv = -CMPLX(1.0, 0.0);
+      rrh = -CMPLX(1.0, 0.0);
+    } else {
+      double complex zr = CMPLX(g->zr_re, g->zr_im);
+      double complex zrsin = csqrt(1. - zr * zr * thz * thz);
+      rrv = -(roz - zr * zrsin) / (roz + zr * zrsin);
+      rrh = (zr * roz - zrsin) / (zr * roz + zrsin);
+    }
+    double complex ccx = to_c(cix), ccy = to_c(ciy), ccz = to_c(ciz);
+    cx ix, iy, iz;
+    ffld_sum(g, abc, rox, roy, -roz, &ix, &iy, &iz);
+    double complex ex = to_c(ix), ey = to_c(iy), ez = to_c(iz);
+    double complex cdp = (ex * phx + ey * phy) * (rrh - rrv);
+    cix = from_c(ccx + ex * rrv + cdp * phx);
+    ciy = from_c(ccy + ey * rrv + cdp * phy);
+    ciz = from_c(ccz - ez * rrv);
   }
   cx et = cmul(cadd(cadd(cmulr(cix, thx), cmulr(ciy, thy)), cmulr(ciz, thz)), const3);
   cx ep = cmul(cadd(cmulr(cix, phx), cmulr(ciy, phy)), const3);
@@ -1351,6 +1463,23 @@ int ora_pattern(const ora_geom* g, const double* abc, double wavelength, double 
     double pha = deg_to_rad(phv[kph]);
     double th = thv[kth];
     double tha = deg_to_rad(th);
+    if (g->gtype != 0 && th > 90.01) {
+      /* below the horizon over a ground the reference skips the direction
+       * (nec_radiation_pattern.cpp:263-264) and its arrays keep unset
+       * values; here they read as no field: -999.99 dB, zero E, sense 3,
+       * averaging weight 0 */
+      gainbuf[idx] = -999.99;
+      if (out->pg_vert) out->pg_vert[idx] = -999.99;
+      if (out->pg_horiz) out->pg_horiz[idx] = -999.99;
+      totbuf[idx] = -999.99; rhbuf[idx] = -999.99; lhbuf[idx] = -999.99;
+      if (out->e_theta) { out->e_theta[2 * idx] = 0.0; out->e_theta[2 * idx + 1] = 0.0; }
+      if (out->e_phi) { out->e_phi[2 * idx] = 0.0; out->e_phi[2 * idx + 1] = 0.0; }
+      if (out->axial_ratio) out->axial_ratio[idx] = 0.0;
+      if (out->tilt) out->tilt[idx] = 0.0;
+      if (out->sense) out->sense[idx] = 3;
+      scbuf[idx] = 0.0;
+      continue;
+    }
     double e_t[2], e_p[2];
     ora_ffld(g, abc, sin(tha), cos(tha), sin(pha), cos(pha), e_t, e_p);
     double ethm2 = e_t[0] * e_t[0] + e_t[1] * e_t[1];

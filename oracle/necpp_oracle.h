@@ -50,6 +50,15 @@ int64_t ora_nseg(const ora_geom* g);
  * NULL. */
 void ora_segments(const ora_geom* g, double* out, int32_t* icon1, int32_t* icon2, int32_t* tags);
 
+/* nec_ground::parse_gn (src/nec_ground.cpp:75-121): iperf -1 free space,
+ * 0 finite ground (reflection-coefficient approximation, epsr and sig
+ * nonzero), 1 perfect ground; no radial wires, no Sommerfeld ground.  Takes
+ * effect at the next ora_frequency_scale (which sets zrati, frati).  The
+ * fill then adds the image term of efld (src/nec_context.cpp:2879-2956),
+ * ffld the structure image and the pattern skips theta > 90.01 deg.
+ * 0 = ok. */
+int ora_set_ground(ora_geom* g, int iperf, double epsr, double sig);
+
 /* c_geometry::get_segment_number (src/c_geometry.cpp:76-99): 1-based segment
  * number, or -1 when no such tag (error text set). */
 int64_t ora_segment_number(const ora_geom* g, int tag, int m);

@@ -41,6 +41,7 @@ def lib():
         L.ora_speed_of_light.restype = d
         L.ora_impedance.restype = d
         L.ora_frequency_scale.argtypes = [p, d]
+        L.ora_set_ground.argtypes = [p, i, d, d]
         L.ora_trio.argtypes = [p, i, p, p, p, p, i]
         L.ora_tbf.argtypes = [p, i, i, p, p, p, p, i]
         L.ora_fill.argtypes = [p, d, i, p, i, p]
@@ -168,6 +169,12 @@ class Geometry:
         if r < 0:
             raise RuntimeError(self._L.ora_last_error().decode())
         return int(r)
+
+    def set_ground(self, iperf, epsr=0.0, sig=0.0):
+        """GN card: -1 free space, 0 finite ground (reflection coefficients),
+        1 perfect ground."""
+        if self._L.ora_set_ground(self._g, iperf, epsr, sig) != 0:
+            raise RuntimeError(self._L.ora_last_error().decode())
 
     def scale(self, f_mhz):
         self._L.ora_frequency_scale(self._g, f_mhz)

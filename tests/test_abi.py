@@ -110,7 +110,9 @@ def test_error_conventions():
         # cards outside the GPU path fail loudly
         assert api.nec_ld_card(ctx, 0, 0, 1, 1, 10.0, 0, 0) == 1
         assert "not supported" in api.nec_error_message()
-        assert api.nec_gn_card(ctx, 1, 0, 0, 0, 0, 0, 0, 0) == 1
+        assert api.nec_gn_card(ctx, 2, 0, 10.0, 0.01, 0, 0, 0, 0) == 1  # Sommerfeld ground
+        assert "not supported" in api.nec_error_message()
+        assert api.nec_gn_card(ctx, 1, 0, 0, 0, 0, 0, 0, 0) == 0  # perfect ground
         assert api.nec_gn_card(ctx, -1, 0, 0, 0, 0, 0, 0, 0) == 0
         assert api.nec_rp_card(ctx, 1, 10, 1, 0, 0, 0, 0, 0, 0, 1, 0, 0, 0) == 1
         # missing results give the -999 sentinel
