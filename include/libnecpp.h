@@ -58,7 +58,8 @@ const char* nec_error_message(void);
 
 /* src/libnecpp.h:240 (MP card; not on the GPU path) */
 long nec_medium_parameters(nec_context* in_context, double permittivity, double permeability);
-/* src/libnecpp.h:264 (GN card; only iperf = -1, free space, is accepted) */
+/* src/libnecpp.h:264 (GN card: iperf -1 free space, 1 perfect ground, 0 finite ground by reflection
+   coefficients without radial wires; 2 (Sommerfeld) and radial-wire screens return 1) */
 long nec_gn_card(nec_context* in_context, int iperf, int nradl, double epse, double sig,
                  double tmp3, double tmp4, double tmp5, double tmp6);
 /* src/libnecpp.h:274, src/libNEC.cpp:166-168 -> nec_context::fr_card */
