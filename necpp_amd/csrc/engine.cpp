@@ -257,15 +257,10 @@ void DeviceSolver::enqueue_factor() {
   lu_work_.ensure(lu_workspace_bytes(n));
   hip_check(hipMemsetAsync(info_.as<void>(), 0, sizeof(int), stream_), "memset");
   if (precision_ == 1) {
-    // mixed: factor an FP32 copy; Z keeps the FP64 matrix for refinement
-    float* z32 = static_cast<float*>(Z32_.ensure((size_t)n * n * 8));
-    if (launch_to_f32(Z_.as<double>(), ld_, z32, n, n, stream_)) hip_check(hipGetLastError(), "copy launch");
-    const int rc = launch_lu_fp32(z32, n, n, piv_.as<int>(), info_.as<int>(), lu_work_.as<void>(), stream_);
-    if (rc == 2) throw NecError("LU: matrix order " + std::to_string(n) + " exceeds the panel exchange area");
-    if (rc) hip_check(hipGetLastError(), "lu launch");
-    ensure_solve_work();
-    if (launch_lu_solve_prepare_f32(z32, n, n, solve_work_.as<void>(), stream_))
-      hip_check(hipGetLastError(), "solve prepare launch");
+    // mixed: factor an FP32 copy (block-local pivots first); Z keeps the
+    // FP64 matrix for refinement
+    const char* v = std::getenv("NECGPU_MIXED_PIVOT");
+    enqueue_factor_f32((v && v[0] == '0') ? 0 : 1);
     return;
   }
   const int rc = launch_lu_fp64(Z_.as<double>(), ld_, n, piv_.as<int>(), info_.as<int>(), lu_work_.as<void>(), stream_);
@@ -273,6 +268,21 @@ void DeviceSolver::enqueue_factor() {
   if (rc) hip_check(hipGetLastError(), "lu launch");
   ensure_solve_work();
   if (launch_lu_solve_prepare(Z_.as<double>(), ld_, n, solve_work_.as<void>(), stream_))
+    hip_check(hipGetLastError(), "solve prepare launch");
+}
+
+void DeviceSolver::enqueue_factor_f32(int pivot_mode) {
+  const int n = n_;
+  mixed_pivot_ = pivot_mode;
+  hip_check(hipMemsetAsync(info_.as<void>(), 0, sizeof(int), stream_), "memset");
+  float* z32 = static_cast<float*>(Z32_.ensure((size_t)n * n * 8));
+  if (launch_to_f32(Z_.as<double>(), ld_, z32, n, n, stream_)) hip_check(hipGetLastError(), "copy launch");
+  const int rc =
+      launch_lu_fp32(z32, n, n, piv_.as<int>(), info_.as<int>(), lu_work_.as<void>(), stream_, pivot_mode);
+  if (rc == 2) throw NecError("LU: matrix order " + std::to_string(n) + " exceeds the panel exchange area");
+  if (rc) hip_check(hipGetLastError(), "lu launch");
+  ensure_solve_work();
+  if (launch_lu_solve_prepare_f32(z32, n, n, solve_work_.as<void>(), stream_))
     hip_check(hipGetLastError(), "solve prepare launch");
 }
 
@@ -291,6 +301,7 @@ void DeviceSolver::refine_solve() {
     hip_check(hipGetLastError(), "solve launch");
   double norms[2] = {0.0, 0.0};
   refine_steps_ = 0;
+  double prev = 0.0;
   for (int it = 0;; ++it) {
     if (launch_residual(Z_.as<double>(), ld_, n, x_.as<double>(), b, r, acc, it == 0, stream_))
       hip_check(hipGetLastError(), "residual launch");
@@ -299,6 +310,22 @@ void DeviceSolver::refine_solve() {
     const double bn = std::sqrt(norms[1]);
     refine_residual_ = bn > 0.0 ? std::sqrt(norms[0]) / bn : std::sqrt(norms[0]);
     if (!(refine_residual_ > refine_tol_)) break;  // converged (or no right-hand side)
+    // Block-local pivots that converge slowly (or not at all): factor again
+    // with partial pivoting and start over from b.
+    if (mixed_pivot_ == 1 && (it >= kMaxLocalRefine || (it >= 2 && !(refine_residual_ < 0.5 * prev)))) {
+      ++refine_fallbacks_;
+      enqueue_factor_f32(0);
+      hip_check(hipMemcpyAsync(x_.as<void>(), b, (size_t)n * 16, hipMemcpyDeviceToDevice, stream_), "rhs copy");
+      ++epoch_;
+      if (launch_lu_solve_apply_f32(z32, n, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(), epoch_,
+                                    stream_))
+        hip_check(hipGetLastError(), "solve launch");
+      it = -1;
+      prev = 0.0;
+      refine_steps_ = 0;
+      continue;
+    }
+    prev = refine_residual_;
     if (it >= kMaxRefine)
       throw NecError("mixed-precision refinement did not reach the tolerance in " + std::to_string(kMaxRefine) +
                      " steps (relative residual " + std::to_string(refine_residual_) + ")");
@@ -387,6 +414,14 @@ void DeviceSolver::factor() {
             "status");
   hip_check(hipStreamSynchronize(stream_), "factor");
   if (status) throw NecError("LU: a panel exchange timed out");
+  if (precision_ == 1 && mixed_pivot_ == 1 && info != 0) {
+    // a zero pivot inside a diagonal block: partial pivoting instead
+    ++refine_fallbacks_;
+    enqueue_factor_f32(0);
+    hip_check(hipEventRecord(ev_[1], stream_), "event");
+    hip_check(hipMemcpyAsync(&info, info_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
+    hip_check(hipStreamSynchronize(stream_), "factor");
+  }
   times_.factor_ms = elapsed(ev_[0], ev_[1]);
   lu_info_ = info;
   factored_ = true;

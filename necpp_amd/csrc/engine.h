@@ -125,6 +125,10 @@ class DeviceSolver {
   void set_refine_tol(double tol) { refine_tol_ = tol; }
   // refinement steps and final relative residual of the last mixed solve
   int refine_steps() const { return refine_steps_; }
+  // pivot mode of the last FP32 factors (1 block-local, 0 partial) and how
+  // often the refinement fell back to partial pivoting
+  int mixed_pivot() const { return mixed_pivot_; }
+  long refine_fallbacks() const { return refine_fallbacks_; }
   double refine_residual() const { return refine_residual_; }
   hipStream_t stream() const { return stream_; }
 
@@ -132,6 +136,7 @@ class DeviceSolver {
   void ensure_device();
   DevSegments dev_segments() const;
   void enqueue_fill(double rkh);
+  void enqueue_factor_f32(int pivot_mode);
   void enqueue_factor();
   void ensure_solve_work();
   void enqueue_solve(const void* rhs_dev, hipMemcpyKind kind, const void* rhs_host);
@@ -156,6 +161,11 @@ class DeviceSolver {
   int refine_steps_ = 0;
   double refine_residual_ = 0.0;
   static constexpr int kMaxRefine = 30;
+  // refinement steps allowed on block-local-pivot factors before falling
+  // back to partial pivoting
+  static constexpr int kMaxLocalRefine = 10;
+  int mixed_pivot_ = 1;        // pivot mode of the current FP32 factors
+  long refine_fallbacks_ = 0;  // refactorisations with partial pivoting
   void refine_solve();
 
   DevBuf seg_, ekind_, colptr_, colj_, colabc_, segptr_, segbasis_, segabc_;

@@ -160,9 +160,13 @@ size_t lu_workspace_bytes(int n);
 int* lu_status(void* work, int n);
 int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream);
 // The same factorisation of an FP32 matrix (the mixed-precision path): FP32
-// storage and trailing update on v_mfma_f32_16x16x4f32; same pivot rule.
-// work: lu_workspace_bytes(n) is large enough for either.
-int launch_lu_fp32(float* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream);
+// storage and trailing update on v_mfma_f32_16x16x4f32.  pivot_mode 0: the
+// same pivot rule (partial pivoting); 1: pivots searched inside each panel's
+// 64 x 64 diagonal block only (no grid-wide exchange; the factors of a
+// block-permuted matrix, for preconditioning).  work: lu_workspace_bytes(n)
+// is large enough for either.
+int launch_lu_fp32(float* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream,
+                   int pivot_mode);
 // one-wave check of the FP64 MFMA operand / result lane layout (64 a, 64 b in, 256 d out)
 int launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t stream);
 int launch_mfma_probe_f32(const float* a, const float* b, float* d, hipStream_t stream);

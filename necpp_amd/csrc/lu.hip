@@ -1021,7 +1021,11 @@ __global__ void __launch_bounds__(256) zgemm_update_kernel(double* __restrict__ 
 
 // The mixed-precision path's FP32 trailing update: zgemm_update_kernel with
 // FP32 operands on v_mfma_f32_16x16x4f32 (FP32 accumulation), whose result
-// register v of lane l holds D[4 (l / 16) + v][l % 16].
+// register v of lane l holds D[4 (l / 16) + v][l % 16].  The complex product
+// takes three real products instead of four (T1 = Ar Br, T2 = Ai Bi,
+// T3 = (Ar + Ai)(Br + Bi); Re = T1 - T2, Im = T3 - T1 - T2): these factors
+// only precondition the FP64 refinement, which absorbs the slightly larger
+// rounding of the imaginary part.
 __global__ void __launch_bounds__(256) zgemm_update_f32_kernel(float* __restrict__ A, long long lda, int m0, int n0, int M,
                                                            int N, int k0, int K) {
   __shared__ float sAr[GK * 64], sAi[GK * 64];  // [k][row]
@@ -1064,13 +1068,14 @@ __global__ void __launch_bounds__(256) zgemm_update_f32_kernel(float* __restrict
         }
       }
     };
-    float4_t accr[2][2], acci[2][2];
+    float4_t t1[2][2], t2[2][2], t3[2][2];
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
-        accr[a][b] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
-        acci[a][b] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+        t1[a][b] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+        t2[a][b] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
+        t3[a][b] = (float4_t){0.0f, 0.0f, 0.0f, 0.0f};
       }
     fetch(0);
     for (int ch = 0; ch < nchunks; ++ch) {
@@ -1098,14 +1103,18 @@ __global__ void __launch_bounds__(256) zgemm_update_f32_kernel(float* __restrict
           yr[b] = sBr[q * 64 + wc * 32 + b * 16 + li];
           yi[b] = sBi[q * 64 + wc * 32 + b * 16 + li];
         }
+        float xs[2], ys[2];
+#pragma unroll
+        for (int a = 0; a < 2; ++a) xs[a] = xr[a] + xi[a];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) ys[b] = yr[b] + yi[b];
 #pragma unroll
         for (int a = 0; a < 2; ++a)
 #pragma unroll
           for (int b = 0; b < 2; ++b) {
-            accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[a], yr[b], accr[a][b], 0, 0, 0);
-            accr[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[a], -yi[b], accr[a][b], 0, 0, 0);
-            acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[a], yi[b], acci[a][b], 0, 0, 0);
-            acci[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[a], yr[b], acci[a][b], 0, 0, 0);
+            t1[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xr[a], yr[b], t1[a][b], 0, 0, 0);
+            t2[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xi[a], yi[b], t2[a][b], 0, 0, 0);
+            t3[a][b] = __builtin_amdgcn_mfma_f32_16x16x4f32(xs[a], ys[b], t3[a][b], 0, 0, 0);
           }
       }
     }
@@ -1120,8 +1129,9 @@ __global__ void __launch_bounds__(256) zgemm_update_f32_kernel(float* __restrict
           const int j = col0 + wc * 32 + b * 16 + li;
           if (i < M && j < N) {
             float* c = A + 2 * ((size_t)(m0 + i) * lda + n0 + j);
-            c[0] -= accr[a][b][v];
-            c[1] -= acci[a][b][v];
+            const float p1 = t1[a][b][v], p2 = t2[a][b][v];
+            c[0] -= p1 - p2;
+            c[1] -= (t3[a][b][v] - p1) - p2;
           }
         }
   }
@@ -1187,6 +1197,19 @@ bool lookahead_enabled() {
   return !(v && v[0] == '0');
 }
 
+// Largest panel (in workgroups, one CU each) that runs beside the trailing
+// update.  A wider panel takes so many CUs from the update, and waits so
+// long for CUs the update holds, that the two measured slower together than
+// one after the other (N = 16,020: 583 ms against 446 ms for the FP64
+// factorisation, 320 against 277 ms for the FP32 one); below it the panel
+// hides behind the update (N = 3,280).  Limits of 24, 32, 48 and 64 groups
+// measured 448, 447, 446, 444 ms (FP64) and 284, 279, 277, 297 ms (FP32) at
+// N = 16,020.  NECGPU_LA_MAX_G overrides, for measurements.
+int lookahead_max_groups() {
+  const char* v = std::getenv("NECGPU_LA_MAX_G");
+  return (v && *v) ? std::atoi(v) : 48;
+}
+
 // Second stream (default priority) and two events per device for the
 // look-ahead: the next panel runs there while the caller's stream finishes
 // the trailing update.  Created once, kept for the life of the process.
@@ -1224,6 +1247,194 @@ int launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t s
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+namespace {
+
+// The mixed path's panel with block-local pivots: the row of column c's
+// pivot is searched among the rows of the panel's nb x nb diagonal block
+// only (first maximum of |z|, as elsewhere), so no workgroup has to see the
+// rows below.  The factors are those of P A with P permuting inside 64-row
+// blocks; they only precondition the FP64 refinement (which checks the
+// result and falls back to partial pivoting when it does not converge).
+// One workgroup factors the diagonal block in LDS and records piv / perm as
+// the partial-pivot panel does.
+constexpr int DB_THREADS = 256;
+constexpr int DB_WAVES = DB_THREADS / 64;
+template <typename T>
+__global__ void __launch_bounds__(DB_THREADS) lu_diag_block_kernel(T* __restrict__ A, long long lda, int n, int k0,
+                                                                    int nb, int* piv, int* perm, int* info) {
+  constexpr int LD = NB + 1;
+  __shared__ T sr[NB * LD], si[NB * LD];  // block [row][col], real and imaginary planes
+  __shared__ int s_p;
+  __shared__ int s_perm[NB];  // perm[k0 ..], exchanged here, written back at the end
+  __shared__ T s_ir, s_ii;    // 1 / pivot
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;  // lanes over columns, waves over rows
+  if (t < nb) s_perm[t] = perm[k0 + t];
+  for (int r = wave; r < nb; r += DB_WAVES)
+    if (lane < nb) {
+      const T* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + lane);
+      sr[r * LD + lane] = v[0];
+      si[r * LD + lane] = v[1];
+    }
+  __syncthreads();
+  for (int c = 0; c < nb; ++c) {
+    if (wave == 0) {
+      // first maximum of |z| over rows c .. nb-1 (lanes over rows here)
+      double v = -1.0;
+      if (lane >= c && lane < nb) v = hypot((double)sr[lane * LD + c], (double)si[lane * LD + c]);
+      double m = v;
+      for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
+      const unsigned long long hit = __ballot(v >= 0.0 && v == m);
+      if (lane == 0) {
+        int p = hit ? (__ffsll((long long)hit) - 1) : c;
+        if (!(m > 0.0)) {
+          if (*info == 0) *info = k0 + c + 1;
+          p = c;
+        }
+        s_p = p;
+        // 1 / pivot (the pivot is row p's entry before the exchange)
+        const T dr = sr[p * LD + c], di = si[p * LD + c];
+        const T den = dr * dr + di * di;
+        s_ir = (den > T(0)) ? dr / den : T(0);
+        s_ii = (den > T(0)) ? -di / den : T(0);
+        piv[k0 + c] = k0 + p;
+        if (p != c) {
+          const int tmp = s_perm[c];
+          s_perm[c] = s_perm[p];
+          s_perm[p] = tmp;
+        }
+      }
+    }
+    __syncthreads();
+    const int p = s_p;
+    if (p != c && wave == 0 && lane < nb) {
+      const T ar = sr[c * LD + lane], ai = si[c * LD + lane];
+      sr[c * LD + lane] = sr[p * LD + lane];
+      si[c * LD + lane] = si[p * LD + lane];
+      sr[p * LD + lane] = ar;
+      si[p * LD + lane] = ai;
+    }
+    __syncthreads();
+    // rows below c, four per step with every load issued first: each lane
+    // forms the row's multiplier (read before lane c overwrites column c)
+    // and updates its own column; lanes at or left of c keep their values
+    const T ir = s_ir, ii = s_ii;
+    const bool right = lane > c && lane < nb;
+    const T ur = right ? sr[c * LD + lane] : T(0), ui = right ? si[c * LD + lane] : T(0);
+    for (int r = c + 1 + wave; r < nb; r += 4 * DB_WAVES) {
+      T ar[4], ai[4], xr[4], xi[4];
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rr = r + q * DB_WAVES;
+        const bool on = rr < nb;
+        ar[q] = on ? sr[rr * LD + c] : T(0);
+        ai[q] = on ? si[rr * LD + c] : T(0);
+        xr[q] = (on && lane < nb) ? sr[rr * LD + lane] : T(0);
+        xi[q] = (on && lane < nb) ? si[rr * LD + lane] : T(0);
+      }
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int rr = r + q * DB_WAVES;
+        if (rr < nb && lane < nb) {
+          const T lr = ar[q] * ir - ai[q] * ii, li = ar[q] * ii + ai[q] * ir;
+          const T nr = xr[q] - (lr * ur - li * ui), ni = xi[q] - (lr * ui + li * ur);
+          sr[rr * LD + lane] = (lane == c) ? lr : nr;
+          si[rr * LD + lane] = (lane == c) ? li : ni;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  for (int r = wave; r < nb; r += DB_WAVES)
+    if (lane < nb) {
+      T* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + lane);
+      v[0] = sr[r * LD + lane];
+      v[1] = si[r * LD + lane];
+    }
+  if (t < nb) perm[k0 + t] = s_perm[t];
+}
+
+// Value of lane src (uniform) of a wave, for 4- and 8-byte types.
+template <typename T>
+__device__ __forceinline__ T lane_value(T v, int src) {
+  if constexpr (sizeof(T) == 4) {
+    return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), src));
+  } else {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), src);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+  }
+}
+
+// L21 = A21 U11^-1 for the rows below the diagonal block (in place): each
+// wave holds L21_ROWS rows in registers, lane j holding column j, and runs
+// the columns in order: l_c = a_c / u_cc (lane c's value, read by the whole
+// wave), then a_j -= l_c u_cj for j > c.  Rows are independent, so the waves
+// never wait for each other.
+constexpr int L21_ROWS = 16;
+template <typename T>
+__global__ void __launch_bounds__(DB_THREADS) lu_l21_kernel(T* __restrict__ A, long long lda, int n, int k0, int nb) {
+  constexpr int LD = NB + 1;
+  __shared__ T ur[NB * LD], ui[NB * LD];  // U11
+  __shared__ T dr[NB], di[NB];            // 1 / u_cc
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  for (int e = t; e < nb * NB; e += DB_THREADS) {
+    const int r = e / NB, c = e % NB;
+    if (c < nb) {
+      const T* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + c);
+      ur[r * LD + c] = v[0];
+      ui[r * LD + c] = v[1];
+    }
+  }
+  __syncthreads();
+  if (t < nb) {
+    const T a = ur[t * LD + t], b = ui[t * LD + t];
+    const T den = a * a + b * b;
+    dr[t] = (den > T(0)) ? a / den : T(0);
+    di[t] = (den > T(0)) ? -b / den : T(0);
+  }
+  __syncthreads();
+  const int r0 = k0 + nb + (blockIdx.x * DB_WAVES + wave) * L21_ROWS;
+  const bool col = lane < nb;
+  T xr[L21_ROWS], xi[L21_ROWS];
+#pragma unroll
+  for (int q = 0; q < L21_ROWS; ++q) {
+    xr[q] = xi[q] = T(0);
+    if (col && r0 + q < n) {
+      const T* v = A + 2 * ((size_t)(r0 + q) * lda + k0 + lane);
+      xr[q] = v[0];
+      xi[q] = v[1];
+    }
+  }
+  for (int c = 0; c < nb; ++c) {
+    const T er = dr[c], ei = di[c];
+    // pivot-row entries right of c (zero elsewhere, so other lanes keep
+    // their values) and lane c's slot for the multiplier: selects, no
+    // branches, so the row arrays stay in place
+    const bool right = col && lane > c, here = lane == c;
+    const T u_r = right ? ur[c * LD + lane] : T(0), u_i = right ? ui[c * LD + lane] : T(0);
+#pragma unroll
+    for (int q = 0; q < L21_ROWS; ++q) {
+      const T ar = lane_value(xr[q], c), ai = lane_value(xi[q], c);
+      const T lr = ar * er - ai * ei, li = ar * ei + ai * er;
+      const T nr = xr[q] - (lr * u_r - li * u_i), ni = xi[q] - (lr * u_i + li * u_r);
+      xr[q] = here ? lr : nr;
+      xi[q] = here ? li : ni;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < L21_ROWS; ++q)
+    if (col && r0 + q < n) {
+      T* v = A + 2 * ((size_t)(r0 + q) * lda + k0 + lane);
+      v[0] = xr[q];
+      v[1] = xi[q];
+    }
+}
+
+}  // namespace
+
 int launch_mfma_probe_f32(const float* a, const float* b, float* d, hipStream_t stream) {
   hipLaunchKernelGGL(mfma_probe_f32_kernel, dim3(1), dim3(64), 0, stream, a, b, d);
   return hipGetLastError() == hipSuccess ? 0 : 1;
@@ -1232,7 +1443,7 @@ int launch_mfma_probe_f32(const float* a, const float* b, float* d, hipStream_t 
 namespace {
 
 template <typename T>
-int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream) {
+int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream, bool local_pivot) {
   constexpr int RMAX_T = rmax_of<T>();
   static bool attr_set = false;
   static int cu_count = 0;
@@ -1261,6 +1472,18 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
   Lookahead* la = (npanels > 1 && lookahead_enabled()) ? lookahead_for_device() : nullptr;
   auto panel = [&](int k0, hipStream_t s) {
     const int nb = std::min(NB, n - k0);
+    if constexpr (sizeof(T) == 4) {  // block-local pivots: the FP32 path only
+      if (local_pivot) {
+        launch_k(K_LU_PANEL, lu_diag_block_kernel<T>, dim3(1), dim3(DB_THREADS), 0u, s, A, lda, n, k0, nb, piv,
+                 perm, info);
+        const int below = n - k0 - nb;
+        constexpr int rows_per_block = DB_WAVES * L21_ROWS;
+        if (below > 0)
+          launch_k(K_LU_PANEL, lu_l21_kernel<T>, dim3((below + rows_per_block - 1) / rows_per_block),
+                   dim3(DB_THREADS), 0u, s, A, lda, n, k0, nb);
+        return;
+      }
+    }
     const int rows = n - k0;
     const int G = (rows + RMAX_T - 1) / RMAX_T;
     const int R = (rows + G - 1) / G;
@@ -1305,19 +1528,30 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
     else
       update(k0, nb, cbeg, cend, INT_MAX, s);
   };
-  if (!la) {
-    // right-looking LU, one stream
-    panel(0, stream);
+  // Right-looking LU on one stream while the panels are wide (or without
+  // look-ahead), then the look-ahead below from panel kla on.
+  const int la_max_g = lookahead_max_groups();
+  auto groups = [&](int k0) { return (n - k0 + RMAX_T - 1) / RMAX_T; };
+  int kla = n;
+  panel(0, stream);
+  if (local_pivot) la = nullptr;  // cheap panels: one stream
+  if (la && groups(0) <= la_max_g) {
+    kla = 0;
+  } else {
     for (int k0 = 0; k0 < n; k0 += NB) {
       const int nb = std::min(NB, n - k0);
       const int next = k0 + nb;
       laswp(k0, nb, 0, k0, next, n, stream);
       if (next >= n) break;
       update(k0, nb, next, n, INT_MAX, stream);
+      if (la && groups(next) <= la_max_g) {
+        kla = next;
+        break;
+      }
       panel(next, stream);
     }
-    return hipGetLastError() == hipSuccess ? 0 : 1;
   }
+  if (kla >= n) return hipGetLastError() == hipSuccess ? 0 : 1;
   // Right-looking LU with a look-ahead of one panel, on two streams:
   //   side stream:   panel p -> [rest of p-1 done] row exchanges of the next
   //                  panel's columns -> their U12 and trailing update (one
@@ -1330,9 +1564,9 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
   // the two streams never touch the same columns at the same time.
   if (hipEventRecord(la->start, stream) != hipSuccess) return 1;
   if (hipStreamWaitEvent(la->side, la->start, 0) != hipSuccess) return 1;
-  panel(0, la->side);
+  if (kla > 0) panel(kla, la->side);
   if (hipEventRecord(la->panel_done, la->side) != hipSuccess) return 1;
-  for (int k0 = 0; k0 < n; k0 += NB) {
+  for (int k0 = kla; k0 < n; k0 += NB) {
     const int nb = std::min(NB, n - k0);
     const int next = k0 + nb;
     if (hipStreamWaitEvent(stream, la->panel_done, 0) != hipSuccess) return 1;
@@ -1341,7 +1575,7 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
       break;
     }
     const int far = next + std::min(NB, n - next);
-    if (k0 > 0 && hipStreamWaitEvent(la->side, la->rest_done, 0) != hipSuccess) return 1;
+    if (k0 > kla && hipStreamWaitEvent(la->side, la->rest_done, 0) != hipSuccess) return 1;
     laswp(k0, nb, next, far, far, far, la->side);
     next_update(k0, nb, next, far, la->side);
     panel(next, la->side);
@@ -1359,11 +1593,12 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
 }  // namespace
 
 int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream) {
-  return launch_lu<double>(A, lda, n, piv, info, work, stream);
+  return launch_lu<double>(A, lda, n, piv, info, work, stream, false);
 }
 
-int launch_lu_fp32(float* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream) {
-  return launch_lu<float>(A, lda, n, piv, info, work, stream);
+int launch_lu_fp32(float* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream,
+                   int pivot_mode) {
+  return launch_lu<float>(A, lda, n, piv, info, work, stream, pivot_mode == 1);
 }
 
 }  // namespace necgpu

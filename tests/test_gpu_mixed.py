@@ -85,6 +85,28 @@ def test_mixed_c2_grid_matches_oracle():
         api.nec_delete(ctx)
 
 
+@pytest.mark.parametrize("mode", ["1", "0"])
+def test_mixed_pivot_modes_match_oracle(mode, monkeypatch):
+    """The FP32 factors with block-local pivots (the default) and with
+    partial pivoting (NECGPU_MIXED_PIVOT=0) both refine to the oracle's
+    Z_in; the grid's partial pivoting moves rows across 64-row blocks, so the
+    block-local factors differ from the reference's and only the refined
+    solution is compared."""
+    monkeypatch.setenv("NECGPU_MIXED_PIVOT", mode)
+    deck = decks.grid(21)
+    ctx, rc = run(deck, 1, with_rp=False)
+    try:
+        assert rc == 0, api.nec_error_message()
+        o = po.run_deck(deck, nthreads=8, with_rp=False)
+        assert rel(zin(ctx), o["zin"][0]) < TOL_ZIN
+        cur = api.gpu_currents(ctx, 0)
+        assert np.linalg.norm(cur - o["current"]) <= TOL_CUR * np.linalg.norm(o["current"])
+        steps, res = api.gpu_refinement(ctx)
+        assert res <= 1e-12 and 1 <= steps <= 10, (steps, res)
+    finally:
+        api.nec_delete(ctx)
+
+
 def test_precision_switch_refactors():
     # FP64 -> mixed -> FP64 on one context: every solve matches the oracle
     deck = decks.dipole(31)
