@@ -1259,101 +1259,6 @@ namespace {
 // the partial-pivot panel does.
 constexpr int DB_THREADS = 256;
 constexpr int DB_WAVES = DB_THREADS / 64;
-template <typename T>
-__global__ void __launch_bounds__(DB_THREADS) lu_diag_block_kernel(T* __restrict__ A, long long lda, int n, int k0,
-                                                                    int nb, int* piv, int* perm, int* info) {
-  constexpr int LD = NB + 1;
-  __shared__ T sr[NB * LD], si[NB * LD];  // block [row][col], real and imaginary planes
-  __shared__ int s_p;
-  __shared__ int s_perm[NB];  // perm[k0 ..], exchanged here, written back at the end
-  __shared__ T s_ir, s_ii;    // 1 / pivot
-  const int t = threadIdx.x;
-  const int lane = t & 63, wave = t >> 6;  // lanes over columns, waves over rows
-  if (t < nb) s_perm[t] = perm[k0 + t];
-  for (int r = wave; r < nb; r += DB_WAVES)
-    if (lane < nb) {
-      const T* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + lane);
-      sr[r * LD + lane] = v[0];
-      si[r * LD + lane] = v[1];
-    }
-  __syncthreads();
-  for (int c = 0; c < nb; ++c) {
-    if (wave == 0) {
-      // first maximum of |z| over rows c .. nb-1 (lanes over rows here)
-      double v = -1.0;
-      if (lane >= c && lane < nb) v = hypot((double)sr[lane * LD + c], (double)si[lane * LD + c]);
-      double m = v;
-      for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-      const unsigned long long hit = __ballot(v >= 0.0 && v == m);
-      if (lane == 0) {
-        int p = hit ? (__ffsll((long long)hit) - 1) : c;
-        if (!(m > 0.0)) {
-          if (*info == 0) *info = k0 + c + 1;
-          p = c;
-        }
-        s_p = p;
-        // 1 / pivot (the pivot is row p's entry before the exchange)
-        const T dr = sr[p * LD + c], di = si[p * LD + c];
-        const T den = dr * dr + di * di;
-        s_ir = (den > T(0)) ? dr / den : T(0);
-        s_ii = (den > T(0)) ? -di / den : T(0);
-        piv[k0 + c] = k0 + p;
-        if (p != c) {
-          const int tmp = s_perm[c];
-          s_perm[c] = s_perm[p];
-          s_perm[p] = tmp;
-        }
-      }
-    }
-    __syncthreads();
-    const int p = s_p;
-    if (p != c && wave == 0 && lane < nb) {
-      const T ar = sr[c * LD + lane], ai = si[c * LD + lane];
-      sr[c * LD + lane] = sr[p * LD + lane];
-      si[c * LD + lane] = si[p * LD + lane];
-      sr[p * LD + lane] = ar;
-      si[p * LD + lane] = ai;
-    }
-    __syncthreads();
-    // rows below c, four per step with every load issued first: each lane
-    // forms the row's multiplier (read before lane c overwrites column c)
-    // and updates its own column; lanes at or left of c keep their values
-    const T ir = s_ir, ii = s_ii;
-    const bool right = lane > c && lane < nb;
-    const T ur = right ? sr[c * LD + lane] : T(0), ui = right ? si[c * LD + lane] : T(0);
-    for (int r = c + 1 + wave; r < nb; r += 4 * DB_WAVES) {
-      T ar[4], ai[4], xr[4], xi[4];
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int rr = r + q * DB_WAVES;
-        const bool on = rr < nb;
-        ar[q] = on ? sr[rr * LD + c] : T(0);
-        ai[q] = on ? si[rr * LD + c] : T(0);
-        xr[q] = (on && lane < nb) ? sr[rr * LD + lane] : T(0);
-        xi[q] = (on && lane < nb) ? si[rr * LD + lane] : T(0);
-      }
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int rr = r + q * DB_WAVES;
-        if (rr < nb && lane < nb) {
-          const T lr = ar[q] * ir - ai[q] * ii, li = ar[q] * ii + ai[q] * ir;
-          const T nr = xr[q] - (lr * ur - li * ui), ni = xi[q] - (lr * ui + li * ur);
-          sr[rr * LD + lane] = (lane == c) ? lr : nr;
-          si[rr * LD + lane] = (lane == c) ? li : ni;
-        }
-      }
-    }
-    __syncthreads();
-  }
-  for (int r = wave; r < nb; r += DB_WAVES)
-    if (lane < nb) {
-      T* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + lane);
-      v[0] = sr[r * LD + lane];
-      v[1] = si[r * LD + lane];
-    }
-  if (t < nb) perm[k0 + t] = s_perm[t];
-}
-
 // Value of lane src (uniform) of a wave, for 4- and 8-byte types.
 template <typename T>
 __device__ __forceinline__ T lane_value(T v, int src) {
@@ -1365,6 +1270,117 @@ __device__ __forceinline__ T lane_value(T v, int src) {
     const int hi = __builtin_amdgcn_readlane((int)(b >> 32), src);
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
   }
+}
+
+// 1024 threads: lanes over columns, 16 waves over rows (4 rows each), so the
+// sequential column loop spends little time per wave.  Per column: every
+// wave forms its rows' new values from the old block (the pivot row p is
+// read before anyone writes it); barrier; the rows are written; barrier;
+// wave 0 finds the next column's pivot from LDS (first maximum of |z|^2, a
+// preconditioner needs no more); barrier.
+constexpr int DG_THREADS = 1024;
+constexpr int DG_WAVES = DG_THREADS / 64;
+constexpr int DG_ROWS = NB / DG_WAVES;  // rows per wave
+template <typename T>
+__global__ void __launch_bounds__(DG_THREADS) lu_diag_block_kernel(T* __restrict__ A, long long lda, int n, int k0,
+                                                                    int nb, int* piv, int* perm, int* info) {
+  constexpr int LD = NB + 1;
+  __shared__ T sr[NB * LD], si[NB * LD];  // block [row][col], real and imaginary planes
+  // perm[k0 ..] and piv[k0 ..] live here until the end: a global store
+  // before a barrier would make every column wait for it to complete
+  __shared__ int s_perm[NB], s_piv[NB];
+  __shared__ int s_p, s_info;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const bool col = lane < nb;
+  if (t < nb) s_perm[t] = perm[k0 + t];
+  if (t == 0) s_info = 0;
+  for (int r = wave; r < nb; r += DG_WAVES)
+    if (col) {
+      const T* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + lane);
+      sr[r * LD + lane] = v[0];
+      si[r * LD + lane] = v[1];
+    }
+  __syncthreads();
+  // wave 0: first maximum of |z|^2 over rows c .. nb-1 of column c -> s_p
+  auto search = [&](int c) {
+    float m = -1.0f;
+    if (lane >= c && lane < nb) {
+      const float a = (float)sr[lane * LD + c], b = (float)si[lane * LD + c];
+      m = a * a + b * b;
+    }
+    float mx = m;
+    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    const unsigned long long hit = __ballot(m >= 0.0f && m == mx);
+    if (lane == 0) {
+      int p = hit ? (__ffsll((long long)hit) - 1) : c;
+      if (!(mx > 0.0f)) {
+        p = c;  // zero column: no exchange, no division
+        if (s_info == 0) s_info = k0 + c + 1;
+      }
+      s_p = p;
+    }
+  };
+  if (wave == 0) search(0);
+  __syncthreads();
+  for (int c = 0; c < nb; ++c) {
+    const int p = s_p;
+    // pivot row (old row p), 1 / pivot
+    const T ur = col ? sr[p * LD + lane] : T(0), ui = col ? si[p * LD + lane] : T(0);
+    const T pr = sr[p * LD + c], pi = si[p * LD + c];
+    const T den = pr * pr + pi * pi;
+    const T inv = (den > T(0)) ? T(1) / den : T(0);
+    const T ir = pr * inv, ii = -pi * inv;
+    const bool right = col && lane > c, here = lane == c;
+    const T vr = right ? ur : T(0), vi = right ? ui : T(0);
+    // this wave's rows r = c+1+wave+16q (row p takes old row c)
+    T nr[DG_ROWS], ni[DG_ROWS];
+#pragma unroll
+    for (int q = 0; q < DG_ROWS; ++q) {
+      const int r = min(c + 1 + wave + DG_WAVES * q, nb - 1);  // clamped: a valid row, result unused
+      const int src = (r == p) ? c : r;
+      const T ar = sr[src * LD + c], ai = si[src * LD + c];
+      const T xr = sr[src * LD + lane], xi = si[src * LD + lane];
+      const T lr = ar * ir - ai * ii, li = ar * ii + ai * ir;
+      nr[q] = here ? lr : xr - (lr * vr - li * vi);
+      ni[q] = here ? li : xi - (lr * vi + li * vr);
+    }
+    __syncthreads();  // every read of the old block is done
+#pragma unroll
+    for (int q = 0; q < DG_ROWS; ++q) {
+      const int r = c + 1 + wave + DG_WAVES * q;
+      if (r < nb && col) {
+        sr[r * LD + lane] = nr[q];
+        si[r * LD + lane] = ni[q];
+      }
+    }
+    if (wave == 1 && col && p != c) {  // row c becomes the pivot row
+      sr[c * LD + lane] = ur;
+      si[c * LD + lane] = ui;
+    }
+    if (t == 64) {
+      s_piv[c] = k0 + p;
+      if (p != c) {
+        const int tmp = s_perm[c];
+        s_perm[c] = s_perm[p];
+        s_perm[p] = tmp;
+      }
+    }
+    __syncthreads();
+    if (wave == 0 && c + 1 < nb) search(c + 1);
+    __syncthreads();
+  }
+  for (int r = wave; r < nb; r += DG_WAVES)
+    if (col) {
+      T* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + lane);
+      v[0] = sr[r * LD + lane];
+      v[1] = si[r * LD + lane];
+    }
+  if (t < nb) {
+    perm[k0 + t] = s_perm[t];
+    piv[k0 + t] = s_piv[t];
+  }
+  if (t == 0 && s_info != 0 && *info == 0) *info = s_info;
 }
 
 // L21 = A21 U11^-1 for the rows below the diagonal block (in place): each
@@ -1474,7 +1490,7 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
     const int nb = std::min(NB, n - k0);
     if constexpr (sizeof(T) == 4) {  // block-local pivots: the FP32 path only
       if (local_pivot) {
-        launch_k(K_LU_PANEL, lu_diag_block_kernel<T>, dim3(1), dim3(DB_THREADS), 0u, s, A, lda, n, k0, nb, piv,
+        launch_k(K_LU_PANEL, lu_diag_block_kernel<T>, dim3(1), dim3(DG_THREADS), 0u, s, A, lda, n, k0, nb, piv,
                  perm, info);
         const int below = n - k0 - nb;
         constexpr int rows_per_block = DB_WAVES * L21_ROWS;
