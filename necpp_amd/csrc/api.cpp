@@ -93,7 +93,10 @@ long nec_ek_card(nec_context* ctx, int itmp1) {
   return guarded([&] { ctx->ek_card(itmp1 != -1); });
 }
 
-long nec_ld_card(nec_context*, int, int, int, int, double, double, double) { return unsupported("LD"); }
+long nec_ld_card(nec_context* ctx, int ldtyp, int ldtag, int ldtagf, int ldtagt, double tmp1, double tmp2,
+                 double tmp3) {
+  return guarded([&] { ctx->ld_card(ldtyp, ldtag, ldtagf, ldtagt, tmp1, tmp2, tmp3); });
+}
 
 long nec_ex_card(nec_context* ctx, int extype, int i2, int i3, int i4, double tmp1, double tmp2, double tmp3,
                  double tmp4, double tmp5, double tmp6) {

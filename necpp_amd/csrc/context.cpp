@@ -21,6 +21,7 @@ void Context::calc_prepare() {
   rkh_ = 1.;
   use_ek_ = false;
   gn_iperf_ = -1;  // ground.default_values(): free space
+  loads_.clear();
   ifar_ = -1;
   freq_mhz_ = em_speed_of_light() / 1.0e6;
   nfrq_ = 1;
@@ -106,7 +107,147 @@ void Context::gn_card(int iperf, int nradl, double epsr, double sig, double t3, 
     gn_epsr_ = epsr;
     gn_sig_ = sig;
   }
+  iflow_ = 4;
   reset_to_structure_loading();
+}
+
+void Context::ld_card(int type, int tag, int tagf, int tagt, double zlr, double zli, double zlc) {
+  // src/nec_context.cpp:424-474
+  if (iflow_ != 3) {
+    iflow_ = 3;
+    loads_.clear();
+    reset_to_structure_loading();
+    if (type == -1) return;
+  }
+  if (tagt == 0) tagt = tagf;
+  if (tagt < tagf)
+    throw NecError("DATA FAULT ON LOADING CARD No: " + std::to_string(loads_.size() + 1) + ": ITAG STEP1: " +
+                   std::to_string(tagf) + " IS GREATER THAN ITAG STEP2: " + std::to_string(tagt));
+  loads_.push_back(LoadCard{type, tag, tagf, tagt, zlr, zli, zlc});
+}
+
+namespace {
+
+// nec_context::zint, src/nec_context.cpp:6321-6406: lambda * internal
+// impedance per unit length of a round wire (sigl = conductivity x
+// wavelength, rolam = radius in wavelengths), from the published polynomial
+// approximations of the Kelvin functions (x <= 8), their asymptotic
+// expansions (x > 8) and the skin-effect limit (x > 110).  The coefficient
+// tables are the reference's, so that the loads agree to the last bit.
+std::complex<double> zint(double sigl, double rolam) {
+  typedef std::complex<double> C;
+  const C cc1(6.0e-7, 1.9e-6), cc2(-3.4e-6, 5.1e-6), cc3(-2.52e-5, 0.0), cc4(-9.06e-5, -9.01e-5);
+  const C cc5(0., -9.765e-4), cc6(.0110486, -.0110485), cc7(0., -.3926991), cc8(1.6e-6, -3.2e-6);
+  const C cc9(1.17e-5, -2.4e-6), cc10(3.46e-5, 3.38e-5), cc11(5.0e-7, 2.452e-4), cc12(-1.3813e-3, 1.3811e-3);
+  const C cc13(-6.25001e-2, -1.0e-7), cc14(.7071068, .7071068), cn = cc14;
+  const double tpcmu = 2.368705e+3, cmotp = 60.00;
+  const C j1(0.0, 1.0);
+  const double x = std::sqrt(tpcmu * sigl) * rolam;
+  auto th = [&](double d) { return (((((cc1 * d + cc2) * d + cc3) * d + cc4) * d + cc5) * d + cc6) * d + cc7; };
+  auto ph = [&](double d) { return (((((cc8 * d + cc9) * d + cc10) * d + cc11) * d + cc12) * d + cc13) * d + cc14; };
+  if (x > 110.0) {
+    const C br1(0.70710678, -0.70710678);
+    return j1 * std::sqrt(cmotp / sigl) * br1 / rolam;
+  }
+  if (x > 8.0) {
+    const C fx = std::sqrt((kPi / 2.0) / x) * std::exp(-cn * x + th(-8. / x));
+    const C gx = std::exp(cn * x + th(8. / x)) / std::sqrt((2.0 * kPi) * x);
+    C br2 = j1 * fx / kPi;
+    C br1 = gx + br2;
+    br2 = gx * ph(8.0 / x) - br2 * ph(-8.0 / x);
+    br1 = br1 / br2;
+    return j1 * std::sqrt(cmotp / sigl) * br1 / rolam;
+  }
+  const double x8 = x / 8.0, y = x8 * x8, s = y * y;
+  double ber =
+      ((((((-9.01e-6 * s + 1.22552e-3) * s - .08349609) * s + 2.6419140) * s - 32.363456) * s + 113.77778) * s - 64.) *
+          s +
+      1.;
+  double bei =
+      ((((((1.1346e-4 * s - .01103667) * s + .52185615) * s - 10.567658) * s + 72.817777) * s - 113.77778) * s + 16.) *
+      y;
+  C br1(ber, bei);
+  ber = (((((((-3.94e-6 * s + 4.5957e-4) * s - 0.02609253) * s + 0.66047849) * s - 6.0681481) * s + 14.222222) * s -
+          4.0) *
+         y) *
+        x;
+  bei = ((((((4.609e-5 * s - 3.79386e-3) * s + .14677204) * s - 2.3116751) * s + 11.377778) * s - 10.666667) * s +
+         .5) *
+        x;
+  const C br2(ber, bei);
+  br1 = br1 / br2;
+  return j1 * std::sqrt(cmotp / sigl) * br1 / rolam;
+}
+
+}  // namespace
+
+std::vector<zc> Context::load_impedances(const ScaledSegments& seg) const {
+  // nec_context::load, src/nec_context.cpp:1861-2055 (no symmetry)
+  std::vector<zc> out;
+  if (loads_.empty()) return out;
+  const int64_t n = geo_.size();
+  std::vector<std::complex<double>> za(n, std::complex<double>(0.0, 0.0));
+  const std::complex<double> tpcj(0.0, 1.883698955e+9);
+  const double wl = wavelength_;
+  for (size_t k = 0; k < loads_.size(); ++k) {
+    const LoadCard& L = loads_[k];
+    if (L.type > 5 || L.type < 0) throw NecError("IMPROPER LOAD TYPE CHOSEN, REQUESTED TYPE IS " + std::to_string(L.type));
+    int ichk = 0;
+    int64_t l1 = 1, l2 = n;
+    if (L.tag == 0 && (L.tagf != 0 || L.tagt != 0)) {
+      l1 = L.tagf;
+      l2 = L.tagt;
+    }
+    for (int64_t i = l1 - 1; i < l2; ++i) {
+      if (i < 0 || i >= n) continue;
+      if (L.tag != 0) {
+        if (L.tag != geo_.tags[i]) continue;
+        if (L.tagf != 0) {
+          ++ichk;
+          if (ichk < L.tagf || ichk > L.tagt) continue;
+        } else {
+          ichk = 1;
+        }
+      } else {
+        ichk = 1;
+      }
+      const double sl = seg.s[i];
+      std::complex<double> zt;
+      switch (L.type + 1) {
+        case 1:
+          zt = L.zlr / sl + tpcj * L.zli / (sl * wl);
+          if (std::fabs(L.zlc) > 1.0e-20) zt += wl / (tpcj * sl * L.zlc);
+          break;
+        case 2:
+          zt = tpcj * sl * L.zlc / wl;
+          if (std::fabs(L.zli) > 1.0e-20) zt += sl * wl / (tpcj * L.zli);
+          if (std::fabs(L.zlr) > 1.0e-20) zt += sl / L.zlr;
+          zt = 1. / zt;
+          break;
+        case 3:
+          zt = L.zlr * wl + tpcj * L.zli;
+          if (std::fabs(L.zlc) > 1.0e-20) zt += 1. / (tpcj * sl * sl * L.zlc);
+          break;
+        case 4:
+          zt = tpcj * sl * sl * L.zlc;
+          if (std::fabs(L.zli) > 1.0e-20) zt += 1. / (tpcj * L.zli);
+          if (std::fabs(L.zlr) > 1.0e-20) zt += 1. / (L.zlr * wl);
+          zt = 1. / zt;
+          break;
+        case 5:
+          zt = std::complex<double>(L.zlr, L.zli) / sl;
+          break;
+        default:
+          zt = zint(L.zlr * wl, seg.b[i]);
+          break;
+      }
+      za[i] += zt;
+    }
+    if (ichk == 0) throw NecError("LOADING DATA CARD ERROR, NO SEGMENT HAS AN ITAG = " + std::to_string(L.tag));
+  }
+  out.resize(n);
+  for (int64_t i = 0; i < n; ++i) out[i] = zmk(za[i].real(), za[i].imag());
+  return out;
 }
 
 GroundSpec Context::ground_spec() {
@@ -191,6 +332,7 @@ void Context::adjoint(const AdjointSpec& spec, AdjointResult* r, std::vector<dou
   if (src_seg_.size() != 1) throw NecError("adjoint: the objective needs exactly one voltage source");
   if (precision_ != 0) throw NecError("adjoint: needs the FP64 path (precision mode 0)");
   if (ground_now_.type != 0) throw NecError("adjoint: free space only (no GN card)");
+  if (!zload_.empty()) throw NecError("adjoint: unloaded structures only (no LD card)");
   const ScaledSegments seg = geo_.scaled(freq_mhz_);
   adjoint_gradient(geo_, seg, static_cast<int>(src_seg_[0] - 1), src_v_[0], wavelength_, rkh_, abc_, device(), spec,
                    r);
@@ -216,6 +358,8 @@ void Context::simulate(bool far_field_flag) {
       device().set_precision(precision_);
       ground_now_ = ground_spec();
       device().set_ground(ground_now_);
+      zload_ = load_impedances(geo_.scaled(freq_mhz_));
+      device().set_loads(zload_);
       device().fill(rkh_);
       device().factor();
       times_.fill_ms = device().times().fill_ms;
@@ -275,6 +419,16 @@ void Context::excitation(int mhz) {
   }
   in.segment_currents.resize(n);
   for (int i = 0; i < n; ++i) in.segment_currents[i] = zmulr(curx[i], wavelength_);
+  // ohmic loss in the loads, nec_context::compute_structure_power_loss
+  // (src/nec_context.cpp:1423-1441)
+  structure_loss_ = 0.0;
+  if (!zload_.empty())
+    for (int i = 0; i < n; ++i)
+      if (std::fabs(zload_[i].re) >= 1.e-20) {
+        const zc ci = zmulr(curx[i], wavelength_);
+        const double cm = std::hypot(ci.re, ci.im);
+        structure_loss_ += 0.5 * cm * cm * zload_[i].re * seg.s[i];
+      }
   inputs_.push_back(std::move(in));
 }
 
@@ -295,7 +449,7 @@ PatternSpec Context::pattern_spec(std::vector<double>* trig_out) const {
   sp.iavp = iavp_;
   sp.wavelength = wavelength_;
   sp.gcop = wavelength_ * wavelength_ * 2.0 * kPi / (eta * pinr);
-  const double prad = pinr - 0.0 - pnlr;
+  const double prad = pinr - structure_loss_ - pnlr;
   sp.gcon = sp.gcop;
   if (ipd_ != 0) sp.gcon = sp.gcon * pinr / prad;
   sp.rfld = rfld_;
@@ -443,6 +597,7 @@ void Context::fill_matrix(double freq_mhz, std::vector<double>* Z) {
   const double wl = wavelength_;
   wavelength_ = em_speed_of_light() / (1.0e6 * freq_mhz);
   d.set_ground(ground_spec());
+  d.set_loads(load_impedances(geo_.scaled(freq_mhz)));
   wavelength_ = wl;
   d.fill(rkh_);
   d.download_matrix(Z);

@@ -55,6 +55,9 @@ class Context {
   // GN: iperf -1 free space, 0 finite ground (reflection coefficients), 1
   // perfect ground; no radial-wire screen, no Sommerfeld ground
   void gn_card(int iperf, int nradl, double epsr, double sig, double t3, double t4, double t5, double t6);
+  // LD: types 0-5 (series / parallel RLC lumped or per metre, fixed
+  // impedance, wire conductivity); consecutive LD cards accumulate, -1 clears
+  void ld_card(int type, int tag, int tagf, int tagt, double zlr, double zli, double zlc);
   void ek_card(bool on);
 
   // results
@@ -112,6 +115,16 @@ class Context {
   // ground of the current frequency (nec_ground::calculate_antenna_environment)
   GroundSpec ground_spec();
   GroundSpec ground_now_;  // the ground of the last fill
+  struct LoadCard {
+    int type, tag, tagf, tagt;
+    double zlr, zli, zlc;
+  };
+  std::vector<LoadCard> loads_;
+  // lambda * impedance per unit length of every segment at the current
+  // frequency (nec_context::load); empty without loads
+  std::vector<zc> load_impedances(const ScaledSegments& seg) const;
+  std::vector<zc> zload_;          // of the last fill
+  double structure_loss_ = 0.0;    // ohmic loss in the loads (W), last solve
   int precision_ = 0;
   // voltage sources (EX 0)
   std::vector<int64_t> src_seg_;

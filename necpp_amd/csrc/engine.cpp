@@ -236,6 +236,19 @@ void DeviceSolver::load(const Geometry& geo, const ScaledSegments& seg, bool use
   hip_check(hipStreamSynchronize(stream_), "load");
 }
 
+void DeviceSolver::set_loads(const std::vector<zc>& z) {
+  has_loads_ = !z.empty();
+  if (!has_loads_) return;
+  if ((int)z.size() != n_) throw NecError("loads: one impedance per segment expected");
+  std::vector<double> v(2 * z.size());
+  for (size_t i = 0; i < z.size(); ++i) {
+    v[2 * i] = z[i].re;
+    v[2 * i + 1] = z[i].im;
+  }
+  upload(zload_dev_, v, stream_);
+  hip_check(hipStreamSynchronize(stream_), "loads");  // v is released on return
+}
+
 void DeviceSolver::enqueue_fill(double rkh) {
   const int n = n_;
   Z_.ensure((size_t)ld_ * n * 16);
@@ -244,8 +257,9 @@ void DeviceSolver::enqueue_fill(double rkh) {
   const DevSegments g = dev_segments();
   for (int i0 = 0; i0 < n; i0 += rows) {
     const int r = std::min(rows, n - i0);
-    if (launch_fill(g, n, i0, r, rkh, use_ek_ ? 1 : 0, ekind_.as<int>(), ground_, E_.as<double>(),
-                    colptr_.as<int>(), colj_.as<int>(), colabc_.as<double>(), Z_.as<double>(), ld_, stream_))
+    if (launch_fill(g, n, i0, r, rkh, use_ek_ ? 1 : 0, ekind_.as<int>(), ground_,
+                    has_loads_ ? zload_dev_.as<double>() : nullptr, E_.as<double>(), colptr_.as<int>(),
+                    colj_.as<int>(), colabc_.as<double>(), Z_.as<double>(), ld_, stream_))
       hip_check(hipGetLastError(), "fill launch");
   }
 }
@@ -455,6 +469,7 @@ void DeviceSolver::adjoint(double rkh, const std::vector<zc>& g, std::vector<zc>
   if (precision_ != 0) throw NecError("the adjoint needs the FP64 factors (precision mode 0)");
   if (use_ek_) throw NecError("the adjoint supports the thin-wire kernel only (no EK card)");
   if (ground_.type != 0) throw NecError("the adjoint supports free space only (no GN card)");
+  if (has_loads_) throw NecError("the adjoint supports unloaded structures only (no LD card)");
   const int n = n_;
   if ((int)g.size() != n) throw NecError("adjoint: right-hand side has the wrong length");
   adj_g_.ensure((size_t)n * 16);

@@ -119,6 +119,9 @@ class DeviceSolver {
   // refinement against Z until |b - Z x| <= refine_tol |b|).
   // ground of the next fills (the image term of efld)
   void set_ground(const GroundSpec& g) { ground_ = g; }
+  // loads of the next fills: lambda * impedance per unit length of every
+  // segment (n values), or none (empty)
+  void set_loads(const std::vector<zc>& z);
   const GroundSpec& ground() const { return ground_; }
   void set_precision(int mode) { precision_ = mode; }
   int precision() const { return precision_; }
@@ -148,6 +151,8 @@ class DeviceSolver {
   long long ld_ = 0;
   bool use_ek_ = false;
   GroundSpec ground_;
+  DevBuf zload_dev_;
+  bool has_loads_ = false;
   bool factored_ = false;
   int lu_info_ = 0;
   unsigned epoch_ = 0;

@@ -572,6 +572,7 @@ __global__ void __launch_bounds__(256) fill_assemble_kernel(const double* __rest
                                                             const int* __restrict__ colptr,
                                                             const int* __restrict__ colj,
                                                             const double* __restrict__ colabc,
+                                                            const double* __restrict__ zload,
                                                             double* __restrict__ Z, long long ldz) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;
@@ -587,6 +588,13 @@ __global__ void __launch_bounds__(256) fill_assemble_kernel(const double* __rest
     const double ti = ((e[1] * a) + (e[3] * b)) + (e[5] * c);
     accr += tr;
     acci += ti;
+    if (zload && colj[q] == i0 + r) {
+      // the load of segment i, right after its own term
+      // (src/nec_context.cpp:2126-2142): -= (A + C) z
+      const double s = a + c;
+      accr -= s * zload[2 * (i0 + r)];
+      acci -= s * zload[2 * (i0 + r) + 1];
+    }
   }
   double* zp = Z + 2 * ((size_t)(i0 + r) * ldz + k);
   zp[0] = accr;
@@ -937,8 +945,8 @@ int launch_batch_fill(const BatchSolve* sol, const int* blk, int nblk, int R, in
 }
 
 int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind,
-                const GroundSpec& ground, double* E, const int* colptr, const int* colj, const double* colabc,
-                double* Z, long long ldz, hipStream_t stream) {
+                const GroundSpec& ground, const double* zload, double* E, const int* colptr, const int* colj,
+                const double* colabc, double* Z, long long ldz, hipStream_t stream) {
   if (n <= 0 || rows <= 0) return 0;
   dim3 block(256);
   dim3 grid((n + 255) / 256, rows);
@@ -948,7 +956,8 @@ int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int u
   else
     launch_k(K_FILL_FIELDS, fill_fields_kernel<false>, grid, block, 0, stream, g, n, i0, rows, rkh, use_ek, ekind,
              ground, E);
-  launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel, grid, block, 0, stream, E, n, i0, rows, colptr, colj, colabc, Z, ldz);
+  launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel, grid, block, 0, stream, E, n, i0, rows, colptr, colj, colabc,
+           zload, Z, ldz);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

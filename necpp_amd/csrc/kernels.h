@@ -145,10 +145,12 @@ int launch_batch_adjoint_fill(const BatchSolve* sol, int S, const int* blk, cons
 
 // fill.hip: fields of rows [i0, i0+rows) into E (rows*n*6 doubles), then
 // gather into Z (row-major complex, leading dimension ldz complex entries).
-// With a ground each pair adds the image term of efld.
+// With a ground each pair adds the image term of efld.  zload (2 doubles per
+// segment, or null): loads, subtracted on each loaded segment's row right
+// after its own term (cmset's order).
 int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind,
-                const GroundSpec& ground, double* E, const int* colptr, const int* colj, const double* colabc,
-                double* Z, long long ldz, hipStream_t stream);
+                const GroundSpec& ground, const double* zload, double* E, const int* colptr, const int* colj,
+                const double* colabc, double* Z, long long ldz, hipStream_t stream);
 
 // lu.hip: in-place partial-pivot LU of the row-major complex n x n matrix A
 // (leading dimension lda complex).  piv (2n ints): piv[k] = row exchanged

@@ -103,6 +103,13 @@ struct ora_geom {
   int gtype;
   double epsr, sig;
   double zr_re, zr_im, fr_re, fr_im;
+  /* loads (LD cards) and, per frequency, lambda * impedance per unit length
+   * of every segment (2 doubles each; set by ora_frequency_scale) */
+  int nload, load_err;
+  int32_t *ldtyp, *ldtag, *ldtagf, *ldtagt;
+  double *zlr, *zli, *zlc;
+  double* zarr;
+  double wl;
 };
 
 static void* xrealloc(void* p, size_t n) {
@@ -119,6 +126,8 @@ void ora_geom_free(ora_geom* g) {
                   &g->xu, &g->yu, &g->zu, &g->su, &g->bu, &g->xs, &g->ys, &g->zs, &g->ss, &g->bs};
   for (size_t i = 0; i < sizeof d / sizeof d[0]; i++) free(*d[i]);
   free(g->tag); free(g->icon1); free(g->icon2);
+  free(g->ldtyp); free(g->ldtag); free(g->ldtagf); free(g->ldtagt);
+  free(g->zlr); free(g->zli); free(g->zlc); free(g->zarr);
   free(g);
 }
 
@@ -351,6 +360,157 @@ int64_t ora_segment_number(const ora_geom* g, int tag, int m) {
   return -1;
 }
 
+/* nec_context::zint, src/nec_context.cpp:6321-6406: internal impedance
+ * per unit length of a round wire of conductivity * wavelength sigl and
+ * radius rolam (wavelengths): the Kelvin-function ratio by its polynomial
+ * approximations for x <= 8, the asymptotic expansions with the theta / phi
+ * series for x > 8, and the skin-effect limit for x > 110. */
+static double complex zint(double sigl, double rolam) {
+  const double complex cc1 = CMPLX(6.0e-7, 1.9e-6), cc2 = CMPLX(-3.4e-6, 5.1e-6), cc3 = CMPLX(-2.52e-5, 0.0);
+  const double complex cc4 = CMPLX(-9.06e-5, -9.01e-5), cc5 = CMPLX(0., -9.765e-4), cc6 = CMPLX(.0110486, -.0110485);
+  const double complex cc7 = CMPLX(0., -.3926991), cc8 = CMPLX(1.6e-6, -3.2e-6), cc9 = CMPLX(1.17e-5, -2.4e-6);
+  const double complex cc10 = CMPLX(3.46e-5, 3.38e-5), cc11 = CMPLX(5.0e-7, 2.452e-4);
+  const double complex cc12 = CMPLX(-1.3813e-3, 1.3811e-3), cc13 = CMPLX(-6.25001e-2, -1.0e-7);
+  const double complex cc14 = CMPLX(.7071068, .7071068), cn = cc14;
+  const double tpcmu = 2.368705e+3, cmotp = 60.00;
+  const double complex j1 = CMPLX(0.0, 1.0);
+  double x = sqrt(tpcmu * sigl) * rolam;
+#define ORA_TH(d) ((((((cc1 * (d) + cc2) * (d) + cc3) * (d) + cc4) * (d) + cc5) * (d) + cc6) * (d) + cc7)
+#define ORA_PH(d) ((((((cc8 * (d) + cc9) * (d) + cc10) * (d) + cc11) * (d) + cc12) * (d) + cc13) * (d) + cc14)
+  if (x > 110.0) {
+    double complex br1 = CMPLX(0.70710678, -0.70710678);
+    return j1 * sqrt(cmotp / sigl) * br1 / rolam;
+  }
+  if (x > 8.0) {
+    double complex fx = sqrt((PI_T / 2.0) / x) * cexp(-cn * x + ORA_TH(-8. / x));
+    double complex gx = cexp(cn * x + ORA_TH(8. / x)) / sqrt(TWO_PI * x);
+    double complex br2 = j1 * fx / PI_T;
+    double complex br1 = gx + br2;
+    br2 = gx * ORA_PH(8.0 / x) - br2 * ORA_PH(-8.0 / x);
+    br1 = br1 / br2;
+    return j1 * sqrt(cmotp / sigl) * br1 / rolam;
+  }
+#undef ORA_TH
+#undef ORA_PH
+  double x8 = x / 8.0;
+  double y = x8 * x8;
+  double s = y * y;
+  double ber = ((((((-9.01e-6 * s + 1.22552e-3) * s - .08349609) * s + 2.6419140) * s - 32.363456) * s + 113.77778) * s - 64.) * s + 1.;
+  double bei = ((((((1.1346e-4 * s - .01103667) * s + .52185615) * s - 10.567658) * s + 72.817777) * s - 113.77778) * s + 16.) * y;
+  double complex br1 = CMPLX(ber, bei);
+  ber = (((((((-3.94e-6 * s + 4.5957e-4) * s - 0.02609253) * s + 0.66047849) * s - 6.0681481) * s + 14.222222) * s - 4.0) * y) * x;
+  bei = ((((((4.609e-5 * s - 3.79386e-3) * s + .14677204) * s - 2.3116751) * s + 11.377778) * s - 10.666667) * s + .5) * x;
+  double complex br2 = CMPLX(ber, bei);
+  br1 = br1 / br2;
+  return j1 * sqrt(cmotp / sigl) * br1 / rolam;
+}
+
+/* nec_context::load, src/nec_context.cpp:1861-2055 (nop = 1): lambda *
+ * impedance per unit length of every loaded segment at the current
+ * scaling (lengths and radii in wavelengths), summed over LD cards. */
+static void compute_loads(ora_geom* g) {
+  g->load_err = 0;
+  if (g->nload == 0) return;
+  int64_t n = g->n;
+  g->zarr = (double*)xrealloc(g->zarr, (size_t)(2 * n) * sizeof(double));
+  double complex* za = (double complex*)g->zarr;
+  for (int64_t i = 0; i < n; i++) za[i] = 0.0;
+  const double complex tpcj = CMPLX(0.0, 1.883698955e+9);
+  const double wl = g->wl;
+  for (int k = 0; k < g->nload; k++) {
+    if (g->ldtyp[k] > 5) { set_err("IMPROPER LOAD TYPE CHOSEN, REQUESTED TYPE IS %d", g->ldtyp[k]); g->load_err = 1; return; }
+    int ldtags = g->ldtag[k], jump = g->ldtyp[k] + 1, ichk = 0;
+    int64_t l1 = 1, l2 = n;
+    if (ldtags == 0 && (g->ldtagf[k] != 0 || g->ldtagt[k] != 0)) { l1 = g->ldtagf[k]; l2 = g->ldtagt[k]; }
+    for (int64_t i = l1 - 1; i < l2; i++) {
+      if (ldtags != 0) {
+        if (ldtags != g->tag[i]) continue;
+        if (g->ldtagf[k] != 0) {
+          ichk++;
+          if (ichk < g->ldtagf[k] || ichk > g->ldtagt[k]) continue;
+        } else {
+          ichk = 1;
+        }
+      } else {
+        ichk = 1;
+      }
+      double sl = g->ss[i];
+      double zlr = g->zlr[k], zli = g->zli[k], zlc = g->zlc[k];
+      double complex zt = 0.0;
+      switch (jump) {
+        case 1:
+          zt = zlr / sl + tpcj * zli / (sl * wl);
+          if (fabs(zlc) > 1.0e-20) zt += wl / (tpcj * sl * zlc);
+          break;
+        case 2:
+          zt = tpcj * sl * zlc / wl;
+          if (fabs(zli) > 1.0e-20) zt += sl * wl / (tpcj * zli);
+          if (fabs(zlr) > 1.0e-20) zt += sl / zlr;
+          zt = 1. / zt;
+          break;
+        case 3:
+          zt = zlr * wl + tpcj * zli;
+          if (fabs(zlc) > 1.0e-20) zt += 1. / (tpcj * sl * sl * zlc);
+          break;
+        case 4:
+          zt = tpcj * sl * sl * zlc;
+          if (fabs(zli) > 1.0e-20) zt += 1. / (tpcj * zli);
+          if (fabs(zlr) > 1.0e-20) zt += 1. / (zlr * wl);
+          zt = 1. / zt;
+          break;
+        case 5:
+          zt = CMPLX(zlr, zli) / sl;
+          break;
+        case 6:
+          zt = zint(zlr * wl, g->bs[i]);
+          break;
+      }
+      za[i] += zt;
+    }
+    if (ichk == 0) {
+      set_err("LOADING DATA CARD ERROR, NO SEGMENT HAS AN ITAG = %d", ldtags);
+      g->load_err = 1;
+      return;
+    }
+  }
+}
+
+/* Test hook: zint (out[2] = re, im). */
+void ora_zint(double sigl, double rolam, double* out) {
+  double complex z = zint(sigl, rolam);
+  out[0] = creal(z);
+  out[1] = cimag(z);
+}
+
+/* nec_context::ld_card, src/nec_context.cpp:424-474: count 0 clears. */
+int ora_set_loads(ora_geom* g, int count, const int32_t* type, const int32_t* tag, const int32_t* tagf,
+                  const int32_t* tagt, const double* zlr, const double* zli, const double* zlc) {
+  g->nload = 0;
+  if (count <= 0) return 0;
+  for (int k = 0; k < count; k++) {
+    int f = tagf[k], t = tagt[k] == 0 ? tagf[k] : tagt[k];
+    if (t < f) { set_err("DATA FAULT ON LOADING CARD No: %d: ITAG STEP1: %d IS GREATER THAN ITAG STEP2: %d", k + 1, f, t); return 1; }
+  }
+  g->ldtyp = (int32_t*)xrealloc(g->ldtyp, (size_t)count * sizeof(int32_t));
+  g->ldtag = (int32_t*)xrealloc(g->ldtag, (size_t)count * sizeof(int32_t));
+  g->ldtagf = (int32_t*)xrealloc(g->ldtagf, (size_t)count * sizeof(int32_t));
+  g->ldtagt = (int32_t*)xrealloc(g->ldtagt, (size_t)count * sizeof(int32_t));
+  g->zlr = (double*)xrealloc(g->zlr, (size_t)count * sizeof(double));
+  g->zli = (double*)xrealloc(g->zli, (size_t)count * sizeof(double));
+  g->zlc = (double*)xrealloc(g->zlc, (size_t)count * sizeof(double));
+  for (int k = 0; k < count; k++) {
+    g->ldtyp[k] = type[k];
+    g->ldtag[k] = tag[k];
+    g->ldtagf[k] = tagf[k];
+    g->ldtagt[k] = tagt[k] == 0 ? tagf[k] : tagt[k];
+    g->zlr[k] = zlr[k];
+    g->zli[k] = zli[k];
+    g->zlc[k] = zlc[k];
+  }
+  g->nload = count;
+  return 0;
+}
+
 /* c_geometry::frequency_scale, src/c_geometry.cpp:3389-3418. */
 void ora_frequency_scale(ora_geom* g, double freq_mhz) {
   double fr = (1.0e6 * freq_mhz) / ora_speed_of_light();
@@ -362,6 +522,8 @@ void ora_frequency_scale(ora_geom* g, double freq_mhz) {
    * frati = 1; for the finite ground zrati = 1 / sqrt(epsc),
    * epsc = epsr - j sig lambda eta / (2 pi); a negative sig is replaced by
    * -sig / (eta lambda / (2 pi)) and kept. */
+  g->wl = ora_speed_of_light() / (freq_mhz * 1.0e6);
+  compute_loads(g);
   g->fr_re = 1.0; g->fr_im = 0.0;
   if (g->gtype == 2) {
     double wavelength = ora_speed_of_light() / (freq_mhz * 1.0e6);
@@ -1171,6 +1333,7 @@ int ora_fill(const ora_geom* g, double rkh, int use_ek, double* Z, int nthreads,
   }
   ora_fill_counts total;
   memset(&total, 0, sizeof total);
+  if (g->nload > 0 && (g->load_err || !g->zarr)) bad = 1;
   if (!bad) {
     if (nthreads < 1) nthreads = 1;
 #pragma omp parallel num_threads(nthreads)
@@ -1212,6 +1375,17 @@ int ora_fill(const ora_geom* g, double rkh, int use_ek, double* Z, int nthreads,
             row[2 * k] += t.re;
             row[2 * k + 1] += t.im;
             local.scatter_terms++;
+          }
+          /* loading of segment j, right after its own scatter
+           * (cmset, src/nec_context.cpp:2126-2142): row j, bases of trio(j) */
+          if (g->nload > 0 && j == i) {
+            const double zr = g->zarr[2 * i], zi = g->zarr[2 * i + 1];
+            for (int q = 0; q < tn[j]; q++) {
+              int64_t k = jj[q] - 1;
+              double sab = ta[j * TRIO_CAP + q] + tc[j * TRIO_CAP + q];
+              row[2 * k] -= sab * zr;
+              row[2 * k + 1] -= sab * zi;
+            }
           }
         }
       }
@@ -1670,7 +1844,17 @@ int ora_run_voltage(ora_geom* g, double freq_mhz, double rkh, int use_ek,
   if (current)
     for (int64_t i = 0; i < n; i++) { current[2 * i] = e[2 * i] * wl; current[2 * i + 1] = e[2 * i + 1] * wl; }
   double t3 = now_s();
-  if (rp && rp_out) ora_pattern(g, abc, wl, pin, 0.0, rp, rp_out, nthreads);
+  /* ohmic loss in the loads, nec_context::compute_structure_power_loss
+   * (src/nec_context.cpp:1423-1441); the pattern's radiated power is the
+   * input power less it (prad, used for ipd = 1) */
+  double ploss = 0.0;
+  if (g->nload > 0)
+    for (int64_t i = 0; i < n; i++)
+      if (fabs(g->zarr[2 * i]) >= 1.e-20) {
+        double cm = hypot(e[2 * i] * wl, e[2 * i + 1] * wl);
+        ploss += 0.5 * cm * cm * g->zarr[2 * i] * g->ss[i];
+      }
+  if (rp && rp_out) ora_pattern(g, abc, wl, pin, ploss, rp, rp_out, nthreads);
   double t4 = now_s();
   if (timing) { timing->fill_s = t1 - t0; timing->factor_s = t2 - t1; timing->solve_s = t3 - t2; timing->pattern_s = t4 - t3; }
   if (!abc_out) free(abc);

@@ -59,6 +59,19 @@ void ora_segments(const ora_geom* g, double* out, int32_t* icon1, int32_t* icon2
  * 0 = ok. */
 int ora_set_ground(ora_geom* g, int iperf, double epsr, double sig);
 
+/* nec_context::ld_card (src/nec_context.cpp:424-474) for count cards (0
+ * clears): types 0-5 (series / parallel RLC, per metre or lumped, fixed
+ * impedance, wire conductivity), tag and segment range as the card.  The
+ * impedances are formed at the next ora_frequency_scale (load(),
+ * :1861-2055); the fill subtracts them on each loaded segment's row
+ * (cmset :2126-2142) and the pattern's radiated power excludes their loss.
+ * 0 = ok. */
+/* Test hook: nec_context::zint (src/nec_context.cpp:6321-6406), lambda *
+ * internal impedance per unit length of a round wire; out[2]. */
+void ora_zint(double sigl, double rolam, double* out);
+int ora_set_loads(ora_geom* g, int count, const int32_t* type, const int32_t* tag, const int32_t* tagf,
+                  const int32_t* tagt, const double* zlr, const double* zli, const double* zlc);
+
 /* c_geometry::get_segment_number (src/c_geometry.cpp:76-99): 1-based segment
  * number, or -1 when no such tag (error text set). */
 int64_t ora_segment_number(const ora_geom* g, int tag, int m);

@@ -42,6 +42,8 @@ def lib():
         L.ora_impedance.restype = d
         L.ora_frequency_scale.argtypes = [p, d]
         L.ora_set_ground.argtypes = [p, i, d, d]
+        L.ora_set_loads.argtypes = [p, i, p, p, p, p, p, p, p]
+        L.ora_zint.argtypes = [d, d, p]
         L.ora_trio.argtypes = [p, i, p, p, p, p, i]
         L.ora_tbf.argtypes = [p, i, i, p, p, p, p, i]
         L.ora_fill.argtypes = [p, d, i, p, i, p]
@@ -128,6 +130,12 @@ def efld(src, obs, ai, not_self=1, rkh=1.0, use_ek=False, ind=(2, 2)):
     return out[0::2] + 1j * out[1::2], r
 
 
+def zint(sigl, rolam):
+    out = np.zeros(2)
+    lib().ora_zint(sigl, rolam, _ptr(out))
+    return complex(out[0], out[1])
+
+
 def wavelength(f_mhz):
     return lib().ora_wavelength(f_mhz)
 
@@ -174,6 +182,16 @@ class Geometry:
         """GN card: -1 free space, 0 finite ground (reflection coefficients),
         1 perfect ground."""
         if self._L.ora_set_ground(self._g, iperf, epsr, sig) != 0:
+            raise RuntimeError(self._L.ora_last_error().decode())
+
+    def set_loads(self, cards):
+        """LD cards: list of (type, tag, tag_from, tag_to, zlr, zli, zlc);
+        an empty list clears."""
+        k = len(cards)
+        cols = list(zip(*cards)) if k else [()] * 7
+        ints = [np.ascontiguousarray(c, np.int32) for c in cols[:4]]
+        dbls = [np.ascontiguousarray(c, np.float64) for c in cols[4:]]
+        if self._L.ora_set_loads(self._g, k, *[_ptr(a) for a in ints + dbls]) != 0:
             raise RuntimeError(self._L.ora_last_error().decode())
 
     def scale(self, f_mhz):
