@@ -66,7 +66,7 @@ long nec_gn_card(nec_context* in_context, int iperf, int nradl, double epse, dou
 long nec_fr_card(nec_context* in_context, int in_ifrq, int in_nfrq, double in_freq_mhz, double in_del_freq);
 /* src/libnecpp.h:282, src/libNEC.cpp:262-265 */
 long nec_ek_card(nec_context* in_context, int itmp1);
-/* src/libnecpp.h:303 (LD card; not on the GPU path) */
+/* src/libnecpp.h:303, src/libNEC.cpp:171-174 -> nec_context::ld_card (types 0-5; the loads enter the fill and the power budget) */
 long nec_ld_card(nec_context* in_context, int ldtyp, int ldtag, int ldtagf, int ldtagt,
                  double tmp1, double tmp2, double tmp3);
 /* src/libnecpp.h:330, src/libNEC.cpp:182-185 -> nec_context::ex_card (type 0 only) */
