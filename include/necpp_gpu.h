@@ -168,6 +168,16 @@ long nec_gpu_batch_adjoint(const nec_gpu_batch* batch, double theta_deg, double 
 long long necgpu_zgetrf_workspace(int n);
 long necgpu_zgetrf(double* a, long long lda, int n, int* piv, int* info, void* work, void* stream);
 
+/* The mixed path's complex FP32 LU on caller device memory: a is n x n
+ * complex64 (row-major, leading dimension lda), factored in place; piv,
+ * info as for necgpu_zgetrf; work: necgpu_zgetrf_workspace(n) bytes.
+ * pivot_mode 0: partial pivoting (first maximum of |z| in the column);
+ * 1: pivots searched inside each 64-row diagonal block only, so P A = L U
+ * with P permuting inside 64-row blocks (the default of the mixed solve,
+ * whose FP64 refinement makes up for the weaker pivoting).  Returns 1 if n
+ * exceeds the largest order the panel supports (57,344). */
+long necgpu_cgetrf(float* a, long long lda, int n, int* piv, int* info, void* work, int pivot_mode, void* stream);
+
 /* Solve with the factors of necgpu_zgetrf: x (device, n complex) is
  * overwritten.  work: device buffer of necgpu_zgetrs_workspace(n) bytes,
  * zero-initialised once; epoch: strictly increasing per call on the same

@@ -470,6 +470,15 @@ long necgpu_zgetrf(double* a, long long lda, int n, int* piv, int* info, void* w
   });
 }
 
+long necgpu_cgetrf(float* a, long long lda, int n, int* piv, int* info, void* work, int pivot_mode, void* stream) {
+  return guarded([&] {
+    if (pivot_mode != 0 && pivot_mode != 1) throw NecError("necgpu_cgetrf: pivot_mode must be 0 or 1");
+    const int rc = necgpu::launch_lu_fp32(a, lda, n, piv, info, work, static_cast<hipStream_t>(stream), pivot_mode);
+    if (rc == 2) throw NecError("necgpu_cgetrf: n too large (the panel needs one CU per 224 rows, at most 256)");
+    if (rc) necgpu::hip_check(hipGetLastError(), "necgpu_cgetrf");
+  });
+}
+
 long necgpu_check_mfma_f64(const double* a, const double* b, double* d, void* stream) {
   return guarded([&] {
     if (necgpu::launch_mfma_probe(a, b, d, static_cast<hipStream_t>(stream)))

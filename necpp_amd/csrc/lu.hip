@@ -1019,6 +1019,235 @@ __global__ void __launch_bounds__(256) zgemm_update_kernel(double* __restrict__ 
   }
 }
 
+// The same update, C[m0.., n0..] (M x N) -= L * U with K <= 64, as a
+// persistent kernel of 256-thread workgroups that stay resident (FP64: one
+// per CU, LDS 133 KB; FP32: two per CU, 66.5 KB each).  LDS holds the whole
+// 64 x K strips of L and U.  Per tile the workgroup
+//   - writes the strips it prefetched into registers to LDS, stores the
+//     previous tile's result, and moves the prefetched C block into the
+//     accumulators,
+//   - runs 16 k-steps of 16 MFMAs per wave (32 x 32 complex per wave, the
+//     complex product in 4M form) with the accumulators counting down from
+//     C: acc_re += (-Lr) Ur + Li Ui, acc_im += (-Lr) Ui + (-Li) Ur; each
+//     k-step also issues three of the 48 loads of the next tile's strips and
+//     C block, so the loads trickle out beside the MFMAs instead of stalling
+//     them in one burst.
+// Storing a tile's result after the next tile's operands are in LDS keeps
+// the waits for the prefetched loads from waiting on stores.  Loads and
+// stores are buffer operations: an element outside the matrix gets an offset
+// beyond the buffer's range, which reads as zero and drops the write, so the
+// tile loop has no branches.  Tiles are taken in bands of 8 tile rows,
+// column by column inside a band; each XCD (launch index mod 8) owns a
+// contiguous range of that order, so the tiles its CUs hold at one time
+// share a few L and U strips in its L2.
+static_assert(NB == 64, "zgemm_tile_kernel stages K <= 64");
+constexpr int ZT_SP = 65;  // LDS row stride in complex (odd: the transposing L store is conflict-free)
+typedef double dbl2_t __attribute__((ext_vector_type(2)));
+typedef float flt2_t __attribute__((ext_vector_type(2)));
+typedef unsigned int u32x4_t __attribute__((ext_vector_type(4)));
+typedef unsigned int u32x2_t __attribute__((ext_vector_type(2)));
+constexpr int ZT_OUT = 0x7fffffff;   // buffer range (bytes); every in-tile offset is below it
+constexpr unsigned ZT_POISON = 0x80000000u;  // an offset beyond the range: reads 0, drops writes
+constexpr int ZT_RSRC_W3 = 0x00020000;  // buffer resource word 3 for gfx9-family raw buffers
+
+// Element traits: complex pair, accumulator, MFMA and the accumulator's
+// row for register v of lane group lk.
+template <typename T>
+struct ZtTraits;
+template <>
+struct ZtTraits<double> {
+  typedef dbl2_t C2;
+  typedef double4_t Acc;
+  static constexpr int kPerCU = 1;
+  __device__ static Acc mfma(double a, double b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
+  // FP64 16x16x4: register v of lane l holds D[(l / 16) + 4 v][l % 16]
+  __device__ static int acc_row(int v, int lk) { return v * 4 + lk; }
+  __device__ static C2 load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(C2, __builtin_amdgcn_raw_buffer_load_b128(r, off, 0, 0));
+  }
+  __device__ static void store(C2 d, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(u32x4_t, d), r, off, 0, 0);
+  }
+};
+template <>
+struct ZtTraits<float> {
+  typedef flt2_t C2;
+  typedef float4_t Acc;
+  static constexpr int kPerCU = 2;
+  __device__ static Acc mfma(float a, float b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
+  // FP32 16x16x4: register v of lane l holds D[4 (l / 16) + v][l % 16]
+  __device__ static int acc_row(int v, int lk) { return lk * 4 + v; }
+  __device__ static C2 load(__amdgpu_buffer_rsrc_t r, unsigned off) {
+    return __builtin_bit_cast(C2, __builtin_amdgcn_raw_buffer_load_b64(r, off, 0, 0));
+  }
+  __device__ static void store(C2 d, __amdgpu_buffer_rsrc_t r, unsigned off) {
+    __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2_t, d), r, off, 0, 0);
+  }
+};
+
+template <typename T>
+constexpr size_t zt_lds_bytes() {
+  return 2u * 64u * ZT_SP * sizeof(typename ZtTraits<T>::C2);
+}
+
+__device__ inline void zt_coords(int idx, int tiles_x, int tiles_y, int* ty, int* tx) {
+  const int band = idx / (8 * tiles_x);
+  const int hb = min(8, tiles_y - 8 * band);
+  const int r = idx - band * 8 * tiles_x;
+  *tx = r / hb;
+  *ty = 8 * band + r % hb;
+}
+
+template <typename T>
+__device__ inline __amdgpu_buffer_rsrc_t zt_rsrc(const T* base, int range) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), (short)0, range, ZT_RSRC_W3);
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(ZtTraits<T>::kPerCU, ZtTraits<T>::kPerCU)))
+zgemm_tile_kernel(T* __restrict__ A, long long lda, int m0, int n0, int M, int N, int k0, int K) {
+  typedef ZtTraits<T> Tr;
+  typedef typename Tr::C2 C2;
+  typedef typename Tr::Acc Acc;
+  extern __shared__ __align__(16) unsigned char zt_raw[];
+  C2* sL = reinterpret_cast<C2*>(zt_raw);  // [k][row]
+  C2* sU = sL + 64 * ZT_SP;                 // [k][col]
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int wr = wave >> 1, wc = wave & 1;
+  const int li = lane & 15, lk = lane >> 4;
+  const int tiles_x = (N + 63) / 64, tiles_y = (M + 63) / 64;
+  const int ntiles = tiles_x * tiles_y;
+  const int G = gridDim.x;
+  const int x = blockIdx.x % 8, y = blockIdx.x / 8, per = G / 8, extra = G % 8;
+  const int gx = per + (x < extra ? 1 : 0);
+  const int before = x * per + min(x, extra);
+  const int tend = (int)((long long)ntiles * (before + gx) / G);
+  int tile = (int)((long long)ntiles * before / G) + y;
+  if (tile >= tend) return;
+  const unsigned esz = 2u * sizeof(T);
+  const unsigned row_bytes = (unsigned)(lda * esz);
+  // Per-tile load sources and offsets.  Slot i (0..15) of this thread:
+  //   L: row wave*16+i of the strip, column `lane`
+  //   U: row wave*16+i of the strip, column `lane`
+  //   C: accumulator element i = (a b v) of the wave's 32 x 32 block
+  __amdgpu_buffer_rsrc_t rl, ru, rc;
+  unsigned ol[16], ou[16], oc[16];
+  auto plan = [&](int tl, bool live) __attribute__((always_inline)) {
+    int ty, tx;
+    zt_coords(tl, tiles_x, tiles_y, &ty, &tx);
+    const int row0 = ty * 64, col0 = tx * 64;
+    const int range = live ? ZT_OUT : 0;
+    rl = zt_rsrc(A + 2 * ((size_t)(m0 + row0) * lda + k0), range);
+    ru = zt_rsrc(A + 2 * ((size_t)k0 * lda + n0 + col0), range);
+    rc = zt_rsrc(A + 2 * ((size_t)(m0 + row0) * lda + n0 + col0), range);
+    const int mr = M - row0, nc = N - col0;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      const int r = wave * 16 + i;
+      ol[i] = (r < mr && lane < K) ? r * row_bytes + lane * esz : ZT_POISON;
+      ou[i] = (r < K && lane < nc) ? r * row_bytes + lane * esz : ZT_POISON;
+      const int a = i >> 3, b = (i >> 2) & 1, v = i & 3;
+      const int cr = wr * 32 + a * 16 + Tr::acc_row(v, lk), cc = wc * 32 + b * 16 + li;
+      oc[i] = (cr < mr && cc < nc) ? cr * row_bytes + cc * esz : ZT_POISON;
+    }
+  };
+  C2 pl[16], pu[16], pc[16];
+  plan(tile, true);
+#pragma unroll
+  for (int i = 0; i < 16; ++i) {
+    pl[i] = Tr::load(rl, ol[i]);
+    pu[i] = Tr::load(ru, ou[i]);
+    pc[i] = Tr::load(rc, oc[i]);
+  }
+  // The first pass stores through a zero-range resource (dropped).
+  Acc accr[2][2], acci[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      accr[a][b] = (Acc){T(0), T(0), T(0), T(0)};
+      acci[a][b] = (Acc){T(0), T(0), T(0), T(0)};
+    }
+  __amdgpu_buffer_rsrc_t rs = zt_rsrc(A, 0);
+  unsigned os[16];
+#pragma unroll
+  for (int i = 0; i < 16; ++i) os[i] = ZT_POISON;
+  for (;;) {
+    __syncthreads();  // the previous tile's operands are no longer read
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+      sL[lane * ZT_SP + wave * 16 + i] = pl[i];
+      sU[(wave * 16 + i) * ZT_SP + lane] = pu[i];
+    }
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) Tr::store((C2){accr[a][b][v], acci[a][b][v]}, rs, os[(a * 2 + b) * 4 + v]);
+#pragma unroll
+    for (int a = 0; a < 2; ++a)
+#pragma unroll
+      for (int b = 0; b < 2; ++b) {
+        const C2* p = pc + (a * 2 + b) * 4;
+        accr[a][b] = (Acc){p[0].x, p[1].x, p[2].x, p[3].x};
+        acci[a][b] = (Acc){p[0].y, p[1].y, p[2].y, p[3].y};
+      }
+    // this tile's store target before plan() moves on to the next tile
+    rs = rc;
+#pragma unroll
+    for (int i = 0; i < 16; ++i) os[i] = oc[i];
+    const int next = tile + gx;
+    const bool live = next < tend;
+    plan(live ? next : tile, live);
+    __syncthreads();
+    C2 xa[2], yb[2];
+#pragma unroll
+    for (int a = 0; a < 2; ++a) xa[a] = sL[lk * ZT_SP + wr * 32 + a * 16 + li];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) yb[b] = sU[lk * ZT_SP + wc * 32 + b * 16 + li];
+#pragma unroll
+    for (int s = 0; s < 16; ++s) {
+      pl[s] = Tr::load(rl, ol[s]);
+      pu[s] = Tr::load(ru, ou[s]);
+      pc[s] = Tr::load(rc, oc[s]);
+      C2 xn[2], yn[2];
+      if (s < 15) {
+        const int q = (s + 1) * 4 + lk;
+#pragma unroll
+        for (int a = 0; a < 2; ++a) xn[a] = sL[q * ZT_SP + wr * 32 + a * 16 + li];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) yn[b] = sU[q * ZT_SP + wc * 32 + b * 16 + li];
+      }
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) {
+          accr[a][b] = Tr::mfma(-xa[a].x, yb[b].x, accr[a][b]);
+          accr[a][b] = Tr::mfma(xa[a].y, yb[b].y, accr[a][b]);
+          acci[a][b] = Tr::mfma(-xa[a].x, yb[b].y, acci[a][b]);
+          acci[a][b] = Tr::mfma(-xa[a].y, yb[b].x, acci[a][b]);
+        }
+      if (s < 15) {
+#pragma unroll
+        for (int a = 0; a < 2; ++a) xa[a] = xn[a];
+#pragma unroll
+        for (int b = 0; b < 2; ++b) yb[b] = yn[b];
+      }
+      __builtin_amdgcn_sched_barrier(0);
+    }
+    if (!live) break;
+    tile = next;
+  }
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) Tr::store((C2){accr[a][b][v], acci[a][b][v]}, rs, os[(a * 2 + b) * 4 + v]);
+}
+
 // The mixed-precision path's FP32 trailing update: zgemm_update_kernel with
 // FP32 operands on v_mfma_f32_16x16x4f32 (FP32 accumulation), whose result
 // register v of lane l holds D[4 (l / 16) + v][l % 16].  The complex product
@@ -1178,6 +1407,16 @@ int panel_stride_override() {
 // 1 while the update has fewer than 16 tiles per free CU (the panel beside
 // it then runs faster, measured at N = 3,280), else 2 (measured faster at
 // N = 16,020).  NECGPU_GEMM_OCC overrides, for measurements.
+// Trailing update kernel (NECGPU_ZGEMM, for measurements): 1 = the
+// persistent tile kernel everywhere, 0 = the per-tile kernels
+// (zgemm_update_kernel, zgemm_update_f32_kernel) everywhere, 2 (default) =
+// the tile kernel while the LU runs on one stream and the per-tile kernels
+// beside the look-ahead panels.
+int zgemm_mode() {
+  const char* v = std::getenv("NECGPU_ZGEMM");
+  return (v && *v) ? std::atoi(v) : 2;
+}
+
 int gemm_occupancy(int tiles, int free_cus) {
   const char* v = std::getenv("NECGPU_GEMM_OCC");
   const int o = (v && *v) ? std::atoi(v) : 0;
@@ -1472,9 +1711,15 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       return 1;
+    if (hipFuncSetAttribute(reinterpret_cast<const void*>(zgemm_tile_kernel<T>),
+                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)zt_lds_bytes<T>()) != hipSuccess)
+      return 1;
     attr_set = true;
   }
   if (n <= 0) return 0;
+  const int gemm_mode = zgemm_mode();
+  // buffer offsets of a 64-row tile stay below 2^31 bytes
+  if ((long long)lda * 2 * (long long)sizeof(T) * 64 >= (1ll << 31)) return 2;
   // every active panel workgroup needs its own CU at the same time (one per
   // CU by LDS), so the first (largest) panel bounds n
   const int G0 = (n + RMAX_T - 1) / RMAX_T;
@@ -1517,13 +1762,21 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
       launch_k(K_LU_LASWP, lu_laswp_kernel<T>, dim3((cols + LASWP_COLS - 1) / LASWP_COLS), dim3(256), 0, s, A, lda, k0, nb,
                (const int*)piv, c0, c1, c2, c3);
   };
-  auto update = [&](int k0, int nb, int cbeg, int cend, int max_groups, hipStream_t s) {
+  auto update = [&](int k0, int nb, int cbeg, int cend, int max_groups, hipStream_t s, bool tile_gemm) {
     // trailing rows [k0+nb, n), columns [cbeg, cend)
     const int cols = cend - cbeg, rows = n - k0 - nb;
     if (cols <= 0) return;
     launch_k(K_LU_TRSM, lu_trsm_kernel<T>, dim3((cols + 63) / 64), dim3(256), 0, s, A, lda, cend, k0, nb, cbeg);
     if (rows <= 0) return;
     const int tiles = ((rows + 63) / 64) * ((cols + 63) / 64);
+    if (tile_gemm) {
+      // persistent: as many workgroups as stay resident (LDS-bound)
+      const int resident = ZtTraits<T>::kPerCU * cu_count;
+      const int grid = std::max(1, std::min(tiles, std::min(max_groups, resident)));
+      launch_k(K_ZGEMM, zgemm_tile_kernel<T>, dim3(grid), dim3(256), (uint32_t)zt_lds_bytes<T>(), s, A, lda, k0 + nb,
+               cbeg, rows, cols, k0, nb);
+      return;
+    }
     const int grid = std::max(1, std::min(tiles, max_groups));
     if constexpr (sizeof(T) == 8)
       launch_k(K_ZGEMM, zgemm_update_kernel, dim3(grid), dim3(256), 0, s, A, lda, k0 + nb, cbeg, rows, cols, k0, nb);
@@ -1542,7 +1795,7 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
       launch_k(K_LU_NEXT, lu_next_update_kernel<T>, dim3(tiles), dim3(256), 0, s, A, lda, n, k0, nb, cbeg, cend,
                w.ticket);
     else
-      update(k0, nb, cbeg, cend, INT_MAX, s);
+      update(k0, nb, cbeg, cend, INT_MAX, s, gemm_mode == 1);
   };
   // Right-looking LU on one stream while the panels are wide (or without
   // look-ahead), then the look-ahead below from panel kla on.
@@ -1559,7 +1812,7 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
       const int next = k0 + nb;
       laswp(k0, nb, 0, k0, next, n, stream);
       if (next >= n) break;
-      update(k0, nb, next, n, INT_MAX, stream);
+      update(k0, nb, next, n, INT_MAX, stream, gemm_mode != 0);
       if (la && groups(next) <= la_max_g) {
         kla = next;
         break;
@@ -1600,7 +1853,9 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
     const int Gnext = (n - next + RMAX_T - 1) / RMAX_T;
     const int free_cus = std::max(1, cu_count - Gnext);
     const int tiles = ((n - next + 63) / 64) * ((n - far + 63) / 64);
-    update(k0, nb, far, n, gemm_occupancy(tiles, free_cus) * free_cus, stream);
+    update(k0, nb, far, n, gemm_mode == 1 ? ZtTraits<T>::kPerCU * free_cus : gemm_occupancy(tiles, free_cus) * free_cus,
+           stream,
+           gemm_mode == 1);
     if (hipEventRecord(la->rest_done, stream) != hipSuccess) return 1;
   }
   return hipGetLastError() == hipSuccess ? 0 : 1;

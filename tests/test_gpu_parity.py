@@ -114,8 +114,13 @@ def test_mfma_f32_lane_layout():
     assert found == expect
 
 
-@pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 127, 130, 257, 300])
-def test_zgetrf_zgetrs_random(n):
+# NECGPU_ZGEMM selects the FP64 trailing-update kernel: "1" the persistent
+# tile kernel everywhere (small n never reaches the one-stream phase that
+# uses it by default), "0" the two-per-CU kernel, "2" the default split
+@pytest.mark.parametrize("mode", ["2", "1", "0"])
+@pytest.mark.parametrize("n", [1, 2, 5, 63, 64, 65, 127, 130, 257, 300, 641])
+def test_zgetrf_zgetrs_random(n, mode, monkeypatch):
+    monkeypatch.setenv("NECGPU_ZGEMM", mode)
     torch = _torch()
     rng = np.random.default_rng(n)
     A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
@@ -309,3 +314,42 @@ def test_device_allocation_failure_is_reported():
     finally:
         del os.environ["NECGPU_FAIL_ALLOC"]
         api.nec_delete(ctx)
+
+
+# the mixed path's FP32 LU on its own: P A = L U to FP32 rounding, for both
+# pivot modes and every trailing-update kernel
+@pytest.mark.parametrize("gemm", ["2", "1", "0"])
+@pytest.mark.parametrize("pivot_mode", [1, 0])
+@pytest.mark.parametrize("n", [1, 5, 64, 65, 130, 300, 641, 1100])
+def test_cgetrf_factors(n, pivot_mode, gemm, monkeypatch):
+    monkeypatch.setenv("NECGPU_ZGEMM", gemm)
+    torch = _torch()
+    rng = np.random.default_rng(1000 + n)
+    A = (rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))).astype(np.complex64)
+    dA = torch.from_numpy(A.copy()).to("cuda")
+    piv = torch.zeros(2 * n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    lw = torch.empty(int(api.necgpu_zgetrf_workspace(n)), dtype=torch.uint8, device="cuda")
+    assert api.necgpu_cgetrf(dA.data_ptr(), n, n, piv.data_ptr(), info.data_ptr(), lw.data_ptr(), pivot_mode,
+                             None) == 0, api.nec_error_message()
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    LU = dA.cpu().numpy().astype(np.complex128)
+    p = piv.cpu().numpy()
+    perm = np.arange(n)
+    for k in range(n):
+        perm[[k, p[k]]] = perm[[p[k], k]]
+    assert np.array_equal(p[n:], perm)
+    if pivot_mode == 1:
+        # pivots stay inside the 64-row diagonal block of their column
+        assert np.all(p[:n] // 64 == np.arange(n) // 64)
+    L = np.tril(LU, -1) + np.eye(n)
+    U = np.triu(LU)
+    err = np.abs(L @ U - A[perm].astype(np.complex128))
+    bound = 8 * n * 6e-8 * (np.abs(L) @ np.abs(U))
+    assert np.all(err <= bound + 1e-30), float(np.max(err / (bound + 1e-30)))
+
+
+def test_cgetrf_rejects_bad_pivot_mode():
+    assert api.necgpu_cgetrf(None, 4, 4, None, None, None, 2, None) == 1
+    assert "pivot_mode" in api.nec_error_message()
