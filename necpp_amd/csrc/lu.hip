@@ -1512,11 +1512,29 @@ __device__ __forceinline__ T lane_value(T v, int src) {
 }
 
 // 1024 threads: lanes over columns, 16 waves over rows (4 rows each), so the
-// sequential column loop spends little time per wave.  Per column: every
-// wave forms its rows' new values from the old block (the pivot row p is
-// read before anyone writes it); barrier; the rows are written; barrier;
-// wave 0 finds the next column's pivot from LDS (first maximum of |z|^2, a
-// preconditioner needs no more); barrier.
+// sequential column loop spends little time per wave.  The active rows
+// (below the current column) alternate between two LDS buffers: column c
+// reads buffer c % 2 and writes its updated rows to the other one, so reads
+// and writes need no barrier between them; the finished row c (the pivot row)
+// goes to a third buffer.  After the one barrier per column, every wave
+// searches the next column's pivot itself (first maximum of |z|^2 over the
+// active rows, a preconditioner needs no more), so all waves agree on it
+// without another barrier.
+// Maximum over the wave of values that are >= 0 or the sentinel -1 (their
+// bit patterns order as signed integers), by DPP row shifts and row
+// broadcasts: lane 63 ends with the maximum, which readlane makes uniform.
+// A few cycles per step, where a shuffle tree takes an LDS round trip each.
+__device__ __forceinline__ float wave_max_nonneg(float v) {
+  int x = __float_as_int(v);
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x111, 0xf, 0xf, false));  // row_shr:1
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x112, 0xf, 0xf, false));  // row_shr:2
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x114, 0xf, 0xf, false));  // row_shr:4
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x118, 0xf, 0xf, false));  // row_shr:8
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x142, 0xa, 0xf, false));  // row_bcast:15
+  x = max(x, __builtin_amdgcn_update_dpp(x, x, 0x143, 0xc, 0xf, false));  // row_bcast:31
+  return __int_as_float(__builtin_amdgcn_readlane(x, 63));
+}
+
 constexpr int DG_THREADS = 1024;
 constexpr int DG_WAVES = DG_THREADS / 64;
 constexpr int DG_ROWS = NB / DG_WAVES;  // rows per wave
@@ -1524,11 +1542,12 @@ template <typename T>
 __global__ void __launch_bounds__(DG_THREADS) lu_diag_block_kernel(T* __restrict__ A, long long lda, int n, int k0,
                                                                     int nb, int* piv, int* perm, int* info) {
   constexpr int LD = NB + 1;
-  __shared__ T sr[NB * LD], si[NB * LD];  // block [row][col], real and imaginary planes
+  // [buffer][row][col]: buffers 0 and 1 hold the active rows, 2 the finished ones
+  __shared__ T sr[3][NB * LD], si[3][NB * LD];
   // perm[k0 ..] and piv[k0 ..] live here until the end: a global store
   // before a barrier would make every column wait for it to complete
   __shared__ int s_perm[NB], s_piv[NB];
-  __shared__ int s_p, s_info;
+  __shared__ int s_info;
   const int t = threadIdx.x;
   const int lane = t & 63, wave = t >> 6;
   const bool col = lane < nb;
@@ -1537,65 +1556,52 @@ __global__ void __launch_bounds__(DG_THREADS) lu_diag_block_kernel(T* __restrict
   for (int r = wave; r < nb; r += DG_WAVES)
     if (col) {
       const T* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + lane);
-      sr[r * LD + lane] = v[0];
-      si[r * LD + lane] = v[1];
+      sr[0][r * LD + lane] = v[0];
+      si[0][r * LD + lane] = v[1];
     }
   __syncthreads();
-  // wave 0: first maximum of |z|^2 over rows c .. nb-1 of column c -> s_p
-  auto search = [&](int c) {
+  // first maximum of |z|^2 over rows c .. nb-1 of column c in buffer b
+  // (wave-uniform result); a zero column gives -1
+  auto search = [&](int b, int c) {
     float m = -1.0f;
     if (lane >= c && lane < nb) {
-      const float a = (float)sr[lane * LD + c], b = (float)si[lane * LD + c];
-      m = a * a + b * b;
+      const float x = (float)sr[b][lane * LD + c], y = (float)si[b][lane * LD + c];
+      m = x * x + y * y;
     }
-    float mx = m;
-    for (int off = 32; off > 0; off >>= 1) mx = fmaxf(mx, __shfl_xor(mx, off, 64));
+    const float mx = wave_max_nonneg(m);
     const unsigned long long hit = __ballot(m >= 0.0f && m == mx);
-    if (lane == 0) {
-      int p = hit ? (__ffsll((long long)hit) - 1) : c;
-      if (!(mx > 0.0f)) {
-        p = c;  // zero column: no exchange, no division
-        if (s_info == 0) s_info = k0 + c + 1;
-      }
-      s_p = p;
-    }
+    return (mx > 0.0f && hit) ? (__ffsll((long long)hit) - 1) : -1;
   };
-  if (wave == 0) search(0);
-  __syncthreads();
+  int p = search(0, 0);
   for (int c = 0; c < nb; ++c) {
-    const int p = s_p;
+    const int cur = c & 1, nxt = cur ^ 1;
+    const bool zero = p < 0;
+    if (zero) p = c;  // zero column: no exchange, no division
     // pivot row (old row p), 1 / pivot
-    const T ur = col ? sr[p * LD + lane] : T(0), ui = col ? si[p * LD + lane] : T(0);
-    const T pr = sr[p * LD + c], pi = si[p * LD + c];
+    const T ur = col ? sr[cur][p * LD + lane] : T(0), ui = col ? si[cur][p * LD + lane] : T(0);
+    const T pr = sr[cur][p * LD + c], pi = si[cur][p * LD + c];
     const T den = pr * pr + pi * pi;
     const T inv = (den > T(0)) ? T(1) / den : T(0);
     const T ir = pr * inv, ii = -pi * inv;
     const bool right = col && lane > c, here = lane == c;
     const T vr = right ? ur : T(0), vi = right ? ui : T(0);
     // this wave's rows r = c+1+wave+16q (row p takes old row c)
-    T nr[DG_ROWS], ni[DG_ROWS];
-#pragma unroll
-    for (int q = 0; q < DG_ROWS; ++q) {
-      const int r = min(c + 1 + wave + DG_WAVES * q, nb - 1);  // clamped: a valid row, result unused
-      const int src = (r == p) ? c : r;
-      const T ar = sr[src * LD + c], ai = si[src * LD + c];
-      const T xr = sr[src * LD + lane], xi = si[src * LD + lane];
-      const T lr = ar * ir - ai * ii, li = ar * ii + ai * ir;
-      nr[q] = here ? lr : xr - (lr * vr - li * vi);
-      ni[q] = here ? li : xi - (lr * vi + li * vr);
-    }
-    __syncthreads();  // every read of the old block is done
 #pragma unroll
     for (int q = 0; q < DG_ROWS; ++q) {
       const int r = c + 1 + wave + DG_WAVES * q;
+      const int rc = min(r, nb - 1);  // clamped: a valid row, result unused
+      const int src = (rc == p) ? c : rc;
+      const T ar = sr[cur][src * LD + c], ai = si[cur][src * LD + c];
+      const T xr = sr[cur][src * LD + lane], xi = si[cur][src * LD + lane];
+      const T lr = ar * ir - ai * ii, li = ar * ii + ai * ir;
       if (r < nb && col) {
-        sr[r * LD + lane] = nr[q];
-        si[r * LD + lane] = ni[q];
+        sr[nxt][r * LD + lane] = here ? lr : xr - (lr * vr - li * vi);
+        si[nxt][r * LD + lane] = here ? li : xi - (lr * vi + li * vr);
       }
     }
-    if (wave == 1 && col && p != c) {  // row c becomes the pivot row
-      sr[c * LD + lane] = ur;
-      si[c * LD + lane] = ui;
+    if (wave == (c & (DG_WAVES - 1)) && col) {  // row c is final: the pivot row
+      sr[2][c * LD + lane] = ur;
+      si[2][c * LD + lane] = ui;
     }
     if (t == 64) {
       s_piv[c] = k0 + p;
@@ -1604,16 +1610,16 @@ __global__ void __launch_bounds__(DG_THREADS) lu_diag_block_kernel(T* __restrict
         s_perm[c] = s_perm[p];
         s_perm[p] = tmp;
       }
+      if (zero && s_info == 0) s_info = k0 + c + 1;
     }
     __syncthreads();
-    if (wave == 0 && c + 1 < nb) search(c + 1);
-    __syncthreads();
+    if (c + 1 < nb) p = search(nxt, c + 1);
   }
   for (int r = wave; r < nb; r += DG_WAVES)
     if (col) {
       T* v = A + 2 * ((size_t)(k0 + r) * lda + k0 + lane);
-      v[0] = sr[r * LD + lane];
-      v[1] = si[r * LD + lane];
+      v[0] = sr[2][r * LD + lane];
+      v[1] = si[2][r * LD + lane];
     }
   if (t < nb) {
     perm[k0 + t] = s_perm[t];
