@@ -19,9 +19,11 @@
 // operation order, so entries agree with nec2++ to the ulp-level differences
 // of the device sin/cos/log.
 #include <algorithm>
+#include <cstdlib>
 #include <type_traits>
 
 #include "dual.h"
+
 #include "kernels.h"
 #include "nec_math.h"
 
@@ -568,6 +570,12 @@ __global__ void __launch_bounds__(256) fill_fields_kernel(DevSegments g, int n, 
   o[5] = etc.im;
 }
 
+// Z[i][k] for RB consecutive rows i of the pass and one basis k per thread:
+// the sum over k's support (ascending source segment j) of etk*a + ets*b +
+// etc*c, as cmset/cmww accumulate it (src/nec_context.cpp:2635-2638), then
+// a load on segment i right after its own term.  The support entries are
+// read once for the RB rows, whose field loads are independent.
+template <int RB>
 __global__ void __launch_bounds__(256) fill_assemble_kernel(const double* __restrict__ E, int n, int i0, int rows,
                                                             const int* __restrict__ colptr,
                                                             const int* __restrict__ colj,
@@ -575,30 +583,55 @@ __global__ void __launch_bounds__(256) fill_assemble_kernel(const double* __rest
                                                             const double* __restrict__ zload,
                                                             double* __restrict__ Z, long long ldz) {
   const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  const int r = blockIdx.y;
-  if (k >= n || r >= rows) return;
-  const double* Er = E + (size_t)r * n * 6;
-  double accr = 0.0, acci = 0.0;
+  const int r0 = blockIdx.y * RB;
+  if (k >= n || r0 >= rows) return;
+  const int nr = min(RB, rows - r0);
+  double accr[RB], acci[RB];
+#pragma unroll
+  for (int u = 0; u < RB; ++u) accr[u] = acci[u] = 0.0;
   const int q1 = colptr[k + 1];
   for (int q = colptr[k]; q < q1; ++q) {
-    const double* e = Er + (size_t)colj[q] * 6;
+    const int j = colj[q];
     const double a = colabc[3 * q], b = colabc[3 * q + 1], c = colabc[3 * q + 2];
-    // etk*a + ets*b + etc*c, then Z += term (src/nec_context.cpp:2635-2638)
-    const double tr = ((e[0] * a) + (e[2] * b)) + (e[4] * c);
-    const double ti = ((e[1] * a) + (e[3] * b)) + (e[5] * c);
-    accr += tr;
-    acci += ti;
-    if (zload && colj[q] == i0 + r) {
-      // the load of segment i, right after its own term
-      // (src/nec_context.cpp:2126-2142): -= (A + C) z
-      const double s = a + c;
-      accr -= s * zload[2 * (i0 + r)];
-      acci -= s * zload[2 * (i0 + r) + 1];
+    double e[RB][6];
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+      const double* p = E + ((size_t)(r0 + min(u, nr - 1)) * n + j) * 6;
+#pragma unroll
+      for (int m = 0; m < 6; ++m) e[u][m] = p[m];
+    }
+#pragma unroll
+    for (int u = 0; u < RB; ++u) {
+      // etk*a + ets*b + etc*c, then Z += term
+      const double tr = ((e[u][0] * a) + (e[u][2] * b)) + (e[u][4] * c);
+      const double ti = ((e[u][1] * a) + (e[u][3] * b)) + (e[u][5] * c);
+      accr[u] += tr;
+      acci[u] += ti;
+      if (zload && j == i0 + r0 + u) {
+        // the load of segment i, right after its own term
+        // (src/nec_context.cpp:2126-2142): -= (A + C) z
+        const double sl = a + c;
+        accr[u] -= sl * zload[2 * j];
+        acci[u] -= sl * zload[2 * j + 1];
+      }
     }
   }
-  double* zp = Z + 2 * ((size_t)(i0 + r) * ldz + k);
-  zp[0] = accr;
-  zp[1] = acci;
+#pragma unroll
+  for (int u = 0; u < RB; ++u)
+    if (u < nr) {
+      double* zp = Z + 2 * ((size_t)(i0 + r0 + u) * ldz + k);
+      zp[0] = accr[u];
+      zp[1] = acci[u];
+    }
+}
+
+// Rows per thread of the fill assembly: 4 (NECGPU_FILL_RB = 1 or 2
+// overrides, for measurements).  At the 3,280-segment grid 4 rows take the
+// assembly from 0.77 to 0.58 ms.
+int fill_assemble_rows() {
+  const char* v = std::getenv("NECGPU_FILL_RB");
+  const int r = (v && *v) ? std::atoi(v) : 4;
+  return (r == 1 || r == 2) ? r : 4;
 }
 
 // Batched fill: one workgroup per (solve, block of R consecutive rows), the
@@ -956,8 +989,17 @@ int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int u
   else
     launch_k(K_FILL_FIELDS, fill_fields_kernel<false>, grid, block, 0, stream, g, n, i0, rows, rkh, use_ek, ekind,
              ground, E);
-  launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel, grid, block, 0, stream, E, n, i0, rows, colptr, colj, colabc,
-           zload, Z, ldz);
+  const int rb = fill_assemble_rows();
+  const dim3 agrid((n + 255) / 256, (rows + rb - 1) / rb);
+  if (rb == 4)
+    launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel<4>, agrid, block, 0, stream, E, n, i0, rows, colptr, colj, colabc,
+             zload, Z, ldz);
+  else if (rb == 2)
+    launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel<2>, agrid, block, 0, stream, E, n, i0, rows, colptr, colj, colabc,
+             zload, Z, ldz);
+  else
+    launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel<1>, agrid, block, 0, stream, E, n, i0, rows, colptr, colj, colabc,
+             zload, Z, ldz);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
