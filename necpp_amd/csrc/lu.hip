@@ -1417,6 +1417,13 @@ int zgemm_mode() {
   return (v && *v) ? std::atoi(v) : 2;
 }
 
+// L21 of the block-local FP32 panel: the MFMA kernel (default) or the
+// column-by-column one (NECGPU_L21_MFMA=0, for measurements).
+bool l21_mfma_enabled() {
+  const char* v = std::getenv("NECGPU_L21_MFMA");
+  return !(v && v[0] == '0');
+}
+
 int gemm_occupancy(int tiles, int free_cus) {
   const char* v = std::getenv("NECGPU_GEMM_OCC");
   const int o = (v && *v) ? std::atoi(v) : 0;
@@ -1694,6 +1701,119 @@ __global__ void __launch_bounds__(DB_THREADS) lu_l21_kernel(T* __restrict__ A, l
     }
 }
 
+// L21 = A21 U11^-1 on FP32 MFMA for nb = 64, one workgroup per 64-row tile
+// (rows independent, so tiles never wait for each other).  U11 (upper, from
+// the diagonal-block LU) is split into 4 x 4 blocks of 16; the workgroup
+// inverts the four 16 x 16 diagonal blocks (one lane per column, back
+// substitution), then each wave takes 16 rows of the tile through the column
+// blocks in order:
+//   T   = A_b - sum_{a<b} X_a U_ab     (MFMA, accumulators start from A_b)
+//   X_b = T U_bb^-1                     (MFMA)
+// so the sequential part is 4 short steps instead of 64 columns.  The FP64
+// refinement that these factors precondition checks the result.
+constexpr int L21M_SP = 65;  // LDS row stride (complex)
+__global__ void __launch_bounds__(256) lu_l21_mfma_kernel(float* __restrict__ A, long long lda, int n, int k0) {
+  __shared__ flt2_t sU[64 * L21M_SP];      // U11 [row][col]
+  __shared__ flt2_t sX[64 * L21M_SP];      // the tile's rows [row][col]: A21, then L21
+  __shared__ flt2_t sInv[4][16 * 17];      // inverses of the 16 x 16 diagonal blocks [row][col]
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int li = lane & 15, lk = lane >> 4;
+  const int r0 = k0 + 64 + blockIdx.x * 64;
+  const int mr = min(64, n - r0);
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    const float* u = A + 2 * ((size_t)(k0 + r) * lda + k0 + c);
+    sU[r * L21M_SP + c] = (flt2_t){u[0], u[1]};
+    flt2_t x = (flt2_t){0.0f, 0.0f};
+    if (r < mr) {
+      const float* v = A + 2 * ((size_t)(r0 + r) * lda + k0 + c);
+      x = (flt2_t){v[0], v[1]};
+    }
+    sX[r * L21M_SP + c] = x;
+  }
+  __syncthreads();
+  if (t < 64) {
+    // column j of the inverse of diagonal block b: x_i = (delta_ij -
+    // sum_{k>i} u_ik x_k) / u_ii for i = 15 .. 0 (zero below the diagonal)
+    const int b = t >> 4, j = t & 15;
+    const flt2_t* U = sU + (16 * b) * L21M_SP + 16 * b;
+    float xr[16], xi[16];
+#pragma unroll
+    for (int i = 15; i >= 0; --i) {
+      float sr = (i == j) ? 1.0f : 0.0f, si = 0.0f;
+#pragma unroll
+      for (int k = i + 1; k < 16; ++k) {
+        const flt2_t u = U[i * L21M_SP + k];
+        sr -= u.x * xr[k] - u.y * xi[k];
+        si -= u.x * xi[k] + u.y * xr[k];
+      }
+      const flt2_t d = U[i * L21M_SP + i];
+      const float den = d.x * d.x + d.y * d.y;
+      const float inv = (den > 0.0f) ? 1.0f / den : 0.0f;
+      const float ir = d.x * inv, ii = -d.y * inv;
+      xr[i] = sr * ir - si * ii;
+      xi[i] = sr * ii + si * ir;
+    }
+#pragma unroll
+    for (int i = 0; i < 16; ++i) sInv[b][i * 17 + j] = (flt2_t){xr[i], xi[i]};
+  }
+  __syncthreads();
+  flt2_t* X = sX + (16 * wave) * L21M_SP;  // this wave's 16 rows
+  for (int b = 0; b < 4; ++b) {
+    // accumulator register v of lane l holds D[4 (l / 16) + v][l % 16]
+    float4_t tr, ti;
+#pragma unroll
+    for (int v = 0; v < 4; ++v) {
+      const flt2_t a = X[(4 * lk + v) * L21M_SP + 16 * b + li];
+      tr[v] = a.x;
+      ti[v] = a.y;
+    }
+    for (int kk = 0; kk < 16 * b; kk += 4) {
+      const flt2_t x = X[li * L21M_SP + kk + lk];
+      const flt2_t u = sU[(kk + lk) * L21M_SP + 16 * b + li];
+      tr = __builtin_amdgcn_mfma_f32_16x16x4f32(-x.x, u.x, tr, 0, 0, 0);
+      tr = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, u.y, tr, 0, 0, 0);
+      ti = __builtin_amdgcn_mfma_f32_16x16x4f32(-x.x, u.y, ti, 0, 0, 0);
+      ti = __builtin_amdgcn_mfma_f32_16x16x4f32(-x.y, u.x, ti, 0, 0, 0);
+    }
+    // T through LDS into the operand layout (this wave's rows only)
+#pragma unroll
+    for (int v = 0; v < 4; ++v) X[(4 * lk + v) * L21M_SP + 16 * b + li] = (flt2_t){tr[v], ti[v]};
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    float4_t xr = (float4_t){0.0f, 0.0f, 0.0f, 0.0f}, xi = xr;
+#pragma unroll
+    for (int kk = 0; kk < 16; kk += 4) {
+      const flt2_t x = X[li * L21M_SP + 16 * b + kk + lk];
+      const flt2_t w = sInv[b][(kk + lk) * 17 + li];
+      xr = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, w.x, xr, 0, 0, 0);
+      xr = __builtin_amdgcn_mfma_f32_16x16x4f32(-x.y, w.y, xr, 0, 0, 0);
+      xi = __builtin_amdgcn_mfma_f32_16x16x4f32(x.x, w.y, xi, 0, 0, 0);
+      xi = __builtin_amdgcn_mfma_f32_16x16x4f32(x.y, w.x, xi, 0, 0, 0);
+    }
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+#pragma unroll
+    for (int v = 0; v < 4; ++v) X[(4 * lk + v) * L21M_SP + 16 * b + li] = (flt2_t){xr[v], xi[v]};
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+  __syncthreads();
+  for (int e = t; e < 64 * 64; e += 256) {
+    const int r = e >> 6, c = e & 63;
+    if (r < mr) {
+      float* v = A + 2 * ((size_t)(r0 + r) * lda + k0 + c);
+      const flt2_t x = sX[r * L21M_SP + c];
+      v[0] = x.x;
+      v[1] = x.y;
+    }
+  }
+}
+
 }  // namespace
 
 int launch_mfma_probe_f32(const float* a, const float* b, float* d, hipStream_t stream) {
@@ -1745,9 +1865,13 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
                  perm, info);
         const int below = n - k0 - nb;
         constexpr int rows_per_block = DB_WAVES * L21_ROWS;
-        if (below > 0)
-          launch_k(K_LU_PANEL, lu_l21_kernel<T>, dim3((below + rows_per_block - 1) / rows_per_block),
-                   dim3(DB_THREADS), 0u, s, A, lda, n, k0, nb);
+        if (below > 0) {
+          if (nb == NB && l21_mfma_enabled())
+            launch_k(K_LU_PANEL, lu_l21_mfma_kernel, dim3((below + 63) / 64), dim3(256), 0u, s, A, lda, n, k0);
+          else
+            launch_k(K_LU_PANEL, lu_l21_kernel<T>, dim3((below + rows_per_block - 1) / rows_per_block),
+                     dim3(DB_THREADS), 0u, s, A, lda, n, k0, nb);
+        }
         return;
       }
     }
