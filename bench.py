@@ -20,6 +20,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "solves/s (fill+LU+RP) at N=3,280 FP64; mixed-vs-FP64 speedup @1e-10 rel Zin"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_FP32_TFLOPS = 157.3  # dense FP32 (matrix) peak of MI355X
 PEAK_HBM_GBS = 8000.0
 NB = 64  # LU panel width (lu.hip)
 
@@ -175,8 +176,15 @@ def mixed_leg(nn, steps=1):
     zref = None
     if n == 16020 and os.path.exists(fixture):
         zref = complex(*json.load(open(fixture))["zin"])
+    # the LU stage against the peaks: complex LU flops 8 n^3 / 3, FP64 factor
+    # on the FP64 peak, the mixed path's FP32 factor on the FP32 peak
+    lu_flops = 8.0 * float(n) ** 3 / 3.0
+    lu64 = lu_flops / st64[1] / 1e9
+    lu32 = lu_flops / st32[1] / 1e9
     return {"workload": f"wire grid {nn}x{nn} nodes, N={n}, fill+LU+solve+Z_in (no pattern)",
             "fp64_ms_per_solve": ms64, "mixed_ms_per_solve": ms32, "speedup": ms64 / ms32,
+            "fp64_lu_stage": {"achieved_tflops": lu64, "frac_fp64_peak": lu64 / PEAK_FP64_TFLOPS, "flops": lu_flops},
+            "mixed_lu_stage": {"achieved_tflops": lu32, "frac_fp32_peak": lu32 / PEAK_FP32_TFLOPS, "flops": lu_flops},
             "fp64_stages_ms": dict(zip(("fill", "factor", "solve", "pattern"), st64)),
             "mixed_stages_ms": dict(zip(("fill", "factor", "solve", "pattern"), st32)),
             "refinement_steps": isteps, "refinement_rel_residual": ires,
