@@ -137,6 +137,16 @@ typedef struct nec_gpu_batch {
  * Returns 0, or 1 with nec_error_message() set. */
 long nec_gpu_batch_solve(const nec_gpu_batch* batch, double* zin, double* gain, double* stats);
 
+/* Run a NEC card deck (the text of a .nec input file) on in_context, as
+ * nec2++ -i deck.nec would (src/nec2cpp.cpp:290-440 job loop; comment
+ * cards, the geometry section GW/GC/GS/GE, then data cards until EN or the
+ * end of the text).  Results are read back with the libnecpp getters.
+ * Cards outside the GPU path stop the deck with their "not supported"
+ * message, as the API reports them; results computed before that card stay
+ * in the context.  Returns 0, or 1 with nec_error_message() naming the deck
+ * line and card. */
+long nec_gpu_deck(nec_context* in_context, const char* deck);
+
 /* Adjoint gradients of a whole batch (the structures and frequencies of
  * `batch`; its pattern and repeat fields are not used): for every (design,
  * frequency) solve s, in the order of nec_gpu_batch_solve, the objective of

@@ -17,6 +17,7 @@ LIB = os.path.join(HERE, "libnecpp_amd.so")
 ARCH = os.environ.get("NECPP_AMD_ARCH", "gfx950")
 
 SOURCES = ["geometry.cpp", "kclock.cpp", "engine.cpp", "batch_host.cpp", "adjoint.cpp", "context.cpp", "api.cpp",
+           "deck.cpp",
            "fill.hip", "lu.hip", "solve.hip", "refine.hip", "batch.hip", "current.hip", "pattern.hip"]
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",

@@ -384,6 +384,10 @@ long nec_gpu_stage_times(nec_context* ctx, double* out) {
   });
 }
 
+long nec_gpu_deck(nec_context* ctx, const char* deck) {
+  return guarded([&] { necgpu::run_deck(ctx, deck); });
+}
+
 long nec_gpu_set_kernel_timing(nec_context* ctx, int on) {
   return guarded([&] { ctx->device().set_kernel_timing(on != 0); });
 }

@@ -141,4 +141,8 @@ class Context {
   std::vector<PatternResult> patterns_;
 };
 
+// The card-deck front end (deck.cpp): runs NEC input text on ctx through
+// the libnecpp calls; throws NecError naming the first failing card.
+void run_deck(Context* ctx, const char* text);
+
 }  // namespace necgpu

@@ -1,0 +1,110 @@
+"""The card-deck front end (nec_gpu_deck) without a GPU: line and field
+rules, the geometry section (GW, GC, GS, GE) against the same wires through
+the API, and the error paths.  Decks stop before XQ / RP, which would touch
+the device."""
+import numpy as np
+import pytest
+
+from necpp_amd import necpp as api
+
+
+def run(text):
+    ctx = api.nec_create()
+    rc = api.nec_gpu_deck(ctx, text.encode())
+    return ctx, rc
+
+
+def segs(ctx):
+    return api.gpu_segments(ctx)
+
+
+def api_ctx(wires, gpflag=0):
+    ctx = api.nec_create()
+    for w in wires:
+        assert api.nec_wire(ctx, *w) == 0
+    assert api.nec_geometry_complete(ctx, gpflag) == 0
+    return ctx
+
+
+def same_geometry(a, b):
+    ta, ia1, ia2, ga = segs(a)
+    tb, ib1, ib2, gb = segs(b)
+    return np.array_equal(ta, tb) and np.array_equal(ia1, ib1) and np.array_equal(ia2, ib2) and np.array_equal(ga, gb)
+
+
+def test_geometry_section_matches_api():
+    text = """CM two wires, one tapered
+CM second comment line
+CE
+GW 1 7 0 0 -0.25 0 0 0.25 0.001
+GW 2 5 0.1 0 -0.2 0.1 0 0.2 0
+GC 0 0 1.1 0.001 0.002
+GE 0
+FR 0 1 0 0 299.8
+EN
+"""
+    ctx, rc = run(text)
+    ref = api_ctx([(1, 7, 0, 0, -0.25, 0, 0, 0.25, 0.001, 1.0, 1.0),
+                   (2, 5, 0.1, 0, -0.2, 0.1, 0, 0.2, 0.001, 1.1, (0.002 / 0.001) ** (1 / 4.0))])
+    try:
+        assert rc == 0, api.nec_error_message()
+        assert same_geometry(ctx, ref)
+    finally:
+        api.nec_delete(ctx)
+        api.nec_delete(ref)
+
+
+def test_gs_scales_wires_defined_before_it():
+    text = "CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGS 0 0 0.5\nGW 2 5 1 0 -1 1 0 1 0.01\nGE 0\nEN\n"
+    ctx, rc = run(text)
+    ref = api_ctx([(1, 5, 0, 0, -0.5, 0, 0, 0.5, 0.005, 1.0, 1.0), (2, 5, 1, 0, -1, 1, 0, 1, 0.01, 1.0, 1.0)])
+    try:
+        assert rc == 0, api.nec_error_message()
+        assert same_geometry(ctx, ref)
+    finally:
+        api.nec_delete(ctx)
+        api.nec_delete(ref)
+
+
+def test_line_rules():
+    # '#' lines and lines starting with a blank are skipped, a quote starts a
+    # comment, commas separate fields, mnemonics are case-insensitive, no CM/CE
+    text = "# a comment line\n gw this line is skipped\ngw 1,7, 0,0,-0.25, 0,0,0.25, 0.001 ' trailing\r\nge 0\r\nen\r\n"
+    ctx, rc = run(text)
+    ref = api_ctx([(1, 7, 0, 0, -0.25, 0, 0, 0.25, 0.001, 1.0, 1.0)])
+    try:
+        assert rc == 0, api.nec_error_message()
+        assert same_geometry(ctx, ref)
+    finally:
+        api.nec_delete(ctx)
+        api.nec_delete(ref)
+
+
+@pytest.mark.parametrize("text,needle", [
+    ("CM no end of comments\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\n", "COMMENT CARD"),
+    ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGM 0 1 0 0 90 0 0 0 0\nGE 0\n", "GM: not supported"),
+    ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGZ 0 0\nGE 0\n", "GEOMETRY DATA CARD ERROR"),
+    ("CE\nGW 1 5 0 0 -1 0 0 1 0.0x1\nGE 0\n", "NON-NUMERICAL CHARACTER 'x'"),
+    ("CE\nGW 1 5 0 0 -1 0 0 1 0\nGE 0\n", "GEOMETRY DATA CARD ERROR"),
+    ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\n", "no GE card"),
+    ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nQQ 1\n", "FAULTY DATA CARD LABEL"),
+    ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nTL 1 3 1 3 50\n", "TL: not supported"),
+    ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nEX 0 7 1 0 1\n", "ITAG"),
+    ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nGN 2 0 0 0 10 0.01\n", "GN"),
+])
+def test_deck_errors(text, needle):
+    ctx, rc = run(text)
+    try:
+        assert rc == 1
+        msg = api.nec_error_message()
+        assert needle in msg, msg
+    finally:
+        api.nec_delete(ctx)
+
+
+def test_error_names_the_deck_line():
+    ctx, rc = run("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nFR 0 1 0 0 300\nTL 1 3 1 3 50\n")
+    try:
+        assert rc == 1 and "deck line 5 (TL)" in api.nec_error_message()
+    finally:
+        api.nec_delete(ctx)
