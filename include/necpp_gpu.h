@@ -147,6 +147,14 @@ long nec_gpu_batch_solve(const nec_gpu_batch* batch, double* zin, double* gain, 
  * line and card. */
 long nec_gpu_deck(nec_context* in_context, const char* deck);
 
+/* The near electric field of NE card number `index` (0-based, in order):
+ * for each point x, y, z (metres) then Ex, Ey, Ez (re, im; V/m), 9 doubles,
+ * at most max_points written to out.  Returns the number of points, or -1
+ * when there is no such result.  NE (src/nec_context.cpp:896-955, nfpat
+ * :5068-5188, nefld :4304-4450) runs on the GPU for wires in free space; the
+ * reference exposes near fields only through its printed output. */
+long nec_gpu_near_field(nec_context* in_context, int index, double* out, long max_points);
+
 /* Adjoint gradients of a whole batch (the structures and frequencies of
  * `batch`; its pattern and repeat fields are not used): for every (design,
  * frequency) solve s, in the order of nec_gpu_batch_solve, the objective of

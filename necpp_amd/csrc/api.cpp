@@ -144,8 +144,9 @@ long nec_kh_card(nec_context* ctx, double tmp1) {
   return guarded([&] { ctx->kh_card(tmp1); });
 }
 
-long nec_ne_card(nec_context*, int, int, int, int, double, double, double, double, double, double) {
-  return unsupported("NE");
+long nec_ne_card(nec_context* ctx, int itmp1, int itmp2, int itmp3, int itmp4, double tmp1, double tmp2, double tmp3,
+                 double tmp4, double tmp5, double tmp6) {
+  return guarded([&] { ctx->ne_card(itmp1, itmp2, itmp3, itmp4, tmp1, tmp2, tmp3, tmp4, tmp5, tmp6); });
 }
 long nec_nh_card(nec_context*, int, int, int, int, double, double, double, double, double, double) {
   return unsupported("NH");
@@ -382,6 +383,23 @@ long nec_gpu_stage_times(nec_context* ctx, double* out) {
     out[2] = t.solve_ms;
     out[3] = t.pattern_ms;
   });
+}
+
+long nec_gpu_near_field(nec_context* ctx, int index, double* out, long max_points) {
+  if (!ctx) return -1;
+  const Context::NearFieldResult* r = ctx->near_field(index);
+  if (!r) return -1;
+  const long np = (long)(r->xyz.size() / 3);
+  if (out)
+    for (long p = 0; p < np && p < max_points; ++p) {
+      double* o = out + 9 * p;
+      for (int c = 0; c < 3; ++c) o[c] = r->xyz[3 * p + c];
+      for (int c = 0; c < 3; ++c) {
+        o[3 + 2 * c] = r->e[3 * p + c].re;
+        o[4 + 2 * c] = r->e[3 * p + c].im;
+      }
+    }
+  return np;
 }
 
 long nec_gpu_deck(nec_context* ctx, const char* deck) {

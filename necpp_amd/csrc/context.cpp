@@ -339,6 +339,82 @@ void Context::adjoint(const AdjointSpec& spec, AdjointResult* r, std::vector<dou
   adjoint_endpoints(seg, wavelength_, freq_mhz_, *r, grad7, dfreq_mhz);
 }
 
+void Context::ne_card(int near, int nrx, int nry, int nrz, double xnr, double ynr, double znr, double dxnr,
+                      double dynr, double dznr) {
+  // src/nec_context.cpp:930-955: blank counts default to 1; with one
+  // frequency the field is computed at once
+  if (ground_spec().type != 0) throw NecError("NE: near fields over a ground are not supported by the GPU path");
+  near_spec_.near = near;
+  near_spec_.nrx = nrx == 0 ? 1 : nrx;
+  near_spec_.nry = nry == 0 ? 1 : nry;
+  near_spec_.nrz = nrz == 0 ? 1 : nrz;
+  near_spec_.xnr = xnr;
+  near_spec_.ynr = ynr;
+  near_spec_.znr = znr;
+  near_spec_.dxnr = dxnr;
+  near_spec_.dynr = dynr;
+  near_spec_.dznr = dznr;
+  iflow_ = 8;
+  if (nfrq_ == 1) simulate(false);
+}
+
+void Context::compute_near_field() {
+  // nfpat, src/nec_context.cpp:5068-5188: the points accumulate as the
+  // reference steps them (z or theta outermost, x or r innermost)
+  const NearSpec& ns = near_spec_;
+  NearFieldResult r;
+  r.freq_hz = freq_mhz_ / 1.e-6;
+  double cth = 0., sth = 0., cph = 0., sph = 0.;
+  double znrt = ns.znr - ns.dznr;
+  for (int i = 0; i < ns.nrz; ++i) {
+    znrt += ns.dznr;
+    if (ns.near != 0) {
+      cth = std::cos(deg_to_rad(znrt));
+      sth = std::sin(deg_to_rad(znrt));
+    }
+    double ynrt = ns.ynr - ns.dynr;
+    for (int j = 0; j < ns.nry; ++j) {
+      ynrt += ns.dynr;
+      if (ns.near != 0) {
+        cph = std::cos(deg_to_rad(ynrt));
+        sph = std::sin(deg_to_rad(ynrt));
+      }
+      double xnrt = ns.xnr - ns.dxnr;
+      for (int k = 0; k < ns.nrx; ++k) {
+        xnrt += ns.dxnr;
+        if (ns.near != 0) {
+          r.xyz.push_back(xnrt * sth * cph);
+          r.xyz.push_back(xnrt * sth * sph);
+          r.xyz.push_back(xnrt * cth);
+        } else {
+          r.xyz.push_back(xnrt);
+          r.xyz.push_back(ynrt);
+          r.xyz.push_back(znrt);
+        }
+      }
+    }
+  }
+  // points in wavelengths, and the radius of the segment each lies on
+  // (nefld, src/nec_context.cpp:4315-4331)
+  const ScaledSegments seg = geo_.scaled(freq_mhz_);
+  const size_t np = r.xyz.size() / 3;
+  std::vector<double> pts(r.xyz.size()), ax(np, 0.0);
+  for (size_t p = 0; p < np; ++p) {
+    for (int c = 0; c < 3; ++c) pts[3 * p + c] = r.xyz[3 * p + c] / wavelength_;
+    for (size_t i = 0; i < seg.x.size(); ++i) {
+      const double xj = pts[3 * p] - seg.x[i], yj = pts[3 * p + 1] - seg.y[i], zj = pts[3 * p + 2] - seg.z[i];
+      double zp = seg.cab[i] * xj + seg.sab[i] * yj + seg.salp[i] * zj;
+      if (std::fabs(zp) > 0.5001 * seg.s[i]) continue;
+      zp = xj * xj + yj * yj + zj * zj - zp * zp;
+      if (zp > 0.9 * seg.b[i] * seg.b[i]) continue;
+      ax[p] = seg.b[i];
+      break;
+    }
+  }
+  r.e = device().near_field_e(pts, ax, rkh_);
+  near_fields_.push_back(std::move(r));
+}
+
 void Context::simulate(bool far_field_flag) {
   // src/nec_context.cpp:1031-1159 for voltage excitation.
   if (!geo_.is_complete()) throw NecError("geometry is not complete");
@@ -374,6 +450,12 @@ void Context::simulate(bool far_field_flag) {
         if (nfrq_ != 1) continue;
         return;
       }
+    }
+    if (iflow_ == 8) {
+      // NE card: the near field of this frequency's solution
+      compute_near_field();
+      if (nfrq_ != 1) continue;
+      return;
     }
     if (igox < NEAR_FIELD) state_ = NEAR_FIELD;
     far_field();

@@ -58,6 +58,19 @@ class Context {
   // LD: types 0-5 (series / parallel RLC lumped or per metre, fixed
   // impedance, wire conductivity); consecutive LD cards accumulate, -1 clears
   void ld_card(int type, int tag, int tagf, int tagt, double zlr, double zli, double zlc);
+  // NE card (src/nec_context.cpp:896-955): near electric field on a
+  // rectangular (near = 0) or spherical (near = 1) grid of points
+  void ne_card(int near, int nrx, int nry, int nrz, double xnr, double ynr, double znr, double dxnr, double dynr,
+               double dznr);
+  // near-field results: points (metres, 3 each) and E (3 complex each)
+  struct NearFieldResult {
+    double freq_hz = 0.0;
+    std::vector<double> xyz;
+    std::vector<zc> e;
+  };
+  const NearFieldResult* near_field(int k) const {
+    return (k >= 0 && k < (int)near_fields_.size()) ? &near_fields_[k] : nullptr;
+  }
   void ek_card(bool on);
 
   // results
@@ -120,6 +133,12 @@ class Context {
     double zlr, zli, zlc;
   };
   std::vector<LoadCard> loads_;
+  struct NearSpec {
+    int near = 0, nrx = 1, nry = 1, nrz = 1;
+    double xnr = 0, ynr = 0, znr = 0, dxnr = 0, dynr = 0, dznr = 0;
+  } near_spec_;
+  std::vector<NearFieldResult> near_fields_;
+  void compute_near_field();
   // lambda * impedance per unit length of every segment at the current
   // frequency (nec_context::load); empty without loads
   std::vector<zc> load_impedances(const ScaledSegments& seg) const;

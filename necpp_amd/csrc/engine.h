@@ -132,6 +132,9 @@ class DeviceSolver {
   // often the refinement fell back to partial pivoting
   int mixed_pivot() const { return mixed_pivot_; }
   long refine_fallbacks() const { return refine_fallbacks_; }
+  // Near electric field of the last solve's currents at points given in
+  // wavelengths (3 per point), ax as for launch_near_e; 3 complex per point.
+  std::vector<zc> near_field_e(const std::vector<double>& pts, const std::vector<double>& ax, double rkh);
   double refine_residual() const { return refine_residual_; }
   hipStream_t stream() const { return stream_; }
 

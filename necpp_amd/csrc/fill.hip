@@ -977,6 +977,78 @@ int launch_batch_fill(const BatchSolve* sol, const int* blk, int nblk, int R, in
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+namespace {
+
+// nefld (src/nec_context.cpp:4304-4450) for wires in free space: one thread
+// per (point, source segment) evaluates efld's nine field components at the
+// point (pair_fields_v; ij = true, the observation radius ax of the point's
+// own segment or 0) and contracts them with the segment's current
+// coefficients A, B, C; the 256 segments of a block are summed in a fixed
+// tree order, the blocks in order by near_e_finish_kernel.
+__global__ void __launch_bounds__(256) near_e_kernel(DevSegments g, int n, const double* __restrict__ pts,
+                                                     const double* __restrict__ ax, const double* __restrict__ abc,
+                                                     int use_ek, const int* __restrict__ ekind, double rkh,
+                                                     double* __restrict__ part) {
+  __shared__ double sred[6][256];
+  const int t = threadIdx.x;
+  const int j = blockIdx.x * 256 + t;
+  const int p = blockIdx.y;
+  double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+  if (j < n) {
+    int ind1 = 2, ind2 = 2;
+    if (use_ek) {
+      ind1 = ekind[2 * j];
+      ind2 = ekind[2 * j + 1];
+    }
+    cx<double> etk, ets, etc;
+    cx<double> e[9];
+    pair_fields_v<double>(pts[3 * p], pts[3 * p + 1], pts[3 * p + 2], ax[p], 1.0, 0.0, 0.0, g.x[j], g.y[j], g.z[j],
+                          g.s[j], g.b[j], g.cab[j], g.sab[j], g.salp[j], false, rkh, use_ek, ind1, ind2, &etk, &ets,
+                          &etc, e);
+    const zc a = zmk(abc[j], abc[n + j]), b = zmk(abc[2 * n + j], abc[3 * n + j]),
+             c = zmk(abc[4 * n + j], abc[5 * n + j]);
+#pragma unroll
+    for (int q = 0; q < 3; ++q) {
+      const zc f = zadd(zadd(zmul(zmk(e[q].re, e[q].im), a), zmul(zmk(e[q + 3].re, e[q + 3].im), b)),
+                        zmul(zmk(e[q + 6].re, e[q + 6].im), c));
+      v[2 * q] = f.re;
+      v[2 * q + 1] = f.im;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < 6; ++q) sred[q][t] = v[q];
+  __syncthreads();
+  for (int w = 128; w > 0; w >>= 1) {
+    if (t < w)
+#pragma unroll
+      for (int q = 0; q < 6; ++q) sred[q][t] += sred[q][t + w];
+    __syncthreads();
+  }
+  if (t < 6) part[((size_t)p * gridDim.x + blockIdx.x) * 6 + t] = sred[t][0];
+}
+
+__global__ void near_e_finish_kernel(const double* __restrict__ part, int npts, int nblk, double* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= npts * 6) return;
+  const int p = e / 6, q = e % 6;
+  double s = 0.0;
+  for (int b = 0; b < nblk; ++b) s += part[((size_t)p * nblk + b) * 6 + q];
+  out[e] = s;
+}
+
+}  // namespace
+
+int launch_near_e(const DevSegments& g, int n, const double* pts, const double* ax, int npts, const double* abc,
+                  int use_ek, const int* ekind, double rkh, double* part, double* out, hipStream_t stream) {
+  if (n <= 0 || npts <= 0) return 0;
+  const int nblk = (n + 255) / 256;
+  launch_k(K_FILL_FIELDS, near_e_kernel, dim3(nblk, npts), dim3(256), 0u, stream, g, n, pts, ax, abc, use_ek, ekind,
+           rkh, part);
+  launch_k(K_FILL_ASSEMBLE, near_e_finish_kernel, dim3((npts * 6 + 255) / 256), dim3(256), 0u, stream, part, npts,
+           nblk, out);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
 int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind,
                 const GroundSpec& ground, const double* zload, double* E, const int* colptr, const int* colj,
                 const double* colabc, double* Z, long long ldz, hipStream_t stream) {

@@ -1273,6 +1273,56 @@ double ora_efld(double xj, double yj, double zj, double cabj, double sabj, doubl
   return r;
 }
 
+/* nefld, src/nec_context.cpp:4304-4450, for wires in free space: the
+ * electric field at npts points (xyz, in wavelengths) of the current
+ * coefficients abc (planes air aii bir bii cir cii).  ax is the radius of
+ * the first segment the point lies on (|zp| <= 0.5001 s, rho^2 <= 0.9 b^2),
+ * else 0; every segment then contributes efld's nine components with
+ * ij = true, e = ek A + es B + ec C per axis.  out: 6 doubles per point
+ * (ex, ey, ez; re, im). */
+int ora_near_e(const ora_geom* g, const double* abc, int64_t npts, const double* xyz, double rkh, int use_ek,
+               double* out) {
+  const int64_t n = g->n;
+  const double *air = abc, *aii = abc + n, *bir = abc + 2 * n, *bii = abc + 3 * n, *cir = abc + 4 * n,
+               *cii = abc + 5 * n;
+  for (int64_t p = 0; p < npts; ++p) {
+    const double xob = xyz[3 * p], yob = xyz[3 * p + 1], zob = xyz[3 * p + 2];
+    double ax = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double xj = xob - g->xs[i], yj = yob - g->ys[i], zj = zob - g->zs[i];
+      double zp = g->cab[i] * xj + g->sab[i] * yj + g->salp[i] * zj;
+      if (fabs(zp) > 0.5001 * g->ss[i]) continue;
+      zp = xj * xj + yj * yj + zj * zj - zp * zp;
+      const double b = g->bs[i];
+      if (zp > 0.9 * b * b) continue;
+      ax = b;
+      break;
+    }
+    cx ex = mk(0.0, 0.0), ey = mk(0.0, 0.0), ez = mk(0.0, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+      int ind1 = 2, ind2 = 2;
+      if (use_ek) ek_ends(g, i, &ind1, &ind2);
+      ora_fill_counts cnt;
+      fill_state st;
+      memset(&st, 0, sizeof st);
+      st.cnt = &cnt;
+      st.s = g->ss[i]; st.b = g->bs[i]; st.xj = g->xs[i]; st.yj = g->ys[i]; st.zj = g->zs[i];
+      st.cabj = g->cab[i]; st.sabj = g->sab[i]; st.salpj = g->salp[i];
+      st.rkh = rkh; st.use_ek = use_ek; st.ind1 = ind1; st.ind2 = ind2;
+      cx e[9];
+      efld_compute(&st, xob - st.xj, yob - st.yj, ax, st.salpj, zob - st.zj, 1, e);
+      const cx acx = mk(air[i], aii[i]), bcx = mk(bir[i], bii[i]), ccx = mk(cir[i], cii[i]);
+      ex = cadd(ex, cadd(cadd(cmul(e[0], acx), cmul(e[3], bcx)), cmul(e[6], ccx)));
+      ey = cadd(ey, cadd(cadd(cmul(e[1], acx), cmul(e[4], bcx)), cmul(e[7], ccx)));
+      ez = cadd(ez, cadd(cadd(cmul(e[2], acx), cmul(e[5], bcx)), cmul(e[8], ccx)));
+    }
+    out[6 * p] = ex.re; out[6 * p + 1] = ex.im;
+    out[6 * p + 2] = ey.re; out[6 * p + 3] = ey.im;
+    out[6 * p + 4] = ez.re; out[6 * p + 5] = ez.im;
+  }
+  return 0;
+}
+
 #define TRIO_CAP 64
 
 /* cmset + cmww, src/nec_context.cpp:2077-2203 and :2514-2654, transposed

@@ -57,6 +57,8 @@ def lib():
         L.ora_eksc.argtypes = [d, d, d, i, p]
         L.ora_ekscx.argtypes = [d, d, d, d, i, i, i, p]
         L.ora_intx.argtypes = [d, d, i, d, d, d, i, p]
+        L.ora_near_e.argtypes = [p, p, i64, p, d, i, p]
+        L.ora_near_e.restype = i
         L.ora_efld.argtypes = [d] * 12 + [i, d, i, i, i, p]
         L.ora_efld.restype = d
         _lib = L
@@ -223,6 +225,15 @@ class Geometry:
         if self._L.ora_fill(self._g, rkh, int(use_ek), _ptr(Z), nthreads, ctypes.byref(cnt)) != 0:
             raise RuntimeError(self._L.ora_last_error().decode())
         return Z, {k: getattr(cnt, k) for k, _ in FillCounts._fields_}
+
+    def near_e(self, abc, pts_wl, rkh=1.0, use_ek=False):
+        """nefld: E (npts x 3 complex) at points given in wavelengths, from
+        the current coefficients abc, at the current scaling."""
+        pts = np.ascontiguousarray(pts_wl, np.float64).reshape(-1, 3)
+        out = np.zeros(6 * len(pts))
+        self._L.ora_near_e(self._g, _ptr(np.ascontiguousarray(abc, np.float64)), len(pts), _ptr(pts), rkh,
+                           int(use_ek), _ptr(out))
+        return (out[0::2] + 1j * out[1::2]).reshape(-1, 3)
 
     def current_coefficients(self, x):
         n = self.n
