@@ -13,7 +13,8 @@
 // reads as zero.
 //
 // Geometry cards: GW (with its GC continuation when the radius is zero, a
-// tapered wire), GS (scales everything defined so far) and GE.  The wires
+// tapered wire), GM (move / rotate, with copies), GS (scales everything
+// defined so far) and GE.  The wires
 // are kept until GE and then passed to nec_wire, scaled as GS asked.  Data
 // cards go to the libnecpp function of the same name; cards outside the GPU
 // path report that as the API does.  The first failing card stops the deck
@@ -27,6 +28,7 @@
 
 #include "../../include/libnecpp.h"
 #include "context.h"
+#include "nec_math.h"
 
 namespace necgpu {
 
@@ -166,7 +168,54 @@ void run_deck(Context* ctx, const char* text) {
         check(nec_wire(ctx, w.tag, w.ns, w.x1, w.y1, w.z1, w.x2, w.y2, w.z2, w.rad, w.rdel, w.rrad), c);
       check(nec_geometry_complete(ctx, iv[0]), c);
       complete = true;
-    } else if (c.name == "GA" || c.name == "GH" || c.name == "GM" || c.name == "GR" || c.name == "GX" ||
+    } else if (c.name == "GM") {
+      // move / rotate, optionally as copies (src/c_geometry.cpp:256-268,
+      // c_geometry::move :830-900): rotations about x, y, z (degrees), then
+      // the translation, applied to the wires from the first one with tag
+      // ITS on; with NRPT copies each copy transforms the previous one and
+      // non-zero tags grow by ITGI per copy
+      fields(c, 2, 7, true, iv, rv);
+      const int itgi = iv[0], nrpt = iv[1];
+      const int its = static_cast<int>(rv[6] + .5);
+      const double sps = std::sin(deg_to_rad(rv[0])), cps = std::cos(deg_to_rad(rv[0]));
+      const double sth = std::sin(deg_to_rad(rv[1])), cth = std::cos(deg_to_rad(rv[1]));
+      const double sph = std::sin(deg_to_rad(rv[2])), cph = std::cos(deg_to_rad(rv[2]));
+      const double xx = cph * cth, xy = cph * sth * sps - sph * cps, xz = cph * sth * cps + sph * sps;
+      const double yx = sph * cth, yy = sph * sth * sps + cph * cps, yz = sph * sth * cps - cph * sps;
+      const double zx = -sth, zy = cth * sps, zz = cth * cps;
+      const double xs = rv[3], ys = rv[4], zs = rv[5];
+      size_t w0 = 0;
+      if (its != 0)
+        for (size_t q = 0; q < wires.size(); ++q)
+          if (wires[q].tag == its) {
+            w0 = q;
+            break;
+          }
+      auto moved = [&](const PendingWire& w) {
+        PendingWire m = w;
+        m.x1 = w.x1 * xx + w.y1 * xy + w.z1 * xz + xs;
+        m.y1 = w.x1 * yx + w.y1 * yy + w.z1 * yz + ys;
+        m.z1 = w.x1 * zx + w.y1 * zy + w.z1 * zz + zs;
+        m.x2 = w.x2 * xx + w.y2 * xy + w.z2 * xz + xs;
+        m.y2 = w.x2 * yx + w.y2 * yy + w.z2 * yz + ys;
+        m.z2 = w.x2 * zx + w.y2 * zy + w.z2 * zz + zs;
+        return m;
+      };
+      if (nrpt == 0) {
+        for (size_t q = w0; q < wires.size(); ++q) wires[q] = moved(wires[q]);
+      } else {
+        size_t b0 = w0;
+        for (int ir = 0; ir < nrpt; ++ir) {
+          const size_t b1 = wires.size();
+          for (size_t q = b0; q < b1; ++q) {
+            PendingWire m = moved(wires[q]);
+            if (m.tag != 0) m.tag += itgi;
+            wires.push_back(m);
+          }
+          b0 = b1;
+        }
+      }
+    } else if (c.name == "GA" || c.name == "GH" || c.name == "GR" || c.name == "GX" ||
                c.name == "SP" || c.name == "SM" || c.name == "SC" || c.name == "GF") {
       throw NecError(where(c) + ": " + c.name + ": not supported by the GPU path");
     } else {

@@ -82,7 +82,7 @@ def test_line_rules():
 
 @pytest.mark.parametrize("text,needle", [
     ("CM no end of comments\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\n", "COMMENT CARD"),
-    ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGM 0 1 0 0 90 0 0 0 0\nGE 0\n", "GM: not supported"),
+    ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGR 0 4\nGE 0\n", "GR: not supported"),
     ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGZ 0 0\nGE 0\n", "GEOMETRY DATA CARD ERROR"),
     ("CE\nGW 1 5 0 0 -1 0 0 1 0.0x1\nGE 0\n", "NON-NUMERICAL CHARACTER 'x'"),
     ("CE\nGW 1 5 0 0 -1 0 0 1 0\nGE 0\n", "GEOMETRY DATA CARD ERROR"),
@@ -108,3 +108,58 @@ def test_error_names_the_deck_line():
         assert rc == 1 and "deck line 5 (TL)" in api.nec_error_message()
     finally:
         api.nec_delete(ctx)
+
+
+KPI = 3.1415926536
+
+
+def gm(wires, itgi, nrpt, rox, roy, roz, xs, ys, zs, its=0):
+    """GM on wire endpoints, as the deck front end applies it."""
+    import math
+    d2r = lambda d: d * ((2 * KPI) / 360.0)
+    sps, cps = math.sin(d2r(rox)), math.cos(d2r(rox))
+    sth, cth = math.sin(d2r(roy)), math.cos(d2r(roy))
+    sph, cph = math.sin(d2r(roz)), math.cos(d2r(roz))
+    xx, xy, xz = cph * cth, cph * sth * sps - sph * cps, cph * sth * cps + sph * sps
+    yx, yy, yz = sph * cth, sph * sth * sps + cph * cps, sph * sth * cps - cph * sps
+    zx, zy, zz = -sth, cth * sps, cth * cps
+
+    def mv(w, add):
+        t, ns, x1, y1, z1, x2, y2, z2 = w[:8]
+        p1 = (x1 * xx + y1 * xy + z1 * xz + xs, x1 * yx + y1 * yy + z1 * yz + ys, x1 * zx + y1 * zy + z1 * zz + zs)
+        p2 = (x2 * xx + y2 * xy + z2 * xz + xs, x2 * yx + y2 * yy + z2 * yz + ys, x2 * zx + y2 * zy + z2 * zz + zs)
+        return (t + add if (t != 0 and add) else t, ns) + p1 + p2 + tuple(w[8:])
+
+    w0 = next((q for q, w in enumerate(wires) if its != 0 and w[0] == its), 0)
+    out = list(wires)
+    if nrpt == 0:
+        return out[:w0] + [mv(w, 0) for w in out[w0:]]
+    b0 = w0
+    for _ in range(nrpt):
+        b1 = len(out)
+        out += [mv(out[q], itgi) for q in range(b0, b1)]
+        b0 = b1
+    return out
+
+
+def test_gm_copies_and_moves():
+    text = """CE
+GW 1 3 0 0 0 0 0 0.1 0.001
+GW 2 3 0 0.1 0 0 0.1 0.1 0.001
+GM 5 3 0 0 30 0.2 0 0 2
+GM 0 0 10 20 0 0 0 0.05 0
+GE 0
+EN
+"""
+    ctx, rc = run(text)
+    w = [(1, 3, 0, 0, 0, 0, 0, 0.1, 0.001, 1.0, 1.0), (2, 3, 0, 0.1, 0, 0, 0.1, 0.1, 0.001, 1.0, 1.0)]
+    w = gm(w, 5, 3, 0, 0, 30, 0.2, 0, 0, its=2)
+    w = gm(w, 0, 0, 10, 20, 0, 0, 0, 0.05)
+    assert len(w) == 5 and [x[0] for x in w] == [1, 2, 7, 12, 17]
+    ref = api_ctx(w)
+    try:
+        assert rc == 0, api.nec_error_message()
+        assert same_geometry(ctx, ref)
+    finally:
+        api.nec_delete(ctx)
+        api.nec_delete(ref)
