@@ -9,13 +9,15 @@
  * (src/libnecpp.h:<line> in the nec2++ tree) and the implementation it
  * mirrors (src/libNEC.cpp:<line>).
  *
- * Supported by the GPU path: straight wires in free space (GW), voltage
- * sources (EX 0), frequency sweeps (FR), the integration-distance card (KH),
- * the extended thin-wire kernel (EK), execute (XQ), far-field patterns
- * (RP mode 0) and all gain / impedance getters.  Card functions for features
- * outside that path (patches, ground, loads, networks, near fields, ...)
- * exist with the reference's signature and return 1 with the message
- * "<card>: not supported by the GPU path".
+ * Supported by the GPU path: straight and tapered wires (GW), moves and
+ * copies (GM), voltage sources (EX 0), frequency sweeps (FR), the
+ * integration-distance card (KH), the extended thin-wire kernel (EK), loads
+ * (LD types 0-5), perfect and reflection-coefficient ground (GN 1, GN 0),
+ * near electric fields of wires in free space (NE), execute (XQ), far-field
+ * patterns (RP mode 0) and all gain / impedance getters.  Card functions for
+ * features outside that path (patches, Sommerfeld ground, radial screens,
+ * NH, coupling, ...) exist with the reference's signature and return 1 with
+ * the message "<card>: not supported by the GPU path".
  */
 #pragma once
 
@@ -43,7 +45,9 @@ long nec_sp_card(nec_context* in_context, int ns, double x1, double y1, double z
 /* src/libnecpp.h:132 (SC card; not on the GPU path) */
 long nec_sc_card(nec_context* in_context, int i2, double x3, double y3, double z3,
                  double x4, double y4, double z4);
-/* src/libnecpp.h:154 (GM card; not on the GPU path) */
+/* src/libnecpp.h:154, src/libNEC.cpp:66-74 -> c_geometry::move
+ * (src/c_geometry.cpp:830-900): rotations in degrees about x, y, z, then the
+ * shift, of the segments from tag its on; nrpt copies with tags + itsi */
 long nec_gm_card(nec_context* in_context, int itsi, int nrpt, double rox, double roy, double roz,
                  double xs, double ys, double zs, int its);
 /* src/libnecpp.h:190 (GX card; not on the GPU path) */

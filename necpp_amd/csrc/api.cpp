@@ -67,8 +67,12 @@ long nec_wire(nec_context* ctx, int tag_id, int segment_count, double xw1, doubl
 
 long nec_sp_card(nec_context*, int, double, double, double, double, double, double) { return unsupported("SP"); }
 long nec_sc_card(nec_context*, int, double, double, double, double, double, double) { return unsupported("SC"); }
-long nec_gm_card(nec_context*, int, int, double, double, double, double, double, double, int) {
-  return unsupported("GM");
+long nec_gm_card(nec_context* ctx, int itsi, int nrpt, double rox, double roy, double roz, double xs, double ys,
+                 double zs, int its) {
+  // src/libNEC.cpp:66-74: the angles arrive in degrees
+  return guarded([&] {
+    ctx->move(necgpu::deg_to_rad(rox), necgpu::deg_to_rad(roy), necgpu::deg_to_rad(roz), xs, ys, zs, its, nrpt, itsi);
+  });
 }
 long nec_gx_card(nec_context*, int, int) { return unsupported("GX"); }
 

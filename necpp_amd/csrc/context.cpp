@@ -40,6 +40,13 @@ void Context::wire(int tag, int ns, double x1, double y1, double z1, double x2, 
   geo_.add_wire(tag, ns, x1, y1, z1, x2, y2, z2, rad, rdel, rrad);
 }
 
+void Context::move(double rox, double roy, double roz, double xs, double ys, double zs, int its, int nrpt,
+                   int itgi) {
+  geo_.move(rox, roy, roz, xs, ys, zs, its, nrpt, itgi);
+}
+
+void Context::scale(double f) { geo_.scale(f); }
+
 void Context::geometry_complete(int gpflag) {
   geo_.complete(gpflag);
   calc_prepare();
@@ -354,6 +361,7 @@ void Context::ne_card(int near, int nrx, int nry, int nrz, double xnr, double yn
   near_spec_.dxnr = dxnr;
   near_spec_.dynr = dynr;
   near_spec_.dznr = dznr;
+  near_pending_ = true;
   iflow_ = 8;
   if (nfrq_ == 1) simulate(false);
 }
@@ -451,13 +459,16 @@ void Context::simulate(bool far_field_flag) {
         return;
       }
     }
-    if (iflow_ == 8) {
-      // NE card: the near field of this frequency's solution
-      compute_near_field();
-      if (nfrq_ != 1) continue;
-      return;
-    }
+    // excitation_compute_near_field, src/nec_context.cpp:1741-1757: a
+    // pending NE card is evaluated at every frequency of the run (before
+    // the far field) and cleared after the last; with one frequency the
+    // run ends there, otherwise the far field of the frequency follows
     if (igox < NEAR_FIELD) state_ = NEAR_FIELD;
+    if (igox != FAR_FIELD && near_pending_) {
+      compute_near_field();
+      if (mhz == nfrq_) near_pending_ = false;
+      if (nfrq_ == 1) return;
+    }
     far_field();
     if (mhz == nfrq_) ifar_ = -1;
     if (nfrq_ == 1) return;

@@ -44,6 +44,10 @@ class Context {
   // cards
   void wire(int tag, int ns, double x1, double y1, double z1, double x2, double y2, double z2, double rad,
             double rdel, double rrad);
+  // GM card (nec_context::move, src/nec_context.cpp:345-350): angles in radians
+  void move(double rox, double roy, double roz, double xs, double ys, double zs, int its, int nrpt, int itgi);
+  // GS card of a deck: scale the segments defined so far
+  void scale(double f);
   void geometry_complete(int gpflag);
   void fr_card(int ifrq, int nfrq, double freq_mhz, double del_freq);
   void ex_card(int type, int i2, int i3, int i4, double f1, double f2, double f3, double f4, double f5, double f6);
@@ -137,6 +141,9 @@ class Context {
     int near = 0, nrx = 1, nry = 1, nrz = 1;
     double xnr = 0, ynr = 0, znr = 0, dxnr = 0, dynr = 0, dznr = 0;
   } near_spec_;
+  // an NE card waits for evaluation (the reference's m_near != -1): in every
+  // frequency of the next run, before the far field, until the last one
+  bool near_pending_ = false;
   std::vector<NearFieldResult> near_fields_;
   void compute_near_field();
   // lambda * impedance per unit length of every segment at the current

@@ -14,8 +14,7 @@
 //
 // Geometry cards: GW (with its GC continuation when the radius is zero, a
 // tapered wire), GM (move / rotate, with copies), GS (scales everything
-// defined so far) and GE.  The wires
-// are kept until GE and then passed to nec_wire, scaled as GS asked.  Data
+// defined so far) and GE, each applied to the segments as it is read.  Data
 // cards go to the libnecpp function of the same name; cards outside the GPU
 // path report that as the API does.  The first failing card stops the deck
 // with an error naming the card; results computed before it stay in the
@@ -106,11 +105,6 @@ void fields(const Card& c, int ni, int nr, bool strict, int* iv, double* rv) {
   }
 }
 
-struct PendingWire {
-  int tag, ns;
-  double x1, y1, z1, x2, y2, z2, rad, rdel, rrad;
-};
-
 void check(long rc, const Card& c) {
   if (rc != 0) throw NecError(where(c) + ": " + nec_error_message());
 }
@@ -127,8 +121,8 @@ void run_deck(Context* ctx, const char* text) {
     if (k >= cards.size() || cards[k].name != "CE") throw NecError("ERROR: INCORRECT LABEL FOR A COMMENT CARD");
     ++k;
   }
-  // geometry section
-  std::vector<PendingWire> wires;
+  // geometry section: each card acts on the segments defined so far, as
+  // c_geometry::parse_geometry does (src/c_geometry.cpp:115-165)
   bool complete = false;
   int iv[4];
   double rv[7];
@@ -136,85 +130,38 @@ void run_deck(Context* ctx, const char* text) {
     const Card& c = cards[k++];
     if (c.name == "GW") {
       fields(c, 2, 7, true, iv, rv);
-      PendingWire w{iv[0], iv[1], rv[0], rv[1], rv[2], rv[3], rv[4], rv[5], rv[6], 1.0, 1.0};
-      if (w.rad == 0.0) {
+      const int tag = iv[0], ns = iv[1];
+      double rad = rv[6], rdel = 1.0, rrad = 1.0;
+      if (rad == 0.0) {
         // tapered wire: GC gives the segment length ratio and the end radii
         // (src/c_geometry.cpp:189-234)
         if (k >= cards.size() || cards[k].name != "GC") throw NecError("GEOMETRY DATA CARD ERROR (" + where(c) + ")");
         const Card& g = cards[k++];
-        fields(g, 2, 7, true, iv, rv);
-        if (rv[1] == 0.0 || rv[2] == 0.0) throw NecError("GEOMETRY DATA CARD ERROR (" + where(g) + ")");
-        w.rdel = rv[0];
-        w.rad = rv[1];
-        w.rrad = std::pow(rv[2] / rv[1], 1.0 / (w.ns - 1.0));
+        double gv[7];
+        fields(g, 2, 7, true, iv, gv);
+        if (gv[1] == 0.0 || gv[2] == 0.0) throw NecError("GEOMETRY DATA CARD ERROR (" + where(g) + ")");
+        rdel = gv[0];
+        rad = gv[1];
+        rrad = std::pow(gv[2] / gv[1], 1.0 / (ns - 1.0));
       }
-      wires.push_back(w);
+      check(nec_wire(ctx, tag, ns, rv[0], rv[1], rv[2], rv[3], rv[4], rv[5], rad, rdel, rrad), c);
     } else if (c.name == "GS") {
-      // scale every wire defined so far (coordinates and radii)
+      // scale every segment defined so far, coordinates and radii
+      // (c_geometry::scale, src/c_geometry.cpp:1308-1318)
       fields(c, 2, 7, true, iv, rv);
-      const double f = rv[0];
-      for (PendingWire& w : wires) {
-        w.x1 *= f;
-        w.y1 *= f;
-        w.z1 *= f;
-        w.x2 *= f;
-        w.y2 *= f;
-        w.z2 *= f;
-        w.rad *= f;
-      }
+      ctx->scale(rv[0]);
     } else if (c.name == "GE") {
       fields(c, 2, 7, true, iv, rv);
-      for (const PendingWire& w : wires)
-        check(nec_wire(ctx, w.tag, w.ns, w.x1, w.y1, w.z1, w.x2, w.y2, w.z2, w.rad, w.rdel, w.rrad), c);
       check(nec_geometry_complete(ctx, iv[0]), c);
       complete = true;
     } else if (c.name == "GM") {
-      // move / rotate, optionally as copies (src/c_geometry.cpp:256-268,
-      // c_geometry::move :830-900): rotations about x, y, z (degrees), then
-      // the translation, applied to the wires from the first one with tag
-      // ITS on; with NRPT copies each copy transforms the previous one and
-      // non-zero tags grow by ITGI per copy
+      // move / rotate, optionally as copies (src/c_geometry.cpp:256-268):
+      // I1 the tag increment, I2 the copies, F1-F3 the rotations (degrees),
+      // F4-F6 the translation, F7 the first tag moved; the same call as
+      // nec_gm_card, so decks and API programs build the same segments
       fields(c, 2, 7, true, iv, rv);
-      const int itgi = iv[0], nrpt = iv[1];
-      const int its = static_cast<int>(rv[6] + .5);
-      const double sps = std::sin(deg_to_rad(rv[0])), cps = std::cos(deg_to_rad(rv[0]));
-      const double sth = std::sin(deg_to_rad(rv[1])), cth = std::cos(deg_to_rad(rv[1]));
-      const double sph = std::sin(deg_to_rad(rv[2])), cph = std::cos(deg_to_rad(rv[2]));
-      const double xx = cph * cth, xy = cph * sth * sps - sph * cps, xz = cph * sth * cps + sph * sps;
-      const double yx = sph * cth, yy = sph * sth * sps + cph * cps, yz = sph * sth * cps - cph * sps;
-      const double zx = -sth, zy = cth * sps, zz = cth * cps;
-      const double xs = rv[3], ys = rv[4], zs = rv[5];
-      size_t w0 = 0;
-      if (its != 0)
-        for (size_t q = 0; q < wires.size(); ++q)
-          if (wires[q].tag == its) {
-            w0 = q;
-            break;
-          }
-      auto moved = [&](const PendingWire& w) {
-        PendingWire m = w;
-        m.x1 = w.x1 * xx + w.y1 * xy + w.z1 * xz + xs;
-        m.y1 = w.x1 * yx + w.y1 * yy + w.z1 * yz + ys;
-        m.z1 = w.x1 * zx + w.y1 * zy + w.z1 * zz + zs;
-        m.x2 = w.x2 * xx + w.y2 * xy + w.z2 * xz + xs;
-        m.y2 = w.x2 * yx + w.y2 * yy + w.z2 * yz + ys;
-        m.z2 = w.x2 * zx + w.y2 * zy + w.z2 * zz + zs;
-        return m;
-      };
-      if (nrpt == 0) {
-        for (size_t q = w0; q < wires.size(); ++q) wires[q] = moved(wires[q]);
-      } else {
-        size_t b0 = w0;
-        for (int ir = 0; ir < nrpt; ++ir) {
-          const size_t b1 = wires.size();
-          for (size_t q = b0; q < b1; ++q) {
-            PendingWire m = moved(wires[q]);
-            if (m.tag != 0) m.tag += itgi;
-            wires.push_back(m);
-          }
-          b0 = b1;
-        }
-      }
+      check(nec_gm_card(ctx, iv[0], iv[1], rv[0], rv[1], rv[2], rv[3], rv[4], rv[5], static_cast<int>(rv[6] + .5)),
+            c);
     } else if (c.name == "GA" || c.name == "GH" || c.name == "GR" || c.name == "GX" ||
                c.name == "SP" || c.name == "SM" || c.name == "SC" || c.name == "GF") {
       throw NecError(where(c) + ": " + c.name + ": not supported by the GPU path");

@@ -351,6 +351,10 @@ void DeviceSolver::refine_solve() {
     if (mixed_pivot_ == 1 && (it >= kMaxLocalRefine || (it >= 2 && !(refine_residual_ < 0.5 * prev)))) {
       ++refine_fallbacks_;
       enqueue_factor_f32(0);
+      const int info = factor_outcome();
+      if (info != 0)
+        throw NecError("mixed-precision LU: zero pivot in column " + std::to_string(info) +
+                       " of the partial-pivot refactorisation");
       hip_check(hipMemcpyAsync(x_.as<void>(), b, (size_t)n * 16, hipMemcpyDeviceToDevice, stream_), "rhs copy");
       ++epoch_;
       if (launch_lu_solve_apply_f32(z32, n, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(), epoch_,
@@ -440,23 +444,29 @@ void DeviceSolver::fill(double rkh) {
   factored_ = false;
 }
 
-void DeviceSolver::factor() {
-  hip_check(hipEventRecord(ev_[0], stream_), "event");
-  enqueue_factor();
-  hip_check(hipEventRecord(ev_[1], stream_), "event");
+int DeviceSolver::factor_outcome() {
+  // waits for the factorisation on the stream; a panel exchange that timed
+  // out is an error, the zero-pivot word (1 + first zero column) is returned
   int info = 0, status = 0;
   hip_check(hipMemcpyAsync(&info, info_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
   hip_check(hipMemcpyAsync(&status, lu_status(lu_work_.as<void>(), n_), sizeof(int), hipMemcpyDeviceToHost, stream_),
             "status");
   hip_check(hipStreamSynchronize(stream_), "factor");
   if (status) throw NecError("LU: a panel exchange timed out");
+  return info;
+}
+
+void DeviceSolver::factor() {
+  hip_check(hipEventRecord(ev_[0], stream_), "event");
+  enqueue_factor();
+  hip_check(hipEventRecord(ev_[1], stream_), "event");
+  int info = factor_outcome();
   if (precision_ == 1 && mixed_pivot_ == 1 && info != 0) {
     // a zero pivot inside a diagonal block: partial pivoting instead
     ++refine_fallbacks_;
     enqueue_factor_f32(0);
     hip_check(hipEventRecord(ev_[1], stream_), "event");
-    hip_check(hipMemcpyAsync(&info, info_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
-    hip_check(hipStreamSynchronize(stream_), "factor");
+    info = factor_outcome();
   }
   times_.factor_ms = elapsed(ev_[0], ev_[1]);
   lu_info_ = info;

@@ -157,6 +157,9 @@ class DeviceSolver {
   DevBuf zload_dev_;
   bool has_loads_ = false;
   bool factored_ = false;
+  // waits for the enqueued factorisation: throws if a panel exchange timed
+  // out, returns the zero-pivot word
+  int factor_outcome();
   int lu_info_ = 0;
   unsigned epoch_ = 0;
   hipStream_t stream_ = nullptr;

@@ -1,6 +1,7 @@
 // geometry.cpp -- wire geometry of a nec_context (host side).
 #include "geometry.h"
 
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 
@@ -66,6 +67,67 @@ void Geometry::add_wire(int tag, int ns, double xw1, double yw1, double zw1, dou
   x2_.back() = xw2;
   y2_.back() = yw2;
   z2_.back() = zw2;
+}
+
+void Geometry::move(double rox, double roy, double roz, double xs, double ys, double zs, int its, int nrpt,
+                    int itgi) {
+  // c_geometry::move, src/c_geometry.cpp:830-900: the rotation matrix of
+  // rotations about x, then y, then z, applied to both end points of every
+  // segment from i1 on; copies are appended after the last segment and
+  // each copy transforms the one before it
+  const double sps = std::sin(rox), cps = std::cos(rox);
+  const double sth = std::sin(roy), cth = std::cos(roy);
+  const double sph = std::sin(roz), cph = std::cos(roz);
+  const double xx = cph * cth, xy = cph * sth * sps - sph * cps, xz = cph * sth * cps + sph * sps;
+  const double yx = sph * cth, yy = sph * sth * sps + cph * cps, yz = sph * sth * cps - cph * sps;
+  const double zx = -sth, zy = cth * sps, zz = cth * cps;
+  if (n_ <= 0) return;
+  complete_ = false;
+  int64_t i1 = std::max<int64_t>(1, segment_number(its, 1));  // throws for a tag no segment has
+  const int nrp = nrpt == 0 ? 1 : nrpt;
+  int64_t k = nrpt == 0 ? i1 - 1 : n_;
+  if (nrpt != 0) {
+    const size_t sz = static_cast<size_t>(n_ + (n_ + 1 - i1) * nrpt);
+    x.resize(sz);
+    y.resize(sz);
+    z.resize(sz);
+    x2_.resize(sz);
+    y2_.resize(sz);
+    z2_.resize(sz);
+    rad.resize(sz);
+    tags.resize(sz);
+  }
+  for (int ir = 0; ir < nrp; ++ir) {
+    for (int64_t i = i1 - 1; i < n_; ++i, ++k) {
+      double xi = x[i], yi = y[i], zi = z[i];
+      x[k] = xi * xx + yi * xy + zi * xz + xs;
+      y[k] = xi * yx + yi * yy + zi * yz + ys;
+      z[k] = xi * zx + yi * zy + zi * zz + zs;
+      xi = x2_[i];
+      yi = y2_[i];
+      zi = z2_[i];
+      x2_[k] = xi * xx + yi * xy + zi * xz + xs;
+      y2_[k] = xi * yx + yi * yy + zi * yz + ys;
+      z2_[k] = xi * zx + yi * zy + zi * zz + zs;
+      rad[k] = rad[i];
+      tags[k] = tags[i] != 0 ? tags[i] + itgi : tags[i];
+    }
+    i1 = n_ + 1;
+    n_ = k;
+  }
+}
+
+void Geometry::scale(double f) {
+  complete_ = false;
+  for (int64_t i = 0; i < n_; ++i) {
+    x[i] = x[i] * f;
+    y[i] = y[i] * f;
+    z[i] = z[i] * f;
+    x2_[i] = x2_[i] * f;
+    y2_[i] = y2_[i] * f;
+    z2_[i] = z2_[i] * f;
+    rad[i] = rad[i] * f;
+  }
 }
 
 void Geometry::connect(int ignd) {

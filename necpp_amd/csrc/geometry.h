@@ -34,6 +34,14 @@ class Geometry {
  public:
   void add_wire(int tag, int ns, double xw1, double yw1, double zw1, double xw2, double yw2, double zw2,
                 double rad, double rdel, double rrad);
+  // GM: rotate about x, y, z by rox, roy, roz (radians), then translate,
+  // the segments from the first one with tag its on (all with its = 0); with
+  // nrpt > 0 append nrpt transformed copies instead, each of the previous
+  // one, non-zero tags raised by itgi per copy (c_geometry::move,
+  // src/c_geometry.cpp:830-900, wires only)
+  void move(double rox, double roy, double roz, double xs, double ys, double zs, int its, int nrpt, int itgi);
+  // GS: every segment's end points and radius times f (src/c_geometry.cpp:1308-1318)
+  void scale(double f);
   void complete(int gpflag);
   bool is_complete() const { return complete_; }
   int64_t size() const { return n_; }

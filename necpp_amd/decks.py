@@ -26,6 +26,21 @@ def dipole(ns=51, half=0.25, rad=0.001, freq_mhz=299.792458, rp=(181, 361)):
     return deck
 
 
+def turnstile(phase=-1j, ns=21, half=0.25, sep=0.05, freq_mhz=299.792458):
+    """Two crossed half-wave dipoles, x-directed at z = 0 and y-directed at
+    z = sep, fed at their centre segments with 1 V and `phase` V (two
+    sources: "sources" lists ((tag, segment), voltage)).  Fed in quadrature
+    the far field is elliptically polarised, nearly circular along z."""
+    return {
+        "name": "turnstile",
+        "wires": [(1, ns, -half, 0.0, 0.0, half, 0.0, 0.0, 0.001, 1.0, 1.0),
+                  (2, ns, 0.0, -half, sep, 0.0, half, sep, 0.001, 1.0, 1.0)],
+        "gpflag": 0,
+        "freq_mhz": freq_mhz,
+        "sources": [((1, ns // 2 + 1), 1.0 + 0.0j), ((2, ns // 2 + 1), complex(phase))],
+    }
+
+
 def _node(k, nn, spacing):
     return float(f"{-(nn - 1) * spacing / 2.0 + k * spacing:.10f}")
 

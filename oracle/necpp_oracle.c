@@ -180,6 +180,55 @@ int ora_wire(ora_geom* g, int tag, int ns, double xw1, double yw1, double zw1,
   return 0;
 }
 
+/* c_geometry::move, src/c_geometry.cpp:830-900 (wires only): rotate the
+ * segments from the first one tagged its (all for its = 0) about x, y and z
+ * by rox, roy, roz radians, then shift them by (xs, ys, zs); with nrpt > 0
+ * append nrpt such copies, each of the one before, with non-zero tags
+ * raised by itgi. */
+int ora_move(ora_geom* g, double rox, double roy, double roz, double xs, double ys, double zs, int its, int nrpt,
+             int itgi) {
+  double sps = sin(rox), cps = cos(rox), sth = sin(roy), cth = cos(roy), sph = sin(roz), cph = cos(roz);
+  double xx = cph * cth, xy = cph * sth * sps - sph * cps, xz = cph * sth * cps + sph * sps;
+  double yx = sph * cth, yy = sph * sth * sps + cph * cps, yz = sph * sth * cps - cph * sps;
+  double zx = -sth, zy = cth * sps, zz = cth * cps;
+  int64_t nrp = nrpt == 0 ? 1 : nrpt;
+  if (g->n <= 0) return 0;
+  int64_t i1 = ora_segment_number(g, its, 1);
+  if (i1 < 0) return 1;
+  if (i1 < 1) i1 = 1;
+  int64_t k = (nrpt == 0) ? i1 - 1 : g->n;
+  if (nrpt != 0) grow(g, g->n + (g->n + 1 - i1) * nrpt);
+  g->complete = 0;
+  for (int64_t ir = 0; ir < nrp; ir++) {
+    for (int64_t i = i1 - 1; i < g->n; i++, k++) {
+      double xi = g->x[i], yi = g->y[i], zi = g->z[i];
+      g->x[k] = xi * xx + yi * xy + zi * xz + xs;
+      g->y[k] = xi * yx + yi * yy + zi * yz + ys;
+      g->z[k] = xi * zx + yi * zy + zi * zz + zs;
+      xi = g->x2[i]; yi = g->y2[i]; zi = g->z2[i];
+      g->x2[k] = xi * xx + yi * xy + zi * xz + xs;
+      g->y2[k] = xi * yx + yi * yy + zi * yz + ys;
+      g->z2[k] = xi * zx + yi * zy + zi * zz + zs;
+      g->rad[k] = g->rad[i];
+      g->tag[k] = g->tag[i] != 0 ? g->tag[i] + itgi : g->tag[i];
+    }
+    i1 = g->n + 1;
+    g->n = k;
+  }
+  return 0;
+}
+
+/* c_geometry::scale, src/c_geometry.cpp:1308-1318 (GS): end points and
+ * radii of every segment times f. */
+void ora_scale(ora_geom* g, double f) {
+  for (int64_t i = 0; i < g->n; i++) {
+    g->x[i] *= f; g->y[i] *= f; g->z[i] *= f;
+    g->x2[i] *= f; g->y2[i] *= f; g->z2[i] *= f;
+    g->rad[i] *= f;
+  }
+  g->complete = 0;
+}
+
 static double l1(double ax, double ay, double az, double bx, double by, double bz) {
   return (fabs(ax - bx) + fabs(ay - by)) + fabs(az - bz);
 }

@@ -83,6 +83,13 @@ double ora_speed_of_light(void);
 double ora_impedance(void);
 
 /* c_geometry::frequency_scale (src/c_geometry.cpp:3389-3418). */
+/* GM (c_geometry::move, src/c_geometry.cpp:830-900), angles in radians;
+ * GS (c_geometry::scale, :1308-1318).  Both act on the segments defined so
+ * far, before ora_geometry_complete.  ora_move returns 1 (message in
+ * ora_last_error) when its names a tag no segment has. */
+int ora_move(ora_geom* g, double rox, double roy, double roz, double xs, double ys, double zs, int its, int nrpt,
+             int itgi);
+void ora_scale(ora_geom* g, double f);
 void ora_frequency_scale(ora_geom* g, double freq_mhz);
 
 /* c_geometry::trio + sbf (src/c_geometry.cpp:2982-3289) on the scaled

@@ -41,6 +41,8 @@ def lib():
         L.ora_speed_of_light.restype = d
         L.ora_impedance.restype = d
         L.ora_frequency_scale.argtypes = [p, d]
+        L.ora_move.argtypes = [p, d, d, d, d, d, d, i, i, i]
+        L.ora_scale.argtypes = [p, d]
         L.ora_set_ground.argtypes = [p, i, d, d]
         L.ora_set_loads.argtypes = [p, i, p, p, p, p, p, p, p]
         L.ora_zint.argtypes = [d, d, p]
@@ -157,6 +159,16 @@ class Geometry:
     def wire(self, tag, ns, x1, y1, z1, x2, y2, z2, rad, rdel=1.0, rrad=1.0):
         self._L.ora_wire(self._g, tag, ns, x1, y1, z1, x2, y2, z2, rad, rdel, rrad)
 
+    def gm(self, itgi, nrpt, rox, roy, roz, xs, ys, zs, its=0):
+        """GM card: rotations in degrees (converted with the reference's
+        pi = 3.1415926536, as nec_gm_card does), translation in metres."""
+        d2r = (2 * 3.1415926536) / 360.0
+        if self._L.ora_move(self._g, rox * d2r, roy * d2r, roz * d2r, xs, ys, zs, its, nrpt, itgi) != 0:
+            raise RuntimeError(self._L.ora_last_error().decode())
+
+    def gs(self, f):
+        self._L.ora_scale(self._g, f)
+
     def complete(self, gpflag=0):
         if self._L.ora_geometry_complete(self._g, gpflag) != 0:
             raise RuntimeError(self._L.ora_last_error().decode())
@@ -269,21 +281,7 @@ class Geometry:
         spec = out = None
         res = {}
         if rp is not None:
-            spec = RpSpec(rp["n_theta"], rp["n_phi"], rp["theta0"], rp["phi0"], rp["dtheta"], rp["dphi"],
-                          rp.get("output_format", 0), rp.get("normalization", 0), rp.get("ipd", 0),
-                          rp.get("iavp", 0), rp.get("rfld", 0.0), rp.get("gnor", 0.0))
-            if spec.n_theta < 2 or spec.n_phi < 2:
-                spec.iavp = 0
-            npt = spec.n_theta * spec.n_phi
-            arrs = {k: np.zeros(npt) for k in ("gain", "pg_vert", "pg_horiz", "pg_tot", "pg_rhcp",
-                                               "pg_lhcp", "axial_ratio", "tilt", "avg_scale")}
-            arrs["sense"] = np.zeros(npt, np.int32)
-            arrs["e_theta"] = np.zeros(2 * npt)
-            arrs["e_phi"] = np.zeros(2 * npt)
-            out = RpOut()
-            for k, a in arrs.items():
-                setattr(out, k, a.ctypes.data)
-            res["pattern"] = arrs
+            spec, out, res["pattern"] = _rp_buffers(rp)
         rc = self._L.ora_run_voltage(self._g, f_mhz, rkh, int(use_ek), len(src), _ptr(segs), _ptr(v),
                                      ctypes.byref(spec) if spec is not None else None,
                                      ctypes.byref(out) if out is not None else None,
@@ -297,13 +295,49 @@ class Geometry:
         res["abc"] = abc
         res["timing"] = {k: getattr(tim, k) for k, _ in Timing._fields_}
         if out is not None:
-            pat = res["pattern"]
-            pat["e_theta"] = pat["e_theta"][0::2] + 1j * pat["e_theta"][1::2]
-            pat["e_phi"] = pat["e_phi"][0::2] + 1j * pat["e_phi"][1::2]
-            res["max_gain"] = out.max_gain
-            res["average_power_gain"] = out.average_power_gain
-            res["stats"] = list(out.stats)
+            _rp_finish(res, out)
         return res
+
+    def pattern(self, abc, f_mhz, input_power, rp, nthreads=1):
+        """The pattern of given current coefficients (6 n doubles, A B C per
+        segment as run_voltage's "abc") at f_mhz with the input power of
+        the solve: ora_pattern alone, free space, no loads."""
+        self._L.ora_frequency_scale(self._g, f_mhz)
+        spec, out, pat = _rp_buffers(rp)
+        abc = np.ascontiguousarray(abc, np.float64)
+        if self._L.ora_pattern(self._g, _ptr(abc), wavelength(f_mhz), input_power, 0.0, ctypes.byref(spec),
+                               ctypes.byref(out), nthreads) != 0:
+            raise RuntimeError(self._L.ora_last_error().decode())
+        res = {"pattern": pat}
+        _rp_finish(res, out)
+        return res
+
+
+def _rp_buffers(rp):
+    spec = RpSpec(rp["n_theta"], rp["n_phi"], rp["theta0"], rp["phi0"], rp["dtheta"], rp["dphi"],
+                  rp.get("output_format", 0), rp.get("normalization", 0), rp.get("ipd", 0),
+                  rp.get("iavp", 0), rp.get("rfld", 0.0), rp.get("gnor", 0.0))
+    if spec.n_theta < 2 or spec.n_phi < 2:
+        spec.iavp = 0
+    npt = spec.n_theta * spec.n_phi
+    arrs = {k: np.zeros(npt) for k in ("gain", "pg_vert", "pg_horiz", "pg_tot", "pg_rhcp",
+                                       "pg_lhcp", "axial_ratio", "tilt", "avg_scale")}
+    arrs["sense"] = np.zeros(npt, np.int32)
+    arrs["e_theta"] = np.zeros(2 * npt)
+    arrs["e_phi"] = np.zeros(2 * npt)
+    out = RpOut()
+    for k, a in arrs.items():
+        setattr(out, k, a.ctypes.data)
+    return spec, out, arrs
+
+
+def _rp_finish(res, out):
+    pat = res["pattern"]
+    pat["e_theta"] = pat["e_theta"][0::2] + 1j * pat["e_theta"][1::2]
+    pat["e_phi"] = pat["e_phi"][0::2] + 1j * pat["e_phi"][1::2]
+    res["max_gain"] = out.max_gain
+    res["average_power_gain"] = out.average_power_gain
+    res["stats"] = list(out.stats)
 
 
 def lu(Z, nthreads=1):
@@ -329,15 +363,20 @@ def geometry_from_deck(deck):
     return g
 
 
+def rp_from_card(card):
+    """run_voltage's rp dict from the nec_rp_card arguments (mode, n_theta,
+    n_phi, output_format, normalization, D, A, theta0, phi0, dtheta, dphi,
+    range, gnor)."""
+    (_, nth, nph, fmt, norm, ipd, iavp, th0, ph0, dth, dph, rfld, gnor) = card
+    return dict(n_theta=nth, n_phi=nph, theta0=th0, phi0=ph0, dtheta=dth, dphi=dph, output_format=fmt,
+                normalization=norm, ipd=ipd, iavp=iavp, rfld=rfld, gnor=gnor)
+
+
 def run_deck(deck, nthreads=1, with_rp=True, rkh=1.0, use_ek=False):
     """Oracle run of a necpp_amd.decks deck: returns run_voltage's dict."""
     g = geometry_from_deck(deck)
     seg = g.segment_number(*deck["source"])
-    rp = None
-    if with_rp and "rp" in deck:
-        (_, nth, nph, fmt, norm, ipd, iavp, th0, ph0, dth, dph, rfld, gnor) = deck["rp"]
-        rp = dict(n_theta=nth, n_phi=nph, theta0=th0, phi0=ph0, dtheta=dth, dphi=dph, output_format=fmt,
-                  normalization=norm, ipd=ipd, iavp=iavp, rfld=rfld, gnor=gnor)
+    rp = rp_from_card(deck["rp"]) if with_rp and "rp" in deck else None
     res = g.run_voltage(deck["freq_mhz"], [(seg, deck["voltage"])], rkh=rkh, use_ek=use_ek, rp=rp,
                         nthreads=nthreads)
     res["geometry"] = g

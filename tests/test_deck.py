@@ -110,56 +110,89 @@ def test_error_names_the_deck_line():
         api.nec_delete(ctx)
 
 
-KPI = 3.1415926536
+def oracle_geometry(cards, gpflag=0):
+    """The oracle's geometry from a list of ("GW", wire) / ("GM", args) /
+    ("GS", f) cards, applied in order to its segments."""
+    from oracle import pyoracle as po
+    g = po.Geometry()
+    for kind, a in cards:
+        if kind == "GW":
+            g.wire(*a)
+        elif kind == "GM":
+            g.gm(*a)
+        else:
+            g.gs(a)
+    g.complete(gpflag)
+    return g
 
 
-def gm(wires, itgi, nrpt, rox, roy, roz, xs, ys, zs, its=0):
-    """GM on wire endpoints, as the deck front end applies it."""
-    import math
-    d2r = lambda d: d * ((2 * KPI) / 360.0)
-    sps, cps = math.sin(d2r(rox)), math.cos(d2r(rox))
-    sth, cth = math.sin(d2r(roy)), math.cos(d2r(roy))
-    sph, cph = math.sin(d2r(roz)), math.cos(d2r(roz))
-    xx, xy, xz = cph * cth, cph * sth * sps - sph * cps, cph * sth * cps + sph * sps
-    yx, yy, yz = sph * cth, sph * sth * sps + cph * cps, sph * sth * cps - cph * sps
-    zx, zy, zz = -sth, cth * sps, cth * cps
-
-    def mv(w, add):
-        t, ns, x1, y1, z1, x2, y2, z2 = w[:8]
-        p1 = (x1 * xx + y1 * xy + z1 * xz + xs, x1 * yx + y1 * yy + z1 * yz + ys, x1 * zx + y1 * zy + z1 * zz + zs)
-        p2 = (x2 * xx + y2 * xy + z2 * xz + xs, x2 * yx + y2 * yy + z2 * yz + ys, x2 * zx + y2 * zy + z2 * zz + zs)
-        return (t + add if (t != 0 and add) else t, ns) + p1 + p2 + tuple(w[8:])
-
-    w0 = next((q for q, w in enumerate(wires) if its != 0 and w[0] == its), 0)
-    out = list(wires)
-    if nrpt == 0:
-        return out[:w0] + [mv(w, 0) for w in out[w0:]]
-    b0 = w0
-    for _ in range(nrpt):
-        b1 = len(out)
-        out += [mv(out[q], itgi) for q in range(b0, b1)]
-        b0 = b1
-    return out
+def same_as_oracle(ctx, g):
+    tab, ic1, ic2, tags = segs(ctx)
+    otab, oic1, oic2, otags = g.segments()
+    return (np.array_equal(tab, otab) and np.array_equal(ic1, oic1) and np.array_equal(ic2, oic2)
+            and np.array_equal(tags, otags))
 
 
 def test_gm_copies_and_moves():
+    # GM acts on segments (c_geometry::move): the deck, the API call and the
+    # oracle's restatement give the same segment tables bit for bit
     text = """CE
 GW 1 3 0 0 0 0 0 0.1 0.001
 GW 2 3 0 0.1 0 0 0.1 0.1 0.001
 GM 5 3 0 0 30 0.2 0 0 2
+GS 0 0 0.9
 GM 0 0 10 20 0 0 0 0.05 0
 GE 0
 EN
 """
     ctx, rc = run(text)
-    w = [(1, 3, 0, 0, 0, 0, 0, 0.1, 0.001, 1.0, 1.0), (2, 3, 0, 0.1, 0, 0, 0.1, 0.1, 0.001, 1.0, 1.0)]
-    w = gm(w, 5, 3, 0, 0, 30, 0.2, 0, 0, its=2)
-    w = gm(w, 0, 0, 10, 20, 0, 0, 0, 0.05)
-    assert len(w) == 5 and [x[0] for x in w] == [1, 2, 7, 12, 17]
-    ref = api_ctx(w)
+    w1 = (1, 3, 0, 0, 0, 0, 0, 0.1, 0.001, 1.0, 1.0)
+    w2 = (2, 3, 0, 0.1, 0, 0, 0.1, 0.1, 0.001, 1.0, 1.0)
+    g = oracle_geometry([("GW", w1), ("GW", w2), ("GM", (5, 3, 0, 0, 30, 0.2, 0, 0, 2)), ("GS", 0.9),
+                         ("GM", (0, 0, 10, 20, 0, 0, 0, 0.05, 0))])
     try:
         assert rc == 0, api.nec_error_message()
+        assert same_as_oracle(ctx, g)
+        _, _, _, tags = segs(ctx)
+        assert list(tags) == [1] * 3 + [2] * 3 + [7] * 3 + [12] * 3 + [17] * 3
+    finally:
+        api.nec_delete(ctx)
+
+
+def test_gm_api_matches_deck():
+    text = "CE\nGW 1 4 0 0 0 0.4 0 0 0.001\nGM 3 2 0 0 45 0 0 0.1 1\nGM 0 0 0 90 0 0.5 0 0 4\nGE 0\nEN\n"
+    ctx, rc = run(text)
+    ref = api.nec_create()
+    try:
+        assert rc == 0, api.nec_error_message()
+        assert api.nec_wire(ref, 1, 4, 0, 0, 0, 0.4, 0, 0, 0.001, 1.0, 1.0) == 0
+        assert api.nec_gm_card(ref, 3, 2, 0, 0, 45, 0, 0, 0.1, 1) == 0, api.nec_error_message()
+        assert api.nec_gm_card(ref, 0, 0, 0, 90, 0, 0.5, 0, 0, 4) == 0, api.nec_error_message()
+        assert api.nec_geometry_complete(ref, 0) == 0
         assert same_geometry(ctx, ref)
+        _, _, _, tags = segs(ref)
+        assert list(tags) == [1] * 4 + [4] * 4 + [7] * 4
     finally:
         api.nec_delete(ctx)
         api.nec_delete(ref)
+
+
+def test_gm_tag_that_no_segment_has_is_an_error():
+    # c_geometry::move looks the tag up with get_segment_number
+    # (src/c_geometry.cpp:76-98, :865), which throws for an unknown tag
+    ctx, rc = run("CE\nGW 1 4 0 0 0 0.4 0 0 0.001\nGM 0 1 0 0 45 0 0 0 9\nGE 0\nEN\n")
+    try:
+        assert rc == 1
+        msg = api.nec_error_message()
+        assert "NO SEGMENT HAS AN ITAG OF 9" in msg and "deck line 3 (GM)" in msg, msg
+    finally:
+        api.nec_delete(ctx)
+    ctx = api.nec_create()
+    try:
+        assert api.nec_wire(ctx, 1, 4, 0, 0, 0, 0.4, 0, 0, 0.001, 1.0, 1.0) == 0
+        assert api.nec_gm_card(ctx, 0, 1, 0, 0, 45, 0, 0, 0, 9) == 1
+        assert "NO SEGMENT HAS AN ITAG OF 9" in api.nec_error_message()
+    finally:
+        api.nec_delete(ctx)
+
+

@@ -127,26 +127,28 @@ def test_deck_equals_api_calls():
 def test_biquad_deck_with_gm_copies_matches_oracle():
     # testharness/data/biquad.nec: GW cards and GM copies (a reflector mesh),
     # a 73 x 73 pattern with averaging, at 300 MHz
-    from test_deck import gm
-    wires = []
+    # the oracle builds the structure with its own restatement of the
+    # segment-level move (ora_move, pinned in test_oracle_geometry.py)
+    g = po.Geometry()
     for line in deck_text("biquad.nec").splitlines():
         f = line.split()
         if not f:
             continue
         if f[0] == "GW":
             t, ns = int(f[1]), int(f[2])
-            wires.append((t, ns) + tuple(float(v) for v in f[3:10]) + (1.0, 1.0))
+            g.wire(t, ns, *(float(v) for v in f[3:10]), 1.0, 1.0)
         elif f[0] == "GM":
             v = [float(x) for x in f[3:10]] + [0.0] * 7
-            wires = gm(wires, int(f[1]), int(f[2]), *v[:6], its=int(v[6] + 0.5))
+            g.gm(int(f[1]), int(f[2]), *v[:6], its=int(v[6] + 0.5))
+    g.complete(0)
     ctx = api.nec_create()
     try:
         rc = api.nec_gpu_deck(ctx, deck_text("biquad.nec").encode())
         assert rc == 0, api.nec_error_message()
-        g = po.Geometry()
-        for w in wires:
-            g.wire(*w)
-        g.complete(0)
+        tab, ic1, ic2, tags = api.gpu_segments(ctx)
+        otab, oic1, oic2, otags = g.segments()
+        assert np.array_equal(tab, otab) and np.array_equal(tags, otags)
+        assert np.array_equal(ic1, oic1) and np.array_equal(ic2, oic2)
         rp = dict(n_theta=73, n_phi=73, theta0=-90.0, phi0=90.0, dtheta=5.0, dphi=5.0, output_format=1, iavp=1,
                   rfld=10000.0)
         o = g.run_voltage(300.0, [(g.segment_number(9, 1), 1.0)], rp=rp, nthreads=8)

@@ -137,23 +137,3 @@ def test_refinement_limit_is_reported():
         assert "refinement did not reach" in api.nec_error_message()
     finally:
         api.nec_delete(ctx)
-
-
-def test_c5_grid_fp64_and_mixed_match_oracle_fixture():
-    # N = 16,020 (4.1 GB FP64 matrix): both paths against the oracle fixture
-    f = json.load(open(os.path.join(GOLDEN, "c5_oracle.json")))
-    ocur = np.load(os.path.join(GOLDEN, "c5_oracle_current.npy"))
-    deck = decks.grid(90, rp=None)
-    for precision in (0, 1):
-        ctx, rc = run(deck, precision, with_rp=False)
-        try:
-            assert rc == 0, api.nec_error_message()
-            z = zin(ctx)
-            print("C5 precision", precision, "Z_in", z, "rel", rel(z, complex(*f["zin"])),
-                  "stages", api.gpu_stage_times(ctx), api.gpu_refinement(ctx) if precision else "")
-            assert rel(z, complex(*f["zin"])) < TOL_ZIN
-            cur = api.gpu_currents(ctx, 0)
-            tol = 1e-12 if precision == 0 else TOL_CUR
-            assert np.linalg.norm(cur - ocur) <= tol * np.linalg.norm(ocur)
-        finally:
-            api.nec_delete(ctx)

@@ -104,6 +104,38 @@ def test_grid_structure_matches_oracle():
         api.nec_delete(ctx)
 
 
+@pytest.mark.parametrize("then", ["xq", "rp"])
+def test_near_field_at_every_frequency_of_a_sweep(then):
+    # FR with 3 frequencies, NE, then XQ or RP: the near field is evaluated
+    # at every frequency (src/nec_context.cpp:1741-1757), and after RP the
+    # pattern of every frequency follows it
+    freqs = (250.0, 275.0, 300.0)
+    ctx = api.nec_create()
+    try:
+        assert api.nec_wire(ctx, *DIPOLE) == 0 and api.nec_geometry_complete(ctx, 0) == 0
+        assert api.nec_fr_card(ctx, 0, 3, freqs[0], 25.0) == 0
+        assert api.nec_ex_card(ctx, 0, 1, 11, 0, 1.0, 0.0, 0, 0, 0, 0) == 0
+        assert api.nec_ne_card(ctx, 0, 3, 1, 4, 0.01, 0.0, -0.15, 0.03, 0.0, 0.1) == 0, api.nec_error_message()
+        assert api.nec_gpu_near_field(ctx, 0, None, 0) == -1  # nothing runs before XQ / RP
+        if then == "xq":
+            assert api.nec_xq_card(ctx, 0) == 0, api.nec_error_message()
+        else:
+            assert api.nec_rp_card(ctx, 0, 5, 4, 1, 0, 0, 0, 0.0, 0.0, 45.0, 90.0, 0.0, 0.0) == 0, \
+                api.nec_error_message()
+        for k, f in enumerate(freqs):
+            pts, E = near(ctx, k)
+            assert pts.shape == (12, 3)
+            check(E, oracle_near([DIPOLE], 0, f, 11, pts))
+            if then == "rp":
+                assert api.nec_gain_max(ctx, k) > 0.0
+        assert api.nec_gpu_near_field(ctx, 3, None, 0) == -1
+        # the card is spent: a later XQ adds no near field
+        assert api.nec_xq_card(ctx, 0) == 0
+        assert api.nec_gpu_near_field(ctx, 3, None, 0) == -1
+    finally:
+        api.nec_delete(ctx)
+
+
 def test_near_field_refusals():
     ctx = api.nec_create()
     try:
