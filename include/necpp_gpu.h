@@ -186,6 +186,17 @@ long nec_gpu_batch_adjoint(const nec_gpu_batch* batch, double theta_deg, double 
 long long necgpu_zgetrf_workspace(int n);
 long necgpu_zgetrf(double* a, long long lda, int n, int* piv, int* info, void* work, void* stream);
 
+/* necgpu_zgetrf without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid allocation).  No counterpart in the reference.
+ * The same factorisation and result conventions as necgpu_zgetrf, on the
+ * caller's stream only (no look-ahead stream), with the workspace held in a
+ * fixed array of the library on each device (about 4.8 MiB, kept until the
+ * process ends).  n <= 4,096: a larger n returns 1 with a message.  Calls
+ * that overlap on one device share that workspace and race; serialise them
+ * (one stream, or events).  piv and info as for necgpu_zgetrf. */
+long necgpu_zgetrf_noalloc(double* a, long long lda, int n, int* piv, int* info, void* stream);
+
 /* The mixed path's complex FP32 LU on caller device memory: a is n x n
  * complex64 (row-major, leading dimension lda), factored in place; piv,
  * info as for necgpu_zgetrf; work: necgpu_zgetrf_workspace(n) bytes.

@@ -496,6 +496,18 @@ long necgpu_zgetrf(double* a, long long lda, int n, int* piv, int* info, void* w
   });
 }
 
+long necgpu_zgetrf_noalloc(double* a, long long lda, int n, int* piv, int* info, void* stream) {
+  return guarded([&] {
+    const int rc = necgpu::launch_lu_fp64_noalloc(a, lda, n, piv, info, static_cast<hipStream_t>(stream));
+    if (rc == 3)
+      throw NecError("necgpu_zgetrf_noalloc: n = " + std::to_string(n) + " exceeds the largest order of its fixed " +
+                     "workspace (" + std::to_string(necgpu::kNoAllocMaxN) + ")");
+    if (rc == 4) throw NecError("necgpu_zgetrf_noalloc: the device address of the fixed workspace is unavailable");
+    if (rc == 2) throw NecError("necgpu_zgetrf_noalloc: n too large for the panel");
+    if (rc) necgpu::hip_check(hipGetLastError(), "necgpu_zgetrf_noalloc");
+  });
+}
+
 long necgpu_cgetrf(float* a, long long lda, int n, int* piv, int* info, void* work, int pivot_mode, void* stream) {
   return guarded([&] {
     if (pivot_mode != 0 && pivot_mode != 1) throw NecError("necgpu_cgetrf: pivot_mode must be 0 or 1");

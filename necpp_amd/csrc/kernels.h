@@ -168,6 +168,13 @@ int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int u
 size_t lu_workspace_bytes(int n);
 int* lu_status(void* work, int n);
 int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream);
+// The same LU with its workspace in a fixed device array of the library
+// (lu_noalloc_scratch_bytes() per device, for n <= kNoAllocMaxN) and no side
+// stream: no allocation of any kind.  Returns 3 for n > kNoAllocMaxN, 4 if
+// the array's device address cannot be found, else as launch_lu_fp64.
+constexpr int kNoAllocMaxN = 4096;
+size_t lu_noalloc_scratch_bytes();
+int launch_lu_fp64_noalloc(double* A, long long lda, int n, int* piv, int* info, hipStream_t stream);
 // The same factorisation of an FP32 matrix (the mixed-precision path): FP32
 // storage and trailing update on v_mfma_f32_16x16x4f32.  pivot_mode 0: the
 // same pivot rule (partial pivoting); 1: pivots searched inside each panel's

@@ -21,6 +21,7 @@
 //              workgroup, operands staged in LDS as split re/im planes.
 #include <climits>
 #include <cstdlib>
+#include <mutex>
 
 #include "kernels.h"
 #include "nec_math.h"
@@ -61,7 +62,7 @@ struct Xchg {
   double* diag;
 };
 
-__host__ __device__ inline size_t xchg_area_bytes(int Gcap) {
+__host__ __device__ constexpr size_t xchg_area_bytes(int Gcap) {
   return ((size_t)NB * Gcap * 8 + 255) / 256 * 256 + ((size_t)NB * Gcap * 4 + 255) / 256 * 256 +
          (size_t)NB * Gcap * 2 * NB * 8 + (size_t)NB * 2 * NB * 8;
 }
@@ -143,12 +144,13 @@ __device__ __forceinline__ double canonical(double v) {
   return (v != v) ? __longlong_as_double(0x7FF8000000000000LL) : v;
 }
 
-__global__ void lu_init_kernel(int* perm, int n, int* status, int* ticket, char* xbase, int Gcap, int G0) {
+__global__ void lu_init_kernel(int* perm, int n, int* status, int* ticket, int* info, char* xbase, int Gcap, int G0) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) perm[i] = i;
   if (i == 0) {
     *status = 0;
     *ticket = 0;
+    *info = 0;
   }
   clear_xchg(xchg_area(xbase, Gcap, 0), Gcap, G0, i, gridDim.x * blockDim.x);
 }
@@ -1386,7 +1388,7 @@ __global__ void mfma_probe_f32_kernel(const float* a, const float* b, float* d) 
 size_t align256(size_t b) { return (b + 255) & ~(size_t)255; }
 
 // exchange-area capacity: the FP64 panel's group count, the larger one
-int gcap_for(int n) { return std::max(1, (n + RMAX - 1) / RMAX); }
+constexpr int gcap_for(int n) { return (n + RMAX - 1) / RMAX > 1 ? (n + RMAX - 1) / RMAX : 1; }
 
 LuWork carve(void* work, int n) {
   char* p = static_cast<char*>(work);
@@ -1486,7 +1488,8 @@ Lookahead* lookahead_for_device() {
 
 int* lu_status(void* work, int n) { return carve(work, n).status; }
 
-size_t lu_workspace_bytes(int n) { return 256 + 2 * xchg_area_bytes(gcap_for(n)); }
+constexpr size_t lu_workspace_bytes_c(int n) { return 256 + 2 * xchg_area_bytes(gcap_for(n)); }
+size_t lu_workspace_bytes(int n) { return lu_workspace_bytes_c(n); }
 
 int launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t stream) {
   hipLaunchKernelGGL(mfma_probe_kernel, dim3(1), dim3(64), 0, stream, a, b, d);
@@ -1824,7 +1827,8 @@ int launch_mfma_probe_f32(const float* a, const float* b, float* d, hipStream_t 
 namespace {
 
 template <typename T>
-int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream, bool local_pivot) {
+int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream, bool local_pivot,
+              bool allow_lookahead = true) {
   constexpr int RMAX_T = rmax_of<T>();
   static bool attr_set = false;
   static int cu_count = 0;
@@ -1854,9 +1858,9 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
   const LuWork w = carve(work, n);
   const int npanels = (n + NB - 1) / NB;
   launch_k(K_LU_INIT, lu_init_kernel, dim3(std::max((n + 255) / 256, 64)), dim3(256), 0, stream, perm, n, w.status,
-           w.ticket, w.xbase, w.Gcap, G0);
+           w.ticket, info, w.xbase, w.Gcap, G0);
   const int forced = panel_stride_override();
-  Lookahead* la = (npanels > 1 && lookahead_enabled()) ? lookahead_for_device() : nullptr;
+  Lookahead* la = (allow_lookahead && npanels > 1 && lookahead_enabled()) ? lookahead_for_device() : nullptr;
   auto panel = [&](int k0, hipStream_t s) {
     const int nb = std::min(NB, n - k0);
     if constexpr (sizeof(T) == 4) {  // block-local pivots: the FP32 path only
@@ -2000,6 +2004,49 @@ int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* w
 int launch_lu_fp32(float* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream,
                    int pivot_mode) {
   return launch_lu<float>(A, lda, n, piv, info, work, stream, pivot_mode == 1);
+}
+
+// ---------------------------------------------------------------------
+// The FP64 LU without any allocation, for callers that cannot allocate
+// device memory: the workspace (status word, panel exchange slots) is the
+// file-scope array below, of the size the largest supported order needs.
+// Each device holds its own copy for the life of the process; its address
+// on the current device comes from hipGetSymbolAddress once per device (the
+// array's name in host code is only a stand-in, never a device address).
+// lu_init_kernel clears what must start at zero on every call, so a call
+// does not depend on the previous call's contents, but two calls that
+// overlap on one device share the array and race.  Everything runs on the
+// caller's stream (no look-ahead side stream is created).
+namespace {
+constexpr size_t kNoAllocBytes = lu_workspace_bytes_c(kNoAllocMaxN);
+static __device__ __align__(256) char scratch[kNoAllocBytes];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// the current device's copy of `scratch`, looked up once per device
+int scratch_on_current_device(void** out) {
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kMaxDevices) return 1;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (!g_scratch_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p) return 1;
+    g_scratch_ptr[dev] = p;
+  }
+  *out = g_scratch_ptr[dev];
+  return 0;
+}
+}  // namespace
+
+size_t lu_noalloc_scratch_bytes() { return kNoAllocBytes; }
+
+int launch_lu_fp64_noalloc(double* A, long long lda, int n, int* piv, int* info, hipStream_t stream) {
+  if (n > kNoAllocMaxN) return 3;
+  void* work = nullptr;
+  if (scratch_on_current_device(&work)) return 4;
+  return launch_lu<double>(A, lda, n, piv, info, work, stream, false, false);
 }
 
 }  // namespace necgpu

@@ -77,6 +77,7 @@ _SIGS = [
     ("necgpu_zgetrf_workspace", _LL, [_I]),
     ("necgpu_zgetrf", _L, [_P, _LL, _I, _P, _P, _P, _P]),
     ("necgpu_cgetrf", _L, [_P, _LL, _I, _P, _P, _P, _I, _P]),
+    ("necgpu_zgetrf_noalloc", _L, [_P, _LL, _I, _P, _P, _P]),
     ("nec_gpu_deck", _L, [_P, ctypes.c_char_p]),
     ("nec_gpu_near_field", _L, [_P, _I, _P, _L]),
     ("necgpu_check_mfma_f64", _L, [_P, _P, _P, _P]),
