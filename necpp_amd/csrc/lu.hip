@@ -155,6 +155,46 @@ __global__ void lu_init_kernel(int* perm, int n, int* status, int* ticket, int* 
   clear_xchg(xchg_area(xbase, Gcap, 0), Gcap, G0, i, gridDim.x * blockDim.x);
 }
 
+// The wave's best (v, i): largest v, ties to the smallest i; v >= 0 or the
+// sentinel -1 (no candidate, with i = INT_MAX), so the bit patterns of v
+// order as signed 64-bit integers.  DPP row shifts then row broadcasts, a
+// few cycles per step where a shuffle is an LDS round trip; the result is
+// wave-uniform.
+__device__ __forceinline__ void wave_argmax(double& v, int& i) {
+  long long b = __double_as_longlong(v);
+  int hi = (int)(b >> 32), lo = (int)(b & 0xffffffffll), ix = i;
+#define NECGPU_ARGMAX_STEP(CTRL, ROWMASK)                                           \
+  {                                                                               \
+    const int h2 = __builtin_amdgcn_update_dpp(hi, hi, CTRL, ROWMASK, 0xf, false); \
+    const int l2 = __builtin_amdgcn_update_dpp(lo, lo, CTRL, ROWMASK, 0xf, false); \
+    const int i2 = __builtin_amdgcn_update_dpp(ix, ix, CTRL, ROWMASK, 0xf, false); \
+    const long long a = ((long long)hi << 32) | (unsigned int)lo;                  \
+    const long long o = ((long long)h2 << 32) | (unsigned int)l2;                  \
+    const bool take = o > a || (o == a && i2 < ix);                                 \
+    hi = take ? h2 : hi;                                                            \
+    lo = take ? l2 : lo;                                                            \
+    ix = take ? i2 : ix;                                                            \
+  }
+  NECGPU_ARGMAX_STEP(0x111, 0xf)  // row_shr:1
+  NECGPU_ARGMAX_STEP(0x112, 0xf)  // row_shr:2
+  NECGPU_ARGMAX_STEP(0x114, 0xf)  // row_shr:4
+  NECGPU_ARGMAX_STEP(0x118, 0xf)  // row_shr:8
+  NECGPU_ARGMAX_STEP(0x142, 0xa)  // row_bcast:15
+  NECGPU_ARGMAX_STEP(0x143, 0xc)  // row_bcast:31
+#undef NECGPU_ARGMAX_STEP
+  hi = __builtin_amdgcn_readlane(hi, 63);
+  lo = __builtin_amdgcn_readlane(lo, 63);
+  i = __builtin_amdgcn_readlane(ix, 63);
+  v = __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
 // Factor panel columns [k0, k0+nb) over rows [k0, n) with G workgroups; the
 // active workgroup g is launch block g*stride.  Rows owned by g:
 // [k0 + g*R, min(n, k0 + (g+1)*R)), at most rmax_of<T>(), one thread per
@@ -173,8 +213,7 @@ __global__ void lu_init_kernel(int* perm, int n, int* status, int* ticket, int* 
 //     the column-(c+1) candidate; the update of columns c+2.. is left to B.
 template <typename T>
 __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(T* __restrict__ A, long long lda, int n, int k0,
-                                                                  int nb, int G, int R, int stride, int* piv, int* perm,
-                                                                  int* info, char* xbase, int Gcap, int parity,
+                                                                  int nb, int G, int R, int stride, int* piv, int* info, char* xbase, int Gcap, int parity,
                                                                   int Gnext, int* status) {
   extern __shared__ double lds_raw[];
   constexpr int RMAX_T = rmax_of<T>();
@@ -232,12 +271,14 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(T* __restrict__
     const int k = k0 + c;
     // A. candidate reduction (rows are held by waves 0 .. NRW-1 only)
     if (wave < NRW) {
-      double m = has ? best : -1.0;
-      for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-      const unsigned long long hit = __ballot(has && best == m);
+      // rows follow thread order, so the lowest row at the maximum is the
+      // first; NaN magnitudes are never candidates
+      double m = (has && best >= 0.0) ? best : -1.0;
+      int mi = (has && best >= 0.0) ? r0 + t : INT_MAX;
+      wave_argmax(m, mi);
       if (lane == 0) {
         s_best[wave] = m;
-        s_idx[wave] = hit ? r0 + wave * 64 + (__ffsll((long long)hit) - 1) : INT_MAX;
+        s_idx[wave] = mi;
       }
     }
     __syncthreads();
@@ -345,22 +386,21 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(T* __restrict__
           e3 = a3;
         }
       }
-      // first maximum: wave max, then the lowest row among the lanes at it
-      double m = wv;
-      for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off, 64));
-      const bool at = wi != INT_MAX && wv == m;
-      int mi = at ? wi : INT_MAX;
-      if (G > 64)  // lanes hold several groups: order by row, not by lane
-        for (int off = 32; off > 0; off >>= 1) mi = min(mi, __shfl_xor(mi, off, 64));
-      const unsigned long long hit = __ballot(at && wi == mi);
+      // first maximum over the groups (lowest row at it), then the winner's
+      // words from its lane
+      const int mine = wi;
+      wave_argmax(wv, wi);
+      const unsigned long long hit = __ballot(wi != INT_MAX && mine == wi);
       const int wl = hit ? (__ffsll((long long)hit) - 1) : 0;
-      wv = hit ? m : -1.0;
-      wi = hit ? __shfl(wi, wl, 64) : INT_MAX;
-      int wg = __shfl(wgl, wl, 64);
-      e0 = __shfl(e0, wl, 64);
-      e1 = __shfl(e1, wl, 64);
-      e2 = __shfl(e2, wl, 64);
-      e3 = __shfl(e3, wl, 64);
+      if (!hit) {
+        wv = -1.0;
+        wi = INT_MAX;
+      }
+      int wg = __builtin_amdgcn_readlane(wgl, wl);
+      e0 = readlane_d(e0, wl);
+      e1 = readlane_d(e1, wl);
+      e2 = readlane_d(e2, wl);
+      e3 = readlane_d(e3, wl);
       if (lane == 0) {
         int p = wi;
         if (p == INT_MAX || !(wv > 0.0)) {
@@ -386,14 +426,7 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(T* __restrict__
         s_dg[1] = d1;
         s_dg[2] = d2;
         s_dg[3] = d3;
-        if (g == 0) {
-          piv[k] = p;
-          if (p != k) {
-            const int tmp = perm[k];
-            perm[k] = perm[p];
-            perm[p] = tmp;
-          }
-        }
+        if (g == 0) piv[k] = p;
       }
     } else if (pending) {
       // B1. the pending update for every other owned row below k - 1
@@ -535,7 +568,8 @@ __global__ void __launch_bounds__(PANEL_THREADS) lu_panel_kernel(T* __restrict__
 constexpr int LASWP_COLS = 32;
 template <typename T>
 __global__ void __launch_bounds__(256) lu_laswp_kernel(T* __restrict__ A, long long lda, int k0, int nb,
-                                                       const int* __restrict__ piv, int c0, int c1, int c2, int c3) {
+                                                       const int* __restrict__ piv, int c0, int c1, int c2, int c3,
+                                                       int* __restrict__ perm) {
   __shared__ int s_dst[2 * NB], s_src[2 * NB];
   __shared__ int s_moves;
   const int t = threadIdx.x;
@@ -594,6 +628,13 @@ __global__ void __launch_bounds__(256) lu_laswp_kernel(T* __restrict__ A, long l
   }
   __syncthreads();
   const int moves = s_moves;
+  if (perm && blockIdx.x == 0) {
+    // the composed permutation takes the same moves (one launch per panel
+    // passes it): every source entry read before any destination is written
+    const int v = t < moves ? perm[s_src[t]] : 0;
+    __syncthreads();
+    if (t < moves) perm[s_dst[t]] = v;
+  }
   const int w1 = c1 - c0;
   const int ncols = w1 + (c3 - c2);
   const int cbase = blockIdx.x * LASWP_COLS;
@@ -1888,13 +1929,17 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
     if (forced > 0 && forced * G <= cu_count) stride = forced;
     const int Gnext = (k0 + nb < n) ? (n - k0 - nb + RMAX_T - 1) / RMAX_T : 0;
     launch_k(K_LU_PANEL, lu_panel_kernel<T>, dim3(G * stride), dim3(PANEL_THREADS), (uint32_t)lds_bytes, s, A, lda, n,
-             k0, nb, G, R, stride, piv, perm, info, w.xbase, w.Gcap, (k0 / NB) & 1, Gnext, w.status);
+             k0, nb, G, R, stride, piv, info, w.xbase, w.Gcap, (k0 / NB) & 1, Gnext, w.status);
   };
-  auto laswp = [&](int k0, int nb, int c0, int c1, int c2, int c3, hipStream_t s) {
+  // row exchanges of panel k0 on columns [c0, c1) and [c2, c3); with_perm:
+  // this launch also applies them to the composed permutation (the cooperative
+  // panel leaves that to one launch per panel; the block-local one keeps it)
+  auto laswp = [&](int k0, int nb, int c0, int c1, int c2, int c3, hipStream_t s, bool with_perm) {
     const int cols = (c1 - c0) + (c3 - c2);
-    if (cols > 0)
-      launch_k(K_LU_LASWP, lu_laswp_kernel<T>, dim3((cols + LASWP_COLS - 1) / LASWP_COLS), dim3(256), 0, s, A, lda, k0, nb,
-               (const int*)piv, c0, c1, c2, c3);
+    int* pm = (with_perm && !local_pivot) ? perm : nullptr;
+    if (cols > 0 || pm)
+      launch_k(K_LU_LASWP, lu_laswp_kernel<T>, dim3(std::max(1, (cols + LASWP_COLS - 1) / LASWP_COLS)), dim3(256), 0, s,
+               A, lda, k0, nb, (const int*)piv, c0, c1, c2, c3, pm);
   };
   auto update = [&](int k0, int nb, int cbeg, int cend, int max_groups, hipStream_t s, bool tile_gemm) {
     // trailing rows [k0+nb, n), columns [cbeg, cend)
@@ -1944,7 +1989,7 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
     for (int k0 = 0; k0 < n; k0 += NB) {
       const int nb = std::min(NB, n - k0);
       const int next = k0 + nb;
-      laswp(k0, nb, 0, k0, next, n, stream);
+      laswp(k0, nb, 0, k0, next, n, stream, true);
       if (next >= n) break;
       update(k0, nb, next, n, INT_MAX, stream, gemm_mode != 0);
       if (la && groups(next) <= la_max_g) {
@@ -1974,16 +2019,16 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
     const int next = k0 + nb;
     if (hipStreamWaitEvent(stream, la->panel_done, 0) != hipSuccess) return 1;
     if (next >= n) {
-      laswp(k0, nb, 0, k0, n, n, stream);
+      laswp(k0, nb, 0, k0, n, n, stream, true);
       break;
     }
     const int far = next + std::min(NB, n - next);
     if (k0 > kla && hipStreamWaitEvent(la->side, la->rest_done, 0) != hipSuccess) return 1;
-    laswp(k0, nb, next, far, far, far, la->side);
+    laswp(k0, nb, next, far, far, far, la->side, false);
     next_update(k0, nb, next, far, la->side);
     panel(next, la->side);
     if (hipEventRecord(la->panel_done, la->side) != hipSuccess) return 1;
-    laswp(k0, nb, 0, k0, far, n, stream);
+    laswp(k0, nb, 0, k0, far, n, stream, true);
     const int Gnext = (n - next + RMAX_T - 1) / RMAX_T;
     const int free_cus = std::max(1, cu_count - Gnext);
     const int tiles = ((n - next + 63) / 64) * ((n - far + 63) / 64);
