@@ -16,6 +16,15 @@
 // while b waits, so that b's critical path is two small products.
 // Workgroups are dispatched in block order, so every block a workgroup
 // waits on is already resident or done.
+//
+// Multiplying by a computed inverse leaves a residual of about
+// u cond(T) |T| |x| (u = 2^-53, T the diagonal block), where substitution
+// leaves u |T| |x|.  The blocks of a well-posed structure have cond(T) of a
+// few hundred, but a nearly singular structure (wires thicker than their
+// segments are long) has blocks with cond(T) up to 1e17, whose inverse
+// would swamp the solution.  So the inversion also estimates
+// ||T||_inf ||T^-1||_inf, and a block above kInvCondMax is solved by
+// substitution (64 dependent steps in one wave) instead.
 #include "kernels.h"
 #include "nec_math.h"
 
@@ -24,16 +33,28 @@ namespace necgpu {
 namespace {
 
 constexpr int SB = 64;
+constexpr double kInvCondMax = 1.0e3;
+
+// x from lane l of the wave (l uniform)
+__device__ __forceinline__ double readlane_d(double x, int l) {
+  const long long b = __double_as_longlong(x);
+  const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+  const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 
 __device__ __forceinline__ double ld(const double* p) { return *p; }
 __device__ __forceinline__ double ld(const float* p) { return static_cast<double>(*p); }
 
 // Inverse of diagonal block b of L (blockIdx.y == 0, unit lower) or U
 // (blockIdx.y == 1, upper with its diagonal), padded to 64 x 64 with the
-// identity.  Lane j computes column j by substitution on e_j.
+// identity.  Lane j computes column j by substitution on e_j.  subst[y nblk
+// + b] = 1 when the block's condition estimate (infinity norms, |re| + |im|
+// per entry) exceeds kInvCondMax or is not finite: that block is then
+// solved by substitution.
 template <typename TF>
 __global__ void __launch_bounds__(64) diag_inverse_kernel(const TF* __restrict__ LU, long long lda, int n,
-                                                          double* __restrict__ inv) {
+                                                          double* __restrict__ inv, int* __restrict__ subst) {
   __shared__ double sT[SB * SB * 2];  // the block, [r][q]
   __shared__ double sX[SB * SB * 2];  // its inverse, [r][j]
   const int b = blockIdx.x;
@@ -90,6 +111,24 @@ __global__ void __launch_bounds__(64) diag_inverse_kernel(const TF* __restrict__
   }
   double* dst = inv + ((size_t)(upper ? nblk : 0) + b) * SB * SB * 2;
   for (int e = j; e < SB * SB * 2; e += 64) dst[e] = sX[e];
+  __syncthreads();
+  // row sums of |T| and |T^-1| for row j (the unit diagonal of L counted),
+  // then their maxima over the wave
+  double nt = 0.0, nx = 0.0;
+  for (int q = 0; q < SB; ++q) {
+    const bool in_t = upper ? (q >= j) : (q < j);
+    if (in_t) nt += fabs(sT[2 * (j * SB + q)]) + fabs(sT[2 * (j * SB + q) + 1]);
+    nx += fabs(sX[2 * (j * SB + q)]) + fabs(sX[2 * (j * SB + q) + 1]);
+  }
+  if (!upper || j >= m) nt += 1.0;  // the unit diagonal of L; the identity padding of U
+  // a NaN would be dropped by fmax: count it as infinite
+  if (!(nt <= 1.0e308)) nt = __builtin_inf();
+  if (!(nx <= 1.0e308)) nx = __builtin_inf();
+  for (int off = 32; off >= 1; off >>= 1) {
+    nt = fmax(nt, __shfl_xor(nt, off));
+    nx = fmax(nx, __shfl_xor(nx, off));
+  }
+  if (j == 0) subst[(upper ? nblk : 0) + b] = (nt * nx <= kInvCondMax) ? 0 : 1;
 }
 
 // x values and flags travel through the device coherence point: written
@@ -110,9 +149,10 @@ __device__ __forceinline__ double xload(const double* p) {
 template <typename TF, bool lower>
 __global__ void __launch_bounds__(256) trsv_kernel(const TF* __restrict__ LU, long long lda, int n,
                                                    const int* __restrict__ perm, const double* rhs, double* out,
-                                                   const double* __restrict__ inv_all, unsigned* flags,
+                                                   const double* __restrict__ inv_all,
+                                                   const int* __restrict__ subst, unsigned* flags,
                                                    unsigned epoch, int* status) {
-  __shared__ double sInv[SB * SB * 2];   // inverse of the diagonal block
+  __shared__ double sInv[SB * SB * 2];   // inverse of the diagonal block (or the block itself)
   __shared__ double sNb[SB * SB * 2];    // the neighbour block
   __shared__ double sPart[4][SB * 2];
   __shared__ double sAcc[SB * 2];
@@ -126,10 +166,26 @@ __global__ void __launch_bounds__(256) trsv_kernel(const TF* __restrict__ LU, lo
   const int t = threadIdx.x;
   const int r = t & 63;     // row within the block
   const int part = t >> 6;  // column quarter of this thread
-  // stage the inverse and the neighbour block while earlier blocks finish
+  const bool by_subst = subst[(lower ? 0 : nblk) + b] != 0;
+  // stage the inverse (or, to substitute, the block padded with the
+  // identity) and the neighbour block while earlier blocks finish
   {
-    const double* src = inv_all + ((size_t)(lower ? 0 : nblk) + b) * SB * SB * 2;
-    for (int e = t; e < SB * SB * 2; e += 256) sInv[e] = src[e];
+    if (by_subst) {
+      for (int e = t; e < SB * SB; e += 256) {
+        const int i = e / SB, q = e % SB;
+        double vr = (i == q) ? 1.0 : 0.0, vi = 0.0;
+        if (i < rows && q < rows) {
+          const TF* v = LU + 2 * ((size_t)(r0 + i) * lda + r0 + q);
+          vr = ld(v);
+          vi = ld(v + 1);
+        }
+        sInv[2 * e] = vr;
+        sInv[2 * e + 1] = vi;
+      }
+    } else {
+      const double* src = inv_all + ((size_t)(lower ? 0 : nblk) + b) * SB * SB * 2;
+      for (int e = t; e < SB * SB * 2; e += 256) sInv[e] = src[e];
+    }
     if (has_nb) {
       const int c0 = nb * SB;
       const int cols = min(SB, n - c0);
@@ -234,21 +290,55 @@ __global__ void __launch_bounds__(256) trsv_kernel(const TF* __restrict__ LU, lo
     sAcc[2 * t + 1] = (t < rows) ? vi - si : 0.0;
   }
   __syncthreads();
-  // x_b = inverse * acc
-  double xr = 0.0, xi = 0.0;
-  for (int j = part; j < SB; j += 4) {
-    const double ir = sInv[2 * (r * SB + j)], ii = sInv[2 * (r * SB + j) + 1];
-    const double ar = sAcc[2 * j], ai = sAcc[2 * j + 1];
-    xr += ir * ar - ii * ai;
-    xi += ir * ai + ii * ar;
-  }
-  sPart[part][2 * r] = xr;
-  sPart[part][2 * r + 1] = xi;
-  __syncthreads();
-  if (t < rows) {
-    xstore(out + 2 * (r0 + t), ((sPart[0][2 * t] + sPart[1][2 * t]) + sPart[2][2 * t]) + sPart[3][2 * t]);
-    xstore(out + 2 * (r0 + t) + 1,
-           ((sPart[0][2 * t + 1] + sPart[1][2 * t + 1]) + sPart[2][2 * t + 1]) + sPart[3][2 * t + 1]);
+  if (!by_subst) {
+    // x_b = inverse * acc
+    double xr = 0.0, xi = 0.0;
+    for (int j = part; j < SB; j += 4) {
+      const double ir = sInv[2 * (r * SB + j)], ii = sInv[2 * (r * SB + j) + 1];
+      const double ar = sAcc[2 * j], ai = sAcc[2 * j + 1];
+      xr += ir * ar - ii * ai;
+      xi += ir * ai + ii * ar;
+    }
+    sPart[part][2 * r] = xr;
+    sPart[part][2 * r + 1] = xi;
+    __syncthreads();
+    if (t < rows) {
+      xstore(out + 2 * (r0 + t), ((sPart[0][2 * t] + sPart[1][2 * t]) + sPart[2][2 * t]) + sPart[3][2 * t]);
+      xstore(out + 2 * (r0 + t) + 1,
+             ((sPart[0][2 * t + 1] + sPart[1][2 * t + 1]) + sPart[2][2 * t + 1]) + sPart[3][2 * t + 1]);
+    }
+  } else if (t < SB) {
+    // substitution in the first wave, lane t holding row t: step q takes
+    // x_q from lane q and subtracts its column from the rows still open
+    double ar = sAcc[2 * t], ai = sAcc[2 * t + 1];
+    if (lower) {
+      for (int q = 0; q < SB - 1; ++q) {
+        const double yr = readlane_d(ar, q), yi = readlane_d(ai, q);
+        if (t > q) {
+          const double lr = sInv[2 * (t * SB + q)], li = sInv[2 * (t * SB + q) + 1];
+          ar -= lr * yr - li * yi;
+          ai -= lr * yi + li * yr;
+        }
+      }
+    } else {
+      for (int q = SB - 1; q >= 0; --q) {
+        if (t == q) {
+          const zc x = zdiv(zmk(ar, ai), zmk(sInv[2 * (q * SB + q)], sInv[2 * (q * SB + q) + 1]));
+          ar = x.re;
+          ai = x.im;
+        }
+        const double xr = readlane_d(ar, q), xi = readlane_d(ai, q);
+        if (t < q) {
+          const double ur = sInv[2 * (t * SB + q)], ui = sInv[2 * (t * SB + q) + 1];
+          ar -= ur * xr - ui * xi;
+          ai -= ur * xi + ui * xr;
+        }
+      }
+    }
+    if (t < rows) {
+      xstore(out + 2 * (r0 + t), ar);
+      xstore(out + 2 * (r0 + t) + 1, ai);
+    }
   }
   // the data stores have completed before the flag is raised
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
@@ -311,6 +401,7 @@ struct SolveWork {
   unsigned *flags_f, *flags_b;
   int* status;
   double* tmp;
+  int* subst;
   double* inv;
 };
 
@@ -327,6 +418,8 @@ SolveWork carve_solve(void* work, int n) {
   p += 256;
   w.tmp = reinterpret_cast<double*>(p);
   p += align256((size_t)n * 16);
+  w.subst = reinterpret_cast<int*>(p);
+  p += align256((size_t)2 * nblk * sizeof(int));
   w.inv = reinterpret_cast<double*>(p);
   return w;
 }
@@ -336,7 +429,8 @@ int prepare(const TF* LU, long long lda, int n, void* work, hipStream_t stream) 
   if (n <= 0) return 0;
   const int nblk = (n + SB - 1) / SB;
   const SolveWork w = carve_solve(work, n);
-  launch_k(K_DIAG_INVERSE, diag_inverse_kernel<TF>, dim3(nblk, 2), dim3(64), 0u, stream, LU, lda, n, w.inv);
+  launch_k(K_DIAG_INVERSE, diag_inverse_kernel<TF>, dim3(nblk, 2), dim3(64), 0u, stream, LU, lda, n, w.inv,
+           w.subst);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -349,9 +443,9 @@ int apply(const TF* LU, long long lda, int n, const int* piv, double* x, void* w
   const int* perm = piv + n;
   launch_k(K_SOLVE_RESET, reset_status_kernel, dim3(1), dim3(64), 0u, stream, w.status);
   launch_k(K_TRSV_LOWER, trsv_kernel<TF, true>, dim3(nblk), dim3(256), 0u, stream, LU, lda, n, perm,
-           (const double*)x, w.tmp, (const double*)w.inv, w.flags_f, epoch, w.status);
+           (const double*)x, w.tmp, (const double*)w.inv, (const int*)w.subst, w.flags_f, epoch, w.status);
   launch_k(K_TRSV_UPPER, trsv_kernel<TF, false>, dim3(nblk), dim3(256), 0u, stream, LU, lda, n, perm,
-           (const double*)w.tmp, x, (const double*)w.inv, w.flags_b, epoch, w.status);
+           (const double*)w.tmp, x, (const double*)w.inv, (const int*)w.subst, w.flags_b, epoch, w.status);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -360,7 +454,7 @@ int apply(const TF* LU, long long lda, int n, const int* piv, double* x, void* w
 size_t solve_workspace_bytes(int n) {
   const int nblk = (n + SB - 1) / SB;
   return align256((size_t)2 * nblk * sizeof(unsigned)) + 256 + align256((size_t)n * 16) +
-         (size_t)2 * nblk * SB * SB * 16;
+         align256((size_t)2 * nblk * sizeof(int)) + (size_t)2 * nblk * SB * SB * 16;
 }
 
 int* solve_status(void* work, int n) { return carve_solve(work, n).status; }

@@ -155,6 +155,39 @@ def test_zgetrf_zgetrs_random(n, mode, monkeypatch):
     assert np.linalg.norm(A @ xs - b) <= 1e-12 * np.linalg.norm(b) * n
 
 
+@pytest.mark.parametrize("n", [70, 300])
+def test_zgetrs_backward_stable_when_nearly_singular(n):
+    """A matrix with a cluster of nearly dependent columns (1e-13 apart) has
+    diagonal blocks of U with condition numbers near 1e13: the solve must
+    still leave the residual of substitution, ||A x - b|| <= n u ||A|| ||x||
+    (u = 2^-53), as the oracle's solve does on the same factors."""
+    torch = _torch()
+    rng = np.random.default_rng(7 + n)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    for k in range(n - 6, n):  # the last block: a rank-deficient cluster
+        A[:, k] = A[:, n - 7] + 1e-13 * (rng.standard_normal(n) + 1j * rng.standard_normal(n))
+    b = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    dA = torch.from_numpy(A.copy()).to("cuda")
+    piv = torch.zeros(2 * n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    lw = torch.empty(int(api.necgpu_zgetrf_workspace(n)), dtype=torch.uint8, device="cuda")
+    assert api.necgpu_zgetrf(dA.data_ptr(), n, n, piv.data_ptr(), info.data_ptr(), lw.data_ptr(), None) == 0
+    ws = torch.zeros(int(api.necgpu_zgetrs_workspace(n)), dtype=torch.uint8, device="cuda")
+    x = torch.from_numpy(b.copy()).to("cuda")
+    assert api.necgpu_zgetrs(dA.data_ptr(), n, n, piv.data_ptr(), x.data_ptr(), ws.data_ptr(), 1, None) == 0
+    torch.cuda.synchronize()
+    xs = x.cpu().numpy()
+    u = 2.0 ** -53
+    scale = np.linalg.norm(A, 2) * np.linalg.norm(xs)
+    assert np.linalg.norm(xs) > 1e8 * np.linalg.norm(b) / np.linalg.norm(A, 2)  # the system is nearly singular
+    assert np.linalg.norm(A @ xs - b) <= n * u * scale, np.linalg.norm(A @ xs - b) / scale
+    # the same factors solved by the oracle (substitution) for comparison
+    LU = dA.cpu().numpy()
+    xo = po.lu_solve(LU, piv.cpu().numpy()[:n].astype(np.int32), b)
+    print("backward error gpu", np.linalg.norm(A @ xs - b) / scale,
+          "oracle", np.linalg.norm(A @ xo - b) / (np.linalg.norm(A, 2) * np.linalg.norm(xo)))
+
+
 def test_zgetrf_reports_zero_pivot():
     torch = _torch()
     n = 70
