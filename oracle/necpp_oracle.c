@@ -899,6 +899,44 @@ int ora_tbf(const ora_geom* g, int i, int icap, int32_t* jco, double* ax, double
 }
 
 /* ------------------------------------------------------------------ */
+/* Decision tape (finite differences with frozen decisions)            */
+/* ------------------------------------------------------------------ */
+/* The fill takes discrete decisions that depend continuously on the
+ * geometry: the near/far branch of efld_compute (r >= rkh), every accept /
+ * refine choice of intx's adaptive Romberg steps, and gf's small-argument
+ * branch of the self term.  Recording them at a point p and replaying them
+ * at p +- h gives central differences of the function with those decisions
+ * held fixed: the derivative that an adjoint of the same code computes.
+ * One tape per thread; a taped fill must run on one thread (its decisions
+ * are recorded and replayed in call order). */
+static _Thread_local int tape_mode;           /* 0 off, 1 record, 2 replay */
+static _Thread_local unsigned char* tape_buf;
+static _Thread_local int64_t tape_len, tape_cap, tape_pos;
+static _Thread_local int tape_bad;            /* replay read past the tape */
+
+static int tape_decide(int actual) {
+  if (tape_mode == 1) {
+    if (tape_len == tape_cap) {
+      tape_cap = tape_cap ? 2 * tape_cap : 1 << 20;
+      tape_buf = (unsigned char*)xrealloc(tape_buf, (size_t)tape_cap);
+    }
+    tape_buf[tape_len++] = (unsigned char)(actual != 0);
+    return actual;
+  }
+  if (tape_mode == 2) {
+    if (tape_pos >= tape_len) { tape_bad = 1; return actual; }
+    return tape_buf[tape_pos++];
+  }
+  return actual;
+}
+
+void ora_tape_record(void) { tape_mode = 1; tape_len = 0; tape_pos = 0; tape_bad = 0; }
+void ora_tape_replay(void) { tape_mode = 2; tape_pos = 0; tape_bad = 0; }
+void ora_tape_off(void) { tape_mode = 0; }
+/* decisions recorded, decisions replayed so far, replay ran past the tape */
+void ora_tape_stats(int64_t* out) { out[0] = tape_len; out[1] = tape_pos; out[2] = tape_bad; }
+
+/* ------------------------------------------------------------------ */
 /* Fill kernels                                                        */
 /* ------------------------------------------------------------------ */
 typedef struct {
@@ -921,7 +959,7 @@ static void gf(fill_state* st, double zk, double* co, double* si) {
   double rk = sqrt(st->rkb2 + zdk * zdk);
   *si = sin(rk) / rk;
   if (st->ija != 0) { *co = cos(rk) / rk; return; }
-  if (rk >= .2) { *co = (cos(rk) - 1.0) / rk; return; }
+  if (tape_decide(rk >= .2)) { *co = (cos(rk) - 1.0) / rk; return; }
   double rks = rk * rk;
   *co = ((c0 * rks + c1) * rks - 0.5) * rk;
 }
@@ -979,7 +1017,7 @@ static void intx(fill_state* st, double el1, double el2, double b, int ij, doubl
     t10r = (4.0 * t01r - t00r) / 3.0;
     t10i = (4.0 * t01i - t00i) / 3.0;
     conv_test(t01r, t10r, &te1r, t01i, t10i, &te1i, 0.);
-    if ((te1i <= rx) && (te1r <= rx)) {
+    if (tape_decide((te1i <= rx) && (te1r <= rx))) {
       *sgr = *sgr + t10r;
       *sgi = *sgi + t10i;
       nt += 2;
@@ -1002,20 +1040,19 @@ static void intx(fill_state* st, double el1, double el2, double b, int ij, doubl
     t20r = (16.0 * t11r - t10r) / 15.0;
     t20i = (16.0 * t11i - t10i) / 15.0;
     conv_test(t11r, t20r, &te2r, t11i, t20i, &te2i, 0.);
-    if ((te2i > rx) || (te2r > rx)) {
+    const int bad5 = (te2i > rx) || (te2r > rx);
+    if (tape_decide(bad5 && ns < nma)) {
       nt = 0;
-      if (ns < nma) {
-        ns = ns * 2;
-        fns = ns;
-        dz = s / fns;
-        dzot = dz * 0.5;
-        g5r = g3r; g5i = g3i;
-        g3r = g2r; g3i = g2i;
-        flag = 0;
-        continue;
-      }
-      /* step size limited: accept the 5-point result */
+      ns = ns * 2;
+      fns = ns;
+      dz = s / fns;
+      dzot = dz * 0.5;
+      g5r = g3r; g5i = g3i;
+      g3r = g2r; g3i = g2i;
+      flag = 0;
+      continue;
     }
+    if (bad5) nt = 0; /* step size limited: accept the 5-point result */
     *sgr = *sgr + t20r;
     *sgi = *sgi + t20i;
     nt++;
@@ -1198,7 +1235,7 @@ static double efld_compute(fill_state* st, double xij, double yij, double ai, do
   if (rh <= 1.e-10) { rhox = 0.0; rhoy = 0.0; rhoz = 0.0; }
   else { rhox /= rh; rhoy /= rh; rhoz /= rh; }
   double r = sqrt(zp * zp + rh * rh);
-  if (r >= st->rkh) {
+  if (tape_decide(r >= st->rkh)) {
     st->cnt->far_pairs++;
     double eta = ora_impedance();
     double rmag = TWO_PI * r;

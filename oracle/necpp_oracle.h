@@ -83,6 +83,17 @@ double ora_speed_of_light(void);
 double ora_impedance(void);
 
 /* c_geometry::frequency_scale (src/c_geometry.cpp:3389-3418). */
+/* Decision tape: record the fill's geometry-dependent decisions (near/far
+ * branch per pair, every adaptive-quadrature accept/refine choice, the
+ * self-term integrand's small-argument branch) during one run, then replay
+ * them in later runs, for central differences with the decisions held
+ * fixed.  Per thread; taped runs must use one thread.  stats: decisions
+ * recorded, decisions replayed, replay ran past the tape (0/1). */
+void ora_tape_record(void);
+void ora_tape_replay(void);
+void ora_tape_off(void);
+void ora_tape_stats(int64_t* out);
+
 /* GM (c_geometry::move, src/c_geometry.cpp:830-900), angles in radians;
  * GS (c_geometry::scale, :1308-1318).  Both act on the segments defined so
  * far, before ora_geometry_complete.  ora_move returns 1 (message in

@@ -42,6 +42,7 @@ def lib():
         L.ora_impedance.restype = d
         L.ora_frequency_scale.argtypes = [p, d]
         L.ora_move.argtypes = [p, d, d, d, d, d, d, i, i, i]
+        L.ora_tape_stats.argtypes = [p]
         L.ora_scale.argtypes = [p, d]
         L.ora_set_ground.argtypes = [p, i, d, d]
         L.ora_set_loads.argtypes = [p, i, p, p, p, p, p, p, p]
@@ -69,6 +70,18 @@ def lib():
 
 def _ptr(a):
     return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def tape(mode):
+    """The fill's decision tape: "record", "replay" or "off" (single-thread runs)."""
+    getattr(lib(), {"record": "ora_tape_record", "replay": "ora_tape_replay", "off": "ora_tape_off"}[mode])()
+
+
+def tape_stats():
+    """(decisions recorded, decisions replayed, replay ran past the tape)."""
+    out = np.zeros(3, np.int64)
+    lib().ora_tape_stats(_ptr(out))
+    return int(out[0]), int(out[1]), bool(out[2])
 
 
 class FillCounts(ctypes.Structure):

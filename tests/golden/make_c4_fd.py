@@ -7,9 +7,19 @@ parameters, 15-element Yagi, 31 segments per element, N = 465, 144 MHz):
           gain (pg_tot) and input impedance;
   fd    = central differences with the step h = 1e-6 |p| (Euclidean norm of
           the row) for every component;
-  fd2   = the same with 2h (where fd and fd2 disagree, J is not smooth at
-          the step's scale: a segment pair crossing the r = rkh switch of the
-          field kernel, or a quadrature decision flipping).
+  fd2   = the same with 2h;
+  fd4   = (4 fd - fd2) / 3, the fourth-order central difference of the same
+          five points: the radii (~0.003 wavelengths) take a step of 5e-4 of
+          their size, and J's third derivative in a radius makes fd's own
+          h^2 truncation error reach 1e-6 of the gradient, which fd4 cancels
+          (a smaller step is worse: J carries ~4e-11 relative rounding noise).
+The fill's geometry-dependent decisions (the near/far switch r >= rkh of the
+field kernel per segment pair, every accept/refine choice of the adaptive
+quadrature, the self-term integrand's small-argument branch) are recorded at
+p and replayed at every p +- h: the differences are those of J with the
+decisions held fixed, the function whose derivative an adjoint of the same
+code computes (without it, a difference that straddles a decision flip is
+off by the jump / 2h).  Each replay must consume exactly the recorded tape.
 Run: python tests/golden/make_c4_fd.py [--designs N] [--procs P]
 """
 import argparse
@@ -36,12 +46,41 @@ def oracle_J(p):
     return -float(r["pattern"]["pg_tot"][0]) + WEIGHT * abs(complex(r["zin"][0]) - Z0) ** 2
 
 
-def task(args):
-    d, k, s, p = args  # k < 0: J itself; s: step multiple (+-1, +-2)
-    q = np.array(p, dtype=float)
-    if k >= 0:
-        q[k] += s * REL_STEP * float(np.linalg.norm(p))
-    return d, k, s, oracle_J(q), (q[k] - p[k]) if k >= 0 else 0.0
+def design_task(args):
+    """J(p) with the decisions recorded, then J at p +- h, p +- 2h for every
+    component with them replayed."""
+    d, p = args
+    p = np.array(p, dtype=float)
+    po.tape("record")
+    J0 = oracle_J(p)
+    n_rec = po.tape_stats()[0]
+    out = {}
+    h = REL_STEP * float(np.linalg.norm(p))
+    for k in range(len(p)):
+        for s in (1, -1, 2, -2):
+            q = p.copy()
+            q[k] += s * h
+            po.tape("replay")
+            v = oracle_J(q)
+            rec, used, bad = po.tape_stats()
+            if bad or used != rec or rec != n_rec:
+                raise RuntimeError(f"design {d} component {k}: the replay did not follow the recorded decisions")
+            out[(k, s)] = (v, q[k] - p[k])
+    po.tape("off")
+    # the point itself again with the tape replayed: the same J
+    po.tape("replay")
+    J1 = oracle_J(p)
+    po.tape("off")
+    if J1 != J0:
+        raise RuntimeError(f"design {d}: replay at p changed J ({J0} -> {J1})")
+    fd = np.zeros(len(p))
+    fd2 = np.zeros(len(p))
+    for k in range(len(p)):
+        (vp, hp), (vm, hm) = out[(k, 1)], out[(k, -1)]
+        fd[k] = (vp - vm) / (hp - hm)
+        (vp, hp), (vm, hm) = out[(k, 2)], out[(k, -2)]
+        fd2[k] = (vp - vm) / (hp - hm)
+    return d, J0, fd, fd2, n_rec
 
 
 def main():
@@ -52,34 +91,22 @@ def main():
     a = ap.parse_args()
     po.build()
     P = decks.c4_params(count=a.designs)
-    tasks = []
-    for d, p in enumerate(P):
-        tasks.append((d, -1, 0, p))
-        for k in range(P.shape[1]):
-            for s in (1, -1, 2, -2):
-                tasks.append((d, k, s, p))
     J = np.zeros(len(P))
-    val = {}
-    t0 = time.time()
-    with Pool(a.procs) as pool:
-        for i, (d, k, s, v, dp) in enumerate(pool.imap_unordered(task, tasks, chunksize=8)):
-            if k < 0:
-                J[d] = v
-            else:
-                val[(d, k, s)] = (v, dp)
-            if (i + 1) % 2000 == 0:
-                print(f"{i + 1}/{len(tasks)} evaluations, {time.time() - t0:.0f} s", flush=True)
     fd = np.zeros(P.shape)
     fd2 = np.zeros(P.shape)
-    for d in range(len(P)):
-        for k in range(P.shape[1]):
-            (vp, hp), (vm, hm) = val[(d, k, 1)], val[(d, k, -1)]
-            fd[d, k] = (vp - vm) / (hp - hm)
-            (vp, hp), (vm, hm) = val[(d, k, 2)], val[(d, k, -2)]
-            fd2[d, k] = (vp - vm) / (hp - hm)
-    np.savez_compressed(a.out, J=J, fd=fd, fd2=fd2, rel_step=REL_STEP, seed=decks.ADJOINT_SEED,
-                        params_sum=P.sum(axis=1))
-    print("wrote", a.out, "in", round(time.time() - t0), "s")
+    ndec = np.zeros(len(P), np.int64)
+    t0 = time.time()
+    with Pool(a.procs) as pool:
+        for i, (d, j0, f1, f2, nd) in enumerate(pool.imap_unordered(design_task, list(enumerate(P)))):
+            J[d], fd[d], fd2[d], ndec[d] = j0, f1, f2, nd
+            if (i + 1) % 16 == 0:
+                print(f"{i + 1}/{len(P)} designs, {time.time() - t0:.0f} s", flush=True)
+    fd4 = (4.0 * fd - fd2) / 3.0
+    np.savez_compressed(a.out, J=J, fd=fd, fd2=fd2, fd4=fd4, rel_step=REL_STEP, seed=decks.ADJOINT_SEED,
+                        params_sum=P.sum(axis=1), decisions=ndec, frozen=True)
+    cons = np.linalg.norm(fd - fd2, axis=1) / np.linalg.norm(fd, axis=1)
+    print("wrote", a.out, "in", round(time.time() - t0), "s; |fd(h) - fd(2h)| / |fd| max", cons.max(),
+          "median", np.median(cons))
 
 
 if __name__ == "__main__":
