@@ -1,0 +1,106 @@
+"""Comparisons of GPU pattern results with the oracle's, shared by the GPU
+tests (test infrastructure; no tests here)."""
+import numpy as np
+
+from necpp_amd import necpp as api
+
+DB_ARRAYS = ("gain", "pg_vert", "pg_horiz", "pg_tot", "pg_rhcp", "pg_lhcp")
+STATS = [f"nec_gain{s}_{k}" for s in ("", "_rhcp", "_lhcp") for k in ("max", "min", "mean", "sd")]
+
+
+def compare_pattern(ctx, o, npt, index=0, span_db=80.0, rel=1e-10):
+    """Every pattern array of result `index` against the oracle's run.
+
+    Bar, for a well-conditioned structure (rel = 1e-10): every gain in linear
+    scale within 1e-10 of the pattern's largest (normwise); in dB within 1e-8
+    where the direction is within span_db of the largest total gain.  A
+    deeper null is a difference of nearly equal contributions: there a field
+    error of 1e-14 of the peak field moves the dB value by 1e-4, so nulls are
+    held to the linear bar only, and the polarisation outputs (defined by the
+    field's direction, not its size) are compared only within span_db as
+    well.  For an ill-conditioned structure the caller passes the forward
+    error its condition number allows (rel > 1e-10); every bar scales with
+    it (the axial ratio, a square root of a difference, with its root)."""
+    op = o["pattern"]
+    s = rel / 1e-10
+    r = np.sqrt(s)
+    g = {k: api.gpu_pattern_array(ctx, index, k, npt) for k in api.PATTERN_ARRAYS}
+    # NaN (a gain with no input power) in the same places
+    for k in api.PATTERN_ARRAYS:
+        assert np.array_equal(np.isnan(g[k]), np.isnan(op[k])), k
+    finite = op["pg_tot"] > -999.0
+    assert np.array_equal(g["pg_tot"] > -999.0, finite)
+    top = np.max(op["pg_tot"][finite]) if finite.any() else 0.0
+    live = finite & (op["pg_tot"] > top - span_db)
+    lin = lambda a: np.where(a > -999.0, 10.0 ** (a / 10.0), 0.0)
+    for k in DB_ARRAYS:
+        f = op[k] > -999.0
+        if f.any():
+            peak = np.max(lin(op[k]))
+            # RHCP / LHCP come from the axial ratio (nec_radiation_pattern.cpp
+            # :417-426), whose minor axis is a difference of nearly equal terms:
+            # for nearly linear fields a rounding of 1e-16 of the total moves
+            # the ratio, and with it the circular split, by ~1e-8 of the total
+            circ = k in ("pg_rhcp", "pg_lhcp")
+            bar = (1e-7 if circ else 1e-10) * s * peak
+            assert np.max(np.abs(lin(g[k]) - lin(op[k])), initial=0.0) <= bar, k
+            # below the bar a value may sit at the dB floor (-999.99) in one
+            # and not the other (the minor-axis gain of a linear field is such
+            # a residue); above it both must be finite
+            sig = lin(op[k]) > bar
+            assert np.all(g[k][sig] > -999.0), k
+            near = f & live & (op[k] > 10.0 * np.log10(peak) - span_db)
+            assert np.all(g[k][near] > -999.0), k
+            assert np.max(np.abs(g[k][near] - op[k][near]), initial=0.0) < (1e-6 if circ else 1e-8) * s, k
+        else:
+            assert np.all(lin(g[k]) <= 1e-10 * s * max(1.0, np.max(lin(g["pg_tot"])))), k
+    for k in ("e_theta", "e_phi"):
+        assert np.max(np.abs(g[k] - op[k]), initial=0.0) <= 1e-10 * s * np.max(np.abs(op[k]), initial=0.0), k
+    assert np.max(np.abs(g["avg_scale"] - op["avg_scale"]), initial=0.0) < 1e-14
+    # axial ratio: the minor axis is a difference of nearly equal terms where
+    # the field is nearly linear (sqrt of 1e-13 relative), so 1e-6 absolute
+    # there and 1e-9 relative where the ellipse is open
+    ar_g, ar_o = g["axial_ratio"], op["axial_ratio"]
+    assert np.max(np.abs(ar_g - ar_o)[live], initial=0.0) < 1e-6 * r
+    open_ = live & (ar_o > 1e-3 * r)
+    if open_.any():
+        assert np.max(np.abs(ar_g - ar_o)[open_] / ar_o[open_]) < 1e-9 * s
+    # sense where it is not at the linear threshold (axial ratio 1e-5)
+    clear = ~finite | (live & ((ar_o > 1e-4 * r) | (ar_o < 1e-6 / r)))
+    assert np.array_equal(g["sense"][clear], op["sense"][clear].astype(float))
+    # tilt where the ellipse has a major axis (not circular), modulo 180
+    tilted = live & (ar_o < 0.9)
+    d = (g["tilt"][tilted] - op["tilt"][tilted] + 90.0) % 180.0 - 90.0
+    assert np.max(np.abs(d), initial=0.0) < 1e-7 * s
+    return g
+
+
+def check_stats(ctx, o, g, index=0):
+    """The twelve statistics (max, min, mean, sd of the total, RHCP and LHCP
+    gain; src/nec_radiation_pattern.h:252-333) against the oracle's.  They are
+    sums over dB values, so a null's dB noise enters them: the bar is what the
+    array differences can move them by (max and min: the differences at the
+    two arrays' extreme directions; mean: the weighted sum of differences;
+    sd: their weighted 2-norm plus the mean's change), plus 1e-12.  The GPU's
+    statistics must also be those of its own arrays."""
+    got = np.array([getattr(api, s)(ctx, index) for s in STATS])
+    want = np.array(o["stats"])
+    w = g["avg_scale"]
+    norm = (w.size * 2) / 3.1415926536
+    for a, name in enumerate(("pg_tot", "pg_rhcp", "pg_lhcp")):
+        ag, ao = g[name], o["pattern"][name]
+        d = ag - ao
+        mean_g = np.sum(ag * w) / norm
+        own = [np.max(ag), np.min(ag), mean_g, np.sqrt(np.sum((ag - mean_g) ** 2 * w) / norm)]
+        assert np.allclose(got[4 * a:4 * a + 4], own, rtol=1e-12, atol=1e-12, equal_nan=True), name
+        if np.isnan(want[4 * a:4 * a + 4]).any():  # NaN gains (no input power): NaN in both
+            assert np.array_equal(np.isnan(got[4 * a:4 * a + 4]), np.isnan(want[4 * a:4 * a + 4])), name
+            continue
+        dmean = np.sum(np.abs(d) * w) / norm
+        # max(ag) - max(ao) lies between d[argmax ao] and d[argmax ag]; so for min
+        dmax = max(abs(d[np.argmax(ag)]), abs(d[np.argmax(ao)]))
+        dmin = max(abs(d[np.argmin(ag)]), abs(d[np.argmin(ao)]))
+        bars = [dmax, dmin, dmean, np.sqrt(np.sum(d * d * w) / norm) + dmean * np.sqrt(np.sum(w) / norm)]
+        for q in range(4):
+            assert abs(got[4 * a + q] - want[4 * a + q]) <= bars[q] + 1e-12 * (1 + abs(want[4 * a + q])), (name, q)
+    return got

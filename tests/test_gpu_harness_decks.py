@@ -1,0 +1,138 @@
+"""The reference's testharness decks (testharness/data/*.nec, kept as
+fixtures under tests/golden/decks) that use only cards of the GPU path,
+through nec_gpu_deck on the GPU against the same decks run on the oracle by
+an independent deck reader (tests/deck_oracle.py): every input impedance
+result and source, every pattern (all twelve arrays and the statistics) and
+every near field, to the bars of the single-structure parity tests.
+
+Decks of the testharness left out, and why: GN 2 (Sommerfeld ground:
+36dip, Collinear_1L, buoy, salt_ground, bruce_sommerfeld, sommerfeld2/3,
+sommerfield1, gn2), EX types 1 and 5 (plane wave / current-slope source:
+example2, passive_test1, ga_pjw_0, plane_wave_excitation), RP mode 1
+(test299), patches (SP/SC/SM), GX/GR/GA/GH, TL/NT, NH, CP, MP, PL (not on
+the path); hang.nec (segments as long as the wire radius and two
+overlapping wires: the adaptive quadrature subdivides to its limit, in the
+reference and in the oracle alike); geometry-only decks (discone, plane,
+plet_helix).
+
+Some of these decks are ill-conditioned on purpose (wire radii as large as
+the segments or larger: excessive_gain, patch_999, patch_999_2,
+plet_helixumts, ga_pjw_1).  Two correct LU solves then differ by up to about
+u * cond(Z) (u = 2^-53), so the forward bars are max(1e-10, 32 u cond(Z)),
+cond(Z) taken from the oracle's matrix.  Where that exceeds 1e-6 (ga_pjw_1:
+cond 1e18, singular to working precision) the forward results are rounding
+noise in the reference as much as here, and only the checks below hold.
+On every deck two checks independent of cond(Z) come first: the GPU's
+impedance matrix against the oracle's entry by entry (the bar of the fill
+parity test, 1e-12 of the entry plus 1e-15 of the largest, widened by twice
+the entry's own rounding noise, see rounding_envelope), and the GPU's
+solution held to the system by its backward error.  With x the basis amplitudes of the segment currents I (I = T x, T
+the map of src/nec_context.cpp's cabc, built column by column from the
+oracle) and Z_gpu, Z_oracle the two matrices, both solves leave residuals of
+their LU's backward error against the same right-hand side, so
+||Z_gpu x_gpu - Z_oracle x_oracle|| <= 1e-13 ||Z|| max(||x_gpu||, ||x_oracle||)."""
+import os
+
+import numpy as np
+import pytest
+import torch  # noqa: F401  (imported at collection: the first import on a fresh box is slow)
+
+import deck_oracle as dor
+from gpu_compare import check_stats, compare_pattern
+from necpp_amd import necpp as api
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DECKS = ["GA487", "example1", "excessive_gain", "ga_pjw_1", "gx_three_planes_control", "intersection",
+         "near_field_electric", "patch_999", "patch_999_2", "plet_helixumts"]
+
+
+def amplitude_map(geo):
+    """T: basis amplitudes -> segment currents (the oracle's
+    current_coefficients applied to unit vectors)."""
+    n = geo.n
+    T = np.zeros((n, n), np.complex128)
+    for k in range(n):
+        e = np.zeros(n, np.complex128)
+        e[k] = 1.0
+        T[:, k] = geo.current_coefficients(e)[0]
+    return T
+
+
+def rounding_envelope(o, Z):
+    """How far each entry of the oracle's matrix moves when the frequency
+    moves by 1 to 3 units in the last place: the entries' own rounding
+    noise.  Most move by ~1e-13 of themselves, but an entry whose adaptive
+    quadrature sits at an accept / refine threshold (src/nec_context.cpp
+    intx) or which is a difference of nearly equal near-field terms moves by
+    up to ~1e-9; a GPU fill that rounds differently may land anywhere in
+    that range."""
+    geo, f = o["geometry"], o["freqs"][-1]
+    env = np.zeros(Z.shape)
+    for k in (1, 2, 3):
+        geo.scale(f * (1.0 + k * 2.0 ** -52))
+        env = np.maximum(env, np.abs(geo.fill(nthreads=16)[0] - Z))
+    geo.scale(f)
+    return env
+
+
+def deck_text(name):
+    with open(os.path.join(HERE, "golden", "decks", name + ".nec")) as f:
+        return f.read()
+
+
+@pytest.mark.parametrize("name", DECKS)
+def test_testharness_deck_matches_oracle(name):
+    text = deck_text(name)
+    o = dor.run(text, nthreads=16)
+    Z, _ = o["geometry"].fill(nthreads=16)  # the last solve's structure, frequency and ground
+    rel = max(1e-10, 32 * 2.0 ** -53 * np.linalg.cond(Z))
+    ctx = api.nec_create()
+    try:
+        print("deck", name, "forward bar", rel, flush=True)
+        assert api.nec_gpu_deck(ctx, text.encode()) == 0, api.nec_error_message()
+        assert api.nec_impedance_real(ctx, len(o["inputs"])) == -999.0  # no extra results
+        # that solve's matrix against the oracle's (the results are kept on
+        # the host: a new fill leaves them), and the backward error
+        T = amplitude_map(o["geometry"])
+        ig = api.gpu_currents(ctx, len(o["currents"]) - 1)
+        assert np.all(np.isfinite(ig))
+        xg, xo = np.linalg.solve(T, ig), np.linalg.solve(T, o["currents"][-1])
+        Zg = api.gpu_fill_matrix(ctx, o["freqs"][-1])
+        err = np.abs(Zg - Z)
+        bound = 1e-12 * np.abs(Z) + 1e-15 * np.max(np.abs(Z)) + 2.0 * rounding_envelope(o, Z)
+        ok = err <= bound
+        assert ok.all(), (name, int(np.count_nonzero(~ok)), float(np.max(err / bound)))
+        scale = np.linalg.norm(Z, 2) * max(np.linalg.norm(xg), np.linalg.norm(xo), 1e-300)
+        r = np.linalg.norm(Zg @ xg - Z @ xo) / scale
+        print("backward error", r, flush=True)
+        assert r <= 1e-13, (name, r)
+        if rel > 1e-6:
+            return
+        # input impedances: every solve and frequency, every source
+        for i, zs in enumerate(o["inputs"]):
+            for f, zo in enumerate(zs):
+                z = complex(api.nec_impedance_real_indexed(ctx, i, f), api.nec_impedance_imag_indexed(ctx, i, f))
+                assert abs(z - zo) <= rel * abs(zo), (name, i, f, z, zo)
+        # patterns
+        for i, p in enumerate(o["patterns"]):
+            npt = p["pattern"]["gain"].size
+            g = compare_pattern(ctx, p, npt, index=i, rel=rel)
+            check_stats(ctx, p, g, index=i)
+        # near fields: the points (in nfpat's order, generated independently)
+        # and the field at them
+        for i, e in enumerate(o["near"]):
+            cnt = api.nec_gpu_near_field(ctx, i, None, 0)
+            out = np.zeros(9 * cnt)
+            assert api.nec_gpu_near_field(ctx, i, out.ctypes.data, cnt) == cnt
+            out = out.reshape(cnt, 9)
+            pts = out[:, 0:3]
+            want = dor.near_points(e["card"])
+            assert pts.shape == want.shape and np.allclose(pts, want, rtol=1e-12, atol=1e-15)
+            E = out[:, 3::2] + 1j * out[:, 4::2]
+            Eo = dor.near_field(o, e, pts)
+            assert np.max(np.abs(E - Eo)) <= rel * np.max(np.abs(Eo)), name
+        assert api.nec_gpu_near_field(ctx, len(o["near"]), None, 0) == -1
+    finally:
+        api.nec_delete(ctx)

@@ -16,15 +16,12 @@ import numpy as np
 import pytest
 import torch  # noqa: F401  (imported at collection: the first import on a fresh box is slow)
 
+from gpu_compare import STATS, check_stats, compare_pattern  # noqa: F401
 from necpp_amd import decks
 from necpp_amd import necpp as api
 from oracle import pyoracle as po
 
 pytestmark = pytest.mark.gpu
-
-DB_ARRAYS = ("gain", "pg_vert", "pg_horiz", "pg_tot", "pg_rhcp", "pg_lhcp")
-STATS = [f"nec_gain{s}_{k}" for s in ("", "_rhcp", "_lhcp") for k in ("max", "min", "mean", "sd")]
-
 
 def gpu_turnstile(phase, rp_card, precision=0):
     deck = decks.turnstile(phase)
@@ -38,82 +35,6 @@ def gpu_turnstile(phase, rp_card, precision=0):
         assert api.nec_ex_card(ctx, 0, tag, seg, 0, v.real, v.imag, 0, 0, 0, 0) == 0
     assert api.nec_rp_card(ctx, *rp_card) == 0, api.nec_error_message()
     return ctx, deck
-
-
-def compare_pattern(ctx, o, npt, index=0, span_db=80.0):
-    """Every pattern array of result `index` against the oracle's run.
-
-    Bar: every gain in linear scale within 1e-10 of the pattern's largest
-    (normwise); in dB within 1e-8 where the direction is within span_db of
-    the largest total gain.  A deeper null is a difference of nearly equal
-    contributions: there a field error of 1e-14 of the peak field moves the
-    dB value by 1e-4, so nulls are held to the linear bar only, and the
-    polarisation outputs (defined by the field's direction, not its size)
-    are compared only within span_db as well."""
-    op = o["pattern"]
-    g = {k: api.gpu_pattern_array(ctx, index, k, npt) for k in api.PATTERN_ARRAYS}
-    finite = op["pg_tot"] > -999.0
-    assert np.array_equal(g["pg_tot"] > -999.0, finite)
-    top = np.max(op["pg_tot"][finite]) if finite.any() else 0.0
-    live = finite & (op["pg_tot"] > top - span_db)
-    lin = lambda a: np.where(a > -999.0, 10.0 ** (a / 10.0), 0.0)
-    for k in DB_ARRAYS:
-        f = op[k] > -999.0
-        assert np.array_equal(g[k] > -999.0, f), k
-        if f.any():
-            peak = np.max(lin(op[k]))
-            # RHCP / LHCP come from the axial ratio (nec_radiation_pattern.cpp
-            # :417-426), whose minor axis is a difference of nearly equal terms:
-            # for nearly linear fields a rounding of 1e-16 of the total moves
-            # the ratio, and with it the circular split, by ~1e-8 of the total
-            circ = k in ("pg_rhcp", "pg_lhcp")
-            assert np.max(np.abs(lin(g[k]) - lin(op[k]))) <= (1e-7 if circ else 1e-10) * peak, k
-            near = f & live & (op[k] > 10.0 * np.log10(peak) - span_db)
-            assert np.max(np.abs(g[k][near] - op[k][near])) < (1e-6 if circ else 1e-8), k
-    for k in ("e_theta", "e_phi"):
-        assert np.max(np.abs(g[k] - op[k])) <= 1e-10 * np.max(np.abs(op[k])), k
-    assert np.max(np.abs(g["avg_scale"] - op["avg_scale"])) < 1e-14
-    # axial ratio: the minor axis is a difference of nearly equal terms where
-    # the field is nearly linear (sqrt of 1e-13 relative), so 1e-6 absolute
-    # there and 1e-9 relative where the ellipse is open
-    ar_g, ar_o = g["axial_ratio"], op["axial_ratio"]
-    assert np.max(np.abs(ar_g - ar_o)[live]) < 1e-6
-    open_ = live & (ar_o > 1e-3)
-    if open_.any():
-        assert np.max(np.abs(ar_g - ar_o)[open_] / ar_o[open_]) < 1e-9
-    # sense where it is not at the linear threshold (axial ratio 1e-5)
-    clear = ~finite | (live & ((ar_o > 1e-4) | (ar_o < 1e-6)))
-    assert np.array_equal(g["sense"][clear], op["sense"][clear].astype(float))
-    # tilt where the ellipse has a major axis (not circular), modulo 180
-    tilted = live & (ar_o < 0.9)
-    d = (g["tilt"][tilted] - op["tilt"][tilted] + 90.0) % 180.0 - 90.0
-    assert np.max(np.abs(d)) < 1e-7
-    return g
-
-
-def check_stats(ctx, o, g, index=0):
-    """The twelve statistics (max, min, mean, sd of the total, RHCP and LHCP
-    gain; src/nec_radiation_pattern.h:252-333) against the oracle's.  They are
-    sums over dB values, so a null's dB noise enters them: the bar is what the
-    array differences can move them by (min: the largest difference; mean:
-    the weighted sum of differences; sd: their weighted 2-norm plus the
-    mean's change), plus 1e-12.  The GPU's statistics must also be those of
-    its own arrays."""
-    got = np.array([getattr(api, s)(ctx, index) for s in STATS])
-    want = np.array(o["stats"])
-    w = g["avg_scale"]
-    norm = (w.size * 2) / 3.1415926536
-    for a, name in enumerate(("pg_tot", "pg_rhcp", "pg_lhcp")):
-        ag, ao = g[name], o["pattern"][name]
-        d = ag - ao
-        mean_g = np.sum(ag * w) / norm
-        own = [np.max(ag), np.min(ag), mean_g, np.sqrt(np.sum((ag - mean_g) ** 2 * w) / norm)]
-        assert np.allclose(got[4 * a:4 * a + 4], own, rtol=1e-12, atol=1e-12), name
-        dmean = np.sum(np.abs(d) * w) / norm
-        bars = [1e-9, np.max(np.abs(d)), dmean, np.sqrt(np.sum(d * d * w) / norm) + dmean * np.sqrt(np.sum(w) / norm)]
-        for q in range(4):
-            assert abs(got[4 * a + q] - want[4 * a + q]) <= bars[q] + 1e-12 * (1 + abs(want[4 * a + q])), (name, q)
-    return got
 
 
 @pytest.mark.parametrize("fmt", [0, 1])
