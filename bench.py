@@ -139,8 +139,10 @@ def copy_probe(mib):
 
 
 def solve_times(deck, precision, steps):
-    """(ms per solve, Z_in, stage ms, refinement) of the on-device pipeline."""
-    from necpp_amd import decks
+    """The on-device pipeline of the deck (fill, LU, solve, Z_in and, with an
+    RP card, the pattern), timed step by step after one warm-up solve:
+    (median ms per solve, Z_in, median stage ms, refinement, results) where
+    results holds the currents and the total-gain pattern of the solve."""
     from necpp_amd import necpp as api
     import numpy as np
     ctx = api.nec_create()
@@ -154,35 +156,71 @@ def solve_times(deck, precision, steps):
         api.nec_fr_card(ctx, 0, 1, deck["freq_mhz"], 0.0)
         s, v = deck["source"], deck["voltage"]
         api.nec_ex_card(ctx, 0, s[0], s[1], 0, v.real, v.imag, 0, 0, 0, 0)
-        if api.nec_xq_card(ctx, 0) != 0:
+        rc = api.nec_rp_card(ctx, *deck["rp"]) if "rp" in deck else api.nec_xq_card(ctx, 0)
+        if rc != 0:
             raise RuntimeError(api.nec_error_message())
-        out = np.zeros(8)
-        if api.nec_gpu_benchmark_solve(ctx, 0, steps, out.ctypes.data) != 0:
-            raise RuntimeError(api.nec_error_message())
+        res = {"current": api.gpu_currents(ctx, 0)}
+        if "rp" in deck:
+            res["pg_tot"] = api.gpu_pattern_array(ctx, 0, "pg_tot", deck["rp"][1] * deck["rp"][2])
         ref = api.gpu_refinement(ctx) if precision == 1 else None
-        return float(out[0]), complex(out[5], out[6]), [float(x) for x in out[1:5]], ref
+        out = np.zeros(8)
+        per, stages = [], []
+        for k in range(steps):
+            if api.nec_gpu_benchmark_solve(ctx, 1 if k == 0 else 0, 1, out.ctypes.data) != 0:
+                raise RuntimeError(api.nec_error_message())
+            per.append(float(out[0]))
+            stages.append([float(x) for x in out[1:5]])
+        med = float(np.median(per))
+        st = [float(x) for x in np.median(np.array(stages), axis=0)]
+        res["step_ms"] = per
+        return med, complex(out[5], out[6]), st, ref, res
     finally:
         api.nec_delete(ctx)
 
 
-def mixed_leg(nn, steps=1):
-    """FP64 vs mixed precision on the nn x nn grid, Z_in checked against the oracle fixture."""
+def pattern_errors(pg, ref, span_db=80.0):
+    """Total power gain against the oracle's: largest |difference| in dB within
+    span_db of the maximum, and the largest difference in linear gain
+    relative to the maximum (nulls deeper than span_db are held to that)."""
+    import numpy as np
+    lin = lambda a: np.where(a > -999.0, 10.0 ** (a / 10.0), 0.0)
+    live = ref > np.max(ref) - span_db
+    return {"max_abs_db_within_%d_db" % int(span_db): float(np.max(np.abs(pg[live] - ref[live]))),
+            "max_linear_rel_to_peak": float(np.max(np.abs(lin(pg) - lin(ref))) / np.max(lin(ref)))}
+
+
+def mixed_leg(nn, steps=5):
+    """FP64 vs mixed precision on the nn x nn grid with the 91 x 361 pattern,
+    each timed as the median of `steps` solves; Z_in, currents and the
+    pattern of both checked against the oracle fixture."""
     from necpp_amd import decks
-    deck = decks.grid(nn, rp=None)
+    import numpy as np
+    deck = decks.grid(nn)
     n = len(deck["wires"])
-    ms64, z64, st64, _ = solve_times(deck, 0, steps)
-    ms32, z32, st32, (isteps, ires) = solve_times(deck, 1, steps)
+    ms64, z64, st64, _, r64 = solve_times(deck, 0, steps)
+    ms32, z32, st32, (isteps, ires), r32 = solve_times(deck, 1, steps)
     fixture = os.path.join(ROOT, "tests", "golden", "c5_oracle.json")
     zref = None
+    errors = None
     if n == 16020 and os.path.exists(fixture):
         zref = complex(*json.load(open(fixture))["zin"])
+        golden = os.path.join(ROOT, "tests", "golden")
+        cur = np.load(os.path.join(golden, "c5_oracle_current.npy"))
+        pat = np.load(os.path.join(golden, "c5_oracle_pattern.npy"))
+        errors = {}
+        for tag, z, r in (("fp64", z64, r64), ("mixed", z32, r32)):
+            errors[tag] = {"zin_rel": abs(z - zref) / abs(zref),
+                           "currents_rel_normwise": float(np.linalg.norm(r["current"] - cur) / np.linalg.norm(cur)),
+                           "pattern_total_gain": pattern_errors(r["pg_tot"], pat)}
     # the LU stage against the peaks: complex LU flops 8 n^3 / 3, FP64 factor
     # on the FP64 peak, the mixed path's FP32 factor on the FP32 peak
     lu_flops = 8.0 * float(n) ** 3 / 3.0
     lu64 = lu_flops / st64[1] / 1e9
     lu32 = lu_flops / st32[1] / 1e9
-    return {"workload": f"wire grid {nn}x{nn} nodes, N={n}, fill+LU+solve+Z_in (no pattern)",
+    return {"workload": f"wire grid {nn}x{nn} nodes, N={n}, fill+LU+solve+Z_in+91x361 pattern",
+            "steps": steps, "statistic": "median",
             "fp64_ms_per_solve": ms64, "mixed_ms_per_solve": ms32, "speedup": ms64 / ms32,
+            "fp64_step_ms": r64["step_ms"], "mixed_step_ms": r32["step_ms"],
             "fp64_lu_stage": {"achieved_tflops": lu64, "frac_fp64_peak": lu64 / PEAK_FP64_TFLOPS, "flops": lu_flops},
             "mixed_lu_stage": {"achieved_tflops": lu32, "frac_fp32_peak": lu32 / PEAK_FP32_TFLOPS, "flops": lu_flops},
             "fp64_stages_ms": dict(zip(("fill", "factor", "solve", "pattern"), st64)),
@@ -191,7 +229,8 @@ def mixed_leg(nn, steps=1):
             "zin_fp64": [z64.real, z64.imag], "zin_mixed": [z32.real, z32.imag],
             "zin_mixed_vs_fp64_rel": abs(z32 - z64) / abs(z64),
             "zin_mixed_vs_oracle_rel": None if zref is None else abs(z32 - zref) / abs(zref),
-            "zin_fp64_vs_oracle_rel": None if zref is None else abs(z64 - zref) / abs(zref)}
+            "zin_fp64_vs_oracle_rel": None if zref is None else abs(z64 - zref) / abs(zref),
+            "errors_vs_oracle": errors}
 
 
 def batch_leg(repeats=3):
@@ -212,7 +251,10 @@ def batch_leg(repeats=3):
             "stages_ms": {"fill": st["fill_ms"], "lu": st["lu_ms"], "solve_pattern": st["solve_pattern_ms"]},
             "lu_tflops": lu_flops / (st["lu_ms"] * 1e-3) / 1e12,
             "panel_width": int(st["panel_width"]), "graph_replays_equal_eager": st["graph_matches_eager"] == 1,
-            "host_prep_ms": st["host_ms"], "call_wall_s": wall}
+            "host_prep_ms": st["host_ms"], "call_wall_s": wall,
+            "end_to_end": {"solves_per_s": st["solves"] / wall,
+                           "note": "one nec_gpu_batch_solve call, host preparation, eager pass, graph capture "
+                                   "and %d timed replays included" % repeats}}
 
 
 def adjoint_leg(count=256, single_sample=16):
@@ -245,28 +287,39 @@ def adjoint_leg(count=256, single_sample=16):
     if os.path.exists(fx):
         z = np.load(fx)
         m = min(count, len(z["J"]))
-        nf = np.linalg.norm(z["fd"][:m], axis=1)
-        err = np.linalg.norm(G[:m] - z["fd"][:m], axis=1) / nf
-        cons = np.linalg.norm(z["fd"][:m] - z["fd2"][:m], axis=1) / nf
+        nf = np.linalg.norm(z["fd4"][:m], axis=1)
+        err = np.linalg.norm(G[:m] - z["fd4"][:m], axis=1) / nf
+        err_h = np.linalg.norm(G[:m] - z["fd"][:m], axis=1) / np.linalg.norm(z["fd"][:m], axis=1)
         out["vs_oracle_central_differences"] = {
-            "step": "1e-6 |p|", "designs": m, "within_1e-6": int((err <= 1e-6).sum()),
+            "step": "1e-6 |p|, fourth-order (4 fd(h) - fd(2h)) / 3, the fill's decisions held fixed",
+            "designs": m, "within_1e-6": int((err <= 1e-6).sum()),
             "median_rel_err": float(np.median(err)), "max_rel_err": float(err.max()),
-            "max_err_over_fd_self_consistency": float(np.max(err / np.maximum(cons, 1e-300))),
+            "second_order_fd_h": {"within_1e-6": int((err_h <= 1e-6).sum()), "max_rel_err": float(err_h.max())},
             "J_max_rel_diff": float(np.max(np.abs(J[:m] - z["J"][:m]) / np.abs(z["J"][:m])))}
     return out
 
 
-def cpu_baseline(deck):
-    """Time the oracle (CPU restatement) on one full solve of the same deck."""
+def cpu_baseline(deck, single_thread=True):
+    """Time the oracle (CPU restatement) on one full solve of the same deck,
+    on the host's cores (OpenMP) and on one thread (the reference as shipped
+    is single-threaded)."""
     from oracle import pyoracle as po
     threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or min(16, os.cpu_count() or 1)
     t = time.perf_counter()
     r = po.run_deck(deck, nthreads=threads)
     wall = time.perf_counter() - t
-    return {"value": 1.0 / wall, "unit": "solves/s", "cores": threads, "kind": "port",
-            "sample": f"1 full solve of the same N={len(deck['wires'])} grid (fill+LU+solve+91x361 pattern) "
-                      f"on {threads} OpenMP threads; stage s: " +
-                      ", ".join(f"{k}={v:.3f}" for k, v in r["timing"].items())}, r
+    out = {"value": 1.0 / wall, "unit": "solves/s", "cores": threads, "kind": "port",
+           "sample": f"1 full solve of the same N={len(deck['wires'])} grid (fill+LU+solve+91x361 pattern) "
+                     f"on {threads} OpenMP threads; stage s: " +
+                     ", ".join(f"{k}={v:.3f}" for k, v in r["timing"].items())}
+    if single_thread:
+        t = time.perf_counter()
+        r1 = po.run_deck(deck, nthreads=1)
+        wall1 = time.perf_counter() - t
+        out["single_thread"] = {"value": 1.0 / wall1, "unit": "solves/s", "cores": 1,
+                                "sample": "1 full solve of the same grid on one thread; stage s: " +
+                                          ", ".join(f"{k}={v:.3f}" for k, v in r1["timing"].items())}
+    return out, r
 
 
 def main():

@@ -12,10 +12,12 @@
 #include "batch.h"
 
 #include <algorithm>
+#include <map>
 #include <atomic>
 #include <chrono>
 #include <mutex>
 #include <thread>
+#include <tuple>
 #include <cmath>
 #include <cstring>
 
@@ -24,6 +26,12 @@
 #include "nec_math.h"
 
 namespace necgpu {
+
+namespace {
+// batched-LU panel widths chosen by timing, per (largest order, solves, threads)
+std::mutex g_tune_mu;
+std::map<std::tuple<int, long long, int>, int> g_tuned_width;
+}  // namespace
 
 namespace {
 
@@ -295,6 +303,13 @@ void run_batch(const BatchInput& in, BatchOutput* out) {
   hip_check(hipEventRecord(e1.b, st), "event");
   int bestP = in.panel_width;
   if (bt == 512 && bestP == 32) bestP = 16;
+  // a width found for the same largest order and batch size is reused
+  const std::tuple<int, long long, int> tune_key(dv.maxn, (long long)S, bt);
+  if (bestP != 8 && bestP != 16 && bestP != 32) {
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    const auto it = g_tuned_width.find(tune_key);
+    if (it != g_tuned_width.end()) bestP = it->second;
+  }
   if (bestP != 8 && bestP != 16 && bestP != 32) {
     void* bak = d_Zbak.ensure((size_t)zents * 16);
     hip_check(hipMemcpyAsync(bak, Z, (size_t)zents * 16, hipMemcpyDeviceToDevice, st), "copy");
@@ -315,6 +330,8 @@ void run_batch(const BatchInput& in, BatchOutput* out) {
     }
     hip_check(hipMemcpyAsync(Z, bak, (size_t)zents * 16, hipMemcpyDeviceToDevice, st), "copy");
     d_Zbak.release();
+    std::lock_guard<std::mutex> lock(g_tune_mu);
+    g_tuned_width[tune_key] = bestP;
   }
   out->panel_width = bestP;
   // one pass with per-stage events
