@@ -38,6 +38,14 @@ long nec_gpu_set_precision(nec_context* ctx, int mode);
 long nec_gpu_set_refine_tolerance(nec_context* ctx, double tol);
 long nec_gpu_refinement(nec_context* ctx, int* steps, double* rel_residual);
 
+/* How often this context's factorisations were repeated: lu = after a panel
+ * exchange timed out (the cooperative panel's workgroups could not all be
+ * resident beside the look-ahead update, e.g. on a device shared with other
+ * work; the repeat keeps every launch on one stream), pivot = mixed
+ * precision's block-local pivots replaced by partial pivoting.  Either
+ * pointer may be NULL.  No counterpart in the reference (new entry point). */
+long nec_gpu_factor_fallbacks(nec_context* ctx, long* lu, long* pivot);
+
 /* Adjoint gradient of the last solve (XQ or RP card; one voltage source,
  * FP64 path, thin-wire kernel) for the design objective
  *   J = -G(theta, phi) + weight * |Z_in - z0|^2,

@@ -360,6 +360,13 @@ long nec_gpu_refinement(nec_context* ctx, int* steps, double* rel_residual) {
   });
 }
 
+long nec_gpu_factor_fallbacks(nec_context* ctx, long* lu, long* pivot) {
+  return guarded([&] {
+    if (lu) *lu = ctx->device().lu_fallbacks();
+    if (pivot) *pivot = ctx->device().refine_fallbacks();
+  });
+}
+
 long nec_gpu_set_precision(nec_context* ctx, int mode) {
   return guarded([&] {
     if (mode != 0 && mode != 1) throw NecError("precision mode must be 0 (FP64) or 1 (mixed)");

@@ -273,6 +273,7 @@ std::vector<zc> DeviceSolver::near_field_e(const std::vector<double>& pts, const
 
 void DeviceSolver::enqueue_fill(double rkh) {
   const int n = n_;
+  last_rkh_ = rkh;
   Z_.ensure((size_t)ld_ * n * 16);
   const int rows = static_cast<int>(std::min<size_t>(n, std::max<size_t>(1, kFieldBudget / ((size_t)n * 48))));
   E_.ensure((size_t)rows * n * 48);
@@ -299,7 +300,8 @@ void DeviceSolver::enqueue_factor() {
     enqueue_factor_f32((v && v[0] == '0') ? 0 : 1);
     return;
   }
-  const int rc = launch_lu_fp64(Z_.as<double>(), ld_, n, piv_.as<int>(), info_.as<int>(), lu_work_.as<void>(), stream_);
+  const int rc = launch_lu_fp64(Z_.as<double>(), ld_, n, piv_.as<int>(), info_.as<int>(), lu_work_.as<void>(), stream_,
+                                lookahead_);
   if (rc == 2) throw NecError("LU: matrix order " + std::to_string(n) + " exceeds the panel exchange area");
   if (rc) hip_check(hipGetLastError(), "lu launch");
   ensure_solve_work();
@@ -314,7 +316,7 @@ void DeviceSolver::enqueue_factor_f32(int pivot_mode) {
   float* z32 = static_cast<float*>(Z32_.ensure((size_t)n * n * 8));
   if (launch_to_f32(Z_.as<double>(), ld_, z32, n, n, stream_)) hip_check(hipGetLastError(), "copy launch");
   const int rc =
-      launch_lu_fp32(z32, n, n, piv_.as<int>(), info_.as<int>(), lu_work_.as<void>(), stream_, pivot_mode);
+      launch_lu_fp32(z32, n, n, piv_.as<int>(), info_.as<int>(), lu_work_.as<void>(), stream_, pivot_mode, lookahead_);
   if (rc == 2) throw NecError("LU: matrix order " + std::to_string(n) + " exceeds the panel exchange area");
   if (rc) hip_check(hipGetLastError(), "lu launch");
   ensure_solve_work();
@@ -444,15 +446,22 @@ void DeviceSolver::fill(double rkh) {
   factored_ = false;
 }
 
-int DeviceSolver::factor_outcome() {
-  // waits for the factorisation on the stream; a panel exchange that timed
-  // out is an error, the zero-pivot word (1 + first zero column) is returned
+int DeviceSolver::factor_wait(bool* timed_out) {
+  // waits for the factorisation on the stream; returns the zero-pivot word
+  // (1 + first zero column)
   int info = 0, status = 0;
   hip_check(hipMemcpyAsync(&info, info_.as<int>(), sizeof(int), hipMemcpyDeviceToHost, stream_), "info");
   hip_check(hipMemcpyAsync(&status, lu_status(lu_work_.as<void>(), n_), sizeof(int), hipMemcpyDeviceToHost, stream_),
             "status");
   hip_check(hipStreamSynchronize(stream_), "factor");
-  if (status) throw NecError("LU: a panel exchange timed out");
+  *timed_out = status != 0;
+  return info;
+}
+
+int DeviceSolver::factor_outcome() {
+  bool timed_out = false;
+  const int info = factor_wait(&timed_out);
+  if (timed_out) throw NecError("LU: a panel exchange timed out");
   return info;
 }
 
@@ -460,7 +469,36 @@ void DeviceSolver::factor() {
   hip_check(hipEventRecord(ev_[0], stream_), "event");
   enqueue_factor();
   hip_check(hipEventRecord(ev_[1], stream_), "event");
-  int info = factor_outcome();
+  bool timed_out = false;
+  int info = factor_wait(&timed_out);
+  // NECGPU_TEST_PANEL_TIMEOUT=1 makes this factorisation count as timed out
+  // (the kernels themselves ran normally), to exercise the fallback
+  const char* force = std::getenv("NECGPU_TEST_PANEL_TIMEOUT");
+  if (force && force[0] == '1') timed_out = true;
+  if (timed_out) {
+    // The cooperative panel's workgroups wait on each other; if they could
+    // not all be resident beside the look-ahead update (a device shared with
+    // other work), their bounded wait gave up and the factors are wrong.
+    // Factor again with every launch on one stream, where a panel never
+    // shares the device with an update; the FP64 path refills Z first (the
+    // factorisation overwrote it).
+    ++lu_fallbacks_;
+    if (precision_ == 0) enqueue_fill(last_rkh_);
+    lookahead_ = false;
+    hip_check(hipEventRecord(ev_[0], stream_), "event");
+    try {
+      if (precision_ == 1)
+        enqueue_factor_f32(mixed_pivot_);
+      else
+        enqueue_factor();
+    } catch (...) {
+      lookahead_ = true;
+      throw;
+    }
+    lookahead_ = true;
+    hip_check(hipEventRecord(ev_[1], stream_), "event");
+    info = factor_outcome();
+  }
   if (precision_ == 1 && mixed_pivot_ == 1 && info != 0) {
     // a zero pivot inside a diagonal block: partial pivoting instead
     ++refine_fallbacks_;

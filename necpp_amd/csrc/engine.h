@@ -132,6 +132,10 @@ class DeviceSolver {
   // often the refinement fell back to partial pivoting
   int mixed_pivot() const { return mixed_pivot_; }
   long refine_fallbacks() const { return refine_fallbacks_; }
+  // factorisations repeated on one stream after a panel exchange timed out
+  // (the panel's workgroups could not all be resident beside the look-ahead
+  // update, e.g. on a device shared with other work)
+  long lu_fallbacks() const { return lu_fallbacks_; }
   // Near electric field of the last solve's currents at points given in
   // wavelengths (3 per point), ax as for launch_near_e; 3 complex per point.
   std::vector<zc> near_field_e(const std::vector<double>& pts, const std::vector<double>& ax, double rkh);
@@ -157,9 +161,15 @@ class DeviceSolver {
   DevBuf zload_dev_;
   bool has_loads_ = false;
   bool factored_ = false;
-  // waits for the enqueued factorisation: throws if a panel exchange timed
-  // out, returns the zero-pivot word
+  // waits for the enqueued factorisation: returns the zero-pivot word and
+  // whether a panel exchange timed out
+  int factor_wait(bool* timed_out);
+  // the same, throwing if a panel exchange timed out
   int factor_outcome();
+  // false: the next factorisations keep every launch on stream_ (set for
+  // the refactorisation after a timed-out panel exchange)
+  bool lookahead_ = true;
+  double last_rkh_ = 1.0;  // rkh of the last fill (a refactorisation refills)
   int lu_info_ = 0;
   unsigned epoch_ = 0;
   hipStream_t stream_ = nullptr;
@@ -176,7 +186,8 @@ class DeviceSolver {
   // back to partial pivoting
   static constexpr int kMaxLocalRefine = 10;
   int mixed_pivot_ = 1;        // pivot mode of the current FP32 factors
-  long refine_fallbacks_ = 0;  // refactorisations with partial pivoting
+  long refine_fallbacks_ = 0;
+  long lu_fallbacks_ = 0;  // refactorisations with partial pivoting
   void refine_solve();
 
   DevBuf seg_, ekind_, colptr_, colj_, colabc_, segptr_, segbasis_, segabc_;

@@ -165,9 +165,12 @@ int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int u
 // receives 1 + the first exactly-zero pivot, or stays 0.  work: device
 // scratch of lu_workspace_bytes(n) (no initialisation needed); lu_status
 // points at its status int, set to 1 if a panel exchange timed out.
+// allow_lookahead false keeps every launch on `stream` (no panel runs beside
+// a trailing update), the fallback after a timed-out exchange.
 size_t lu_workspace_bytes(int n);
 int* lu_status(void* work, int n);
-int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream);
+int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream,
+                   bool allow_lookahead = true);
 // The same LU with its workspace in a fixed device array of the library
 // (lu_noalloc_scratch_bytes() per device, for n <= kNoAllocMaxN) and no side
 // stream: no allocation of any kind.  Returns 3 for n > kNoAllocMaxN, 4 if
@@ -182,7 +185,7 @@ int launch_lu_fp64_noalloc(double* A, long long lda, int n, int* piv, int* info,
 // block-permuted matrix, for preconditioning).  work: lu_workspace_bytes(n)
 // is large enough for either.
 int launch_lu_fp32(float* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream,
-                   int pivot_mode);
+                   int pivot_mode, bool allow_lookahead = true);
 // one-wave check of the FP64 MFMA operand / result lane layout (64 a, 64 b in, 256 d out)
 int launch_mfma_probe(const double* a, const double* b, double* d, hipStream_t stream);
 int launch_mfma_probe_f32(const float* a, const float* b, float* d, hipStream_t stream);

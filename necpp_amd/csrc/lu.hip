@@ -1873,15 +1873,22 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
   constexpr int RMAX_T = rmax_of<T>();
   static bool attr_set = false;
   static int cu_count = 0;
+  static int panel_capacity = 0;  // panel workgroups that can be resident at once
   const size_t lds_bytes = (size_t)(2 * RMAX_T * (NB + 1) + 2 * NB) * sizeof(T);
   if (!attr_set) {
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(lu_panel_kernel<T>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds_bytes) != hipSuccess)
       return 1;
+    int occ = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&occ, lu_panel_kernel<T>, PANEL_THREADS, lds_bytes) !=
+        hipSuccess)
+      return 1;
+    panel_capacity = occ;
     int dev = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cu_count, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
       return 1;
+    panel_capacity *= cu_count;
     if (hipFuncSetAttribute(reinterpret_cast<const void*>(zgemm_tile_kernel<T>),
                             hipFuncAttributeMaxDynamicSharedMemorySize, (int)zt_lds_bytes<T>()) != hipSuccess)
       return 1;
@@ -1891,10 +1898,11 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
   const int gemm_mode = zgemm_mode();
   // buffer offsets of a 64-row tile stay below 2^31 bytes
   if ((long long)lda * 2 * (long long)sizeof(T) * 64 >= (1ll << 31)) return 2;
-  // every active panel workgroup needs its own CU at the same time (one per
-  // CU by LDS), so the first (largest) panel bounds n
+  // the cooperative panel's workgroups wait on each other, so all of them
+  // must fit on the device at once (one per CU by LDS: the occupancy query
+  // says how many); the first (largest) panel bounds n
   const int G0 = (n + RMAX_T - 1) / RMAX_T;
-  if (G0 > GMAX || G0 > cu_count) return 2;
+  if (G0 > GMAX || G0 > cu_count || G0 > panel_capacity) return 2;
   int* perm = piv + n;
   const LuWork w = carve(work, n);
   const int npanels = (n + NB - 1) / NB;
@@ -2042,13 +2050,14 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
 
 }  // namespace
 
-int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream) {
-  return launch_lu<double>(A, lda, n, piv, info, work, stream, false);
+int launch_lu_fp64(double* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream,
+                   bool allow_lookahead) {
+  return launch_lu<double>(A, lda, n, piv, info, work, stream, false, allow_lookahead);
 }
 
 int launch_lu_fp32(float* A, long long lda, int n, int* piv, int* info, void* work, hipStream_t stream,
-                   int pivot_mode) {
-  return launch_lu<float>(A, lda, n, piv, info, work, stream, pivot_mode == 1);
+                   int pivot_mode, bool allow_lookahead) {
+  return launch_lu<float>(A, lda, n, piv, info, work, stream, pivot_mode == 1, allow_lookahead);
 }
 
 // ---------------------------------------------------------------------

@@ -64,6 +64,7 @@ _SIGS = [
     ("nec_gpu_batch_adjoint", _L, [_P, _D, _D, _D, _D, _D, _P, _P, _P, _P, _P]),
     ("nec_gpu_set_refine_tolerance", _L, [_P, _D]),
     ("nec_gpu_refinement", _L, [_P, _P, _P]),
+    ("nec_gpu_factor_fallbacks", _L, [_P, _P, _P]),
     ("nec_gpu_adjoint_gradient", _L, [_P, _D, _D, _D, _D, _D, _P, _P, _P, _P]),
     ("nec_gpu_currents", _L, [_P, _I, _P, _L]),
     ("nec_gpu_stage_times", _L, [_P, _P]),
@@ -189,6 +190,16 @@ def gpu_refinement(ctx):
     if lib().nec_gpu_refinement(ctx, ctypes.byref(steps), ctypes.byref(res)) != 0:
         raise RuntimeError(nec_error_message())
     return steps.value, res.value
+
+
+def gpu_factor_fallbacks(ctx):
+    """(lu, pivot): factorisations repeated after a timed-out panel
+    exchange, and mixed precision's fallbacks to partial pivoting."""
+    lu = ctypes.c_long(0)
+    pv = ctypes.c_long(0)
+    if lib().nec_gpu_factor_fallbacks(ctx, ctypes.byref(lu), ctypes.byref(pv)) != 0:
+        raise RuntimeError(nec_error_message())
+    return lu.value, pv.value
 
 
 ADJOINT_INFO = ("gain_db", "zin_re", "zin_im", "host_ms", "device_ms", "dJ_dlambda")

@@ -297,6 +297,31 @@ def test_c2_grid_gpu(kats):
         api.nec_delete(ctx)
 
 
+@pytest.mark.parametrize("precision", [0, 1])
+def test_timed_out_panel_exchange_refactors_on_one_stream(precision, monkeypatch):
+    """A factorisation whose panel exchange timed out is repeated with every
+    launch on one stream (the FP64 path refilling the matrix the first
+    attempt overwrote) and gives the oracle's solution.  The timeout is
+    forced from the host (NECGPU_TEST_PANEL_TIMEOUT): the kernels run
+    normally."""
+    deck = decks.grid(21, rp=None)
+    o = po.run_deck(deck, nthreads=16, with_rp=False)
+    monkeypatch.setenv("NECGPU_TEST_PANEL_TIMEOUT", "1")
+    ctx = make(deck)
+    try:
+        assert api.nec_gpu_set_precision(ctx, precision) == 0
+        s, v = deck["source"], deck["voltage"]
+        assert api.nec_fr_card(ctx, 0, 1, deck["freq_mhz"], 0.0) == 0
+        assert api.nec_ex_card(ctx, 0, s[0], s[1], 0, v.real, v.imag, 0, 0, 0, 0) == 0
+        assert api.nec_xq_card(ctx, 0) == 0, api.nec_error_message()
+        assert api.gpu_factor_fallbacks(ctx)[0] == 1
+        assert rel(zin(ctx), o["zin"][0]) < TOL
+        cur = api.gpu_currents(ctx, 0)
+        assert np.linalg.norm(cur - o["current"]) <= TOL * np.linalg.norm(o["current"])
+    finally:
+        api.nec_delete(ctx)
+
+
 def test_frequency_sweep_and_reuse():
     deck = decks.dipole(21, rp=(19, 5))
     ctx = make(deck)
