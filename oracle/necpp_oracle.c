@@ -1359,13 +1359,17 @@ double ora_efld(double xj, double yj, double zj, double cabj, double sabj, doubl
   return r;
 }
 
-/* nefld, src/nec_context.cpp:4304-4450, for wires in free space: the
+static double complex to_c(cx a) { return CMPLX(a.re, a.im); }
+static cx from_c(double complex a) { return mk(creal(a), cimag(a)); }
+static void ground_image(const ora_geom* g, double xij, double yij, double zij, double r, cx t[9], cx e[9]);
+
+/* nefld, src/nec_context.cpp:4304-4450, for wires (no patches): the
  * electric field at npts points (xyz, in wavelengths) of the current
  * coefficients abc (planes air aii bir bii cir cii).  ax is the radius of
  * the first segment the point lies on (|zp| <= 0.5001 s, rho^2 <= 0.9 b^2),
  * else 0; every segment then contributes efld's nine components with
- * ij = true, e = ek A + es B + ec C per axis.  out: 6 doubles per point
- * (ex, ey, ez; re, im). */
+ * ij = true, plus over a ground the reflected field of efld, e = ek A +
+ * es B + ec C per axis.  out: 6 doubles per point (ex, ey, ez; re, im). */
 int ora_near_e(const ora_geom* g, const double* abc, int64_t npts, const double* xyz, double rkh, int use_ek,
                double* out) {
   const int64_t n = g->n;
@@ -1397,6 +1401,13 @@ int ora_near_e(const ora_geom* g, const double* abc, int64_t npts, const double*
       st.rkh = rkh; st.use_ek = use_ek; st.ind1 = ind1; st.ind2 = ind2;
       cx e[9];
       efld_compute(&st, xob - st.xj, yob - st.yj, ax, st.salpj, zob - st.zj, 1, e);
+      if (g->gtype != 0) {
+        /* the reflected field, as in the fill (efld, src/nec_context.cpp:2880-2956) */
+        cx t[9];
+        const double zij = zob + st.zj;
+        const double r = efld_compute(&st, xob - st.xj, yob - st.yj, ax, -st.salpj, zij, 1, t);
+        ground_image(g, xob - st.xj, yob - st.yj, zij, r, t, e);
+      }
       const cx acx = mk(air[i], aii[i]), bcx = mk(bir[i], bii[i]), ccx = mk(cir[i], cii[i]);
       ex = cadd(ex, cadd(cadd(cmul(e[0], acx), cmul(e[3], bcx)), cmul(e[6], ccx)));
       ey = cadd(ey, cadd(cadd(cmul(e[1], acx), cmul(e[4], bcx)), cmul(e[7], ccx)));
@@ -1409,13 +1420,271 @@ int ora_near_e(const ora_geom* g, const double* abc, int64_t npts, const double*
   return 0;
 }
 
+/* ------------------------------------------------------------------ */
+/* Near magnetic field (nhfld)                                         */
+/* ------------------------------------------------------------------ */
+
+/* gh, src/nec_context.cpp:3582-3596: the integrand of hfk at z, for the
+ * point at (rho^2 = rhks, z = zpka), distances in radians (2 pi x
+ * wavelengths): hr + j hi = (sin r / r^2 + cos r / r^3) + j (cos r / r^2 -
+ * sin r / r^3). */
+static void gh_integrand(double zk, double zpka, double rhks, double* hr, double* hi) {
+  double dz = zk - zpka;
+  const double rs = rhks + dz * dz;
+  const double r = sqrt(rs);
+  const double ckr = cos(r), skr = sin(r);
+  const double rr2 = 1. / rs;
+  const double rr3 = rr2 / r;
+  *hr = skr * rr2 + ckr * rr3;
+  *hi = ckr * rr2 - skr * rr3;
+}
+
+/* hfk, src/nec_context.cpp:3681-3822: the same adaptive Romberg scheme as
+ * intx (rx = 1e-4, steps halved after nts accepted, doubled up to 65536) of
+ * gh over [el1, el2], times rhk / 2. */
+static void hfk(double el1, double el2, double rhk, double zpkx, double* sgr, double* sgi) {
+  const int nx = 1, nma = 65536, nts = 4;
+  const double rx = 1.0e-4;
+  const double zpka = zpkx, rhks = rhk * rhk;
+  double z = el1;
+  const double ze = el2;
+  const double s = ze - z;
+  const double ep = s / (10. * nma);
+  const double zend = ze - ep;
+  double accr = 0.0, acci = 0.0;
+  int ns = nx, nt = 0, fresh = 1;
+  double dz = 0., dzot = 0., g1r, g1i, g2r, g2i, g3r = 0., g3i = 0., g4r, g4i, g5r = 0., g5i = 0.;
+  gh_integrand(z, zpka, rhks, &g1r, &g1i);
+  for (;;) {
+    if (fresh) {
+      dz = s / ns;
+      if (z + dz > ze) {
+        dz = ze - z;
+        if (fabs(dz) <= ep) break;
+      }
+      dzot = dz * .5;
+      gh_integrand(z + dzot, zpka, rhks, &g3r, &g3i);
+      gh_integrand(z + dz, zpka, rhks, &g5r, &g5i);
+    }
+    const double t00r = (g1r + g5r) * dzot, t00i = (g1i + g5i) * dzot;
+    const double t01r = (t00r + dz * g3r) * 0.5, t01i = (t00i + dz * g3i) * 0.5;
+    const double t10r = (4.0 * t01r - t00r) / 3.0, t10i = (4.0 * t01i - t00i) / 3.0;
+    double te1r, te1i;
+    conv_test(t01r, t10r, &te1r, t01i, t10i, &te1i, 0.);
+    double addr, addi;
+    if ((te1i <= rx) && (te1r <= rx)) {
+      addr = t10r;
+      addi = t10i;
+      nt += 2;
+    } else {
+      gh_integrand(z + dz * 0.25, zpka, rhks, &g2r, &g2i);
+      gh_integrand(z + dz * 0.75, zpka, rhks, &g4r, &g4i);
+      const double t02r = (t01r + dzot * (g2r + g4r)) * 0.5, t02i = (t01i + dzot * (g2i + g4i)) * 0.5;
+      const double t11r = (4.0 * t02r - t01r) / 3.0, t11i = (4.0 * t02i - t01i) / 3.0;
+      const double t20r = (16.0 * t11r - t10r) / 15.0, t20i = (16.0 * t11i - t10i) / 15.0;
+      double te2r, te2i;
+      conv_test(t11r, t20r, &te2r, t11i, t20i, &te2i, 0.);
+      const int bad = (te2i > rx) || (te2r > rx);
+      if (bad && ns < nma) {
+        /* halve the step and try again from z with the points in hand */
+        nt = 0;
+        ns *= 2;
+        dz = s / ns;
+        dzot = dz * 0.5;
+        g5r = g3r; g5i = g3i;
+        g3r = g2r; g3i = g2i;
+        fresh = 0;
+        continue;
+      }
+      if (bad) nt = 0; /* step size limited: accept the 5-point result */
+      addr = t20r;
+      addi = t20i;
+      nt++;
+    }
+    accr += addr;
+    acci += addi;
+    z += dz;
+    if (z >= zend) break;
+    g1r = g5r;
+    g1i = g5i;
+    if (nt >= nts && ns > nx) { ns /= 2; nt = 1; }
+    fresh = 1;
+  }
+  *sgr = accr * rhk * .5;
+  *sgi = acci * rhk * .5;
+}
+
+/* hsflx, src/nec_context.cpp:4059-4131: the azimuthal H of the constant
+ * (hpk), sine (hps) and cosine (hpc) currents of a segment of length s at
+ * radial distance rh and axial offset zpx from its centre (wavelengths). */
+static void hsflx(double s, double rh, double zpx, cx* hpk, cx* hps, cx* hpc) {
+  if (rh < 1.0e-10) {
+    *hpk = mk(0, 0); *hps = mk(0, 0); *hpc = mk(0, 0);
+    return;
+  }
+  const double hss = zpx >= 0.0 ? 1.0 : -1.0;
+  const double zp = fabs(zpx);
+  const double dh = 0.5 * s;
+  const double z1 = zp + dh, z2a = zp - dh;
+  const double rhz = z2a >= 1.0e-7 ? rh / z2a : 1.0;
+  const double dk = TWO_PI * dh;
+  const double cdk = cos(dk), sdk = sin(dk);
+  double hkr, hki;
+  hfk(-dk, dk, rh * TWO_PI, zp * TWO_PI, &hkr, &hki);
+  *hpk = mk(hkr, hki);
+  if (rhz >= 1.0e-3) {
+    const double rh2 = rh * rh;
+    const double r1 = sqrt(rh2 + z1 * z1), r2 = sqrt(rh2 + z2a * z2a);
+    /* exp(-j 2 pi r) */
+    const cx ekr1 = mk(cos(TWO_PI * r1), -sin(TWO_PI * r1));
+    const cx ekr2 = mk(cos(TWO_PI * r2), -sin(TWO_PI * r2));
+    const cx t1 = cdivr(cmulr(ekr1, z1), r1);
+    const cx t2 = cdivr(cmulr(ekr2, z2a), r2);
+    /* hps = (cdk (ekr2 - ekr1) - j sdk (t2 + t1)) hss; hpc = -sdk (ekr2 +
+     * ekr1) - j cdk (t2 - t1); both times cons = -j / (4 pi rh) */
+    const cx d = csub(ekr2, ekr1), sm = cadd(t2, t1);
+    cx ps = mk(cdk * d.re + sdk * sm.im, cdk * d.im - sdk * sm.re);
+    ps = cmulr(ps, hss);
+    const cx a2 = cadd(ekr2, ekr1), df = csub(t2, t1);
+    const cx pc = mk(-sdk * a2.re + cdk * df.im, -sdk * a2.im - cdk * df.re);
+    const double cons = -1.0 / (2.0 * TWO_PI * rh); /* times j */
+    *hps = mk(-cons * ps.im, cons * ps.re);
+    *hpc = mk(-cons * pc.im, cons * pc.re);
+    return;
+  }
+  /* close to the segment's axis beyond an end */
+  const cx ekr1 = cdivr(mk(cdk, sdk), z2a * z2a);
+  const cx ekr2 = cdivr(mk(cdk, -sdk), z1 * z1);
+  const double t1 = TWO_PI * (1.0 / z1 - 1.0 / z2a);
+  const cx t2 = cdivr(cmulr(mk(cos(TWO_PI * zp), -sin(TWO_PI * zp)), rh), PI_T * 8.0);
+  const cx sps = cadd(mk(t1, 0.0), cmulr(cadd(ekr1, ekr2), sdk));
+  *hps = cmulr(cmul(t2, sps), hss);
+  const cx dpc = cmulr(csub(ekr1, ekr2), cdk);
+  *hpc = cmul(t2, mk(dpc.re, dpc.im - t1));
+}
+
+/* hsfld, src/nec_context.cpp:3927-4055: the H field at (xi, yi, zi) of the
+ * constant, sine and cosine currents of source segment st (h[0..2] k,
+ * h[3..5] s, h[6..8] c; x, y, z each), ai the observation radius.  Over a
+ * ground the image segment (z and the z cosine negated) is subtracted,
+ * weighted for the finite ground by the reflection coefficients of the
+ * specular direction.  An evaluation point on a segment's axis (rho <=
+ * 1e-10) zeroes all nine components, the image's too. */
+static void hsfld(const ora_geom* g, const fill_state* st, double xi, double yi, double zi, double ai, cx h[9]) {
+  const double xij = xi - st->xj, yij = yi - st->yj;
+  const int loops = g->gtype != 0 ? 2 : 1;
+  for (int q = 0; q < 9; ++q) h[q] = mk(0, 0);
+  for (int loop = 0; loop < loops; ++loop) {
+    const double rfl = loop == 0 ? 1.0 : -1.0;
+    const double salpr = st->salpj * rfl;
+    const double zij = zi - rfl * st->zj;
+    const double zp = xij * st->cabj + yij * st->sabj + zij * salpr;
+    double rhox = xij - st->cabj * zp, rhoy = yij - st->sabj * zp, rhoz = zij - salpr * zp;
+    const double rh = sqrt(rhox * rhox + rhoy * rhoy + rhoz * rhoz + ai * ai);
+    if (rh <= 1.0e-10) {
+      for (int q = 0; q < 9; ++q) h[q] = mk(0, 0);
+      continue;
+    }
+    rhox /= rh; rhoy /= rh; rhoz /= rh;
+    /* phi-hat = source direction x rho-hat */
+    const double phx = st->sabj * rhoz - salpr * rhoy;
+    const double phy = salpr * rhox - st->cabj * rhoz;
+    const double phz = st->cabj * rhoy - st->sabj * rhox;
+    cx hp[3];
+    hsflx(st->s, rh, zp, &hp[0], &hp[1], &hp[2]);
+    if (loop == 0) {
+      for (int k = 0; k < 3; ++k) {
+        h[3 * k] = cmulr(hp[k], phx);
+        h[3 * k + 1] = cmulr(hp[k], phy);
+        h[3 * k + 2] = cmulr(hp[k], phz);
+      }
+      continue;
+    }
+    if (g->gtype == 1) {
+      for (int k = 0; k < 3; ++k) {
+        h[3 * k] = csub(h[3 * k], cmulr(hp[k], phx));
+        h[3 * k + 1] = csub(h[3 * k + 1], cmulr(hp[k], phy));
+        h[3 * k + 2] = csub(h[3 * k + 2], cmulr(hp[k], phz));
+      }
+      continue;
+    }
+    /* finite ground: reflection coefficients (vertical rrv, horizontal rrh) */
+    const double complex zratx = CMPLX(g->zr_re, g->zr_im);
+    const double rmag = sqrt(zp * zp + rh * rh);
+    const double xymag = sqrt(xij * xij + yij * yij);
+    double px, py, cth;
+    double complex rrv;
+    if (xymag <= 1.0e-6) {
+      px = 0.; py = 0.; cth = 1.; rrv = 1.0;
+    } else {
+      px = -yij / xymag;
+      py = xij / xymag;
+      cth = zij / rmag;
+      rrv = csqrt(1.0 - zratx * zratx * (1.0 - cth * cth));
+    }
+    double complex rrh = zratx * cth;
+    rrh = -(rrh - rrv) / (rrh + rrv);
+    rrv = zratx * rrv;
+    rrv = (cth - rrv) / (cth + rrv);
+    double complex qy = (phx * px + phy * py) * (rrv - rrh);
+    const double complex qx = qy * px + phx * rrh;
+    qy = qy * py + phy * rrh;
+    const double complex qz = phz * rrh;
+    for (int k = 0; k < 3; ++k) {
+      const double complex hk = CMPLX(hp[k].re, hp[k].im);
+      h[3 * k] = from_c(to_c(h[3 * k]) - hk * qx);
+      h[3 * k + 1] = from_c(to_c(h[3 * k + 1]) - hk * qy);
+      h[3 * k + 2] = from_c(to_c(h[3 * k + 2]) - hk * qz);
+    }
+  }
+}
+
+/* nhfld, src/nec_context.cpp:5189-5262, for wires (no patches): the
+ * magnetic field at npts points (xyz in wavelengths), the observation radius
+ * found as nefld does, H = hk A + hs B + hc C per axis.  out: 6 doubles per
+ * point (hx, hy, hz; re, im). */
+int ora_near_h(const ora_geom* g, const double* abc, int64_t npts, const double* xyz, double* out) {
+  const int64_t n = g->n;
+  const double *air = abc, *aii = abc + n, *bir = abc + 2 * n, *bii = abc + 3 * n, *cir = abc + 4 * n,
+               *cii = abc + 5 * n;
+  for (int64_t p = 0; p < npts; ++p) {
+    const double xob = xyz[3 * p], yob = xyz[3 * p + 1], zob = xyz[3 * p + 2];
+    double ax = 0.0;
+    for (int64_t i = 0; i < n; ++i) {
+      const double xj = xob - g->xs[i], yj = yob - g->ys[i], zj = zob - g->zs[i];
+      double zp = g->cab[i] * xj + g->sab[i] * yj + g->salp[i] * zj;
+      if (fabs(zp) > 0.5001 * g->ss[i]) continue;
+      zp = xj * xj + yj * yj + zj * zj - zp * zp;
+      const double b = g->bs[i];
+      if (zp > 0.9 * b * b) continue;
+      ax = b;
+      break;
+    }
+    cx hx = mk(0.0, 0.0), hy = mk(0.0, 0.0), hz = mk(0.0, 0.0);
+    for (int64_t i = 0; i < n; ++i) {
+      fill_state st;
+      memset(&st, 0, sizeof st);
+      st.s = g->ss[i]; st.b = g->bs[i]; st.xj = g->xs[i]; st.yj = g->ys[i]; st.zj = g->zs[i];
+      st.cabj = g->cab[i]; st.sabj = g->sab[i]; st.salpj = g->salp[i];
+      cx h[9];
+      hsfld(g, &st, xob, yob, zob, ax, h);
+      const cx acx = mk(air[i], aii[i]), bcx = mk(bir[i], bii[i]), ccx = mk(cir[i], cii[i]);
+      hx = cadd(hx, cadd(cadd(cmul(h[0], acx), cmul(h[3], bcx)), cmul(h[6], ccx)));
+      hy = cadd(hy, cadd(cadd(cmul(h[1], acx), cmul(h[4], bcx)), cmul(h[7], ccx)));
+      hz = cadd(hz, cadd(cadd(cmul(h[2], acx), cmul(h[5], bcx)), cmul(h[8], ccx)));
+    }
+    out[6 * p] = hx.re; out[6 * p + 1] = hx.im;
+    out[6 * p + 2] = hy.re; out[6 * p + 3] = hy.im;
+    out[6 * p + 4] = hz.re; out[6 * p + 5] = hz.im;
+  }
+  return 0;
+}
+
 #define TRIO_CAP 64
 
 /* cmset + cmww, src/nec_context.cpp:2077-2203 and :2514-2654, transposed
  * fill (itrp = 1): Z[i][k] += etk*ax + ets*bx + etc*cx for each basis k in
  * trio(j), sources j in ascending order. */
-static double complex to_c(cx a) { return CMPLX(a.re, a.im); }
-static cx from_c(double complex a) { return mk(creal(a), cimag(a)); }
 
 /* The reflected field of efld, src/nec_context.cpp:2880-2956: t holds the
  * image fields (efld_compute with zij = zi + zj and -salpj); for the finite

@@ -157,11 +157,15 @@ void ora_ffld(const ora_geom* g, const double* abc, double sth, double cth, doub
  * distance rh; ij = 0 for the self term.
  * ora_intx: out[2] = (cint, sint) of intx over [el1, el2] with the gf state
  * (zpk, rkb2, ija).
- * ora_near_e: nefld (src/nec_context.cpp:4304-4450) for wires in free space:
- * E at npts points (xyz in wavelengths, at the last ora_frequency_scale) of
- * the current coefficients abc; out holds 6 doubles per point. */
+ * ora_near_e: nefld (src/nec_context.cpp:4304-4450) for wires, in free
+ * space or over the ground of ora_set_ground: E at npts points (xyz in
+ * wavelengths, at the last ora_frequency_scale) of the current coefficients
+ * abc; out holds 6 doubles per point.
+ * ora_near_h: nhfld (src/nec_context.cpp:5189-5262) for wires, likewise: H
+ * at the points, 6 doubles per point. */
 int ora_near_e(const ora_geom* g, const double* abc, int64_t npts, const double* xyz, double rkh, int use_ek,
                double* out);
+int ora_near_h(const ora_geom* g, const double* abc, int64_t npts, const double* xyz, double* out);
 
 /*
  * ora_efld: out[18] = the 9 complex field components egnd of efld_compute

@@ -62,6 +62,8 @@ def lib():
         L.ora_intx.argtypes = [d, d, i, d, d, d, i, p]
         L.ora_near_e.argtypes = [p, p, i64, p, d, i, p]
         L.ora_near_e.restype = i
+        L.ora_near_h.argtypes = [p, p, i64, p, p]
+        L.ora_near_h.restype = i
         L.ora_efld.argtypes = [d] * 12 + [i, d, i, i, i, p]
         L.ora_efld.restype = d
         _lib = L
@@ -258,6 +260,14 @@ class Geometry:
         out = np.zeros(6 * len(pts))
         self._L.ora_near_e(self._g, _ptr(np.ascontiguousarray(abc, np.float64)), len(pts), _ptr(pts), rkh,
                            int(use_ek), _ptr(out))
+        return (out[0::2] + 1j * out[1::2]).reshape(-1, 3)
+
+    def near_h(self, abc, pts_wl):
+        """nhfld: H (npts x 3 complex) at points given in wavelengths, from
+        the current coefficients abc, at the current scaling and ground."""
+        pts = np.ascontiguousarray(pts_wl, np.float64).reshape(-1, 3)
+        out = np.zeros(6 * len(pts))
+        self._L.ora_near_h(self._g, _ptr(np.ascontiguousarray(abc, np.float64)), len(pts), _ptr(pts), _ptr(out))
         return (out[0::2] + 1j * out[1::2]).reshape(-1, 3)
 
     def current_coefficients(self, x):
