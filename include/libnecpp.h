@@ -103,7 +103,8 @@ long nec_pt_card(nec_context* in_context, int itmp1, int itmp2, int itmp3, int i
 long nec_pq_card(nec_context* in_context, int itmp1, int itmp2, int itmp3, int itmp4);
 /* src/libnecpp.h:509, src/libNEC.cpp:250-253 -> nec_context::kh_card */
 long nec_kh_card(nec_context* in_context, double tmp1);
-/* src/libnecpp.h:510-511 -> nec_context::ne_card / nh_card (NE runs on the GPU path for wires in free space, results through nec_gpu_near_field; NH is not on the GPU path) */
+/* src/libnecpp.h:510-511 -> nec_context::ne_card / nh_card (both run on the GPU path for wires in free space or
+ * over a ground without radial wires; results through nec_gpu_near_field) */
 long nec_ne_card(nec_context* in_context, int itmp1, int itmp2, int itmp3, int itmp4, double tmp1,
                  double tmp2, double tmp3, double tmp4, double tmp5, double tmp6);
 long nec_nh_card(nec_context* in_context, int itmp1, int itmp2, int itmp3, int itmp4, double tmp1,

@@ -155,13 +155,20 @@ long nec_gpu_batch_solve(const nec_gpu_batch* batch, double* zin, double* gain, 
  * line and card. */
 long nec_gpu_deck(nec_context* in_context, const char* deck);
 
-/* The near electric field of NE card number `index` (0-based, in order):
- * for each point x, y, z (metres) then Ex, Ey, Ez (re, im; V/m), 9 doubles,
- * at most max_points written to out.  Returns the number of points, or -1
- * when there is no such result.  NE (src/nec_context.cpp:896-955, nfpat
- * :5068-5188, nefld :4304-4450) runs on the GPU for wires in free space; the
+/* The near field of NE or NH result number `index` (0-based, in the order
+ * computed): for each point x, y, z (metres) then the field's x, y, z
+ * components (re, im; V/m for NE, A/m for NH), 9 doubles, at most
+ * max_points written to out.  Returns the number of points, or -1 when there
+ * is no such result.  NE / NH (src/nec_context.cpp:896-955, nfpat
+ * :5068-5188, nefld :4304-4450, nhfld :5189-5262) run on the GPU for wires
+ * in free space or over a perfect or reflection-coefficient ground; the
  * reference exposes near fields only through its printed output. */
 long nec_gpu_near_field(nec_context* in_context, int index, double* out, long max_points);
+
+/* Which field near-field result `index` holds: 0 electric (NE card), 1
+ * magnetic (NH card), -1 no such result.  No counterpart in the reference
+ * (its printed output names the field). */
+long nec_gpu_near_field_kind(nec_context* in_context, int index);
 
 /* Adjoint gradients of a whole batch (the structures and frequencies of
  * `batch`; its pattern and repeat fields are not used): for every (design,

@@ -152,8 +152,9 @@ long nec_ne_card(nec_context* ctx, int itmp1, int itmp2, int itmp3, int itmp4, d
                  double tmp4, double tmp5, double tmp6) {
   return guarded([&] { ctx->ne_card(itmp1, itmp2, itmp3, itmp4, tmp1, tmp2, tmp3, tmp4, tmp5, tmp6); });
 }
-long nec_nh_card(nec_context*, int, int, int, int, double, double, double, double, double, double) {
-  return unsupported("NH");
+long nec_nh_card(nec_context* ctx, int itmp1, int itmp2, int itmp3, int itmp4, double tmp1, double tmp2, double tmp3,
+                 double tmp4, double tmp5, double tmp6) {
+  return guarded([&] { ctx->nh_card(itmp1, itmp2, itmp3, itmp4, tmp1, tmp2, tmp3, tmp4, tmp5, tmp6); });
 }
 long nec_cp_card(nec_context*, int, int, int, int) { return unsupported("CP"); }
 long nec_pl_card(nec_context*, char*, int, int, int, int) { return unsupported("PL"); }
@@ -411,6 +412,13 @@ long nec_gpu_near_field(nec_context* ctx, int index, double* out, long max_point
       }
     }
   return np;
+}
+
+long nec_gpu_near_field_kind(nec_context* ctx, int index) {
+  if (!ctx) return -1;
+  const Context::NearFieldResult* r = ctx->near_field(index);
+  if (!r) return -1;
+  return r->magnetic ? 1 : 0;
 }
 
 long nec_gpu_deck(nec_context* ctx, const char* deck) {

@@ -348,9 +348,20 @@ void Context::adjoint(const AdjointSpec& spec, AdjointResult* r, std::vector<dou
 
 void Context::ne_card(int near, int nrx, int nry, int nrz, double xnr, double ynr, double znr, double dxnr,
                       double dynr, double dznr) {
-  // src/nec_context.cpp:930-955: blank counts default to 1; with one
-  // frequency the field is computed at once
-  if (ground_spec().type != 0) throw NecError("NE: near fields over a ground are not supported by the GPU path");
+  near_card(0, near, nrx, nry, nrz, xnr, ynr, znr, dxnr, dynr, dznr);
+}
+
+void Context::nh_card(int near, int nrx, int nry, int nrz, double xnr, double ynr, double znr, double dxnr,
+                      double dynr, double dznr) {
+  near_card(1, near, nrx, nry, nrz, xnr, ynr, znr, dxnr, dynr, dznr);
+}
+
+void Context::near_card(int nfeh, int near, int nrx, int nry, int nrz, double xnr, double ynr, double znr,
+                        double dxnr, double dynr, double dznr) {
+  // ne_nh_card, src/nec_context.cpp:930-955: the last NE or NH card is the
+  // pending request; blank counts default to 1; with one frequency the
+  // field is computed at once
+  near_spec_.nfeh = nfeh;
   near_spec_.near = near;
   near_spec_.nrx = nrx == 0 ? 1 : nrx;
   near_spec_.nry = nry == 0 ? 1 : nry;
@@ -419,7 +430,8 @@ void Context::compute_near_field() {
       break;
     }
   }
-  r.e = device().near_field_e(pts, ax, rkh_);
+  r.magnetic = ns.nfeh == 1;
+  r.e = device().near_field(r.magnetic, pts, ax, rkh_);
   near_fields_.push_back(std::move(r));
 }
 

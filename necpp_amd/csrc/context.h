@@ -66,11 +66,14 @@ class Context {
   // rectangular (near = 0) or spherical (near = 1) grid of points
   void ne_card(int near, int nrx, int nry, int nrz, double xnr, double ynr, double znr, double dxnr, double dynr,
                double dznr);
-  // near-field results: points (metres, 3 each) and E (3 complex each)
+  void nh_card(int near, int nrx, int nry, int nrz, double xnr, double ynr, double znr, double dxnr, double dynr,
+               double dznr);
+  // near-field results: points (metres, 3 each) and E or H (3 complex each)
   struct NearFieldResult {
     double freq_hz = 0.0;
+    bool magnetic = false;  // H (NH card) rather than E (NE card)
     std::vector<double> xyz;
-    std::vector<zc> e;
+    std::vector<zc> e;      // the field, 3 complex per point
   };
   const NearFieldResult* near_field(int k) const {
     return (k >= 0 && k < (int)near_fields_.size()) ? &near_fields_[k] : nullptr;
@@ -138,6 +141,7 @@ class Context {
   };
   std::vector<LoadCard> loads_;
   struct NearSpec {
+    int nfeh = 0;  // 0 electric (NE), 1 magnetic (NH)
     int near = 0, nrx = 1, nry = 1, nrz = 1;
     double xnr = 0, ynr = 0, znr = 0, dxnr = 0, dynr = 0, dznr = 0;
   } near_spec_;
@@ -146,6 +150,8 @@ class Context {
   bool near_pending_ = false;
   std::vector<NearFieldResult> near_fields_;
   void compute_near_field();
+  void near_card(int nfeh, int near, int nrx, int nry, int nrz, double xnr, double ynr, double znr, double dxnr,
+                 double dynr, double dznr);
   // lambda * impedance per unit length of every segment at the current
   // frequency (nec_context::load); empty without loads
   std::vector<zc> load_impedances(const ScaledSegments& seg) const;

@@ -249,8 +249,8 @@ void DeviceSolver::set_loads(const std::vector<zc>& z) {
   hip_check(hipStreamSynchronize(stream_), "loads");  // v is released on return
 }
 
-std::vector<zc> DeviceSolver::near_field_e(const std::vector<double>& pts, const std::vector<double>& ax,
-                                           double rkh) {
+std::vector<zc> DeviceSolver::near_field(bool magnetic, const std::vector<double>& pts,
+                                         const std::vector<double>& ax, double rkh) {
   const int np = static_cast<int>(ax.size());
   std::vector<zc> out;
   if (np == 0) return out;
@@ -259,8 +259,9 @@ std::vector<zc> DeviceSolver::near_field_e(const std::vector<double>& pts, const
   upload(dax, ax, stream_);
   dpart.ensure((size_t)np * ((n_ + 255) / 256) * 6 * sizeof(double));
   dout.ensure((size_t)np * 6 * sizeof(double));
-  if (launch_near_e(dev_segments(), n_, dpts.as<double>(), dax.as<double>(), np, abc_.as<double>(), use_ek_ ? 1 : 0,
-                    ekind_.as<int>(), rkh, dpart.as<double>(), dout.as<double>(), stream_))
+  if (launch_near_field(magnetic, dev_segments(), n_, dpts.as<double>(), dax.as<double>(), np, abc_.as<double>(),
+                        use_ek_ ? 1 : 0, ekind_.as<int>(), rkh, ground_, dpart.as<double>(), dout.as<double>(),
+                        stream_))
     hip_check(hipGetLastError(), "near field launch");
   std::vector<double> h((size_t)np * 6);
   hip_check(hipMemcpyAsync(h.data(), dout.as<void>(), h.size() * sizeof(double), hipMemcpyDeviceToHost, stream_),

@@ -136,9 +136,12 @@ class DeviceSolver {
   // (the panel's workgroups could not all be resident beside the look-ahead
   // update, e.g. on a device shared with other work)
   long lu_fallbacks() const { return lu_fallbacks_; }
-  // Near electric field of the last solve's currents at points given in
-  // wavelengths (3 per point), ax as for launch_near_e; 3 complex per point.
-  std::vector<zc> near_field_e(const std::vector<double>& pts, const std::vector<double>& ax, double rkh);
+  // Near electric (magnetic false) or magnetic field of the last solve's
+  // currents, over the ground of the last fill, at points given in
+  // wavelengths (3 per point), ax as for launch_near_field; 3 complex per
+  // point.
+  std::vector<zc> near_field(bool magnetic, const std::vector<double>& pts, const std::vector<double>& ax,
+                             double rkh);
   double refine_residual() const { return refine_residual_; }
   hipStream_t stream() const { return stream_; }
 

@@ -481,24 +481,26 @@ __device__ void pair_fields(const DevSegments& g, int i, int j, double rkh, int 
   *etc = zmk(c.re, c.im);
 }
 
-// Pair (i, j) over a ground, efld src/nec_context.cpp:2845-2956: the direct
+// The fields at (xi, yi, zi) (observation radius ai) of segment j's unit
+// constant / sine / cosine currents over a ground, efld
+// src/nec_context.cpp:2845-2956, unprojected (x, y, z of each): the direct
 // fields e and the image fields t (efld_compute with zij = zi + zj, the
 // source direction's z cosine negated, never the self term), t weighted for
 // the finite ground by the reflection coefficients at the specular point,
-// then e - t * frati projected on the observation direction.
-__device__ void pair_fields_ground(const DevSegments& g, int i, int j, double rkh, int use_ek, int ind1, int ind2,
-                                   const GroundSpec& gs, zc* etk, zc* ets, zc* etc) {
+// then d = e - t * frati.
+__device__ void point_fields_ground(const DevSegments& g, int j, double xi, double yi, double zi, double ai,
+                                    bool self, double rkh, int use_ek, int ind1, int ind2, const GroundSpec& gs,
+                                    zc d[9]) {
   cx<double> a, b, c, e[9], t[9];
-  pair_fields_v<double>(g.x[i], g.y[i], g.z[i], g.b[i], g.cab[i], g.sab[i], g.salp[i], g.x[j], g.y[j], g.z[j],
-                        g.s[j], g.b[j], g.cab[j], g.sab[j], g.salp[j], i == j, rkh, use_ek, ind1, ind2, &a, &b, &c, e);
+  pair_fields_v<double>(xi, yi, zi, ai, 1.0, 0.0, 0.0, g.x[j], g.y[j], g.z[j], g.s[j], g.b[j], g.cab[j], g.sab[j],
+                        g.salp[j], self, rkh, use_ek, ind1, ind2, &a, &b, &c, e);
   double r;
   // zi - (-zj) == zi + zj exactly
-  pair_fields_v<double>(g.x[i], g.y[i], g.z[i], g.b[i], g.cab[i], g.sab[i], g.salp[i], g.x[j], g.y[j], -g.z[j],
-                        g.s[j], g.b[j], g.cab[j], g.sab[j], -g.salp[j], false, rkh, use_ek, ind1, ind2, &a, &b, &c, t,
-                        &r);
+  pair_fields_v<double>(xi, yi, zi, ai, 1.0, 0.0, 0.0, g.x[j], g.y[j], -g.z[j], g.s[j], g.b[j], g.cab[j], g.sab[j],
+                        -g.salp[j], false, rkh, use_ek, ind1, ind2, &a, &b, &c, t, &r);
   if (gs.type == 2) {
     const zc zratx = zmk(gs.zr_re, gs.zr_im);
-    const double xij = g.x[i] - g.x[j], yij = g.y[i] - g.y[j], zij = g.z[i] + g.z[j];
+    const double xij = xi - g.x[j], yij = yi - g.y[j], zij = zi + g.z[j];
     const double rmag = r;
     const double xymag = sqrt(xij * xij + yij * yij);
     double px, py, cth;
@@ -535,8 +537,15 @@ __device__ void pair_fields_ground(const DevSegments& g, int i, int j, double rk
     }
   }
   const zc fr = zmk(gs.fr_re, gs.fr_im);
-  zc d[9];
   for (int q = 0; q < 9; ++q) d[q] = zsub(zmk(e[q].re, e[q].im), zmul(zmk(t[q].re, t[q].im), fr));
+}
+
+// Pair (i, j) over a ground: the point fields at segment i's centre with its
+// radius, projected on i's direction.
+__device__ void pair_fields_ground(const DevSegments& g, int i, int j, double rkh, int use_ek, int ind1, int ind2,
+                                   const GroundSpec& gs, zc* etk, zc* ets, zc* etc) {
+  zc d[9];
+  point_fields_ground(g, j, g.x[i], g.y[i], g.z[i], g.b[i], i == j, rkh, use_ek, ind1, ind2, gs, d);
   const double ci = g.cab[i], si = g.sab[i], li = g.salp[i];
   *etk = zadd(zadd(zmulr(d[0], ci), zmulr(d[1], si)), zmulr(d[2], li));
   *ets = zadd(zadd(zmulr(d[3], ci), zmulr(d[4], si)), zmulr(d[5], li));
@@ -979,38 +988,289 @@ int launch_batch_fill(const BatchSolve* sol, const int* blk, int nblk, int R, in
 
 namespace {
 
-// nefld (src/nec_context.cpp:4304-4450) for wires in free space: one thread
-// per (point, source segment) evaluates efld's nine field components at the
-// point (pair_fields_v; ij = true, the observation radius ax of the point's
-// own segment or 0) and contracts them with the segment's current
-// coefficients A, B, C; the 256 segments of a block are summed in a fixed
-// tree order, the blocks in order by near_e_finish_kernel.
-__global__ void __launch_bounds__(256) near_e_kernel(DevSegments g, int n, const double* __restrict__ pts,
-                                                     const double* __restrict__ ax, const double* __restrict__ abc,
-                                                     int use_ek, const int* __restrict__ ekind, double rkh,
-                                                     double* __restrict__ part) {
+// ------------------------------------------------------------ near fields
+
+// gh, src/nec_context.cpp:3582-3596: hr + j hi = (sin r / r^2 + cos r /
+// r^3) + j (cos r / r^2 - sin r / r^3), r the distance from (rho^2 = rhks,
+// z = zpka) to the axis point zk (radians).
+__device__ __forceinline__ void gh(double zk, double zpka, double rhks, double* hr, double* hi) {
+  const double dz = zk - zpka;
+  const double rs = rhks + dz * dz;
+  const double r = sqrt(rs);
+  double skr, ckr;
+  sincos(r, &skr, &ckr);
+  const double rr2 = 1. / rs;
+  const double rr3 = rr2 / r;
+  *hr = skr * rr2 + ckr * rr3;
+  *hi = ckr * rr2 - skr * rr3;
+}
+
+// hfk, src/nec_context.cpp:3681-3822: gh integrated over [el1, el2] by the
+// adaptive Romberg scheme of intx (the same decisions), times rhk / 2.
+__device__ void hfk(double el1, double el2, double rhk, double zpka, double* sgr, double* sgi) {
+  const int nx = 1, nma = 65536, nts = 4;
+  const double rx = 1.0e-4;
+  const double rhks = rhk * rhk;
+  double z = el1;
+  const double ze = el2;
+  const double s = ze - z;
+  const double ep = s / (10. * nma);
+  const double zend = ze - ep;
+  double accr = 0.0, acci = 0.0;
+  int ns = nx, nt = 0;
+  bool fresh = true;
+  double dz = 0., dzot = 0., g1r, g1i, g2r, g2i, g3r = 0., g3i = 0., g4r, g4i, g5r = 0., g5i = 0.;
+  gh(z, zpka, rhks, &g1r, &g1i);
+  // at most 65536 subdivisions of a finite interval; the guard only stops a
+  // non-finite one from spinning
+  for (long guard = 0; guard < (1L << 22); ++guard) {
+    if (fresh) {
+      const double fns = ns;
+      dz = s / fns;
+      if (z + dz > ze) {
+        dz = ze - z;
+        if (fabs(dz) <= ep) break;
+      }
+      dzot = dz * .5;
+      gh(z + dzot, zpka, rhks, &g3r, &g3i);
+      gh(z + dz, zpka, rhks, &g5r, &g5i);
+    }
+    const double t00r = (g1r + g5r) * dzot, t00i = (g1i + g5i) * dzot;
+    const double t01r = (t00r + dz * g3r) * 0.5, t01i = (t00i + dz * g3i) * 0.5;
+    const double t10r = (4.0 * t01r - t00r) / 3.0, t10i = (4.0 * t01i - t00i) / 3.0;
+    double te1r, te1i;
+    conv_test(t01r, t10r, &te1r, t01i, t10i, &te1i);
+    double addr, addi;
+    if ((te1i <= rx) && (te1r <= rx)) {
+      addr = t10r;
+      addi = t10i;
+      nt += 2;
+    } else {
+      gh(z + dz * 0.25, zpka, rhks, &g2r, &g2i);
+      gh(z + dz * 0.75, zpka, rhks, &g4r, &g4i);
+      const double t02r = (t01r + dzot * (g2r + g4r)) * 0.5, t02i = (t01i + dzot * (g2i + g4i)) * 0.5;
+      const double t11r = (4.0 * t02r - t01r) / 3.0, t11i = (4.0 * t02i - t01i) / 3.0;
+      const double t20r = (16.0 * t11r - t10r) / 15.0, t20i = (16.0 * t11i - t10i) / 15.0;
+      double te2r, te2i;
+      conv_test(t11r, t20r, &te2r, t11i, t20i, &te2i);
+      const bool bad = (te2i > rx) || (te2r > rx);
+      if (bad && ns < nma) {
+        nt = 0;
+        ns *= 2;
+        const double fns = ns;
+        dz = s / fns;
+        dzot = dz * 0.5;
+        g5r = g3r;
+        g5i = g3i;
+        g3r = g2r;
+        g3i = g2i;
+        fresh = false;
+        continue;
+      }
+      if (bad) nt = 0;  // step size limited: the 5-point value is accepted
+      addr = t20r;
+      addi = t20i;
+      nt++;
+    }
+    accr += addr;
+    acci += addi;
+    z += dz;
+    if (z >= zend) break;
+    g1r = g5r;
+    g1i = g5i;
+    if (nt >= nts && ns > nx) {
+      ns /= 2;
+      nt = 1;
+    }
+    fresh = true;
+  }
+  *sgr = accr * rhk * .5;
+  *sgi = acci * rhk * .5;
+}
+
+// hsflx, src/nec_context.cpp:4059-4131: the azimuthal H of a segment's
+// constant (hpk), sine (hps) and cosine (hpc) currents at radial distance rh
+// and axial offset zpx from its centre (wavelengths).
+__device__ void hsflx(double s, double rh, double zpx, zc* hpk, zc* hps, zc* hpc) {
+  if (rh < 1.0e-10) {
+    *hpk = *hps = *hpc = zmk(0.0, 0.0);
+    return;
+  }
+  const double hss = zpx >= 0.0 ? 1.0 : -1.0;
+  const double zp = fabs(zpx);
+  const double dh = 0.5 * s;
+  const double z1 = zp + dh, z2a = zp - dh;
+  const double rhz = z2a >= 1.0e-7 ? rh / z2a : 1.0;
+  const double dk = kTwoPi * dh;
+  double sdk, cdk;
+  sincos(dk, &sdk, &cdk);
+  double hkr, hki;
+  hfk(-dk, dk, rh * kTwoPi, zp * kTwoPi, &hkr, &hki);
+  *hpk = zmk(hkr, hki);
+  if (rhz >= 1.0e-3) {
+    const double rh2 = rh * rh;
+    const double r1 = sqrt(rh2 + z1 * z1), r2 = sqrt(rh2 + z2a * z2a);
+    double s1, c1, s2, c2;
+    sincos(kTwoPi * r1, &s1, &c1);
+    sincos(kTwoPi * r2, &s2, &c2);
+    const zc ekr1 = zmk(c1, -s1), ekr2 = zmk(c2, -s2);  // exp(-j 2 pi r)
+    const zc t1 = zdivr(zmulr(ekr1, z1), r1);
+    const zc t2 = zdivr(zmulr(ekr2, z2a), r2);
+    // hps = (cdk (ekr2 - ekr1) - j sdk (t2 + t1)) hss, hpc = -sdk (ekr2 +
+    // ekr1) - j cdk (t2 - t1), both times -j / (4 pi rh)
+    const zc d = zsub(ekr2, ekr1), sm = zadd(t2, t1);
+    const zc ps = zmulr(zmk(cdk * d.re + sdk * sm.im, cdk * d.im - sdk * sm.re), hss);
+    const zc a2 = zadd(ekr2, ekr1), df = zsub(t2, t1);
+    const zc pc = zmk(-sdk * a2.re + cdk * df.im, -sdk * a2.im - cdk * df.re);
+    const double cons = -1.0 / (2.0 * kTwoPi * rh);
+    *hps = zmk(-cons * ps.im, cons * ps.re);
+    *hpc = zmk(-cons * pc.im, cons * pc.re);
+    return;
+  }
+  // near the axis beyond an end
+  const zc ekr1 = zdivr(zmk(cdk, sdk), z2a * z2a);
+  const zc ekr2 = zdivr(zmk(cdk, -sdk), z1 * z1);
+  const double t1 = kTwoPi * (1.0 / z1 - 1.0 / z2a);
+  double sz, cz;
+  sincos(kTwoPi * zp, &sz, &cz);
+  const zc t2 = zdivr(zmulr(zmk(cz, -sz), rh), kPi * 8.0);
+  const zc sps = zadd(zmk(t1, 0.0), zmulr(zadd(ekr1, ekr2), sdk));
+  *hps = zmulr(zmul(t2, sps), hss);
+  const zc dpc = zmulr(zsub(ekr1, ekr2), cdk);
+  *hpc = zmul(t2, zmk(dpc.re, dpc.im - t1));
+}
+
+// hsfld, src/nec_context.cpp:3927-4055: H at (xi, yi, zi) (observation
+// radius ai) of segment j's constant / sine / cosine currents, h[3 m + axis]
+// for m = k, s, c.  Over a ground the image segment (z and its z cosine
+// negated) is subtracted, weighted for the finite ground by the reflection
+// coefficients of the specular direction; a point on an axis (rho <= 1e-10)
+// zeroes the nine components, the image's too.
+template <bool GROUND>
+__device__ void hsfld(const DevSegments& g, int j, double xi, double yi, double zi, double ai,
+                      const GroundSpec& gs, zc h[9]) {
+  const double xij = xi - g.x[j], yij = yi - g.y[j];
+  const double cabj = g.cab[j], sabj = g.sab[j];
+#pragma unroll
+  for (int q = 0; q < 9; ++q) h[q] = zmk(0.0, 0.0);
+  for (int loop = 0; loop < (GROUND ? 2 : 1); ++loop) {
+    const double rfl = loop == 0 ? 1.0 : -1.0;
+    const double salpr = g.salp[j] * rfl;
+    const double zij = zi - rfl * g.z[j];
+    const double zp = xij * cabj + yij * sabj + zij * salpr;
+    double rhox = xij - cabj * zp, rhoy = yij - sabj * zp, rhoz = zij - salpr * zp;
+    const double rh = sqrt(rhox * rhox + rhoy * rhoy + rhoz * rhoz + ai * ai);
+    if (rh <= 1.0e-10) {
+#pragma unroll
+      for (int q = 0; q < 9; ++q) h[q] = zmk(0.0, 0.0);
+      continue;
+    }
+    rhox = rhox / rh;
+    rhoy = rhoy / rh;
+    rhoz = rhoz / rh;
+    const double phx = sabj * rhoz - salpr * rhoy;
+    const double phy = salpr * rhox - cabj * rhoz;
+    const double phz = cabj * rhoy - sabj * rhox;
+    zc hp[3];
+    hsflx(g.s[j], rh, zp, &hp[0], &hp[1], &hp[2]);
+    if (loop == 0) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        h[3 * m] = zmulr(hp[m], phx);
+        h[3 * m + 1] = zmulr(hp[m], phy);
+        h[3 * m + 2] = zmulr(hp[m], phz);
+      }
+      continue;
+    }
+    if (gs.type == 1) {
+#pragma unroll
+      for (int m = 0; m < 3; ++m) {
+        h[3 * m] = zsub(h[3 * m], zmulr(hp[m], phx));
+        h[3 * m + 1] = zsub(h[3 * m + 1], zmulr(hp[m], phy));
+        h[3 * m + 2] = zsub(h[3 * m + 2], zmulr(hp[m], phz));
+      }
+      continue;
+    }
+    // finite ground: vertical (rrv) and horizontal (rrh) reflection coefficients
+    const zc zratx = zmk(gs.zr_re, gs.zr_im);
+    const double rmag = sqrt(zp * zp + rh * rh);
+    const double xymag = sqrt(xij * xij + yij * yij);
+    double px, py, cth;
+    zc rrv;
+    if (xymag <= 1.0e-6) {
+      px = 0.;
+      py = 0.;
+      cth = 1.;
+      rrv = zmk(1.0, 0.0);
+    } else {
+      px = -yij / xymag;
+      py = xij / xymag;
+      cth = zij / rmag;
+      const zc q = zmulr(zmul(zratx, zratx), 1.0 - cth * cth);
+      rrv = zsqrt(zmk(1.0 - q.re, -q.im));
+    }
+    zc rrh = zmulr(zratx, cth);
+    rrh = zdiv(zneg(zsub(rrh, rrv)), zadd(rrh, rrv));
+    rrv = zmul(zratx, rrv);
+    rrv = zdiv(zmk(cth - rrv.re, -rrv.im), zmk(cth + rrv.re, rrv.im));
+    zc qy = zmulr(zsub(rrv, rrh), phx * px + phy * py);
+    const zc qx = zadd(zmulr(qy, px), zmulr(rrh, phx));
+    qy = zadd(zmulr(qy, py), zmulr(rrh, phy));
+    const zc qz = zmulr(rrh, phz);
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+      h[3 * m] = zsub(h[3 * m], zmul(hp[m], qx));
+      h[3 * m + 1] = zsub(h[3 * m + 1], zmul(hp[m], qy));
+      h[3 * m + 2] = zsub(h[3 * m + 2], zmul(hp[m], qz));
+    }
+  }
+}
+
+// nefld (src/nec_context.cpp:4304-4450, MAGNETIC false) or nhfld (:5189-5262,
+// MAGNETIC true) for wires: one thread per (point, source segment) evaluates
+// the nine field components of the segment's currents at the point (the
+// observation radius ax of the point's own segment or 0; for E efld with
+// ij = true, over a ground with its image) and contracts them with the
+// segment's current coefficients A, B, C; the 256 segments of a block are
+// summed in a fixed tree order, the blocks in order by near_finish_kernel.
+template <bool MAGNETIC, bool GROUND>
+__global__ void __launch_bounds__(256) near_field_kernel(DevSegments g, int n, const double* __restrict__ pts,
+                                                         const double* __restrict__ ax,
+                                                         const double* __restrict__ abc, int use_ek,
+                                                         const int* __restrict__ ekind, double rkh, GroundSpec gs,
+                                                         double* __restrict__ part) {
   __shared__ double sred[6][256];
   const int t = threadIdx.x;
   const int j = blockIdx.x * 256 + t;
   const int p = blockIdx.y;
   double v[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   if (j < n) {
-    int ind1 = 2, ind2 = 2;
-    if (use_ek) {
-      ind1 = ekind[2 * j];
-      ind2 = ekind[2 * j + 1];
+    zc e[9];
+    if constexpr (MAGNETIC) {
+      hsfld<GROUND>(g, j, pts[3 * p], pts[3 * p + 1], pts[3 * p + 2], ax[p], gs, e);
+    } else {
+      int ind1 = 2, ind2 = 2;
+      if (use_ek) {
+        ind1 = ekind[2 * j];
+        ind2 = ekind[2 * j + 1];
+      }
+      if constexpr (GROUND) {
+        point_fields_ground(g, j, pts[3 * p], pts[3 * p + 1], pts[3 * p + 2], ax[p], false, rkh, use_ek, ind1, ind2,
+                            gs, e);
+      } else {
+        cx<double> etk, ets, etc, ev[9];
+        pair_fields_v<double>(pts[3 * p], pts[3 * p + 1], pts[3 * p + 2], ax[p], 1.0, 0.0, 0.0, g.x[j], g.y[j],
+                              g.z[j], g.s[j], g.b[j], g.cab[j], g.sab[j], g.salp[j], false, rkh, use_ek, ind1, ind2,
+                              &etk, &ets, &etc, ev);
+#pragma unroll
+        for (int q = 0; q < 9; ++q) e[q] = zmk(ev[q].re, ev[q].im);
+      }
     }
-    cx<double> etk, ets, etc;
-    cx<double> e[9];
-    pair_fields_v<double>(pts[3 * p], pts[3 * p + 1], pts[3 * p + 2], ax[p], 1.0, 0.0, 0.0, g.x[j], g.y[j], g.z[j],
-                          g.s[j], g.b[j], g.cab[j], g.sab[j], g.salp[j], false, rkh, use_ek, ind1, ind2, &etk, &ets,
-                          &etc, e);
     const zc a = zmk(abc[j], abc[n + j]), b = zmk(abc[2 * n + j], abc[3 * n + j]),
              c = zmk(abc[4 * n + j], abc[5 * n + j]);
 #pragma unroll
     for (int q = 0; q < 3; ++q) {
-      const zc f = zadd(zadd(zmul(zmk(e[q].re, e[q].im), a), zmul(zmk(e[q + 3].re, e[q + 3].im), b)),
-                        zmul(zmk(e[q + 6].re, e[q + 6].im), c));
+      const zc f = zadd(zadd(zmul(e[q], a), zmul(e[q + 3], b)), zmul(e[q + 6], c));
       v[2 * q] = f.re;
       v[2 * q + 1] = f.im;
     }
@@ -1027,7 +1287,7 @@ __global__ void __launch_bounds__(256) near_e_kernel(DevSegments g, int n, const
   if (t < 6) part[((size_t)p * gridDim.x + blockIdx.x) * 6 + t] = sred[t][0];
 }
 
-__global__ void near_e_finish_kernel(const double* __restrict__ part, int npts, int nblk, double* __restrict__ out) {
+__global__ void near_finish_kernel(const double* __restrict__ part, int npts, int nblk, double* __restrict__ out) {
   const int e = blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= npts * 6) return;
   const int p = e / 6, q = e % 6;
@@ -1038,14 +1298,27 @@ __global__ void near_e_finish_kernel(const double* __restrict__ part, int npts, 
 
 }  // namespace
 
-int launch_near_e(const DevSegments& g, int n, const double* pts, const double* ax, int npts, const double* abc,
-                  int use_ek, const int* ekind, double rkh, double* part, double* out, hipStream_t stream) {
+int launch_near_field(bool magnetic, const DevSegments& g, int n, const double* pts, const double* ax, int npts,
+                      const double* abc, int use_ek, const int* ekind, double rkh, const GroundSpec& ground,
+                      double* part, double* out, hipStream_t stream) {
   if (n <= 0 || npts <= 0) return 0;
   const int nblk = (n + 255) / 256;
-  launch_k(K_FILL_FIELDS, near_e_kernel, dim3(nblk, npts), dim3(256), 0u, stream, g, n, pts, ax, abc, use_ek, ekind,
-           rkh, part);
-  launch_k(K_FILL_ASSEMBLE, near_e_finish_kernel, dim3((npts * 6 + 255) / 256), dim3(256), 0u, stream, part, npts,
-           nblk, out);
+  const dim3 grid(nblk, npts), block(256);
+  const bool gr = ground.type != 0;
+  if (magnetic && gr)
+    launch_k(K_NEAR_FIELD, near_field_kernel<true, true>, grid, block, 0u, stream, g, n, pts, ax, abc, use_ek, ekind,
+             rkh, ground, part);
+  else if (magnetic)
+    launch_k(K_NEAR_FIELD, near_field_kernel<true, false>, grid, block, 0u, stream, g, n, pts, ax, abc, use_ek,
+             ekind, rkh, ground, part);
+  else if (gr)
+    launch_k(K_NEAR_FIELD, near_field_kernel<false, true>, grid, block, 0u, stream, g, n, pts, ax, abc, use_ek,
+             ekind, rkh, ground, part);
+  else
+    launch_k(K_NEAR_FIELD, near_field_kernel<false, false>, grid, block, 0u, stream, g, n, pts, ax, abc, use_ek,
+             ekind, rkh, ground, part);
+  launch_k(K_NEAR_FIELD, near_finish_kernel, dim3((npts * 6 + 255) / 256), dim3(256), 0u, stream, part, npts, nblk,
+           out);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

@@ -37,6 +37,7 @@ enum KernelKind {
   K_ADJOINT_SOLVE,
   K_ADJOINT_REDUCE,
   K_LU_NEXT,
+  K_NEAR_FIELD,
   K_NKINDS
 };
 const char* kernel_kind_name(int kind);
@@ -148,12 +149,14 @@ int launch_batch_adjoint_fill(const BatchSolve* sol, int S, const int* blk, cons
 // With a ground each pair adds the image term of efld.  zload (2 doubles per
 // segment, or null): loads, subtracted on each loaded segment's row right
 // after its own term (cmset's order).
-// Near electric field (nefld, wires in free space) at npts points (3
-// doubles each, wavelengths), ax: per point the radius of the segment it
+// Near electric (nefld; magnetic false) or magnetic (nhfld; magnetic true)
+// field of wires, in free space or over the ground, at npts points (3
+// doubles each, wavelengths); ax: per point the radius of the segment it
 // lies on or 0; abc: the current coefficients (6 planes of n); part:
 // npts * ceil(n/256) * 6 doubles of scratch; out: 6 doubles per point.
-int launch_near_e(const DevSegments& g, int n, const double* pts, const double* ax, int npts, const double* abc,
-                  int use_ek, const int* ekind, double rkh, double* part, double* out, hipStream_t stream);
+int launch_near_field(bool magnetic, const DevSegments& g, int n, const double* pts, const double* ax, int npts,
+                      const double* abc, int use_ek, const int* ekind, double rkh, const GroundSpec& ground,
+                      double* part, double* out, hipStream_t stream);
 
 int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind,
                 const GroundSpec& ground, const double* zload, double* E, const int* colptr, const int* colj,

@@ -81,6 +81,7 @@ _SIGS = [
     ("necgpu_zgetrf_noalloc", _L, [_P, _LL, _I, _P, _P, _P]),
     ("nec_gpu_deck", _L, [_P, ctypes.c_char_p]),
     ("nec_gpu_near_field", _L, [_P, _I, _P, _L]),
+    ("nec_gpu_near_field_kind", _L, [_P, _I]),
     ("necgpu_check_mfma_f64", _L, [_P, _P, _P, _P]),
     ("necgpu_check_mfma_f32", _L, [_P, _P, _P, _P]),
     ("necgpu_zgetrs_workspace", _LL, [_I]),

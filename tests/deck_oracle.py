@@ -9,14 +9,15 @@ fields are zero) and follows nec_context's card semantics for the cards the
 GPU path runs (src/nec_context.cpp:406-955, 1031-1159):
   geometry  GW (GC for a tapered wire), GM, GS, GE
   data      FR, EX 0 (consecutive EX cards add sources), LD (consecutive LD
-            cards accumulate), GN, EK, KH, XQ, RP (mode 0), NE, PT/PQ
+            cards accumulate), GN, EK, KH, XQ, RP (mode 0), NE, NH, PT/PQ
             (printing only), EN
-A solve (fill, LU, currents) runs when XQ, RP or NE needs one and the
+A solve (fill, LU, currents) runs when XQ, RP, NE or NH needs one and the
 structure, frequency or excitation changed since the last; RP and NE after
 a solve at one frequency reuse it.  Results, in the order the reference
 adds them: `inputs` (per solve and frequency: the input impedances of the
 sources), `patterns` (per RP and frequency: the oracle's pattern dict),
-`near` (per NE: the field at the product's points, see near_points), and
+`near` (per NE or NH: the field at the product's points, see
+near_points and near_field), and
 `currents` and `freqs` beside `inputs` (the segment currents and the
 frequency of each solve).
 """
@@ -182,16 +183,17 @@ def run(text, nthreads=8):
                     res["currents"].append(pat["current"])
                     res["freqs"].append(f)
                 res["patterns"].append(pat)
-        elif name == "NE":
+        elif name in ("NE", "NH"):
             if len(freq) != 1:
-                raise ValueError("NE with one frequency only")
+                raise ValueError("NE / NH with one frequency only")
             f = freq[0]
             if f not in solved:
                 solved[f] = solve(f)
                 res["inputs"].append(solved[f]["zin"])
                 res["currents"].append(solved[f]["current"])
                 res["freqs"].append(f)
-            res["near"].append({"card": iv + rv, "abc": solved[f]["abc"], "freq": f, "rkh": rkh, "use_ek": use_ek})
+            res["near"].append({"card": iv + rv, "abc": solved[f]["abc"], "freq": f, "rkh": rkh, "use_ek": use_ek,
+                                "ground": ground, "magnetic": name == "NH"})
         else:
             raise ValueError(f"data card {name} (line {line}) not handled")
         last = name
@@ -199,9 +201,14 @@ def run(text, nthreads=8):
 
 
 def near_field(res, entry, pts_m):
-    """The oracle's near electric field of a near entry at points in metres."""
+    """The oracle's near field (E for NE, H for NH) of a near entry at
+    points in metres, over the ground of its solve."""
     g = res["geometry"]
     f = entry["freq"]
+    g.set_ground(*entry["ground"])
     g.scale(f)
     lam = po.speed_of_light() / (f * 1e6)
-    return g.near_e(entry["abc"], np.asarray(pts_m) / lam, rkh=entry["rkh"], use_ek=entry["use_ek"])
+    pts = np.asarray(pts_m) / lam
+    if entry["magnetic"]:
+        return g.near_h(entry["abc"], pts)
+    return g.near_e(entry["abc"], pts, rkh=entry["rkh"], use_ek=entry["use_ek"])

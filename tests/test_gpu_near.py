@@ -1,7 +1,8 @@
-"""Near electric fields (NE card) on the GPU against the oracle's nefld
-restatement (pinned in test_oracle_near.py): rectangular and spherical
-grids, points on the wire, near it and far away, the extended kernel, a
-grid structure with junctions, and the refusals."""
+"""Near electric (NE card) and magnetic (NH card) fields on the GPU against
+the oracle's nefld and nhfld restatements (pinned in test_oracle_near.py):
+rectangular and spherical grids, points on the wire, near it and far away,
+the extended kernel, a grid structure with junctions, free space and the
+perfect and reflection-coefficient grounds, and the refusals."""
 import numpy as np
 import pytest
 import torch  # noqa: F401  (imported at collection: the first import on a fresh box is slow)
@@ -26,14 +27,16 @@ def near(ctx, k):
     return out[:, 0:3], out[:, 3::2] + 1j * out[:, 4::2]
 
 
-def oracle_near(wires, gpflag, f, src, pts_m, use_ek=False):
+def oracle_near(wires, gpflag, f, src, pts_m, use_ek=False, magnetic=False, ground=(-1, 0.0, 0.0)):
     g = po.Geometry()
     for w in wires:
         g.wire(*w)
     g.complete(gpflag)
+    g.set_ground(*ground)
     o = g.run_voltage(f, [(src, 1.0)], use_ek=use_ek, nthreads=8)
     g.scale(f)
-    return g.near_e(o["abc"], pts_m / (C / (f * 1e6)), use_ek=use_ek)
+    pts = pts_m / (C / (f * 1e6))
+    return g.near_h(o["abc"], pts) if magnetic else g.near_e(o["abc"], pts, use_ek=use_ek)
 
 
 def check(E, Eo):
@@ -136,15 +139,70 @@ def test_near_field_at_every_frequency_of_a_sweep(then):
         api.nec_delete(ctx)
 
 
+def test_magnetic_rectangular_grid_matches_oracle():
+    # NH: x from the wire's surface outwards, z along the wire and beyond its
+    # ends (the near-axis branch of hsflx)
+    f = 299.792458
+    ctx = api.nec_create()
+    try:
+        assert api.nec_wire(ctx, *DIPOLE) == 0 and api.nec_geometry_complete(ctx, 0) == 0
+        assert api.nec_fr_card(ctx, 0, 1, f, 0.0) == 0
+        assert api.nec_ex_card(ctx, 0, 1, 11, 0, 1.0, 0.0, 0, 0, 0, 0) == 0
+        assert api.nec_nh_card(ctx, 0, 4, 1, 8, 0.0005, 0.0, -0.31, 0.02, 0.0, 0.09) == 0, api.nec_error_message()
+        assert api.nec_gpu_near_field_kind(ctx, 0) == 1
+        pts, H = near(ctx, 0)
+        assert pts.shape == (32, 3)
+        check(H, oracle_near([DIPOLE], 0, f, 11, pts, magnetic=True))
+        # then NE on the same solve: an electric result after it
+        assert api.nec_ne_card(ctx, 0, 2, 1, 2, 0.05, 0.0, 0.0, 0.05, 0.0, 0.1) == 0
+        assert api.nec_gpu_near_field_kind(ctx, 1) == 0 and api.nec_gpu_near_field_kind(ctx, 2) == -1
+        pts, E = near(ctx, 1)
+        check(E, oracle_near([DIPOLE], 0, f, 11, pts))
+    finally:
+        api.nec_delete(ctx)
+
+
+GROUNDS = {"perfect": (1, 0.0, 0.0), "finite": (0, 13.0, 0.005)}
+ABOVE = [(1, 9, -0.24, 0.05, 0.2, 0.24, -0.05, 0.2, 0.001, 1.0, 1.0),  # tilted horizontal dipole
+         (2, 7, 0.4, 0.0, 0.05, 0.4, 0.0, 0.35, 0.001, 1.0, 1.0)]      # vertical wire above the ground
+
+
+@pytest.mark.parametrize("field", ["NE", "NH"])
+@pytest.mark.parametrize("ground", list(GROUNDS))
+def test_near_fields_over_ground_match_oracle(field, ground):
+    # points above the ground, on the ground plane (z = 0) and beside both
+    # wires; the image terms of efld / hsfld, with the reflection
+    # coefficients of the finite ground
+    f = 299.792458
+    iperf, epsr, sig = GROUNDS[ground]
+    ctx = api.nec_create()
+    try:
+        for w in ABOVE:
+            assert api.nec_wire(ctx, *w) == 0
+        assert api.nec_geometry_complete(ctx, 1) == 0
+        assert api.nec_gn_card(ctx, iperf, 0, epsr, sig, 0, 0, 0, 0) == 0, api.nec_error_message()
+        assert api.nec_fr_card(ctx, 0, 1, f, 0.0) == 0
+        assert api.nec_ex_card(ctx, 0, 1, 5, 0, 1.0, 0.0, 0, 0, 0, 0) == 0
+        card = api.nec_ne_card if field == "NE" else api.nec_nh_card
+        assert card(ctx, 0, 5, 3, 4, -0.3, -0.2, 0.0, 0.17, 0.2, 0.1) == 0, api.nec_error_message()
+        assert api.nec_gpu_near_field_kind(ctx, 0) == (field == "NH")
+        pts, F = near(ctx, 0)
+        assert pts.shape == (60, 3) and np.all(pts[:15, 2] == 0.0)
+        check(F, oracle_near(ABOVE, 1, f, 5, pts, magnetic=field == "NH", ground=GROUNDS[ground]))
+    finally:
+        api.nec_delete(ctx)
+
+
 def test_near_field_refusals():
     ctx = api.nec_create()
     try:
         assert api.nec_wire(ctx, 1, 11, 0, 0, 0.1, 0, 0, 0.6, 0.001, 1, 1) == 0
         assert api.nec_geometry_complete(ctx, 0) == 0
-        assert api.nec_gn_card(ctx, 1, 0, 0, 0, 0, 0, 0, 0) == 0
-        assert api.nec_ex_card(ctx, 0, 1, 6, 0, 1.0, 0.0, 0, 0, 0, 0) == 0
-        assert api.nec_ne_card(ctx, 0, 1, 1, 1, 0.0, 0.0, 0.3, 0, 0, 0) == 1
-        assert "ground" in api.nec_error_message()
-        assert api.nec_nh_card(ctx, 0, 1, 1, 1, 0.0, 0.0, 0.3, 0, 0, 0) == 1
+        # grounds outside the GPU path: Sommerfeld, radial-wire screens
+        assert api.nec_gn_card(ctx, 2, 0, 13.0, 0.005, 0, 0, 0, 0) == 1
+        assert "Sommerfeld" in api.nec_error_message()
+        assert api.nec_gn_card(ctx, 0, 12, 13.0, 0.005, 10.0, 0.001, 0, 0) == 1
+        assert "radial" in api.nec_error_message()
+        assert api.nec_gpu_near_field_kind(ctx, 0) == -1
     finally:
         api.nec_delete(ctx)
