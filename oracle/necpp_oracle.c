@@ -110,6 +110,15 @@ struct ora_geom {
   double *zlr, *zli, *zlc;
   double* zarr;
   double wl;
+  /* networks and transmission lines (NT / TL cards): type 1 network, 2
+   * straight line, 3 crossed line; the two port segments (1-based); x: y11
+   * y12 y22 (re, im) for a network, Z0, length (m; <= 0: the distance of the
+   * port segments' centres, set at the first run), then the shunt
+   * admittances of ends 1 and 2 for a line */
+  int nnet;
+  int32_t* ntyp;
+  int64_t *nseg1, *nseg2;
+  double* nx;
 };
 
 static void* xrealloc(void* p, size_t n) {
@@ -128,6 +137,7 @@ void ora_geom_free(ora_geom* g) {
   free(g->tag); free(g->icon1); free(g->icon2);
   free(g->ldtyp); free(g->ldtag); free(g->ldtagf); free(g->ldtagt);
   free(g->zlr); free(g->zli); free(g->zlc); free(g->zarr);
+  free(g->ntyp); free(g->nseg1); free(g->nseg2); free(g->nx);
   free(g);
 }
 
@@ -2206,6 +2216,223 @@ static double now_s(void) {
   return ts.tv_sec + 1e-9 * ts.tv_nsec;
 }
 
+/* ------------------------------------------------------------------ */
+/* Networks and transmission lines                                     */
+/* ------------------------------------------------------------------ */
+
+/* tl_card / nt_card, src/nec_context.cpp:630-726: replaces the networks
+ * (count 0 clears them); a line with Z0 <= 0 is crossed (type 3, |Z0|). */
+int ora_set_networks(ora_geom* g, int count, const int32_t* type, const int64_t* seg1, const int64_t* seg2,
+                     const double* x) {
+  g->nnet = count;
+  g->ntyp = (int32_t*)xrealloc(g->ntyp, (size_t)count * sizeof(int32_t));
+  g->nseg1 = (int64_t*)xrealloc(g->nseg1, (size_t)count * sizeof(int64_t));
+  g->nseg2 = (int64_t*)xrealloc(g->nseg2, (size_t)count * sizeof(int64_t));
+  g->nx = (double*)xrealloc(g->nx, (size_t)count * 6 * sizeof(double));
+  for (int k = 0; k < count; ++k) {
+    if (seg1[k] < 1 || seg1[k] > g->n || seg2[k] < 1 || seg2[k] > g->n) {
+      set_err("network %d: port segment out of range", k + 1);
+      g->nnet = 0;
+      return 1;
+    }
+    g->ntyp[k] = type[k];
+    g->nseg1[k] = seg1[k];
+    g->nseg2[k] = seg2[k];
+    for (int q = 0; q < 6; ++q) g->nx[6 * k + q] = x[6 * k + q];
+    if (type[k] == 2 && x[6 * k] <= 0.0) {
+      g->ntyp[k] = 3;
+      g->nx[6 * k] = -x[6 * k];
+    }
+  }
+  return 0;
+}
+
+/* The short-circuit admittance parameters of network k at the current
+ * wavelength, netwk_solve_and_check src/nec_context.cpp:4660-4688: a line
+ * of Z0 and length l (beta l = 2 pi l / lambda) has y12 = j / (Z0 sin beta
+ * l), y11 = y22 = -j cot(beta l) / Z0 plus its end admittances; crossed, y12
+ * changes sign.  Lengths <= 0 were replaced by the distance of the port
+ * centres (calculate_network_data, :1210-1241). */
+static void network_y(const ora_geom* g, int k, double wl, cx* y11, cx* y12, cx* y22) {
+  const double* x = g->nx + 6 * k;
+  if (g->ntyp[k] <= 1) {
+    *y11 = mk(x[0], x[1]);
+    *y12 = mk(x[2], x[3]);
+    *y22 = mk(x[4], x[5]);
+    return;
+  }
+  const double bl = TWO_PI * x[1] / wl;
+  const double y12i = 1. / (x[0] * sin(bl));
+  const double yc = -y12i * cos(bl);
+  *y11 = mk(x[2], yc + x[3]);
+  *y12 = g->ntyp[k] == 2 ? mk(0., y12i) : mk(-0., -y12i);
+  *y22 = mk(x[4], yc + x[5]);
+}
+
+/* netwk, src/nec_context.cpp:4496-5045 (ntsol = 0 each run): the port
+ * segments that are not sources get network equations, numbered in order of
+ * appearance; ports on sources are kept as source rows (the voltage known).
+ * The network matrix M (nteq x nteq) is the structure's short-circuit
+ * admittance between the equation segments, M(j, i) = current at segment j
+ * of a unit field at segment i (one structure solve each), less y s of the
+ * networks; M v = rhnx + structure currents of the incident field, then the
+ * port fields v are subtracted from the incident field and the structure
+ * solved again.  e: the incident field on entry, the segment currents (not
+ * scaled by the wavelength) on return; abc: their coefficients.  A source
+ * on a port gets the network's current added (cur_src); netloss: the
+ * network's power loss (network_power_loss). */
+static int solve_with_networks(ora_geom* g, const double* LU, const int32_t* piv, double wl, int nsrc,
+                               const int64_t* src_seg, const double* v, double* e, double* abc, cx* cur_src,
+                               double* netloss) {
+  const int64_t n = g->n;
+  const int nn = g->nnet;
+  /* TL lengths from the port centres, once (the reference stores them) */
+  for (int k = 0; k < nn; ++k)
+    if (g->ntyp[k] >= 2 && g->nx[6 * k + 1] <= 0.0) {
+      const int64_t a = g->nseg1[k] - 1, b = g->nseg2[k] - 1;
+      const double xx = g->xs[b] - g->xs[a], yy = g->ys[b] - g->ys[a], zz = g->zs[b] - g->zs[a];
+      g->nx[6 * k + 1] = wl * sqrt(xx * xx + yy * yy + zz * zz);
+    }
+  const int ndimn = 2 * nn + nsrc;
+  int64_t* nteqa = (int64_t*)calloc((size_t)ndimn, sizeof(int64_t));
+  int64_t* ntsca = (int64_t*)calloc((size_t)ndimn, sizeof(int64_t));
+  cx* vsrc = (cx*)calloc((size_t)ndimn, sizeof(cx));
+  cx* cmn = (cx*)calloc((size_t)ndimn * ndimn, sizeof(cx)); /* fill(i, j) at i + j ndimn */
+  cx* rhnx = (cx*)calloc((size_t)ndimn, sizeof(cx));
+  double* rhs = (double*)malloc((size_t)2 * n * sizeof(double));
+  double* tmp_abc = (double*)malloc((size_t)6 * n * sizeof(double));
+  int nteq = 0, ntsc = 0, rc = 0;
+  /* the row of a port segment: a source row from the end, or an equation */
+  for (int k = 0; k < nn; ++k) {
+    cx y11, y12, y22;
+    network_y(g, k, wl, &y11, &y12, &y22);
+    int isc[2], row[2];
+    const int64_t seg[2] = {g->nseg1[k], g->nseg2[k]};
+    for (int p = 0; p < 2; ++p) {
+      isc[p] = -1;
+      for (int i = 0; i < nsrc; ++i)
+        if (seg[p] == src_seg[i]) { isc[p] = i; break; }
+      int found = -1;
+      if (isc[p] < 0) {
+        for (int i = 0; i < nteq; ++i)
+          if (nteqa[i] == seg[p]) { found = i; break; }
+        if (found < 0) { found = nteq; nteqa[nteq++] = seg[p]; }
+        row[p] = found;
+      } else {
+        for (int i = 0; i < ntsc; ++i)
+          if (ntsca[i] == seg[p]) { found = ndimn - (i + 1); break; }
+        if (found < 0) {
+          found = ndimn - (ntsc + 1);
+          ntsca[ntsc] = seg[p];
+          vsrc[ntsc] = mk(v[2 * isc[p]], v[2 * isc[p] + 1]);
+          ntsc++;
+        }
+        row[p] = found;
+      }
+    }
+    const double s1 = g->ss[seg[0] - 1], s2 = g->ss[seg[1] - 1];
+    if (isc[0] < 0) {
+      cmn[row[0] + row[0] * ndimn] = csub(cmn[row[0] + row[0] * ndimn], cmulr(y11, s1));
+      cmn[row[0] + row[1] * ndimn] = csub(cmn[row[0] + row[1] * ndimn], cmulr(y12, s1));
+    } else {
+      const cx vs = mk(v[2 * isc[0]], v[2 * isc[0] + 1]);
+      rhnx[row[0]] = cadd(rhnx[row[0]], cdivr(cmul(y11, vs), wl));
+      rhnx[row[1]] = cadd(rhnx[row[1]], cdivr(cmul(y12, vs), wl));
+    }
+    if (isc[1] < 0) {
+      cmn[row[1] + row[1] * ndimn] = csub(cmn[row[1] + row[1] * ndimn], cmulr(y22, s2));
+      cmn[row[1] + row[0] * ndimn] = csub(cmn[row[1] + row[0] * ndimn], cmulr(y12, s2));
+    } else {
+      const cx vs = mk(v[2 * isc[1]], v[2 * isc[1] + 1]);
+      rhnx[row[0]] = cadd(rhnx[row[0]], cdivr(cmul(y12, vs), wl));
+      rhnx[row[1]] = cadd(rhnx[row[1]], cdivr(cmul(y22, vs), wl));
+    }
+  }
+  /* structure admittances between the equation segments */
+  for (int i = 0; i < nteq; ++i) {
+    memset(rhs, 0, (size_t)2 * n * sizeof(double));
+    rhs[2 * (nteqa[i] - 1)] = 1.0;
+    ora_lu_solve(n, LU, piv, rhs);
+    if (ora_current_coefficients(g, rhs, tmp_abc)) { rc = 1; goto out; }
+    for (int j = 0; j < nteq; ++j) {
+      const int64_t r = nteqa[j] - 1;
+      cmn[i + j * ndimn] = cadd(cmn[i + j * ndimn], mk(rhs[2 * r], rhs[2 * r + 1]));
+    }
+  }
+  {
+    /* M(r, c) = fill(c, r), factored with partial pivoting */
+    double* M = (double*)malloc((size_t)2 * (nteq > 0 ? nteq : 1) * (nteq > 0 ? nteq : 1) * sizeof(double));
+    int32_t* mpiv = (int32_t*)malloc((size_t)(nteq > 0 ? nteq : 1) * sizeof(int32_t));
+    for (int r = 0; r < nteq; ++r)
+      for (int c = 0; c < nteq; ++c) {
+        M[2 * (r * nteq + c)] = cmn[c + r * ndimn].re;
+        M[2 * (r * nteq + c) + 1] = cmn[c + r * ndimn].im;
+      }
+    if (nteq > 0) ora_lu(nteq, M, mpiv, 1);
+    /* the incident field's structure currents at the equation segments */
+    memcpy(rhs, e, (size_t)2 * n * sizeof(double));
+    ora_lu_solve(n, LU, piv, rhs);
+    if (ora_current_coefficients(g, rhs, tmp_abc)) { free(M); free(mpiv); rc = 1; goto out; }
+    double* rhnt = (double*)calloc((size_t)2 * (nteq > 0 ? nteq : 1), sizeof(double));
+    for (int i = 0; i < nteq; ++i) {
+      const int64_t r = nteqa[i] - 1;
+      rhnt[2 * i] = rhnx[i].re + rhs[2 * r];
+      rhnt[2 * i + 1] = rhnx[i].im + rhs[2 * r + 1];
+    }
+    if (nteq > 0) ora_lu_solve(nteq, M, mpiv, rhnt);
+    for (int i = 0; i < nteq; ++i) {
+      const int64_t r = nteqa[i] - 1;
+      e[2 * r] -= rhnt[2 * i];
+      e[2 * r + 1] -= rhnt[2 * i + 1];
+    }
+    ora_lu_solve(n, LU, piv, e);
+    if (ora_current_coefficients(g, e, abc)) { free(M); free(mpiv); free(rhnt); rc = 1; goto out; }
+    /* power of the network ports (netwk_compute_currents, :4891-4918) */
+    double loss = 0.0;
+    for (int i = 0; i < nteq; ++i) {
+      const int64_t r = nteqa[i] - 1;
+      const cx volt = cmulr(cmulr(mk(rhnt[2 * i], rhnt[2 * i + 1]), g->ss[r]), wl);
+      const cx cur = cmulr(mk(e[2 * r], e[2 * r + 1]), wl);
+      loss -= 0.5 * cmul(volt, mk(cur.re, -cur.im)).re;
+    }
+    for (int i = 0; i < ntsc; ++i) {
+      const int64_t r = ntsca[i] - 1;
+      const cx cur = cmulr(mk(e[2 * r], e[2 * r + 1]), wl);
+      loss -= 0.5 * cmul(vsrc[i], mk(cur.re, -cur.im)).re;
+    }
+    /* sources on ports: the network's current joins the structure's
+     * (netwk_compute_inputs, :4954-4985) */
+    for (int s = 0; s < nsrc; ++s) {
+      const int64_t r = src_seg[s] - 1;
+      cx cur = mk(e[2 * r], e[2 * r + 1]);
+      if (ntsc != 0) {
+        int j = -1;
+        for (int q = 0; q < ntsc; ++q)
+          if (ntsca[q] == src_seg[s]) { j = q; break; }
+        if (j < 0) {
+          set_err("NETWORK DATA ERROR: SOURCE SEGMENT [%lld] NOT FOUND IN NETWORK TABLE", (long long)src_seg[s]);
+          free(M); free(mpiv); free(rhnt);
+          rc = 1;
+          goto out;
+        }
+        const int rowi = ndimn - (j + 1);
+        cx t = rhnx[rowi];
+        for (int k = 0; k < nteq; ++k) t = csub(t, cmul(cmn[k + rowi * ndimn], mk(rhnt[2 * k], rhnt[2 * k + 1])));
+        cur = cadd(cur, t);
+        const cx c = cmulr(cur, wl);
+        const cx vs = mk(v[2 * s], v[2 * s + 1]);
+        loss += 0.5 * cmul(vs, mk(c.re, -c.im)).re;
+      }
+      cur_src[s] = cur;
+    }
+    *netloss = loss;
+    free(M); free(mpiv); free(rhnt);
+  }
+out:
+  free(nteqa); free(ntsca); free(vsrc); free(cmn); free(rhnx); free(rhs); free(tmp_abc);
+  return rc;
+}
+
 /* One voltage-source run at one frequency: simulate() states
  * MEMORY_ALLOC -> STRUCTURE_LOADING -> EXCITATION (src/nec_context.cpp:1031-1159,
  * :1523-1558, :1622-1791, :3193-3210, :4850-5047). */
@@ -2232,14 +2459,24 @@ int ora_run_voltage(ora_geom* g, double freq_mhz, double rkh, int use_ek,
     e[2 * k] = val.re;
     e[2 * k + 1] = val.im;
   }
-  ora_lu_solve(n, Z, piv, e);
   double* abc = abc_out ? abc_out : (double*)malloc((size_t)6 * n * sizeof(double));
-  if (ora_current_coefficients(g, e, abc)) { free(Z); free(piv); free(e); return 1; }
+  cx* cur_src = (cx*)calloc((size_t)(nsrc > 0 ? nsrc : 1), sizeof(cx));
+  double netloss = 0.0;
+  if (g->nnet > 0) {
+    if (solve_with_networks(g, Z, piv, wl, nsrc, src_seg, v, e, abc, cur_src, &netloss)) {
+      free(Z); free(piv); free(e); free(cur_src);
+      if (!abc_out) free(abc);
+      return 1;
+    }
+  } else {
+    ora_lu_solve(n, Z, piv, e);
+    if (ora_current_coefficients(g, e, abc)) { free(Z); free(piv); free(e); free(cur_src); return 1; }
+    for (int s = 0; s < nsrc; s++) cur_src[s] = mk(e[2 * (src_seg[s] - 1)], e[2 * (src_seg[s] - 1) + 1]);
+  }
   double pin = 0.0;
   for (int s = 0; s < nsrc; s++) {
-    int64_t k = src_seg[s] - 1;
     cx volt = mk(v[2 * s], v[2 * s + 1]);
-    cx cur = cmulr(mk(e[2 * k], e[2 * k + 1]), wl);
+    cx cur = cmulr(cur_src[s], wl);
     cx z = cdiv(volt, cur);
     if (zin) { zin[2 * s] = z.re; zin[2 * s + 1] = z.im; }
     double p = 0.5 * cmul(volt, mk(cur.re, -cur.im)).re;
@@ -2259,7 +2496,8 @@ int ora_run_voltage(ora_geom* g, double freq_mhz, double rkh, int use_ek,
         double cm = hypot(e[2 * i] * wl, e[2 * i + 1] * wl);
         ploss += 0.5 * cm * cm * g->zarr[2 * i] * g->ss[i];
       }
-  if (rp && rp_out) ora_pattern(g, abc, wl, pin, ploss, rp, rp_out, nthreads);
+  if (rp && rp_out) ora_pattern(g, abc, wl, pin, ploss + netloss, rp, rp_out, nthreads);
+  free(cur_src);
   double t4 = now_s();
   if (timing) { timing->fill_s = t1 - t0; timing->factor_s = t2 - t1; timing->solve_s = t3 - t2; timing->pattern_s = t4 - t3; }
   if (!abc_out) free(abc);

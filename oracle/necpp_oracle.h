@@ -229,6 +229,15 @@ typedef struct {
   double fill_s, factor_s, solve_s, pattern_s;
 } ora_timing;
 
+/* tl_card / nt_card (src/nec_context.cpp:630-726): replaces the networks
+ * of the structure (count 0 clears them).  type 1 network (x: y11, y12, y22
+ * as re, im), 2 transmission line (x: Z0 (<= 0: crossed), length in m (<=
+ * 0: the distance of the port segments' centres), shunt admittance of end 1
+ * and end 2 (re, im)); seg1, seg2 the 1-based port segments.  ora_run_voltage
+ * then solves with them (netwk, :4496-5045). */
+int ora_set_networks(ora_geom* g, int count, const int32_t* type, const int64_t* seg1, const int64_t* seg2,
+                     const double* x);
+
 int ora_run_voltage(ora_geom* g, double freq_mhz, double rkh, int use_ek,
                     int nsrc, const int64_t* src_seg, const double* v,
                     const ora_rp_spec* rp, ora_rp_out* rp_out,

@@ -64,6 +64,8 @@ def lib():
         L.ora_near_e.restype = i
         L.ora_near_h.argtypes = [p, p, i64, p, p]
         L.ora_near_h.restype = i
+        L.ora_set_networks.argtypes = [p, i, p, p, p, p]
+        L.ora_set_networks.restype = i
         L.ora_efld.argtypes = [d] * 12 + [i, d, i, i, i, p]
         L.ora_efld.restype = d
         _lib = L
@@ -221,6 +223,16 @@ class Geometry:
         ints = [np.ascontiguousarray(c, np.int32) for c in cols[:4]]
         dbls = [np.ascontiguousarray(c, np.float64) for c in cols[4:]]
         if self._L.ora_set_loads(self._g, k, *[_ptr(a) for a in ints + dbls]) != 0:
+            raise RuntimeError(self._L.ora_last_error().decode())
+
+    def set_networks(self, nets):
+        """TL / NT cards: a list of (type, seg1, seg2, x6) with type "NT" or
+        "TL", 1-based port segments and the card's six numbers."""
+        typ = np.array([1 if t == "NT" else 2 for t, _, _, _ in nets], np.int32)
+        s1 = np.array([a for _, a, _, _ in nets], np.int64)
+        s2 = np.array([b for _, _, b, _ in nets], np.int64)
+        x = np.array([list(v) for _, _, _, v in nets], np.float64).reshape(-1)
+        if self._L.ora_set_networks(self._g, len(nets), _ptr(typ), _ptr(s1), _ptr(s2), _ptr(x)) != 0:
             raise RuntimeError(self._L.ora_last_error().decode())
 
     def scale(self, f_mhz):
