@@ -84,7 +84,8 @@ long nec_excitation_current(nec_context* in_context, double x, double y, double 
 /* src/libnecpp.h:384 (plane wave; not on the GPU path) */
 long nec_excitation_planewave(nec_context* in_context, int n_theta, int n_phi, double theta, double phi,
                               double eta, double dtheta, double dphi, double pol_ratio);
-/* src/libnecpp.h:387-388 (TL / NT; not on the GPU path) */
+/* src/libnecpp.h:387-388 -> nec_context::tl_card / nt_card (src/nec_context.cpp:630-726); the network equations
+ * (netwk, :4496-5045) are solved on the host around GPU structure solves */
 long nec_tl_card(nec_context* in_context, int itmp1, int itmp2, int itmp3, int itmp4, double tmp1,
                  double tmp2, double tmp3, double tmp4, double tmp5, double tmp6);
 long nec_nt_card(nec_context* in_context, int itmp1, int itmp2, int itmp3, int itmp4, double tmp1,

@@ -119,11 +119,13 @@ long nec_excitation_planewave(nec_context*, int, int, double, double, double, do
   return unsupported("EX 1");
 }
 
-long nec_tl_card(nec_context*, int, int, int, int, double, double, double, double, double, double) {
-  return unsupported("TL");
+long nec_tl_card(nec_context* ctx, int itmp1, int itmp2, int itmp3, int itmp4, double tmp1, double tmp2, double tmp3,
+                 double tmp4, double tmp5, double tmp6) {
+  return guarded([&] { ctx->tl_card(itmp1, itmp2, itmp3, itmp4, tmp1, tmp2, tmp3, tmp4, tmp5, tmp6); });
 }
-long nec_nt_card(nec_context*, int, int, int, int, double, double, double, double, double, double) {
-  return unsupported("NT");
+long nec_nt_card(nec_context* ctx, int itmp1, int itmp2, int itmp3, int itmp4, double tmp1, double tmp2, double tmp3,
+                 double tmp4, double tmp5, double tmp6) {
+  return guarded([&] { ctx->nt_card(itmp1, itmp2, itmp3, itmp4, tmp1, tmp2, tmp3, tmp4, tmp5, tmp6); });
 }
 
 long nec_xq_card(nec_context* ctx, int itmp1) {

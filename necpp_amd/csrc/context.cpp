@@ -22,6 +22,7 @@ void Context::calc_prepare() {
   use_ek_ = false;
   gn_iperf_ = -1;  // ground.default_values(): free space
   loads_.clear();
+  networks_.clear();
   ifar_ = -1;
   freq_mhz_ = em_speed_of_light() / 1.0e6;
   nfrq_ = 1;
@@ -86,6 +87,44 @@ void Context::ex_card(int type, int i2, int i3, int i4, double f1, double f2, do
   if (std::hypot(v.re, v.im) < 1.e-20) v = zmk(1.0, 0.0);
   src_seg_.push_back(seg);
   src_v_.push_back(v);
+}
+
+void Context::network_card(int type, int i1, int i2, int i3, int i4, const double* x) {
+  // tl_card / nt_card, src/nec_context.cpp:630-726: consecutive TL / NT
+  // cards accumulate; the first after any other card starts a new list, and
+  // with a port-1 segment of -1 only clears it
+  if (!geo_.is_complete()) throw NecError(std::string(type == 1 ? "NT" : "TL") + ": geometry is not complete");
+  if (iflow_ != 6) {
+    networks_.clear();
+    iflow_ = 6;
+    reset_to_excitation_setup();
+    if (i2 == -1) return;
+  }
+  Network net;
+  net.type = type;
+  net.seg1 = geo_.segment_number(i1, i2);
+  net.seg2 = geo_.segment_number(i3, i4);
+  if (net.seg1 < 1 || net.seg1 > geo_.size() || net.seg2 < 1 || net.seg2 > geo_.size())
+    throw NecError("NETWORK DATA ERROR: PORT SEGMENT OUT OF RANGE");
+  for (int q = 0; q < 6; ++q) net.x[q] = x[q];
+  if (type == 2 && x[0] <= 0.0) {
+    // a negative characteristic impedance: the crossed line
+    net.type = 3;
+    net.x[0] = -x[0];
+  }
+  networks_.push_back(net);
+}
+
+void Context::tl_card(int i1, int i2, int i3, int i4, double z0, double len, double y1r, double y1i, double y2r,
+                      double y2i) {
+  const double x[6] = {z0, len, y1r, y1i, y2r, y2i};
+  network_card(2, i1, i2, i3, i4, x);
+}
+
+void Context::nt_card(int i1, int i2, int i3, int i4, double y11r, double y11i, double y12r, double y12i,
+                      double y22r, double y22i) {
+  const double x[6] = {y11r, y11i, y12r, y12i, y22r, y22i};
+  network_card(1, i1, i2, i3, i4, x);
 }
 
 void Context::kh_card(double rkh) {
@@ -500,7 +539,14 @@ void Context::excitation(int mhz) {
     e[k] = zdivr(zneg(src_v_[s]), seg.s[k] * wavelength_);
   }
   std::vector<zc> curx;
-  device().solve(e, &curx, &abc_);
+  std::vector<zc> src_cur(src_seg_.size());
+  network_loss_ = 0.0;
+  if (networks_.empty()) {
+    device().solve(e, &curx, &abc_);
+    for (size_t s = 0; s < src_seg_.size(); ++s) src_cur[s] = curx[src_seg_[s] - 1];
+  } else {
+    solve_with_networks(seg, e, &curx, &src_cur);
+  }
   times_.solve_ms = device().times().solve_ms;
   if (src_seg_.empty()) return;  // the reference records no antenna input without sources
   AntennaInput in;
@@ -509,7 +555,7 @@ void Context::excitation(int mhz) {
   for (size_t s = 0; s < src_seg_.size(); ++s) {
     const int64_t k = src_seg_[s] - 1;
     const zc volt = src_v_[s];
-    const zc cur = zmulr(curx[k], wavelength_);
+    const zc cur = zmulr(src_cur[s], wavelength_);
     const zc adm = zdiv(cur, volt);
     const zc imp = zdiv(volt, cur);
     const double power = 0.5 * zmul(volt, zconj(cur)).re;
@@ -537,11 +583,216 @@ void Context::excitation(int mhz) {
   inputs_.push_back(std::move(in));
 }
 
+namespace {
+
+// In-place partial-pivot LU of the row-major complex m x m matrix A (pivot:
+// the first largest |z|, Eigen's PartialPivLU rule) and the solve with it:
+// the network equations are a few dozen at most, so they stay on the host,
+// as the reference keeps them (lu_decompose / solve, src/matrix_algebra.cpp
+// :217-307).
+void small_lu(std::vector<zc>& A, int m, std::vector<int>& piv) {
+  piv.assign(m, 0);
+  for (int k = 0; k < m; ++k) {
+    int p = k;
+    double best = std::hypot(A[(size_t)k * m + k].re, A[(size_t)k * m + k].im);
+    for (int i = k + 1; i < m; ++i) {
+      const double a = std::hypot(A[(size_t)i * m + k].re, A[(size_t)i * m + k].im);
+      if (a > best) {
+        best = a;
+        p = i;
+      }
+    }
+    piv[k] = p;
+    if (p != k)
+      for (int c = 0; c < m; ++c) std::swap(A[(size_t)k * m + c], A[(size_t)p * m + c]);
+    const zc d = A[(size_t)k * m + k];
+    if (d.re == 0.0 && d.im == 0.0) throw NecError("NETWORK DATA ERROR: SINGULAR NETWORK EQUATIONS");
+    for (int i = k + 1; i < m; ++i) {
+      const zc l = zdiv(A[(size_t)i * m + k], d);
+      A[(size_t)i * m + k] = l;
+      for (int c = k + 1; c < m; ++c) A[(size_t)i * m + c] = zsub(A[(size_t)i * m + c], zmul(l, A[(size_t)k * m + c]));
+    }
+  }
+}
+
+void small_solve(const std::vector<zc>& LU, int m, const std::vector<int>& piv, std::vector<zc>& b) {
+  for (int k = 0; k < m; ++k)
+    if (piv[k] != k) std::swap(b[k], b[piv[k]]);
+  for (int i = 1; i < m; ++i)
+    for (int k = 0; k < i; ++k) b[i] = zsub(b[i], zmul(LU[(size_t)i * m + k], b[k]));
+  for (int i = m - 1; i >= 0; --i) {
+    for (int k = i + 1; k < m; ++k) b[i] = zsub(b[i], zmul(LU[(size_t)i * m + k], b[k]));
+    b[i] = zdiv(b[i], LU[(size_t)i * m + i]);
+  }
+}
+
+}  // namespace
+
+void Context::solve_with_networks(const ScaledSegments& seg, std::vector<zc>& e, std::vector<zc>* curx,
+                                  std::vector<zc>* src_cur) {
+  // netwk, src/nec_context.cpp:4496-5045, with the structure solves on the
+  // GPU.  Port segments that are not sources get network equations, in
+  // order of appearance; ports on sources become source rows (numbered from
+  // the end, their voltage known).  M(j, i) = current at equation segment j
+  // of a unit field at equation segment i, less y s of the networks; M v =
+  // rhnx + the incident field's currents, then the port fields v are taken
+  // from the incident field and the structure solved again.
+  const int n = static_cast<int>(geo_.size());
+  const double wl = wavelength_;
+  // lines of length <= 0: the distance of the port centres, set once
+  // (calculate_network_data, :1210-1241)
+  for (Network& net : networks_)
+    if (net.type >= 2 && net.x[1] <= 0.0) {
+      const int64_t a = net.seg1 - 1, b = net.seg2 - 1;
+      const double xx = seg.x[b] - seg.x[a], yy = seg.y[b] - seg.y[a], zz = seg.z[b] - seg.z[a];
+      net.x[1] = wl * std::sqrt(xx * xx + yy * yy + zz * zz);
+    }
+  const int nsrc = static_cast<int>(src_seg_.size());
+  const int ndimn = 2 * static_cast<int>(networks_.size()) + nsrc;
+  std::vector<int64_t> nteqa, ntsca;
+  std::vector<zc> vsrc;
+  std::vector<zc> cmn((size_t)ndimn * ndimn, zmk(0.0, 0.0));  // fill(i, j) at i + j ndimn
+  std::vector<zc> rhnx(ndimn, zmk(0.0, 0.0));
+  for (const Network& net : networks_) {
+    // short-circuit admittance parameters (:4660-4688)
+    zc y11, y12, y22;
+    if (net.type <= 1) {
+      y11 = zmk(net.x[0], net.x[1]);
+      y12 = zmk(net.x[2], net.x[3]);
+      y22 = zmk(net.x[4], net.x[5]);
+    } else {
+      const double bl = (2.0 * kPi) * net.x[1] / wl;
+      const double y12i = 1. / (net.x[0] * std::sin(bl));
+      const double yc = -y12i * std::cos(bl);
+      y11 = zmk(net.x[2], yc + net.x[3]);
+      y12 = net.type == 2 ? zmk(0., y12i) : zmk(-0., -y12i);
+      y22 = zmk(net.x[4], yc + net.x[5]);
+    }
+    const int64_t sp[2] = {net.seg1, net.seg2};
+    int isc[2], row[2];
+    for (int p = 0; p < 2; ++p) {
+      isc[p] = -1;
+      for (int i = 0; i < nsrc; ++i)
+        if (sp[p] == src_seg_[i]) {
+          isc[p] = i;
+          break;
+        }
+      int found = -1;
+      if (isc[p] < 0) {
+        for (size_t i = 0; i < nteqa.size(); ++i)
+          if (nteqa[i] == sp[p]) {
+            found = static_cast<int>(i);
+            break;
+          }
+        if (found < 0) {
+          found = static_cast<int>(nteqa.size());
+          nteqa.push_back(sp[p]);
+        }
+      } else {
+        for (size_t i = 0; i < ntsca.size(); ++i)
+          if (ntsca[i] == sp[p]) {
+            found = ndimn - static_cast<int>(i + 1);
+            break;
+          }
+        if (found < 0) {
+          found = ndimn - static_cast<int>(ntsca.size() + 1);
+          ntsca.push_back(sp[p]);
+          vsrc.push_back(src_v_[isc[p]]);
+        }
+      }
+      row[p] = found;
+    }
+    const double s1 = seg.s[sp[0] - 1], s2 = seg.s[sp[1] - 1];
+    auto at = [&](int i, int j) -> zc& { return cmn[(size_t)i + (size_t)j * ndimn]; };
+    if (isc[0] < 0) {
+      at(row[0], row[0]) = zsub(at(row[0], row[0]), zmulr(y11, s1));
+      at(row[0], row[1]) = zsub(at(row[0], row[1]), zmulr(y12, s1));
+    } else {
+      const zc v = src_v_[isc[0]];
+      rhnx[row[0]] = zadd(rhnx[row[0]], zdivr(zmul(y11, v), wl));
+      rhnx[row[1]] = zadd(rhnx[row[1]], zdivr(zmul(y12, v), wl));
+    }
+    if (isc[1] < 0) {
+      at(row[1], row[1]) = zsub(at(row[1], row[1]), zmulr(y22, s2));
+      at(row[1], row[0]) = zsub(at(row[1], row[0]), zmulr(y12, s2));
+    } else {
+      const zc v = src_v_[isc[1]];
+      rhnx[row[0]] = zadd(rhnx[row[0]], zdivr(zmul(y12, v), wl));
+      rhnx[row[1]] = zadd(rhnx[row[1]], zdivr(zmul(y22, v), wl));
+    }
+  }
+  const int nteq = static_cast<int>(nteqa.size());
+  const int ntsc = static_cast<int>(ntsca.size());
+  DeviceSolver& d = device();
+  std::vector<double> abc_tmp;
+  // the structure's admittances between the equation segments: one GPU
+  // solve per equation segment
+  for (int i = 0; i < nteq; ++i) {
+    std::vector<zc> unit(n, zmk(0.0, 0.0)), c;
+    unit[nteqa[i] - 1] = zmk(1.0, 0.0);
+    d.solve(unit, &c, &abc_tmp);
+    for (int j = 0; j < nteq; ++j)
+      cmn[(size_t)i + (size_t)j * ndimn] = zadd(cmn[(size_t)i + (size_t)j * ndimn], c[nteqa[j] - 1]);
+  }
+  // M(r, c) = fill(c, r)
+  std::vector<zc> M((size_t)nteq * nteq);
+  for (int r = 0; r < nteq; ++r)
+    for (int c = 0; c < nteq; ++c) M[(size_t)r * nteq + c] = cmn[(size_t)c + (size_t)r * ndimn];
+  std::vector<int> mpiv;
+  if (nteq > 0) small_lu(M, nteq, mpiv);
+  // the incident field's currents at the equation segments, the network
+  // equations, the port fields off the incident field, the final solve
+  std::vector<zc> c0;
+  d.solve(e, &c0, &abc_tmp);
+  std::vector<zc> rhnt(nteq);
+  for (int i = 0; i < nteq; ++i) rhnt[i] = zadd(rhnx[i], c0[nteqa[i] - 1]);
+  if (nteq > 0) small_solve(M, nteq, mpiv, rhnt);
+  for (int i = 0; i < nteq; ++i) e[nteqa[i] - 1] = zsub(e[nteqa[i] - 1], rhnt[i]);
+  d.solve(e, curx, &abc_);
+  const std::vector<zc>& cur = *curx;
+  // the power the networks take (netwk_compute_currents, :4891-4918)
+  double loss = 0.0;
+  for (int i = 0; i < nteq; ++i) {
+    const int64_t r = nteqa[i] - 1;
+    const zc volt = zmulr(zmulr(rhnt[i], seg.s[r]), wl);
+    const zc c = zmulr(cur[r], wl);
+    loss -= 0.5 * zmul(volt, zconj(c)).re;
+  }
+  for (int i = 0; i < ntsc; ++i) {
+    const zc c = zmulr(cur[ntsca[i] - 1], wl);
+    loss -= 0.5 * zmul(vsrc[i], zconj(c)).re;
+  }
+  // a source on a port: the network's current joins the structure's
+  // (netwk_compute_inputs, :4954-4985)
+  for (int s = 0; s < nsrc; ++s) {
+    zc c = cur[src_seg_[s] - 1];
+    if (ntsc != 0) {
+      int j = -1;
+      for (int q = 0; q < ntsc; ++q)
+        if (ntsca[q] == src_seg_[s]) {
+          j = q;
+          break;
+        }
+      if (j < 0)
+        throw NecError("NETWORK DATA ERROR: SOURCE SEGMENT [" + std::to_string(src_seg_[s]) +
+                       "] NOT FOUND IN NETWORK TABLE");
+      const int rowi = ndimn - (j + 1);
+      zc t = rhnx[rowi];
+      for (int k = 0; k < nteq; ++k) t = zsub(t, zmul(cmn[(size_t)k + (size_t)rowi * ndimn], rhnt[k]));
+      c = zadd(c, t);
+      const zc ci = zmulr(c, wl);
+      loss += 0.5 * zmul(src_v_[s], zconj(ci)).re;
+    }
+    (*src_cur)[s] = c;
+  }
+  network_loss_ = loss;
+}
+
 PatternSpec Context::pattern_spec(std::vector<double>* trig_out) const {
   // gain constants and angle tables of nec_radiation_pattern::analyze
   // (src/nec_radiation_pattern.cpp:221-262)
   const double eta = em_impedance();
-  const double pinr = input_power_, pnlr = 0.0;
+  const double pinr = input_power_, pnlr = network_loss_;
   PatternSpec sp;
   sp.n_theta = nth_;
   sp.n_phi = nph_;

@@ -64,6 +64,10 @@ class Context {
   void ld_card(int type, int tag, int tagf, int tagt, double zlr, double zli, double zlc);
   // NE card (src/nec_context.cpp:896-955): near electric field on a
   // rectangular (near = 0) or spherical (near = 1) grid of points
+  void tl_card(int i1, int i2, int i3, int i4, double z0, double len, double y1r, double y1i, double y2r,
+               double y2i);
+  void nt_card(int i1, int i2, int i3, int i4, double y11r, double y11i, double y12r, double y12i, double y22r,
+               double y22i);
   void ne_card(int near, int nrx, int nry, int nrz, double xnr, double ynr, double znr, double dxnr, double dynr,
                double dznr);
   void nh_card(int near, int nrx, int nry, int nrz, double xnr, double ynr, double znr, double dxnr, double dynr,
@@ -161,6 +165,21 @@ class Context {
   // voltage sources (EX 0)
   std::vector<int64_t> src_seg_;
   std::vector<zc> src_v_;
+  // networks and transmission lines (NT / TL cards, src/nec_context.cpp
+  // :630-726): type 1 network, 2 straight line, 3 crossed line; 1-based
+  // port segments; x as the card gives it (a line's length <= 0 is replaced
+  // by the distance of the port centres at the first solve)
+  struct Network {
+    int type;
+    int64_t seg1, seg2;
+    double x[6];
+  };
+  std::vector<Network> networks_;
+  double network_loss_ = 0.0;  // network_power_loss of the last solve (W)
+  void network_card(int type, int i1, int i2, int i3, int i4, const double* x);
+  // the solve of the structure with its networks (netwk), see excitation
+  void solve_with_networks(const ScaledSegments& seg, std::vector<zc>& e, std::vector<zc>* curx,
+                           std::vector<zc>* src_current);
   // RP card
   int ifar_ = -1;
   int nth_ = 0, nph_ = 0, rp_format_ = 0, rp_norm_ = 0, ipd_ = 0, iavp_ = 0;

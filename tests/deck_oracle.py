@@ -9,8 +9,9 @@ fields are zero) and follows nec_context's card semantics for the cards the
 GPU path runs (src/nec_context.cpp:406-955, 1031-1159):
   geometry  GW (GC for a tapered wire), GM, GS, GE
   data      FR, EX 0 (consecutive EX cards add sources), LD (consecutive LD
-            cards accumulate), GN, EK, KH, XQ, RP (mode 0), NE, NH, PT/PQ
-            (printing only), EN
+            cards accumulate), GN, EK, KH, XQ, RP (mode 0), NE, NH, TL / NT
+            (consecutive cards accumulate; a port I2 of -1 on the first clears
+            them), PT/PQ (printing only), EN
 A solve (fill, LU, currents) runs when XQ, RP, NE or NH needs one and the
 structure, frequency or excitation changed since the last; RP and NE after
 a solve at one frequency reuse it.  Results, in the order the reference
@@ -106,7 +107,7 @@ def run(text, nthreads=8):
             raise ValueError(f"geometry card {name} not handled")
     res = {"inputs": [], "currents": [], "freqs": [], "patterns": [], "near": [], "geometry": g}
     freq = [po.speed_of_light() / 1.0e6]  # nec_context's default frequency (src/nec_context.cpp:158)
-    src, loads = [], []
+    src, loads, nets = [], [], []
     last = None
     use_ek, rkh, ground = False, 1.0, (-1, 0.0, 0.0)
     solved = {}  # frequency -> run_voltage result of the current structure and excitation
@@ -114,6 +115,7 @@ def run(text, nthreads=8):
     def solve(f, rp=None):
         g.set_ground(*ground)
         g.set_loads(loads)
+        g.set_networks(nets)
         segs = [(g.segment_number(t, s), v) for t, s, v in src]
         r = g.run_voltage(f, segs, rkh=rkh, use_ek=use_ek, rp=rp, nthreads=nthreads)
         r["freq"] = f
@@ -145,6 +147,16 @@ def run(text, nthreads=8):
                 loads = []
             else:
                 loads.append((iv[0], iv[1], iv[2], iv[3], rv[0], rv[1], rv[2]))
+            solved = {}
+        elif name in ("TL", "NT"):
+            # src/nec_context.cpp:630-726
+            if last not in ("TL", "NT"):
+                nets = []
+                if iv[1] == -1:
+                    last = name
+                    solved = {}
+                    continue
+            nets.append((name, g.segment_number(iv[0], iv[1]), g.segment_number(iv[2], iv[3]), tuple(rv)))
             solved = {}
         elif name == "GN":
             ground = (iv[0], rv[0], rv[1]) if iv[0] != -1 else (-1, 0.0, 0.0)

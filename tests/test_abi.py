@@ -108,8 +108,10 @@ def test_error_conventions():
         assert api.nec_ex_card(ctx, 0, 0, 9, 0, 1.0, 0, 0, 0, 0, 0) == 1
         assert "TOO LARGE" in api.nec_error_message()
         # cards outside the GPU path fail loudly
-        assert api.nec_tl_card(ctx, 1, 1, 1, 5, 50.0, 0, 0, 0, 0, 0) == 1
+        assert api.nec_cp_card(ctx, 1, 1, 1, 5) == 1
         assert "not supported" in api.nec_error_message()
+        # TL on the path: a port segment that does not exist is a data fault
+        assert api.nec_tl_card(ctx, 1, 9, 1, 5, 50.0, 0, 0, 0, 0, 0) == 1
         # LD runs on the GPU path; a bad tag range is a data fault
         assert api.nec_ld_card(ctx, 0, 0, 1, 1, 10.0, 0, 0) == 0
         assert api.nec_ld_card(ctx, 0, 0, 3, 2, 10.0, 0, 0) == 1

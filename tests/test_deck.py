@@ -88,7 +88,8 @@ def test_line_rules():
     ("CE\nGW 1 5 0 0 -1 0 0 1 0\nGE 0\n", "GEOMETRY DATA CARD ERROR"),
     ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\n", "no GE card"),
     ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nQQ 1\n", "FAULTY DATA CARD LABEL"),
-    ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nTL 1 3 1 3 50\n", "TL: not supported"),
+    ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nCP 1 3 1 3\n", "CP: not supported"),
+    ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nTL 1 3 2 3 50\n", "ITAG"),
     ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nEX 0 7 1 0 1\n", "ITAG"),
     ("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nGN 2 0 0 0 10 0.01\n", "GN"),
 ])
@@ -103,9 +104,9 @@ def test_deck_errors(text, needle):
 
 
 def test_error_names_the_deck_line():
-    ctx, rc = run("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nFR 0 1 0 0 300\nTL 1 3 1 3 50\n")
+    ctx, rc = run("CE\nGW 1 5 0 0 -1 0 0 1 0.01\nGE 0\nFR 0 1 0 0 300\nCP 1 3 1 3\n")
     try:
-        assert rc == 1 and "deck line 5 (TL)" in api.nec_error_message()
+        assert rc == 1 and "deck line 5 (CP)" in api.nec_error_message()
     finally:
         api.nec_delete(ctx)
 

@@ -9,8 +9,7 @@ Decks of the testharness left out, and why: GN 2 (Sommerfeld ground:
 36dip, Collinear_1L, buoy, salt_ground, bruce_sommerfeld, sommerfeld2/3,
 sommerfield1, gn2), EX types 1 and 5 (plane wave / current-slope source:
 example2, passive_test1, ga_pjw_0, plane_wave_excitation), RP mode 1
-(test299), patches (SP/SC/SM), GX/GR/GA/GH, TL/NT, CP, MP, PL (not on
-the path); hang.nec (segments as long as the wire radius and two
+(test299), patches (SP/SC/SM), GX/GR/GA/GH, CP, MP, PL (not on the path); hang.nec (segments as long as the wire radius and two
 overlapping wires: the adaptive quadrature subdivides to its limit, in the
 reference and in the oracle alike); geometry-only decks (discone, plane,
 plet_helix).
@@ -44,9 +43,9 @@ from necpp_amd import necpp as api
 pytestmark = pytest.mark.gpu
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-DECKS = ["GA487", "dipole_anim", "example1", "excessive_gain", "ga_pjw_1", "gx_three_planes_control",
+DECKS = ["GA487", "dipole_anim", "example1", "example5", "excessive_gain", "ga_pjw_1", "gx_three_planes_control",
          "herzian_dipole", "intersection", "near_field_electric", "near_field_magnetic", "patch_999", "patch_999_2",
-         "plet_helixumts"]
+         "plet_helixumts", "yagi"]
 
 
 def amplitude_map(geo):
