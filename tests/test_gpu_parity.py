@@ -16,6 +16,7 @@ import torch  # noqa: F401  (imported at collection: the first import on a fresh
 from necpp_amd import decks
 from necpp_amd import necpp as api
 from oracle import pyoracle as po
+from gpu_compare import check_stats, compare_pattern
 
 pytestmark = pytest.mark.gpu
 
@@ -260,22 +261,9 @@ def test_c1_dipole_gpu(kats):
         assert np.linalg.norm(cur - o["current"]) <= TOL * np.linalg.norm(o["current"])
         assert abs(api.nec_gain(ctx, 0, 90, 0) - k["gain_theta90_phi0"]) < 1e-9
         assert abs(api.nec_gain_max(ctx, 0) - k["gain_max"]) < 1e-9
-        npt = 181 * 361
-        for name in ("gain", "pg_vert", "pg_horiz", "pg_tot", "pg_rhcp", "pg_lhcp"):
-            a = api.gpu_pattern_array(ctx, 0, name, npt)
-            b = o["pattern"][name]
-            finite = b > -999
-            assert np.array_equal(a > -999, finite), name
-            # dB values: absolute agreement; deep nulls amplify relative field errors
-            if finite.any():
-                assert np.max(np.abs(a[finite] - b[finite])) < 1e-8, name
-        for name in ("e_theta", "e_phi"):
-            a = api.gpu_pattern_array(ctx, 0, name, npt)
-            b = o["pattern"][name]
-            assert np.max(np.abs(a - b)) <= TOL * np.max(np.abs(b)), name
-        stats = [api.nec_gain_max(ctx, 0), api.nec_gain_min(ctx, 0), api.nec_gain_mean(ctx, 0),
-                 api.nec_gain_sd(ctx, 0)]
-        assert np.allclose(stats[:1] + stats[2:], [o["stats"][0]] + o["stats"][2:4], rtol=1e-10, atol=1e-9)
+        # all twelve pattern arrays and the twelve statistics
+        g = compare_pattern(ctx, o, 181 * 361)
+        check_stats(ctx, o, g)
     finally:
         api.nec_delete(ctx)
 
