@@ -12,6 +12,7 @@
 // tracked by a permutation instead of being moved.
 #include <climits>
 
+#include "basis.h"
 #include "kernels.h"
 #include "nec_math.h"
 
@@ -425,7 +426,91 @@ __global__ void __launch_bounds__(BT) batch_pattern_kernel(const BatchSolve* __r
   }
 }
 
+// Every solve's tables (see launch_batch_tables).  The segment table is
+// scaled as Geometry::scaled does (each coordinate, length and radius times
+// fr), so it is bit-identical to the host's; trio and tbf run the shared
+// basis.h code on the solve's lengths and radii (staged in LDS).
+constexpr int kTableCapDev = 64;
+
+__global__ void __launch_bounds__(256) batch_tables_kernel(
+    const BatchSolve* __restrict__ sol, const BatchDesign* __restrict__ des, const double* __restrict__ dgeo,
+    const int* __restrict__ dicon, const int* __restrict__ dcolptr, const int* __restrict__ dcolj,
+    const int* __restrict__ dtptr, const int* __restrict__ dcpos, const int* __restrict__ dsegptr,
+    const int* __restrict__ dsegbasis, const int* __restrict__ dbptr, const int* __restrict__ dspos,
+    double* __restrict__ seg, int* __restrict__ colptr, int* __restrict__ colj, double* __restrict__ colabc,
+    int* __restrict__ segptr, int* __restrict__ segbasis, double* __restrict__ segabc, int* status) {
+  __shared__ double sS[kBatchMaxN], sB[kBatchMaxN];
+  const BatchSolve d = sol[blockIdx.x];
+  const BatchDesign D = des[d.design];
+  const int n = d.n;
+  const int t = threadIdx.x;
+  const double fr = d.fr;
+  const double* g = dgeo + D.g0;
+  double* o = seg + d.seg0;
+  for (int i = t; i < n; i += blockDim.x) {
+    o[i] = g[i] * fr;
+    o[n + i] = g[n + i] * fr;
+    o[2 * n + i] = g[2 * n + i] * fr;
+    const double s = g[3 * n + i] * fr, b = g[4 * n + i] * fr;
+    o[3 * n + i] = s;
+    o[4 * n + i] = b;
+    o[5 * n + i] = g[5 * n + i];
+    o[6 * n + i] = g[6 * n + i];
+    o[7 * n + i] = g[7 * n + i];
+    sS[i] = s;
+    sB[i] = b;
+  }
+  const int nnzc = dcolptr[D.cptr0 + n], nnzs = dsegptr[D.sptr0 + n];
+  for (int e = t; e <= n; e += blockDim.x) {
+    colptr[d.cptr0 + e] = dcolptr[D.cptr0 + e];
+    segptr[d.sptr0 + e] = dsegptr[D.sptr0 + e];
+  }
+  for (int e = t; e < nnzc; e += blockDim.x) colj[d.csr0 + e] = dcolj[D.csr0 + e];
+  for (int e = t; e < nnzs; e += blockDim.x) segbasis[d.ssr0 + e] = dsegbasis[D.ssr0 + e];
+  __syncthreads();
+  const SegView sv{dicon + D.ic0, dicon + D.ic0 + n, sS, sB};
+  int jco[kTableCapDev];
+  double ax[kTableCapDev], bx[kTableCapDev], cx[kTableCapDev];
+  for (int j = t; j < n; j += blockDim.x) {
+    int cnt = trio(sv, j + 1, jco, ax, bx, cx, kTableCapDev);
+    int e0 = dtptr[D.tptr0 + j];
+    if (cnt != dtptr[D.tptr0 + j + 1] - e0) {
+      __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      for (int q = 0; q < cnt; ++q) {
+        const long long p = d.csr0 + dcpos[D.csr0 + e0 + q];
+        colabc[3 * p] = ax[q];
+        colabc[3 * p + 1] = bx[q];
+        colabc[3 * p + 2] = cx[q];
+      }
+    }
+    cnt = tbf(sv, j + 1, 1, jco, ax, bx, cx, kTableCapDev);
+    e0 = dbptr[D.bptr0 + j];
+    if (cnt != dbptr[D.bptr0 + j + 1] - e0) {
+      __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    } else {
+      for (int q = 0; q < cnt; ++q) {
+        const long long p = d.ssr0 + dspos[D.ssr0 + e0 + q];
+        segabc[3 * p] = ax[q];
+        segabc[3 * p + 1] = bx[q];
+        segabc[3 * p + 2] = cx[q];
+      }
+    }
+  }
+}
+
 }  // namespace
+
+int launch_batch_tables(const BatchSolve* sol, int S, const BatchDesign* des, const double* dgeo, const int* dicon,
+                        const int* dcolptr, const int* dcolj, const int* dtptr, const int* dcpos, const int* dsegptr,
+                        const int* dsegbasis, const int* dbptr, const int* dspos, double* seg, int* colptr, int* colj,
+                        double* colabc, int* segptr, int* segbasis, double* segabc, int* status, hipStream_t stream) {
+  if (S <= 0) return 0;
+  launch_k(K_BATCH_TABLES, batch_tables_kernel, dim3((unsigned)S), dim3(256), 0u, stream, sol, des, dgeo, dicon,
+           dcolptr, dcolj, dtptr, dcpos, dsegptr, dsegbasis, dbptr, dspos, seg, colptr, colj, colabc, segptr, segbasis,
+           segabc, status);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
 
 int launch_batch_lu(const BatchSolve* sol, int S, double* Z, int* rp, int* info, int P, int bt, hipStream_t stream) {
   if (S <= 0) return 0;

@@ -75,12 +75,15 @@ struct EventPair {
 struct BatchPrep {
   int S = 0, maxn = 0;
   std::vector<BatchSolve> sol;
-  std::vector<double> seg;
-  std::vector<int> colptr, colj, segptr, segbasis;
-  std::vector<double> colabc, segabc;
   std::vector<Geometry> geo;  // per design
   std::vector<double> freq;   // per solve, MHz
-  long long rows = 0, zents = 0;
+  // per design, concatenated: what the device needs to build every solve's
+  // tables (BatchDesign offsets)
+  std::vector<BatchDesign> des;
+  std::vector<double> dgeo;
+  std::vector<int> dicon, dcolptr, dcolj, dtptr, dcpos, dsegptr, dsegbasis, dbptr, dspos;
+  // sizes of the per-solve arrays the device fills
+  long long rows = 0, zents = 0, nseg = 0, ncptr = 0, ncsr = 0, nsptr = 0, nssr = 0;
   int fill_rows = 1;          // rows per fill workgroup
   std::vector<int> fill_blk;  // (solve, first row) per fill workgroup
 };
@@ -124,12 +127,11 @@ void prepare_batch(const BatchInput& in, BatchPrep* p) {
   p->sol.assign(S, BatchSolve{});
   p->geo.assign(in.n_designs, Geometry{});
   p->freq.assign(S, 0.0);
-  // every design's geometry and every solve's tables, on the host's cores
-  struct Part {
-    ScaledSegments sc;
-    BasisTables t;
-  };
-  std::vector<Part> part(S);
+  // 1. every design's geometry and the shape of its basis tables (which
+  //    depends on the connections only, not on the frequency), on the
+  //    host's cores
+  std::vector<BasisShape> shape(in.n_designs);
+  std::vector<int> src_of(in.n_designs);
   parallel_for(in.n_designs, [&](int d) {
     Geometry& geo = p->geo[d];
     for (int w = in.wire_ptr[d]; w < in.wire_ptr[d + 1]; ++w) {
@@ -141,53 +143,73 @@ void prepare_batch(const BatchInput& in, BatchPrep* p) {
     if (n < 1 || n > kBatchMaxN)
       throw NecError("batch: design " + std::to_string(d) + " has " + std::to_string(n) +
                      " segments; the batched path takes 1 to " + std::to_string(kBatchMaxN));
-    const int64_t srcno = geo.segment_number(in.src_tag[d], in.src_seg[d]);
-    const int src = static_cast<int>(srcno - 1);
-    const zc v = zmk(in.src_volt[2 * d], in.src_volt[2 * d + 1]);
-    for (int f = 0; f < in.n_freq; ++f) {
-      const size_t s = (size_t)d * in.n_freq + f;
-      const double fmhz = in.freq_mhz[s];
-      Part& q = part[s];
-      q.sc = geo.scaled(fmhz);
-      build_basis_tables(geo, q.sc, &q.t);
-      BatchSolve& b = p->sol[s];
-      p->freq[s] = fmhz;
-      b.n = n;
-      b.src = src;
-      b.wavelength = em_speed_of_light() / (1.0e6 * fmhz);
-      // e[src] = -V / (s lambda), s the scaled segment length
-      const zc e = zdivr(zneg(v), q.sc.s[src] * b.wavelength);
-      b.er = e.re;
-      b.ei = e.im;
-      b.vr = v.re;
-      b.vi = v.im;
-    }
+    src_of[d] = static_cast<int>(geo.segment_number(in.src_tag[d], in.src_seg[d]) - 1);
+    build_basis_shape(geo, geo.scaled(in.freq_mhz[(size_t)d * in.n_freq]), &shape[d]);
   });
-  // offsets and concatenation, in solve order
-  long long& rows = p->rows;
-  long long& zents = p->zents;
+  // 2. the design arrays, concatenated
+  p->des.resize(in.n_designs);
+  for (int d = 0; d < in.n_designs; ++d) {
+    const Geometry& geo = p->geo[d];
+    const BasisShape& sh = shape[d];
+    BatchDesign& D = p->des[d];
+    D.n = static_cast<int>(geo.size());
+    D.g0 = (long long)p->dgeo.size();
+    for (const std::vector<double>* v8 : {&geo.x, &geo.y, &geo.z, &geo.len, &geo.rad, &geo.cab, &geo.sab, &geo.salp})
+      p->dgeo.insert(p->dgeo.end(), v8->begin(), v8->end());
+    D.ic0 = (long long)p->dicon.size();
+    p->dicon.insert(p->dicon.end(), geo.icon1.begin(), geo.icon1.end());
+    p->dicon.insert(p->dicon.end(), geo.icon2.begin(), geo.icon2.end());
+    D.cptr0 = (long long)p->dcolptr.size();
+    p->dcolptr.insert(p->dcolptr.end(), sh.colptr.begin(), sh.colptr.end());
+    D.csr0 = (long long)p->dcolj.size();
+    p->dcolj.insert(p->dcolj.end(), sh.colj.begin(), sh.colj.end());
+    p->dcpos.insert(p->dcpos.end(), sh.cpos.begin(), sh.cpos.end());
+    D.tptr0 = (long long)p->dtptr.size();
+    p->dtptr.insert(p->dtptr.end(), sh.tptr.begin(), sh.tptr.end());
+    D.sptr0 = (long long)p->dsegptr.size();
+    p->dsegptr.insert(p->dsegptr.end(), sh.segptr.begin(), sh.segptr.end());
+    D.ssr0 = (long long)p->dsegbasis.size();
+    p->dsegbasis.insert(p->dsegbasis.end(), sh.segbasis.begin(), sh.segbasis.end());
+    p->dspos.insert(p->dspos.end(), sh.spos.begin(), sh.spos.end());
+    D.bptr0 = (long long)p->dbptr.size();
+    p->dbptr.insert(p->dbptr.end(), sh.bptr.begin(), sh.bptr.end());
+  }
+  // 3. every solve: offsets, frequency scale and right-hand side
   for (int s = 0; s < S; ++s) {
+    const int d = s / in.n_freq;
+    const Geometry& geo = p->geo[d];
     BatchSolve& b = p->sol[s];
-    const Part& q = part[s];
-    const int n = b.n;
+    const int n = static_cast<int>(geo.size());
+    const double fmhz = in.freq_mhz[s];
+    b.n = n;
+    b.src = src_of[d];
+    b.design = d;
     p->maxn = std::max(p->maxn, n);
-    b.row0 = rows;
-    b.seg0 = (long long)p->seg.size();
-    b.cptr0 = (long long)p->colptr.size();
-    b.csr0 = (long long)p->colj.size();
-    b.sptr0 = (long long)p->segptr.size();
-    b.ssr0 = (long long)p->segbasis.size();
-    b.z0 = zents;
-    for (const std::vector<double>* a : {&q.sc.x, &q.sc.y, &q.sc.z, &q.sc.s, &q.sc.b, &q.sc.cab, &q.sc.sab, &q.sc.salp})
-      p->seg.insert(p->seg.end(), a->begin(), a->end());
-    p->colptr.insert(p->colptr.end(), q.t.colptr.begin(), q.t.colptr.end());
-    p->colj.insert(p->colj.end(), q.t.colj.begin(), q.t.colj.end());
-    p->colabc.insert(p->colabc.end(), q.t.colabc.begin(), q.t.colabc.end());
-    p->segptr.insert(p->segptr.end(), q.t.segptr.begin(), q.t.segptr.end());
-    p->segbasis.insert(p->segbasis.end(), q.t.segbasis.begin(), q.t.segbasis.end());
-    p->segabc.insert(p->segabc.end(), q.t.segabc.begin(), q.t.segabc.end());
-    rows += n;
-    zents += (long long)n * n;
+    b.row0 = p->rows;
+    b.seg0 = p->nseg;
+    b.cptr0 = p->ncptr;
+    b.csr0 = p->ncsr;
+    b.sptr0 = p->nsptr;
+    b.ssr0 = p->nssr;
+    b.z0 = p->zents;
+    p->rows += n;
+    p->zents += (long long)n * n;
+    p->nseg += 8LL * n;
+    p->ncptr += n + 1;
+    p->nsptr += n + 1;
+    p->ncsr += (long long)shape[d].colj.size();
+    p->nssr += (long long)shape[d].segbasis.size();
+    p->freq[s] = fmhz;
+    // as Geometry::scaled: wavelengths per metre, the source segment's length
+    b.fr = (1.0e6 * fmhz) / em_speed_of_light();
+    b.wavelength = em_speed_of_light() / (1.0e6 * fmhz);
+    const zc v = zmk(in.src_volt[2 * d], in.src_volt[2 * d + 1]);
+    // e[src] = -V / (s lambda), s the scaled segment length
+    const zc e = zdivr(zneg(v), (geo.len[b.src] * b.fr) * b.wavelength);
+    b.er = e.re;
+    b.ei = e.im;
+    b.vr = v.re;
+    b.vi = v.im;
   }
   p->fill_rows = batch_fill_rows(p->maxn);
   for (int s = 0; s < S; ++s)
@@ -197,9 +219,11 @@ void prepare_batch(const BatchInput& in, BatchPrep* p) {
     }
 }
 
-// Device copies of a batch's tables and the solve buffers.
+// Device copies of a batch's design tables, the per-solve tables the
+// device builds from them, and the solve buffers.
 struct BatchDev {
-  DevBuf sol, seg, colptr, colj, colabc, segptr, segbasis, segabc, fill_blk;
+  DevBuf sol, des, dgeo, dicon, dcolptr, dcolj, dtptr, dcpos, dsegptr, dsegbasis, dbptr, dspos, fill_blk;
+  DevBuf seg, colptr, colj, colabc, segptr, segbasis, segabc, status;
   DevBuf Z, rp, info, x, abc, zin;
   int fill_rows = 1, fill_nblk = 0, maxn = 0;
   void upload_all(const BatchPrep& p, hipStream_t st) {
@@ -208,19 +232,45 @@ struct BatchDev {
     fill_rows = p.fill_rows;
     fill_nblk = static_cast<int>(p.fill_blk.size() / 2);
     maxn = p.maxn;
-    upload(seg, p.seg, st);
-    upload(colptr, p.colptr, st);
-    upload(colj, p.colj, st);
-    upload(colabc, p.colabc, st);
-    upload(segptr, p.segptr, st);
-    upload(segbasis, p.segbasis, st);
-    upload(segabc, p.segabc, st);
+    upload(des, p.des, st);
+    upload(dgeo, p.dgeo, st);
+    upload(dicon, p.dicon, st);
+    upload(dcolptr, p.dcolptr, st);
+    upload(dcolj, p.dcolj, st);
+    upload(dtptr, p.dtptr, st);
+    upload(dcpos, p.dcpos, st);
+    upload(dsegptr, p.dsegptr, st);
+    upload(dsegbasis, p.dsegbasis, st);
+    upload(dbptr, p.dbptr, st);
+    upload(dspos, p.dspos, st);
+    seg.ensure((size_t)p.nseg * 8);
+    colptr.ensure((size_t)p.ncptr * 4);
+    segptr.ensure((size_t)p.nsptr * 4);
+    colj.ensure((size_t)std::max(1LL, p.ncsr) * 4);
+    colabc.ensure((size_t)std::max(1LL, p.ncsr) * 24);
+    segbasis.ensure((size_t)std::max(1LL, p.nssr) * 4);
+    segabc.ensure((size_t)std::max(1LL, p.nssr) * 24);
+    status.ensure(sizeof(int));
     Z.ensure((size_t)p.zents * 16);
     rp.ensure((size_t)p.rows * sizeof(int));
     info.ensure((size_t)p.S * sizeof(int));
     x.ensure((size_t)p.rows * 16);
     abc.ensure((size_t)p.rows * 48);
     zin.ensure((size_t)p.S * 32);
+  }
+  // every solve's segment table and basis tables, on the device
+  void build_tables(int S, hipStream_t st) {
+    hip_check(hipMemsetAsync(status.as<void>(), 0, sizeof(int), st), "memset");
+    if (launch_batch_tables(sol.as<BatchSolve>(), S, des.as<BatchDesign>(), dgeo.as<double>(), dicon.as<int>(),
+                            dcolptr.as<int>(), dcolj.as<int>(), dtptr.as<int>(), dcpos.as<int>(), dsegptr.as<int>(),
+                            dsegbasis.as<int>(), dbptr.as<int>(), dspos.as<int>(), seg.as<double>(), colptr.as<int>(),
+                            colj.as<int>(), colabc.as<double>(), segptr.as<int>(), segbasis.as<int>(),
+                            segabc.as<double>(), status.as<int>(), st))
+      hip_check(hipGetLastError(), "batch tables");
+    int h = 0;
+    hip_check(hipMemcpyAsync(&h, status.as<void>(), sizeof(int), hipMemcpyDeviceToHost, st), "status");
+    hip_check(hipStreamSynchronize(st), "batch tables");
+    if (h) throw NecError("batch: a basis table's shape changed with the frequency");
   }
 };
 
@@ -260,6 +310,7 @@ void run_batch(const BatchInput& in, BatchOutput* out) {
   const hipStream_t st = sg.s;
   BatchDev dv;
   dv.upload_all(prep, st);
+  dv.build_tables(S, st);
   DevBuf d_dir, d_Zbak, d_gain;
   upload(d_dir, dir, st);
   double* Z = dv.Z.as<double>();
@@ -421,6 +472,7 @@ void run_batch_adjoint(const BatchInput& in, const AdjointSpec& spec, BatchAdjoi
   const hipStream_t st = sg.s;
   BatchDev dv;
   dv.upload_all(prep, st);
+  dv.build_tables(S, st);
   const BatchSolve* dsol = dv.sol.as<BatchSolve>();
   EventPair e1, e2, e3, e4;
   hip_check(hipEventRecord(e1.a, st), "event");

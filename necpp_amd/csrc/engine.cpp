@@ -140,6 +140,61 @@ void build_basis_tables(const Geometry& geo, const ScaledSegments& seg, BasisTab
   }
 }
 
+void build_basis_shape(const Geometry& geo, const ScaledSegments& seg, BasisShape* out) {
+  // The index side of build_basis_tables: the same trio / tbf walks and
+  // counting sorts, keeping for every entry where its coefficients go.
+  const int n = static_cast<int>(geo.size());
+  std::vector<int> icon1(geo.icon1.begin(), geo.icon1.end()), icon2(geo.icon2.begin(), geo.icon2.end());
+  SegView sv{icon1.data(), icon2.data(), seg.s.data(), seg.b.data()};
+  int jco[kTableCap];
+  double ax[kTableCap], bx[kTableCap], cx[kTableCap];
+  std::vector<int> tk, sj;
+  out->tptr.assign(n + 1, 0);
+  out->bptr.assign(n + 1, 0);
+  std::vector<int> ccount(n + 1, 0), scount(n + 1, 0);
+  for (int j = 0; j < n; ++j) {
+    const int cnt = trio(sv, j + 1, jco, ax, bx, cx, kTableCap);
+    if (cnt < 0) throw NecError("TRIO - SEGMENT CONNECTION ERROR FOR SEGMENT " + std::to_string(j + 1));
+    for (int q = 0; q < cnt; ++q) {
+      tk.push_back(jco[q] - 1);
+      ++ccount[jco[q]];
+    }
+    out->tptr[j + 1] = out->tptr[j] + cnt;
+  }
+  for (int i = 0; i < n; ++i) {
+    const int cnt = tbf(sv, i + 1, 1, jco, ax, bx, cx, kTableCap);
+    if (cnt < 0) throw NecError("TBF - SEGMENT CONNECTION ERROR FOR SEGMENT " + std::to_string(i + 1));
+    for (int q = 0; q < cnt; ++q) {
+      sj.push_back(jco[q] - 1);
+      ++scount[jco[q]];
+    }
+    out->bptr[i + 1] = out->bptr[i] + cnt;
+  }
+  out->colptr.assign(n + 1, 0);
+  out->segptr.assign(n + 1, 0);
+  for (int k = 0; k < n; ++k) {
+    out->colptr[k + 1] = out->colptr[k] + ccount[k + 1];
+    out->segptr[k + 1] = out->segptr[k] + scount[k + 1];
+  }
+  out->colj.assign(tk.size(), 0);
+  out->cpos.assign(tk.size(), 0);
+  out->segbasis.assign(sj.size(), 0);
+  out->spos.assign(sj.size(), 0);
+  std::vector<int> cfill(out->colptr.begin(), out->colptr.end() - 1), sfill(out->segptr.begin(), out->segptr.end() - 1);
+  for (int j = 0; j < n; ++j)
+    for (int e = out->tptr[j]; e < out->tptr[j + 1]; ++e) {
+      const int p = cfill[tk[e]]++;
+      out->colj[p] = j;
+      out->cpos[e] = p;
+    }
+  for (int i = 0; i < n; ++i)
+    for (int e = out->bptr[i]; e < out->bptr[i + 1]; ++e) {
+      const int p = sfill[sj[e]]++;
+      out->segbasis[p] = i;
+      out->spos[e] = p;
+    }
+}
+
 DevBuf::~DevBuf() { release(); }
 
 void DevBuf::release() {

@@ -67,6 +67,15 @@ struct BasisTables {
   std::vector<double> colabc, segabc;
 };
 void build_basis_tables(const Geometry& geo, const ScaledSegments& seg, BasisTables* out);
+// The index arrays of build_basis_tables (which depend on the connections
+// only) and, for the q-th entry of trio(j) (at tptr[j] + q) and of tbf(i)
+// (at bptr[i] + q), the position cpos / spos of its coefficients in the
+// column / segment CSR: what the device needs to place the coefficients
+// it computes.
+struct BasisShape {
+  std::vector<int> colptr, colj, tptr, cpos, segptr, segbasis, bptr, spos;
+};
+void build_basis_shape(const Geometry& geo, const ScaledSegments& seg, BasisShape* out);
 
 class DeviceSolver {
  public:

@@ -38,6 +38,7 @@ enum KernelKind {
   K_ADJOINT_REDUCE,
   K_LU_NEXT,
   K_NEAR_FIELD,
+  K_BATCH_TABLES,
   K_NKINDS
 };
 const char* kernel_kind_name(int kind);
@@ -107,8 +108,33 @@ struct BatchSolve {
   double wavelength;
   double er, ei;    // right-hand side at the source: -V / (s lambda)
   double vr, vi;    // source voltage
+  int design;       // index into the BatchDesign table
+  double fr;        // frequency scale: wavelengths per metre
 };
 constexpr int kBatchMaxN = 512;  // largest structure of the batched path
+
+// One design of a batch: what does not change with the frequency.  Offsets
+// into the concatenated design arrays: geometry g0 (8 n doubles: x, y, z,
+// length, radius in metres, then cab, sab, salp), connections ic0 (icon1 n,
+// then icon2 n), the column CSR cptr0 (colptr n + 1) / csr0 (colj and cpos,
+// nnz each), trio's entry starts tptr0 (n + 1), the segment CSR sptr0
+// (segptr n + 1) / ssr0 (segbasis and spos, nnz each) and tbf's entry
+// starts bptr0 (n + 1).  cpos / spos give, for the q-th entry of trio(j) /
+// tbf(i), its position in the column / segment CSR.
+struct BatchDesign {
+  int n;
+  long long g0, ic0, cptr0, csr0, tptr0, sptr0, ssr0, bptr0;
+};
+
+// batch.hip: every solve's tables on the device, one workgroup per solve:
+// the segment table in wavelengths (seg, 8 n doubles at sol.seg0), the CSR
+// index arrays copied from its design, and the basis coefficients (colabc,
+// segabc) from trio / tbf at the solve's frequency.  status: set to 1 if a
+// trio / tbf count differs from the design's (never for a valid design).
+int launch_batch_tables(const BatchSolve* sol, int S, const BatchDesign* des, const double* dgeo, const int* dicon,
+                        const int* dcolptr, const int* dcolj, const int* dtptr, const int* dcpos, const int* dsegptr,
+                        const int* dsegbasis, const int* dbptr, const int* dspos, double* seg, int* colptr, int* colj,
+                        double* colabc, int* segptr, int* segbasis, double* segabc, int* status, hipStream_t stream);
 
 // fill.hip: one workgroup per block of R consecutive rows of one solve
 // (R = batch_fill_rows(largest n)); blk holds (solve, first row) of each of

@@ -104,3 +104,17 @@ def test_batch_full_population_sample():
         z, g = single(designs[d], freqs[f])
         assert abs(zin[d, f] - z) <= TOL * abs(z), (d, f)
         assert np.max(np.abs(gain[d, f] - g)) < 1e-9, (d, f)
+    # a stratified sample straight against the oracle: the first design of
+    # every element count (3 .. 12) at a random frequency of the sweep
+    counts = [len(dk["wires"]) for dk in designs]
+    rp = dict(n_theta=1, n_phi=361, theta0=90.0, phi0=0.0, dtheta=0.0, dphi=1.0)
+    for k in range(3, 13):
+        d = counts.index(k)
+        f = int(rng.integers(0, 64))
+        dk = dict(designs[d])
+        dk["freq_mhz"] = float(freqs[f])
+        g = po.geometry_from_deck(dk)
+        o = g.run_voltage(dk["freq_mhz"], [(g.segment_number(*dk["source"]), dk["voltage"])], rp=rp, nthreads=8)
+        assert abs(zin[d, f] - o["zin"][0]) <= TOL * abs(o["zin"][0]), (k, d, f)
+        og = o["pattern"]["pg_tot"]
+        assert np.max(np.abs(gain[d, f] - og)) < 1e-8, (k, d, f)
