@@ -280,7 +280,44 @@ void DeviceSolver::load(const Geometry& geo, const ScaledSegments& seg, bool use
   if (use_ek)
     for (int j = 0; j < n; ++j) ek_ends(geo, seg, j, &ek[2 * j], &ek[2 * j + 1]);
 
+  // The fill's field order: segments sorted along a Morton (Z-order) curve
+  // of their centres, so that the bases a gather wave handles, and the
+  // sources their supports share at junctions, sit close together in E.
+  // NECGPU_FILL_ORDER=0 keeps the natural order (for measurements).
+  std::vector<int> ord(n), pos(n), cjp(t.colj.size());
+  for (int i = 0; i < n; ++i) ord[i] = i;
+  const char* fo = std::getenv("NECGPU_FILL_ORDER");
+  if (!(fo && fo[0] == '0') && n > 1) {
+    double lo[3] = {seg.x[0], seg.y[0], seg.z[0]}, hi[3] = {lo[0], lo[1], lo[2]};
+    for (int i = 0; i < n; ++i) {
+      const double c[3] = {seg.x[i], seg.y[i], seg.z[i]};
+      for (int a = 0; a < 3; ++a) {
+        lo[a] = std::min(lo[a], c[a]);
+        hi[a] = std::max(hi[a], c[a]);
+      }
+    }
+    const double span = std::max(hi[0] - lo[0], std::max(hi[1] - lo[1], hi[2] - lo[2]));
+    std::vector<uint64_t> key(n, 0);
+    for (int i = 0; i < n; ++i) {
+      const double c[3] = {seg.x[i], seg.y[i], seg.z[i]};
+      uint64_t q[3];
+      for (int a = 0; a < 3; ++a) {
+        const double u = span > 0.0 ? (c[a] - lo[a]) / span : 0.0;
+        q[a] = (uint64_t)std::min(2097151.0, std::max(0.0, u * 2097151.0));
+      }
+      uint64_t m = 0;
+      for (int bit = 20; bit >= 0; --bit)
+        for (int a = 0; a < 3; ++a) m = (m << 1) | ((q[a] >> bit) & 1u);
+      key[i] = m;
+    }
+    std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return key[a] < key[b]; });
+  }
+  for (int p = 0; p < n; ++p) pos[ord[p]] = p;
+  for (size_t q = 0; q < t.colj.size(); ++q) cjp[q] = pos[t.colj[q]];
+
   upload(seg_, segv, stream_);
+  upload(ford_, ord, stream_);
+  upload(fcjp_, cjp, stream_);
   upload(colptr_, t.colptr, stream_);
   upload(colj_, t.colj, stream_);
   upload(colabc_, t.colabc, stream_);
@@ -338,7 +375,8 @@ void DeviceSolver::enqueue_fill(double rkh) {
     const int r = std::min(rows, n - i0);
     if (launch_fill(g, n, i0, r, rkh, use_ek_ ? 1 : 0, ekind_.as<int>(), ground_,
                     has_loads_ ? zload_dev_.as<double>() : nullptr, E_.as<double>(), colptr_.as<int>(),
-                    colj_.as<int>(), colabc_.as<double>(), Z_.as<double>(), ld_, stream_))
+                    colj_.as<int>(), fcjp_.as<int>(), ford_.as<int>(), colabc_.as<double>(), Z_.as<double>(), ld_,
+                    stream_))
       hip_check(hipGetLastError(), "fill launch");
   }
 }

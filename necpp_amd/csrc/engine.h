@@ -203,6 +203,7 @@ class DeviceSolver {
   void refine_solve();
 
   DevBuf seg_, ekind_, colptr_, colj_, colabc_, segptr_, segbasis_, segabc_;
+  DevBuf ford_, fcjp_;  // the fill's field order and the column CSR's sources in it
   DevBuf Z_, E_, piv_, info_, lu_work_, x_, abc_, curx_, solve_work_, rhs_, src_, zin_;
   DevBuf trig_, pat_part_, pat_out_, pat_sense_;
   DevBuf Z32_, b_, r_, acc_;  // mixed path: FP32 factors, right-hand side, residual, norms

@@ -557,10 +557,11 @@ __device__ void pair_fields_ground(const DevSegments& g, int i, int j, double rk
 template <bool GROUND>
 __global__ void __launch_bounds__(256) fill_fields_kernel(DevSegments g, int n, int i0, int rows, double rkh,
                                                           int use_ek, const int* ekind, GroundSpec gs,
-                                                          double* __restrict__ E) {
-  const int j = blockIdx.x * blockDim.x + threadIdx.x;
+                                                          const int* __restrict__ ord, double* __restrict__ E) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int r = blockIdx.y;
-  if (j >= n || r >= rows) return;
+  if (t >= n || r >= rows) return;
+  const int j = ord[t];  // the source at position t of the field order
   const int i = i0 + r;
   int ind1 = 2, ind2 = 2;
   if (use_ek) {
@@ -570,7 +571,7 @@ __global__ void __launch_bounds__(256) fill_fields_kernel(DevSegments g, int n, 
   zc etk, ets, etc;
   if (GROUND) pair_fields_ground(g, i, j, rkh, use_ek, ind1, ind2, gs, &etk, &ets, &etc);
   else pair_fields(g, i, j, rkh, use_ek, ind1, ind2, &etk, &ets, &etc);
-  double* o = E + ((size_t)r * n + j) * 6;
+  double* o = E + ((size_t)r * n + t) * 6;
   o[0] = etk.re;
   o[1] = etk.im;
   o[2] = ets.re;
@@ -583,29 +584,35 @@ __global__ void __launch_bounds__(256) fill_fields_kernel(DevSegments g, int n, 
 // the sum over k's support (ascending source segment j) of etk*a + ets*b +
 // etc*c, as cmset/cmww accumulate it (src/nec_context.cpp:2635-2638), then
 // a load on segment i right after its own term.  The support entries are
-// read once for the RB rows, whose field loads are independent.
+// read once for the RB rows, whose field loads are independent.  Thread t
+// takes basis ord[t] and finds source j's fields at position cjp[q] (the
+// field order), so that neighbouring threads, whose supports meet at the
+// same junctions, read neighbouring records.
 template <int RB>
 __global__ void __launch_bounds__(256) fill_assemble_kernel(const double* __restrict__ E, int n, int i0, int rows,
                                                             const int* __restrict__ colptr,
                                                             const int* __restrict__ colj,
+                                                            const int* __restrict__ cjp,
+                                                            const int* __restrict__ ord,
                                                             const double* __restrict__ colabc,
                                                             const double* __restrict__ zload,
                                                             double* __restrict__ Z, long long ldz) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
   const int r0 = blockIdx.y * RB;
-  if (k >= n || r0 >= rows) return;
+  if (t >= n || r0 >= rows) return;
+  const int k = ord[t];
   const int nr = min(RB, rows - r0);
   double accr[RB], acci[RB];
 #pragma unroll
   for (int u = 0; u < RB; ++u) accr[u] = acci[u] = 0.0;
   const int q1 = colptr[k + 1];
   for (int q = colptr[k]; q < q1; ++q) {
-    const int j = colj[q];
+    const int jp = cjp[q];
     const double a = colabc[3 * q], b = colabc[3 * q + 1], c = colabc[3 * q + 2];
     double e[RB][6];
 #pragma unroll
     for (int u = 0; u < RB; ++u) {
-      const double* p = E + ((size_t)(r0 + min(u, nr - 1)) * n + j) * 6;
+      const double* p = E + ((size_t)(r0 + min(u, nr - 1)) * n + jp) * 6;
 #pragma unroll
       for (int m = 0; m < 6; ++m) e[u][m] = p[m];
     }
@@ -616,9 +623,10 @@ __global__ void __launch_bounds__(256) fill_assemble_kernel(const double* __rest
       const double ti = ((e[u][1] * a) + (e[u][3] * b)) + (e[u][5] * c);
       accr[u] += tr;
       acci[u] += ti;
-      if (zload && j == i0 + r0 + u) {
+      if (zload && colj[q] == i0 + r0 + u) {
         // the load of segment i, right after its own term
         // (src/nec_context.cpp:2126-2142): -= (A + C) z
+        const int j = colj[q];
         const double sl = a + c;
         accr[u] -= sl * zload[2 * j];
         acci[u] -= sl * zload[2 * j + 1];
@@ -634,13 +642,15 @@ __global__ void __launch_bounds__(256) fill_assemble_kernel(const double* __rest
     }
 }
 
-// Rows per thread of the fill assembly: 4 (NECGPU_FILL_RB = 1 or 2
-// overrides, for measurements).  At the 3,280-segment grid 4 rows take the
-// assembly from 0.77 to 0.58 ms.
-int fill_assemble_rows() {
+// Rows per thread of the fill assembly: 4, or 8 above 8,192 segments
+// (NECGPU_FILL_RB = 1, 2, 4 or 8 overrides, for measurements).  With the
+// Morton field order, the 3,280-segment grid assembles in 510 us with 4
+// rows against 580 with 8; the 16,020-segment grid fills in 29.1 ms with 8
+// rows against 30.7 with 4.
+int fill_assemble_rows(int n) {
   const char* v = std::getenv("NECGPU_FILL_RB");
-  const int r = (v && *v) ? std::atoi(v) : 4;
-  return (r == 1 || r == 2) ? r : 4;
+  const int r = (v && *v) ? std::atoi(v) : (n > 8192 ? 8 : 4);
+  return (r == 1 || r == 2 || r == 8) ? r : 4;
 }
 
 // Batched fill: one workgroup per (solve, block of R consecutive rows), the
@@ -1324,27 +1334,30 @@ int launch_near_field(bool magnetic, const DevSegments& g, int n, const double* 
 
 int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind,
                 const GroundSpec& ground, const double* zload, double* E, const int* colptr, const int* colj,
-                const double* colabc, double* Z, long long ldz, hipStream_t stream) {
+                const int* cjp, const int* ord, const double* colabc, double* Z, long long ldz, hipStream_t stream) {
   if (n <= 0 || rows <= 0) return 0;
   dim3 block(256);
   dim3 grid((n + 255) / 256, rows);
   if (ground.type != 0)
     launch_k(K_FILL_FIELDS, fill_fields_kernel<true>, grid, block, 0, stream, g, n, i0, rows, rkh, use_ek, ekind,
-             ground, E);
+             ground, ord, E);
   else
     launch_k(K_FILL_FIELDS, fill_fields_kernel<false>, grid, block, 0, stream, g, n, i0, rows, rkh, use_ek, ekind,
-             ground, E);
-  const int rb = fill_assemble_rows();
+             ground, ord, E);
+  const int rb = fill_assemble_rows(n);
   const dim3 agrid((n + 255) / 256, (rows + rb - 1) / rb);
-  if (rb == 4)
-    launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel<4>, agrid, block, 0, stream, E, n, i0, rows, colptr, colj, colabc,
-             zload, Z, ldz);
+  if (rb == 8)
+    launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel<8>, agrid, block, 0, stream, E, n, i0, rows, colptr, colj, cjp,
+             ord, colabc, zload, Z, ldz);
+  else if (rb == 4)
+    launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel<4>, agrid, block, 0, stream, E, n, i0, rows, colptr, colj, cjp,
+             ord, colabc, zload, Z, ldz);
   else if (rb == 2)
-    launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel<2>, agrid, block, 0, stream, E, n, i0, rows, colptr, colj, colabc,
-             zload, Z, ldz);
+    launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel<2>, agrid, block, 0, stream, E, n, i0, rows, colptr, colj, cjp,
+             ord, colabc, zload, Z, ldz);
   else
-    launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel<1>, agrid, block, 0, stream, E, n, i0, rows, colptr, colj, colabc,
-             zload, Z, ldz);
+    launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel<1>, agrid, block, 0, stream, E, n, i0, rows, colptr, colj, cjp,
+             ord, colabc, zload, Z, ldz);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

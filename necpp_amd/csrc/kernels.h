@@ -174,7 +174,9 @@ int launch_batch_adjoint_fill(const BatchSolve* sol, int S, const int* blk, cons
 // gather into Z (row-major complex, leading dimension ldz complex entries).
 // With a ground each pair adds the image term of efld.  zload (2 doubles per
 // segment, or null): loads, subtracted on each loaded segment's row right
-// after its own term (cmset's order).
+// after its own term (cmset's order).  ord: the field order (E's column t
+// holds source ord[t]; the gather takes basis ord[t] on thread t); cjp: for
+// every column-CSR entry, the position of its source in that order.
 // Near electric (nefld; magnetic false) or magnetic (nhfld; magnetic true)
 // field of wires, in free space or over the ground, at npts points (3
 // doubles each, wavelengths); ax: per point the radius of the segment it
@@ -186,7 +188,7 @@ int launch_near_field(bool magnetic, const DevSegments& g, int n, const double* 
 
 int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind,
                 const GroundSpec& ground, const double* zload, double* E, const int* colptr, const int* colj,
-                const double* colabc, double* Z, long long ldz, hipStream_t stream);
+                const int* cjp, const int* ord, const double* colabc, double* Z, long long ldz, hipStream_t stream);
 
 // lu.hip: in-place partial-pivot LU of the row-major complex n x n matrix A
 // (leading dimension lda complex).  piv (2n ints): piv[k] = row exchanged
