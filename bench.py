@@ -71,15 +71,80 @@ def lu_work(n):
             "lu_laswp": ("byte", laswp_bytes)}
 
 
-def read_traffic(kernel, workload_tag):
-    """HBM bytes per launch from the committed rocprofv3 --pmc summary, or None."""
-    path = os.path.join(ROOT, "profiles", f"{kernel}_{workload_tag}_traffic.txt")
+def read_profile_value(name, key):
+    """A "key=value" figure from a committed rocprofv3 summary under
+    profiles/, or None."""
+    path = os.path.join(ROOT, "profiles", name)
     if not os.path.exists(path):
         return None
     for line in open(path):
-        if line.startswith("traffic_bytes_per_launch="):
+        if line.startswith(key + "="):
             return float(line.split("=", 1)[1])
     return None
+
+
+def read_traffic(kernel, workload_tag):
+    """HBM bytes per launch from the committed rocprofv3 --pmc summary, or None."""
+    return read_profile_value(f"{kernel}_{workload_tag}_traffic.txt", "traffic_bytes_per_launch")
+
+
+def op_counts(workload_tag):
+    """The oracle's operation tally of the fill and the pattern sum for a
+    wire-grid workload (tests/golden/fill_counts.json, made by
+    tests/golden/make_fill_counts.py), or None for another workload."""
+    path = os.path.join(ROOT, "tests", "golden", "fill_counts.json")
+    if not os.path.exists(path):
+        return None
+    return json.load(open(path)).get(workload_tag)
+
+
+def fill_rates(counts, fill_kernel_ms, stage_ms, workload_tag=None):
+    """Achieved FP64 rate of the fill from the tallied flops, and its HBM
+    traffic (fields plus assembly kernels) from the committed --pmc summary
+    against the 16 N^2 bytes of Z."""
+    if counts is None:
+        return None
+    c = counts["fill"]
+    tf = c["flops"] / (fill_kernel_ms * 1e-3) / 1e12
+    traffic = None
+    if workload_tag is not None:
+        parts = [read_profile_value(f"fill_and_pattern_{workload_tag}_traffic.txt", f"{k}_traffic_bytes_per_launch")
+                 for k in ("fill_fields", "fill_assemble")]
+        traffic = None if None in parts else sum(parts)
+    z_bytes = 16.0 * counts["n"] ** 2
+    return {"hbm_traffic_bytes": traffic, "z_bytes": z_bytes,
+            "traffic_over_z_bytes": None if traffic is None else traffic / z_bytes,
+            "hbm_gbs": None if traffic is None else traffic / (fill_kernel_ms * 1e-3) / 1e9,
+            "flops": c["flops"], "divs": c["divs"], "sqrts": c["sqrts"], "specials": c["specials"],
+            "pairs": c["pairs"], "near_pairs": c["near_pairs"], "kernel_ms": fill_kernel_ms,
+            "achieved_tflops": tf, "frac_fp64_vector_peak": tf / PEAK_FP64_TFLOPS,
+            "specials_per_s": c["specials"] / (fill_kernel_ms * 1e-3),
+            "stage_ms": stage_ms, "achieved_tflops_stage": c["flops"] / (stage_ms * 1e-3) / 1e12,
+            "count": "oracle tally, tests/golden/fill_counts.json"}
+
+
+def pattern_rates(counts, n, kernel_ms, workload_tag):
+    """Achieved FP64 rate and HBM rates of the pattern's segment sums.
+
+    Algorithmic bytes: the segment data read once (x, y, z, the three
+    direction cosines, the length and the six current coefficients, 13
+    doubles per segment) and the three complex sums written once per
+    direction (48 bytes)."""
+    if counts is None:
+        return None
+    c = counts["pattern"]
+    n_dir = counts["pattern_directions"]
+    alg_bytes = 13.0 * 8.0 * n + 48.0 * n_dir
+    t = kernel_ms * 1e-3
+    tf = c["flops"] / t / 1e12
+    traffic = read_profile_value(f"fill_and_pattern_{workload_tag}_traffic.txt", "pattern_partial_traffic_bytes_per_launch")
+    return {"flops": c["flops"], "divs": c["divs"], "specials": c["specials"], "terms": c["pairs"],
+            "kernel_ms": kernel_ms, "achieved_tflops": tf, "frac_fp64_vector_peak": tf / PEAK_FP64_TFLOPS,
+            "specials_per_s": c["specials"] / t,
+            "algorithmic_bytes": alg_bytes, "achieved_gbs": alg_bytes / t / 1e9,
+            "hbm_traffic_bytes": traffic, "hbm_gbs": None if traffic is None else traffic / t / 1e9,
+            "frac_hbm_peak": None if traffic is None else traffic / t / 1e9 / PEAK_HBM_GBS,
+            "count": "oracle tally, tests/golden/fill_counts.json"}
 
 
 def short_name(name):
@@ -230,7 +295,9 @@ def mixed_leg(nn, steps=5):
             "zin_mixed_vs_fp64_rel": abs(z32 - z64) / abs(z64),
             "zin_mixed_vs_oracle_rel": None if zref is None else abs(z32 - zref) / abs(zref),
             "zin_fp64_vs_oracle_rel": None if zref is None else abs(z64 - zref) / abs(zref),
-            "errors_vs_oracle": errors}
+            "errors_vs_oracle": errors,
+            "fp64_fill_stage": fill_rates(op_counts(f"grid{nn}"), st64[0], st64[0]),
+            "fp64_pattern_stage": pattern_rates(op_counts(f"grid{nn}"), n, st64[3], f"grid{nn}")}
 
 
 def batch_leg(repeats=3):
@@ -401,6 +468,20 @@ def main():
                 "frac": achieved / PEAK_HBM_GBS}
     else:
         roof = {"bound": None, "achieved": None, "peak": None, "unit": None, "frac": None}
+    counts = op_counts(workload_tag)
+    fill_kernel_ms = sum(kernels[k]["ms_per_step"] for k in ("fill_fields", "fill_assemble") if k in kernels)
+    fill_stage = fill_rates(counts, fill_kernel_ms, fill_ms, workload_tag) if fill_kernel_ms > 0 else None
+    pat_kernel_ms = kernels.get("pattern_partial", {}).get("ms_per_step")
+    pattern_stage = pattern_rates(counts, n, pat_kernel_ms, workload_tag) if pat_kernel_ms else None
+    hw_name = f"fill_and_pattern_{workload_tag}_fp64_counters.txt"
+    for stage, keys in ((fill_stage, ("fill_fields", "fill_assemble")), (pattern_stage, ("pattern_partial",))):
+        if stage is None:
+            continue
+        hw = [read_profile_value(hw_name, f"{k}_fp64_flops_per_launch") for k in keys]
+        if all(v is not None for v in hw):
+            stage["rocprof_fp64_flops"] = sum(hw)
+            stage["rocprof_fp64_tflops"] = sum(hw) / (stage["kernel_ms"] * 1e-3) / 1e12
+            stage["rocprof_over_tally"] = sum(hw) / stage["flops"]
     roof["traffic"] = read_traffic(dom, workload_tag)
     roof["kernel"] = ROCPROF_NAME.get(dom, dom)
     roof["avg_launch_us"] = dk["avg_us"]
@@ -426,6 +507,8 @@ def main():
         "roofline": roof,
         "lu_stage": {"achieved_tflops": lu_tf, "frac_fp64_peak": lu_tf / PEAK_FP64_TFLOPS,
                      "flops": lu_flops},
+        "fill_stage": fill_stage,
+        "pattern_stage": pattern_stage,
         "kernels": kernels,
         "kernel_timing": {"steps": kt_steps, "ms_per_step_with_kernel_events": ms_with_kernel_events},
         "mixed_vs_fp64": None,

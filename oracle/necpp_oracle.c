@@ -961,6 +961,12 @@ typedef struct {
   ora_fill_counts* cnt;
 } fill_state;
 
+/* Operation tally of one basic block (the counters' comment in the header
+ * gives the convention): f flops, of which d divisions and q square roots,
+ * and sp specials. */
+#define TALLY(c, f, d, q, sp) \
+  do { (c)->flops += (f); (c)->divs += (d); (c)->sqrts += (q); (c)->specials += (sp); } while (0)
+
 /* gf, src/nec_context.cpp:3553-3576. */
 static void gf(fill_state* st, double zk, double* co, double* si) {
   static const double c0 = -1.38888889e-3, c1 = 4.16666667e-2;
@@ -968,20 +974,24 @@ static void gf(fill_state* st, double zk, double* co, double* si) {
   double zdk = zk - st->zpk;
   double rk = sqrt(st->rkb2 + zdk * zdk);
   *si = sin(rk) / rk;
-  if (st->ija != 0) { *co = cos(rk) / rk; return; }
-  if (tape_decide(rk >= .2)) { *co = (cos(rk) - 1.0) / rk; return; }
+  TALLY(st->cnt, 5, 1, 1, 1);
+  if (st->ija != 0) { *co = cos(rk) / rk; TALLY(st->cnt, 1, 1, 0, 1); return; }
+  if (tape_decide(rk >= .2)) { *co = (cos(rk) - 1.0) / rk; TALLY(st->cnt, 2, 1, 0, 1); return; }
   double rks = rk * rk;
   *co = ((c0 * rks + c1) * rks - 0.5) * rk;
+  TALLY(st->cnt, 6, 0, 0, 0);
 }
 
-/* test, src/c_evlcom.cpp:845-868. */
-static void conv_test(double f1r, double f2r, double* tr, double f1i, double f2i, double* ti, double dmin) {
+/* test, src/c_evlcom.cpp:845-868 (cnt: the tally, or NULL for none). */
+static void conv_test(ora_fill_counts* cnt, double f1r, double f2r, double* tr, double f1i, double f2i, double* ti,
+                      double dmin) {
   double den = fabs(f2r), t = fabs(f2i);
   if (den < t) den = t;
   if (den < dmin) den = dmin;
   if (den < 1.0e-37) { *tr = 0.0; *ti = 0.0; return; }
   *tr = fabs((f1r - f2r) / den);
   *ti = fabs((f1i - f2i) / den);
+  if (cnt) TALLY(cnt, 4, 2, 0, 0);
 }
 
 /* intx, src/nec_context.cpp:4135-4297: adaptive Romberg integration of the
@@ -993,6 +1003,7 @@ static void intx(fill_state* st, double el1, double el2, double b, int ij, doubl
   double g2r, g4r, t02i, g2i, g4i, t11r, t11i, t20r, t20i, te2i, te2r;
   const double rx = 1.0e-4;
   st->cnt->intx_calls++;
+  if (ij == 0) st->cnt->intx_self++;
   z = el1;
   ze = el2;
   if (ij == 0) ze = 0.;
@@ -1000,6 +1011,7 @@ static void intx(fill_state* st, double el1, double el2, double b, int ij, doubl
   fnm = nma;
   ep = s / (10. * fnm);
   zend = ze - ep;
+  TALLY(st->cnt, 4, 1, 0, 0);
   *sgr = 0.;
   *sgi = 0.;
   ns = nx;
@@ -1010,8 +1022,10 @@ static void intx(fill_state* st, double el1, double el2, double b, int ij, doubl
       fns = ns;
       dz = s / fns;
       zp = z + dz;
+      TALLY(st->cnt, 2, 1, 0, 0);
       if (zp > ze) {
         dz = ze - z;
+        TALLY(st->cnt, 1, 0, 0, 0);
         if (fabs(dz) <= ep) goto done;
       }
       dzot = dz * .5;
@@ -1019,6 +1033,7 @@ static void intx(fill_state* st, double el1, double el2, double b, int ij, doubl
       gf(st, zp, &g3r, &g3i);
       zp = z + dz;
       gf(st, zp, &g5r, &g5i);
+      TALLY(st->cnt, 3, 0, 0, 0);
     }
     t00r = (g1r + g5r) * dzot;
     t00i = (g1i + g5i) * dzot;
@@ -1026,12 +1041,15 @@ static void intx(fill_state* st, double el1, double el2, double b, int ij, doubl
     t01i = (t00i + dz * g3i) * 0.5;
     t10r = (4.0 * t01r - t00r) / 3.0;
     t10i = (4.0 * t01i - t00i) / 3.0;
-    conv_test(t01r, t10r, &te1r, t01i, t10i, &te1i, 0.);
+    TALLY(st->cnt, 16, 2, 0, 0);
+    conv_test(st->cnt, t01r, t10r, &te1r, t01i, t10i, &te1i, 0.);
+    st->cnt->intx_steps++;
     if (tape_decide((te1i <= rx) && (te1r <= rx))) {
       *sgr = *sgr + t10r;
       *sgi = *sgi + t10i;
       nt += 2;
       z += dz;
+      TALLY(st->cnt, 3, 0, 0, 0);
       if (z >= zend) goto done;
       g1r = g5r;
       g1i = g5i;
@@ -1049,7 +1067,9 @@ static void intx(fill_state* st, double el1, double el2, double b, int ij, doubl
     t11i = (4.0 * t02i - t01i) / 3.0;
     t20r = (16.0 * t11r - t10r) / 15.0;
     t20i = (16.0 * t11i - t10i) / 15.0;
-    conv_test(t11r, t20r, &te2r, t11i, t20i, &te2i, 0.);
+    TALLY(st->cnt, 24, 4, 0, 0);
+    conv_test(st->cnt, t11r, t20r, &te2r, t11i, t20i, &te2i, 0.);
+    st->cnt->intx_refines++;
     const int bad5 = (te2i > rx) || (te2r > rx);
     if (tape_decide(bad5 && ns < nma)) {
       nt = 0;
@@ -1057,6 +1077,7 @@ static void intx(fill_state* st, double el1, double el2, double b, int ij, doubl
       fns = ns;
       dz = s / fns;
       dzot = dz * 0.5;
+      TALLY(st->cnt, 2, 1, 0, 0);
       g5r = g3r; g5i = g3i;
       g3r = g2r; g3i = g2i;
       flag = 0;
@@ -1067,6 +1088,7 @@ static void intx(fill_state* st, double el1, double el2, double b, int ij, doubl
     *sgi = *sgi + t20i;
     nt++;
     z += dz;
+    TALLY(st->cnt, 3, 0, 0, 0);
     if (z >= zend) goto done;
     g1r = g5r;
     g1i = g5i;
@@ -1077,6 +1099,7 @@ done:
   if (ij == 0) {
     *sgr = 2. * (*sgr + log((sqrt(b * b + s * s) + s) / b));
     *sgi = 2. * *sgi;
+    TALLY(st->cnt, 9, 1, 1, 1);
   }
 }
 
@@ -1090,6 +1113,7 @@ static void gx(fill_state* st, double zz, double rh, double xk, cx* gz, cx* gzp)
   temp = cdivr(temp, r);
   *gz = temp;
   *gzp = cdivr(cmul(cneg(mk(1.0, rkz)), temp), r2);
+  TALLY(st->cnt, 15, 4, 1, 2);
 }
 
 /* gxx, src/nec_context.cpp:3623-3673. */
@@ -1117,12 +1141,15 @@ static void gxx(fill_state* st, double zz, double rh, double a, double a2, doubl
   *gzp = cmul(cmulr(c2, t2), gz);
   *g3 = cadd(*g2p, *gzp);
   *g1p = cmulr(*g3, zz);
+  TALLY(st->cnt, 67, 6, 1, 2);
   if (ira != 1) {
     *g3 = cmulr(cadd(*g3, *gzp), rh);
     *gzp = cmul(cmulr(c1, -zz), gz);
+    TALLY(st->cnt, 12, 0, 0, 0);
     if (rh <= 1.0e-10) { *g2 = mk(0., 0.); *g2p = mk(0., 0.); return; }
     *g2 = cdivr(*g2, rh);
     *g2p = cdivr(cmulr(*g2p, zz), rh);
+    TALLY(st->cnt, 6, 4, 0, 0);
     return;
   }
   t2 = .5 * a;
@@ -1131,6 +1158,7 @@ static void gxx(fill_state* st, double zz, double rh, double a, double a2, doubl
   *g3 = csub(cmulr(*g2p, rh2), cmul(cmulr(gz, a), c1));
   *g2p = cmulr(*g2p, zz);
   *gzp = cmul(cmulr(c1, -zz), gz);
+  TALLY(st->cnt, 41, 2, 0, 0);
 }
 
 static const cx CONST1 = {0.0, 4.771341189};
@@ -1149,6 +1177,7 @@ static void eksc(fill_state* st, double s, double z, double rh, double xk, int i
   double cs = cos(shk);
   double z2a = sh - z;
   double z1a = -(sh + z);
+  TALLY(st->cnt, 7, 0, 0, 2);
   cx gz1, gz2, gp1, gp2;
   gx(st, z1a, rh, xk, &gz1, &gp1);
   gx(st, z2a, rh, xk, &gz2, &gp2);
@@ -1158,9 +1187,11 @@ static void eksc(fill_state* st, double s, double z, double rh, double xk, int i
   *ezs = cmul(CONST1, csub(cmulr(cmulr(csub(gz2, gz1), cs), xk), cmulr(cadd(gzp2, gzp1), ss)));
   *ezc = cmul(mc1, cadd(cmulr(cmulr(cadd(gz2, gz1), ss), xk), cmulr(csub(gzp2, gzp1), cs)));
   *erk = cmulr(cmul(CONST1, csub(gp2, gp1)), rh);
+  TALLY(st->cnt, 50, 0, 0, 0);
   double cint, sint;
   intx(st, -shk, shk, rhk, ij, &cint, &sint);
   *ezk = cmul(mc1, cadd(csub(gzp2, gzp1), cmulr(mk(cint, -sint), xk * xk)));
+  TALLY(st->cnt, 13, 0, 0, 0);
   if (rh >= 1.0e-10) {
     gzp1 = cmulr(gzp1, z1a);
     gzp2 = cmulr(gzp2, z2a);
@@ -1170,6 +1201,7 @@ static void eksc(fill_state* st, double s, double z, double rh, double xk, int i
     cx u1 = cmulr(csub(cadd(csub(gzp2, gzp1), gz2), gz1), cs);
     cx u2 = cmulr(cmulr(cadd(cmulr(gz2, z2a), cmulr(gz1, z1a)), ss), xk);
     *erc = cdivr(cmul(mc1, cadd(u1, u2)), rh);
+    TALLY(st->cnt, 60, 4, 0, 0);
     return;
   }
   *ers = mk(0., 0.);
@@ -1194,6 +1226,7 @@ static void ekscx(fill_state* st, double bx, double s, double z, double rhx, dou
   double z2a = sh - z;
   double z1a = -(sh + z);
   double a2 = b * b;
+  TALLY(st->cnt, 8, 0, 0, 2);
   cx gz1, gz2, gzp1, gzp2, gr1, gr2, grp1, grp2, grk1, grk2, gzz1, gzz2;
   if (inx1 != 2) gxx(st, z1a, rh, b, a2, xk, ira, &gz1, &gzp1, &gr1, &grp1, &grk1, &gzz1);
   else {
@@ -1203,6 +1236,7 @@ static void ekscx(fill_state* st, double bx, double s, double z, double rhx, dou
     grp1 = cdivr(gzp1, rhx);
     grk1 = cmulr(grk1, rhx);
     gzz1 = mk(0., 0.);
+    TALLY(st->cnt, 8, 4, 0, 0);
   }
   if (inx2 != 2) gxx(st, z2a, rh, b, a2, xk, ira, &gz2, &gzp2, &gr2, &grp2, &grk2, &gzz2);
   else {
@@ -1212,6 +1246,7 @@ static void ekscx(fill_state* st, double bx, double s, double z, double rhx, dou
     grp2 = cdivr(gzp2, rhx);
     grk2 = cmulr(grk2, rhx);
     gzz2 = mk(0., 0.);
+    TALLY(st->cnt, 8, 4, 0, 0);
   }
   cx mc1 = cneg(CONST1);
   *ezs = cmul(CONST1, csub(cmulr(cmulr(csub(gz2, gz1), cs), xk), cmulr(cadd(gzp2, gzp1), ss)));
@@ -1225,12 +1260,14 @@ static void ekscx(fill_state* st, double bx, double s, double z, double rhx, dou
     *erc = cmul(mc1, cadd(u1, u2));
   }
   *erk = cmul(CONST1, csub(grk2, grk1));
+  TALLY(st->cnt, 104, 0, 0, 0);
   double cint, sint;
   intx(st, -shk, shk, rhk, ij, &cint, &sint);
   double bk = b * xk;
   double bk2 = bk * bk * .25;
   cx term = cmulr(mk(cint, -sint), (xk * xk) * (1. - bk2));
   *ezk = cmul(mc1, csub(cadd(csub(gzp2, gzp1), term), cmulr(csub(gzz2, gzz1), bk2)));
+  TALLY(st->cnt, 24, 0, 0, 0);
 }
 
 /* efld_compute, src/nec_context.cpp:2777-2837 (direct field only). */
@@ -1243,8 +1280,9 @@ static double efld_compute(fill_state* st, double xij, double yij, double ai, do
   double rhoz = zij - salpr * zp;
   double rh = sqrt(rhox * rhox + rhoy * rhoy + rhoz * rhoz + ai * ai);
   if (rh <= 1.e-10) { rhox = 0.0; rhoy = 0.0; rhoz = 0.0; }
-  else { rhox /= rh; rhoy /= rh; rhoz /= rh; }
+  else { rhox /= rh; rhoy /= rh; rhoz /= rh; TALLY(st->cnt, 3, 3, 0, 0); }
   double r = sqrt(zp * zp + rh * rh);
+  TALLY(st->cnt, 23, 0, 2, 0);
   if (tape_decide(r >= st->rkh)) {
     st->cnt->far_pairs++;
     double eta = ora_impedance();
@@ -1262,6 +1300,7 @@ static double efld_compute(fill_state* st, double xij, double yij, double ai, do
     terc = cmulr(terk, rmag);
     tezk = cmulr(tezk, st->s);
     terk = cmulr(terk, st->s);
+    TALLY(st->cnt, 53, 9, 0, 3);
     egnd[3] = mk(0, 0); egnd[4] = mk(0, 0); egnd[5] = mk(0, 0);
   } else {
     st->cnt->near_pairs++;
@@ -1272,6 +1311,7 @@ static double efld_compute(fill_state* st, double xij, double yij, double ai, do
     egnd[3] = cadd(cmulr(tezs, cabj), cmulr(ters, rhox));
     egnd[4] = cadd(cmulr(tezs, sabj), cmulr(ters, rhoy));
     egnd[5] = cadd(cmulr(tezs, salpr), cmulr(ters, rhoz));
+    TALLY(st->cnt, 18, 0, 0, 0);
   }
   egnd[0] = cadd(cmulr(tezk, cabj), cmulr(terk, rhox));
   egnd[1] = cadd(cmulr(tezk, sabj), cmulr(terk, rhoy));
@@ -1279,6 +1319,7 @@ static double efld_compute(fill_state* st, double xij, double yij, double ai, do
   egnd[6] = cadd(cmulr(tezc, cabj), cmulr(terc, rhox));
   egnd[7] = cadd(cmulr(tezc, sabj), cmulr(terc, rhoy));
   egnd[8] = cadd(cmulr(tezc, salpr), cmulr(terc, rhoz));
+  TALLY(st->cnt, 36, 0, 0, 0);
   return r;
 }
 
@@ -1371,7 +1412,8 @@ double ora_efld(double xj, double yj, double zj, double cabj, double sabj, doubl
 
 static double complex to_c(cx a) { return CMPLX(a.re, a.im); }
 static cx from_c(double complex a) { return mk(creal(a), cimag(a)); }
-static void ground_image(const ora_geom* g, double xij, double yij, double zij, double r, cx t[9], cx e[9]);
+static void ground_image(const ora_geom* g, double xij, double yij, double zij, double r, cx t[9], cx e[9],
+                         ora_fill_counts* cnt);
 
 /* nefld, src/nec_context.cpp:4304-4450, for wires (no patches): the
  * electric field at npts points (xyz, in wavelengths) of the current
@@ -1416,7 +1458,7 @@ int ora_near_e(const ora_geom* g, const double* abc, int64_t npts, const double*
         cx t[9];
         const double zij = zob + st.zj;
         const double r = efld_compute(&st, xob - st.xj, yob - st.yj, ax, -st.salpj, zij, 1, t);
-        ground_image(g, xob - st.xj, yob - st.yj, zij, r, t, e);
+        ground_image(g, xob - st.xj, yob - st.yj, zij, r, t, e, NULL);
       }
       const cx acx = mk(air[i], aii[i]), bcx = mk(bir[i], bii[i]), ccx = mk(cir[i], cii[i]);
       ex = cadd(ex, cadd(cadd(cmul(e[0], acx), cmul(e[3], bcx)), cmul(e[6], ccx)));
@@ -1480,7 +1522,7 @@ static void hfk(double el1, double el2, double rhk, double zpkx, double* sgr, do
     const double t01r = (t00r + dz * g3r) * 0.5, t01i = (t00i + dz * g3i) * 0.5;
     const double t10r = (4.0 * t01r - t00r) / 3.0, t10i = (4.0 * t01i - t00i) / 3.0;
     double te1r, te1i;
-    conv_test(t01r, t10r, &te1r, t01i, t10i, &te1i, 0.);
+    conv_test(NULL, t01r, t10r, &te1r, t01i, t10i, &te1i, 0.);
     double addr, addi;
     if ((te1i <= rx) && (te1r <= rx)) {
       addr = t10r;
@@ -1493,7 +1535,7 @@ static void hfk(double el1, double el2, double rhk, double zpkx, double* sgr, do
       const double t11r = (4.0 * t02r - t01r) / 3.0, t11i = (4.0 * t02i - t01i) / 3.0;
       const double t20r = (16.0 * t11r - t10r) / 15.0, t20i = (16.0 * t11i - t10i) / 15.0;
       double te2r, te2i;
-      conv_test(t11r, t20r, &te2r, t11i, t20i, &te2i, 0.);
+      conv_test(NULL, t11r, t20r, &te2r, t11i, t20i, &te2i, 0.);
       const int bad = (te2i > rx) || (te2r > rx);
       if (bad && ns < nma) {
         /* halve the step and try again from z with the points in hand */
@@ -1700,7 +1742,8 @@ int ora_near_h(const ora_geom* g, const double* abc, int64_t npts, const double*
  * image fields (efld_compute with zij = zi + zj and -salpj); for the finite
  * ground they are weighted by the reflection coefficients of the specular
  * point (no radial-wire screen), then subtracted from e times frati. */
-static void ground_image(const ora_geom* g, double xij, double yij, double zij, double r, cx t[9], cx e[9]) {
+static void ground_image(const ora_geom* g, double xij, double yij, double zij, double r, cx t[9], cx e[9],
+                         ora_fill_counts* cnt) {
   if (g->gtype == 2) {
     double complex zratx = CMPLX(g->zr_re, g->zr_im);
     double rmag = r;
@@ -1714,7 +1757,11 @@ static void ground_image(const ora_geom* g, double xij, double yij, double zij, 
       py = xij / xymag;
       cth = zij / rmag;
       zrsin = csqrt(1.0 - zratx * zratx * (1.0 - cth * cth));
+      /* a complex square root tallied as 9 flops (2 square roots, 1 division)
+       * and a complex division as 9 (3 divisions) */
+      if (cnt) TALLY(cnt, 23, 4, 2, 0);
     }
+    if (cnt) TALLY(cnt, 4 + 23 + 17 + 2 + 132, 6, 1, 0);
     double complex refs = (cth - zratx * zrsin) / (cth + zratx * zrsin);
     double complex refps = -(zratx * cth - zrsin) / (zratx * cth + zrsin);
     refps = refps - refs;
@@ -1730,6 +1777,7 @@ static void ground_image(const ora_geom* g, double xij, double yij, double zij, 
   }
   cx fr = mk(g->fr_re, g->fr_im);
   for (int q = 0; q < 9; q++) e[q] = csub(e[q], cmul(t[q], fr));
+  if (cnt) TALLY(cnt, 72, 0, 0, 0);
 }
 
 int ora_fill(const ora_geom* g, double rkh, int use_ek, double* Z, int nthreads, ora_fill_counts* counts) {
@@ -1773,15 +1821,18 @@ int ora_fill(const ora_geom* g, double rkh, int use_ek, double* Z, int nthreads,
           cx e[9];
           local.pairs++;
           efld_compute(&st, xi - st.xj, yi - st.yj, ai, st.salpj, zi - st.zj, i != j, e);
+          TALLY(&local, 3, 0, 0, 0);
           if (g->gtype != 0) {
             cx t[9];
             double zij = zi + st.zj;
             double r = efld_compute(&st, xi - st.xj, yi - st.yj, ai, -st.salpj, zij, 1, t);
-            ground_image(g, xi - st.xj, yi - st.yj, zij, r, t, e);
+            ground_image(g, xi - st.xj, yi - st.yj, zij, r, t, e, &local);
+            TALLY(&local, 5, 0, 0, 0);
           }
           cx etk = cadd(cadd(cmulr(e[0], cabi), cmulr(e[1], sabi)), cmulr(e[2], salpi));
           cx ets = cadd(cadd(cmulr(e[3], cabi), cmulr(e[4], sabi)), cmulr(e[5], salpi));
           cx etc = cadd(cadd(cmulr(e[6], cabi), cmulr(e[7], sabi)), cmulr(e[8], salpi));
+          TALLY(&local, 30, 0, 0, 0);
           const int32_t* jj = tj + j * TRIO_CAP;
           for (int q = 0; q < tn[j]; q++) {
             int64_t k = jj[q] - 1;
@@ -1790,6 +1841,7 @@ int ora_fill(const ora_geom* g, double rkh, int use_ek, double* Z, int nthreads,
             row[2 * k] += t.re;
             row[2 * k + 1] += t.im;
             local.scatter_terms++;
+            TALLY(&local, 12, 0, 0, 0);
           }
           /* loading of segment j, right after its own scatter
            * (cmset, src/nec_context.cpp:2126-2142): row j, bases of trio(j) */
@@ -1800,6 +1852,7 @@ int ora_fill(const ora_geom* g, double rkh, int use_ek, double* Z, int nthreads,
               double sab = ta[j * TRIO_CAP + q] + tc[j * TRIO_CAP + q];
               row[2 * k] -= sab * zr;
               row[2 * k + 1] -= sab * zi;
+              TALLY(&local, 5, 0, 0, 0);
             }
           }
         }
@@ -1809,6 +1862,10 @@ int ora_fill(const ora_geom* g, double rkh, int use_ek, double* Z, int nthreads,
         total.pairs += local.pairs; total.far_pairs += local.far_pairs; total.near_pairs += local.near_pairs;
         total.gx_calls += local.gx_calls; total.gxx_calls += local.gxx_calls; total.intx_calls += local.intx_calls;
         total.gf_calls += local.gf_calls; total.scatter_terms += local.scatter_terms;
+        total.intx_steps += local.intx_steps; total.intx_refines += local.intx_refines;
+        total.intx_self += local.intx_self;
+        total.flops += local.flops; total.divs += local.divs; total.sqrts += local.sqrts;
+        total.specials += local.specials;
       }
     }
   }
@@ -1938,6 +1995,9 @@ int ora_current_coefficients(const ora_geom* g, double* cur, double* abc) {
 /* Far field                                                           */
 /* ------------------------------------------------------------------ */
 
+/* When set, ffld_sum tallies its operations here (ora_pattern_tally). */
+static _Thread_local ora_fill_counts* ffld_tally;
+
 /* The segment sum of ffld, src/nec_context.cpp:6706-6742, for the
  * direction cosines (rox, roy, roz). */
 static void ffld_sum(const ora_geom* g, const double* abc, double rox, double roy, double roz, cx* cix, cx* ciy,
@@ -1958,6 +2018,13 @@ static void ffld_sum(const ora_geom* g, const double* abc, double rox, double ro
     else too = 1. - top * top / 6.;
     if (fabs(bot) >= 1.0e-7) boo = sin(bot) / bot;
     else boo = 1. - bot * bot / 6.;
+    if (ffld_tally) {
+      TALLY(ffld_tally, 9 + 38, 0, 0, 2);
+      if (fabs(omega) >= 1.0e-7) TALLY(ffld_tally, 2, 1, 0, 1); else TALLY(ffld_tally, 6, 1, 0, 0);
+      if (fabs(top) >= 1.0e-7) TALLY(ffld_tally, 1, 1, 0, 1); else TALLY(ffld_tally, 3, 1, 0, 0);
+      if (fabs(bot) >= 1.0e-7) TALLY(ffld_tally, 1, 1, 0, 1); else TALLY(ffld_tally, 3, 1, 0, 0);
+      ffld_tally->pairs++;
+    }
     double b = el * (boo - too);
     double c = el * (boo + too);
     double rr = a * air[i] + b * bii[i] + c * cir[i];
@@ -2208,6 +2275,47 @@ int ora_pattern(const ora_geom* g, const double* abc, double wavelength, double 
   if (!out->avg_scale) free(scbuf);
   free(pintv); free(thv); free(phv);
   return 0;
+}
+
+/* The operation tally of the pattern's segment sums (ffld_sum) over the
+ * directions of rp, in the convention of ora_fill_counts: pairs counts the
+ * (direction, segment) terms; the per-direction work around the sums is not
+ * tallied.  Directions below the horizon over a ground are skipped as in
+ * ora_pattern. */
+void ora_pattern_tally(const ora_geom* g, const double* abc, const ora_rp_spec* rp, int nthreads,
+                       ora_fill_counts* out) {
+  int nth = rp->n_theta, nph = rp->n_phi;
+  double* thv = (double*)malloc((size_t)nth * sizeof(double));
+  double* phv = (double*)malloc((size_t)nph * sizeof(double));
+  double phi = rp->phi0 - rp->dphi;
+  for (int k = 0; k < nph; k++) { phi += rp->dphi; phv[k] = phi; }
+  double thet = rp->theta0 - rp->dtheta;
+  for (int k = 0; k < nth; k++) { thet += rp->dtheta; thv[k] = thet; }
+  ora_fill_counts total;
+  memset(&total, 0, sizeof total);
+  if (nthreads < 1) nthreads = 1;
+#pragma omp parallel num_threads(nthreads)
+  {
+    ora_fill_counts local;
+    memset(&local, 0, sizeof local);
+    ffld_tally = &local;
+#pragma omp for schedule(dynamic, 16)
+    for (int64_t idx = 0; idx < (int64_t)nth * nph; idx++) {
+      int kph = (int)(idx / nth), kth = (int)(idx % nth);
+      if (g->gtype != 0 && thv[kth] > 90.01) continue;
+      double pha = deg_to_rad(phv[kph]), tha = deg_to_rad(thv[kth]);
+      double e_t[2], e_p[2];
+      ora_ffld(g, abc, sin(tha), cos(tha), sin(pha), cos(pha), e_t, e_p);
+    }
+    ffld_tally = NULL;
+#pragma omp critical
+    {
+      total.pairs += local.pairs; total.flops += local.flops; total.divs += local.divs;
+      total.sqrts += local.sqrts; total.specials += local.specials;
+    }
+  }
+  free(thv); free(phv);
+  *out = total;
 }
 
 static double now_s(void) {

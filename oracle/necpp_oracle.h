@@ -121,6 +121,17 @@ typedef struct {
   int64_t intx_calls;
   int64_t gf_calls;
   int64_t scatter_terms;/* sum over pairs of jsno(j)                      */
+  int64_t intx_steps;   /* intx: three-point estimates (one per step tried) */
+  int64_t intx_refines; /* intx: five-point estimates (steps not accepted at three) */
+  int64_t intx_self;    /* intx calls for the self term (ij = 0)           */
+  /* Operation tally, kept per basic block: each +, -, *, / and square root
+   * executed is one flop (divisions and square roots are also tallied
+   * alone); each sin, cos and log is one "special" and no flop.  Negation,
+   * fabs, comparisons and expressions of constants count nothing. */
+  int64_t flops;
+  int64_t divs;
+  int64_t sqrts;
+  int64_t specials;
 } ora_fill_counts;
 
 /* nec_context::cmset / cmww / efld / efld_compute / eksc / ekscx / gf / gx /
@@ -218,6 +229,12 @@ typedef struct {
  * nthreads > 1 splits directions over OpenMP threads. */
 int ora_pattern(const ora_geom* g, const double* abc, double wavelength, double pinr, double pnlr,
                 const ora_rp_spec* rp, ora_rp_out* out, int nthreads);
+
+/* Operation tally of ora_pattern's segment sums over rp's directions (pairs
+ * = direction-segment terms; flops, divs, sqrts, specials as in
+ * ora_fill_counts; the other fields stay zero). */
+void ora_pattern_tally(const ora_geom* g, const double* abc, const ora_rp_spec* rp, int nthreads,
+                       ora_fill_counts* out);
 
 /* One complete voltage-source run at one frequency, as XQ/RP would do it:
  * frequency_scale, fill, factor, RHS (etmns_voltage_source), solve, current

@@ -11,7 +11,7 @@ import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB_PATH = os.path.join(_HERE, "liboracle.so")
-_lib = None
+_libs = {}
 
 
 def build():
@@ -19,12 +19,14 @@ def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
 
-def lib():
-    global _lib
-    if _lib is None:
-        if not os.path.exists(_LIB_PATH):
-            raise RuntimeError("oracle/liboracle.so is not built (run make -C oracle)")
-        L = ctypes.CDLL(_LIB_PATH)
+def lib(path=None):
+    """The restatement's library (oracle/liboracle.so, or another build of
+    the same source at path), loaded and typed once per path."""
+    path = path or _LIB_PATH
+    if path not in _libs:
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} is not built (run make -C oracle)")
+        L = ctypes.CDLL(path)
         d, i, i64, p = ctypes.c_double, ctypes.c_int, ctypes.c_int64, ctypes.c_void_p
         L.ora_last_error.restype = ctypes.c_char_p
         L.ora_geom_new.restype = p
@@ -56,6 +58,7 @@ def lib():
         L.ora_current_coefficients.argtypes = [p, p, p]
         L.ora_ffld.argtypes = [p, p, d, d, d, d, p, p]
         L.ora_pattern.argtypes = [p, p, d, d, d, p, p, i]
+        L.ora_pattern_tally.argtypes = [p, p, p, i, p]
         L.ora_run_voltage.argtypes = [p, d, d, i, i, p, p, p, p, p, p, p, p, i, p]
         L.ora_eksc.argtypes = [d, d, d, i, p]
         L.ora_ekscx.argtypes = [d, d, d, d, i, i, i, p]
@@ -68,8 +71,8 @@ def lib():
         L.ora_set_networks.restype = i
         L.ora_efld.argtypes = [d] * 12 + [i, d, i, i, i, p]
         L.ora_efld.restype = d
-        _lib = L
-    return _lib
+        _libs[path] = L
+    return _libs[path]
 
 
 def _ptr(a):
@@ -91,7 +94,8 @@ def tape_stats():
 class FillCounts(ctypes.Structure):
     _fields_ = [(n, ctypes.c_int64) for n in
                 ("pairs", "far_pairs", "near_pairs", "gx_calls", "gxx_calls",
-                 "intx_calls", "gf_calls", "scatter_terms")]
+                 "intx_calls", "gf_calls", "scatter_terms", "intx_steps", "intx_refines", "intx_self",
+                 "flops", "divs", "sqrts", "specials")]
 
 
 class RpSpec(ctypes.Structure):
@@ -164,8 +168,8 @@ def wavelength(f_mhz):
 class Geometry:
     """Wire geometry held by the oracle (c_geometry restated)."""
 
-    def __init__(self):
-        self._L = lib()
+    def __init__(self, lib_path=None):
+        self._L = lib(lib_path)
         self._g = self._L.ora_geom_new()
 
     def __del__(self):
@@ -346,6 +350,16 @@ class Geometry:
         res = {"pattern": pat}
         _rp_finish(res, out)
         return res
+
+    def pattern_tally(self, abc, f_mhz, rp, nthreads=1):
+        """The operation tally of the pattern's segment sums over rp's
+        directions (ora_pattern_tally): pairs, flops, divs, sqrts, specials."""
+        self._L.ora_frequency_scale(self._g, f_mhz)
+        spec, _, _ = _rp_buffers(rp)
+        abc = np.ascontiguousarray(abc, np.float64)
+        cnt = FillCounts()
+        self._L.ora_pattern_tally(self._g, _ptr(abc), ctypes.byref(spec), nthreads, ctypes.byref(cnt))
+        return {k: getattr(cnt, k) for k in ("pairs", "flops", "divs", "sqrts", "specials")}
 
 
 def _rp_buffers(rp):
