@@ -469,7 +469,8 @@ def main():
     else:
         roof = {"bound": None, "achieved": None, "peak": None, "unit": None, "frac": None}
     counts = op_counts(workload_tag)
-    fill_kernel_ms = sum(kernels[k]["ms_per_step"] for k in ("fill_fields", "fill_assemble") if k in kernels)
+    fill_kernel_ms = sum(kernels[k]["ms_per_step"] for k in ("fill_fields", "fill_assemble", "fill_tile")
+                         if k in kernels)
     fill_stage = fill_rates(counts, fill_kernel_ms, fill_ms, workload_tag) if fill_kernel_ms > 0 else None
     pat_kernel_ms = kernels.get("pattern_partial", {}).get("ms_per_step")
     pattern_stage = pattern_rates(counts, n, pat_kernel_ms, workload_tag) if pat_kernel_ms else None

@@ -315,6 +315,50 @@ void DeviceSolver::load(const Geometry& geo, const ScaledSegments& seg, bool use
   for (int p = 0; p < n; ++p) pos[ord[p]] = p;
   for (size_t q = 0; q < t.colj.size(); ++q) cjp[q] = pos[t.colj[q]];
 
+  // Clusters of the one-pass fill: runs of T bases of the Morton order (a
+  // compact patch of the structure), each with the sorted list of every
+  // source its bases' supports draw on.  T = 1024 (NECGPU_FILL_TILE
+  // overrides it, for measurements): on the planar grids a source list is
+  // then 1.11-1.14 times the cluster, the fields evaluated twice at the
+  // clusters' edges.
+  {
+    const char* tv = std::getenv("NECGPU_FILL_TILE");
+    const int T = std::max(8, (tv && *tv) ? std::atoi(tv) : 1024);
+    std::vector<int> srcp{0}, src, basep{0}, base, cjl(t.colj.size());
+    int smax = 0;
+    for (int p0 = 0; p0 < n; p0 += T) {
+      const int p1 = std::min(n, p0 + T);
+      std::vector<int> bases(ord.begin() + p0, ord.begin() + p1);
+      std::sort(bases.begin(), bases.end());
+      std::vector<int> s;
+      for (int k : bases)
+        for (int q = t.colptr[k]; q < t.colptr[k + 1]; ++q) s.push_back(t.colj[q]);
+      std::sort(s.begin(), s.end());
+      s.erase(std::unique(s.begin(), s.end()), s.end());
+      for (int k : bases)
+        for (int q = t.colptr[k]; q < t.colptr[k + 1]; ++q)
+          cjl[q] = static_cast<int>(std::lower_bound(s.begin(), s.end(), t.colj[q]) - s.begin());
+      smax = std::max(smax, static_cast<int>(s.size()));
+      src.insert(src.end(), s.begin(), s.end());
+      srcp.push_back(static_cast<int>(src.size()));
+      base.insert(base.end(), bases.begin(), bases.end());
+      basep.push_back(static_cast<int>(base.size()));
+    }
+    fill_clusters_ = static_cast<int>(basep.size()) - 1;
+    fill_smax_ = smax;
+    fill_rb_ = 0;
+    for (int rb : {4, 2, 1})
+      if ((size_t)rb * smax * 48 <= kFillTileLdsMax) {
+        fill_rb_ = rb;
+        break;
+      }
+    upload(cl_src_ptr_, srcp, stream_);
+    upload(cl_src_, src, stream_);
+    upload(cl_base_ptr_, basep, stream_);
+    upload(cl_base_, base, stream_);
+    upload(cjl_, cjl, stream_);
+  }
+
   upload(seg_, segv, stream_);
   upload(ford_, ord, stream_);
   upload(fcjp_, cjp, stream_);
@@ -368,6 +412,18 @@ void DeviceSolver::enqueue_fill(double rkh) {
   const int n = n_;
   last_rkh_ = rkh;
   Z_.ensure((size_t)ld_ * n * 16);
+  // one pass with the fields in LDS (NECGPU_FILL_FUSED=0 keeps the field
+  // buffer and the gather, for measurements)
+  const char* fv = std::getenv("NECGPU_FILL_FUSED");
+  if (fill_rb_ > 0 && !(fv && fv[0] == '0')) {
+    if (launch_fill_tile(dev_segments(), n, rkh, use_ek_ ? 1 : 0, ekind_.as<int>(), ground_,
+                         has_loads_ ? zload_dev_.as<double>() : nullptr, fill_clusters_, cl_src_ptr_.as<int>(),
+                         cl_src_.as<int>(), cl_base_ptr_.as<int>(), cl_base_.as<int>(), colptr_.as<int>(),
+                         colj_.as<int>(), cjl_.as<int>(), colabc_.as<double>(), fill_smax_, fill_rb_, Z_.as<double>(),
+                         ld_, stream_))
+      hip_check(hipGetLastError(), "fill launch");
+    return;
+  }
   const int rows = static_cast<int>(std::min<size_t>(n, std::max<size_t>(1, kFieldBudget / ((size_t)n * 48))));
   E_.ensure((size_t)rows * n * 48);
   const DevSegments g = dev_segments();

@@ -204,6 +204,10 @@ class DeviceSolver {
 
   DevBuf seg_, ekind_, colptr_, colj_, colabc_, segptr_, segbasis_, segabc_;
   DevBuf ford_, fcjp_;  // the fill's field order and the column CSR's sources in it
+  // the one-pass fill's clusters of bases (fill_tile_kernel); fill_rb_ = 0
+  // when the longest cluster source list does not fit the LDS
+  DevBuf cl_src_ptr_, cl_src_, cl_base_ptr_, cl_base_, cjl_;
+  int fill_clusters_ = 0, fill_smax_ = 0, fill_rb_ = 0;
   DevBuf Z_, E_, piv_, info_, lu_work_, x_, abc_, curx_, solve_work_, rhs_, src_, zin_;
   DevBuf trig_, pat_part_, pat_out_, pat_sense_;
   DevBuf Z32_, b_, r_, acc_;  // mixed path: FP32 factors, right-hand side, residual, norms

@@ -642,6 +642,97 @@ __global__ void __launch_bounds__(256) fill_assemble_kernel(const double* __rest
     }
 }
 
+// The fill in one pass.  A workgroup takes RB consecutive observation rows
+// and one cluster of bases: segments consecutive in the Morton order of
+// their centres, listed in ascending index (cl_base), together with every
+// source their supports draw on (cl_src, ascending; a basis' support entry
+// q finds its source at position cjl[q] of that list).  The fields of the
+// rows against the cluster's sources go to LDS (six planes of RB x smax
+// doubles), then each entry Z[i][k] gathers them over k's support in
+// ascending source order with the arithmetic of fill_assemble_kernel, and
+// the load of segment i right after its own term.  Sources shared by two
+// clusters are evaluated once for each; Z is written once and the fields
+// never leave the CU.
+template <bool GROUND, int RB>
+__global__ void __launch_bounds__(256) fill_tile_kernel(DevSegments g, int n, double rkh, int use_ek,
+                                                        const int* __restrict__ ekind, GroundSpec gs,
+                                                        const int* __restrict__ cl_src_ptr,
+                                                        const int* __restrict__ cl_src,
+                                                        const int* __restrict__ cl_base_ptr,
+                                                        const int* __restrict__ cl_base,
+                                                        const int* __restrict__ colptr, const int* __restrict__ colj,
+                                                        const int* __restrict__ cjl,
+                                                        const double* __restrict__ colabc,
+                                                        const double* __restrict__ zload, int smax,
+                                                        double* __restrict__ Z, long long ldz) {
+  extern __shared__ double fe[];  // [6][RB][smax]
+  const int q = blockIdx.x;
+  const int i0 = blockIdx.y * RB;
+  const int nr = min(RB, n - i0);
+  const int s0 = cl_src_ptr[q];
+  const int ns = cl_src_ptr[q + 1] - s0;
+  const size_t pl = (size_t)RB * smax;
+  for (int e = threadIdx.x; e < nr * ns; e += blockDim.x) {
+    const int r = e / ns, sidx = e - r * ns;
+    const int i = i0 + r, j = cl_src[s0 + sidx];
+    int ind1 = 2, ind2 = 2;
+    if (use_ek) {
+      ind1 = ekind[2 * j];
+      ind2 = ekind[2 * j + 1];
+    }
+    zc etk, ets, etc;
+    if (GROUND) pair_fields_ground(g, i, j, rkh, use_ek, ind1, ind2, gs, &etk, &ets, &etc);
+    else pair_fields(g, i, j, rkh, use_ek, ind1, ind2, &etk, &ets, &etc);
+    double* o = fe + (size_t)r * smax + sidx;
+    o[0] = etk.re;
+    o[pl] = etk.im;
+    o[2 * pl] = ets.re;
+    o[3 * pl] = ets.im;
+    o[4 * pl] = etc.re;
+    o[5 * pl] = etc.im;
+  }
+  __syncthreads();
+  const int b0 = cl_base_ptr[q];
+  const int nbq = cl_base_ptr[q + 1] - b0;
+  for (int e = threadIdx.x; e < nbq; e += blockDim.x) {
+    const int k = cl_base[b0 + e];
+    double accr[RB], acci[RB];
+#pragma unroll
+    for (int u = 0; u < RB; ++u) accr[u] = acci[u] = 0.0;
+    const int q1 = colptr[k + 1];
+    for (int qq = colptr[k]; qq < q1; ++qq) {
+      const int sidx = cjl[qq];
+      const double a = colabc[3 * qq], b = colabc[3 * qq + 1], c = colabc[3 * qq + 2];
+#pragma unroll
+      for (int u = 0; u < RB; ++u) {
+        if (u < nr) {
+          const double* f = fe + (size_t)u * smax + sidx;
+          // etk*a + ets*b + etc*c, then Z += term
+          const double tr = ((f[0] * a) + (f[2 * pl] * b)) + (f[4 * pl] * c);
+          const double ti = ((f[pl] * a) + (f[3 * pl] * b)) + (f[5 * pl] * c);
+          accr[u] += tr;
+          acci[u] += ti;
+          if (zload && colj[qq] == i0 + u) {
+            // the load of segment i, right after its own term
+            // (src/nec_context.cpp:2126-2142): -= (A + C) z
+            const int j = colj[qq];
+            const double sl = a + c;
+            accr[u] -= sl * zload[2 * j];
+            acci[u] -= sl * zload[2 * j + 1];
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < RB; ++u)
+      if (u < nr) {
+        double* zp = Z + 2 * ((size_t)(i0 + u) * ldz + k);
+        zp[0] = accr[u];
+        zp[1] = acci[u];
+      }
+  }
+}
+
 // Rows per thread of the fill assembly: 4, or 8 above 8,192 segments
 // (NECGPU_FILL_RB = 1, 2, 4 or 8 overrides, for measurements).  With the
 // Morton field order, the 3,280-segment grid assembles in 510 us with 4
@@ -1358,6 +1449,37 @@ int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int u
   else
     launch_k(K_FILL_ASSEMBLE, fill_assemble_kernel<1>, agrid, block, 0, stream, E, n, i0, rows, colptr, colj, cjp,
              ord, colabc, zload, Z, ldz);
+  return hipGetLastError() == hipSuccess ? 0 : 1;
+}
+
+// The one-pass fill (fill_tile_kernel) over all rows; rb rows per
+// workgroup (1, 2 or 4), smax the longest cluster source list.
+int launch_fill_tile(const DevSegments& g, int n, double rkh, int use_ek, const int* ekind, const GroundSpec& ground,
+                     const double* zload, int nclusters, const int* cl_src_ptr, const int* cl_src,
+                     const int* cl_base_ptr, const int* cl_base, const int* colptr, const int* colj, const int* cjl,
+                     const double* colabc, int smax, int rb, double* Z, long long ldz, hipStream_t stream) {
+  if (n <= 0 || nclusters <= 0) return 0;
+  const size_t lds = (size_t)6 * rb * smax * sizeof(double);
+  const dim3 grid(nclusters, (n + rb - 1) / rb), block(256);
+  auto go = [&](auto kern) {
+    static bool attr = false;  // one attribute per instantiation
+    if (!attr) {
+      if (hipFuncSetAttribute(reinterpret_cast<const void*>(kern), hipFuncAttributeMaxDynamicSharedMemorySize,
+                              (int)kFillTileLdsMax) != hipSuccess)
+        return 1;
+      attr = true;
+    }
+    launch_k(K_FILL_TILE, kern, grid, block, (uint32_t)lds, stream, g, n, rkh, use_ek, ekind, ground, cl_src_ptr,
+             cl_src, cl_base_ptr, cl_base, colptr, colj, cjl, colabc, zload, smax, Z, ldz);
+    return 0;
+  };
+  int rc;
+  if (ground.type != 0)
+    rc = rb == 4 ? go(fill_tile_kernel<true, 4>) : rb == 2 ? go(fill_tile_kernel<true, 2>) : go(fill_tile_kernel<true, 1>);
+  else
+    rc = rb == 4 ? go(fill_tile_kernel<false, 4>) : rb == 2 ? go(fill_tile_kernel<false, 2>)
+                                                    : go(fill_tile_kernel<false, 1>);
+  if (rc) return rc;
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 

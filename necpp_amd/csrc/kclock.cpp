@@ -12,7 +12,7 @@ const char* kernel_kind_name(int kind) {
                                         "pattern_partial", "pattern_finish", "refine", "residual",
                                         "batch_fill", "batch_lu", "batch_solve", "batch_pattern",
                                         "adjoint_fill", "adjoint_solve", "adjoint_reduce",
-                                        "lu_next", "near_field", "batch_tables"};
+                                        "lu_next", "near_field", "batch_tables", "fill_tile"};
   return (kind >= 0 && kind < K_NKINDS) ? names[kind] : "?";
 }
 

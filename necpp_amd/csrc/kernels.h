@@ -39,6 +39,7 @@ enum KernelKind {
   K_LU_NEXT,
   K_NEAR_FIELD,
   K_BATCH_TABLES,
+  K_FILL_TILE,
   K_NKINDS
 };
 const char* kernel_kind_name(int kind);
@@ -189,6 +190,17 @@ int launch_near_field(bool magnetic, const DevSegments& g, int n, const double* 
 int launch_fill(const DevSegments& g, int n, int i0, int rows, double rkh, int use_ek, const int* ekind,
                 const GroundSpec& ground, const double* zload, double* E, const int* colptr, const int* colj,
                 const int* cjp, const int* ord, const double* colabc, double* Z, long long ldz, hipStream_t stream);
+
+// The fill in one pass (no field buffer): workgroups of rb rows (1, 2 or 4)
+// and one cluster of bases each.  Cluster q: sources cl_src[cl_src_ptr[q] ..
+// cl_src_ptr[q+1]) (ascending), bases cl_base[cl_base_ptr[q] ..) (ascending);
+// cjl[e]: position, in its basis' cluster source list, of the source of
+// column-CSR entry e.  rb * smax * 48 bytes of LDS, at most kFillTileLdsMax.
+constexpr size_t kFillTileLdsMax = 64 * 1024;
+int launch_fill_tile(const DevSegments& g, int n, double rkh, int use_ek, const int* ekind, const GroundSpec& ground,
+                     const double* zload, int nclusters, const int* cl_src_ptr, const int* cl_src,
+                     const int* cl_base_ptr, const int* cl_base, const int* colptr, const int* colj, const int* cjl,
+                     const double* colabc, int smax, int rb, double* Z, long long ldz, hipStream_t stream);
 
 // lu.hip: in-place partial-pivot LU of the row-major complex n x n matrix A
 // (leading dimension lda complex).  piv (2n ints): piv[k] = row exchanged

@@ -231,6 +231,55 @@ def test_fill_matches_oracle(name, deck, ek):
         api.nec_delete(ctx)
 
 
+# a sloping wire and a horizontal one joined above ground, one end grounded
+BENT = [(1, 8, 0.0, 0.0, 0.0, 0.1, 0.0, 0.3, 0.002, 1.0, 1.0),
+        (2, 9, 0.1, 0.0, 0.3, 0.45, 0.2, 0.3, 0.002, 1.0, 1.0)]
+YAGI4 = decks.yagi([0.5, 0.48, 0.45, 0.44], [0.4, 0.3, 0.35], 0.003, 11, rp=None)
+ONE_PASS_CASES = {
+    "grid9": (decks.grid(9, rp=None), False, None, None),
+    "grid15_ek": (decks.grid(15, rp=None), True, None, None),
+    "yagi4_loads": (YAGI4, False, None, [(0, 2, 5, 7, 12.0, 2.0e-9, 8.0e-11), (4, 0, 0, 0, 3.0, 40.0, 0.0)]),
+    "bent_finite_ground": ({"wires": BENT, "gpflag": 1, "freq_mhz": 299.792458}, False, (0, 0, 13.0, 0.005), None),
+    "bent_perfect_ground_ek": ({"wires": BENT, "gpflag": 1, "freq_mhz": 299.792458}, True, (1, 0, 0.0, 0.0), None),
+}
+
+
+@pytest.mark.parametrize("case", sorted(ONE_PASS_CASES))
+def test_one_pass_fill_equals_field_buffer_fill(case, monkeypatch):
+    """The one-pass fill (fields of a cluster of bases in LDS) against the
+    field-buffer fill (fields to HBM, then the gather), bit for bit, with
+    clusters of 8 bases so that sources are shared between clusters; and
+    both against the oracle's fill."""
+    deck, ek, gn, loads = ONE_PASS_CASES[case]
+
+    def fill(fused):
+        monkeypatch.setenv("NECGPU_FILL_TILE", "8")
+        monkeypatch.setenv("NECGPU_FILL_FUSED", "1" if fused else "0")
+        ctx = make(deck, ek)
+        try:
+            if gn is not None:
+                assert api.nec_gn_card(ctx, gn[0], gn[1], gn[2], gn[3], 0, 0, 0, 0) == 0, api.nec_error_message()
+            for ld in loads or []:
+                assert api.nec_ld_card(ctx, *ld) == 0, api.nec_error_message()
+            return api.gpu_fill_matrix(ctx, deck["freq_mhz"])
+        finally:
+            api.nec_delete(ctx)
+
+    Z1, Z0 = fill(True), fill(False)
+    assert Z1.shape[0] > 8
+    assert np.array_equal(Z1.view(np.float64), Z0.view(np.float64))
+    g = po.geometry_from_deck(deck)
+    if gn is not None:
+        g.set_ground(gn[0], gn[2], gn[3])
+    if loads:
+        g.set_loads(loads)
+    g.scale(deck["freq_mhz"])
+    oZ, _ = g.fill(use_ek=ek, nthreads=4)
+    err = np.abs(Z1 - oZ)
+    bound = 1e-12 * np.abs(oZ) + 1e-15 * np.max(np.abs(oZ))
+    assert np.all(err <= bound), (np.max(err / (np.abs(oZ) + 1e-300)), np.count_nonzero(err > bound))
+
+
 # ---------------------------------------------------------------- end to end
 
 def test_example1_kat_gpu(kats):
