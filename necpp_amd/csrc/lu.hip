@@ -1069,12 +1069,16 @@ __global__ void __launch_bounds__(256) zgemm_update_kernel(double* __restrict__ 
 //   - writes the strips it prefetched into registers to LDS, stores the
 //     previous tile's result, and moves the prefetched C block into the
 //     accumulators,
-//   - runs 16 k-steps of 16 MFMAs per wave (32 x 32 complex per wave, the
-//     complex product in 4M form) with the accumulators counting down from
-//     C: acc_re += (-Lr) Ur + Li Ui, acc_im += (-Lr) Ui + (-Li) Ur; each
-//     k-step also issues three of the 48 loads of the next tile's strips and
-//     C block, so the loads trickle out beside the MFMAs instead of stalling
-//     them in one burst.
+//   - runs 16 k-steps of MFMAs per wave (32 x 32 complex per wave) with the
+//     accumulators counting down from C.  FP32: the complex product in 4M
+//     form, 16 MFMAs per k-step: acc_re += (-Lr) Ur + Li Ui, acc_im +=
+//     (-Lr) Ui + (-Li) Ur.  FP64: three real products (3M), 12 MFMAs per
+//     k-step: X += (-Lr) Ur, Y += Li Ui, Z += -(Lr + Li)(Ur + Ui) from
+//     X = Cr, Y = 0, Z = Ci + Cr, and the result Re = X + Y, Im = (Z - X)
+//     + Y (the FP32 kernel is not MFMA-bound: three products measured no
+//     faster there).  Each k-step also issues three of the 48 loads of the
+//     next tile's strips and C block, so the loads trickle out beside the
+//     MFMAs instead of stalling them in one burst.
 // Storing a tile's result after the next tile's operands are in LDS keeps
 // the waits for the prefetched loads from waiting on stores.  Loads and
 // stores are buffer operations: an element outside the matrix gets an offset
@@ -1102,6 +1106,7 @@ struct ZtTraits<double> {
   typedef dbl2_t C2;
   typedef double4_t Acc;
   static constexpr int kPerCU = 1;
+  static constexpr bool k3M = true;  // three real products per complex one
   __device__ static Acc mfma(double a, double b, Acc c) { return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0); }
   // FP64 16x16x4: register v of lane l holds D[(l / 16) + 4 v][l % 16]
   __device__ static int acc_row(int v, int lk) { return v * 4 + lk; }
@@ -1117,6 +1122,7 @@ struct ZtTraits<float> {
   typedef flt2_t C2;
   typedef float4_t Acc;
   static constexpr int kPerCU = 2;
+  static constexpr bool k3M = false;
   __device__ static Acc mfma(float a, float b, Acc c) { return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0); }
   // FP32 16x16x4: register v of lane l holds D[4 (l / 16) + v][l % 16]
   __device__ static int acc_row(int v, int lk) { return lk * 4 + v; }
@@ -1144,6 +1150,28 @@ __device__ inline void zt_coords(int idx, int tiles_x, int tiles_y, int* ty, int
 template <typename T>
 __device__ inline __amdgpu_buffer_rsrc_t zt_rsrc(const T* base, int range) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<T*>(base), (short)0, range, ZT_RSRC_W3);
+}
+
+// A tile's result from its accumulators (see zgemm_tile_kernel), stored
+// through the buffer resource rs at offsets os.
+template <typename Tr>
+__device__ __forceinline__ void zt_store_result(const typename Tr::Acc (&accr)[2][2], const typename Tr::Acc (&acci)[2][2],
+                                                const typename Tr::Acc (&accz)[2][2], __amdgpu_buffer_rsrc_t rs,
+                                                const unsigned (&os)[16]) {
+  typedef typename Tr::C2 C2;
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int v = 0; v < 4; ++v) {
+        if constexpr (Tr::k3M) {
+          const auto x = accr[a][b][v], y = acci[a][b][v], z = accz[a][b][v];
+          Tr::store((C2){x + y, (z - x) + y}, rs, os[(a * 2 + b) * 4 + v]);
+        } else {
+          Tr::store((C2){accr[a][b][v], acci[a][b][v]}, rs, os[(a * 2 + b) * 4 + v]);
+        }
+      }
 }
 
 template <typename T>
@@ -1204,13 +1232,18 @@ zgemm_tile_kernel(T* __restrict__ A, long long lda, int m0, int n0, int M, int N
     pc[i] = Tr::load(rc, oc[i]);
   }
   // The first pass stores through a zero-range resource (dropped).
-  Acc accr[2][2], acci[2][2];
+  // 4M: accr = Cr - sum Lr Ur + Li Ui, acci = Ci - sum Lr Ui + Li Ur.
+  // 3M: accr = X = Cr - sum Lr Ur, acci = Y = sum Li Ui,
+  //     accz = Z = Ci + Cr - sum (Lr + Li)(Ur + Ui); then the result is
+  //     Re = X + Y, Im = (Z - X) + Y.
+  Acc accr[2][2], acci[2][2], accz[2][2];
 #pragma unroll
   for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
       accr[a][b] = (Acc){T(0), T(0), T(0), T(0)};
       acci[a][b] = (Acc){T(0), T(0), T(0), T(0)};
+      accz[a][b] = (Acc){T(0), T(0), T(0), T(0)};
     }
   __amdgpu_buffer_rsrc_t rs = zt_rsrc(A, 0);
   unsigned os[16];
@@ -1223,19 +1256,19 @@ zgemm_tile_kernel(T* __restrict__ A, long long lda, int m0, int n0, int M, int N
       sL[lane * ZT_SP + wave * 16 + i] = pl[i];
       sU[(wave * 16 + i) * ZT_SP + lane] = pu[i];
     }
-#pragma unroll
-    for (int a = 0; a < 2; ++a)
-#pragma unroll
-      for (int b = 0; b < 2; ++b)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) Tr::store((C2){accr[a][b][v], acci[a][b][v]}, rs, os[(a * 2 + b) * 4 + v]);
+    zt_store_result<Tr>(accr, acci, accz, rs, os);
 #pragma unroll
     for (int a = 0; a < 2; ++a)
 #pragma unroll
       for (int b = 0; b < 2; ++b) {
         const C2* p = pc + (a * 2 + b) * 4;
         accr[a][b] = (Acc){p[0].x, p[1].x, p[2].x, p[3].x};
-        acci[a][b] = (Acc){p[0].y, p[1].y, p[2].y, p[3].y};
+        if constexpr (Tr::k3M) {
+          acci[a][b] = (Acc){T(0), T(0), T(0), T(0)};
+          accz[a][b] = (Acc){p[0].y + p[0].x, p[1].y + p[1].x, p[2].y + p[2].x, p[3].y + p[3].x};
+        } else {
+          acci[a][b] = (Acc){p[0].y, p[1].y, p[2].y, p[3].y};
+        }
       }
     // this tile's store target before plan() moves on to the next tile
     rs = rc;
@@ -1263,15 +1296,31 @@ zgemm_tile_kernel(T* __restrict__ A, long long lda, int m0, int n0, int M, int N
 #pragma unroll
         for (int b = 0; b < 2; ++b) yn[b] = sU[q * ZT_SP + wc * 32 + b * 16 + li];
       }
+      if constexpr (Tr::k3M) {
+        T sa[2], sb[2];
 #pragma unroll
-      for (int a = 0; a < 2; ++a)
+        for (int a = 0; a < 2; ++a) sa[a] = xa[a].x + xa[a].y;
 #pragma unroll
-        for (int b = 0; b < 2; ++b) {
-          accr[a][b] = Tr::mfma(-xa[a].x, yb[b].x, accr[a][b]);
-          accr[a][b] = Tr::mfma(xa[a].y, yb[b].y, accr[a][b]);
-          acci[a][b] = Tr::mfma(-xa[a].x, yb[b].y, acci[a][b]);
-          acci[a][b] = Tr::mfma(-xa[a].y, yb[b].x, acci[a][b]);
-        }
+        for (int b = 0; b < 2; ++b) sb[b] = yb[b].x + yb[b].y;
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            accr[a][b] = Tr::mfma(-xa[a].x, yb[b].x, accr[a][b]);
+            acci[a][b] = Tr::mfma(xa[a].y, yb[b].y, acci[a][b]);
+            accz[a][b] = Tr::mfma(-sa[a], sb[b], accz[a][b]);
+          }
+      } else {
+#pragma unroll
+        for (int a = 0; a < 2; ++a)
+#pragma unroll
+          for (int b = 0; b < 2; ++b) {
+            accr[a][b] = Tr::mfma(-xa[a].x, yb[b].x, accr[a][b]);
+            accr[a][b] = Tr::mfma(xa[a].y, yb[b].y, accr[a][b]);
+            acci[a][b] = Tr::mfma(-xa[a].x, yb[b].y, acci[a][b]);
+            acci[a][b] = Tr::mfma(-xa[a].y, yb[b].x, acci[a][b]);
+          }
+      }
       if (s < 15) {
 #pragma unroll
         for (int a = 0; a < 2; ++a) xa[a] = xn[a];
@@ -1283,12 +1332,7 @@ zgemm_tile_kernel(T* __restrict__ A, long long lda, int m0, int n0, int M, int N
     if (!live) break;
     tile = next;
   }
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int v = 0; v < 4; ++v) Tr::store((C2){accr[a][b][v], acci[a][b][v]}, rs, os[(a * 2 + b) * 4 + v]);
+  zt_store_result<Tr>(accr, acci, accz, rs, os);
 }
 
 // The mixed-precision path's FP32 trailing update: zgemm_update_kernel with
