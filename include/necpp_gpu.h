@@ -223,9 +223,10 @@ long necgpu_zgetrf_noalloc(double* a, long long lda, int n, int* piv, int* info,
 long necgpu_cgetrf(float* a, long long lda, int n, int* piv, int* info, void* work, int pivot_mode, void* stream);
 
 /* Solve with the factors of necgpu_zgetrf: x (device, n complex) is
- * overwritten.  work: device buffer of necgpu_zgetrs_workspace(n) bytes,
- * zero-initialised once; epoch: strictly increasing per call on the same
- * workspace. */
+ * overwritten.  work: device buffer of necgpu_zgetrs_workspace(n) bytes
+ * (no initialisation needed; calls on one workspace are ordered by their
+ * stream).  epoch: ignored (each call clears what its sweeps poll); kept so
+ * that callers of the earlier epoch-chained sweeps still link. */
 long long necgpu_zgetrs_workspace(int n);
 long necgpu_zgetrs(const double* lu, long long lda, int n, const int* piv, double* x, void* work, unsigned epoch,
                    void* stream);

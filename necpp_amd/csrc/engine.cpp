@@ -484,8 +484,7 @@ void DeviceSolver::refine_solve() {
   double* acc = static_cast<double*>(acc_.ensure(64));
   const float* z32 = Z32_.as<float>();
   hip_check(hipMemcpyAsync(b, x_.as<void>(), (size_t)n * 16, hipMemcpyDeviceToDevice, stream_), "rhs copy");
-  ++epoch_;
-  if (launch_lu_solve_apply_f32(z32, n, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(), epoch_, stream_))
+  if (launch_lu_solve_apply_f32(z32, n, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(), stream_))
     hip_check(hipGetLastError(), "solve launch");
   double norms[2] = {0.0, 0.0};
   refine_steps_ = 0;
@@ -508,9 +507,7 @@ void DeviceSolver::refine_solve() {
         throw NecError("mixed-precision LU: zero pivot in column " + std::to_string(info) +
                        " of the partial-pivot refactorisation");
       hip_check(hipMemcpyAsync(x_.as<void>(), b, (size_t)n * 16, hipMemcpyDeviceToDevice, stream_), "rhs copy");
-      ++epoch_;
-      if (launch_lu_solve_apply_f32(z32, n, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(), epoch_,
-                                    stream_))
+      if (launch_lu_solve_apply_f32(z32, n, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(), stream_))
         hip_check(hipGetLastError(), "solve launch");
       it = -1;
       prev = 0.0;
@@ -521,8 +518,7 @@ void DeviceSolver::refine_solve() {
     if (it >= kMaxRefine)
       throw NecError("mixed-precision refinement did not reach the tolerance in " + std::to_string(kMaxRefine) +
                      " steps (relative residual " + std::to_string(refine_residual_) + ")");
-    ++epoch_;
-    if (launch_lu_solve_apply_f32(z32, n, n, piv_.as<int>(), r, solve_work_.as<void>(), epoch_, stream_))
+    if (launch_lu_solve_apply_f32(z32, n, n, piv_.as<int>(), r, solve_work_.as<void>(), stream_))
       hip_check(hipGetLastError(), "solve launch");
     if (launch_update(x_.as<double>(), r, n, stream_)) hip_check(hipGetLastError(), "update launch");
     ++refine_steps_;
@@ -530,12 +526,7 @@ void DeviceSolver::refine_solve() {
 }
 
 void DeviceSolver::ensure_solve_work() {
-  const size_t wbytes = solve_workspace_bytes(n_);
-  if (solve_work_.size() < wbytes) {
-    solve_work_.ensure(wbytes);
-    hip_check(hipMemsetAsync(solve_work_.as<void>(), 0, wbytes, stream_), "memset");
-    epoch_ = 0;
-  }
+  solve_work_.ensure(solve_workspace_bytes(n_));
 }
 
 void DeviceSolver::enqueue_solve(const void* rhs_dev, hipMemcpyKind kind, const void* rhs_host) {
@@ -548,9 +539,8 @@ void DeviceSolver::enqueue_solve(const void* rhs_dev, hipMemcpyKind kind, const 
   if (precision_ == 1) {
     refine_solve();
   } else {
-    ++epoch_;
     if (launch_lu_solve_apply(Z_.as<double>(), ld_, n, piv_.as<int>(), x_.as<double>(), solve_work_.as<void>(),
-                              epoch_, stream_))
+                              stream_))
       hip_check(hipGetLastError(), "solve launch");
   }
   if (launch_currents(n, x_.as<double>(), segptr_.as<int>(), segbasis_.as<int>(), segabc_.as<double>(),

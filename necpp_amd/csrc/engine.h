@@ -183,7 +183,6 @@ class DeviceSolver {
   bool lookahead_ = true;
   double last_rkh_ = 1.0;  // rkh of the last fill (a refactorisation refills)
   int lu_info_ = 0;
-  unsigned epoch_ = 0;
   hipStream_t stream_ = nullptr;
   hipEvent_t ev_[6] = {};
   StageTimes times_;

@@ -235,20 +235,20 @@ int launch_mfma_probe_f32(const float* a, const float* b, float* d, hipStream_t 
 
 // solve.hip: x <- A^-1 x with the LU factors of launch_lu_fp64 (or the FP32
 // factors of the mixed path; x stays FP64).  work: device scratch of
-// solve_workspace_bytes(n) bytes, zeroed once; epoch: a value that increases
-// with every apply on the same workspace.  prepare (inverts the diagonal
-// blocks) runs once per factorisation, before any apply.
+// solve_workspace_bytes(n) bytes (no initialisation needed: every apply
+// clears what it polls).  prepare (inverts the diagonal blocks) runs once
+// per factorisation, before any apply.
 size_t solve_workspace_bytes(int n);
 int* solve_status(void* work, int n);
 int launch_lu_solve_prepare(const double* LU, long long lda, int n, void* work, hipStream_t stream);
 int launch_lu_solve_prepare_f32(const float* LU, long long lda, int n, void* work, hipStream_t stream);
 int launch_lu_solve_apply(const double* LU, long long lda, int n, const int* piv, double* x, void* work,
-                          unsigned epoch, hipStream_t stream);
+                          hipStream_t stream);
 int launch_lu_solve_apply_f32(const float* LU, long long lda, int n, const int* piv, double* x, void* work,
-                              unsigned epoch, hipStream_t stream);
+                              hipStream_t stream);
 // prepare + apply
 int launch_lu_solve_fp64(const double* LU, long long lda, int n, const int* piv, double* x, void* work,
-                         unsigned epoch, hipStream_t stream);
+                         hipStream_t stream);
 
 // Adjoint (gradient) path.  solve.hip: Z^T lam = g with the FP64 factors
 // (acc: n complex of scratch).  fill.hip: the fill's adjoint contraction

@@ -7,15 +7,14 @@
 // Once per factorisation, the 64 x 64 diagonal blocks of L and U are
 // inverted (one wave per block, all blocks at once).  Each triangular sweep
 // is then one launch of one workgroup per 64-row block: block b subtracts
-// the products with every finished block as soon as that block's flag is
-// raised (flags carry an epoch, so no reset is needed between calls; one
-// poll of 64 consecutive flags takes every block finished so far), then
-// applies its diagonal block's inverse (a 64 x 64 matrix-vector product
-// instead of 64 dependent steps) and raises its flag.  The block that
-// finishes last before b (the neighbour) and the inverse are staged in LDS
-// while b waits, so that b's critical path is two small products.
-// Workgroups are dispatched in block order, so every block a workgroup
-// waits on is already resident or done.
+// the products with every earlier block as soon as that block's values
+// appear (the outputs are cleared to a sentinel before the sweeps, so the
+// values themselves say they are final), then applies its diagonal block's
+// inverse (a 64 x 64 matrix-vector product instead of 64 dependent steps).
+// The factor rows of the next block to come are loaded while b waits, so
+// that b's critical path after its neighbour is a product with registers,
+// one reduction and the inverse.  Workgroups are dispatched in block order,
+// so every block a workgroup waits on is already resident or done.
 //
 // Multiplying by a computed inverse leaves a residual of about
 // u cond(T) |T| |x| (u = 2^-53, T the diagonal block), where substitution
@@ -23,8 +22,11 @@
 // few hundred, but a nearly singular structure (wires thicker than their
 // segments are long) has blocks with cond(T) up to 1e17, whose inverse
 // would swamp the solution.  So the inversion also estimates
-// ||T||_inf ||T^-1||_inf, and a block above kInvCondMax is solved by
-// substitution (64 dependent steps in one wave) instead.
+// ||T||_inf ||T^-1||_inf, and a block above inv_cond_max is solved by
+// substitution (64 dependent steps in one wave) instead.  FP32 factors (the
+// mixed path) take a far higher limit: their own rounding, about 6e-8
+// relative, is what the FP64 refinement corrects, and an inverse formed in
+// FP64 from the FP32 block adds only about 1e-16 cond(T) on top of it.
 #include "kernels.h"
 #include "nec_math.h"
 
@@ -33,7 +35,10 @@ namespace necgpu {
 namespace {
 
 constexpr int SB = 64;
-constexpr double kInvCondMax = 1.0e3;
+template <typename TF>
+constexpr double inv_cond_max() {
+  return sizeof(TF) == 4 ? 1.0e8 : 1.0e3;
+}
 
 // x from lane l of the wave (l uniform)
 __device__ __forceinline__ double readlane_d(double x, int l) {
@@ -45,12 +50,27 @@ __device__ __forceinline__ double readlane_d(double x, int l) {
 
 __device__ __forceinline__ double ld(const double* p) { return *p; }
 __device__ __forceinline__ double ld(const float* p) { return static_cast<double>(*p); }
+// one complex entry (re, im adjacent) in one vector load, kept in the
+// factors' own type: a conversion right after the load would make the
+// load's wait immediate and spoil the prefetch
+typedef double sv_dbl2 __attribute__((ext_vector_type(2)));
+typedef float sv_flt2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ void ldc(const double* p, double& re, double& im) {
+  const sv_dbl2 v = *reinterpret_cast<const sv_dbl2*>(p);
+  re = v.x;
+  im = v.y;
+}
+__device__ __forceinline__ void ldc(const float* p, float& re, float& im) {
+  const sv_flt2 v = *reinterpret_cast<const sv_flt2*>(p);
+  re = v.x;
+  im = v.y;
+}
 
 // Inverse of diagonal block b of L (blockIdx.y == 0, unit lower) or U
 // (blockIdx.y == 1, upper with its diagonal), padded to 64 x 64 with the
 // identity.  Lane j computes column j by substitution on e_j.  subst[y nblk
 // + b] = 1 when the block's condition estimate (infinity norms, |re| + |im|
-// per entry) exceeds kInvCondMax or is not finite: that block is then
+// per entry) exceeds inv_cond_max<TF>() or is not finite: that block is then
 // solved by substitution.
 template <typename TF>
 __global__ void __launch_bounds__(64) diag_inverse_kernel(const TF* __restrict__ LU, long long lda, int n,
@@ -128,224 +148,304 @@ __global__ void __launch_bounds__(64) diag_inverse_kernel(const TF* __restrict__
     nt = fmax(nt, __shfl_xor(nt, off));
     nx = fmax(nx, __shfl_xor(nx, off));
   }
-  if (j == 0) subst[(upper ? nblk : 0) + b] = (nt * nx <= kInvCondMax) ? 0 : 1;
+  if (j == 0) subst[(upper ? nblk : 0) + b] = (nt * nx <= inv_cond_max<TF>()) ? 0 : 1;
 }
 
-// x values and flags travel through the device coherence point: written
-// with agent-scope atomic stores (write-through), the flag after the data
-// stores have completed, and read with agent-scope atomic loads, so no
-// cache write-back or invalidate is needed.
+// x values travel through the device coherence point: written with
+// agent-scope atomic stores (write-through) and read with agent-scope
+// atomic loads, so no cache write-back or invalidate is needed.
 __device__ __forceinline__ void xstore(double* p, double v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 __device__ __forceinline__ double xload(const double* p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
+__device__ __forceinline__ int xload_i(const int* p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// A value the sweep has not written yet: all-ones bits, a NaN payload that
+// no arithmetic produces (a computed NaN is stored as the canonical one).
+constexpr long long kUnwritten = -1LL;
+__device__ __forceinline__ bool unwritten(double v) { return __double_as_longlong(v) == kUnwritten; }
+__device__ __forceinline__ double canonical_x(double v) {
+  return (v != v) ? __longlong_as_double(0x7FF8000000000000LL) : v;
+}
+
+// Bounded wait for entry i of a sweep's output (a timeout is sticky).
+__device__ __forceinline__ double wait_x(const double* p, int* status) {
+  double v = xload(p);
+  long spins = 0;
+  while (unwritten(v)) {
+    if (++spins > (1L << 22) || (spins % 64 == 0 && xload_i(status) != 0)) {
+      __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return 0.0;
+    }
+    __builtin_amdgcn_s_sleep(1);
+    v = xload(p);
+  }
+  return v;
+}
 
 // One triangular sweep.  lower: out[r] = y_r with L y = P rhs (rhs read
-// through perm).  upper: out[r] = x_r with U x = rhs.  Blocks finish in
-// order, so one poll of 64 consecutive flags tells how many more finished
-// blocks can be taken at once.
+// through perm).  upper: out[r] = x_r with U x = rhs.  out starts as
+// kUnwritten everywhere (solve_reset_kernel), so a reader knows an entry is
+// final when it differs from kUnwritten, and no flag is needed; xcopy, if
+// given, receives a plain copy of out.  Workgroup b (64 rows) owns rows
+// r0 .. r0+63; wave w holds rows r0 + 16 w .. r0 + 16 w + 15 and lane j
+// column j of every 64-column block, so every load of the factors is one
+// contiguous row piece per wave.
+//  - The blocks before b, in the order they finish: each wave, on its own,
+//    adds L[r][c0+j] y[c0+j] into its per-lane partial sums (16 rows per
+//    lane).  The two blocks just before b are awaited by polling their
+//    values, with their factor rows in registers from the start, so the
+//    wait for the neighbour's values is the one memory round trip on the
+//    chain from block to block; earlier blocks go by their flags.
+//  - The partial sums are reduced over the lanes through LDS (row order of
+//    the lanes fixed, so the result is deterministic), subtracted from the
+//    right-hand side, and multiplied by the diagonal block's inverse (or
+//    solved by substitution when the block is ill-conditioned).
+//  - x_b goes to out with agent-scope stores (write-through); once they
+//    have completed, the block's flag is raised for the batched readers.
 template <typename TF, bool lower>
 __global__ void __launch_bounds__(256) trsv_kernel(const TF* __restrict__ LU, long long lda, int n,
                                                    const int* __restrict__ perm, const double* rhs, double* out,
-                                                   const double* __restrict__ inv_all,
-                                                   const int* __restrict__ subst, unsigned* flags,
-                                                   unsigned epoch, int* status) {
-  __shared__ double sInv[SB * SB * 2];   // inverse of the diagonal block (or the block itself)
-  __shared__ double sNb[SB * SB * 2];    // the neighbour block
+                                                   double* xcopy, const double* __restrict__ inv_all,
+                                                   const int* __restrict__ subst, int* flags, int* status) {
+  constexpr int RS = SB + 1;             // LDS row stride of the reduction (complex)
+  constexpr int IS = SB + 1;             // LDS column stride of sInv (complex)
+  // inverse of the diagonal block (or the block itself), by columns: entry
+  // (i, q) at 2 (q IS + i), so lanes over rows read consecutive words
+  __shared__ double sInv[SB * IS * 2];
+  __shared__ double sRed[SB * RS * 2];   // per-lane partial sums, [row][lane]
   __shared__ double sPart[4][SB * 2];
   __shared__ double sAcc[SB * 2];
-  __shared__ int s_ready;
   const int nblk = (n + SB - 1) / SB;
   const int b = lower ? (int)blockIdx.x : nblk - 1 - (int)blockIdx.x;
-  const int nb = lower ? b - 1 : b + 1;  // neighbour block (last to finish)
-  const bool has_nb = lower ? (b > 0) : (b < nblk - 1);
   const int r0 = b * SB;
   const int rows = min(SB, n - r0);
   const int t = threadIdx.x;
-  const int r = t & 63;     // row within the block
-  const int part = t >> 6;  // column quarter of this thread
+  const int lane = t & 63, wave = t >> 6;
   const bool by_subst = subst[(lower ? 0 : nblk) + b] != 0;
+  // this block's right-hand side, loaded now rather than on the chain
+  double rhs_r = 0.0, rhs_i = 0.0;
+  if (t < rows) {
+    const int src = lower ? perm[r0 + t] : r0 + t;
+    rhs_r = rhs[2 * src];
+    rhs_i = rhs[2 * src + 1];
+  }
   // stage the inverse (or, to substitute, the block padded with the
-  // identity) and the neighbour block while earlier blocks finish
-  {
-    if (by_subst) {
-      for (int e = t; e < SB * SB; e += 256) {
-        const int i = e / SB, q = e % SB;
-        double vr = (i == q) ? 1.0 : 0.0, vi = 0.0;
-        if (i < rows && q < rows) {
-          const TF* v = LU + 2 * ((size_t)(r0 + i) * lda + r0 + q);
-          vr = ld(v);
-          vi = ld(v + 1);
-        }
-        sInv[2 * e] = vr;
-        sInv[2 * e + 1] = vi;
+  // identity) while earlier blocks finish
+  if (by_subst) {
+    for (int e = t; e < SB * SB; e += 256) {
+      const int i = e / SB, q = e % SB;
+      double vr = (i == q) ? 1.0 : 0.0, vi = 0.0;
+      if (i < rows && q < rows) {
+        const TF* v = LU + 2 * ((size_t)(r0 + i) * lda + r0 + q);
+        vr = ld(v);
+        vi = ld(v + 1);
       }
-    } else {
-      const double* src = inv_all + ((size_t)(lower ? 0 : nblk) + b) * SB * SB * 2;
-      for (int e = t; e < SB * SB * 2; e += 256) sInv[e] = src[e];
+      sInv[2 * (q * IS + i)] = vr;
+      sInv[2 * (q * IS + i) + 1] = vi;
     }
-    if (has_nb) {
-      const int c0 = nb * SB;
-      const int cols = min(SB, n - c0);
-      for (int e = t; e < SB * SB; e += 256) {
-        const int i = e / SB, q = e % SB;
-        double vr = 0.0, vi = 0.0;
-        if (i < rows && q < cols) {
-          const TF* v = LU + 2 * ((size_t)(r0 + i) * lda + c0 + q);
-          vr = ld(v);
-          vi = ld(v + 1);
-        }
-        sNb[2 * e] = vr;
-        sNb[2 * e + 1] = vi;
-      }
+  } else {
+    // every load of a thread in flight at once, then the LDS stores
+    const sv_dbl2* src = reinterpret_cast<const sv_dbl2*>(inv_all + ((size_t)(lower ? 0 : nblk) + b) * SB * SB * 2);
+    sv_dbl2 v[SB * SB / 256];
+#pragma unroll
+    for (int u = 0; u < SB * SB / 256; ++u) v[u] = src[t + 256 * u];
+#pragma unroll
+    for (int u = 0; u < SB * SB / 256; ++u) {
+      const int e = t + 256 * u, i = e / SB, q = e % SB;
+      sInv[2 * (q * IS + i)] = v[u].x;
+      sInv[2 * (q * IS + i) + 1] = v[u].y;
     }
   }
-  double accr = 0.0, acci = 0.0;
-  // blocks other than the neighbour, taken in finished batches
-  const int others = lower ? max(0, b - 1) : max(0, nblk - 2 - b);
-  int done = 0;
+  double ar[16], ai[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) ar[q] = ai[q] = 0.0;
+  // the blocks before b, in finishing order: step st is block c(st)
+  const int before = lower ? b : nblk - 1 - b;
+  // this wave's 16 rows: row r0 + 16 w + q for q < rlim; the addresses of
+  // rows past n and of columns past n are clamped to row r0, column 0, and
+  // their values taken as zero
+  const TF* base = LU + 2 * (size_t)r0 * lda;
+  const int rlim = rows - wave * 16;
+  size_t roff[16];
+#pragma unroll
+  for (int q = 0; q < 16; ++q) roff[q] = (q < rlim) ? (size_t)(wave * 16 + q) * lda : 0;
+  // The last two blocks before b are on the chain: their factor rows are
+  // loaded now, and their values polled directly as soon as the earlier
+  // blocks are in.  The blocks before them have at least two links of the
+  // chain to spare: they are taken in batches as their flags show them
+  // finished, with an interval between flag polls that grows with the
+  // number of blocks still to come before b, so that the many workgroups
+  // far down the chain put little traffic on the flags.
+  TF nr[2][16], ni[2][16];  // [0]: block b-1 (the neighbour), [1]: block b-2
+  int ncol0[2];
+  bool nb_col[2];
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    ncol0[h] = before > h ? ((lower ? b - 1 - h : b + 1 + h) * SB) : 0;
+    nb_col[h] = before > h && ncol0[h] + lane < n;
+    const int cc = nb_col[h] ? ncol0[h] + lane : 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const TF* v = base + 2 * (roff[q] + cc);
+      nr[h][q] = ni[h][q] = TF(0);
+      if (nb_col[h] && q < rlim) ldc(v, nr[h][q], ni[h][q]);
+    }
+  }
+  const int* fl = flags + (lower ? 0 : nblk);
+  const int others = max(0, before - 2);
+  // older blocks one at a time, in finishing order: the rows of the next
+  // one are loaded while its flag is awaited (its values only after the
+  // flag, so that no load issued after them holds the values back)
+  TF pr[16], pi[16];
+  auto older_rows = [&](int st) __attribute__((always_inline)) {
+    const int c0 = (lower ? st : nblk - 1 - st) * SB;
+    const bool col = st < others && c0 + lane < n;
+    const int cc = col ? c0 + lane : 0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const TF* v = base + 2 * (roff[q] + cc);
+      pr[q] = pi[q] = TF(0);
+      if (col && q < rlim) ldc(v, pr[q], pi[q]);
+    }
+  };
+  if (others > 0) older_rows(0);
   long spins = 0;
-  while (done < others) {
-    if (t < 64) {
-      const int step = done + t;
-      bool ok = false;
-      if (step < others) {
-        const int c = lower ? step : nblk - 1 - step;
-        ok = __hip_atomic_load(flags + c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == epoch;
-      }
-      const unsigned long long m = __ballot(ok);
-      // count of consecutive finished blocks from `done`
-      const int cnt = (~m == 0ull) ? 64 : __ffsll((long long)~m) - 1;
-      if (t == 0) {
-        int ready = cnt;
-        if (cnt == 0) {
-          // bounded wait; a timeout is sticky so every later wait returns at once
-          if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0 || ++spins > (1L << 22)) {
-            __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            ready = others - done;  // give up waiting: finish with what is there
-          } else {
-            __builtin_amdgcn_s_sleep(1);
-          }
-        }
-        s_ready = ready;
+  for (int st = 0; st < others;) {
+    const int c = lower ? st : nblk - 1 - st;
+    if (xload_i(fl + c) == 0) {
+      // bounded wait; a timeout is sticky so every later wait returns at once
+      if (xload_i(status) != 0 || ++spins > (1L << 22)) {
+        __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      } else {
+        __builtin_amdgcn_s_sleep(2);
+        continue;
       }
     }
-    __syncthreads();
-    const int cnt = s_ready;
-    __syncthreads();
-    for (int step = done; step < done + cnt; ++step) {
-      const int c = lower ? step : nblk - 1 - step;
-      const int c0 = c * SB;
-      const int cols = min(SB, n - c0);
-      if (r < rows) {
-        const TF* lrow = LU + 2 * ((size_t)(r0 + r) * lda + c0);
-        for (int j = part; j < cols; j += 4) {
-          const double yr = xload(out + 2 * (c0 + j));
-          const double yi = xload(out + 2 * (c0 + j) + 1);
-          const double lr = ld(lrow + 2 * j), li = ld(lrow + 2 * j + 1);
-          accr += lr * yr - li * yi;
-          acci += lr * yi + li * yr;
-        }
-      }
+    const bool col = c * SB + lane < n;
+    const double yr = col ? xload(out + 2 * (c * SB + lane)) : 0.0;
+    const double yi = col ? xload(out + 2 * (c * SB + lane) + 1) : 0.0;
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const double lr = pr[q], li = pi[q];
+      ar[q] += lr * yr - li * yi;
+      ai[q] += lr * yi + li * yr;
     }
-    done += cnt;
+    ++st;
+    older_rows(st);
   }
-  if (has_nb) {
-    if (t == 0) {
-      long sp = 0;
-      while (__hip_atomic_load(flags + nb, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != epoch) {
-        if (__hip_atomic_load(status, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0) break;
-        if (++sp > (1L << 22)) {
-          __hip_atomic_store(status, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-          break;
-        }
-        __builtin_amdgcn_s_sleep(1);
+  // block b-2, then the neighbour b-1, in finishing order
+#pragma unroll
+  for (int h = 1; h >= 0; --h) {
+    if (nb_col[h]) {
+      const double yr = wait_x(out + 2 * (ncol0[h] + lane), status);
+      const double yi = wait_x(out + 2 * (ncol0[h] + lane) + 1, status);
+#pragma unroll
+      for (int q = 0; q < 16; ++q) {
+        const double lr = nr[h][q], li = ni[h][q];
+        ar[q] += lr * yr - li * yi;
+        ai[q] += lr * yi + li * yr;
       }
-    }
-    __syncthreads();
-    const int c0 = nb * SB;
-    for (int j = part; j < SB; j += 4) {
-      const double yr = (c0 + j < n) ? xload(out + 2 * (c0 + j)) : 0.0;
-      const double yi = (c0 + j < n) ? xload(out + 2 * (c0 + j) + 1) : 0.0;
-      const double lr = sNb[2 * (r * SB + j)], li = sNb[2 * (r * SB + j) + 1];
-      accr += lr * yr - li * yi;
-      acci += lr * yi + li * yr;
     }
   }
-  sPart[part][2 * r] = accr;
-  sPart[part][2 * r + 1] = acci;
+  // sums over the lanes: thread (row r, quarter p) adds lanes 16p .. 16p+15
+#pragma unroll
+  for (int q = 0; q < 16; ++q) {
+    sRed[2 * ((wave * 16 + q) * RS + lane)] = ar[q];
+    sRed[2 * ((wave * 16 + q) * RS + lane) + 1] = ai[q];
+  }
+  __syncthreads();
+  {
+    const int r = t >> 2, p = t & 3;
+    double sr = 0.0, si = 0.0;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) {
+      sr += sRed[2 * (r * RS + 16 * p + j)];
+      si += sRed[2 * (r * RS + 16 * p + j) + 1];
+    }
+    sPart[p][2 * r] = sr;
+    sPart[p][2 * r + 1] = si;
+  }
   __syncthreads();
   if (t < SB) {
-    double vr = 0.0, vi = 0.0;
-    if (t < rows) {
-      const int src = lower ? perm[r0 + t] : r0 + t;
-      vr = rhs[2 * src];
-      vi = rhs[2 * src + 1];
-    }
+    const double vr = rhs_r, vi = rhs_i;
     const double sr = ((sPart[0][2 * t] + sPart[1][2 * t]) + sPart[2][2 * t]) + sPart[3][2 * t];
     const double si = ((sPart[0][2 * t + 1] + sPart[1][2 * t + 1]) + sPart[2][2 * t + 1]) + sPart[3][2 * t + 1];
     sAcc[2 * t] = (t < rows) ? vr - sr : 0.0;
     sAcc[2 * t + 1] = (t < rows) ? vi - si : 0.0;
   }
   __syncthreads();
+  const int r = t & 63;     // row within the block
+  const int part = t >> 6;  // column quarter of this thread
   if (!by_subst) {
     // x_b = inverse * acc
     double xr = 0.0, xi = 0.0;
     for (int j = part; j < SB; j += 4) {
-      const double ir = sInv[2 * (r * SB + j)], ii = sInv[2 * (r * SB + j) + 1];
-      const double ar = sAcc[2 * j], ai = sAcc[2 * j + 1];
-      xr += ir * ar - ii * ai;
-      xi += ir * ai + ii * ar;
+      const double ir = sInv[2 * (j * IS + r)], ii = sInv[2 * (j * IS + r) + 1];
+      const double vr = sAcc[2 * j], vi = sAcc[2 * j + 1];
+      xr += ir * vr - ii * vi;
+      xi += ir * vi + ii * vr;
     }
     sPart[part][2 * r] = xr;
     sPart[part][2 * r + 1] = xi;
     __syncthreads();
     if (t < rows) {
-      xstore(out + 2 * (r0 + t), ((sPart[0][2 * t] + sPart[1][2 * t]) + sPart[2][2 * t]) + sPart[3][2 * t]);
-      xstore(out + 2 * (r0 + t) + 1,
-             ((sPart[0][2 * t + 1] + sPart[1][2 * t + 1]) + sPart[2][2 * t + 1]) + sPart[3][2 * t + 1]);
+      const double x0 = canonical_x(((sPart[0][2 * t] + sPart[1][2 * t]) + sPart[2][2 * t]) + sPart[3][2 * t]);
+      const double x1 =
+          canonical_x(((sPart[0][2 * t + 1] + sPart[1][2 * t + 1]) + sPart[2][2 * t + 1]) + sPart[3][2 * t + 1]);
+      xstore(out + 2 * (r0 + t), x0);
+      xstore(out + 2 * (r0 + t) + 1, x1);
+      if (xcopy) {
+        xcopy[2 * (r0 + t)] = x0;
+        xcopy[2 * (r0 + t) + 1] = x1;
+      }
     }
   } else if (t < SB) {
     // substitution in the first wave, lane t holding row t: step q takes
     // x_q from lane q and subtracts its column from the rows still open
-    double ar = sAcc[2 * t], ai = sAcc[2 * t + 1];
+    double vr = sAcc[2 * t], vi = sAcc[2 * t + 1];
     if (lower) {
       for (int q = 0; q < SB - 1; ++q) {
-        const double yr = readlane_d(ar, q), yi = readlane_d(ai, q);
+        const double yr = readlane_d(vr, q), yi = readlane_d(vi, q);
         if (t > q) {
-          const double lr = sInv[2 * (t * SB + q)], li = sInv[2 * (t * SB + q) + 1];
-          ar -= lr * yr - li * yi;
-          ai -= lr * yi + li * yr;
+          const double lr = sInv[2 * (q * IS + t)], li = sInv[2 * (q * IS + t) + 1];
+          vr -= lr * yr - li * yi;
+          vi -= lr * yi + li * yr;
         }
       }
     } else {
       for (int q = SB - 1; q >= 0; --q) {
         if (t == q) {
-          const zc x = zdiv(zmk(ar, ai), zmk(sInv[2 * (q * SB + q)], sInv[2 * (q * SB + q) + 1]));
-          ar = x.re;
-          ai = x.im;
+          const zc x = zdiv(zmk(vr, vi), zmk(sInv[2 * (q * IS + q)], sInv[2 * (q * IS + q) + 1]));
+          vr = x.re;
+          vi = x.im;
         }
-        const double xr = readlane_d(ar, q), xi = readlane_d(ai, q);
+        const double xr = readlane_d(vr, q), xi = readlane_d(vi, q);
         if (t < q) {
-          const double ur = sInv[2 * (t * SB + q)], ui = sInv[2 * (t * SB + q) + 1];
-          ar -= ur * xr - ui * xi;
-          ai -= ur * xi + ui * xr;
+          const double ur = sInv[2 * (q * IS + t)], ui = sInv[2 * (q * IS + t) + 1];
+          vr -= ur * xr - ui * xi;
+          vi -= ur * xi + ui * xr;
         }
       }
     }
     if (t < rows) {
-      xstore(out + 2 * (r0 + t), ar);
-      xstore(out + 2 * (r0 + t) + 1, ai);
+      xstore(out + 2 * (r0 + t), canonical_x(vr));
+      xstore(out + 2 * (r0 + t) + 1, canonical_x(vi));
+      if (xcopy) {
+        xcopy[2 * (r0 + t)] = canonical_x(vr);
+        xcopy[2 * (r0 + t) + 1] = canonical_x(vi);
+      }
     }
   }
-  // the data stores have completed before the flag is raised
+  // the flag once this block's values have reached the coherence point
   asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
   __syncthreads();
-  if (t == 0) __hip_atomic_store(flags + b, epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (t == 0) __hip_atomic_store(flags + (lower ? 0 : nblk) + b, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-
 // Z^T lam = g with the FP64 factors (P Z = L U, perm = piv + n):
 // U^T w = g (forward, U^T lower), L^T v = w (backward, unit), lam[perm[i]]
 // = v[i].  One workgroup; each step reads one contiguous row of the factors.
@@ -393,14 +493,23 @@ __global__ void __launch_bounds__(1024) lu_solve_transposed_kernel(const double*
   }
 }
 
-__global__ void reset_status_kernel(int* status) {
-  if (threadIdx.x == 0) *status = 0;
+// Status word and flags to zero, both sweeps' outputs to kUnwritten.
+__global__ void solve_reset_kernel(int* status, int* flags, int nflags, double* a, double* b, int m) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *status = 0;
+  if (i < nflags) flags[i] = 0;
+  const double u = __longlong_as_double(kUnwritten);
+  if (i < m) {
+    a[i] = u;
+    b[i] = u;
+  }
 }
 
 struct SolveWork {
-  unsigned *flags_f, *flags_b;
+  int* flags;  // per block and sweep: its values are final
   int* status;
-  double* tmp;
+  double* tmp;   // the lower sweep's output
+  double* tmp2;  // the upper sweep's output (copied to x as it is formed)
   int* subst;
   double* inv;
 };
@@ -411,12 +520,13 @@ SolveWork carve_solve(void* work, int n) {
   const int nblk = (n + SB - 1) / SB;
   char* p = static_cast<char*>(work);
   SolveWork w;
-  w.flags_f = reinterpret_cast<unsigned*>(p);
-  w.flags_b = w.flags_f + nblk;
-  p += align256((size_t)2 * nblk * sizeof(unsigned));
+  w.flags = reinterpret_cast<int*>(p);
+  p += align256((size_t)2 * nblk * sizeof(int));
   w.status = reinterpret_cast<int*>(p);
   p += 256;
   w.tmp = reinterpret_cast<double*>(p);
+  p += align256((size_t)n * 16);
+  w.tmp2 = reinterpret_cast<double*>(p);
   p += align256((size_t)n * 16);
   w.subst = reinterpret_cast<int*>(p);
   p += align256((size_t)2 * nblk * sizeof(int));
@@ -435,17 +545,17 @@ int prepare(const TF* LU, long long lda, int n, void* work, hipStream_t stream) 
 }
 
 template <typename TF>
-int apply(const TF* LU, long long lda, int n, const int* piv, double* x, void* work, unsigned epoch,
-          hipStream_t stream) {
+int apply(const TF* LU, long long lda, int n, const int* piv, double* x, void* work, hipStream_t stream) {
   if (n <= 0) return 0;
   const int nblk = (n + SB - 1) / SB;
   const SolveWork w = carve_solve(work, n);
   const int* perm = piv + n;
-  launch_k(K_SOLVE_RESET, reset_status_kernel, dim3(1), dim3(64), 0u, stream, w.status);
+  launch_k(K_SOLVE_RESET, solve_reset_kernel, dim3((2 * n + 255) / 256), dim3(256), 0u, stream, w.status, w.flags,
+           2 * nblk, w.tmp, w.tmp2, 2 * n);
   launch_k(K_TRSV_LOWER, trsv_kernel<TF, true>, dim3(nblk), dim3(256), 0u, stream, LU, lda, n, perm,
-           (const double*)x, w.tmp, (const double*)w.inv, (const int*)w.subst, w.flags_f, epoch, w.status);
+           (const double*)x, w.tmp, (double*)nullptr, (const double*)w.inv, (const int*)w.subst, w.flags, w.status);
   launch_k(K_TRSV_UPPER, trsv_kernel<TF, false>, dim3(nblk), dim3(256), 0u, stream, LU, lda, n, perm,
-           (const double*)w.tmp, x, (const double*)w.inv, (const int*)w.subst, w.flags_b, epoch, w.status);
+           (const double*)w.tmp, w.tmp2, x, (const double*)w.inv, (const int*)w.subst, w.flags, w.status);
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
@@ -453,7 +563,7 @@ int apply(const TF* LU, long long lda, int n, const int* piv, double* x, void* w
 
 size_t solve_workspace_bytes(int n) {
   const int nblk = (n + SB - 1) / SB;
-  return align256((size_t)2 * nblk * sizeof(unsigned)) + 256 + align256((size_t)n * 16) +
+  return align256((size_t)2 * nblk * sizeof(int)) + 256 + 2 * align256((size_t)n * 16) +
          align256((size_t)2 * nblk * sizeof(int)) + (size_t)2 * nblk * SB * SB * 16;
 }
 
@@ -474,18 +584,18 @@ int launch_lu_solve_prepare_f32(const float* LU, long long lda, int n, void* wor
   return prepare<float>(LU, lda, n, work, stream);
 }
 int launch_lu_solve_apply(const double* LU, long long lda, int n, const int* piv, double* x, void* work,
-                          unsigned epoch, hipStream_t stream) {
-  return apply<double>(LU, lda, n, piv, x, work, epoch, stream);
+                          hipStream_t stream) {
+  return apply<double>(LU, lda, n, piv, x, work, stream);
 }
 int launch_lu_solve_apply_f32(const float* LU, long long lda, int n, const int* piv, double* x, void* work,
-                              unsigned epoch, hipStream_t stream) {
-  return apply<float>(LU, lda, n, piv, x, work, epoch, stream);
+                              hipStream_t stream) {
+  return apply<float>(LU, lda, n, piv, x, work, stream);
 }
 
 int launch_lu_solve_fp64(const double* LU, long long lda, int n, const int* piv, double* x, void* work,
-                         unsigned epoch, hipStream_t stream) {
+                         hipStream_t stream) {
   if (prepare<double>(LU, lda, n, work, stream)) return 1;
-  return apply<double>(LU, lda, n, piv, x, work, epoch, stream);
+  return apply<double>(LU, lda, n, piv, x, work, stream);
 }
 
 }  // namespace necgpu
