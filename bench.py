@@ -26,7 +26,7 @@ NB = 64  # LU panel width (lu.hip)
 
 # rocprofv3 kernel names of the kernel families nec_gpu_kernel_times reports
 ROCPROF_NAME = {"lu_panel": "lu_panel_kernel", "zgemm": "zgemm_update_kernel", "lu_trsm": "lu_trsm_kernel",
-                "lu_laswp": "lu_laswp_kernel", "fill_fields": "fill_fields_kernel",
+                "lu_laswp": "lu_laswp_kernel", "fill_tile": "fill_tile_kernel", "fill_fields": "fill_fields_kernel",
                 "fill_assemble": "fill_assemble_kernel", "pattern_partial": "pattern_partial_kernel",
                 "trsv_lower": "trsv_blocked_kernel<true>", "trsv_upper": "trsv_blocked_kernel<false>"}
 
@@ -98,10 +98,11 @@ def op_counts(workload_tag):
     return json.load(open(path)).get(workload_tag)
 
 
-def fill_rates(counts, fill_kernel_ms, stage_ms, workload_tag=None):
+def fill_rates(counts, fill_kernel_ms, stage_ms, workload_tag=None, keys=("fill_tile",)):
     """Achieved FP64 rate of the fill from the tallied flops, and its HBM
-    traffic (fields plus assembly kernels) from the committed --pmc summary
-    against the 16 N^2 bytes of Z."""
+    traffic (the one-pass fill kernel, or the field and assembly kernels
+    when those ran) from the committed --pmc summary against the 16 N^2
+    bytes of Z."""
     if counts is None:
         return None
     c = counts["fill"]
@@ -109,7 +110,7 @@ def fill_rates(counts, fill_kernel_ms, stage_ms, workload_tag=None):
     traffic = None
     if workload_tag is not None:
         parts = [read_profile_value(f"fill_and_pattern_{workload_tag}_traffic.txt", f"{k}_traffic_bytes_per_launch")
-                 for k in ("fill_fields", "fill_assemble")]
+                 for k in keys]
         traffic = None if None in parts else sum(parts)
     z_bytes = 16.0 * counts["n"] ** 2
     return {"hbm_traffic_bytes": traffic, "z_bytes": z_bytes,
@@ -469,13 +470,13 @@ def main():
     else:
         roof = {"bound": None, "achieved": None, "peak": None, "unit": None, "frac": None}
     counts = op_counts(workload_tag)
-    fill_kernel_ms = sum(kernels[k]["ms_per_step"] for k in ("fill_fields", "fill_assemble", "fill_tile")
-                         if k in kernels)
-    fill_stage = fill_rates(counts, fill_kernel_ms, fill_ms, workload_tag) if fill_kernel_ms > 0 else None
+    fill_keys = tuple(k for k in ("fill_tile", "fill_fields", "fill_assemble") if k in kernels)
+    fill_kernel_ms = sum(kernels[k]["ms_per_step"] for k in fill_keys)
+    fill_stage = fill_rates(counts, fill_kernel_ms, fill_ms, workload_tag, fill_keys) if fill_kernel_ms > 0 else None
     pat_kernel_ms = kernels.get("pattern_partial", {}).get("ms_per_step")
     pattern_stage = pattern_rates(counts, n, pat_kernel_ms, workload_tag) if pat_kernel_ms else None
     hw_name = f"fill_and_pattern_{workload_tag}_fp64_counters.txt"
-    for stage, keys in ((fill_stage, ("fill_fields", "fill_assemble")), (pattern_stage, ("pattern_partial",))):
+    for stage, keys in ((fill_stage, fill_keys), (pattern_stage, ("pattern_partial",))):
         if stage is None:
             continue
         hw = [read_profile_value(hw_name, f"{k}_fp64_flops_per_launch") for k in keys]

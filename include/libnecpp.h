@@ -10,14 +10,16 @@
  * mirrors (src/libNEC.cpp:<line>).
  *
  * Supported by the GPU path: straight and tapered wires (GW), moves and
- * copies (GM), voltage sources (EX 0), frequency sweeps (FR), the
- * integration-distance card (KH), the extended thin-wire kernel (EK), loads
- * (LD types 0-5), perfect and reflection-coefficient ground (GN 1, GN 0),
- * near electric fields of wires in free space (NE), execute (XQ), far-field
- * patterns (RP mode 0) and all gain / impedance getters.  Card functions for
- * features outside that path (patches, Sommerfeld ground, radial screens,
- * NH, coupling, ...) exist with the reference's signature and return 1 with
- * the message "<card>: not supported by the GPU path".
+ * copies (GM, in decks and through nec_gm_card), voltage sources (EX 0),
+ * frequency sweeps (FR), the integration-distance card (KH), the extended
+ * thin-wire kernel (EK), loads (LD types 0-5), networks and transmission
+ * lines (NT, TL), perfect and reflection-coefficient ground (GN 1, GN 0),
+ * near electric and magnetic fields (NE, NH) in free space and over those
+ * grounds, execute (XQ), far-field patterns (RP mode 0) and all gain /
+ * impedance getters.  Card functions for features outside that path
+ * (patches, Sommerfeld ground, radial screens, coupling, ...) exist with the
+ * reference's signature and return 1 with the message "<card>: not
+ * supported by the GPU path".
  */
 #pragma once
 
