@@ -317,13 +317,15 @@ void DeviceSolver::load(const Geometry& geo, const ScaledSegments& seg, bool use
 
   // Clusters of the one-pass fill: runs of T bases of the Morton order (a
   // compact patch of the structure), each with the sorted list of every
-  // source its bases' supports draw on.  T = 1024 (NECGPU_FILL_TILE
-  // overrides it, for measurements): on the planar grids a source list is
-  // then 1.11-1.14 times the cluster, the fields evaluated twice at the
-  // clusters' edges.
+  // source its bases' supports draw on.  T = 512 up to 8,192 segments, else
+  // 1,024 (NECGPU_FILL_TILE overrides it, for measurements): on the planar
+  // grids a source list is then 1.11-1.2 times the cluster, the fields
+  // evaluated twice at the clusters' edges.  Measured fill times, T = 128 ..
+  // 2,048: 3,280 segments 1.10, 1.02, 0.91, 1.16, 1.20 ms; 16,020 segments
+  // (T = 256 .. 2,048) 17.4, 21.9, 16.4, 28.7 ms.
   {
     const char* tv = std::getenv("NECGPU_FILL_TILE");
-    const int T = std::max(8, (tv && *tv) ? std::atoi(tv) : 1024);
+    const int T = std::max(8, (tv && *tv) ? std::atoi(tv) : (n <= 8192 ? 512 : 1024));
     std::vector<int> srcp{0}, src, basep{0}, base, cjl(t.colj.size());
     int smax = 0;
     for (int p0 = 0; p0 < n; p0 += T) {

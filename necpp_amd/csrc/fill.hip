@@ -402,9 +402,12 @@ __device__ void pair_fields_v(const T& xi_, const T& yi_, const T& zi_, const T&
     rhoy = T(0.0);
     rhoz = T(0.0);
   } else {
-    rhox = rhox / rh;
-    rhoy = rhoy / rh;
-    rhoz = rhoz / rh;
+    // one reciprocal for the three components (a rounding apart from the
+    // reference's three divisions; no decision depends on them)
+    const T irh = T(1.0) / rh;
+    rhox = rhox * irh;
+    rhoy = rhoy * irh;
+    rhoz = rhoz * irh;
   }
   const T r = sqrt(zp * zp + rh * rh);
   if (rout) *rout = val(r);
@@ -412,17 +415,22 @@ __device__ void pair_fields_v(const T& xi_, const T& yi_, const T& zi_, const T&
   const cx<T> zero = cmk<T>(T(0), T(0));
   cx<T> e3 = zero, e4 = zero, e5 = zero;
   if (val(r) >= rkh) {
-    // lumped current element, src/nec_context.cpp:2798-2816
+    // lumped current element, src/nec_context.cpp:2798-2816; the divisions
+    // by r, by py and by 2 py and the two of 1 / rmag are multiplications by
+    // three reciprocals here (no decision depends on these values)
     const double eta = em_impedance();
     T rmag = kTwoPi * r;
-    const T cth = zp / r;
-    const T px = rh / r;
+    const T ir = T(1.0) / r;
+    const T cth = zp * ir;
+    const T px = rh * ir;
     T sr, cr;
     sincos(rmag, &sr, &cr);
     const cx<T> e0 = cmk<T>(cr, -sr);
     const T py = kTwoPi * r * r;
-    const cx<T> e1 = cdivr(cmul(cmulr(e0, eta * cth), cmk<T>(T(1.0), -1.0 / rmag)), py);
-    const cx<T> e2 = cdivr(cmul(cmulr(e0, eta * px), cmk<T>(T(1.0), rmag - 1.0 / rmag)), 2. * py);
+    const T ipy = T(1.0) / py;
+    const T irm = T(1.0) / rmag;
+    const cx<T> e1 = cmulr(cmul(cmulr(e0, eta * cth), cmk<T>(T(1.0), -irm)), ipy);
+    const cx<T> e2 = cmulr(cmul(cmulr(e0, eta * px), cmk<T>(T(1.0), rmag - irm)), 0.5 * ipy);
     tezk = csub(cmulr(e1, cth), cmulr(e2, px));
     terk = cadd(cmulr(e1, px), cmulr(e2, cth));
     rmag = sin(kPi * sj) / kPi;
