@@ -1087,7 +1087,7 @@ __global__ void __launch_bounds__(256) zgemm_update_kernel(double* __restrict__ 
 // column by column inside a band; each XCD (launch index mod 8) owns a
 // contiguous range of that order, so the tiles its CUs hold at one time
 // share a few L and U strips in its L2.
-static_assert(NB == 64, "zgemm_tile_kernel stages K <= 64");
+static_assert(NB == 64, "zgemm_tile_kernel stages K in chunks of 64");
 constexpr int ZT_SP = 65;  // LDS row stride in complex (odd: the transposing L store is conflict-free)
 typedef double dbl2_t __attribute__((ext_vector_type(2)));
 typedef float flt2_t __attribute__((ext_vector_type(2)));
@@ -1198,33 +1198,40 @@ zgemm_tile_kernel(T* __restrict__ A, long long lda, int m0, int n0, int M, int N
   if (tile >= tend) return;
   const unsigned esz = 2u * sizeof(T);
   const unsigned row_bytes = (unsigned)(lda * esz);
-  // Per-tile load sources and offsets.  Slot i (0..15) of this thread:
-  //   L: row wave*16+i of the strip, column `lane`
-  //   U: row wave*16+i of the strip, column `lane`
-  //   C: accumulator element i = (a b v) of the wave's 32 x 32 block
+  // A unit of work is one 64-deep chunk of K of one tile (K <= 128: one or
+  // two chunks per tile); the accumulators carry over from a tile's first
+  // chunk to its second, C is loaded before the first and stored after the
+  // last.  Per-unit load sources and offsets.  Slot i (0..15) of this thread:
+  //   L: row wave*16+i of the strip, column `lane` of the chunk
+  //   U: row wave*16+i of the chunk, column `lane`
+  //   C: accumulator element i = (a b v) of the wave's 32 x 32 block (only
+  //      before a tile's first chunk; otherwise a zero-range resource)
+  const int nch = (K + 63) / 64;
   __amdgpu_buffer_rsrc_t rl, ru, rc;
   unsigned ol[16], ou[16], oc[16];
-  auto plan = [&](int tl, bool live) __attribute__((always_inline)) {
+  auto plan = [&](int tl, int ch, bool live) __attribute__((always_inline)) {
     int ty, tx;
     zt_coords(tl, tiles_x, tiles_y, &ty, &tx);
     const int row0 = ty * 64, col0 = tx * 64;
     const int range = live ? ZT_OUT : 0;
-    rl = zt_rsrc(A + 2 * ((size_t)(m0 + row0) * lda + k0), range);
-    ru = zt_rsrc(A + 2 * ((size_t)k0 * lda + n0 + col0), range);
-    rc = zt_rsrc(A + 2 * ((size_t)(m0 + row0) * lda + n0 + col0), range);
+    const int kc = k0 + 64 * ch, Kc = min(64, K - 64 * ch);
+    rl = zt_rsrc(A + 2 * ((size_t)(m0 + row0) * lda + kc), range);
+    ru = zt_rsrc(A + 2 * ((size_t)kc * lda + n0 + col0), range);
+    rc = zt_rsrc(A + 2 * ((size_t)(m0 + row0) * lda + n0 + col0), ch == 0 ? range : 0);
     const int mr = M - row0, nc = N - col0;
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       const int r = wave * 16 + i;
-      ol[i] = (r < mr && lane < K) ? r * row_bytes + lane * esz : ZT_POISON;
-      ou[i] = (r < K && lane < nc) ? r * row_bytes + lane * esz : ZT_POISON;
+      ol[i] = (r < mr && lane < Kc) ? r * row_bytes + lane * esz : ZT_POISON;
+      ou[i] = (r < Kc && lane < nc) ? r * row_bytes + lane * esz : ZT_POISON;
       const int a = i >> 3, b = (i >> 2) & 1, v = i & 3;
       const int cr = wr * 32 + a * 16 + Tr::acc_row(v, lk), cc = wc * 32 + b * 16 + li;
       oc[i] = (cr < mr && cc < nc) ? cr * row_bytes + cc * esz : ZT_POISON;
     }
   };
   C2 pl[16], pu[16], pc[16];
-  plan(tile, true);
+  int ch = 0;
+  plan(tile, 0, true);
 #pragma unroll
   for (int i = 0; i < 16; ++i) {
     pl[i] = Tr::load(rl, ol[i]);
@@ -1250,33 +1257,41 @@ zgemm_tile_kernel(T* __restrict__ A, long long lda, int m0, int n0, int M, int N
 #pragma unroll
   for (int i = 0; i < 16; ++i) os[i] = ZT_POISON;
   for (;;) {
-    __syncthreads();  // the previous tile's operands are no longer read
+    __syncthreads();  // the previous unit's operands are no longer read
 #pragma unroll
     for (int i = 0; i < 16; ++i) {
       sL[lane * ZT_SP + wave * 16 + i] = pl[i];
       sU[(wave * 16 + i) * ZT_SP + lane] = pu[i];
     }
-    zt_store_result<Tr>(accr, acci, accz, rs, os);
+    if (ch == 0) {
+      // a new tile: the previous tile's result out, this tile's C in
+      zt_store_result<Tr>(accr, acci, accz, rs, os);
 #pragma unroll
-    for (int a = 0; a < 2; ++a)
+      for (int a = 0; a < 2; ++a)
 #pragma unroll
-      for (int b = 0; b < 2; ++b) {
-        const C2* p = pc + (a * 2 + b) * 4;
-        accr[a][b] = (Acc){p[0].x, p[1].x, p[2].x, p[3].x};
-        if constexpr (Tr::k3M) {
-          acci[a][b] = (Acc){T(0), T(0), T(0), T(0)};
-          accz[a][b] = (Acc){p[0].y + p[0].x, p[1].y + p[1].x, p[2].y + p[2].x, p[3].y + p[3].x};
-        } else {
-          acci[a][b] = (Acc){p[0].y, p[1].y, p[2].y, p[3].y};
+        for (int b = 0; b < 2; ++b) {
+          const C2* p = pc + (a * 2 + b) * 4;
+          accr[a][b] = (Acc){p[0].x, p[1].x, p[2].x, p[3].x};
+          if constexpr (Tr::k3M) {
+            acci[a][b] = (Acc){T(0), T(0), T(0), T(0)};
+            accz[a][b] = (Acc){p[0].y + p[0].x, p[1].y + p[1].x, p[2].y + p[2].x, p[3].y + p[3].x};
+          } else {
+            acci[a][b] = (Acc){p[0].y, p[1].y, p[2].y, p[3].y};
+          }
         }
-      }
-    // this tile's store target before plan() moves on to the next tile
-    rs = rc;
+      // this tile's store target before plan() moves on
+      rs = rc;
 #pragma unroll
-    for (int i = 0; i < 16; ++i) os[i] = oc[i];
-    const int next = tile + gx;
+      for (int i = 0; i < 16; ++i) os[i] = oc[i];
+    }
+    // the next unit: this tile's next chunk, or the next tile's first
+    int next = tile, nextch = ch + 1;
+    if (nextch == nch) {
+      next = tile + gx;
+      nextch = 0;
+    }
     const bool live = next < tend;
-    plan(live ? next : tile, live);
+    plan(live ? next : tile, live ? nextch : 0, live);
     __syncthreads();
     C2 xa[2], yb[2];
 #pragma unroll
@@ -1331,6 +1346,7 @@ zgemm_tile_kernel(T* __restrict__ A, long long lda, int m0, int n0, int M, int N
     }
     if (!live) break;
     tile = next;
+    ch = nextch;
   }
   zt_store_result<Tr>(accr, acci, accz, rs, os);
 }
@@ -1527,6 +1543,14 @@ int next_fused_max_tiles() {
 
 bool lookahead_enabled() {
   const char* v = std::getenv("NECGPU_LU_LOOKAHEAD");
+  return !(v && v[0] == '0');
+}
+
+// The one-stream phase of the LU in pairs of panels with one 128-deep
+// trailing update per pair (default), or one update per panel
+// (NECGPU_LU_PAIRS=0, for measurements).
+bool lu_pairs_enabled() {
+  const char* v = std::getenv("NECGPU_LU_PAIRS");
   return !(v && v[0] == '0');
 }
 
@@ -2038,17 +2062,57 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
   if (la && groups(0) <= la_max_g) {
     kla = 0;
   } else {
-    for (int k0 = 0; k0 < n; k0 += NB) {
+    // Panels in pairs while both stay on the one stream (and the tile
+    // kernel does the update): after panel a, only the next panel's columns
+    // are updated; after panel b, U12 of both (rows of a by L_aa^-1, rows
+    // of b by L_bb^-1 after subtracting L_ba U12_a) and one trailing update
+    // 128 deep, which reads and writes the trailing matrix once per pair of
+    // panels instead of once per panel.  NECGPU_LU_PAIRS=0 keeps one update
+    // per panel, for measurements.
+    const bool pairs = gemm_mode != 0 && lu_pairs_enabled();
+    const int resident = ZtTraits<T>::kPerCU * cu_count;
+    auto tile_update = [&](int m0, int n0, int M, int N, int kk, int K) {
+      if (M <= 0 || N <= 0) return;
+      const int tiles = ((M + 63) / 64) * ((N + 63) / 64);
+      launch_k(K_ZGEMM, zgemm_tile_kernel<T>, dim3(std::max(1, std::min(tiles, resident))), dim3(256),
+               (uint32_t)zt_lds_bytes<T>(), stream, A, lda, m0, n0, M, N, kk, K);
+    };
+    auto trsm = [&](int kk, int nb, int cbeg, int cend) {
+      if (cend > cbeg)
+        launch_k(K_LU_TRSM, lu_trsm_kernel<T>, dim3((cend - cbeg + 63) / 64), dim3(256), 0, stream, A, lda, cend, kk,
+                 nb, cbeg);
+    };
+    for (int k0 = 0; k0 < n;) {
       const int nb = std::min(NB, n - k0);
-      const int next = k0 + nb;
-      laswp(k0, nb, 0, k0, next, n, stream, true);
-      if (next >= n) break;
-      update(k0, nb, next, n, INT_MAX, stream, gemm_mode != 0);
-      if (la && groups(next) <= la_max_g) {
-        kla = next;
+      const int k1 = k0 + nb;
+      laswp(k0, nb, 0, k0, k1, n, stream, true);
+      if (k1 >= n) break;
+      const bool b_on_stream = !(la && groups(k1) <= la_max_g);
+      const int nb1 = std::min(NB, n - k1);
+      const int k2 = k1 + nb1;
+      if (!pairs || !b_on_stream || k2 >= n) {
+        update(k0, nb, k1, n, INT_MAX, stream, gemm_mode != 0);
+        if (!b_on_stream) {
+          kla = k1;
+          break;
+        }
+        panel(k1, stream);
+        k0 = k1;
+        continue;
+      }
+      next_update(k0, nb, k1, k2, stream);  // panel b's columns
+      panel(k1, stream);
+      laswp(k1, nb1, 0, k1, k2, n, stream, true);
+      trsm(k0, nb, k2, n);                       // U12_a
+      tile_update(k1, k2, nb1, n - k2, k0, nb);  // rows of b: -= L_ba U12_a
+      trsm(k1, nb1, k2, n);                      // U12_b
+      tile_update(k2, k2, n - k2, n - k2, k0, nb + nb1);
+      if (la && groups(k2) <= la_max_g) {
+        kla = k2;
         break;
       }
-      panel(next, stream);
+      panel(k2, stream);
+      k0 = k2;
     }
   }
   if (kla >= n) return hipGetLastError() == hipSuccess ? 0 : 1;

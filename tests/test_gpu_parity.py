@@ -156,6 +156,30 @@ def test_zgetrf_zgetrs_random(n, mode, monkeypatch):
     assert np.linalg.norm(A @ xs - b) <= 1e-12 * np.linalg.norm(b) * n
 
 
+@pytest.mark.parametrize("pairs", ["1", "0"])
+@pytest.mark.parametrize("n", [129, 130, 191, 257, 300, 641, 1100])
+def test_zgetrf_one_stream_panel_pairs(n, pairs, monkeypatch):
+    """Without the look-ahead every panel runs on one stream, in pairs with
+    one 128-deep trailing update per pair (the default) or one update per
+    panel: the oracle's pivots and factors either way."""
+    monkeypatch.setenv("NECGPU_LU_LOOKAHEAD", "0")
+    monkeypatch.setenv("NECGPU_LU_PAIRS", pairs)
+    torch = _torch()
+    rng = np.random.default_rng(3 * n + 1)
+    A = rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n))
+    dA = torch.from_numpy(A.copy()).to("cuda")
+    piv = torch.zeros(2 * n, dtype=torch.int32, device="cuda")
+    info = torch.zeros(1, dtype=torch.int32, device="cuda")
+    lw = torch.empty(int(api.necgpu_zgetrf_workspace(n)), dtype=torch.uint8, device="cuda")
+    assert api.necgpu_zgetrf(dA.data_ptr(), n, n, piv.data_ptr(), info.data_ptr(), lw.data_ptr(), None) == 0
+    torch.cuda.synchronize()
+    assert int(info.item()) == 0
+    oLU, opiv, _ = po.lu(A)
+    assert np.array_equal(piv.cpu().numpy()[:n], opiv)
+    scale = np.max(np.abs(oLU))
+    assert np.max(np.abs(dA.cpu().numpy() - oLU)) <= 1e-12 * scale * max(1, n / 64)
+
+
 @pytest.mark.parametrize("n", [70, 300])
 def test_zgetrs_backward_stable_when_nearly_singular(n):
     """A matrix with a cluster of nearly dependent columns (1e-13 apart) has
