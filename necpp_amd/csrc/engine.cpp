@@ -825,7 +825,7 @@ void DeviceSolver::benchmark_pipeline(double rkh, const std::vector<zc>& e, cons
     acc.factor_ms += elapsed(s[1], s[2]);
     acc.solve_ms += elapsed(s[2], s[3]);
   }
-  if (kclock_.collect() != 0) throw NecError("kernel timing events failed");
+  if (kclock_.collect() != 0) throw NecError("kernel timing events failed: " + kclock_.failure);
   *ms_per_solve = steps > 0 ? total / steps : 0.0;
   if (stage_ms) {
     stage_ms->fill_ms = acc.fill_ms / std::max(1, steps);

@@ -56,11 +56,19 @@ bool KernelClock::reserve(size_t slots) {
 
 int KernelClock::collect() {
   int rc = 0;
+  auto fail = [&](size_t i, hipError_t e, const char* what) {
+    if (rc == 0)
+      failure = std::string(what) + " of " + kernel_kind_name(kind_[i]) + " (launch " + std::to_string(i) + " of " +
+                std::to_string(used_) + "): " + hipGetErrorString(e);
+    rc = 1;
+  };
   for (size_t i = 0; i < used_; ++i) {
-    if (hipEventSynchronize(ev_[2 * i + 1]) != hipSuccess) rc = 1;
+    hipError_t e = hipEventSynchronize(ev_[2 * i + 1]);
+    if (e != hipSuccess) fail(i, e, "hipEventSynchronize");
     float ms = 0.0f;
-    if (hipEventElapsedTime(&ms, ev_[2 * i], ev_[2 * i + 1]) != hipSuccess) {
-      rc = 1;
+    e = hipEventElapsedTime(&ms, ev_[2 * i], ev_[2 * i + 1]);
+    if (e != hipSuccess) {
+      fail(i, e, "hipEventElapsedTime");
       continue;
     }
     total_ms[kind_[i]] += ms;

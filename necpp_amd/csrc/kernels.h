@@ -5,6 +5,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 #include <vector>
 
 namespace necgpu {
@@ -56,6 +57,7 @@ class KernelClock {
   bool next(int kind, hipEvent_t* start, hipEvent_t* stop);
   // After the timed work has completed: add every pending pair to the totals.
   int collect();
+  std::string failure;  // the first failed pair of the last collect()
   void reset();
   size_t pending() const { return used_; }
   // pre-create event pairs so that later launches create none

@@ -1513,8 +1513,8 @@ int panel_stride_override() {
 // Trailing update kernel (NECGPU_ZGEMM, for measurements): 1 = the
 // persistent tile kernel everywhere, 0 = the per-tile kernels
 // (zgemm_update_kernel, zgemm_update_f32_kernel) everywhere, 2 (default) =
-// the tile kernel while the LU runs on one stream and the per-tile kernels
-// beside the look-ahead panels.
+// the tile kernel while the LU runs on one stream and for the large updates
+// beside the look-ahead panels, the per-tile kernels for the small ones.
 int zgemm_mode() {
   const char* v = std::getenv("NECGPU_ZGEMM");
   return (v && *v) ? std::atoi(v) : 2;
@@ -2148,9 +2148,12 @@ int launch_lu(T* A, long long lda, int n, int* piv, int* info, void* work, hipSt
     const int Gnext = (n - next + RMAX_T - 1) / RMAX_T;
     const int free_cus = std::max(1, cu_count - Gnext);
     const int tiles = ((n - next + 63) / 64) * ((n - far + 63) / 64);
-    update(k0, nb, far, n, gemm_mode == 1 ? ZtTraits<T>::kPerCU * free_cus : gemm_occupancy(tiles, free_cus) * free_cus,
-           stream,
-           gemm_mode == 1);
+    // the tile kernel for updates of 16 tiles or more per free CU (N =
+    // 16,020: 246.6 against 250.6 ms for the FP64 factorisation), the
+    // per-tile kernels for smaller ones (N = 3,280: 14.9 against 15.5 ms)
+    const bool tile_k = gemm_mode == 1 || (gemm_mode == 2 && tiles >= 16 * free_cus);
+    update(k0, nb, far, n, tile_k ? ZtTraits<T>::kPerCU * free_cus : gemm_occupancy(tiles, free_cus) * free_cus, stream,
+           tile_k);
     if (hipEventRecord(la->rest_done, stream) != hipSuccess) return 1;
   }
   return hipGetLastError() == hipSuccess ? 0 : 1;
