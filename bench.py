@@ -409,8 +409,8 @@ def main():
         copy_probe(a.copy_probe)
         return
     if a.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported (see DESIGN.md "
-                                   "for how the solve would shard)", "n_gpus": a.gpus}))
+        print(json.dumps({"error": "this build runs on one GPU; --gpus N>1 is not supported "
+                                   "(the solve is not sharded across GPUs)", "n_gpus": a.gpus}))
         sys.exit(2)
     from necpp_amd import decks
     from necpp_amd import necpp as api

@@ -22,7 +22,8 @@ SOURCES = ["geometry.cpp", "kclock.cpp", "engine.cpp", "batch_host.cpp", "adjoin
 
 COMMON = ["-O3", "-fPIC", "-std=c++17", "-ffp-contract=off",
           f"--offload-arch={ARCH}",
-          # packed FP32 instructions stay off (toolchain notes)
+          # packed FP32 instructions stay off: some of their forms have given
+          # wrong results on this hardware without any error
           "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
           "-Wall", "-Wno-unused-function", "-fvisibility=hidden", "-I", os.path.join(HERE, "..", "include")]
 
