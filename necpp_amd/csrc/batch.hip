@@ -11,6 +11,7 @@
 // or 512 threads, one per row), with the panel in LDS and the pivot rows
 // tracked by a permutation instead of being moved.
 #include <climits>
+#include <cstdlib>
 
 #include "basis.h"
 #include "kernels.h"
@@ -45,6 +46,65 @@ __device__ int block_first_max(double v, double* s_val, int* s_row) {
   return best;
 }
 
+// Factor the panel of logical rows k0 .. n-1 and columns k0 .. k0+P-1 (the
+// matrix's rows through the permutation s_rp) in LDS, column-major
+// (sP[c * BT + r]: logical row k0 + r, column k0 + c), with partial pivoting
+// (first maximum of |z|), and write it back; sP keeps the factored panel.
+template <int BT, int P>
+__device__ __forceinline__ void batch_panel(double2* __restrict__ Z, int n, int k0, double2* sP, int* s_rp,
+                                            double* s_val, int* s_row, int& first_zero) {
+  const int t = threadIdx.x;
+  const int pw = min(P, n - k0);
+  const int m = n - k0;
+  if (t < m) {
+    const double2* src = Z + (long long)s_rp[k0 + t] * n + k0;
+#pragma unroll
+    for (int c = 0; c < P; ++c) sP[c * BT + t] = c < pw ? src[c] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int c = 0; c < pw; ++c) {
+    const int k = k0 + c;
+    double v = -1.0;
+    if (t >= c && t < m) v = hypot(sP[c * BT + t].x, sP[c * BT + t].y);
+    const int pr = block_first_max<BT>(v, s_val, s_row);
+    if (pr >= 0 && pr != c) {
+      if (t < P) {
+        const double2 a = sP[t * BT + c];
+        sP[t * BT + c] = sP[t * BT + pr];
+        sP[t * BT + pr] = a;
+      }
+      if (t == 0) {
+        const int tmp = s_rp[k0 + c];
+        s_rp[k0 + c] = s_rp[k0 + pr];
+        s_rp[k0 + pr] = tmp;
+      }
+    }
+    __syncthreads();
+    const double2 dp = sP[c * BT + c];
+    const zc dpiv = zmk(dp.x, dp.y);
+    const bool zero = (dpiv.re == 0.0 && dpiv.im == 0.0);
+    if (zero && first_zero == 0) first_zero = k + 1;
+    if (!zero && t > c && t < m) {
+      const double2 a = sP[c * BT + t];
+      const zc l = zdiv(zmk(a.x, a.y), dpiv);
+      sP[c * BT + t] = make_double2(l.re, l.im);
+      for (int cc = c + 1; cc < pw; ++cc) {
+        const double2 u = sP[cc * BT + c];
+        const zc pu = zmul(l, zmk(u.x, u.y));
+        double2 e = sP[cc * BT + t];
+        e.x -= pu.re;
+        e.y -= pu.im;
+        sP[cc * BT + t] = e;
+      }
+    }
+    __syncthreads();
+  }
+  if (t < m) {
+    double2* dst = Z + (long long)s_rp[k0 + t] * n + k0;
+    for (int c = 0; c < pw; ++c) dst[c] = sP[c * BT + t];
+  }
+}
+
 // In-place LU with partial pivoting (first maximum of |z|, as the
 // single-structure path) of each solve's n x n matrix, n <= BT.
 //
@@ -74,53 +134,7 @@ __global__ void __launch_bounds__(BT) batch_lu_kernel(const BatchSolve* __restri
   for (int k0 = 0; k0 < n; k0 += P) {
     const int pw = min(P, n - k0);
     const int m = n - k0;
-    if (t < m) {
-      const double2* src = Z + (long long)s_rp[k0 + t] * n + k0;
-#pragma unroll
-      for (int c = 0; c < P; ++c) sP[c * BT + t] = c < pw ? src[c] : make_double2(0.0, 0.0);
-    }
-    __syncthreads();
-    for (int c = 0; c < pw; ++c) {
-      const int k = k0 + c;
-      double v = -1.0;
-      if (t >= c && t < m) v = hypot(sP[c * BT + t].x, sP[c * BT + t].y);
-      const int pr = block_first_max<BT>(v, s_val, s_row);
-      if (pr >= 0 && pr != c) {
-        if (t < P) {
-          const double2 a = sP[t * BT + c];
-          sP[t * BT + c] = sP[t * BT + pr];
-          sP[t * BT + pr] = a;
-        }
-        if (t == 0) {
-          const int tmp = s_rp[k0 + c];
-          s_rp[k0 + c] = s_rp[k0 + pr];
-          s_rp[k0 + pr] = tmp;
-        }
-      }
-      __syncthreads();
-      const double2 dp = sP[c * BT + c];
-      const zc dpiv = zmk(dp.x, dp.y);
-      const bool zero = (dpiv.re == 0.0 && dpiv.im == 0.0);
-      if (zero && first_zero == 0) first_zero = k + 1;
-      if (!zero && t > c && t < m) {
-        const double2 a = sP[c * BT + t];
-        const zc l = zdiv(zmk(a.x, a.y), dpiv);
-        sP[c * BT + t] = make_double2(l.re, l.im);
-        for (int cc = c + 1; cc < pw; ++cc) {
-          const double2 u = sP[cc * BT + c];
-          const zc pu = zmul(l, zmk(u.x, u.y));
-          double2 e = sP[cc * BT + t];
-          e.x -= pu.re;
-          e.y -= pu.im;
-          sP[cc * BT + t] = e;
-        }
-      }
-      __syncthreads();
-    }
-    if (t < m) {
-      double2* dst = Z + (long long)s_rp[k0 + t] * n + k0;
-      for (int c = 0; c < pw; ++c) dst[c] = sP[c * BT + t];
-    }
+    batch_panel<BT, P>(Z, n, k0, sP, s_rp, s_val, s_row, first_zero);
     const int w = m - pw;  // trailing columns
     if (w > 0) {
       const int W = (w + 63) & ~63;
@@ -191,6 +205,127 @@ __global__ void __launch_bounds__(BT) batch_lu_kernel(const BatchSolve* __restri
             }
           }
           *z0 = a0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (t < n) rpall[d.row0 + t] = s_rp[t];
+  if (t == 0) info[blockIdx.x] = first_zero;
+}
+
+// batch_lu_kernel with the trailing update on FP64 MFMA (BT = 256, four
+// waves).  After each panel, thread t < w forms column t of U12 = L11^-1 A12
+// in registers and writes it to the matrix; then the (m - pw) x w trailing
+// block is updated in 16 x 16 tiles of v_mfma_f64_16x16x4f64, four real
+// products per complex one (acc_re += (-Lr) Ur + Li Ui, acc_im += (-Lr) Ui +
+// (-Li) Ur, from acc = C).  Wave v takes the tile columns v, v + 4, ...,
+// holds that column strip's U12 fragments in registers and walks down its
+// row tiles: C through the row permutation (lane l, register q: row
+// l / 16 + 4 q, column l % 16 of the tile), L21 from the panel in LDS.
+template <int BT, int P>
+__global__ void __launch_bounds__(BT) batch_lu_mfma_kernel(const BatchSolve* __restrict__ sol,
+                                                           double* __restrict__ Zall, int* __restrict__ rpall,
+                                                           int* __restrict__ info) {
+  static_assert(P % 4 == 0, "MFMA k-steps of 4");
+  typedef double double4_t __attribute__((ext_vector_type(4)));
+  __shared__ double2 sP[P * BT];
+  __shared__ int s_rp[BT];
+  __shared__ double s_val[BT / 64];
+  __shared__ int s_row[BT / 64];
+  const BatchSolve d = sol[blockIdx.x];
+  const int n = d.n;
+  const int t = threadIdx.x;
+  const int lane = t & 63, wave = t >> 6;
+  const int l16 = lane & 15, l4 = lane >> 4;
+  double2* Z = reinterpret_cast<double2*>(Zall) + d.z0;
+  if (t < n) s_rp[t] = t;
+  int first_zero = 0;
+  __syncthreads();
+  for (int k0 = 0; k0 < n; k0 += P) {
+    const int pw = min(P, n - k0);
+    const int m = n - k0;
+    batch_panel<BT, P>(Z, n, k0, sP, s_rp, s_val, s_row, first_zero);
+    const int w = m - pw;  // trailing columns (and rows)
+    if (w > 0) {
+      if (t < w) {
+        const int j = k0 + pw + t;
+        double ur[P], ui[P];
+#pragma unroll
+        for (int c = 0; c < P; ++c) {
+          double2 a = make_double2(0.0, 0.0);
+          if (c < pw) a = Z[(long long)s_rp[k0 + c] * n + j];
+          ur[c] = a.x;
+          ui[c] = a.y;
+        }
+#pragma unroll
+        for (int c = 1; c < P; ++c) {
+          if (c < pw) {
+#pragma unroll
+            for (int c2 = 0; c2 < c; ++c2) {
+              const double2 l = sP[c2 * BT + c];
+              ur[c] -= l.x * ur[c2] - l.y * ui[c2];
+              ui[c] -= l.x * ui[c2] + l.y * ur[c2];
+            }
+          }
+        }
+#pragma unroll
+        for (int c = 0; c < P; ++c)
+          if (c < pw) Z[(long long)s_rp[k0 + c] * n + j] = make_double2(ur[c], ui[c]);
+      }
+      __syncthreads();  // U12 in the matrix, visible to the workgroup
+      const int tiles_r = (w + 15) / 16, tiles_c = (w + 15) / 16;
+      for (int tc = wave; tc < tiles_c; tc += BT / 64) {
+        // U12 fragments of this column strip: k-step s, lane (j, kk) holds
+        // U12[4 s + kk][16 tc + j]
+        const int jj = 16 * tc + l16;
+        double br[P / 4], bi[P / 4];
+#pragma unroll
+        for (int s = 0; s < P / 4; ++s) {
+          const int c = 4 * s + l4;
+          double2 b = make_double2(0.0, 0.0);
+          if (c < pw && jj < w) b = Z[(long long)s_rp[k0 + c] * n + k0 + pw + jj];
+          br[s] = b.x;
+          bi[s] = b.y;
+        }
+        // C of row tile tr (the loads for tile tr + 2 go out before tile tr's MFMAs)
+        auto load_c = [&](int tr, double2(&c)[4]) __attribute__((always_inline)) {
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int r = pw + 16 * tr + l4 + 4 * q;
+            c[q] = make_double2(0.0, 0.0);
+            if (r < m && jj < w) c[q] = Z[(long long)s_rp[k0 + r] * n + k0 + pw + jj];
+          }
+        };
+        double2 cv[4], cn[4], cn2[4];
+        load_c(0, cv);
+        if (1 < tiles_r) load_c(1, cn);
+        for (int tr = 0; tr < tiles_r; ++tr) {
+          const int r0 = pw + 16 * tr;  // logical row (from k0) of the tile's first row
+          if (tr + 2 < tiles_r) load_c(tr + 2, cn2);
+          double4_t accr = (double4_t){cv[0].x, cv[1].x, cv[2].x, cv[3].x};
+          double4_t acci = (double4_t){cv[0].y, cv[1].y, cv[2].y, cv[3].y};
+#pragma unroll
+          for (int s = 0; s < P / 4; ++s) {
+            // L21 fragment: lane (i, kk) holds L[r0 + i][4 s + kk]
+            const int c = 4 * s + l4, r = r0 + l16;
+            double2 a = make_double2(0.0, 0.0);
+            if (c < pw && r < m) a = sP[c * BT + r];
+            accr = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.x, br[s], accr, 0, 0, 0);
+            accr = __builtin_amdgcn_mfma_f64_16x16x4f64(a.y, bi[s], accr, 0, 0, 0);
+            acci = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.x, bi[s], acci, 0, 0, 0);
+            acci = __builtin_amdgcn_mfma_f64_16x16x4f64(-a.y, br[s], acci, 0, 0, 0);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            const int r = r0 + l4 + 4 * q;
+            if (r < m && jj < w) Z[(long long)s_rp[k0 + r] * n + k0 + pw + jj] = make_double2(accr[q], acci[q]);
+          }
+#pragma unroll
+          for (int q = 0; q < 4; ++q) {
+            cv[q] = cn[q];
+            cn[q] = cn2[q];
+          }
         }
       }
     }
@@ -512,8 +647,24 @@ int launch_batch_tables(const BatchSolve* sol, int S, const BatchDesign* des, co
   return hipGetLastError() == hipSuccess ? 0 : 1;
 }
 
+// The 256-thread LU's trailing update on MFMA (batch_lu_mfma_kernel, the
+// default) or on the vector units (NECGPU_BATCH_MFMA=0, for measurements).
+bool batch_mfma_enabled() {
+  const char* v = std::getenv("NECGPU_BATCH_MFMA");
+  return !(v && v[0] == '0');
+}
+
 int launch_batch_lu(const BatchSolve* sol, int S, double* Z, int* rp, int* info, int P, int bt, hipStream_t stream) {
   if (S <= 0) return 0;
+  if (bt != 512 && batch_mfma_enabled()) {
+    if (P == 32)
+      launch_k(K_BATCH_LU, batch_lu_mfma_kernel<256, 32>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, info);
+    else if (P == 8)
+      launch_k(K_BATCH_LU, batch_lu_mfma_kernel<256, 8>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, info);
+    else
+      launch_k(K_BATCH_LU, batch_lu_mfma_kernel<256, 16>, dim3(S), dim3(256), 0u, stream, sol, Z, rp, info);
+    return hipGetLastError() == hipSuccess ? 0 : 1;
+  }
   if (bt == 512) {
     // the 512-row panel takes P * 16 KB of LDS: P = 8 or 16
     if (P == 8)
