@@ -91,7 +91,12 @@ def check_stats(ctx, o, g, index=0):
         ag, ao = g[name], o["pattern"][name]
         d = ag - ao
         mean_g = np.sum(ag * w) / norm
-        own = [np.max(ag), np.min(ag), mean_g, np.sqrt(np.sum((ag - mean_g) ** 2 * w) / norm)]
+        # a deck whose directions run past theta = 180 degrees has negative
+        # averaging weights, and its sd is the root of a negative sum: NaN,
+        # as in the reference's own formula
+        with np.errstate(invalid="ignore"):
+            sd_g = np.sqrt(np.sum((ag - mean_g) ** 2 * w) / norm)
+        own = [np.max(ag), np.min(ag), mean_g, sd_g]
         assert np.allclose(got[4 * a:4 * a + 4], own, rtol=1e-12, atol=1e-12, equal_nan=True), name
         if np.isnan(want[4 * a:4 * a + 4]).any():  # NaN gains (no input power): NaN in both
             assert np.array_equal(np.isnan(got[4 * a:4 * a + 4]), np.isnan(want[4 * a:4 * a + 4])), name
