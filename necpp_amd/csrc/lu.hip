@@ -582,14 +582,17 @@ __global__ void __launch_bounds__(256) lu_laswp_kernel(T* __restrict__ A, long l
     int ext_row = -1, ext_cont = -1;              // outside rows
     int next_ext = 0;
     const int my_piv = (lane < nb) ? piv[k0 + lane] : 0;  // one load per lane, up front
+    // every source lane below is wave-uniform (the step, a pivot read from
+    // a lane, a ballot's first lane), so the reads are readlanes, not LDS
+    // permutes: the chain of 64 steps is the launch's critical path
     for (int q = 0; q < nb; ++q) {
       const int k = k0 + q;
-      const int p = __shfl(my_piv, q, 64);
+      const int p = __builtin_amdgcn_readlane(my_piv, q);
       if (p == k) continue;
-      int ck = __shfl(cont_in, q, 64);
+      int ck = __builtin_amdgcn_readlane(cont_in, q);
       int cp;
       if (p < k0 + nb) {
-        cp = __shfl(cont_in, p - k0, 64);
+        cp = __builtin_amdgcn_readlane(cont_in, p - k0);
         if (lane == p - k0) cont_in = ck;
       } else {
         const unsigned long long hit = __ballot(ext_row == p);
@@ -603,7 +606,7 @@ __global__ void __launch_bounds__(256) lu_laswp_kernel(T* __restrict__ A, long l
             ext_cont = p;
           }
         }
-        cp = __shfl(ext_cont, e, 64);
+        cp = __builtin_amdgcn_readlane(ext_cont, e);
         if (lane == e) ext_cont = ck;
       }
       if (lane == q) cont_in = cp;
