@@ -1065,13 +1065,15 @@ __global__ void __launch_bounds__(256) zgemm_update_kernel(double* __restrict__ 
   }
 }
 
-// The same update, C[m0.., n0..] (M x N) -= L * U with K <= 64, as a
-// persistent kernel of 256-thread workgroups that stay resident (FP64: one
-// per CU, LDS 133 KB; FP32: two per CU, 66.5 KB each).  LDS holds the whole
-// 64 x K strips of L and U.  Per tile the workgroup
-//   - writes the strips it prefetched into registers to LDS, stores the
-//     previous tile's result, and moves the prefetched C block into the
-//     accumulators,
+// The same update, C[m0.., n0..] (M x N) -= L * U, as a persistent kernel
+// of 256-thread workgroups that stay resident (FP64: one per CU, LDS 133 KB;
+// FP32: two per CU, 66.5 KB each).  K is taken in 64-deep chunks (one per
+// panel: K = 128 when the one-stream LU updates a pair of panels at once);
+// LDS holds one chunk's 64 x 64 strips of L and U.  Per chunk the workgroup
+//   - writes the strips it prefetched into registers to LDS and, at a tile's
+//     first chunk, stores the previous tile's result and moves the
+//     prefetched C block into the accumulators (which carry over from one
+//     chunk of a tile to the next),
 //   - runs 16 k-steps of MFMAs per wave (32 x 32 complex per wave) with the
 //     accumulators counting down from C.  FP32: the complex product in 4M
 //     form, 16 MFMAs per k-step: acc_re += (-Lr) Ur + Li Ui, acc_im +=
@@ -1080,8 +1082,9 @@ __global__ void __launch_bounds__(256) zgemm_update_kernel(double* __restrict__ 
 //     X = Cr, Y = 0, Z = Ci + Cr, and the result Re = X + Y, Im = (Z - X)
 //     + Y (the FP32 kernel is not MFMA-bound: three products measured no
 //     faster there).  Each k-step also issues three of the 48 loads of the
-//     next tile's strips and C block, so the loads trickle out beside the
-//     MFMAs instead of stalling them in one burst.
+//     next chunk's strips and C block (C at a tile's first chunk only), so
+//     the loads trickle out beside the MFMAs instead of stalling them in
+//     one burst.
 // Storing a tile's result after the next tile's operands are in LDS keeps
 // the waits for the prefetched loads from waiting on stores.  Loads and
 // stores are buffer operations: an element outside the matrix gets an offset
