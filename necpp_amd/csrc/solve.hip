@@ -99,7 +99,27 @@ __global__ void __launch_bounds__(64) diag_inverse_kernel(const TF* __restrict__
     for (int r = 0; r < SB; ++r) {
       double sr = (r == j) ? 1.0 : 0.0, si = 0.0;
       if (r < m) {
-        for (int q = j; q < r; ++q) {
+        // four terms' LDS reads go out together, then the same sums in the
+        // same order (one wait per four terms instead of one per term:
+        // 200 -> 130 us for the 52 x 2 blocks at N = 3,280; eight measured
+        // no better)
+        int q = j;
+        for (; q + 4 <= r; q += 4) {
+          double lr[4], li[4], xr[4], xi[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            lr[u] = sT[2 * (r * SB + q + u)];
+            li[u] = sT[2 * (r * SB + q + u) + 1];
+            xr[u] = sX[2 * ((q + u) * SB + j)];
+            xi[u] = sX[2 * ((q + u) * SB + j) + 1];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            sr -= lr[u] * xr[u] - li[u] * xi[u];
+            si -= lr[u] * xi[u] + li[u] * xr[u];
+          }
+        }
+        for (; q < r; ++q) {
           const double lr = sT[2 * (r * SB + q)], li = sT[2 * (r * SB + q) + 1];
           const double xr = sX[2 * (q * SB + j)], xi = sX[2 * (q * SB + j) + 1];
           sr -= lr * xr - li * xi;
@@ -113,7 +133,24 @@ __global__ void __launch_bounds__(64) diag_inverse_kernel(const TF* __restrict__
     for (int r = SB - 1; r >= 0; --r) {
       double sr = (r == j) ? 1.0 : 0.0, si = 0.0;
       if (r < m && r <= j) {
-        for (int q = r + 1; q <= j && q < m; ++q) {
+        const int qe = min(j, m - 1);  // last q of the sum
+        int q = r + 1;
+        for (; q + 3 <= qe; q += 4) {
+          double ur[4], ui[4], xr[4], xi[4];
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            ur[u] = sT[2 * (r * SB + q + u)];
+            ui[u] = sT[2 * (r * SB + q + u) + 1];
+            xr[u] = sX[2 * ((q + u) * SB + j)];
+            xi[u] = sX[2 * ((q + u) * SB + j) + 1];
+          }
+#pragma unroll
+          for (int u = 0; u < 4; ++u) {
+            sr -= ur[u] * xr[u] - ui[u] * xi[u];
+            si -= ur[u] * xi[u] + ui[u] * xr[u];
+          }
+        }
+        for (; q <= qe; ++q) {
           const double ur = sT[2 * (r * SB + q)], ui = sT[2 * (r * SB + q) + 1];
           const double xr = sX[2 * (q * SB + j)], xi = sX[2 * (q * SB + j) + 1];
           sr -= ur * xr - ui * xi;
